@@ -1,0 +1,243 @@
+"""Benchmark of the single-GPU Barnes-Hut pass on MI355X.
+
+One step = one full pass over one particle system already resident in HBM:
+SFC keys -> stable radix sort -> permute -> cornerstone leaves -> linked octree -> FP64 upsweep (centers, P2M,
+M2M, MAC) -> fixed 64-target groups -> traversal (M2P + P2P), plus the Ewald correction for periodic workloads.
+
+Default workload: N = 2^24 Plummer sphere (scale radius 1, truncated at r = 30), equal masses 1/N, fp32,
+softening 1e-3, theta 0.5, Cartesian quadrupoles, bucket 64, open boundaries.
+
+Prints ONE JSON line. value = N / (mean wall time of a pass) in particles/s. The roofline object is for the
+dominant kernel (the traversal): algorithmic flops per launch (23 per P2P + 128 per M2P, the north-star
+count) over the kernel's mean duration from HIP events on the library's stream, against the FP32 vector peak.
+"""
+
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+PEAK_FP32_TFLOPS = 157.3  # MI355X FP32 vector peak (AMD published)
+PEAK_HBM_GBS = 8000.0
+METRIC = "particles/s + P2P/M2P TFLOP/s at θ=0.5; served p50/p99 vs plain solves"
+
+WORKLOADS = {
+    "plummer24": dict(kind="plummer", n=1 << 24, periodic=False),
+    "uniform26": dict(kind="uniform", n=1 << 26, periodic=False),
+    "uniform26p": dict(kind="uniform_periodic", n=1 << 26, periodic=True),
+    "hernquist24": dict(kind="hernquist", n=1 << 24, periodic=False),
+}
+
+
+def make_system(name, n=None, seed=42):
+    from ryoanji_amd import workloads as W
+
+    spec = WORKLOADS[name]
+    n = n or spec["n"]
+    if spec["kind"] == "plummer":
+        return W.plummer(n, seed=seed)
+    if spec["kind"] == "uniform":
+        return W.uniform_cube(n, seed=seed, lo=-1.0, hi=1.0)
+    if spec["kind"] == "uniform_periodic":
+        return W.uniform_cube(n, seed=seed, lo=0.0, hi=1.0, periodic=True)
+    return W.hernquist_mix(n, seed=seed)
+
+
+def traffic_from_profiles(kernel="traverseKernel", workload="plummer24"):
+    """HBM bytes per launch from the committed rocprofv3 --pmc summary, if there is one."""
+    path = os.path.join(ROOT, "profiles", f"traffic_{kernel}_{workload}.json")
+    if not os.path.exists(path):
+        return None
+    try:
+        with open(path) as f:
+            return json.load(f).get("bytes_per_launch")
+    except Exception:
+        return None
+
+
+def cpu_baseline(s, sample_frac=1 / 128, seed=0):
+    """Reference CPU path (oracle/_ref, the reference's OpenMP code) when it was built, else the restatement:
+    keys, stable sort, permutation, leaves, linked tree and upsweep on the full system; the traversal on a seeded
+    sample of leaf ranges, scaled to all particles."""
+    import numpy as np
+
+    import oracle as O
+
+    kind = "reference" if O.have_ref() else "port"
+    lib = "ref" if kind == "reference" else "orc"
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    t0 = time.perf_counter()
+    keys = O.sfc_keys(s.x, s.y, s.z, s.limits, lib=lib)
+    sk, perm = O.sort_by_key(keys, lib=lib)
+    x, y, z, m, h = (a[perm] for a in s.arrays())
+    leaves, counts = O.compute_octree(sk, 64, lib=lib)
+    layout = O.layout_from_counts(counts)
+    c, q = O.upsweep(x, y, z, m, leaves, layout, s.limits, 2.0, lib=lib)
+    t_build = time.perf_counter() - t0
+    L = leaves.size - 1
+    rng = np.random.default_rng(seed)
+    nchunks = 16
+    span = max(1, int(L * sample_frac / nchunks))
+    starts = np.sort(rng.choice(max(1, L - span), nchunks, replace=False))
+    t1 = time.perf_counter()
+    nsample = 0
+    if kind == "reference":
+        for a in starts:
+            O.compute_gravity(leaves, layout, c, q, x, y, z, h, m, s.limits, lib="ref",
+                              leaf_range=(int(a), int(a + span)), want_pot=False)
+            nsample += int(layout[a + span] - layout[a])
+    else:
+        groups = []
+        for a in starts:
+            g0, g1 = int(layout[a]) // 16, (int(layout[a + span]) + 15) // 16
+            groups.extend(range(g0, g1))
+        groups = np.unique(np.array(groups, np.int32))
+        O.compute_gravity(leaves, layout, c, q, x, y, z, h, m, s.limits, group_list=groups, want_pot=False)
+        nsample = int(groups.size * 16)
+    t_trav = time.perf_counter() - t1
+    t_full = t_build + t_trav * s.n / max(nsample, 1)
+    return {
+        "value": s.n / t_full,
+        "unit": "particles/s",
+        "cores": threads,
+        "kind": kind,
+        "sample": (f"full keys/sort/permute/leaves/tree/upsweep of the same N={s.n} system ({t_build:.2f} s), "
+                   f"traversal (16-target groups) timed on {nsample} particles in {nchunks} seeded leaf ranges "
+                   f"({t_trav:.2f} s) and scaled by N/{nsample}"),
+    }
+
+
+def accuracy(ctx, sysm, s, order, ax, ay, az, pot, num_targets=16384, seed=1):
+    """1st-percentile and max relative acceleration error vs an FP64 direct sum on sampled targets."""
+    import numpy as np
+    import torch
+
+    import ryoanji_amd as R
+
+    arrs = [torch.from_numpy(a[order]).cuda() for a in (s.x, s.y, s.z, s.h, s.m)]
+    tgt = np.sort(np.random.default_rng(seed).choice(s.n, min(num_targets, s.n), replace=False)).astype(np.int64)
+    out = torch.zeros(tgt.size * 4, dtype=torch.float64, device="cuda")
+    ctx.call("ryo_direct_sum_f64", *(R.ptr(a) for a in arrs), s.n, R.ptr(torch.from_numpy(tgt).cuda()), tgt.size,
+             1.0, None, 0, R.ptr(out))
+    ref = out.cpu().numpy().reshape(-1, 4)
+    d = np.sqrt((ax[tgt] - ref[:, 1]) ** 2 + (ay[tgt] - ref[:, 2]) ** 2 + (az[tgt] - ref[:, 3]) ** 2)
+    e = np.sort(d / np.sqrt(ref[:, 1] ** 2 + ref[:, 2] ** 2 + ref[:, 3] ** 2))
+    return {"targets": int(tgt.size), "err_p1": float(e[int(0.99 * e.size)]), "err_max": float(e[-1]),
+            "err_median": float(e[e.size // 2])}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--workload", default="plummer24", choices=list(WORKLOADS))
+    ap.add_argument("--n", type=int, default=None, help="override the particle count")
+    ap.add_argument("--theta", type=float, default=0.5)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-accuracy", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU decomposition (see DESIGN.md, "
+              "'How the path would shard'); run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+
+    import ryoanji_amd as R
+
+    torch.cuda.set_device(0)
+    s = make_system(args.workload, args.n)
+    spec = WORKLOADS[args.workload]
+    ctx = R.Context()
+    R.lib().ryo_ctx_set_timing(ctx.h, 1)
+    sysm = R.System(ctx, s.n)
+    dev = [torch.from_numpy(a).cuda() for a in s.arrays()]
+    prm = R.default_params(theta=args.theta, group_size=64, bucket_size=64)
+    if spec["periodic"]:
+        prm.num_shells, prm.ewald = 1, 1
+
+    def one_pass():
+        # inputs resident in HBM: reset to the unsorted initial state (device-to-device), then a full pass
+        sysm.load(*dev, s.limits, periodic=spec["periodic"])
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        sysm.build(prm)
+        pot = sysm.gravity(prm)
+        torch.cuda.synchronize()
+        return time.perf_counter() - t, pot
+
+    for _ in range(args.warmup):
+        one_pass()
+    times, trav_ms, phase = [], [], []
+    stats = None
+    pot = 0.0
+    for _ in range(args.steps):
+        dt, pot = one_pass()
+        times.append(dt)
+        ph = sysm.phase_ms()
+        phase.append(ph)
+        trav_ms.append(ph["traverse"])
+        stats = ctx.stats()
+    t_pass = float(np.mean(times))
+    t_trav = float(np.mean(trav_ms)) / 1e3
+    info = sysm.info()
+    flops_ns = 23.0 * stats["sumP2P"] + 128.0 * stats["sumM2P"]
+    flops_code = 23.0 * stats["sumP2P"] + 65.0 * stats["sumM2P"]
+    achieved = flops_ns / t_trav / 1e12
+    p, ax, ay, az, order = (t.cpu().numpy() for t in sysm.results())
+
+    out = {
+        "metric": METRIC,
+        "value": s.n / t_pass,
+        "unit": "particles/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": t_pass * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": f"{args.workload}: N={s.n}, theta={args.theta}, cartesian quadrupole, bucket 64, "
+                        f"softening 1e-3, 64-target groups" + (", periodic 1 shell + Ewald" if spec["periodic"] else
+                                                               ", open boundaries"),
+            "n": s.n, "theta": args.theta, "bucket": 64, "group_size": 64,
+        },
+        "roofline": {
+            "bound": "valu",
+            "achieved": achieved,
+            "peak": PEAK_FP32_TFLOPS,
+            "unit": "TFLOP/s",
+            "frac": achieved / PEAK_FP32_TFLOPS,
+            "traffic": traffic_from_profiles(workload=args.workload),
+            "kernel": "traverseKernel",
+            "flops_per_launch": flops_ns,
+            "mean_launch_ms": t_trav * 1e3,
+        },
+        "tflops_code_count": flops_code / t_trav / 1e12,
+        "particles_per_s_traversal": s.n / t_trav,
+        "interactions_per_particle": {"p2p": stats["sumP2P"] / s.n, "m2p": stats["sumM2P"] / s.n,
+                                      "max_stack": stats["maxStack"]},
+        "phases_ms": {k: float(np.mean([ph[k] for ph in phase])) for k in phase[0]},
+        "tree": info,
+        "total_potential": pot,
+    }
+    if not args.no_accuracy:
+        out["accuracy_vs_fp64_direct"] = accuracy(ctx, sysm, s, order, ax, ay, az, pot)
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(s)
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

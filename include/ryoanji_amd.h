@@ -1,0 +1,216 @@
+/*
+ * ryoanji_amd.h -- C ABI of the MI355X (gfx950) Barnes-Hut gravity path.
+ *
+ * Plain pointers and sizes only. Every array argument is DEVICE memory owned by the caller unless
+ * it is documented as a host pointer. Every function is stream-ordered on the context's HIP stream,
+ * returns an int status (RYO_OK or an error code, message via ryo_last_error) and never exits the
+ * process. All per-call state (counters, statistics, scratch) lives in the context, so contexts on
+ * different streams are independent; calls that share one context must not overlap.
+ *
+ * Each entry point names the reference interface it replaces (paths relative to the reference
+ * checkout: cstone/include/cstone/..., ryoanji/src/ryoanji/...).
+ *
+ * Array conventions (reference types): keys are 64-bit SFC keys (HilbertKey<uint64_t>), node
+ * indices are int32 (TreeNodeIndex), particle indices uint32 (LocalIndex). Source centers are
+ * float4 {x, y, z, mac^2}; quadrupoles are 8 floats {M, qxx, qxy, qxz, qyy, qyz, qzz, trace}
+ * (CartesianQuadrupole<float>, ryoanji/nbody/cartesian_qpole.hpp:27-51).
+ */
+#ifndef RYOANJI_AMD_H
+#define RYOANJI_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum
+{
+    RYO_OK              = 0,
+    RYO_ERR_ARG         = 1, /* invalid argument (reference: std::runtime_error)          */
+    RYO_ERR_HIP         = 2, /* HIP runtime error (reference: checkGpuErrors -> exit())   */
+    RYO_ERR_UNSUPPORTED = 3, /* e.g. non-cubic box for keys or Ewald                      */
+    RYO_ERR_CAPACITY    = 4, /* output capacity too small; the needed size is reported    */
+    RYO_ERR_TRAVERSAL   = 5  /* a single target exhausted the traversal stack             */
+};
+
+typedef struct ryo_ctx ryo_ctx;
+
+/* ---------------------------------------------------------------- context */
+
+/* Replaces the reference's process-global device state (traversal_gpu.cu:351-356, csarray_gpu.cu:68,130,
+ * ewald.cu:23) with per-context state. `stream` is a hipStream_t (NULL: the null stream). */
+int ryo_ctx_create(ryo_ctx** ctx, void* stream);
+int ryo_ctx_destroy(ryo_ctx* ctx);
+const char* ryo_last_error(const ryo_ctx* ctx);
+int ryo_ctx_sync(ryo_ctx* ctx);
+/* enable HIP-event timing of the traversal kernel (ryo_last_traverse_ms) */
+int ryo_ctx_set_timing(ryo_ctx* ctx, int enable);
+double ryo_last_traverse_ms(const ryo_ctx* ctx);
+
+/* ---------------------------------------------------------------- a2: SFC keys */
+
+/* cstone::computeSfcKeys(execution::Gpu, x, y, z, keys, n, Box<float>) -- cstone/sfc/sfc_gpu.h:37,
+ * sfc_gpu.cu:23-44. limits = host float[6] {xmin,xmax,ymin,ymax,zmin,zmax}; kind 0 = Hilbert, 1 = Morton.
+ * With skip_removed != 0 a key already equal to 2^63 (removeKey) is left untouched, as in the reference;
+ * with 0 the keys array is write-only. Cubic boxes only (RYO_ERR_UNSUPPORTED otherwise). */
+int ryo_sfc_keys_f32(ryo_ctx* ctx, const float* x, const float* y, const float* z, uint64_t* keys, size_t n,
+                     const float* limits, int kind, int skip_removed);
+
+/* ---------------------------------------------------------------- a3: stable radix sort */
+
+/* cstone::sortByKeyTempStorage<uint64_t, int>(n) -- cstone/primitives/primitives_gpu.h:120-124 */
+size_t ryo_sort_temp_bytes(size_t n);
+/* cstone::sortByKey(exec, keys, keys + n, values, keyBuf, valueBuf, tmp, tmpBytes) --
+ * primitives_gpu.h:126-130, primitives_gpu.cu:310-358: stable LSD sort of (u64 key, i32 value) pairs over
+ * all 64 key bits; the result is in keys/values, keyBuf/valueBuf are scratch of n elements each. */
+int ryo_sort_pairs_u64_i32(ryo_ctx* ctx, uint64_t* keys, int32_t* values, size_t n, uint64_t* keyBuf,
+                           int32_t* valueBuf, void* tmp, size_t tmpBytes);
+/* cstone::sequence (primitives_gpu.cu:265-273) */
+int ryo_sequence_i32(ryo_ctx* ctx, int32_t* out, size_t n, int32_t start);
+
+/* ---------------------------------------------------------------- a4: permutation */
+
+/* out_k[i] = in_k[perm[i]] for k < num_arrays (<= 8) -- the thrust::gather of TreeBuilder::update
+ * (ryoanji/interface/treebuilder.cuh:74-79), applied to all arrays at once. `in`/`out` are HOST arrays of
+ * device pointers. */
+int ryo_gather_f32(ryo_ctx* ctx, const int32_t* perm, size_t n, int num_arrays, const float* const* in,
+                   float* const* out);
+
+/* ---------------------------------------------------------------- a5+a7: cornerstone leaves */
+
+/* cstone::computeOctree / updateOctreeGpu iterated to convergence (csarray.hpp:408-440, update_gpu.cuh:43-68)
+ * plus TreeBuilder's layout (treebuilder.cuh:95-98), built directly in one pass:
+ *   leaves  [capacity + 1] u64 : cornerstone leaf keys, leaves[L] = 2^63
+ *   counts  [capacity]     u32 : particles per leaf
+ *   layout  [capacity + 1] u32 : first particle of each leaf, layout[L] = n
+ * *num_leaves (host) receives L. If L > capacity, returns RYO_ERR_CAPACITY and *num_leaves = L. */
+int ryo_build_leaves_u64(ryo_ctx* ctx, const uint64_t* sorted_keys, size_t n, unsigned bucket_size,
+                         uint64_t* leaves, uint32_t* counts, uint32_t* layout, size_t capacity, int* num_leaves);
+
+/* ---------------------------------------------------------------- a6: linked octree */
+
+/* cstone::buildOctreeGpu(exec, leaves, OctreeView) -- cstone/tree/octree_gpu.h:35-45, octree_gpu.cu:138-200.
+ * numInternal = (L-1)/7, numNodes = L + numInternal. Sizes: prefixes, internal_to_leaf, leaf_to_internal
+ * [numNodes]; child_offsets [numNodes + 1]; parents [max(1, (numNodes-1)/8)]; level_range [23] on the
+ * device; level_range_host (optional, host int[23]) receives a copy, as OctreeData::levelRange. */
+int ryo_link_octree_u64(ryo_ctx* ctx, const uint64_t* leaves, int num_leaves, uint64_t* prefixes,
+                        int32_t* child_offsets, int32_t* parents, int32_t* level_range, int32_t* internal_to_leaf,
+                        int32_t* leaf_to_internal, int32_t* level_range_host);
+
+/* ---------------------------------------------------------------- a8-a12: centers, MAC, multipoles */
+
+/* Fused upsweep in FP64 (one launch, no per-level host loop): leaf mass centers and P2M
+ * (cstone::computeLeafSourceCenterGpu, ryoanji::computeLeafMultipoles), the |m|-weighted center and M2M
+ * combination of internal nodes up to the root (cstone::upsweepCentersGpu, ryoanji::upsweepMultipoles,
+ * source_center_gpu.h:66-116, upsweep_gpu.h:25,40) and the vector-MAC radius (cstone::setMacGpu,
+ * source_center_gpu.cu:284-310). Particles are SFC-ordered float x,y,z,m; tree arrays from
+ * ryo_link_octree_u64 and the leaf layout. Outputs: centers float4[numNodes] {com, mac^2},
+ * multipoles float[numNodes][8]; centers64 double[numNodes][4] {com, sum|m|} and multipoles64
+ * double[numNodes][8] are optional (NULL: kept internally). limits = host float[6]. */
+int ryo_upsweep_f32(ryo_ctx* ctx, const float* x, const float* y, const float* z, const float* m,
+                    const uint32_t* layout, int num_leaves, const uint64_t* prefixes, const int32_t* child_offsets,
+                    const int32_t* parents, const int32_t* leaf_to_internal, const float* limits, float inv_theta,
+                    float* centers, float* multipoles, double* centers64, double* multipoles64);
+
+/* ---------------------------------------------------------------- a13: target groups */
+
+/* cstone::computeFixedGroups (traversal/groups_gpu.cu:45-58): groups[g] = first + g*group_size,
+ * groups[G] = last, G = ceil((last-first)/group_size); groups has G+1 entries; *num_groups (host) = G. */
+int ryo_fixed_groups(ryo_ctx* ctx, uint32_t first, uint32_t last, unsigned group_size, uint32_t* groups,
+                     int* num_groups);
+
+/* ---------------------------------------------------------------- a14-a17: traversal */
+
+/* ryoanji::traverse (nbody/traversal_gpu.h:51-56, traversal_gpu.cu:572-623) for the float instantiation
+ * (Tc,Th,Tm,Ta,Tf) = (f,f,f,f,f) with CartesianQuadrupole<float>.
+ * group_start/group_end: num_groups target ranges (group_end may alias group_start + 1).
+ * init_node 0: the traversal starts at the root, as the reference CPU traversal (traversal.hpp:26-69);
+ * init_node 1: it starts with the root's children, as the reference GPU kernel.
+ * Outputs are ADDED to p (may be NULL; adds G m_i phi_i), ax, ay, az. *total_pot (host, may be NULL)
+ * receives 0.5 G sum_i m_i phi_i, reduced in FP64 in a fixed order. A group whose traversal overflows the
+ * per-wave stack is split and traversed again (splitFailedGroups, traversal_gpu.cu:647-677); a single
+ * target that still overflows gives RYO_ERR_TRAVERSAL. */
+int ryo_traverse_f32(ryo_ctx* ctx, int num_groups, const uint32_t* group_start, const uint32_t* group_end,
+                     int init_node, const float* xt, const float* yt, const float* zt, const float* mt,
+                     const float* ht, const float* xs, const float* ys, const float* zs, const float* ms,
+                     const float* hs, const int32_t* child_offsets, const int32_t* internal_to_leaf,
+                     const uint32_t* layout, const float* centers, const float* multipoles, float G, int num_shells,
+                     const float* box_l /* host float[3] */, float* p, float* ax, float* ay, float* az,
+                     double* total_pot);
+
+/* ryoanji::readBhStats (traversal_gpu.cu:688-700): host uint64[5] {sumP2P, maxP2P, sumM2P, maxM2P, maxStack}
+ * of the last ryo_traverse_f32 call (sums are interactions summed over targets). */
+int ryo_read_stats(ryo_ctx* ctx, uint64_t* stats);
+/* ryoanji::readTraversalRetries (traversal_gpu.cu:645) */
+uint64_t ryo_traversal_retries(const ryo_ctx* ctx);
+
+/* ---------------------------------------------------------------- a18: Ewald */
+
+/* ryoanji::computeGravityEwaldGpu (interface/ewald.cuh:25-27, ewald.cu:70-99) with an FP64 k-space and
+ * real-space sum, one particle per lane. root_multipole = host double[8] (FP64 root quadrupole),
+ * root_center = host double[3], limits = host float[6] (cubic), settings = host double[5]
+ * {numReplicaShells, lCut, hCut, alpha_scale, small_R_scale_factor} (EwaldSettings, nbody/ewald.h:24-31).
+ * Adds G m phi to ugrav (may be NULL) and G a to ax, ay, az for particles [first, last);
+ * *total_pot (host, may be NULL) receives 0.5 G sum m phi. */
+int ryo_ewald_f32(ryo_ctx* ctx, const double* root_multipole, const double* root_center, uint32_t first,
+                  uint32_t last, const float* x, const float* y, const float* z, const float* m, const float* limits,
+                  float G, const double* settings, float* ugrav, float* ax, float* ay, float* az, double* total_pot);
+
+/* ---------------------------------------------------------------- a19: direct sum (accuracy reference) */
+
+/* ryoanji::directSum (nbody/direct.cuh:88-107, traversal_cpu.hpp:232-268) with P2P<Exact> evaluated and
+ * accumulated in FP64 for a list of targets (targets == NULL: targets 0..num_targets-1) over all n sources and
+ * (2*num_shells+1)^3 images of box_l (host double[3]). out = double[num_targets][4] {G m_t phi_t, G a}. */
+int ryo_direct_sum_f64(ryo_ctx* ctx, const float* x, const float* y, const float* z, const float* h,
+                       const float* m, size_t n, const int64_t* targets, size_t num_targets, double G,
+                       const double* box_l, int num_shells, double* out);
+
+/* ---------------------------------------------------------------- system pipeline */
+
+/* One resident particle system: the ryoanji::TreeBuilder (interface/treebuilder.cuh:35-127) and
+ * MultipoleHolder (interface/multipole_holder.cuh:28-89) facades in one object. It owns device buffers for
+ * up to `capacity` particles and keeps keys, tree and multipoles on the device across calls. */
+typedef struct ryo_system ryo_system;
+
+typedef struct
+{
+    unsigned bucket_size; /* leaf capacity, 64 in the reference demos                   */
+    float theta;          /* opening angle                                              */
+    float G;              /* gravitational constant                                     */
+    int group_size;       /* targets per traversal group: 64 (one wave) or 16 (parity)  */
+    int num_shells;       /* periodic image shells in the traversal (0 = open)          */
+    int ewald;            /* add the Ewald correction (periodic boxes)                  */
+    double ewald_settings[5];
+    int init_node;        /* 0: traversal starts at the root, 1: at its children       */
+} ryo_params;
+
+void ryo_default_params(ryo_params* p);
+int ryo_system_create(ryo_ctx* ctx, size_t capacity, ryo_system** sys);
+int ryo_system_destroy(ryo_system* sys);
+/* device pointers of the system's particle arrays (x, y, z, m, h, vx, vy, vz), in its current (SFC) order */
+int ryo_system_arrays(ryo_system* sys, float** xyzmh_v /* host float*[8] */);
+/* copy n particles in (device or host memory, hipMemcpyDefault) */
+int ryo_system_load(ryo_system* sys, size_t n, const float* x, const float* y, const float* z, const float* m,
+                    const float* h, const float* limits /* host float[6] */, int periodic);
+/* keys -> sort -> permute (x,y,z,m,h and velocities) -> leaves -> linked octree -> FP64 upsweep */
+int ryo_system_build(ryo_system* sys, const ryo_params* params);
+/* traversal (+ Ewald); outputs in the system's SFC order, written (not added) to the system's p, ax, ay, az */
+int ryo_system_gravity(ryo_system* sys, const ryo_params* params, double* total_pot);
+/* device pointers of the results {p, ax, ay, az} and the original index of each particle (int32) */
+int ryo_system_results(ryo_system* sys, float** p_ax_ay_az /* host float*[4] */, int32_t** order);
+/* sizes: {n, numLeaves, numNodes, numGroups} */
+int ryo_system_info(ryo_system* sys, int64_t* info);
+/* tree arrays (device pointers) of the last build, for parity checks:
+ * {keys(u64), leaves(u64), counts(u32), layout(u32), prefixes(u64), childOffsets, parents, levelRange,
+ *  internalToLeaf, leafToInternal, centers(f32x4), multipoles(f32x8), centers64, multipoles64} */
+int ryo_system_tree(ryo_system* sys, void** ptrs /* host void*[14] */);
+/* phase times (ms) of the last build+gravity: {keys, sort, permute, leaves, link, upsweep, traverse, ewald} */
+int ryo_system_phase_ms(ryo_system* sys, double* ms);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* RYOANJI_AMD_H */

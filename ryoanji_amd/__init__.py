@@ -1,0 +1,294 @@
+"""MI355X (gfx950) Barnes-Hut gravity: Python host bindings over the C ABI (include/ryoanji_amd.h).
+
+The compute path is the HIP library ``libryoanji_amd.so`` built in this directory by ``build()`` in
+``__graft_entry__.py`` (``make -C ryoanji_amd/csrc``). Nothing here falls back to a CPU path: if the
+library is missing, importing ``lib()`` raises.
+
+Device memory is passed as raw pointers; helpers accept torch tensors on the GPU (``.data_ptr()``)
+and use torch only as an allocator and for host<->device copies.
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libryoanji_amd.so")
+
+RYO_OK, RYO_ERR_ARG, RYO_ERR_HIP, RYO_ERR_UNSUPPORTED, RYO_ERR_CAPACITY, RYO_ERR_TRAVERSAL = range(6)
+
+_P = C.c_void_p
+_I = C.c_int
+_U = C.c_uint
+_SZ = C.c_size_t
+_F = C.c_float
+_D = C.c_double
+
+
+class RyoError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"ryoanji_amd error {code}: {msg}")
+        self.code = code
+
+
+class Params(C.Structure):
+    """ryo_params: physics and tuning parameters of a solve."""
+
+    _fields_ = [
+        ("bucket_size", C.c_uint),
+        ("theta", C.c_float),
+        ("G", C.c_float),
+        ("group_size", C.c_int),
+        ("num_shells", C.c_int),
+        ("ewald", C.c_int),
+        ("ewald_settings", C.c_double * 5),
+        ("init_node", C.c_int),
+    ]
+
+
+# exported symbol -> (restype, argtypes)
+SIGNATURES = {
+    "ryo_ctx_create": (_I, [C.POINTER(_P), _P]),
+    "ryo_ctx_destroy": (_I, [_P]),
+    "ryo_last_error": (C.c_char_p, [_P]),
+    "ryo_ctx_sync": (_I, [_P]),
+    "ryo_ctx_set_timing": (_I, [_P, _I]),
+    "ryo_last_traverse_ms": (_D, [_P]),
+    "ryo_sfc_keys_f32": (_I, [_P, _P, _P, _P, _P, _SZ, _P, _I, _I]),
+    "ryo_sort_temp_bytes": (_SZ, [_SZ]),
+    "ryo_sort_pairs_u64_i32": (_I, [_P, _P, _P, _SZ, _P, _P, _P, _SZ]),
+    "ryo_sequence_i32": (_I, [_P, _P, _SZ, C.c_int32]),
+    "ryo_gather_f32": (_I, [_P, _P, _SZ, _I, _P, _P]),
+    "ryo_build_leaves_u64": (_I, [_P, _P, _SZ, _U, _P, _P, _P, _SZ, C.POINTER(_I)]),
+    "ryo_link_octree_u64": (_I, [_P, _P, _I, _P, _P, _P, _P, _P, _P, _P]),
+    "ryo_upsweep_f32": (_I, [_P, _P, _P, _P, _P, _P, _I, _P, _P, _P, _P, _P, _F, _P, _P, _P, _P]),
+    "ryo_fixed_groups": (_I, [_P, C.c_uint32, C.c_uint32, _U, _P, C.POINTER(_I)]),
+    "ryo_traverse_f32": (_I, [_P, _I, _P, _P, _I] + [_P] * 10 + [_P, _P, _P, _P, _P, _F, _I, _P, _P, _P, _P, _P,
+                                                                   _P]),
+    "ryo_read_stats": (_I, [_P, _P]),
+    "ryo_traversal_retries": (C.c_uint64, [_P]),
+    "ryo_ewald_f32": (_I, [_P, _P, _P, C.c_uint32, C.c_uint32, _P, _P, _P, _P, _P, _F, _P, _P, _P, _P, _P, _P]),
+    "ryo_direct_sum_f64": (_I, [_P, _P, _P, _P, _P, _P, _SZ, _P, _SZ, _D, _P, _I, _P]),
+    "ryo_default_params": (None, [C.POINTER(Params)]),
+    "ryo_system_create": (_I, [_P, _SZ, C.POINTER(_P)]),
+    "ryo_system_destroy": (_I, [_P]),
+    "ryo_system_arrays": (_I, [_P, _P]),
+    "ryo_system_load": (_I, [_P, _SZ, _P, _P, _P, _P, _P, _P, _I]),
+    "ryo_system_build": (_I, [_P, C.POINTER(Params)]),
+    "ryo_system_gravity": (_I, [_P, C.POINTER(Params), C.POINTER(_D)]),
+    "ryo_system_results": (_I, [_P, _P, _P]),
+    "ryo_system_info": (_I, [_P, _P]),
+    "ryo_system_tree": (_I, [_P, _P]),
+    "ryo_system_phase_ms": (_I, [_P, _P]),
+}
+# entry points that are not part of the public header (test hooks)
+EXTRA_SIGNATURES = {
+    "ryo_traverse_f32_stackcap": (_I, [_P, _I, _I, _P, _P, _I] + [_P] * 10 + [_P, _P, _P, _P, _P, _F, _I, _P, _P,
+                                                                            _P, _P, _P, _P]),
+}
+
+_lib = None
+
+
+def lib() -> C.CDLL:
+    """Load the gfx950 library (raises if it was not built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(f"{LIB_PATH} is missing: build it with `make -C ryoanji_amd/csrc` "
+                           "(or __graft_entry__.build())")
+    L = C.CDLL(LIB_PATH)
+    for name, (res, args) in {**SIGNATURES, **EXTRA_SIGNATURES}.items():
+        f = getattr(L, name)
+        f.restype = res
+        f.argtypes = args
+    _lib = L
+    return L
+
+
+def default_params(**kw) -> Params:
+    p = Params()
+    lib().ryo_default_params(C.byref(p))
+    for k, v in kw.items():
+        if k == "ewald_settings":
+            for i, s in enumerate(v):
+                p.ewald_settings[i] = s
+        else:
+            setattr(p, k, v)
+    return p
+
+
+def ptr(t) -> int | None:
+    """Raw pointer of a torch tensor / numpy array (None passes NULL)."""
+    if t is None:
+        return None
+    if hasattr(t, "data_ptr"):
+        return t.data_ptr()
+    return t.ctypes.data
+
+
+def host_f32(a):
+    return (C.c_float * len(a))(*[float(v) for v in a])
+
+
+def host_f64(a):
+    return (C.c_double * len(a))(*[float(v) for v in a])
+
+
+class Context:
+    """A ryo_ctx bound to a HIP stream (default: torch's current stream on the current device)."""
+
+    def __init__(self, stream=None):
+        import torch
+
+        if stream is None:
+            stream = torch.cuda.current_stream().cuda_stream
+        self.h = _P()
+        self._check(lib().ryo_ctx_create(C.byref(self.h), _P(stream)))
+
+    def _check(self, rc):
+        if rc != RYO_OK:
+            msg = lib().ryo_last_error(self.h).decode() if self.h else ""
+            raise RyoError(rc, msg)
+        return rc
+
+    def call(self, name, *args):
+        return self._check(getattr(lib(), name)(self.h, *args))
+
+    def close(self):
+        if self.h:
+            lib().ryo_ctx_destroy(self.h)
+            self.h = _P()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def stats(self):
+        s = (C.c_uint64 * 5)()
+        self.call("ryo_read_stats", s)
+        return dict(zip(("sumP2P", "maxP2P", "sumM2P", "maxM2P", "maxStack"), list(s)))
+
+
+class System:
+    """A resident particle system (ryo_system): keys, tree and multipoles stay on the device."""
+
+    def __init__(self, ctx: Context, capacity: int):
+        self.ctx = ctx
+        self.h = _P()
+        ctx._check(lib().ryo_system_create(ctx.h, capacity, C.byref(self.h)))
+        self.capacity = capacity
+
+    def _check(self, rc):
+        return self.ctx._check(rc)
+
+    def load(self, x, y, z, m, h, limits, periodic=False):
+        n = int(x.numel() if hasattr(x, "numel") else x.size)
+        self._check(lib().ryo_system_load(self.h, n, ptr(x), ptr(y), ptr(z), ptr(m), ptr(h), host_f32(limits),
+                                          int(periodic)))
+        self.n = n
+
+    def build(self, params: Params):
+        self._check(lib().ryo_system_build(self.h, C.byref(params)))
+
+    def gravity(self, params: Params) -> float:
+        pot = _D()
+        self._check(lib().ryo_system_gravity(self.h, C.byref(params), C.byref(pot)))
+        return pot.value
+
+    def info(self):
+        v = (C.c_int64 * 4)()
+        self._check(lib().ryo_system_info(self.h, v))
+        return dict(zip(("n", "numLeaves", "numNodes", "numGroups"), list(v)))
+
+    def phase_ms(self):
+        v = (C.c_double * 8)()
+        self._check(lib().ryo_system_phase_ms(self.h, v))
+        return dict(zip(("keys", "sort", "permute", "leaves", "link", "upsweep", "traverse", "ewald"), list(v)))
+
+    def _tensor(self, p, count, dtype):
+        """Copy `count` elements at device pointer p into a new torch tensor."""
+        import torch
+
+        out = torch.empty(count, dtype=dtype, device="cuda")
+        nbytes = out.element_size() * count
+        if count:
+            torch.cuda.current_stream().synchronize()
+            _memcpy_d2d(out.data_ptr(), p, nbytes)
+        return out
+
+    def arrays(self):
+        import torch
+
+        ptrs = (_P * 8)()
+        self._check(lib().ryo_system_arrays(self.h, ptrs))
+        return [self._tensor(ptrs[k], self.n, torch.float32) for k in range(8)]
+
+    def results(self):
+        """(p, ax, ay, az, order) as torch tensors, in the system's SFC order."""
+        import torch
+
+        outs = (_P * 4)()
+        order = _P()
+        self._check(lib().ryo_system_results(self.h, outs, C.byref(order)))
+        res = [self._tensor(outs[k], self.n, torch.float32) for k in range(4)]
+        res.append(self._tensor(order.value, self.n, torch.int32))
+        return res
+
+    def tree(self):
+        """Device tree arrays of the last build, copied into torch tensors."""
+        import torch
+
+        info = self.info()
+        L, nn = info["numLeaves"], info["numNodes"]
+        p = (_P * 14)()
+        self._check(lib().ryo_system_tree(self.h, p))
+        spec = [("keys", self.n, torch.int64), ("leaves", L + 1, torch.int64), ("counts", L, torch.int32),
+                ("layout", L + 1, torch.int32), ("prefixes", nn, torch.int64), ("childOffsets", nn, torch.int32),
+                ("parents", max(1, (nn - 1) // 8), torch.int32), ("levelRange", 23, torch.int32),
+                ("internalToLeaf", nn, torch.int32), ("leafToInternal", nn, torch.int32),
+                ("centers", 4 * nn, torch.float32), ("multipoles", 8 * nn, torch.float32),
+                ("centers64", 4 * nn, torch.float64), ("multipoles64", 8 * nn, torch.float64)]
+        return {name: self._tensor(p[k], cnt, dt) for k, (name, cnt, dt) in enumerate(spec)}
+
+    def close(self):
+        if self.h:
+            lib().ryo_system_destroy(self.h)
+            self.h = _P()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def _memcpy_d2d(dst, src, nbytes):
+    import torch
+
+    # a typed view over the raw pointer is not available in torch; use hipMemcpy via the HIP runtime
+    _hip().hipMemcpy(_P(dst), _P(src), _SZ(nbytes), 3)  # hipMemcpyDeviceToDevice
+    del torch
+
+
+_hiprt = None
+
+
+def _hip():
+    global _hiprt
+    if _hiprt is None:
+        for name in ("libamdhip64.so", "libamdhip64.so.7", "libamdhip64.so.6"):
+            try:
+                _hiprt = C.CDLL(name)
+                break
+            except OSError:
+                continue
+        if _hiprt is None:
+            raise RuntimeError("HIP runtime library not found")
+        _hiprt.hipMemcpy.restype = _I
+        _hiprt.hipMemcpy.argtypes = [_P, _P, _SZ, _I]
+    return _hiprt

@@ -1,0 +1,580 @@
+// C ABI (include/ryoanji_amd.h): contexts, the per-stage entry points and the resident-system pipeline.
+#include "common.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstring>
+
+namespace ryo
+{
+
+// stage implementations (keys_sort.hip, tree.hip, upsweep.hip, traverse.hip, ewald_direct.hip)
+int sfcKeys(Context*, const float*, const float*, const float*, uint64_t*, size_t, const float*, int, int);
+size_t sortTempBytes(size_t n);
+int sortPairs(Context*, uint64_t*, int32_t*, size_t, uint64_t*, int32_t*, void*, size_t);
+int sequence(Context*, int32_t*, size_t, int32_t);
+int gather(Context*, const int32_t*, size_t, int, const float* const*, float* const*);
+int buildLeaves(Context*, const uint64_t*, size_t, unsigned, uint64_t*, uint32_t*, uint32_t*, size_t, int*);
+int linkOctree(Context*, const uint64_t*, int, uint64_t*, int32_t*, int32_t*, int32_t*, int32_t*, int32_t*, int32_t*);
+int upsweep(Context*, const float*, const float*, const float*, const float*, const uint32_t*, int, const uint64_t*,
+            const int32_t*, const int32_t*, const int32_t*, const float*, float, float*, float*, double*, double*);
+int fixedGroups(Context*, uint32_t, uint32_t, unsigned, uint32_t*, int*);
+int ewald(Context*, const double*, const double*, uint32_t, uint32_t, const float*, const float*, const float*,
+          const float*, const float*, float, const double*, float*, float*, float*, float*, double*);
+int directSum(Context*, const float*, const float*, const float*, const float*, const float*, size_t,
+              const int64_t*, size_t, double, const double*, int, double*);
+
+struct TravArgs;
+int traverseEntry(Context* ctx, int numGroups, const uint32_t* gs, const uint32_t* ge, int initNode, const float* xt,
+                  const float* yt, const float* zt, const float* mt, const float* ht, const float* xs,
+                  const float* ys, const float* zs, const float* ms, const float* hs, const int32_t* child,
+                  const int32_t* i2l, const uint32_t* layout, const float* centers, const float* mp, float G,
+                  int numShells, const float* boxL, float* p, float* ax, float* ay, float* az, double* totalPot,
+                  int stackCap);
+
+int setError(Context* ctx, int code, const char* fmt, ...)
+{
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    if (ctx) ctx->lastError = buf;
+    return code;
+}
+
+void* Context::getScratch(int slot, size_t bytes)
+{
+    if (scratch.size() < size_t(kNumSlots)) scratch.resize(kNumSlots, nullptr), scratchBytes.resize(kNumSlots, 0);
+    if (bytes == 0) bytes = 1;
+    if (scratchBytes[slot] >= bytes) return scratch[slot];
+    if (scratch[slot])
+    {
+        // the old buffer may still be read by queued work
+        hipStreamSynchronize(stream);
+        hipFree(scratch[slot]);
+        scratch[slot] = nullptr, scratchBytes[slot] = 0;
+    }
+    size_t sz = std::max(bytes, size_t(4096));
+    sz        = sz + sz / 8;
+    if (hipMalloc(&scratch[slot], sz) != hipSuccess)
+    {
+        setError(this, RYO_ERR_HIP, "out of device memory allocating %zu bytes of scratch", sz);
+        scratch[slot] = nullptr;
+        return nullptr;
+    }
+    scratchBytes[slot] = sz;
+    return scratch[slot];
+}
+
+} // namespace ryo
+
+using namespace ryo;
+
+#define CTX_OR_FAIL(h)                                                                                                 \
+    if (!(h)) return RYO_ERR_ARG;                                                                                      \
+    Context* ctx = &(h)->c
+
+extern "C" {
+
+int ryo_ctx_create(ryo_ctx** out, void* stream)
+{
+    if (!out) return RYO_ERR_ARG;
+    auto* h      = new ryo_ctx();
+    Context* c   = &h->c;
+    c->stream    = static_cast<hipStream_t>(stream);
+    if (hipGetDevice(&c->device) != hipSuccess)
+    {
+        delete h;
+        return RYO_ERR_HIP;
+    }
+    int cus = 0;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c->device) == hipSuccess && cus > 0)
+        c->numCUs = cus;
+    if (hipMalloc(&c->dStats, 256) != hipSuccess)
+    {
+        delete h;
+        return RYO_ERR_HIP;
+    }
+    hipMemset(c->dStats, 0, 256);
+    hipEventCreate(&c->ev[0]);
+    hipEventCreate(&c->ev[1]);
+    *out = h;
+    return RYO_OK;
+}
+
+int ryo_ctx_destroy(ryo_ctx* h)
+{
+    if (!h) return RYO_OK;
+    Context* c = &h->c;
+    hipStreamSynchronize(c->stream);
+    for (void* p : c->scratch)
+        if (p) hipFree(p);
+    if (c->dStats) hipFree(c->dStats);
+    if (c->ev[0]) hipEventDestroy(c->ev[0]);
+    if (c->ev[1]) hipEventDestroy(c->ev[1]);
+    delete h;
+    return RYO_OK;
+}
+
+const char* ryo_last_error(const ryo_ctx* h) { return h ? h->c.lastError.c_str() : "null context"; }
+
+int ryo_ctx_sync(ryo_ctx* h)
+{
+    CTX_OR_FAIL(h);
+    RYO_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return RYO_OK;
+}
+
+int ryo_ctx_set_timing(ryo_ctx* h, int enable)
+{
+    CTX_OR_FAIL(h);
+    ctx->timing = enable != 0;
+    return RYO_OK;
+}
+
+double ryo_last_traverse_ms(const ryo_ctx* h) { return h ? h->c.lastTraverseMs : 0.0; }
+
+int ryo_sfc_keys_f32(ryo_ctx* h, const float* x, const float* y, const float* z, uint64_t* keys, size_t n,
+                     const float* limits, int kind, int skip_removed)
+{
+    CTX_OR_FAIL(h);
+    if (!limits || (n && (!x || !y || !z || !keys))) return setError(ctx, RYO_ERR_ARG, "sfc keys: null pointer");
+    return sfcKeys(ctx, x, y, z, keys, n, limits, kind, skip_removed);
+}
+
+size_t ryo_sort_temp_bytes(size_t n) { return sortTempBytes(n); }
+
+int ryo_sort_pairs_u64_i32(ryo_ctx* h, uint64_t* keys, int32_t* values, size_t n, uint64_t* keyBuf,
+                           int32_t* valueBuf, void* tmp, size_t tmpBytes)
+{
+    CTX_OR_FAIL(h);
+    if (n && (!keys || !values || !keyBuf || !valueBuf || !tmp)) return setError(ctx, RYO_ERR_ARG, "sort: null pointer");
+    if (n >= (size_t(1) << 31)) return setError(ctx, RYO_ERR_ARG, "sort: n must be < 2^31");
+    return sortPairs(ctx, keys, values, n, keyBuf, valueBuf, tmp, tmpBytes);
+}
+
+int ryo_sequence_i32(ryo_ctx* h, int32_t* out, size_t n, int32_t start)
+{
+    CTX_OR_FAIL(h);
+    return sequence(ctx, out, n, start);
+}
+
+int ryo_gather_f32(ryo_ctx* h, const int32_t* perm, size_t n, int num_arrays, const float* const* in,
+                   float* const* out)
+{
+    CTX_OR_FAIL(h);
+    return gather(ctx, perm, n, num_arrays, in, out);
+}
+
+int ryo_build_leaves_u64(ryo_ctx* h, const uint64_t* sorted_keys, size_t n, unsigned bucket_size, uint64_t* leaves,
+                         uint32_t* counts, uint32_t* layout, size_t capacity, int* num_leaves)
+{
+    CTX_OR_FAIL(h);
+    if (!num_leaves || !leaves || !counts || !layout) return setError(ctx, RYO_ERR_ARG, "leaves: null pointer");
+    return buildLeaves(ctx, sorted_keys, n, bucket_size, leaves, counts, layout, capacity, num_leaves);
+}
+
+int ryo_link_octree_u64(ryo_ctx* h, const uint64_t* leaves, int num_leaves, uint64_t* prefixes, int32_t* child,
+                        int32_t* parents, int32_t* level_range, int32_t* i2l, int32_t* l2i, int32_t* lr_host)
+{
+    CTX_OR_FAIL(h);
+    return linkOctree(ctx, leaves, num_leaves, prefixes, child, parents, level_range, i2l, l2i, lr_host);
+}
+
+int ryo_upsweep_f32(ryo_ctx* h, const float* x, const float* y, const float* z, const float* m,
+                    const uint32_t* layout, int num_leaves, const uint64_t* prefixes, const int32_t* child,
+                    const int32_t* parents, const int32_t* l2i, const float* limits, float inv_theta,
+                    float* centers, float* mp, double* c64, double* q64)
+{
+    CTX_OR_FAIL(h);
+    return upsweep(ctx, x, y, z, m, layout, num_leaves, prefixes, child, parents, l2i, limits, inv_theta, centers,
+                   mp, c64, q64);
+}
+
+int ryo_fixed_groups(ryo_ctx* h, uint32_t first, uint32_t last, unsigned group_size, uint32_t* groups,
+                     int* num_groups)
+{
+    CTX_OR_FAIL(h);
+    if (!groups || !num_groups) return setError(ctx, RYO_ERR_ARG, "groups: null pointer");
+    return fixedGroups(ctx, first, last, group_size, groups, num_groups);
+}
+
+int ryo_traverse_f32(ryo_ctx* h, int num_groups, const uint32_t* gs, const uint32_t* ge, int init_node,
+                     const float* xt, const float* yt, const float* zt, const float* mt, const float* ht,
+                     const float* xs, const float* ys, const float* zs, const float* ms, const float* hs,
+                     const int32_t* child, const int32_t* i2l, const uint32_t* layout, const float* centers,
+                     const float* mp, float G, int num_shells, const float* box_l, float* p, float* ax, float* ay,
+                     float* az, double* total_pot)
+{
+    CTX_OR_FAIL(h);
+    if (init_node != 0 && init_node != 1) return setError(ctx, RYO_ERR_ARG, "traverse: init_node must be 0 or 1");
+    if (num_shells < 0 || (num_shells > 0 && !box_l)) return setError(ctx, RYO_ERR_ARG, "traverse: bad shells");
+    return traverseEntry(ctx, num_groups, gs, ge, init_node, xt, yt, zt, mt, ht, xs, ys, zs, ms, hs, child, i2l,
+                         layout, centers, mp, G, num_shells, box_l, p, ax, ay, az, total_pot, 0);
+}
+
+/* test hook: the traversal with a reduced per-wave stack capacity (forces the split/retry path) */
+int ryo_traverse_f32_stackcap(ryo_ctx* h, int stack_cap, int num_groups, const uint32_t* gs, const uint32_t* ge,
+                              int init_node, const float* xt, const float* yt, const float* zt, const float* mt,
+                              const float* ht, const float* xs, const float* ys, const float* zs, const float* ms,
+                              const float* hs, const int32_t* child, const int32_t* i2l, const uint32_t* layout,
+                              const float* centers, const float* mp, float G, int num_shells, const float* box_l,
+                              float* p, float* ax, float* ay, float* az, double* total_pot)
+{
+    CTX_OR_FAIL(h);
+    return traverseEntry(ctx, num_groups, gs, ge, init_node, xt, yt, zt, mt, ht, xs, ys, zs, ms, hs, child, i2l,
+                         layout, centers, mp, G, num_shells, box_l, p, ax, ay, az, total_pot, stack_cap);
+}
+
+int ryo_read_stats(ryo_ctx* h, uint64_t* stats)
+{
+    CTX_OR_FAIL(h);
+    for (int k = 0; k < 5; ++k)
+        stats[k] = ctx->hStats[k];
+    return RYO_OK;
+}
+
+uint64_t ryo_traversal_retries(const ryo_ctx* h) { return h ? h->c.retriedGroups : 0; }
+
+int ryo_ewald_f32(ryo_ctx* h, const double* Mroot, const double* center, uint32_t first, uint32_t last,
+                  const float* x, const float* y, const float* z, const float* m, const float* limits, float G,
+                  const double* settings, float* ugrav, float* ax, float* ay, float* az, double* total_pot)
+{
+    CTX_OR_FAIL(h);
+    return ewald(ctx, Mroot, center, first, last, x, y, z, m, limits, G, settings, ugrav, ax, ay, az, total_pot);
+}
+
+int ryo_direct_sum_f64(ryo_ctx* h, const float* x, const float* y, const float* z, const float* hh, const float* m,
+                       size_t n, const int64_t* targets, size_t nt, double G, const double* box_l, int num_shells,
+                       double* out)
+{
+    CTX_OR_FAIL(h);
+    return directSum(ctx, x, y, z, hh, m, n, targets, nt, G, box_l, num_shells, out);
+}
+
+} // extern "C"
+
+/* ============================================================================ resident system */
+
+struct ryo_system
+{
+    Context* ctx;
+    ryo_ctx* handle;
+    size_t cap = 0, n = 0;
+    float limits[6] = {0, 0, 0, 0, 0, 0};
+    int periodic     = 0;
+    // particle arrays, current (SFC) order, and a second set for permutation
+    float* a[2][9] = {}; // x y z m h vx vy vz order(i32 bits)
+    int cur = 0;
+    uint64_t *keys = nullptr, *keyBuf = nullptr;
+    int32_t *perm = nullptr, *permBuf = nullptr;
+    void* sortTmp = nullptr;
+    size_t sortTmpBytes = 0;
+    float *p = nullptr, *ax = nullptr, *ay = nullptr, *az = nullptr;
+    // tree
+    size_t leafCap = 0;
+    int L = 0, nNodes = 0, G = 0;
+    uint64_t* leaves = nullptr;
+    uint32_t *counts = nullptr, *layout = nullptr;
+    size_t nodeCap = 0;
+    uint64_t* prefixes = nullptr;
+    int32_t *child = nullptr, *parents = nullptr, *levelRange = nullptr, *i2l = nullptr, *l2i = nullptr;
+    float *centers = nullptr, *mp = nullptr;
+    double *c64 = nullptr, *q64 = nullptr;
+    size_t groupCap = 0;
+    uint32_t* groups = nullptr;
+    int levelRangeHost[24] = {};
+    hipEvent_t ev[9] = {};
+    double phaseMs[8] = {};
+};
+
+namespace
+{
+template<class T>
+int devAlloc(ryo_system* s, T** p, size_t count)
+{
+    if (*p) hipFree(*p);
+    *p = nullptr;
+    if (hipMalloc(reinterpret_cast<void**>(p), std::max<size_t>(count, 1) * sizeof(T)) != hipSuccess)
+        return setError(s->ctx, RYO_ERR_HIP, "system: out of device memory (%zu bytes)", count * sizeof(T));
+    return RYO_OK;
+}
+
+int growTree(ryo_system* s, int L)
+{
+    int rc = RYO_OK;
+    if (size_t(L) > s->leafCap)
+    {
+        size_t c = size_t(L) + size_t(L) / 4 + 64;
+        if ((rc = devAlloc(s, &s->leaves, c + 1)) || (rc = devAlloc(s, &s->counts, c)) ||
+            (rc = devAlloc(s, &s->layout, c + 1)))
+            return rc;
+        s->leafCap = c;
+    }
+    size_t nodes = size_t(L) + size_t(L - 1) / 7;
+    if (nodes > s->nodeCap)
+    {
+        size_t c = nodes + nodes / 4 + 64;
+        if ((rc = devAlloc(s, &s->prefixes, c)) || (rc = devAlloc(s, &s->child, c + 1)) ||
+            (rc = devAlloc(s, &s->parents, c / 8 + 2)) || (rc = devAlloc(s, &s->i2l, c)) ||
+            (rc = devAlloc(s, &s->l2i, c)) || (rc = devAlloc(s, &s->centers, 4 * c)) ||
+            (rc = devAlloc(s, &s->mp, 8 * c)) || (rc = devAlloc(s, &s->c64, 4 * c)) || (rc = devAlloc(s, &s->q64, 8 * c)))
+            return rc;
+        s->nodeCap = c;
+    }
+    return RYO_OK;
+}
+} // namespace
+
+extern "C" {
+
+void ryo_default_params(ryo_params* p)
+{
+    p->bucket_size = 64;
+    p->theta       = 0.5f;
+    p->G           = 1.0f;
+    p->group_size  = 64;
+    p->num_shells  = 0;
+    p->ewald       = 0;
+    const double def[5] = {1, 2.6, 2.8, 2.0, 3.0e-3};
+    for (int k = 0; k < 5; ++k)
+        p->ewald_settings[k] = def[k];
+    p->init_node = 0;
+}
+
+int ryo_system_create(ryo_ctx* h, size_t capacity, ryo_system** out)
+{
+    CTX_OR_FAIL(h);
+    if (!out || capacity == 0) return setError(ctx, RYO_ERR_ARG, "system: capacity must be positive");
+    auto* s   = new ryo_system();
+    s->ctx    = ctx;
+    s->handle = h;
+    s->cap    = capacity;
+    int rc    = RYO_OK;
+    for (int b = 0; b < 2 && !rc; ++b)
+        for (int k = 0; k < 9 && !rc; ++k)
+            rc = devAlloc(s, &s->a[b][k], capacity);
+    if (!rc) rc = devAlloc(s, &s->keys, capacity);
+    if (!rc) rc = devAlloc(s, &s->keyBuf, capacity);
+    if (!rc) rc = devAlloc(s, &s->perm, capacity);
+    if (!rc) rc = devAlloc(s, &s->permBuf, capacity);
+    s->sortTmpBytes = sortTempBytes(capacity);
+    if (!rc) rc = devAlloc(s, reinterpret_cast<char**>(&s->sortTmp), s->sortTmpBytes);
+    if (!rc) rc = devAlloc(s, &s->p, capacity);
+    if (!rc) rc = devAlloc(s, &s->ax, capacity);
+    if (!rc) rc = devAlloc(s, &s->ay, capacity);
+    if (!rc) rc = devAlloc(s, &s->az, capacity);
+    for (auto& e : s->ev)
+        if (!rc && hipEventCreate(&e) != hipSuccess) rc = setError(ctx, RYO_ERR_HIP, "system: event creation");
+    if (rc)
+    {
+        ryo_system_destroy(s);
+        return rc;
+    }
+    *out = s;
+    return RYO_OK;
+}
+
+int ryo_system_destroy(ryo_system* s)
+{
+    if (!s) return RYO_OK;
+    hipStreamSynchronize(s->ctx->stream);
+    for (auto& b : s->a)
+        for (auto* p : b)
+            if (p) hipFree(p);
+    void* ptrs[] = {s->keys,    s->keyBuf, s->perm,    s->permBuf, s->sortTmp, s->p,   s->ax,   s->ay,
+                    s->az,      s->leaves, s->counts,  s->layout,  s->prefixes, s->child, s->parents, s->i2l,
+                    s->l2i,     s->centers, s->mp,     s->c64,     s->q64,     s->groups, s->levelRange};
+    for (void* p : ptrs)
+        if (p) hipFree(p);
+    for (auto& e : s->ev)
+        if (e) hipEventDestroy(e);
+    delete s;
+    return RYO_OK;
+}
+
+int ryo_system_arrays(ryo_system* s, float** out)
+{
+    if (!s || !out) return RYO_ERR_ARG;
+    for (int k = 0; k < 8; ++k)
+        out[k] = s->a[s->cur][k];
+    return RYO_OK;
+}
+
+int ryo_system_load(ryo_system* s, size_t n, const float* x, const float* y, const float* z, const float* m,
+                    const float* h, const float* limits, int periodic)
+{
+    if (!s) return RYO_ERR_ARG;
+    Context* ctx = s->ctx;
+    if (n > s->cap) return setError(ctx, RYO_ERR_CAPACITY, "system: %zu particles > capacity %zu", n, s->cap);
+    s->n = n;
+    std::memcpy(s->limits, limits, sizeof(s->limits));
+    s->periodic = periodic;
+    s->cur      = 0;
+    const float* src[5] = {x, y, z, m, h};
+    for (int k = 0; k < 5; ++k)
+        RYO_HIP(ctx, hipMemcpyAsync(s->a[0][k], src[k], n * sizeof(float), hipMemcpyDefault, ctx->stream));
+    for (int k = 5; k < 8; ++k)
+        RYO_HIP(ctx, hipMemsetAsync(s->a[0][k], 0, n * sizeof(float), ctx->stream));
+    int rc = sequence(ctx, reinterpret_cast<int32_t*>(s->a[0][8]), n, 0);
+    if (rc) return rc;
+    RYO_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return RYO_OK;
+}
+
+static void mark(ryo_system* s, int i)
+{
+    if (s->ctx->timing) hipEventRecord(s->ev[i], s->ctx->stream);
+}
+
+int ryo_system_build(ryo_system* s, const ryo_params* prm)
+{
+    if (!s || !prm) return RYO_ERR_ARG;
+    Context* ctx = s->ctx;
+    const size_t n = s->n;
+    if (n == 0) return setError(ctx, RYO_ERR_ARG, "system: no particles loaded");
+    float** A = s->a[s->cur];
+    float** B = s->a[s->cur ^ 1];
+    int rc;
+    mark(s, 0);
+    if ((rc = sfcKeys(ctx, A[0], A[1], A[2], s->keys, n, s->limits, 0, 0))) return rc;
+    mark(s, 1);
+    if ((rc = sequence(ctx, s->perm, n, 0))) return rc;
+    if ((rc = sortPairs(ctx, s->keys, s->perm, n, s->keyBuf, s->permBuf, s->sortTmp, s->sortTmpBytes))) return rc;
+    mark(s, 2);
+    if ((rc = gather(ctx, s->perm, n, 8, const_cast<const float* const*>(A), B))) return rc;
+    {
+        const float* in1[1] = {A[8]};
+        float* out1[1]      = {B[8]};
+        if ((rc = gather(ctx, s->perm, n, 1, in1, out1))) return rc;
+    }
+    s->cur ^= 1;
+    A = s->a[s->cur];
+    mark(s, 3);
+    // leaves: count pass + host read of L, then emission into buffers of the right size
+    int L = 0;
+    if (s->leafCap == 0 && (rc = growTree(s, int(std::max<size_t>(64, n / 8))))) return rc;
+    rc = buildLeaves(ctx, s->keys, n, prm->bucket_size, s->leaves, s->counts, s->layout, s->leafCap, &L);
+    if (rc == RYO_ERR_CAPACITY)
+    {
+        if ((rc = growTree(s, L))) return rc;
+        rc = buildLeaves(ctx, s->keys, n, prm->bucket_size, s->leaves, s->counts, s->layout, s->leafCap, &L);
+    }
+    if (rc) return rc;
+    if ((rc = growTree(s, L))) return rc;
+    s->L      = L;
+    s->nNodes = L + (L - 1) / 7;
+    mark(s, 4);
+    if (!s->levelRange && (rc = devAlloc(s, &s->levelRange, 24))) return rc;
+    if ((rc = linkOctree(ctx, s->leaves, L, s->prefixes, s->child, s->parents, s->levelRange, s->i2l, s->l2i,
+                         nullptr)))
+        return rc;
+    mark(s, 5);
+    if ((rc = upsweep(ctx, A[0], A[1], A[2], A[3], s->layout, L, s->prefixes, s->child, s->parents, s->l2i, s->limits,
+                      1.0f / prm->theta, s->centers, s->mp, s->c64, s->q64)))
+        return rc;
+    mark(s, 6);
+    int gsz = prm->group_size > 0 ? prm->group_size : 64;
+    if (gsz > 64) return setError(ctx, RYO_ERR_ARG, "system: group size must be <= 64");
+    size_t G = (n + gsz - 1) / gsz;
+    if (G + 1 > s->groupCap)
+    {
+        if ((rc = devAlloc(s, &s->groups, G + 1 + G / 4))) return rc;
+        s->groupCap = G + 1 + G / 4;
+    }
+    if ((rc = fixedGroups(ctx, 0, uint32_t(n), unsigned(gsz), s->groups, &s->G))) return rc;
+    if (ctx->timing)
+    {
+        RYO_HIP(ctx, hipEventSynchronize(s->ev[6]));
+        for (int k = 0; k < 6; ++k)
+        {
+            float ms = 0;
+            hipEventElapsedTime(&ms, s->ev[k], s->ev[k + 1]);
+            s->phaseMs[k] = ms;
+        }
+    }
+    return RYO_OK;
+}
+
+int ryo_system_gravity(ryo_system* s, const ryo_params* prm, double* totalPot)
+{
+    if (!s || !prm) return RYO_ERR_ARG;
+    Context* ctx = s->ctx;
+    const size_t n = s->n;
+    float** A = s->a[s->cur];
+    int rc;
+    for (float* o : {s->p, s->ax, s->ay, s->az})
+        RYO_HIP(ctx, hipMemsetAsync(o, 0, n * sizeof(float), ctx->stream));
+    float boxL[3] = {s->limits[1] - s->limits[0], s->limits[3] - s->limits[2], s->limits[5] - s->limits[4]};
+    int shells    = s->periodic ? prm->num_shells : 0;
+    double pot    = 0;
+    rc = traverseEntry(ctx, s->G, s->groups, s->groups + 1, prm->init_node, A[0], A[1], A[2], A[3], A[4], A[0], A[1],
+                       A[2], A[3], A[4], s->child, s->i2l, s->layout, s->centers, s->mp, prm->G, shells, boxL, s->p,
+                       s->ax, s->ay, s->az, &pot, 0);
+    if (rc) return rc;
+    s->phaseMs[6] = ctx->lastTraverseMs;
+    s->phaseMs[7] = 0;
+    if (s->periodic && prm->ewald)
+    {
+        double root[8], rc64[4];
+        RYO_HIP(ctx, hipMemcpyAsync(root, s->q64, sizeof(root), hipMemcpyDeviceToHost, ctx->stream));
+        RYO_HIP(ctx, hipMemcpyAsync(rc64, s->c64, sizeof(rc64), hipMemcpyDeviceToHost, ctx->stream));
+        RYO_HIP(ctx, hipStreamSynchronize(ctx->stream));
+        double ep = 0;
+        mark(s, 7);
+        if ((rc = ewald(ctx, root, rc64, 0, uint32_t(n), A[0], A[1], A[2], A[3], s->limits, prm->G,
+                        prm->ewald_settings, s->p, s->ax, s->ay, s->az, &ep)))
+            return rc;
+        mark(s, 8);
+        if (ctx->timing)
+        {
+            float ms = 0;
+            RYO_HIP(ctx, hipEventSynchronize(s->ev[8]));
+            hipEventElapsedTime(&ms, s->ev[7], s->ev[8]);
+            s->phaseMs[7] = ms;
+        }
+        pot += ep;
+    }
+    if (totalPot) *totalPot = pot;
+    return RYO_OK;
+}
+
+int ryo_system_results(ryo_system* s, float** out, int32_t** order)
+{
+    if (!s) return RYO_ERR_ARG;
+    if (out)
+    {
+        out[0] = s->p, out[1] = s->ax, out[2] = s->ay, out[3] = s->az;
+    }
+    if (order) *order = reinterpret_cast<int32_t*>(s->a[s->cur][8]);
+    return RYO_OK;
+}
+
+int ryo_system_info(ryo_system* s, int64_t* info)
+{
+    if (!s || !info) return RYO_ERR_ARG;
+    info[0] = int64_t(s->n), info[1] = s->L, info[2] = s->nNodes, info[3] = s->G;
+    return RYO_OK;
+}
+
+int ryo_system_tree(ryo_system* s, void** p)
+{
+    if (!s || !p) return RYO_ERR_ARG;
+    void* v[14] = {s->keys, s->leaves, s->counts, s->layout, s->prefixes, s->child, s->parents, s->levelRange,
+                   s->i2l,  s->l2i,    s->centers, s->mp,    s->c64,      s->q64};
+    for (int k = 0; k < 14; ++k)
+        p[k] = v[k];
+    return RYO_OK;
+}
+
+int ryo_system_phase_ms(ryo_system* s, double* ms)
+{
+    if (!s || !ms) return RYO_ERR_ARG;
+    for (int k = 0; k < 8; ++k)
+        ms[k] = s->phaseMs[k];
+    return RYO_OK;
+}
+
+} // extern "C"

@@ -1,0 +1,344 @@
+// FP64 Ewald periodic correction and the FP64 direct-sum accuracy reference (gfx950).
+//
+// Ewald: the correction from the root quadrupole of a cubic periodic box (ryoanji/nbody/ewald.hpp:90-327):
+// real-space images within lCut (full erfc outside the replica shells, -erf inside them, a series for small
+// R) and a k-space sum over the lattice vectors |h| <= hCut. One particle per lane; every term in FP64; the
+// k-space table is built on the host (ewaldInitParameters, ewald.hpp:133-198) and staged in LDS.
+// Direct sum: P2P<Exact> (kernel.hpp:99-120) over all sources and images, accumulated in FP64, for a
+// list of sampled targets (the accuracy oracle; direct.cuh:36-107 is its FP32 reference counterpart).
+#include "common.hpp"
+
+#include <cmath>
+
+namespace ryo
+{
+
+constexpr int kMaxH = 343;
+
+struct EwaldTable
+{
+    double Mroot[8];
+    double center[3];
+    int numReplicaShells, numEwaldShells, numH;
+    double L, lCut, alphaScale, smallR;
+    double h[kMaxH][5]; // {2 pi h / L (3), hfac_cos, hfac_sin}
+};
+
+// gamma-weighted quadrupole evaluation (ewaldEvalMultipoleComplete, ewald.hpp:90-115)
+__host__ __device__ inline void ewaldEval(double* pa, const double* r, const double* g, const double* mp)
+{
+    double M   = mp[0];
+    double qxx = (mp[1] + mp[7]) / 3.0, qyy = (mp[4] + mp[7]) / 3.0, qzz = (mp[6] + mp[7]) / 3.0;
+    double qxy = mp[2] / 3.0, qxz = mp[3] / 3.0, qyz = mp[5] / 3.0;
+    double Qr[3] = {r[0] * qxx + r[1] * qxy + r[2] * qxz, r[0] * qxy + r[1] * qyy + r[2] * qyz,
+                    r[0] * qxz + r[1] * qyz + r[2] * qzz};
+    double rQr = 0.5 * (r[0] * Qr[0] + r[1] * Qr[1] + r[2] * Qr[2]);
+    double Qtr = 0.5 * mp[7];
+    pa[0] += -g[0] * M + g[1] * Qtr - g[2] * rQr;
+    double f = g[1] * M - g[2] * Qtr + g[3] * rQr;
+    for (int d = 0; d < 3; ++d)
+        pa[1 + d] += g[2] * Qr[d] - r[d] * f;
+}
+
+void ewaldInitTable(EwaldTable& t, const double* Mroot, const double* center, int numReplicaShells, double L,
+                    double lCut, double hCut, double alphaScale, double smallR)
+{
+    if (lCut == 0 && hCut == 0 && alphaScale == 0) numReplicaShells = 0;
+    for (int k = 0; k < 8; ++k)
+        t.Mroot[k] = Mroot[k];
+    for (int k = 0; k < 3; ++k)
+        t.center[k] = center[k];
+    t.numReplicaShells = numReplicaShells;
+    t.numEwaldShells   = std::max(int(std::ceil(lCut)), numReplicaShells);
+    t.L = L, t.lCut = lCut, t.alphaScale = alphaScale, t.smallR = smallR;
+    t.numH = 0;
+    if (t.numEwaldShells == 0) return;
+    const int hReps    = int(std::ceil(hCut));
+    const double alpha = alphaScale / L;
+    const double k4    = M_PI * M_PI / (alpha * alpha * L * L);
+    for (int hx = -hReps; hx <= hReps; hx++)
+        for (int hy = -hReps; hy <= hReps; hy++)
+            for (int hz = -hReps; hz <= hReps; hz++)
+            {
+                double hr[3] = {double(hx), double(hy), double(hz)};
+                double h2    = hr[0] * hr[0] + hr[1] * hr[1] + hr[2] * hr[2];
+                if (h2 == 0 || h2 > hCut * hCut || t.numH >= kMaxH) continue;
+                double g0 = std::exp(-k4 * h2) / (M_PI * h2 * L);
+                double g1 = 2 * M_PI / L * g0;
+                double g2 = -2 * M_PI / L * g1;
+                double g3 = 2 * M_PI / L * g2;
+                double g4 = -2 * M_PI / L * g3;
+                double g5 = 2 * M_PI / L * g4;
+                double gc[6] = {g0, 0, g2, 0, g4, 0}, gs[6] = {0, g1, 0, g3, 0, g5};
+                double a[4] = {0, 0, 0, 0}, b[4] = {0, 0, 0, 0};
+                ewaldEval(a, hr, gc, t.Mroot);
+                ewaldEval(b, hr, gs, t.Mroot);
+                double* e = t.h[t.numH++];
+                for (int d = 0; d < 3; ++d)
+                    e[d] = 2 * M_PI / L * hr[d];
+                e[3] = a[0];
+                e[4] = b[0];
+            }
+}
+
+__global__ __launch_bounds__(256) void ewaldKernel(const EwaldTable* __restrict__ tab, uint32_t first, uint32_t last,
+                                                  const float* __restrict__ x, const float* __restrict__ y,
+                                                  const float* __restrict__ z, const float* __restrict__ m, float G,
+                                                  float* ugrav, float* ax, float* ay, float* az,
+                                                  double* __restrict__ blockPot)
+{
+    __shared__ double sh[kMaxH][5];
+    __shared__ double sMp[8];
+    __shared__ double sPot[256];
+    const int numH = tab->numH;
+    for (int i = threadIdx.x; i < numH * 5; i += blockDim.x)
+        (&sh[0][0])[i] = (&tab->h[0][0])[i];
+    if (threadIdx.x < 8) sMp[threadIdx.x] = tab->Mroot[threadIdx.x];
+    __syncthreads();
+
+    const double L = tab->L, lCut = tab->lCut;
+    const double lCut2  = lCut * lCut * L * L;
+    const double alpha  = tab->alphaScale / L, alpha2 = alpha * alpha;
+    const double k1     = M_PI / (alpha2 * L * L * L);
+    const double ka     = 2.0 * alpha / sqrt(M_PI);
+    const double smallR2 = tab->smallR * L * L;
+    const int ns = tab->numEwaldShells, nr = tab->numReplicaShells;
+
+    uint32_t i = first + blockIdx.x * blockDim.x + threadIdx.x;
+    double myPot = 0;
+    if (i < last)
+    {
+        double px = x[i], py = y[i], pz = z[i];
+        double r[3] = {px - tab->center[0], py - tab->center[1], pz - tab->center[2]};
+        double pa[4] = {k1 * sMp[0], 0, 0, 0};
+        for (int ix = -ns; ix <= ns; ix++)
+            for (int iy = -ns; iy <= ns; iy++)
+                for (int iz = -ns; iz <= ns; iz++)
+                {
+                    bool inner = (-nr <= ix && ix <= nr) && (-nr <= iy && iy <= nr) && (-nr <= iz && iz <= nr);
+                    double R[3] = {r[0] + ix * L, r[1] + iy * L, r[2] + iz * L};
+                    double R2   = R[0] * R[0] + R[1] * R[1] + R[2] * R[2];
+                    if (R2 > lCut2 && !inner) continue;
+                    double g[6];
+                    if (R2 < smallR2 && ka > 0)
+                    {
+                        double c0 = ka, R2a2 = R2 * alpha2;
+                        g[0] = c0 * (R2a2 / 3.0 - 1.0);
+                        c0 *= 2 * alpha2;
+                        g[1] = c0 * (R2a2 / 5.0 - 1.0 / 3.0);
+                        c0 *= 2 * alpha2;
+                        g[2] = c0 * (R2a2 / 7.0 - 1.0 / 5.0);
+                        c0 *= 2 * alpha2;
+                        g[3] = c0 * (R2a2 / 9.0 - 1.0 / 7.0);
+                        c0 *= 2 * alpha2;
+                        g[4] = c0 * (R2a2 / 11.0 - 1.0 / 9.0);
+                        c0 *= 2 * alpha2;
+                        g[5] = c0 * (R2a2 / 13.0 - 1.0 / 11.0);
+                    }
+                    else
+                    {
+                        double Rm = sqrt(R2), invR = 1.0 / Rm, invR2 = invR * invR;
+                        double a  = exp(-R2 * alpha2) * ka * invR2;
+                        double an = 1.0;
+                        double fn = inner ? -erf(alpha * Rm) : erfc(alpha * Rm);
+                        g[0] = fn * invR;
+                        g[1] = g[0] * invR2 + a;
+                        an *= 2 * alpha2;
+                        g[2] = 3 * g[1] * invR2 + an * a;
+                        an *= 2 * alpha2;
+                        g[3] = 5 * g[2] * invR2 + an * a;
+                        an *= 2 * alpha2;
+                        g[4] = 7 * g[3] * invR2 + an * a;
+                        an *= 2 * alpha2;
+                        g[5] = 9 * g[4] * invR2 + an * a;
+                    }
+                    ewaldEval(pa, R, g, sMp);
+                }
+        double kp[4] = {0, 0, 0, 0};
+        for (int k = 0; k < numH; ++k)
+        {
+            double hx = sh[k][0] * r[0] + sh[k][1] * r[1] + sh[k][2] * r[2];
+            double s, c;
+            sincos(hx, &s, &c);
+            double sum = sh[k][3] * c + sh[k][4] * s;
+            double dif = sh[k][3] * s - sh[k][4] * c;
+            kp[0] -= sum;
+            kp[1] += dif * sh[k][0];
+            kp[2] += dif * sh[k][1];
+            kp[3] += dif * sh[k][2];
+        }
+        double pot = pa[0] + kp[0];
+        double mi  = m[i];
+        if (ugrav) ugrav[i] += float(double(G) * pot * mi);
+        ax[i] += float(double(G) * (pa[1] + kp[1]));
+        ay[i] += float(double(G) * (pa[2] + kp[2]));
+        az[i] += float(double(G) * (pa[3] + kp[3]));
+        myPot = pot * mi;
+    }
+    sPot[threadIdx.x] = myPot;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1)
+    {
+        if (int(threadIdx.x) < o) sPot[threadIdx.x] += sPot[threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) blockPot[blockIdx.x] = sPot[0];
+}
+
+__global__ __launch_bounds__(256) void sumFixedKernel(const double* __restrict__ v, int n, double* __restrict__ out)
+{
+    __shared__ double ss[256];
+    double s = 0;
+    for (int i = threadIdx.x; i < n; i += 256)
+        s += v[i];
+    ss[threadIdx.x] = s;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1)
+    {
+        if (int(threadIdx.x) < o) ss[threadIdx.x] += ss[threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) *out = ss[0];
+}
+
+int ewald(Context* ctx, const double* Mroot, const double* center, uint32_t first, uint32_t last, const float* x,
+          const float* y, const float* z, const float* m, const float* limits, float G, const double* settings,
+          float* ugrav, float* ax, float* ay, float* az, double* totalPot)
+{
+    float lx = limits[1] - limits[0], ly = limits[3] - limits[2], lz = limits[5] - limits[4];
+    if (!(lx == ly && ly == lz)) return setError(ctx, RYO_ERR_UNSUPPORTED, "Ewald gravity requires cubic boxes");
+    EwaldTable* t = new EwaldTable();
+    ewaldInitTable(*t, Mroot, center, int(settings[0]), double(lx), settings[1], settings[2], settings[3],
+                   settings[4]);
+    if (t->numEwaldShells == 0 || last <= first)
+    {
+        delete t;
+        if (totalPot) *totalPot = 0;
+        return RYO_OK;
+    }
+    unsigned blocks = iceil(last - first, 256);
+    size_t need     = sizeof(EwaldTable) + 8 * size_t(blocks) + 64;
+    auto* buf       = static_cast<char*>(ctx->getScratch(kSlotEwald, need));
+    if (!buf)
+    {
+        delete t;
+        return RYO_ERR_HIP;
+    }
+    auto* dt       = reinterpret_cast<EwaldTable*>(buf);
+    auto* blockPot = reinterpret_cast<double*>(buf + ((sizeof(EwaldTable) + 63) & ~size_t(63)));
+    double* dsum   = blockPot + blocks;
+    hipStream_t s  = ctx->stream;
+    hipError_t e   = hipMemcpyAsync(dt, t, sizeof(EwaldTable), hipMemcpyHostToDevice, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s); // the host table is freed below
+    delete t;
+    if (e != hipSuccess) return setError(ctx, RYO_ERR_HIP, "ewald: %s", hipGetErrorString(e));
+    ewaldKernel<<<blocks, 256, 0, s>>>(dt, first, last, x, y, z, m, G, ugrav, ax, ay, az, blockPot);
+    sumFixedKernel<<<1, 256, 0, s>>>(blockPot, int(blocks), dsum);
+    RYO_LAUNCH_CHECK(ctx);
+    if (totalPot)
+    {
+        double U = 0;
+        RYO_HIP(ctx, hipMemcpyAsync(&U, dsum, sizeof(double), hipMemcpyDeviceToHost, s));
+        RYO_HIP(ctx, hipStreamSynchronize(s));
+        *totalPot = 0.5 * double(G) * U;
+    }
+    return RYO_OK;
+}
+
+/* ============================================================================ direct sum */
+
+constexpr int kDirTile = 256;
+
+// partial sums of one source chunk for a block of targets: out[chunk][target][4]
+__global__ __launch_bounds__(kDirTile) void directKernel(const float* __restrict__ x, const float* __restrict__ y,
+                                                        const float* __restrict__ z, const float* __restrict__ h,
+                                                        const float* __restrict__ m, size_t n,
+                                                        const int64_t* __restrict__ targets, size_t nt,
+                                                        double3 boxL, int numShells, size_t chunkLen,
+                                                        double* __restrict__ part)
+{
+    __shared__ float sx[kDirTile], sy[kDirTile], sz[kDirTile], sm[kDirTile], shh[kDirTile];
+    const size_t ti = size_t(blockIdx.x) * kDirTile + threadIdx.x;
+    const int chunk = blockIdx.y;
+    const size_t s0 = size_t(chunk) * chunkLen;
+    const size_t s1 = std::min(n, s0 + chunkLen);
+    size_t t = 0;
+    double px = 0, py = 0, pz = 0, th = 0;
+    if (ti < nt)
+    {
+        t  = targets ? size_t(targets[ti]) : ti;
+        px = x[t], py = y[t], pz = z[t], th = h[t];
+    }
+    double a0 = 0, a1 = 0, a2 = 0, a3 = 0;
+    for (size_t b = s0; b < s1; b += kDirTile)
+    {
+        size_t j = b + threadIdx.x;
+        __syncthreads();
+        if (j < s1) sx[threadIdx.x] = x[j], sy[threadIdx.x] = y[j], sz[threadIdx.x] = z[j], sm[threadIdx.x] = m[j],
+                    shh[threadIdx.x] = h[j];
+        else sx[threadIdx.x] = 0, sy[threadIdx.x] = 0, sz[threadIdx.x] = 0, sm[threadIdx.x] = 0, shh[threadIdx.x] = 1;
+        __syncthreads();
+        const int cnt = int(std::min<size_t>(kDirTile, s1 - b));
+        for (int iz = -numShells; iz <= numShells; ++iz)
+            for (int iy = -numShells; iy <= numShells; ++iy)
+                for (int ix = -numShells; ix <= numShells; ++ix)
+                {
+                    double qx = px + ix * boxL.x, qy = py + iy * boxL.y, qz = pz + iz * boxL.z;
+                    for (int k = 0; k < cnt; ++k)
+                    {
+                        double dx = double(sx[k]) - qx, dy = double(sy[k]) - qy, dz = double(sz[k]) - qz;
+                        double R2    = dx * dx + dy * dy + dz * dz;
+                        double hij   = th + double(shh[k]);
+                        double hij2  = hij * hij;
+                        double R2eff = R2 < hij2 ? hij2 : R2;
+                        double invR  = 1.0 / sqrt(R2eff);
+                        double invR3m = double(sm[k]) * invR * (invR * invR);
+                        a0 -= invR3m * R2;
+                        a1 += dx * invR3m;
+                        a2 += dy * invR3m;
+                        a3 += dz * invR3m;
+                    }
+                }
+    }
+    if (ti < nt)
+    {
+        double* o = part + (size_t(chunk) * nt + ti) * 4;
+        o[0] = a0, o[1] = a1, o[2] = a2, o[3] = a3;
+    }
+}
+
+__global__ void directReduceKernel(const double* __restrict__ part, int chunks, size_t nt,
+                                   const int64_t* __restrict__ targets, const float* __restrict__ m, double G,
+                                   double* __restrict__ out)
+{
+    size_t ti = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (ti >= nt) return;
+    double a[4] = {0, 0, 0, 0};
+    for (int c = 0; c < chunks; ++c)
+        for (int k = 0; k < 4; ++k)
+            a[k] += part[(size_t(c) * nt + ti) * 4 + k];
+    size_t t = targets ? size_t(targets[ti]) : ti;
+    out[4 * ti + 0] = G * double(m[t]) * a[0];
+    for (int k = 1; k < 4; ++k)
+        out[4 * ti + k] = G * a[k];
+}
+
+int directSum(Context* ctx, const float* x, const float* y, const float* z, const float* h, const float* m, size_t n,
+              const int64_t* targets, size_t nt, double G, const double* boxL, int numShells, double* out)
+{
+    if (nt == 0) return RYO_OK;
+    unsigned tb = iceil(nt, kDirTile);
+    int chunks  = int(std::max<size_t>(1, std::min<size_t>((size_t(ctx->numCUs) * 8 + tb - 1) / tb,
+                                                            (n + 4095) / 4096)));
+    size_t chunkLen = (n + chunks - 1) / chunks;
+    auto* part      = static_cast<double*>(ctx->getScratch(kSlotMisc, size_t(chunks) * nt * 4 * sizeof(double)));
+    if (!part) return RYO_ERR_HIP;
+    double3 bl = boxL ? make_double3(boxL[0], boxL[1], boxL[2]) : make_double3(0, 0, 0);
+    directKernel<<<dim3(tb, chunks), kDirTile, 0, ctx->stream>>>(x, y, z, h, m, n, targets, nt, bl, numShells,
+                                                                  chunkLen, part);
+    directReduceKernel<<<iceil(nt, 256), 256, 0, ctx->stream>>>(part, chunks, nt, targets, m, G, out);
+    RYO_LAUNCH_CHECK(ctx);
+    return RYO_OK;
+}
+
+} // namespace ryo

@@ -1,0 +1,356 @@
+// SFC keys, the stable LSD radix sort of (key, index) pairs and the fused permutation (gfx950).
+//
+// Keys: cstone/include/cstone/sfc/sfc.hpp:153-207 (cell index with the fork's integer/fraction split) and
+// hilbert.hpp:43-94 (3D Hilbert), morton.hpp:94-110. Sort: the semantics of cub::DeviceRadixSort::SortPairs
+// as used by cstone::sortByKey (primitives_gpu.cu:310-358), i.e. std::stable_sort order on all 64 bits.
+#include "common.hpp"
+
+#include <cstdarg>
+
+namespace ryo
+{
+
+/* ============================================================================ keys */
+
+// Cell index floor((x - xmin) * m) with x*m and xmin*m rounded once each and split into integer and
+// fraction (sfc.hpp:153-159). The explicit _rn intrinsics keep the compiler from fusing a*b - c.
+__device__ __forceinline__ int cellIndex(float x, float xmin, float m)
+{
+    const float a = __fmul_rn(x, m), b = __fmul_rn(xmin, m);
+    const float fa = floorf(a), fb = floorf(b);
+    return int(__fsub_rn(fa, fb)) - int(__fsub_rn(a, fa) < __fsub_rn(b, fb));
+}
+
+// Hilbert key of integer coordinates, 21 levels (hilbert.hpp:43-94): octant digit via the Morton->Hilbert
+// map {0,1,3,2,7,6,4,5}, then the coordinate reflections and the axis rotation/swap of the next level.
+__device__ __forceinline__ uint64_t hilbertKey(unsigned px, unsigned py, unsigned pz)
+{
+    constexpr uint32_t toHilbert = (0u << 0) | (1u << 3) | (3u << 6) | (2u << 9) | (7u << 12) | (6u << 15) |
+                                   (4u << 18) | (5u << 21); // 3-bit table packed in a word
+    uint64_t key = 0;
+#pragma unroll
+    for (int level = int(kMaxLevel) - 1; level >= 0; --level)
+    {
+        unsigned xi = (px >> level) & 1u;
+        unsigned yi = (py >> level) & 1u;
+        unsigned zi = (pz >> level) & 1u;
+        unsigned oct = (xi << 2) | (yi << 1) | zi;
+        key = (key << 3) | ((toHilbert >> (3 * oct)) & 7u);
+        unsigned nyi = yi ^ 1u, nzi = zi ^ 1u;
+        px ^= 0u - (xi & (nyi | zi));
+        py ^= 0u - ((xi & (yi | zi)) | (yi & nzi));
+        pz ^= 0u - ((xi & nyi & nzi) | (yi & nzi));
+        unsigned tx = px, ty = py, tz = pz;
+        // zi: cyclic (x,y,z) <- (y,z,x); else !yi: swap x and z
+        px = zi ? ty : (nyi ? tz : tx);
+        py = zi ? tz : ty;
+        pz = zi ? tx : (nyi ? tx : tz);
+    }
+    return key;
+}
+
+__device__ __forceinline__ uint64_t spread3(unsigned v)
+{
+    uint64_t x = v & 0x1fffffu;
+    x = (x | x << 32) & 0x001f00000000ffffull;
+    x = (x | x << 16) & 0x001f0000ff0000ffull;
+    x = (x | x << 8) & 0x100f00f00f00f00full;
+    x = (x | x << 4) & 0x10c30c30c30c30c3ull;
+    x = (x | x << 2) & 0x1249249249249249ull;
+    return x;
+}
+
+struct KeyBox
+{
+    float lo[3];
+    float m[3];
+};
+
+__global__ __launch_bounds__(256) void sfcKeysKernel(const float* __restrict__ x, const float* __restrict__ y,
+                                                     const float* __restrict__ z, uint64_t* __restrict__ keys,
+                                                     size_t n, KeyBox box, int kind, int skipRemoved)
+{
+    const int mc = (1 << kMaxLevel) - 1;
+    for (size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += size_t(gridDim.x) * blockDim.x)
+    {
+        if (skipRemoved && keys[i] == kKeyEnd) continue;
+        int ix = min(cellIndex(x[i], box.lo[0], box.m[0]), mc);
+        int iy = min(cellIndex(y[i], box.lo[1], box.m[1]), mc);
+        int iz = min(cellIndex(z[i], box.lo[2], box.m[2]), mc);
+        keys[i] = kind == 0 ? hilbertKey(ix, iy, iz) : (spread3(ix) << 2) | (spread3(iy) << 1) | spread3(iz);
+    }
+}
+
+/* ============================================================================ radix sort */
+
+constexpr int kSortThreads = 256;
+constexpr int kSortItems   = 16;
+constexpr int kSortTile    = kSortThreads * kSortItems; // 4096 keys per tile
+constexpr int kDigits      = 256;
+constexpr int kPasses      = 8;
+
+// all 8 digit histograms in one read of the keys
+__global__ __launch_bounds__(256) void histAllKernel(const uint64_t* __restrict__ keys, size_t n,
+                                                    uint32_t* __restrict__ hist /* [8][256] */)
+{
+    __shared__ uint32_t h[kPasses * kDigits];
+    for (int i = threadIdx.x; i < kPasses * kDigits; i += blockDim.x)
+        h[i] = 0;
+    __syncthreads();
+    for (size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += size_t(gridDim.x) * blockDim.x)
+    {
+        uint64_t k = keys[i];
+#pragma unroll
+        for (int p = 0; p < kPasses; ++p)
+            atomicAdd(&h[p * kDigits + ((k >> (8 * p)) & 255u)], 1u);
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < kPasses * kDigits; i += blockDim.x)
+        if (h[i]) atomicAdd(&hist[i], h[i]);
+}
+
+// digit bases per pass: exclusive scan over digits of each pass's histogram
+__global__ __launch_bounds__(256) void digitBaseKernel(const uint32_t* __restrict__ hist, uint32_t* __restrict__ base)
+{
+    __shared__ uint32_t sw[4];
+    int p = blockIdx.x, d = threadIdx.x;
+    uint32_t tot;
+    base[p * kDigits + d] = blockExclusiveScan256(hist[p * kDigits + d], sw, &tot);
+}
+
+// per-tile digit counts of one pass: tileHist[d * numTiles + tile]
+__global__ __launch_bounds__(kSortThreads) void tileHistKernel(const uint64_t* __restrict__ keys, size_t n, int shift,
+                                                              uint32_t* __restrict__ tileHist, int numTiles)
+{
+    __shared__ uint32_t h[kDigits];
+    h[threadIdx.x] = 0;
+    __syncthreads();
+    size_t base = size_t(blockIdx.x) * kSortTile;
+#pragma unroll 4
+    for (int r = 0; r < kSortItems; ++r)
+    {
+        size_t i = base + r * kSortThreads + threadIdx.x;
+        if (i < n) atomicAdd(&h[(keys[i] >> shift) & 255u], 1u);
+    }
+    __syncthreads();
+    tileHist[size_t(threadIdx.x) * numTiles + blockIdx.x] = h[threadIdx.x];
+}
+
+// per digit (one block each): exclusive scan over tiles plus the digit base -> global tile offsets
+__global__ __launch_bounds__(256) void tileScanKernel(uint32_t* __restrict__ tileHist, int numTiles,
+                                                     const uint32_t* __restrict__ digitBase)
+{
+    __shared__ uint32_t sw[4];
+    const int d = blockIdx.x;
+    uint32_t* row = tileHist + size_t(d) * numTiles;
+    uint32_t carry = digitBase[d];
+    for (int t0 = 0; t0 < numTiles; t0 += 256)
+    {
+        int t = t0 + threadIdx.x;
+        uint32_t v = t < numTiles ? row[t] : 0;
+        uint32_t tot;
+        uint32_t ex = blockExclusiveScan256(v, sw, &tot);
+        if (t < numTiles) row[t] = carry + ex;
+        carry += tot;
+    }
+}
+
+// stable scatter of one tile: wave-level match ranking (8 ballots per key), block-level offsets, staging in
+// LDS so that the writes of one digit run are contiguous
+__global__ __launch_bounds__(kSortThreads) void scatterKernel(const uint64_t* __restrict__ keysIn,
+                                                             const int32_t* __restrict__ valsIn,
+                                                             uint64_t* __restrict__ keysOut,
+                                                             int32_t* __restrict__ valsOut, size_t n, int shift,
+                                                             const uint32_t* __restrict__ tileOffsets, int numTiles)
+{
+    __shared__ uint64_t sk[kSortTile];
+    __shared__ int32_t sv[kSortTile];
+    __shared__ uint32_t wcnt[4][kDigits];
+    __shared__ uint32_t tstart[kDigits];
+    __shared__ uint32_t gbase[kDigits];
+    __shared__ uint32_t sw[4];
+
+    const int tile = blockIdx.x;
+    const size_t base = size_t(tile) * kSortTile;
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    for (int i = threadIdx.x; i < 4 * kDigits; i += kSortThreads)
+        (&wcnt[0][0])[i] = 0;
+    __syncthreads();
+
+    uint64_t k[kSortItems];
+    int32_t v[kSortItems];
+    uint32_t rank[kSortItems];
+    const uint64_t ltMask = (1ull << lane) - 1ull;
+#pragma unroll
+    for (int r = 0; r < kSortItems; ++r)
+    {
+        size_t idx = base + size_t(w) * (kSortItems * 64) + r * 64 + lane;
+        bool valid = idx < n;
+        k[r] = valid ? keysIn[idx] : 0;
+        v[r] = valid ? valsIn[idx] : 0;
+        unsigned d = unsigned(k[r] >> shift) & 255u;
+        uint64_t match = __ballot(valid);
+#pragma unroll
+        for (int b = 0; b < 8; ++b)
+        {
+            bool bit = (d >> b) & 1u;
+            uint64_t bb = __ballot(bit);
+            match &= bit ? bb : ~bb;
+        }
+        uint32_t pre = valid ? wcnt[w][d] : 0;
+        rank[r] = pre + __popcll(match & ltMask);
+        bool leader = valid && (match >> lane) == 1ull;
+        if (leader) wcnt[w][d] = pre + __popcll(match);
+    }
+    __syncthreads();
+    {
+        int d = threadIdx.x;
+        uint32_t c0 = wcnt[0][d], c1 = wcnt[1][d], c2 = wcnt[2][d], c3 = wcnt[3][d];
+        uint32_t tot;
+        uint32_t start = blockExclusiveScan256(c0 + c1 + c2 + c3, sw, &tot);
+        wcnt[0][d] = 0;
+        wcnt[1][d] = c0;
+        wcnt[2][d] = c0 + c1;
+        wcnt[3][d] = c0 + c1 + c2;
+        tstart[d] = start;
+        gbase[d] = tileOffsets[size_t(d) * numTiles + tile];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < kSortItems; ++r)
+    {
+        size_t idx = base + size_t(w) * (kSortItems * 64) + r * 64 + lane;
+        if (idx < n)
+        {
+            unsigned d = unsigned(k[r] >> shift) & 255u;
+            uint32_t pos = tstart[d] + wcnt[w][d] + rank[r];
+            sk[pos] = k[r];
+            sv[pos] = v[r];
+        }
+    }
+    __syncthreads();
+    size_t nValid = n - base < size_t(kSortTile) ? n - base : size_t(kSortTile);
+    for (int j = threadIdx.x; j < int(nValid); j += kSortThreads)
+    {
+        uint64_t key = sk[j];
+        unsigned d = unsigned(key >> shift) & 255u;
+        size_t out = size_t(gbase[d]) + (j - tstart[d]);
+        keysOut[out] = key;
+        valsOut[out] = sv[j];
+    }
+}
+
+__global__ void iotaKernel(int32_t* out, size_t n, int32_t start)
+{
+    for (size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += size_t(gridDim.x) * blockDim.x)
+        out[i] = start + int32_t(i);
+}
+
+struct GatherArgs
+{
+    const float* in[8];
+    float* out[8];
+    int num;
+};
+
+__global__ __launch_bounds__(256) void gatherKernel(const int32_t* __restrict__ perm, size_t n, GatherArgs a)
+{
+    for (size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += size_t(gridDim.x) * blockDim.x)
+    {
+        int32_t j = perm[i];
+#pragma unroll
+        for (int k = 0; k < 8; ++k)
+            if (k < a.num) a.out[k][i] = a.in[k][j];
+    }
+}
+
+static unsigned gridFor(size_t n, int threads, int maxBlocks)
+{
+    size_t b = (n + threads - 1) / threads;
+    if (b > size_t(maxBlocks)) b = maxBlocks;
+    return unsigned(b ? b : 1);
+}
+
+size_t sortTempBytes(size_t n)
+{
+    size_t numTiles = (n + kSortTile - 1) / kSortTile;
+    return 256 * sizeof(uint32_t) * (2 * kPasses) + size_t(kDigits) * numTiles * sizeof(uint32_t) + 256;
+}
+
+int sortPairs(Context* ctx, uint64_t* keys, int32_t* vals, size_t n, uint64_t* keyBuf, int32_t* valBuf, void* tmp,
+              size_t tmpBytes)
+{
+    if (n == 0) return RYO_OK;
+    if (tmpBytes < sortTempBytes(n))
+        return setError(ctx, RYO_ERR_ARG, "sort: temp storage too small (%zu < %zu)", tmpBytes, sortTempBytes(n));
+    const int numTiles = int((n + kSortTile - 1) / kSortTile);
+    uint32_t* hist     = reinterpret_cast<uint32_t*>(tmp);
+    uint32_t* dbase    = hist + kPasses * kDigits;
+    uint32_t* tileHist = dbase + kPasses * kDigits;
+    hipStream_t s = ctx->stream;
+    RYO_HIP(ctx, hipMemsetAsync(hist, 0, kPasses * kDigits * sizeof(uint32_t), s));
+    histAllKernel<<<gridFor(n, 256, ctx->numCUs * 4), 256, 0, s>>>(keys, n, hist);
+    digitBaseKernel<<<kPasses, 256, 0, s>>>(hist, dbase);
+    uint64_t* kin = keys;
+    uint64_t* kout = keyBuf;
+    int32_t* vin = vals;
+    int32_t* vout = valBuf;
+    for (int p = 0; p < kPasses; ++p)
+    {
+        int shift = 8 * p;
+        tileHistKernel<<<numTiles, kSortThreads, 0, s>>>(kin, n, shift, tileHist, numTiles);
+        tileScanKernel<<<kDigits, 256, 0, s>>>(tileHist, numTiles, dbase + p * kDigits);
+        scatterKernel<<<numTiles, kSortThreads, 0, s>>>(kin, vin, kout, vout, n, shift, tileHist, numTiles);
+        std::swap(kin, kout);
+        std::swap(vin, vout);
+    }
+    RYO_LAUNCH_CHECK(ctx);
+    return RYO_OK;
+}
+
+int sfcKeys(Context* ctx, const float* x, const float* y, const float* z, uint64_t* keys, size_t n,
+            const float* limits, int kind, int skipRemoved)
+{
+    if (n == 0) return RYO_OK;
+    KeyBox b;
+    float len[3];
+    for (int d = 0; d < 3; ++d)
+    {
+        float lo = limits[2 * d], hi = limits[2 * d + 1];
+        len[d]   = hi - lo;
+        float il = 1.0f / (hi - lo); // Box<float>::inverseLengths_ (box.hpp:107), in float on the host
+        b.lo[d]  = lo;
+        b.m[d]   = float(1u << kMaxLevel) * il;
+    }
+    // box.hpp:193-198: any axis shorter than 0.75 x the longest one changes the key layout (mixed-D keys)
+    float mx = std::max(std::max(len[0], len[1]), len[2]);
+    for (int d = 0; d < 3; ++d)
+        if (unsigned(std::floor(std::log2(mx / len[d]) + 0.5849625007211563f)) != 0)
+            return setError(ctx, RYO_ERR_UNSUPPORTED, "sfc keys: non-cubic (mixed-dimension) boxes are not supported");
+    sfcKeysKernel<<<gridFor(n, 256, ctx->numCUs * 16), 256, 0, ctx->stream>>>(x, y, z, keys, n, b, kind, skipRemoved);
+    RYO_LAUNCH_CHECK(ctx);
+    return RYO_OK;
+}
+
+int sequence(Context* ctx, int32_t* out, size_t n, int32_t start)
+{
+    if (n == 0) return RYO_OK;
+    iotaKernel<<<gridFor(n, 256, ctx->numCUs * 16), 256, 0, ctx->stream>>>(out, n, start);
+    RYO_LAUNCH_CHECK(ctx);
+    return RYO_OK;
+}
+
+int gather(Context* ctx, const int32_t* perm, size_t n, int num, const float* const* in, float* const* out)
+{
+    if (num < 0 || num > 8) return setError(ctx, RYO_ERR_ARG, "gather: at most 8 arrays");
+    if (n == 0 || num == 0) return RYO_OK;
+    GatherArgs a{};
+    a.num = num;
+    for (int k = 0; k < num; ++k)
+        a.in[k] = in[k], a.out[k] = out[k];
+    gatherKernel<<<gridFor(n, 256, ctx->numCUs * 16), 256, 0, ctx->stream>>>(perm, n, a);
+    RYO_LAUNCH_CHECK(ctx);
+    return RYO_OK;
+}
+
+} // namespace ryo

@@ -1,0 +1,434 @@
+// Barnes-Hut traversal for wave64 (gfx950): one wavefront per target group, one target per lane.
+//
+// Semantics follow ryoanji's traversal (nbody/traversal_gpu.cu:170-570; CPU traversal_cpu.hpp:67-230):
+//   * the target group is described by the bounding box of its (valid) targets;
+//   * a source cell is "close" when the squared distance from its expansion center to the box is below its
+//     squared MAC radius (applyMAC, traversal_gpu.cu:77-84 / macs.hpp:93-101);
+//   * far cells apply their quadrupole (M2P, cartesian_qpole.hpp:152-176) to every target, close leaves
+//     apply their particles (P2P with the softening clamp, kernel.hpp:99-120), close internal cells are
+//     opened; periodic images repeat the walk with shifted targets.
+// The walk itself is designed for CDNA: the 64 lanes evaluate 8 sibling groups (64 cells) per step, opened
+// cells are pushed to a per-wave LIFO in LDS (the level-ordered LIFO holds at most 64 entries per tree
+// level, so 21 x 64 entries can never overflow), and the interactions are applied right away with the
+// source index uniform across the wave, so the source data of a cell or body is fetched once per wave
+// with scalar loads and broadcast to all 64 targets.
+#include "common.hpp"
+
+namespace ryo
+{
+
+constexpr int kTravWaves = 4;              // waves per block
+constexpr int kStackCap  = 22 * 64;        // LIFO entries per wave (>= 21 levels x 64)
+
+struct TravArgs
+{
+    int numGroups;
+    const uint32_t* gStart;
+    const uint32_t* gEnd;
+    int initNode;
+    const float *xt, *yt, *zt, *mt, *ht;
+    const float *xs, *ys, *zs, *ms, *hs;
+    const int32_t* child;
+    const int32_t* i2l;
+    const uint32_t* layout;
+    const float4* centers;
+    const float4* mp; // two float4 per node: {M, qxx, qxy, qxz}, {qyy, qyz, qzz, trace}
+    float G;
+    int numShells;
+    float boxL[3];
+    float *p, *ax, *ay, *az;
+    double* groupPot;
+    unsigned* work;
+    unsigned long long* stats;
+    uint32_t* failed;
+    int stackCap;
+};
+
+struct Acc
+{
+    float p, x, y, z;
+};
+
+__device__ __forceinline__ void m2p(Acc& a, float tx, float ty, float tz, float4 c, float4 q0, float4 q1)
+{
+    float rx = tx - c.x, ry = ty - c.y, rz = tz - c.z;
+    float r2  = rx * rx + ry * ry + rz * rz;
+    float r1  = rsqrtf(r2);
+    float ri2 = r1 * r1;
+    float r5  = ri2 * ri2 * r1;
+    float M = q0.x, qxx = q0.y, qxy = q0.z, qxz = q0.w, qyy = q1.x, qyz = q1.y, qzz = q1.z;
+    float Qrx  = rx * qxx + ry * qxy + rz * qxz;
+    float Qry  = rx * qxy + ry * qyy + rz * qyz;
+    float Qrz  = rx * qxz + ry * qyz + rz * qzz;
+    float rQr  = rx * Qrx + ry * Qry + rz * Qrz;
+    float mono = (-2.5f * rQr * r5 - M * r1) * ri2;
+    a.p -= M * r1 + 0.5f * r5 * rQr;
+    a.x += r5 * Qrx + mono * rx;
+    a.y += r5 * Qry + mono * ry;
+    a.z += r5 * Qrz + mono * rz;
+}
+
+__device__ __forceinline__ void p2p(Acc& a, float tx, float ty, float tz, float th, float sx, float sy, float sz,
+                                    float sm, float sh)
+{
+    float dx = sx - tx, dy = sy - ty, dz = sz - tz;
+    float r2    = dx * dx + dy * dy + dz * dz;
+    float hij   = th + sh;
+    float hij2  = hij * hij;
+    float r2eff = r2 < hij2 ? hij2 : r2;
+    float invR  = rsqrtf(r2eff);
+    float invR3m = sm * invR * (invR * invR);
+    a.p -= invR3m * r2;
+    a.x += dx * invR3m;
+    a.y += dy * invR3m;
+    a.z += dz * invR3m;
+}
+
+// MAC as the reference evaluates it: positive part of |tc - com| - ts per axis, squared norm (right fold),
+// compared with mac^2. Unfused operations so that equal inputs give the CPU's decision.
+__device__ __forceinline__ bool macClose(float4 c, float tcx, float tcy, float tcz, float tsx, float tsy, float tsz)
+{
+    float dx = __fsub_rn(fabsf(__fsub_rn(tcx, c.x)), tsx);
+    float dy = __fsub_rn(fabsf(__fsub_rn(tcy, c.y)), tsy);
+    float dz = __fsub_rn(fabsf(__fsub_rn(tcz, c.z)), tsz);
+    dx = __fmul_rn(__fadd_rn(dx, fabsf(dx)), 0.5f);
+    dy = __fmul_rn(__fadd_rn(dy, fabsf(dy)), 0.5f);
+    dz = __fmul_rn(__fadd_rn(dz, fabsf(dz)), 0.5f);
+    float R2 = __fadd_rn(__fmul_rn(dx, dx), __fadd_rn(__fmul_rn(dy, dy), __fmul_rn(dz, dz)));
+    return R2 < fabsf(c.w);
+}
+
+struct WalkOut
+{
+    unsigned m2p, p2p, maxStack;
+    bool overflow;
+};
+
+// apply all bodies of leaf ranges flagged in `dmask` (lane -> [rb, re)) to the wave's targets
+__device__ __forceinline__ unsigned applyLeaves(const TravArgs& a, uint64_t dmask, uint32_t rb, uint32_t re, Acc& acc,
+                                                float tx, float ty, float tz, float th)
+{
+    unsigned cnt = 0;
+    while (dmask)
+    {
+        int l = __ffsll((unsigned long long)dmask) - 1;
+        dmask &= dmask - 1;
+        uint32_t b = readlane(rb, l), e = readlane(re, l);
+        cnt += e - b;
+        for (uint32_t j = b; j < e; ++j)
+            p2p(acc, tx, ty, tz, th, a.xs[j], a.ys[j], a.zs[j], a.ms[j], a.hs[j]);
+    }
+    return cnt;
+}
+
+__device__ __forceinline__ unsigned applyCells(const TravArgs& a, uint64_t amask, int cell, Acc& acc, float tx,
+                                               float ty, float tz)
+{
+    unsigned cnt = __popcll(amask);
+    while (amask)
+    {
+        int l = __ffsll((unsigned long long)amask) - 1;
+        amask &= amask - 1;
+        int c = readlane(cell, l);
+        m2p(acc, tx, ty, tz, a.centers[c], a.mp[2 * c], a.mp[2 * c + 1]);
+    }
+    return cnt;
+}
+
+__device__ WalkOut walk(const TravArgs& a, int* stack, float tx, float ty, float tz, float th, float tcx, float tcy,
+                        float tcz, float tsx, float tsy, float tsz, Acc& acc)
+{
+    const int lane      = threadIdx.x & 63;
+    const uint64_t lt   = (1ull << lane) - 1ull;
+    WalkOut o{0, 0, 0, false};
+    int sp = 0;
+
+    const int rootChild = a.child[0];
+    if (a.initNode == 0 || rootChild == 0)
+    {
+        float4 c   = a.centers[0];
+        bool close = macClose(c, tcx, tcy, tcz, tsx, tsy, tsz);
+        if (!close)
+        {
+            if (c.w != 0.0f) o.m2p += applyCells(a, 1ull, 0, acc, tx, ty, tz);
+            return o;
+        }
+        if (rootChild == 0)
+        {
+            o.p2p += applyLeaves(a, 1ull, a.layout[0], a.layout[1], acc, tx, ty, tz, th);
+            return o;
+        }
+    }
+    if (lane == 0) stack[0] = rootChild;
+    sp = 1;
+
+    while (sp > 0)
+    {
+        const int k   = sp < 8 ? sp : 8;
+        const int e   = lane >> 3;
+        const bool ok = e < k;
+        __builtin_amdgcn_wave_barrier();
+        int cell = ok ? stack[sp - k + e] + (lane & 7) : 0;
+        sp -= k;
+
+        float4 c    = a.centers[cell];
+        int ch      = a.child[cell];
+        bool close  = ok && macClose(c, tcx, tcy, tcz, tsx, tsy, tsz);
+        bool isNode = ch != 0;
+        uint32_t rb = 0, re = 0;
+        if (close && !isNode)
+        {
+            int li = a.i2l[cell];
+            rb = a.layout[li], re = a.layout[li + 1];
+        }
+        // a leaf without local particles but with sources elsewhere (remote) is always approximated
+        bool remote = close && !isNode && rb == re && c.w != 0.0f;
+        bool split  = close && isNode;
+        bool approx = ok && ((!close && c.w != 0.0f) || remote);
+        bool direct = close && !isNode && rb != re;
+
+        uint64_t sm = __ballot(split);
+        int cnt     = __popcll(sm);
+        if (sp + cnt > a.stackCap)
+        {
+            o.overflow = true;
+            o.maxStack = max(o.maxStack, unsigned(sp + cnt));
+            return o;
+        }
+        if (split) stack[sp + __popcll(sm & lt)] = ch;
+        sp += cnt;
+        o.maxStack = max(o.maxStack, unsigned(sp));
+
+        o.m2p += applyCells(a, __ballot(approx), cell, acc, tx, ty, tz);
+        o.p2p += applyLeaves(a, __ballot(direct), rb, re, acc, tx, ty, tz, th);
+    }
+    return o;
+}
+
+__global__ __launch_bounds__(64 * kTravWaves) void traverseKernel(TravArgs a)
+{
+    __shared__ int stackMem[kTravWaves][kStackCap];
+    const int w    = threadIdx.x >> 6;
+    const int lane = threadIdx.x & 63;
+    int* stack     = stackMem[w];
+
+    for (;;)
+    {
+        unsigned g = 0;
+        if (lane == 0) g = atomicAdd(a.work, 1u);
+        g = __shfl(g, 0);
+        if (g >= unsigned(a.numGroups)) return;
+
+        const uint32_t b = a.gStart[g], e = a.gEnd[g];
+        const bool valid = b + lane < e;
+        const uint32_t ti = valid ? b + lane : e - 1;
+        float tx = a.xt[ti], ty = a.yt[ti], tz = a.zt[ti], th = a.ht[ti];
+
+        float mnx = waveMin(tx), mny = waveMin(ty), mnz = waveMin(tz);
+        float mxx = waveMax(tx), mxy = waveMax(ty), mxz = waveMax(tz);
+        float tcx = (mxx + mnx) * 0.5f, tcy = (mxy + mny) * 0.5f, tcz = (mxz + mnz) * 0.5f;
+        float tsx = (mxx - mnx) * 0.5f, tsy = (mxy - mny) * 0.5f, tsz = (mxz - mnz) * 0.5f;
+
+        Acc acc{0.f, 0.f, 0.f, 0.f};
+        unsigned nM2P = 0, nP2P = 0, maxStack = 0;
+        bool overflow = false;
+        const int S = a.numShells;
+        for (int iz = -S; iz <= S && !overflow; ++iz)
+            for (int iy = -S; iy <= S && !overflow; ++iy)
+                for (int ix = -S; ix <= S && !overflow; ++ix)
+                {
+                    float sx = ix * a.boxL[0], sy = iy * a.boxL[1], sz = iz * a.boxL[2];
+                    WalkOut o = walk(a, stack, tx + sx, ty + sy, tz + sz, th, tcx + sx, tcy + sy, tcz + sz, tsx, tsy,
+                                     tsz, acc);
+                    nM2P += o.m2p;
+                    nP2P += o.p2p;
+                    maxStack = max(maxStack, o.maxStack);
+                    overflow = o.overflow;
+                }
+
+        if (overflow)
+        {
+            if (lane == 0)
+            {
+                unsigned slot = atomicAdd(reinterpret_cast<unsigned*>(&a.stats[5]), 1u);
+                a.failed[slot] = g;
+                atomicMax(&a.stats[4], (unsigned long long)maxStack);
+            }
+            continue;
+        }
+
+        double myPot = 0;
+        if (valid)
+        {
+            float m = a.mt[ti];
+            if (a.p) a.p[ti] += a.G * m * acc.p;
+            a.ax[ti] += a.G * acc.x;
+            a.ay[ti] += a.G * acc.y;
+            a.az[ti] += a.G * acc.z;
+            myPot = double(m) * double(acc.p);
+        }
+        myPot = waveSum(myPot);
+        if (lane == 0)
+        {
+            a.groupPot[g] = myPot;
+            unsigned long long nt = e - b;
+            atomicAdd(&a.stats[0], nt * nP2P);
+            atomicMax(&a.stats[1], (unsigned long long)nP2P);
+            atomicAdd(&a.stats[2], nt * nM2P);
+            atomicMax(&a.stats[3], (unsigned long long)nM2P);
+            atomicMax(&a.stats[4], (unsigned long long)maxStack);
+        }
+    }
+}
+
+// fixed-order FP64 sum (Neumaier) of v[0..n) into *out; one block
+__global__ __launch_bounds__(256) void sumKernel(const double* __restrict__ v, int n, double* __restrict__ out)
+{
+    __shared__ double ss[256], sc[256];
+    double s = 0, c = 0;
+    for (int i = threadIdx.x; i < n; i += 256)
+    {
+        double x = v[i], t = s + x;
+        c += fabs(s) >= fabs(x) ? (s - t) + x : (x - t) + s;
+        s = t;
+    }
+    ss[threadIdx.x] = s;
+    sc[threadIdx.x] = c;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1)
+    {
+        if (int(threadIdx.x) < o)
+        {
+            double x = ss[threadIdx.x + o], y = ss[threadIdx.x];
+            double t = x + y;
+            sc[threadIdx.x] += sc[threadIdx.x + o] + (fabs(y) >= fabs(x) ? (y - t) + x : (x - t) + y);
+            ss[threadIdx.x] = t;
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) *out = ss[0] + sc[0];
+}
+
+__global__ void zeroKernelU(unsigned* a, int n)
+{
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+        a[i] = 0;
+}
+
+int traverse(Context* ctx, TravArgs a, double* totalPot)
+{
+    hipStream_t s = ctx->stream;
+    for (int k = 0; k < 6; ++k)
+        ctx->hStats[k] = 0;
+    ctx->retriedGroups = 0;
+    if (a.numGroups <= 0)
+    {
+        if (totalPot) *totalPot = 0;
+        return RYO_OK;
+    }
+    if (a.stackCap <= 8 || a.stackCap > kStackCap) a.stackCap = kStackCap;
+    // ctx->dStats: [0] work counter (u32), [8..13] stats u64 {sumP2P, maxP2P, sumM2P, maxM2P, maxStack, failed},
+    // [16] FP64 potential sum
+    unsigned* work            = reinterpret_cast<unsigned*>(ctx->dStats);
+    unsigned long long* stats = ctx->dStats + 8;
+    double* potSum            = reinterpret_cast<double*>(ctx->dStats + 16);
+    a.work  = work;
+    a.stats = stats;
+    zeroKernelU<<<1, 64, 0, s>>>(work, 64);
+
+    const int maxBlocks = ctx->numCUs * 8;
+    double pot          = 0;
+    unsigned long long h[6] = {0, 0, 0, 0, 0, 0};
+    std::vector<uint32_t> ns, ne;
+    for (int pass = 0;; ++pass)
+    {
+        size_t G  = size_t(a.numGroups);
+        auto* buf = static_cast<char*>(ctx->getScratch(kSlotTraverse, 12 * G + 256));
+        if (!buf) return RYO_ERR_HIP;
+        a.groupPot = reinterpret_cast<double*>(buf);
+        a.failed   = reinterpret_cast<uint32_t*>(buf + 8 * G);
+        if (pass > 0)
+        {
+            zeroKernelU<<<1, 64, 0, s>>>(work, 1);
+            zeroKernelU<<<1, 64, 0, s>>>(reinterpret_cast<unsigned*>(stats + 5), 2);
+        }
+        zeroKernelU<<<std::min(iceil(2 * G, 256), 4096u), 256, 0, s>>>(reinterpret_cast<unsigned*>(a.groupPot),
+                                                                       int(2 * G));
+        int blocks = std::min(maxBlocks, int(iceil(G, kTravWaves)));
+        if (ctx->timing && pass == 0) RYO_HIP(ctx, hipEventRecord(ctx->ev[0], s));
+        traverseKernel<<<blocks, 64 * kTravWaves, 0, s>>>(a);
+        if (ctx->timing && pass == 0) RYO_HIP(ctx, hipEventRecord(ctx->ev[1], s));
+        RYO_LAUNCH_CHECK(ctx);
+        sumKernel<<<1, 256, 0, s>>>(a.groupPot, a.numGroups, potSum);
+        double hp = 0;
+        RYO_HIP(ctx, hipMemcpyAsync(h, stats, sizeof(h), hipMemcpyDeviceToHost, s));
+        RYO_HIP(ctx, hipMemcpyAsync(&hp, potSum, sizeof(double), hipMemcpyDeviceToHost, s));
+        RYO_HIP(ctx, hipStreamSynchronize(s));
+        if (ctx->timing && pass == 0)
+        {
+            float ms = 0;
+            RYO_HIP(ctx, hipEventElapsedTime(&ms, ctx->ev[0], ctx->ev[1]));
+            ctx->lastTraverseMs = ms;
+        }
+        pot += hp; // failed groups left their (zeroed) slot untouched
+        unsigned numFailed = unsigned(h[5] & 0xffffffffu);
+        if (numFailed == 0) break;
+        if (pass > 40) return setError(ctx, RYO_ERR_TRAVERSAL, "traversal: overflow not resolved by splitting");
+        ctx->retriedGroups += numFailed;
+        // splitFailedGroups: halve each failed group along the SFC and traverse the halves again
+        std::vector<uint32_t> hf(numFailed), hs(G), he(G);
+        RYO_HIP(ctx, hipMemcpyAsync(hf.data(), a.failed, numFailed * 4, hipMemcpyDeviceToHost, s));
+        RYO_HIP(ctx, hipMemcpyAsync(hs.data(), a.gStart, G * 4, hipMemcpyDeviceToHost, s));
+        RYO_HIP(ctx, hipMemcpyAsync(he.data(), a.gEnd, G * 4, hipMemcpyDeviceToHost, s));
+        RYO_HIP(ctx, hipStreamSynchronize(s));
+        ns.clear(), ne.clear();
+        for (uint32_t g : hf)
+        {
+            if (he[g] - hs[g] < 2)
+                return setError(ctx, RYO_ERR_TRAVERSAL, "traversal: a single target exhausted the stack");
+            uint32_t mid = hs[g] + (he[g] - hs[g]) / 2;
+            ns.push_back(hs[g]), ns.push_back(mid);
+            ne.push_back(mid), ne.push_back(he[g]);
+        }
+        auto* sb = static_cast<uint32_t*>(ctx->getScratch(kSlotGroups, 8 * ns.size() + 64));
+        if (!sb) return RYO_ERR_HIP;
+        RYO_HIP(ctx, hipMemcpyAsync(sb, ns.data(), 4 * ns.size(), hipMemcpyHostToDevice, s));
+        RYO_HIP(ctx, hipMemcpyAsync(sb + ns.size(), ne.data(), 4 * ne.size(), hipMemcpyHostToDevice, s));
+        RYO_HIP(ctx, hipStreamSynchronize(s));
+        a.numGroups = int(ns.size());
+        a.gStart    = sb;
+        a.gEnd      = sb + ns.size();
+    }
+    for (int k = 0; k < 5; ++k)
+        ctx->hStats[k] = h[k];
+    if (totalPot) *totalPot = 0.5 * double(a.G) * pot;
+    return RYO_OK;
+}
+
+int traverseEntry(Context* ctx, int numGroups, const uint32_t* gs, const uint32_t* ge, int initNode, const float* xt,
+                  const float* yt, const float* zt, const float* mt, const float* ht, const float* xs,
+                  const float* ys, const float* zs, const float* ms, const float* hs, const int32_t* child,
+                  const int32_t* i2l, const uint32_t* layout, const float* centers, const float* mp, float G,
+                  int numShells, const float* boxL, float* p, float* ax, float* ay, float* az, double* totalPot,
+                  int stackCap)
+{
+    if (numGroups > 0 && (!gs || !ge || !xt || !yt || !zt || !mt || !ht || !xs || !ys || !zs || !ms || !hs ||
+                          !child || !i2l || !layout || !centers || !mp || !ax || !ay || !az))
+        return setError(ctx, RYO_ERR_ARG, "traverse: null pointer");
+    TravArgs a{};
+    a.numGroups = numGroups;
+    a.gStart = gs, a.gEnd = ge, a.initNode = initNode;
+    a.xt = xt, a.yt = yt, a.zt = zt, a.mt = mt, a.ht = ht;
+    a.xs = xs, a.ys = ys, a.zs = zs, a.ms = ms, a.hs = hs;
+    a.child = child, a.i2l = i2l, a.layout = layout;
+    a.centers = reinterpret_cast<const float4*>(centers);
+    a.mp      = reinterpret_cast<const float4*>(mp);
+    a.G = G, a.numShells = numShells;
+    for (int d = 0; d < 3; ++d)
+        a.boxL[d] = boxL ? boxL[d] : 0.0f;
+    a.p = p, a.ax = ax, a.ay = ay, a.az = az;
+    a.stackCap = stackCap;
+    return traverse(ctx, a, totalPot);
+}
+
+} // namespace ryo

@@ -1,0 +1,401 @@
+// Cornerstone leaf array, per-leaf layout, the linked octree and fixed target groups (gfx950).
+//
+// Leaves. The reference reaches its leaf array by iterating split/merge passes with host syncs
+// (csarray.hpp:267-440, csarray_gpu.cu:86-231). The fixed point of those passes is unique: a node is internal
+// iff it holds more than `bucket` particles and its level is below 21, and the leaves are the children of
+// internal nodes that are not internal themselves. This file builds that tree directly from the sorted keys
+// in one parallel pass:
+//   c_j = common octree level of keys j and j+bucket (the window of bucket+1 particles starting at j),
+//   M_i = max c_j over the windows containing particle i (its deepest ancestor with > bucket particles),
+//   the leaf of particle i is at level min(M_i + 1, 21) and starts at key_i truncated to that level.
+// Distinct particle leaves are the non-empty leaves; the gaps between them are covered by the coarsest
+// aligned octree nodes (the empty leaves, as cstone's spanSfcRange, common.hpp:413-461).
+//
+// Linked octree. The same algorithm as buildOctreeGpu (octree_gpu.cu:138-200): Warren-Salmon prefixes of
+// leaves and internal nodes in the binary-tree layout (octree.hpp:77-100, binaryKeyWeight :55-65), a sort by
+// prefix, the inverse permutation, level ranges and child links -- the arrays it defines are part of the
+// interface (internalToLeaf / leafToInternal expose the unsorted layout), so they are produced identically.
+#include "common.hpp"
+
+namespace ryo
+{
+
+int sortPairs(Context* ctx, uint64_t* keys, int32_t* vals, size_t n, uint64_t* keyBuf, int32_t* valBuf, void* tmp,
+              size_t tmpBytes);
+size_t sortTempBytes(size_t n);
+
+/* ============================================================================ leaves */
+
+constexpr int kLeafThreads = 256;
+constexpr int kLeafTile    = 1024; // particles per block
+constexpr int kMaxBucket   = 1024;
+
+// finest-first decomposition of [a, b) into aligned octree nodes; the level of the node starting at a
+__device__ __forceinline__ unsigned spanLevel(uint64_t a, uint64_t b)
+{
+    unsigned l = a == 0 ? 0u : kMaxLevel - unsigned(ctz64(a)) / 3u;
+    while (a + nodeRange(l) > b)
+        ++l;
+    return l;
+}
+__device__ inline int spanCount(uint64_t a, uint64_t b)
+{
+    int c = 0;
+    while (a < b)
+    {
+        a += nodeRange(spanLevel(a, b));
+        ++c;
+    }
+    return c;
+}
+
+struct LeafTileSmem
+{
+    uint64_t keys[kLeafTile + 2 * kMaxBucket + 2];
+    int c[2][kLeafTile + kMaxBucket + 2];
+    uint64_t start[kLeafTile + 1]; // leaf start key of particles t0-1 .. t1-1
+    unsigned char level[kLeafTile + 1];
+    uint32_t sw[4];
+};
+
+/* Per tile [t0, t1): leaf start/level of particles t0-1..t1-1 into s.start/s.level, and the number of
+ * leaves each particle of the tile emits (returned for the first kLeafTile/256 items of each thread). */
+__device__ void leafTile(const uint64_t* __restrict__ k, long n, int B, long t0, LeafTileSmem& s, int em[4])
+{
+    const long t1    = min(t0 + long(kLeafTile), n);
+    const long kbase = t0 - 1 - B; // s.keys[j - kbase] = k[j]
+    const int nk     = int(t1 + B - kbase);
+    for (int j = threadIdx.x; j < nk; j += kLeafThreads)
+    {
+        long g      = kbase + j;
+        s.keys[j] = (g >= 0 && g < n) ? k[g] : 0;
+    }
+    __syncthreads();
+    // c over windows j in [kbase, t1)
+    const int nc = int(t1 - kbase);
+    const long lastWindow = n - 1 - B;
+    for (int j = threadIdx.x; j < nc; j += kLeafThreads)
+    {
+        long g = kbase + j;
+        int v  = -1;
+        if (g >= 0 && g <= lastWindow)
+        {
+            int cp = commonPrefix(s.keys[j], s.keys[j + B]);
+            v      = min(cp / 3, int(kMaxLevel));
+        }
+        s.c[0][j] = v;
+    }
+    __syncthreads();
+    // doubling: c[P][j] = max(c[j .. j+P-1]) for the largest power of two P <= B+1
+    const int W = B + 1;
+    int P       = 1;
+    int cur     = 0;
+    while (2 * P <= W)
+    {
+        for (int j = threadIdx.x; j < nc; j += kLeafThreads)
+        {
+            int a = s.c[cur][j];
+            int b = (j + P < nc) ? s.c[cur][j + P] : -1;
+            s.c[cur ^ 1][j] = max(a, b);
+        }
+        __syncthreads();
+        cur ^= 1;
+        P *= 2;
+    }
+    // particles i in [t0-1, t1): window max over j in [i-B, i] = max(c[i-B .. i-B+P-1], c[i+1-P .. i])
+    const int ni = int(t1 - (t0 - 1));
+    for (int q = threadIdx.x; q < ni; q += kLeafThreads)
+    {
+        long i = t0 - 1 + q;
+        if (i < 0) continue;
+        int ja = int(i - B - kbase), jb = int(i + 1 - P - kbase);
+        int M  = max(s.c[cur][ja], s.c[cur][jb]);
+        unsigned L = unsigned(min(M + 1, int(kMaxLevel)));
+        uint64_t key = s.keys[int(i - kbase)];
+        s.start[q] = key & ~(nodeRange(L) - 1);
+        s.level[q] = (unsigned char)L;
+    }
+    __syncthreads();
+    for (int r = 0; r < kLeafTile / kLeafThreads; ++r)
+    {
+        int q  = r * kLeafThreads + threadIdx.x; // particle t0 + q, s index q + 1
+        long i = t0 + q;
+        int e  = 0;
+        if (i < t1)
+        {
+            uint64_t si = s.start[q + 1];
+            bool boundary = (i == 0) || si != s.start[q];
+            if (boundary)
+            {
+                uint64_t prevEnd = (i == 0) ? 0 : s.start[q] + nodeRange(s.level[q]);
+                e                = spanCount(prevEnd, si) + 1;
+            }
+            if (i == n - 1) e += spanCount(si + nodeRange(s.level[q + 1]), kKeyEnd);
+        }
+        em[r] = e;
+    }
+}
+
+__global__ __launch_bounds__(kLeafThreads) void leafCountKernel(const uint64_t* __restrict__ k, long n, int B,
+                                                               uint32_t* __restrict__ blockSums)
+{
+    __shared__ LeafTileSmem s;
+    int em[4];
+    long t0 = long(blockIdx.x) * kLeafTile;
+    leafTile(k, n, B, t0, s, em);
+    uint32_t v = em[0] + em[1] + em[2] + em[3], tot;
+    blockExclusiveScan256(v, s.sw, &tot);
+    if (threadIdx.x == 0) blockSums[blockIdx.x] = tot;
+}
+
+// exclusive scan of the block sums in place (one block); total into blockSums[numBlocks]
+__global__ __launch_bounds__(256) void scanBlocksKernel(uint32_t* __restrict__ sums, int numBlocks)
+{
+    __shared__ uint32_t sw[4];
+    uint32_t carry = 0;
+    for (int b0 = 0; b0 < numBlocks; b0 += 256)
+    {
+        int b      = b0 + threadIdx.x;
+        uint32_t v = b < numBlocks ? sums[b] : 0, tot;
+        uint32_t ex = blockExclusiveScan256(v, sw, &tot);
+        if (b < numBlocks) sums[b] = carry + ex;
+        carry += tot;
+    }
+    if (threadIdx.x == 0) sums[numBlocks] = carry;
+}
+
+__global__ __launch_bounds__(kLeafThreads) void leafEmitKernel(const uint64_t* __restrict__ k, long n, int B,
+                                                              const uint32_t* __restrict__ blockOffsets,
+                                                              uint64_t* __restrict__ leaves,
+                                                              uint32_t* __restrict__ layout, long capacity)
+{
+    __shared__ LeafTileSmem s;
+    int em[4];
+    long t0 = long(blockIdx.x) * kLeafTile;
+    leafTile(k, n, B, t0, s, em);
+    // thread-contiguous order: item r of thread t is particle t0 + r*256 + t -> scan in that order
+    // (the emission order must follow particle order, so scan over q = r*256 + t by rows)
+    uint32_t base = blockOffsets[blockIdx.x];
+    for (int r = 0; r < kLeafTile / kLeafThreads; ++r)
+    {
+        uint32_t tot;
+        uint32_t ex = blockExclusiveScan256(uint32_t(em[r]), s.sw, &tot);
+        uint32_t pos = base + ex;
+        base += tot;
+        if (em[r] == 0) continue;
+        int q  = r * kLeafThreads + threadIdx.x;
+        long i = t0 + q;
+        uint64_t si = s.start[q + 1];
+        bool boundary = (i == 0) || si != s.start[q];
+        if (boundary)
+        {
+            uint64_t a = (i == 0) ? 0 : s.start[q] + nodeRange(s.level[q]);
+            while (a < si)
+            {
+                if (pos < capacity) leaves[pos] = a, layout[pos] = uint32_t(i);
+                ++pos;
+                a += nodeRange(spanLevel(a, si));
+            }
+            if (pos < capacity) leaves[pos] = si, layout[pos] = uint32_t(i);
+            ++pos;
+        }
+        if (i == n - 1)
+        {
+            uint64_t a = si + nodeRange(s.level[q + 1]);
+            while (a < kKeyEnd)
+            {
+                if (pos < capacity) leaves[pos] = a, layout[pos] = uint32_t(n);
+                ++pos;
+                a += nodeRange(spanLevel(a, kKeyEnd));
+            }
+            if (pos <= capacity) leaves[pos] = kKeyEnd, layout[pos] = uint32_t(n);
+        }
+    }
+}
+
+__global__ void leafCountsKernel(const uint32_t* __restrict__ layout, uint32_t* __restrict__ counts, int L)
+{
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < L; i += gridDim.x * blockDim.x)
+        counts[i] = layout[i + 1] - layout[i];
+}
+
+__global__ void singleLeafKernel(uint64_t* leaves, uint32_t* counts, uint32_t* layout, uint32_t n)
+{
+    leaves[0] = 0, leaves[1] = kKeyEnd;
+    counts[0] = n;
+    layout[0] = 0, layout[1] = n;
+}
+
+int buildLeaves(Context* ctx, const uint64_t* keys, size_t n, unsigned bucket, uint64_t* leaves, uint32_t* counts,
+                uint32_t* layout, size_t capacity, int* numLeaves)
+{
+    hipStream_t s = ctx->stream;
+    if (bucket == 0) return setError(ctx, RYO_ERR_ARG, "leaves: bucket size must be positive");
+    if (n <= bucket)
+    {
+        *numLeaves = 1;
+        if (capacity < 1) return setError(ctx, RYO_ERR_CAPACITY, "leaves: capacity 0 < 1");
+        singleLeafKernel<<<1, 1, 0, s>>>(leaves, counts, layout, uint32_t(n));
+        RYO_LAUNCH_CHECK(ctx);
+        return RYO_OK;
+    }
+    if (bucket > unsigned(kMaxBucket))
+        return setError(ctx, RYO_ERR_UNSUPPORTED, "leaves: bucket size %u > %d", bucket, kMaxBucket);
+    int numBlocks = int((n + kLeafTile - 1) / kLeafTile);
+    auto* sums    = static_cast<uint32_t*>(ctx->getScratch(kSlotLeafBlocks, (numBlocks + 1) * sizeof(uint32_t)));
+    if (!sums) return RYO_ERR_HIP;
+    leafCountKernel<<<numBlocks, kLeafThreads, 0, s>>>(keys, long(n), int(bucket), sums);
+    scanBlocksKernel<<<1, 256, 0, s>>>(sums, numBlocks);
+    RYO_LAUNCH_CHECK(ctx);
+    uint32_t L = 0;
+    RYO_HIP(ctx, hipMemcpyAsync(&L, sums + numBlocks, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+    RYO_HIP(ctx, hipStreamSynchronize(s));
+    *numLeaves = int(L);
+    if (size_t(L) > capacity) return setError(ctx, RYO_ERR_CAPACITY, "leaves: %u leaves > capacity %zu", L, capacity);
+    leafEmitKernel<<<numBlocks, kLeafThreads, 0, s>>>(keys, long(n), int(bucket), sums, leaves, layout, long(L));
+    leafCountsKernel<<<iceil(L, 256), 256, 0, s>>>(layout, counts, int(L));
+    RYO_LAUNCH_CHECK(ctx);
+    return RYO_OK;
+}
+
+/* ============================================================================ linked octree */
+
+__device__ __forceinline__ int digitWeight(int d)
+{
+    int ge4 = -int(d >= 4);
+    return ((7 - d) & ge4) - (d & ~ge4);
+}
+
+// prefixes and unsorted-layout indices of leaves (at numInternal + leaf) and internal nodes (at the
+// binary-tree index derived from the leaf boundary between the node's 4th and 5th child)
+__global__ void unsortedLayoutKernel(const uint64_t* __restrict__ leaves, int L, int nI, uint64_t* __restrict__ pre,
+                                     int32_t* __restrict__ idx)
+{
+    for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < L; t += gridDim.x * blockDim.x)
+    {
+        uint64_t key  = leaves[t], next = leaves[t + 1];
+        unsigned lvl  = treeLevel(next - key);
+        pre[t + nI]   = encodePlaceholder(key, 3 * lvl);
+        idx[t + nI]   = t + nI;
+        int pl        = commonPrefix(key, next);
+        if (pl % 3 == 0 && t < L - 1)
+        {
+            unsigned level = unsigned(pl) / 3u;
+            int w          = 0;
+            for (unsigned l = 1; l <= level + 1; ++l)
+                w += digitWeight(int(octalDigit(key, l)));
+            int o  = (t + w) / 7;
+            pre[o] = encodePlaceholder(key, pl);
+            idx[o] = o;
+        }
+    }
+}
+
+__global__ void invertKernel(int32_t* __restrict__ i2l, int32_t* __restrict__ l2i, int n, int nI)
+{
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+    {
+        int u  = i2l[i];
+        l2i[u] = i;
+        i2l[i] = u - nI;
+    }
+}
+
+__device__ inline int lowerBound(const uint64_t* a, int lo, int hi, uint64_t v)
+{
+    while (lo < hi)
+    {
+        int mid = (lo + hi) >> 1;
+        if (a[mid] < v) lo = mid + 1;
+        else hi = mid;
+    }
+    return lo;
+}
+
+__global__ void levelRangeKernel(const uint64_t* __restrict__ pre, int n, int32_t* __restrict__ levelRange)
+{
+    int l = threadIdx.x;
+    if (l <= int(kMaxLevel)) levelRange[l] = lowerBound(pre, 0, n, encodePlaceholder(0, 3 * l));
+    if (l == int(kMaxLevel) + 1) levelRange[l] = n;
+}
+
+__global__ void zeroI32Kernel(int32_t* a, int n)
+{
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+        a[i] = 0;
+}
+
+__global__ void linkKernel(const uint64_t* __restrict__ pre, int nI, const int32_t* __restrict__ l2i,
+                           const int32_t* __restrict__ levelRange, int32_t* __restrict__ child,
+                           int32_t* __restrict__ parents)
+{
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < nI; i += gridDim.x * blockDim.x)
+    {
+        int a           = l2i[i];
+        uint64_t prefix = pre[a];
+        unsigned pl     = prefixLength(prefix);
+        unsigned level  = pl / 3;
+        uint64_t cp     = encodePlaceholder(decodePlaceholder(prefix), int(pl) + 3);
+        int lo = levelRange[level + 1], hi = levelRange[level + 2];
+        int c = lowerBound(pre, lo, hi, cp);
+        if (c != hi && pre[c] == cp)
+        {
+            child[a]             = c;
+            parents[(c - 1) / 8] = a;
+        }
+    }
+}
+
+int linkOctree(Context* ctx, const uint64_t* leaves, int L, uint64_t* prefixes, int32_t* child, int32_t* parents,
+               int32_t* levelRange, int32_t* i2l, int32_t* l2i, int32_t* levelRangeHost)
+{
+    if (L < 1) return setError(ctx, RYO_ERR_ARG, "octree: need at least one leaf");
+    hipStream_t s = ctx->stream;
+    int nI = (L - 1) / 7, n = L + nI;
+    unsigned g = std::min(iceil(n, 256), unsigned(ctx->numCUs * 8));
+    unsortedLayoutKernel<<<g, 256, 0, s>>>(leaves, L, nI, prefixes, i2l);
+    size_t tb   = sortTempBytes(n);
+    size_t need = n * (sizeof(uint64_t) + sizeof(int32_t)) + tb + 512;
+    auto* buf   = static_cast<char*>(ctx->getScratch(kSlotLinkPairs, need));
+    if (!buf) return RYO_ERR_HIP;
+    auto* kb = reinterpret_cast<uint64_t*>(buf);
+    auto* vb = reinterpret_cast<int32_t*>(buf + n * sizeof(uint64_t));
+    void* tmp = buf + ((n * (sizeof(uint64_t) + sizeof(int32_t)) + 255) & ~size_t(255));
+    int rc = sortPairs(ctx, prefixes, i2l, n, kb, vb, tmp, tb);
+    if (rc) return rc;
+    invertKernel<<<g, 256, 0, s>>>(i2l, l2i, n, nI);
+    levelRangeKernel<<<1, 64, 0, s>>>(prefixes, n, levelRange);
+    zeroI32Kernel<<<g, 256, 0, s>>>(child, n + 1);
+    int np = std::max(1, (n - 1) / 8);
+    zeroI32Kernel<<<iceil(np, 256), 256, 0, s>>>(parents, np);
+    if (nI > 0) linkKernel<<<std::max(1u, std::min(iceil(nI, 256), unsigned(ctx->numCUs * 8))), 256, 0, s>>>(
+        prefixes, nI, l2i, levelRange, child, parents);
+    RYO_LAUNCH_CHECK(ctx);
+    if (levelRangeHost)
+    {
+        RYO_HIP(ctx, hipMemcpyAsync(levelRangeHost, levelRange, (kMaxLevel + 2) * sizeof(int32_t),
+                                    hipMemcpyDeviceToHost, s));
+        RYO_HIP(ctx, hipStreamSynchronize(s));
+    }
+    return RYO_OK;
+}
+
+/* ============================================================================ fixed groups */
+
+__global__ void fixedGroupsKernel(uint32_t first, uint32_t last, unsigned gs, uint32_t* groups, int numGroups)
+{
+    for (int g = blockIdx.x * blockDim.x + threadIdx.x; g <= numGroups; g += gridDim.x * blockDim.x)
+        groups[g] = g == numGroups ? last : first + uint32_t(g) * gs;
+}
+
+int fixedGroups(Context* ctx, uint32_t first, uint32_t last, unsigned gs, uint32_t* groups, int* numGroups)
+{
+    if (gs == 0 || last < first) return setError(ctx, RYO_ERR_ARG, "groups: invalid range or group size");
+    int G      = int((last - first + gs - 1) / gs);
+    *numGroups = G;
+    fixedGroupsKernel<<<iceil(G + 1, 256), 256, 0, ctx->stream>>>(first, last, gs, groups, G);
+    RYO_LAUNCH_CHECK(ctx);
+    return RYO_OK;
+}
+
+} // namespace ryo

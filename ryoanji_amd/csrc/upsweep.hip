@@ -1,0 +1,248 @@
+// Fused FP64 upsweep (gfx950): leaf mass centers and P2M, |m|-weighted centers and M2M of internal nodes up to
+// the root, and the vector-MAC radius, in ONE launch without per-level host loops.
+//
+// Each leaf is reduced by 8 lanes (particles strided by 8, shuffle reduction). The lane that finishes a node
+// increments its parent's arrival counter (acquire-release at agent scope); the 8th arrival combines the
+// 8 children and climbs on, so every internal node is computed exactly once, after all of its children.
+//
+// Formulas: addBody/normalizeMass/CombineSourceCenter (cstone/focus/source_center.hpp:46-98),
+// P2M_add/P2M_finalize (ryoanji/nbody/cartesian_qpole.hpp:67-117), addQuadrupole (:187-210),
+// computeVecMacR2 (cstone/traversal/macs.hpp:64-82) with hilbertIBox/centerAndSize (hilbert.hpp:549-585,
+// box.hpp:357-381). All arithmetic is FP64; the traversal copies are rounded to FP32 once.
+#include "common.hpp"
+
+namespace ryo
+{
+
+struct MacBox
+{
+    double lo[3];
+    double half[3]; // half of the smallest cell edge, per axis
+    float invTheta;
+};
+
+// integer coordinates of the corner of the cell with Hilbert start key `key` (inverse of the encoding)
+__device__ inline void hilbertDecode(uint64_t key, unsigned& ox, unsigned& oy, unsigned& oz)
+{
+    unsigned px = 0, py = 0, pz = 0;
+    for (unsigned level = 0; level < kMaxLevel; ++level)
+    {
+        unsigned oct = unsigned(key >> (3 * level)) & 7u;
+        unsigned xi = oct >> 2u, yi = (oct >> 1u) & 1u, zi = oct & 1u;
+        unsigned tx = px, ty = py, tz = pz;
+        if (yi ^ zi) // inverse cyclic rotation
+        {
+            px = tz, pz = ty, py = tx;
+        }
+        else if ((xi | yi | zi) == 0u || (xi & yi & zi) == 1u) // swap x and z
+        {
+            px = tz, pz = tx;
+        }
+        unsigned mask = (1u << level) - 1u;
+        unsigned nxi = xi ^ 1u, nyi = yi ^ 1u, nzi = zi ^ 1u;
+        px ^= mask & (0u - (xi & (yi | zi)));
+        py ^= mask & (0u - ((xi & (nyi | nzi)) | (nxi & yi & zi)));
+        pz ^= mask & (0u - ((xi & nyi & nzi) | (yi & zi)));
+        px |= xi << level;
+        py |= (xi ^ yi) << level;
+        pz |= (yi ^ zi) << level;
+    }
+    ox = px, oy = py, oz = pz;
+}
+
+// squared vector-MAC radius (2 max(size)/theta + |com - geometric center|)^2 of the node with `prefix`
+__device__ inline double macSquared(uint64_t prefix, double cx, double cy, double cz, const MacBox& b)
+{
+    uint64_t key   = decodePlaceholder(prefix);
+    unsigned level = prefixLength(prefix) / 3u;
+    unsigned ix, iy, iz;
+    hilbertDecode(key, ix, iy, iz);
+    unsigned cube = 1u << (kMaxLevel - level);
+    ix &= ~(cube - 1), iy &= ~(cube - 1), iz &= ~(cube - 1);
+    double gc[3], gs[3];
+    unsigned lo[3] = {ix, iy, iz};
+    for (int d = 0; d < 3; ++d)
+    {
+        gc[d] = b.lo[d] + double(2 * lo[d] + cube) * b.half[d];
+        gs[d] = double(cube) * b.half[d];
+    }
+    double dx = cx - gc[0], dy = cy - gc[1], dz = cz - gc[2];
+    double s   = sqrt(dx * dx + dy * dy + dz * dz);
+    double l   = 2.0 * fmax(fmax(gs[0], gs[1]), gs[2]);
+    double mac = l * double(b.invTheta) + s;
+    return mac * mac;
+}
+
+struct UpsweepArgs
+{
+    const float *x, *y, *z, *m;
+    const uint32_t* layout;
+    int numLeaves;
+    const int32_t* leafNode; // leafToInternal + numInternal
+    const int32_t* parents;
+    const int32_t* child;
+    const uint64_t* prefixes;
+    float4* c32;
+    float* q32;
+    double* c64; // [n][4] {com, sum|m|}
+    double* q64; // [n][8]
+    int* arrivals;
+    MacBox box;
+};
+
+__device__ inline void storeNode(const UpsweepArgs& a, int node, const double c[4], const double q[8])
+{
+    double* c64 = a.c64 + 4 * size_t(node);
+    double* q64 = a.q64 + 8 * size_t(node);
+    for (int k = 0; k < 4; ++k)
+        c64[k] = c[k];
+    for (int k = 0; k < 8; ++k)
+        q64[k] = q[k];
+    double mac2 = c[3] != 0.0 ? macSquared(a.prefixes[node], c[0], c[1], c[2], a.box) : 0.0;
+    a.c32[node] = make_float4(float(c[0]), float(c[1]), float(c[2]), float(mac2));
+    float* q32  = a.q32 + 8 * size_t(node);
+    for (int k = 0; k < 8; ++k)
+        q32[k] = float(q[k]);
+}
+
+__device__ inline double red8(double v)
+{
+    v += __shfl_xor(v, 1);
+    v += __shfl_xor(v, 2);
+    v += __shfl_xor(v, 4);
+    return v;
+}
+
+__global__ __launch_bounds__(256) void upsweepKernel(UpsweepArgs a)
+{
+    const int leaf = int((size_t(blockIdx.x) * blockDim.x + threadIdx.x) >> 3);
+    const int sub  = threadIdx.x & 7;
+    const bool ok  = leaf < a.numLeaves;
+    uint32_t b = 0, e = 0;
+    if (ok) b = a.layout[leaf], e = a.layout[leaf + 1];
+
+    double c[4] = {0, 0, 0, 0};
+    for (uint32_t p = b + sub; p < e; p += 8)
+    {
+        double w = fabs(double(a.m[p]));
+        c[0] += w * double(a.x[p]);
+        c[1] += w * double(a.y[p]);
+        c[2] += w * double(a.z[p]);
+        c[3] += w;
+    }
+    for (int k = 0; k < 4; ++k)
+        c[k] = red8(c[k]);
+    double inv = c[3] != 0.0 ? 1.0 / c[3] : 1.0;
+    c[0] *= inv, c[1] *= inv, c[2] *= inv;
+
+    double q[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    for (uint32_t p = b + sub; p < e; p += 8)
+    {
+        double rx = double(a.x[p]) - c[0], ry = double(a.y[p]) - c[1], rz = double(a.z[p]) - c[2];
+        double mi = double(a.m[p]);
+        q[0] += mi;
+        q[1] += rx * rx * mi;
+        q[2] += rx * ry * mi;
+        q[3] += rx * rz * mi;
+        q[4] += ry * ry * mi;
+        q[5] += ry * rz * mi;
+        q[6] += rz * rz * mi;
+    }
+    for (int k = 0; k < 7; ++k)
+        q[k] = red8(q[k]);
+    double tr = q[1] + q[4] + q[6];
+    q[7] = tr;
+    q[1] = 3 * q[1] - tr;
+    q[4] = 3 * q[4] - tr;
+    q[6] = 3 * q[6] - tr;
+    q[2] *= 3, q[3] *= 3, q[5] *= 3;
+
+    if (!ok || sub != 0) return;
+    int node = a.leafNode[leaf];
+    storeNode(a, node, c, q);
+
+    // climb: the last of 8 siblings to arrive computes the parent
+    while (node != 0)
+    {
+        int parent = a.parents[(node - 1) / 8];
+        int old    = __hip_atomic_fetch_add(&a.arrivals[parent], 1, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+        if (old != 7) break;
+        int ch      = a.child[parent];
+        double pc[4] = {0, 0, 0, 0};
+        for (int k = 0; k < 8; ++k)
+        {
+            const double* s = a.c64 + 4 * size_t(ch + k);
+            double w        = fabs(s[3]);
+            pc[0] += w * s[0];
+            pc[1] += w * s[1];
+            pc[2] += w * s[2];
+            pc[3] += w;
+        }
+        double pinv = pc[3] != 0.0 ? 1.0 / pc[3] : 1.0;
+        pc[0] *= pinv, pc[1] *= pinv, pc[2] *= pinv;
+        double pq[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+        for (int k = 0; k < 8; ++k)
+        {
+            const double* s  = a.c64 + 4 * size_t(ch + k);
+            const double* cq = a.q64 + 8 * size_t(ch + k);
+            double rx = pc[0] - s[0], ry = pc[1] - s[1], rz = pc[2] - s[2];
+            double rx2 = rx * rx, ry2 = ry * ry, rz2 = rz * rz;
+            double r2  = (rx2 + ry2 + rz2) * (1.0 / 3.0);
+            double ml  = cq[0] * 3;
+            pq[7] += cq[7] + ml * r2;
+            pq[0] += cq[0];
+            pq[1] += cq[1] + ml * (rx2 - r2);
+            pq[2] += cq[2] + ml * rx * ry;
+            pq[3] += cq[3] + ml * rx * rz;
+            pq[4] += cq[4] + ml * (ry2 - r2);
+            pq[5] += cq[5] + ml * ry * rz;
+            pq[6] += cq[6] + ml * (rz2 - r2);
+        }
+        storeNode(a, parent, pc, pq);
+        node = parent;
+    }
+}
+
+__global__ void zeroIntKernel(int* a, int n)
+{
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+        a[i] = 0;
+}
+
+int upsweep(Context* ctx, const float* x, const float* y, const float* z, const float* m, const uint32_t* layout,
+            int L, const uint64_t* prefixes, const int32_t* child, const int32_t* parents, const int32_t* l2i,
+            const float* limits, float invTheta, float* centers, float* mpoles, double* c64, double* q64)
+{
+    if (L < 1) return setError(ctx, RYO_ERR_ARG, "upsweep: need at least one leaf");
+    int nI = (L - 1) / 7, n = L + nI;
+    size_t need = 0;
+    if (!c64) need += size_t(n) * 4 * sizeof(double);
+    if (!q64) need += size_t(n) * 8 * sizeof(double);
+    need += size_t(n) * sizeof(int) + 256;
+    auto* buf = static_cast<char*>(ctx->getScratch(kSlotUpsweep, need));
+    if (!buf) return RYO_ERR_HIP;
+    size_t off = 0;
+    if (!c64) c64 = reinterpret_cast<double*>(buf + off), off += size_t(n) * 4 * sizeof(double);
+    if (!q64) q64 = reinterpret_cast<double*>(buf + off), off += size_t(n) * 8 * sizeof(double);
+    int* arrivals = reinterpret_cast<int*>(buf + off);
+
+    UpsweepArgs a;
+    a.x = x, a.y = y, a.z = z, a.m = m;
+    a.layout = layout, a.numLeaves = L;
+    a.leafNode = l2i + nI, a.parents = parents, a.child = child, a.prefixes = prefixes;
+    a.c32 = reinterpret_cast<float4*>(centers), a.q32 = mpoles, a.c64 = c64, a.q64 = q64, a.arrivals = arrivals;
+    for (int d = 0; d < 3; ++d)
+    {
+        double lo = limits[2 * d], hi = limits[2 * d + 1];
+        a.box.lo[d]   = lo;
+        a.box.half[d] = 0.5 * (hi - lo) / double(1u << kMaxLevel);
+    }
+    a.box.invTheta = invTheta;
+    hipStream_t s = ctx->stream;
+    zeroIntKernel<<<std::min(iceil(n, 256), unsigned(ctx->numCUs * 8)), 256, 0, s>>>(arrivals, n);
+    upsweepKernel<<<iceil(size_t(L) * 8, 256), 256, 0, s>>>(a);
+    RYO_LAUNCH_CHECK(ctx);
+    return RYO_OK;
+}
+
+} // namespace ryo

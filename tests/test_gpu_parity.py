@@ -1,0 +1,191 @@
+"""GPU parity of every stage of the path against the oracle, through the C ABI.
+
+Integer stages (keys, sort permutation, leaves/counts/layout, linked-octree arrays) must be bit-exact.
+The FP64 upsweep is compared with the oracle's FP64 instantiation (1e-12 relative) and with the float
+reference pipeline (1e-5). Traversal parity uses identical float inputs and the reference grouping of 16
+targets: per-particle relative acceleration error p99 <= 1e-5 (the path's stated tolerance).
+"""
+
+import numpy as np
+import pytest
+
+import oracle as O
+from ryoanji_amd import workloads as W
+
+from gpu_util import (gpu_keys, gpu_leaves, gpu_link, gpu_sort, gpu_traverse, gpu_upsweep, oracle_tree, rel_err)
+
+pytestmark = pytest.mark.gpu
+
+SYSTEMS = {
+    "uniform65536": lambda: W.uniform_cube(65536, seed=1),
+    "plummer32768": lambda: W.plummer(32768, seed=2),
+    "hernquist40000": lambda: W.hernquist_mix(40000, seed=3),
+}
+
+
+def _dups(n=5000, seed=4):
+    """many particles on a few identical positions: level-21 leaves with more than `bucket` particles"""
+    s = W.uniform_cube(n, seed=seed)
+    k = np.arange(n) % 37
+    for a in (s.x, s.y, s.z):
+        a[:] = a[k]
+    return s
+
+
+@pytest.mark.parametrize("name", list(SYSTEMS))
+def test_keys_bit_exact(ctx, name):
+    s = SYSTEMS[name]()
+    for kind, hil in ((0, True), (1, False)):
+        g = gpu_keys(ctx, s, kind=kind)
+        o = O.sfc_keys(s.x, s.y, s.z, s.limits, hilbert=hil)
+        assert np.array_equal(g, o), f"{name} kind {kind}: {np.count_nonzero(g != o)} keys differ"
+
+
+def test_keys_remove_key_untouched(ctx):
+    s = W.uniform_cube(10000, seed=5)
+    pre = np.zeros(s.n, np.uint64)
+    pre[::7] = np.uint64(1 << 63)
+    g = gpu_keys(ctx, s, keys=pre, skip_removed=1)
+    o = O.sfc_keys(s.x, s.y, s.z, s.limits, keys=pre)
+    assert np.array_equal(g, o)
+    assert np.all(g[::7] == np.uint64(1 << 63))
+
+
+@pytest.mark.parametrize("n", [1, 2, 100, 4095, 4096, 4097, 70001, 1 << 20])
+def test_sort_stable_bit_exact(ctx, n):
+    rng = np.random.default_rng(n)
+    # few distinct values in the high bytes and ties: stability matters
+    keys = (rng.integers(0, 1 << 20, n, dtype=np.uint64) << np.uint64(40)) | rng.integers(0, 4, n, dtype=np.uint64)
+    vals = rng.permutation(n).astype(np.int32)
+    gk, gv = gpu_sort(ctx, keys, vals)
+    ok, ov = O.sort_by_key(keys, vals)
+    assert np.array_equal(gk, ok)
+    assert np.array_equal(gv, ov)
+
+
+def test_sort_full_64bit(ctx):
+    rng = np.random.default_rng(9)
+    keys = rng.integers(0, np.iinfo(np.uint64).max, 200000, dtype=np.uint64, endpoint=True)
+    keys[::3] = keys[1::3][: keys[::3].size]
+    gk, gv = gpu_sort(ctx, keys)
+    ok, ov = O.sort_by_key(keys)
+    assert np.array_equal(gk, ok) and np.array_equal(gv, ov)
+
+
+CASES = list(SYSTEMS) + ["dups", "n1", "n64", "n65", "n1000"]
+
+
+def _case(name):
+    if name in SYSTEMS:
+        return SYSTEMS[name]()
+    if name == "dups":
+        return _dups()
+    return W.uniform_cube(int(name[1:]), seed=11)
+
+
+@pytest.mark.parametrize("name", CASES)
+@pytest.mark.parametrize("bucket", [64, 16])
+def test_leaves_and_layout_bit_exact(ctx, name, bucket):
+    s = _case(name)
+    sk, _ = O.sort_by_key(O.sfc_keys(s.x, s.y, s.z, s.limits))
+    leaves, counts = O.compute_octree(sk, bucket)
+    layout = O.layout_from_counts(counts)
+    gl, gc, gy = gpu_leaves(ctx, sk, bucket)
+    assert gl.size == leaves.size, f"{gl.size - 1} leaves vs {leaves.size - 1}"
+    assert np.array_equal(gl, leaves)
+    assert np.array_equal(gc, counts)
+    assert np.array_equal(gy, layout)
+
+
+@pytest.mark.parametrize("name", CASES)
+def test_linked_octree_bit_exact(ctx, name):
+    s = _case(name)
+    sk, _ = O.sort_by_key(O.sfc_keys(s.x, s.y, s.z, s.limits))
+    leaves, _ = O.compute_octree(sk, 64)
+    o = O.link_octree(leaves)
+    g = gpu_link(ctx, leaves)
+    for k in ("prefixes", "childOffsets", "parents", "levelRange", "internalToLeaf", "leafToInternal"):
+        assert np.array_equal(g[k], o[k]), k
+    assert np.array_equal(g["levelRangeHost"], o["levelRange"])
+
+
+@pytest.mark.parametrize("name", list(SYSTEMS) + ["dups"])
+def test_upsweep_fp64(ctx, name):
+    s = _case(name)
+    t = oracle_tree(s)
+    c32, q32, c64, q64 = gpu_upsweep(ctx, t["x"], t["y"], t["z"], t["m"], t["layout"], t["tree"], s.limits, 2.0)
+    oc, oq = O.upsweep(t["x"], t["y"], t["z"], t["m"], t["leaves"], t["layout"], s.limits.astype(np.float64), 2.0,
+                       precision="f64")
+    # FP64 against the FP64 instantiation: only the summation order differs
+    scale = np.abs(oq).max(axis=0) + 1e-300
+    assert np.allclose(c64[:, :3], oc[:, :3], rtol=1e-12, atol=1e-12 * np.abs(oc[:, :3]).max())
+    assert np.all(np.abs(q64 - oq) <= 1e-11 * scale)
+    # the FP32 copies against the reference float pipeline (mac^2 in column 3)
+    fc, fq = t["centers"], t["mp"]
+    assert np.allclose(c32[:, :3], fc[:, :3], rtol=0, atol=2e-6 * np.abs(fc[:, :3]).max())
+    assert np.allclose(c32[:, 3], fc[:, 3], rtol=2e-5, atol=0)
+    fscale = np.abs(fq).max(axis=0) + 1e-30
+    assert np.all(np.abs(q32 - fq) <= 1e-4 * fscale)
+
+
+@pytest.mark.parametrize("name", list(SYSTEMS))
+def test_traversal_parity_16_targets(ctx, name):
+    """same tree, same float centers/multipoles, reference grouping of 16: <= 1e-5 at p99"""
+    s = SYSTEMS[name]()
+    t = oracle_tree(s)
+    args = (t["x"], t["y"], t["z"], t["h"], t["m"])
+    otot, oug, oax, oay, oaz, ost = O.compute_gravity(t["leaves"], t["layout"], t["centers"], t["mp"], *args, s.limits,
+                                                      group_size=16)
+    gtot, gp, gax, gay, gaz, gst = gpu_traverse(ctx, t["x"], t["y"], t["z"], t["m"], t["h"], t["tree"], t["layout"],
+                                                t["centers"], t["mp"], 16)
+    e = np.sort(rel_err(gax, gay, gaz, oax, oay, oaz))
+    assert e[int(0.99 * e.size)] <= 1e-5, f"p99 {e[int(0.99 * e.size)]:.3e} max {e[-1]:.3e}"
+    assert abs(gtot - otot) <= 1e-5 * abs(otot)
+    pe = np.abs(gp - oug) / np.abs(oug)
+    assert np.quantile(pe, 0.99) <= 1e-5
+    # interaction counts: the oracle also counts M2P with empty cells; P2P counts agree exactly
+    assert gst["sumP2P"] == ost[0]
+    assert gst["sumM2P"] <= ost[1]
+
+
+@pytest.mark.parametrize("name", ["uniform65536", "plummer32768"])
+def test_traversal_parity_64_targets(ctx, name):
+    """the production grouping (one wave = 64 targets) against the oracle run with the same grouping"""
+    s = SYSTEMS[name]()
+    t = oracle_tree(s)
+    args = (t["x"], t["y"], t["z"], t["h"], t["m"])
+    otot, _, oax, oay, oaz, ost = O.compute_gravity(t["leaves"], t["layout"], t["centers"], t["mp"], *args, s.limits,
+                                                    group_size=64)
+    for init in (0, 1):
+        gtot, _, gax, gay, gaz, gst = gpu_traverse(ctx, t["x"], t["y"], t["z"], t["m"], t["h"], t["tree"],
+                                                   t["layout"], t["centers"], t["mp"], 64, init_node=init)
+        e = np.sort(rel_err(gax, gay, gaz, oax, oay, oaz))
+        assert e[int(0.99 * e.size)] <= 1e-5, (init, e[int(0.99 * e.size)])
+        assert gst["sumP2P"] == ost[0]
+
+
+def test_traversal_split_retry_identical(ctx):
+    """a stack capacity of 64 entries forces overflow -> split/retry; results equal the normal run"""
+    s = W.plummer(20000, seed=8)
+    t = oracle_tree(s)
+    a = gpu_traverse(ctx, t["x"], t["y"], t["z"], t["m"], t["h"], t["tree"], t["layout"], t["centers"], t["mp"], 64)
+    b = gpu_traverse(ctx, t["x"], t["y"], t["z"], t["m"], t["h"], t["tree"], t["layout"], t["centers"], t["mp"], 64,
+                     stack_cap=64)
+    import ryoanji_amd as R
+
+    assert R.lib().ryo_traversal_retries(ctx.h) > 0
+    e = rel_err(b[2], b[3], b[4], a[2], a[3], a[4])
+    assert e.max() <= 1e-5
+    assert abs(a[0] - b[0]) <= 1e-6 * abs(a[0])
+
+
+def test_single_leaf_and_tiny(ctx):
+    for n in (1, 2, 30, 64):
+        s = W.uniform_cube(n, seed=n)
+        t = oracle_tree(s)
+        args = (t["x"], t["y"], t["z"], t["h"], t["m"])
+        _, _, oax, oay, oaz, _ = O.compute_gravity(t["leaves"], t["layout"], t["centers"], t["mp"], *args, s.limits,
+                                                   group_size=16)
+        _, _, gax, gay, gaz, _ = gpu_traverse(ctx, t["x"], t["y"], t["z"], t["m"], t["h"], t["tree"], t["layout"],
+                                              t["centers"], t["mp"], 16, init_node=1)
+        assert np.allclose(gax, oax, rtol=1e-5, atol=1e-6 * np.abs(oax).max() + 1e-30)

@@ -1,0 +1,45 @@
+"""CPU checks of the product library: it is built for gfx950, loads, and exports every C entry point that
+include/*.h declares (no compute call without a GPU)."""
+
+import glob
+import os
+import re
+
+import ryoanji_amd as R
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_symbols():
+    names = set()
+    for h in glob.glob(os.path.join(ROOT, "include", "*.h")):
+        src = open(h).read()
+        src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
+        for m in re.finditer(r"^\s*(?:[A-Za-z_][\w\s\*]*?)\b(ryo_\w+)\s*\(", src, flags=re.M):
+            names.add(m.group(1))
+    return names
+
+
+def test_header_declares_entry_points():
+    names = declared_symbols()
+    assert {"ryo_sfc_keys_f32", "ryo_sort_pairs_u64_i32", "ryo_build_leaves_u64", "ryo_link_octree_u64",
+            "ryo_upsweep_f32", "ryo_traverse_f32", "ryo_ewald_f32", "ryo_direct_sum_f64"} <= names
+
+
+def test_library_exports_every_declared_symbol():
+    lib = R.lib()
+    missing = [n for n in sorted(declared_symbols()) if not hasattr(lib, n)]
+    assert not missing, missing
+
+
+def test_python_signatures_cover_the_header():
+    assert declared_symbols() <= set(R.SIGNATURES)
+
+
+def test_library_is_built_for_gfx950():
+    # the embedded HIP fat binary names its offload target
+    assert b"amdgcn-amd-amdhsa--gfx950" in open(R.LIB_PATH, "rb").read()
+
+
+def test_sort_temp_bytes_is_host_only():
+    assert R.lib().ryo_sort_temp_bytes(1 << 20) > 0

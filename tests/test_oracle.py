@@ -1,0 +1,278 @@
+"""Pins of the CPU oracle (no GPU): the reference's own goldens and known-answer tests, closed forms and brute
+force, the committed reference fixtures (tests/golden) and, where it was built, the reference harness."""
+
+import glob
+import math
+import os
+
+import numpy as np
+import pytest
+
+import oracle as O
+from ryoanji_amd import workloads as W
+
+GOLDEN = sorted(glob.glob(os.path.join(os.path.dirname(__file__), "golden", "ref_*.npz")))
+
+
+# ---------------------------------------------------------------- kernels: reference goldens (kernel.cpp:27-78)
+
+@pytest.mark.parametrize("dh,pot,acc", [(-0.001, -0.76980035891950138, 0.17106674642655587),
+                                        (+0.001, -0.7678049688481372, 0.1704016164027678)])
+def test_p2p_reference_goldens(dh, pot, acc):
+    h = math.sqrt(3) / 2 + dh
+    for exact in (True, False):
+        a = np.zeros(4)
+        a = O.p2p(a, [1, 1, 1], [2, 2, 2], 1.0, h, h, exact=exact)
+        a = O.p2p(a, [1, 1, 1], [-2, -2, -2], 1.0, h, h, exact=exact)
+        tol = 1e-10 if exact else 1e-7  # the non-exact kernel uses a single-precision rsqrt
+        assert abs(a[0] - pot) < tol
+        assert np.all(np.abs(a[1:] - acc) < tol)
+
+
+def test_p2p_self_interaction_is_zero():
+    a = O.p2p(np.zeros(4), [0, 0, 0], [0, 0, 0], 1.0, 0.1, 0.1)
+    assert np.array_equal(a, np.zeros(4))
+
+
+def test_p2p_softening_clamp_closed_form():
+    # R < h_i + h_j: potential -m R^2/(h_ij)^3 ... and acceleration m dX / h_ij^3
+    a = O.p2p(np.zeros(4), [0, 0, 0], [0.01, 0, 0], 2.0, 0.1, 0.1, exact=True)
+    assert np.isclose(a[1], 2.0 * 0.01 / 0.2 ** 3, rtol=1e-14)
+    assert np.isclose(a[0], -2.0 * 0.01 ** 2 / 0.2 ** 3, rtol=1e-14)
+
+
+def _cluster(n, seed):
+    rng = np.random.default_rng(seed)
+    return (rng.uniform(-1, 1, n), rng.uniform(-1, 1, n), rng.uniform(-1, 1, n), rng.uniform(0, 1, n))
+
+
+def test_m2m_equals_p2m_composite():
+    """cartesian_qpole.cpp:84-125: 8 sub-multipoles shifted to the common center equal the direct P2M"""
+    x, y, z, m = _cluster(160, 1)
+    com, ref = O.p2m(x, y, z, m)
+    xs, ms = [], []
+    for i in range(8):
+        sl = slice(20 * i, 20 * (i + 1))
+        c, q = O.p2m(x[sl], y[sl], z[sl], m[sl])
+        xs.append(c)
+        ms.append(q)
+    comp = O.m2m(com, np.array(xs), np.array(ms))
+    assert np.allclose(comp, ref, rtol=0, atol=1e-10)
+
+
+def test_m2p_against_direct_sum():
+    """cartesian_qpole.cpp:68-75: the quadrupole field at distance 8 is within 1e-3 of the direct sum"""
+    x, y, z, m = _cluster(100, 2)
+    com, q = O.p2m(x, y, z, m)
+    t = np.array([-8.0, 0, 0])
+    d = np.zeros(4)
+    for i in range(100):
+        d = O.p2p(d, t, [x[i], y[i], z[i]], m[i], 0.0, 0.0, exact=True)
+    a = O.m2p(np.zeros(4), t, com[:3], q)
+    assert np.all(np.abs(a - d) < 1e-3)
+    # dropping the quadrupole must be measurably worse than keeping it
+    mono = O.m2p(np.zeros(4), t, com[:3], np.r_[q[0], np.zeros(7)])
+    assert np.linalg.norm((a - d)[1:]) < np.linalg.norm((mono - d)[1:])
+
+
+# ---------------------------------------------------------------- Hilbert keys
+
+def test_hilbert_is_a_continuous_bijection_order3():
+    pts = [(x, y, z) for x in range(8) for y in range(8) for z in range(8)]
+    keys = {O.hilbert3d(*p, order=3): p for p in pts}
+    assert sorted(keys) == list(range(512))
+    for k in range(511):
+        a, b = keys[k], keys[k + 1]
+        assert sum(abs(u - v) for u, v in zip(a, b)) == 1  # consecutive keys are face neighbours
+
+
+def test_hilbert_first_octant_map():
+    # hilbert.cpp:84-119: octant (x,y,z) bits -> first digit {0,1,3,2,7,6,4,5}
+    m = {(0, 0, 0): 0, (0, 0, 1): 1, (0, 1, 0): 3, (0, 1, 1): 2, (1, 0, 0): 7, (1, 0, 1): 6, (1, 1, 0): 4,
+         (1, 1, 1): 5}
+    for (x, y, z), d in m.items():
+        assert O.hilbert3d(x, y, z, order=1) == d
+
+
+def test_hilbert_decode_inverts_encode():
+    rng = np.random.default_rng(3)
+    for _ in range(2000):
+        p = tuple(int(v) for v in rng.integers(0, 1 << 21, 3))
+        assert O.decode_hilbert3d(O.hilbert3d(*p)) == p
+
+
+def test_key_cell_contains_point():
+    """sfc.cpp:63-102 (fork patch): the integer cell of a key contains its point"""
+    s = W.uniform_cube(20000, seed=5, lo=-0.37, hi=1.91)
+    keys = O.sfc_keys(s.x, s.y, s.z, s.limits)
+    L = np.float64(s.limits[1]) - np.float64(s.limits[0])
+    for i in range(0, s.n, 97):
+        ix, iy, iz = O.decode_hilbert3d(int(keys[i]))
+        for v, c in ((s.x[i], ix), (s.y[i], iy), (s.z[i], iz)):
+            lo = s.limits[0] + c * L / 2 ** 21
+            assert lo - 1e-6 <= v <= lo + L / 2 ** 21 + 1e-6
+
+
+# ---------------------------------------------------------------- trees
+
+def test_uniform_level2_tree_counts():
+    """octree.cpp:73-199: a uniform level-2 leaf array has level counts 1/8/64 and 73 nodes"""
+    leaves = np.array([i * (1 << (3 * 19)) for i in range(64)] + [1 << 63], np.uint64)
+    t = O.link_octree(leaves)
+    lr = t["levelRange"]
+    assert list(np.diff(lr[:4])) == [1, 8, 64]
+    assert t["numNodes"] == 73
+
+
+@pytest.mark.parametrize("seed", [1, 2])
+def test_octree_is_canonical(seed):
+    """the converged leaf array: every leaf holds <= bucket particles (or sits at level 21), every internal node
+    holds more, and the leaves tile the whole key range"""
+    s = W.plummer(30000, seed=seed)
+    sk, _ = O.sort_by_key(O.sfc_keys(s.x, s.y, s.z, s.limits))
+    leaves, counts = O.compute_octree(sk, 16)
+    assert leaves[0] == 0 and leaves[-1] == np.uint64(1 << 63)
+    rng = (leaves[1:] - leaves[:-1]).astype(np.float64)
+    assert np.all(np.log2(rng) % 3 == 0)  # powers of 8
+    lvl = 21 - np.log2(rng).astype(int) // 3
+    assert np.all((counts <= 16) | (lvl == 21))
+    t = O.link_octree(leaves)
+    lay = O.layout_from_counts(counts)
+    # internal node counts from the layout of their first/last descendant leaves
+    child = t["childOffsets"]
+    i2l = t["internalToLeaf"]
+
+    def span(i):
+        while child[i]:
+            i = child[i]
+        lo = i2l[i]
+        return lo
+
+    for node in np.nonzero(child)[0][:2000]:
+        first = span(node)
+        j = node
+        while child[j]:
+            j = child[j] + 7
+        last = i2l[j]
+        assert lay[last + 1] - lay[first] > 16
+
+
+# ---------------------------------------------------------------- Ewald
+
+def _expected_single_particle_energy(m, N, L, G=1.0, alpha_scale=2.0):
+    """ewald_cpu.cpp:211-234 analytic single-particle Ewald energy"""
+    alpha = alpha_scale / L
+    psi = 0.0
+    for hz in range(10):
+        for hy in range(10):
+            for hx in range(10):
+                if not (hx | hy | hz):
+                    continue
+                h2 = hx * hx + hy * hy + hz * hz
+                h = math.sqrt(h2)
+                psi += math.erfc(alpha * h * L) / h + math.exp(-(math.pi * h / (alpha * L)) ** 2) / (math.pi * h2)
+    U = math.pi / (alpha * alpha * L ** 3) + 2 * alpha / math.sqrt(math.pi) - psi / L
+    return G / 2.0 * m * m * N * U
+
+
+def test_ewald_reference_goldens_and_single_particle():
+    assert abs(_expected_single_particle_energy(1.0, 1, 1.0) - 1.4717464459658256) < 1e-10
+    assert abs(_expected_single_particle_energy(1.0, 1, 2.0) - 0.7358732229829128) < 1e-10
+    # one particle in a periodic box: tree (1 replica shell) + Ewald = the analytic energy (10% as the reference)
+    for L in (2.0, 1.0):
+        x = y = z = np.array([0.0], np.float32)
+        m = np.array([1.0], np.float32)
+        h = np.array([1e-3], np.float32)
+        lim = np.array([-L / 2, L / 2] * 3, np.float32)
+        leaves = np.array([0, 1 << 63], np.uint64)
+        lay = np.array([0, 1], np.uint32)
+        c, q = O.upsweep(x, y, z, m, leaves, lay, lim, 2.0)
+        tot, *_ = O.compute_gravity(leaves, lay, c, q, x, y, z, h, m, lim, num_shells=1)
+        c64, q64 = O.upsweep(x, y, z, m, leaves, lay, lim.astype(np.float64), 2.0, precision="f64")
+        etot, *_ = O.ewald(q64[0], c64[0, :3], x, y, z, m, L)
+        U = tot + etot
+        exp = _expected_single_particle_energy(1.0, 1, L)
+        assert abs(U - exp) / abs(exp) < 0.1
+
+
+def test_ewald_disabled_is_zero():
+    s = W.uniform_cube(500, seed=3, lo=0, hi=1)
+    tot, ug, ax, ay, az = O.ewald(np.ones(8), np.zeros(3), s.x, s.y, s.z, s.m, 1.0, settings=(1, 0, 0, 0, 3e-3))
+    assert tot == 0 and not ax.any() and not ug.any()
+
+
+# ---------------------------------------------------------------- gravity accuracy
+
+def test_gravity_accuracy_vs_fp64_direct():
+    """traversal_cpu.cpp:118-132 style bounds at theta 0.5 (global_forces_gpu.cpp:218-220)"""
+    s = W.uniform_cube(8192, seed=7)
+    keys = O.sfc_keys(s.x, s.y, s.z, s.limits)
+    sk, perm = O.sort_by_key(keys)
+    x, y, z, m, h = (a[perm] for a in s.arrays())
+    leaves, counts = O.compute_octree(sk, 64)
+    lay = O.layout_from_counts(counts)
+    c, q = O.upsweep(x, y, z, m, leaves, lay, s.limits, 2.0)
+    tot, ug, ax, ay, az, _ = O.compute_gravity(leaves, lay, c, q, x, y, z, h, m, s.limits)
+    ref = O.direct_sum(x, y, z, h, m)
+    e = np.sort(np.sqrt((ax - ref[:, 1]) ** 2 + (ay - ref[:, 2]) ** 2 + (az - ref[:, 3]) ** 2) /
+                np.linalg.norm(ref[:, 1:], axis=1))
+    assert e[int(0.99 * e.size)] < 2e-3 and e[-1] < 3e-2
+    assert abs(tot - 0.5 * ref[:, 0].sum()) / abs(0.5 * ref[:, 0].sum()) < 1e-2
+
+
+# ---------------------------------------------------------------- the reference's own outputs
+
+@pytest.mark.parametrize("path", GOLDEN, ids=[os.path.basename(p) for p in GOLDEN])
+def test_oracle_reproduces_reference_fixture(path):
+    g = np.load(path)
+    lim = g["limits"]
+    x, y, z, m, h = (g[k] for k in ("x", "y", "z", "m", "h"))
+    keys = O.sfc_keys(x, y, z, lim)
+    assert np.array_equal(keys, g["keys"])
+    sk, perm = O.sort_by_key(keys)
+    assert np.array_equal(sk, g["sorted_keys"]) and np.array_equal(perm, g["perm"])
+    leaves, counts = O.compute_octree(sk, int(g["bucket"]))
+    assert np.array_equal(leaves, g["leaves"]) and np.array_equal(counts, g["counts"])
+    t = O.link_octree(leaves)
+    for k in ("prefixes", "childOffsets", "parents", "levelRange", "internalToLeaf", "leafToInternal"):
+        assert np.array_equal(t[k], g[k]), k
+    xs, ys, zs, ms, hs = (a[perm] for a in (x, y, z, m, h))
+    lay = O.layout_from_counts(counts)
+    c, q = O.upsweep(xs, ys, zs, ms, leaves, lay, lim, 1.0 / float(g["theta"]))
+    assert np.array_equal(c, g["centers"]) and np.array_equal(q, g["multipoles"])
+    tot, ug, ax, ay, az, _ = O.compute_gravity(leaves, lay, c, q, xs, ys, zs, hs, ms, lim,
+                                               num_shells=int(g["num_shells"]))
+    for k, v in (("ax", ax), ("ay", ay), ("az", az), ("ugrav", ug)):
+        assert np.array_equal(v, g[k]), k
+    if "ewald_total" in g:
+        etot, eug, eax, eay, eaz = O.ewald(g["root_multipole"], g["root_center"], xs, ys, zs, ms,
+                                           float(lim[1] - lim[0]))
+        assert np.allclose(eax, g["ewald_ax"], rtol=1e-12, atol=1e-15)
+        assert abs(etot - float(g["ewald_total"])) < 1e-12 * abs(etot)
+
+
+@pytest.mark.skipif(not O.have_ref(), reason="reference harness not built (no reference checkout)")
+@pytest.mark.parametrize("mk", [lambda: W.uniform_cube(65536, seed=11), lambda: W.plummer(50000, seed=12),
+                                lambda: W.hernquist_mix(50000, seed=13)])
+def test_oracle_equals_reference_harness(mk):
+    s = mk()
+    ko = O.sfc_keys(s.x, s.y, s.z, s.limits)
+    assert np.array_equal(ko, O.sfc_keys(s.x, s.y, s.z, s.limits, lib="ref"))
+    so, po = O.sort_by_key(ko)
+    sr, pr = O.sort_by_key(ko, lib="ref")
+    assert np.array_equal(so, sr) and np.array_equal(po, pr)
+    lo, co = O.compute_octree(so, 64)
+    lr, cr = O.compute_octree(so, 64, lib="ref")
+    assert np.array_equal(lo, lr) and np.array_equal(co, cr)
+    to, tr = O.link_octree(lo), O.link_octree(lo, lib="ref")
+    for k in ("prefixes", "childOffsets", "parents", "levelRange", "internalToLeaf", "leafToInternal"):
+        assert np.array_equal(to[k], tr[k]), k
+    x, y, z, m, h = (a[po] for a in s.arrays())
+    lay = O.layout_from_counts(co)
+    c, q = O.upsweep(x, y, z, m, lo, lay, s.limits, 2.0)
+    cr_, qr = O.upsweep(x, y, z, m, lo, lay, s.limits, 2.0, lib="ref")
+    assert np.array_equal(c, cr_) and np.array_equal(q, qr)
+    go = O.compute_gravity(lo, lay, c, q, x, y, z, h, m, s.limits)
+    gr = O.compute_gravity(lo, lay, c, q, x, y, z, h, m, s.limits, lib="ref")
+    for i in (1, 2, 3, 4):
+        assert np.array_equal(go[i], gr[i])
