@@ -194,6 +194,8 @@ int ryo_system_arrays(ryo_system* sys, float** xyzmh_v /* host float*[8] */);
 /* copy n particles in (device or host memory, hipMemcpyDefault) */
 int ryo_system_load(ryo_system* sys, size_t n, const float* x, const float* y, const float* z, const float* m,
                     const float* h, const float* limits /* host float[6] */, int periodic);
+/* copy the system's particles (current SFC order) into caller device arrays; any pointer may be NULL */
+int ryo_system_store(ryo_system* sys, float* x, float* y, float* z, float* m, float* h);
 /* keys -> sort -> permute (x,y,z,m,h and velocities) -> leaves -> linked octree -> FP64 upsweep */
 int ryo_system_build(ryo_system* sys, const ryo_params* params);
 /* traversal (+ Ewald); outputs in the system's SFC order, written (not added) to the system's p, ax, ay, az */

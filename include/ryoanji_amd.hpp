@@ -1,0 +1,331 @@
+/*
+ * ryoanji_amd.hpp -- C++ entry points with the reference's names and argument meaning, over the C ABI.
+ *
+ * Host-only, header-only. A caller written against cstone/ryoanji's GPU functions finds the same names,
+ * namespaces and argument order here (the reference's own types are mirrored minimally: Box<float>,
+ * GroupView, OctreeView, execution::Gpu, Vec3/Vec4, CartesianQuadrupole<float>). Errors are reported as
+ * std::runtime_error (the reference throws for configuration errors and exits on GPU errors; this layer
+ * never exits). The context (stream, scratch, statistics) is the thread's current Session.
+ *
+ * Paths cited are relative to the reference checkout.
+ */
+#pragma once
+
+#include <array>
+#include <cstdint>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "ryoanji_amd.h"
+
+namespace ryoanji_amd
+{
+//! the calling thread's C-ABI context (stream, scratch, statistics); created on first use on the null stream
+inline ryo_ctx*& threadContext()
+{
+    thread_local ryo_ctx* c = nullptr;
+    return c;
+}
+
+inline void check(ryo_ctx* c, int rc)
+{
+    if (rc != RYO_OK) throw std::runtime_error(std::string("ryoanji_amd: ") + (c ? ryo_last_error(c) : "error"));
+}
+
+inline ryo_ctx* ctx()
+{
+    ryo_ctx*& c = threadContext();
+    if (!c) check(nullptr, ryo_ctx_create(&c, nullptr));
+    return c;
+}
+
+//! run this thread's calls on `hipStream` (a hipStream_t)
+inline void bindStream(void* hipStream)
+{
+    ryo_ctx*& c = threadContext();
+    if (c) ryo_ctx_destroy(c), c = nullptr;
+    check(nullptr, ryo_ctx_create(&c, hipStream));
+}
+
+inline void check(int rc) { check(ctx(), rc); }
+} // namespace ryoanji_amd
+
+namespace cstone
+{
+using TreeNodeIndex = int;      // cstone/tree/definitions.h:25
+using LocalIndex    = unsigned; // cstone/tree/definitions.h:27
+
+template<class T>
+using Vec3 = std::array<T, 3>;
+template<class T>
+using Vec4 = std::array<T, 4>;
+
+namespace execution
+{
+//! a HIP stream (cstone/execution.hpp:37-56); the C-ABI context of the calling thread runs the work
+struct Gpu
+{
+    void* stream = nullptr;
+};
+inline constexpr Gpu gpuDefaultStream{};
+} // namespace execution
+
+enum class BoundaryType : uint8_t
+{
+    open     = 0,
+    periodic = 1
+};
+
+//! cstone/sfc/box.hpp:99-214 (cubic boxes)
+template<class T>
+class Box
+{
+public:
+    Box(T lo, T hi, BoundaryType b = BoundaryType::open)
+        : lim_{lo, hi, lo, hi, lo, hi}
+        , b_(b)
+    {
+    }
+    Box(T xmin, T xmax, T ymin, T ymax, T zmin, T zmax, BoundaryType b = BoundaryType::open)
+        : lim_{xmin, xmax, ymin, ymax, zmin, zmax}
+        , b_(b)
+    {
+    }
+    T xmin() const { return lim_[0]; }
+    T xmax() const { return lim_[1]; }
+    T ymin() const { return lim_[2]; }
+    T ymax() const { return lim_[3]; }
+    T zmin() const { return lim_[4]; }
+    T zmax() const { return lim_[5]; }
+    T lx() const { return lim_[1] - lim_[0]; }
+    T ly() const { return lim_[3] - lim_[2]; }
+    T lz() const { return lim_[5] - lim_[4]; }
+    BoundaryType boundaryX() const { return b_; }
+    const float* limits() const
+    {
+        for (int k = 0; k < 6; ++k)
+            f_[k] = float(lim_[k]);
+        return f_;
+    }
+
+private:
+    T lim_[6];
+    BoundaryType b_;
+    mutable float f_[6];
+};
+
+//! cstone/traversal/groups.hpp:28-34
+struct GroupView
+{
+    LocalIndex firstBody, lastBody;
+    LocalIndex numGroups;
+    const LocalIndex* groupStart;
+    const LocalIndex* groupEnd;
+};
+
+//! cstone/tree/octree.hpp:234-260 (device pointers; levelRange on the host, d_levelRange on the device)
+template<class KeyType>
+struct OctreeView
+{
+    TreeNodeIndex numLeafNodes, numInternalNodes, numNodes, numParents;
+    KeyType* prefixes;
+    TreeNodeIndex* childOffsets;
+    TreeNodeIndex* parents;
+    TreeNodeIndex* levelRange;
+    TreeNodeIndex* d_levelRange;
+    TreeNodeIndex* internalToLeaf;
+    TreeNodeIndex* leafToInternal;
+};
+
+//! cstone::computeSfcKeys -- cstone/sfc/sfc_gpu.h:37 (Hilbert keys, 64-bit)
+inline void computeSfcKeys(execution::Gpu, const float* x, const float* y, const float* z, uint64_t* keys, size_t n,
+                           const Box<float>& box)
+{
+    ryoanji_amd::check(ryo_sfc_keys_f32(ryoanji_amd::ctx(), x, y, z, keys, n, box.limits(), 0, 1));
+}
+
+//! cstone::sortByKeyTempStorage -- cstone/primitives/primitives_gpu.h:120-124
+template<class K, class V>
+inline uint64_t sortByKeyTempStorage(uint64_t n)
+{
+    static_assert(sizeof(K) == 8 && sizeof(V) == 4, "(uint64 key, int32 value) pairs");
+    return ryo_sort_temp_bytes(n);
+}
+
+//! cstone::sortByKey -- cstone/primitives/primitives_gpu.h:126-130
+inline void sortByKey(execution::Gpu, uint64_t* first, uint64_t* last, int* values, uint64_t* keyBuf, int* valueBuf,
+                      void* tmp, uint64_t tmpBytes)
+{
+    ryoanji_amd::check(ryo_sort_pairs_u64_i32(ryoanji_amd::ctx(), first, values, size_t(last - first), keyBuf,
+                                              valueBuf, tmp, tmpBytes));
+}
+
+//! cstone::sequence -- primitives_gpu.cu:265-273
+inline void sequence(execution::Gpu, int* out, size_t n, int start)
+{
+    ryoanji_amd::check(ryo_sequence_i32(ryoanji_amd::ctx(), out, n, start));
+}
+
+/*! converged cornerstone leaves + counts (cstone::computeOctree / updateOctreeGpu to convergence,
+ *  csarray.hpp:429-440, update_gpu.cuh:43-68) and TreeBuilder's layout; returns the number of leaves */
+inline int computeOctreeGpu(execution::Gpu, const uint64_t* sortedKeys, size_t n, unsigned bucketSize,
+                            uint64_t* leaves, unsigned* counts, LocalIndex* layout, size_t capacity)
+{
+    int L = 0;
+    ryoanji_amd::check(
+        ryo_build_leaves_u64(ryoanji_amd::ctx(), sortedKeys, n, bucketSize, leaves, counts, layout, capacity, &L));
+    return L;
+}
+
+//! cstone::buildOctreeGpu(exec, leaves, OctreeView) -- cstone/tree/octree_gpu.h:35-45
+inline void buildOctreeGpu(execution::Gpu, const uint64_t* leaves, OctreeView<uint64_t> o)
+{
+    ryoanji_amd::check(ryo_link_octree_u64(ryoanji_amd::ctx(), leaves, o.numLeafNodes, o.prefixes, o.childOffsets,
+                                           o.parents, o.d_levelRange, o.internalToLeaf, o.leafToInternal,
+                                           o.levelRange));
+}
+
+//! cstone::computeFixedGroups -- cstone/traversal/groups_gpu.cu:45-58 (groups has numGroups + 1 entries)
+inline GroupView computeFixedGroups(execution::Gpu, LocalIndex first, LocalIndex last, unsigned groupSize,
+                                    LocalIndex* groups)
+{
+    int G = 0;
+    ryoanji_amd::check(ryo_fixed_groups(ryoanji_amd::ctx(), first, last, groupSize, groups, &G));
+    return GroupView{first, last, LocalIndex(G), groups, groups + 1};
+}
+} // namespace cstone
+
+namespace ryoanji
+{
+using cstone::GroupView;
+using cstone::LocalIndex;
+using cstone::TreeNodeIndex;
+using cstone::Vec3;
+using cstone::Vec4;
+
+template<class T>
+using CartesianQuadrupole = std::array<T, 8>; // nbody/cartesian_qpole.hpp:27-51
+
+/*! fused FP64 replacement of computeLeafSourceCenterGpu + upsweepCentersGpu + computeLeafMultipoles +
+ *  upsweepMultipoles + setMacGpu (upsweep_gpu.h:25,40; source_center_gpu.h:66-116) */
+inline void upsweep(const float* x, const float* y, const float* z, const float* m, const LocalIndex* layout,
+                    int numLeaves, const uint64_t* prefixes, const TreeNodeIndex* childOffsets,
+                    const TreeNodeIndex* parents, const TreeNodeIndex* leafToInternal, const cstone::Box<float>& box,
+                    float theta, Vec4<float>* centers, CartesianQuadrupole<float>* multipoles)
+{
+    ryoanji_amd::check(ryo_upsweep_f32(ryoanji_amd::ctx(), x, y, z, m, layout, numLeaves, prefixes, childOffsets,
+                                       parents, leafToInternal, box.limits(), 1.0f / theta,
+                                       reinterpret_cast<float*>(centers), reinterpret_cast<float*>(multipoles),
+                                       nullptr, nullptr));
+}
+
+//! ryoanji::traverse -- nbody/traversal_gpu.h:51-56, float instantiation; gmPool is not needed (LDS stacks)
+inline double traverse(GroupView grp, int initNodeIdx, const float* xt, const float* yt, const float* zt,
+                       const float* mt, const float* ht, const float* xs, const float* ys, const float* zs,
+                       const float* ms, const float* hs, const TreeNodeIndex* childOffsets,
+                       const TreeNodeIndex* internalToLeaf, const LocalIndex* layout, const Vec4<float>* sourceCenter,
+                       const CartesianQuadrupole<float>* multipoles, float G, int numShells, Vec3<float> boxL,
+                       float* p, float* ax, float* ay, float* az, int* /*gmPool*/ = nullptr)
+{
+    double pot = 0;
+    ryoanji_amd::check(ryo_traverse_f32(
+        ryoanji_amd::ctx(), int(grp.numGroups), grp.groupStart, grp.groupEnd, initNodeIdx, xt, yt, zt, mt, ht, xs, ys,
+        zs, ms, hs, childOffsets, internalToLeaf, layout, reinterpret_cast<const float*>(sourceCenter),
+        reinterpret_cast<const float*>(multipoles), G, numShells, boxL.data(), p, ax, ay, az, &pot));
+    return pot;
+}
+
+//! ryoanji::bhMaxTargetSize -- traversal_gpu.cu:643 (one wavefront of targets)
+inline int bhMaxTargetSize() { return 64; }
+
+//! ryoanji::readBhStats -- traversal_gpu.cu:688-700
+inline std::array<uint64_t, 5> readBhStats()
+{
+    std::array<uint64_t, 5> s{};
+    ryoanji_amd::check(ryo_read_stats(ryoanji_amd::ctx(), s.data()));
+    return s;
+}
+
+//! ryoanji::readTraversalRetries -- traversal_gpu.cu:645
+inline uint64_t readTraversalRetries() { return ryo_traversal_retries(ryoanji_amd::ctx()); }
+
+//! EwaldSettings -- nbody/ewald.h:24-31
+struct EwaldSettings
+{
+    int numReplicaShells        = 1;
+    double lCut                 = 2.6;
+    double hCut                 = 2.8;
+    double alpha_scale          = 2.0;
+    double small_R_scale_factor = 3.0e-3;
+};
+
+//! ryoanji::computeGravityEwaldGpu -- interface/ewald.cuh:25-27 (FP64 root multipole and center)
+inline void computeGravityEwaldGpu(const Vec3<double>& rootCenter, const CartesianQuadrupole<double>& Mroot,
+                                   GroupView grp, const float* x, const float* y, const float* z, const float* m,
+                                   const cstone::Box<float>& box, float G, float* ugrav, float* ax, float* ay,
+                                   float* az, double* ugravTot, EwaldSettings s = {})
+{
+    double set[5] = {double(s.numReplicaShells), s.lCut, s.hCut, s.alpha_scale, s.small_R_scale_factor};
+    double pot    = 0;
+    ryoanji_amd::check(ryo_ewald_f32(ryoanji_amd::ctx(), Mroot.data(), rootCenter.data(), grp.firstBody, grp.lastBody,
+                                     x, y, z, m, box.limits(), G, set, ugrav, ax, ay, az, &pot));
+    if (ugravTot) *ugravTot += pot;
+}
+
+/*! ryoanji::TreeBuilder -- interface/treebuilder.cuh:35-127: keys, sort, permutation, leaves, linked tree and
+ *  layout on the device. Unlike the reference (Q1), m and h are permuted together with x, y, z. */
+class TreeBuilder
+{
+public:
+    explicit TreeBuilder(unsigned ncrit)
+        : bucket_(ncrit)
+    {
+    }
+    ~TreeBuilder()
+    {
+        if (sys_) ryo_system_destroy(sys_);
+    }
+    TreeBuilder(const TreeBuilder&) = delete;
+    TreeBuilder& operator=(const TreeBuilder&) = delete;
+
+    //! builds the octree of n bodies (device arrays, rewritten in SFC order); returns the number of tree nodes
+    int update(float* x, float* y, float* z, float* m, float* h, size_t n, const cstone::Box<float>& box)
+    {
+        if (!sys_ || cap_ < n)
+        {
+            if (sys_) ryo_system_destroy(sys_);
+            ryoanji_amd::check(ryo_system_create(ryoanji_amd::ctx(), n, &sys_));
+            cap_ = n;
+        }
+        ryoanji_amd::check(ryo_system_load(sys_, n, x, y, z, m, h, box.limits(), 0));
+        ryo_params p;
+        ryo_default_params(&p);
+        p.bucket_size = bucket_;
+        ryoanji_amd::check(ryo_system_build(sys_, &p));
+        ryoanji_amd::check(ryo_system_store(sys_, x, y, z, m, h));
+        ryoanji_amd::check(ryo_system_info(sys_, info_));
+        ryoanji_amd::check(ryo_system_tree(sys_, tree_));
+        return int(info_[2]);
+    }
+    const LocalIndex* layout() const { return static_cast<const LocalIndex*>(tree_[3]); }
+    const uint64_t* nodeKeys() const { return static_cast<const uint64_t*>(tree_[4]); }
+    const TreeNodeIndex* childOffsets() const { return static_cast<const TreeNodeIndex*>(tree_[5]); }
+    const TreeNodeIndex* parents() const { return static_cast<const TreeNodeIndex*>(tree_[6]); }
+    const TreeNodeIndex* internalToLeaf() const { return static_cast<const TreeNodeIndex*>(tree_[8]); }
+    const TreeNodeIndex* leafToInternal() const
+    {
+        return static_cast<const TreeNodeIndex*>(tree_[9]) + (info_[1] - 1) / 7;
+    }
+    TreeNodeIndex numLeafNodes() const { return TreeNodeIndex(info_[1]); }
+    unsigned maxTreeLevel() const { return 21; }
+    ryo_system* system() const { return sys_; }
+
+private:
+    unsigned bucket_;
+    ryo_system* sys_ = nullptr;
+    size_t cap_      = 0;
+    int64_t info_[4] = {0, 0, 0, 0};
+    void* tree_[14]  = {};
+};
+} // namespace ryoanji

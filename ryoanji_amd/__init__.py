@@ -75,6 +75,7 @@ SIGNATURES = {
     "ryo_system_destroy": (_I, [_P]),
     "ryo_system_arrays": (_I, [_P, _P]),
     "ryo_system_load": (_I, [_P, _SZ, _P, _P, _P, _P, _P, _P, _I]),
+    "ryo_system_store": (_I, [_P, _P, _P, _P, _P, _P]),
     "ryo_system_build": (_I, [_P, C.POINTER(Params)]),
     "ryo_system_gravity": (_I, [_P, C.POINTER(Params), C.POINTER(_D)]),
     "ryo_system_results": (_I, [_P, _P, _P]),

@@ -424,6 +424,18 @@ int ryo_system_load(ryo_system* s, size_t n, const float* x, const float* y, con
     return RYO_OK;
 }
 
+int ryo_system_store(ryo_system* s, float* x, float* y, float* z, float* m, float* h)
+{
+    if (!s) return RYO_ERR_ARG;
+    Context* ctx = s->ctx;
+    float* dst[5] = {x, y, z, m, h};
+    for (int k = 0; k < 5; ++k)
+        if (dst[k])
+            RYO_HIP(ctx, hipMemcpyAsync(dst[k], s->a[s->cur][k], s->n * sizeof(float), hipMemcpyDeviceToDevice,
+                                        ctx->stream));
+    return RYO_OK;
+}
+
 static void mark(ryo_system* s, int i)
 {
     if (s->ctx->timing) hipEventRecord(s->ev[i], s->ctx->stream);

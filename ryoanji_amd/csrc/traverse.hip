@@ -101,7 +101,7 @@ __device__ __forceinline__ bool macClose(float4 c, float tcx, float tcy, float t
 struct WalkOut
 {
     unsigned m2p, p2p, maxStack;
-    bool overflow;
+    bool overflow, corrupt;
 };
 
 // apply all bodies of leaf ranges flagged in `dmask` (lane -> [rb, re)) to the wave's targets
@@ -140,7 +140,7 @@ __device__ WalkOut walk(const TravArgs& a, int* stack, float tx, float ty, float
 {
     const int lane      = threadIdx.x & 63;
     const uint64_t lt   = (1ull << lane) - 1ull;
-    WalkOut o{0, 0, 0, false};
+    WalkOut o{0, 0, 0, false, false};
     int sp = 0;
 
     const int rootChild = a.child[0];
@@ -162,8 +162,16 @@ __device__ WalkOut walk(const TravArgs& a, int* stack, float tx, float ty, float
     if (lane == 0) stack[0] = rootChild;
     sp = 1;
 
+    // a valid tree is walked in at most numNodes/8 + 1 steps; the cap only guards against a corrupt tree
+    int steps = 0;
     while (sp > 0)
     {
+        if (++steps > (1 << 24))
+        {
+            o.overflow = true;
+            o.corrupt  = true;
+            return o;
+        }
         const int k   = sp < 8 ? sp : 8;
         const int e   = lane >> 3;
         const bool ok = e < k;
@@ -231,7 +239,7 @@ __global__ __launch_bounds__(64 * kTravWaves) void traverseKernel(TravArgs a)
 
         Acc acc{0.f, 0.f, 0.f, 0.f};
         unsigned nM2P = 0, nP2P = 0, maxStack = 0;
-        bool overflow = false;
+        bool overflow = false, corrupt = false;
         const int S = a.numShells;
         for (int iz = -S; iz <= S && !overflow; ++iz)
             for (int iy = -S; iy <= S && !overflow; ++iy)
@@ -244,6 +252,7 @@ __global__ __launch_bounds__(64 * kTravWaves) void traverseKernel(TravArgs a)
                     nP2P += o.p2p;
                     maxStack = max(maxStack, o.maxStack);
                     overflow = o.overflow;
+                    corrupt  = o.corrupt;
                 }
 
         if (overflow)
@@ -253,6 +262,7 @@ __global__ __launch_bounds__(64 * kTravWaves) void traverseKernel(TravArgs a)
                 unsigned slot = atomicAdd(reinterpret_cast<unsigned*>(&a.stats[5]), 1u);
                 a.failed[slot] = g;
                 atomicMax(&a.stats[4], (unsigned long long)maxStack);
+                if (corrupt) atomicAdd(&a.stats[6], 1ull);
             }
             continue;
         }
@@ -350,7 +360,7 @@ int traverse(Context* ctx, TravArgs a, double* totalPot)
         if (pass > 0)
         {
             zeroKernelU<<<1, 64, 0, s>>>(work, 1);
-            zeroKernelU<<<1, 64, 0, s>>>(reinterpret_cast<unsigned*>(stats + 5), 2);
+            zeroKernelU<<<1, 64, 0, s>>>(reinterpret_cast<unsigned*>(stats + 5), 4);
         }
         zeroKernelU<<<std::min(iceil(2 * G, 256), 4096u), 256, 0, s>>>(reinterpret_cast<unsigned*>(a.groupPot),
                                                                        int(2 * G));
@@ -373,6 +383,9 @@ int traverse(Context* ctx, TravArgs a, double* totalPot)
         pot += hp; // failed groups left their (zeroed) slot untouched
         unsigned numFailed = unsigned(h[5] & 0xffffffffu);
         if (numFailed == 0) break;
+        unsigned long long corrupt = 0;
+        RYO_HIP(ctx, hipMemcpy(&corrupt, stats + 6, sizeof(corrupt), hipMemcpyDeviceToHost));
+        if (corrupt) return setError(ctx, RYO_ERR_TRAVERSAL, "traversal: step limit exceeded (corrupt tree?)");
         if (pass > 40) return setError(ctx, RYO_ERR_TRAVERSAL, "traversal: overflow not resolved by splitting");
         ctx->retriedGroups += numFailed;
         // splitFailedGroups: halve each failed group along the SFC and traverse the halves again

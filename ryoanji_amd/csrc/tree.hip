@@ -284,9 +284,12 @@ __global__ void unsortedLayoutKernel(const uint64_t* __restrict__ leaves, int L,
             int w          = 0;
             for (unsigned l = 1; l <= level + 1; ++l)
                 w += digitWeight(int(octalDigit(key, l)));
-            int o  = (t + w) / 7;
-            pre[o] = encodePlaceholder(key, pl);
-            idx[o] = o;
+            int o = (t + w) / 7;
+            if (o >= 0 && o < nI)
+            {
+                pre[o] = encodePlaceholder(key, pl);
+                idx[o] = o;
+            }
         }
     }
 }
@@ -295,8 +298,8 @@ __global__ void invertKernel(int32_t* __restrict__ i2l, int32_t* __restrict__ l2
 {
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
     {
-        int u  = i2l[i];
-        l2i[u] = i;
+        int u = i2l[i];
+        if (u >= 0 && u < n) l2i[u] = i;
         i2l[i] = u - nI;
     }
 }
@@ -325,16 +328,18 @@ __global__ void zeroI32Kernel(int32_t* a, int n)
         a[i] = 0;
 }
 
-__global__ void linkKernel(const uint64_t* __restrict__ pre, int nI, const int32_t* __restrict__ l2i,
+__global__ void linkKernel(const uint64_t* __restrict__ pre, int n, int nI, const int32_t* __restrict__ l2i,
                            const int32_t* __restrict__ levelRange, int32_t* __restrict__ child,
                            int32_t* __restrict__ parents)
 {
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < nI; i += gridDim.x * blockDim.x)
     {
-        int a           = l2i[i];
+        int a = l2i[i];
+        if (a < 0 || a >= n) continue;
         uint64_t prefix = pre[a];
         unsigned pl     = prefixLength(prefix);
         unsigned level  = pl / 3;
+        if (level >= kMaxLevel) continue;
         uint64_t cp     = encodePlaceholder(decodePlaceholder(prefix), int(pl) + 3);
         int lo = levelRange[level + 1], hi = levelRange[level + 2];
         int c = lowerBound(pre, lo, hi, cp);
@@ -369,7 +374,7 @@ int linkOctree(Context* ctx, const uint64_t* leaves, int L, uint64_t* prefixes, 
     int np = std::max(1, (n - 1) / 8);
     zeroI32Kernel<<<iceil(np, 256), 256, 0, s>>>(parents, np);
     if (nI > 0) linkKernel<<<std::max(1u, std::min(iceil(nI, 256), unsigned(ctx->numCUs * 8))), 256, 0, s>>>(
-        prefixes, nI, l2i, levelRange, child, parents);
+        prefixes, n, nI, l2i, levelRange, child, parents);
     RYO_LAUNCH_CHECK(ctx);
     if (levelRangeHost)
     {
