@@ -208,8 +208,48 @@ int ryo_system_info(ryo_system* sys, int64_t* info);
  * {keys(u64), leaves(u64), counts(u32), layout(u32), prefixes(u64), childOffsets, parents, levelRange,
  *  internalToLeaf, leafToInternal, centers(f32x4), multipoles(f32x8), centers64, multipoles64} */
 int ryo_system_tree(ryo_system* sys, void** ptrs /* host void*[14] */);
+/* `steps` kick-drift-kick leapfrog steps of size dt (velocities live in the system, zero after load); each step
+ * rebuilds the tree (open systems first refit a cubic box around the particles) and solves gravity; the first
+ * call also solves gravity for the initial positions. *total_pot (host, may be NULL): the last potential. */
+int ryo_system_leapfrog(ryo_system* sys, const ryo_params* params, float dt, int steps, double* total_pot);
+/* the system's current box (host float[6]) */
+int ryo_system_limits(ryo_system* sys, float* limits);
 /* phase times (ms) of the last build+gravity: {keys, sort, permute, leaves, link, upsweep, traverse, ewald} */
 int ryo_system_phase_ms(ryo_system* sys, double* ms);
+
+/* ---------------------------------------------------------------- serving (no reference counterpart) */
+
+/* A served request: a particle system (host or device arrays) that runs `steps` leapfrog steps. */
+typedef struct
+{
+    size_t n;
+    const float *x, *y, *z, *m, *h;
+    float limits[6];
+    int periodic;
+    float theta, G, dt;
+    int steps;
+    unsigned bucket_size; /* 0: 64 */
+    int group_size;       /* 0: 64 */
+    double arrival_ms;    /* relative to the start of ryo_server_run; requests sorted by arrival */
+} ryo_request;
+
+typedef struct
+{
+    double arrival_ms, admit_ms, finish_ms, total_pot;
+    int status, slot;
+} ryo_result;
+
+typedef struct ryo_server ryo_server;
+
+/* `slots` concurrent workers (each: a HIP stream, a context, a resident system of capacity max_n particles);
+ * requests are admitted while their summed HBM footprint stays within hbm_budget bytes (0: unlimited). */
+int ryo_server_create(int slots, size_t hbm_budget, size_t max_n, ryo_server** server);
+int ryo_server_destroy(ryo_server* server);
+/* estimated HBM bytes of a resident system of n particles (the admission weight) */
+size_t ryo_server_footprint(size_t n);
+/* run the request stream to completion (continuous batching); out_state may be NULL, else out_state[i] NULL or a
+ * host buffer of 7*n floats for request i: x, y, z, ax, ay, az and the original index (int32), in SFC order */
+int ryo_server_run(ryo_server* server, int num, const ryo_request* requests, ryo_result* results, float** out_state);
 
 #ifdef __cplusplus
 }

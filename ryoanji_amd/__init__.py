@@ -47,8 +47,37 @@ class Params(C.Structure):
     ]
 
 
+class Request(C.Structure):
+    """ryo_request: one served particle system."""
+
+    _fields_ = [
+        ("n", C.c_size_t),
+        ("x", _P), ("y", _P), ("z", _P), ("m", _P), ("h", _P),
+        ("limits", C.c_float * 6),
+        ("periodic", C.c_int),
+        ("theta", C.c_float), ("G", C.c_float), ("dt", C.c_float),
+        ("steps", C.c_int),
+        ("bucket_size", C.c_uint),
+        ("group_size", C.c_int),
+        ("arrival_ms", C.c_double),
+    ]
+
+
+class Result(C.Structure):
+    """ryo_result: timing and outcome of one served request."""
+
+    _fields_ = [("arrival_ms", C.c_double), ("admit_ms", C.c_double), ("finish_ms", C.c_double),
+                ("total_pot", C.c_double), ("status", C.c_int), ("slot", C.c_int)]
+
+
 # exported symbol -> (restype, argtypes)
 SIGNATURES = {
+    "ryo_system_leapfrog": (_I, [_P, C.POINTER(Params), _F, _I, C.POINTER(_D)]),
+    "ryo_system_limits": (_I, [_P, _P]),
+    "ryo_server_create": (_I, [_I, _SZ, _SZ, C.POINTER(_P)]),
+    "ryo_server_destroy": (_I, [_P]),
+    "ryo_server_footprint": (_SZ, [_SZ]),
+    "ryo_server_run": (_I, [_P, _I, C.POINTER(Request), C.POINTER(Result), _P]),
     "ryo_ctx_create": (_I, [C.POINTER(_P), _P]),
     "ryo_ctx_destroy": (_I, [_P]),
     "ryo_last_error": (C.c_char_p, [_P]),

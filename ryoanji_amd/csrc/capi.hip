@@ -24,6 +24,9 @@ int ewald(Context*, const double*, const double*, uint32_t, uint32_t, const floa
           const float*, const float*, float, const double*, float*, float*, float*, float*, double*);
 int directSum(Context*, const float*, const float*, const float*, const float*, const float*, size_t,
               const int64_t*, size_t, double, const double*, int, double*);
+int kickDrift(Context*, float*, float*, float*, float*, float*, float*, const float*, const float*, const float*,
+              size_t, float, float, int, const float*);
+int fitBox(Context*, const float*, const float*, const float*, size_t, float*);
 
 struct TravArgs;
 int traverseEntry(Context* ctx, int numGroups, const uint32_t* gs, const uint32_t* ge, int initNode, const float* xt,
@@ -286,6 +289,7 @@ struct ryo_system
     size_t groupCap = 0;
     uint32_t* groups = nullptr;
     int levelRangeHost[24] = {};
+    bool haveAcc = false; // accelerations of the current positions are in ax, ay, az
     hipEvent_t ev[9] = {};
     double phaseMs[8] = {};
 };
@@ -413,6 +417,7 @@ int ryo_system_load(ryo_system* s, size_t n, const float* x, const float* y, con
     std::memcpy(s->limits, limits, sizeof(s->limits));
     s->periodic = periodic;
     s->cur      = 0;
+    s->haveAcc  = false;
     const float* src[5] = {x, y, z, m, h};
     for (int k = 0; k < 5; ++k)
         RYO_HIP(ctx, hipMemcpyAsync(s->a[0][k], src[k], n * sizeof(float), hipMemcpyDefault, ctx->stream));
@@ -450,6 +455,7 @@ int ryo_system_build(ryo_system* s, const ryo_params* prm)
     float** A = s->a[s->cur];
     float** B = s->a[s->cur ^ 1];
     int rc;
+    s->haveAcc = false;
     mark(s, 0);
     if ((rc = sfcKeys(ctx, A[0], A[1], A[2], s->keys, n, s->limits, 0, 0))) return rc;
     mark(s, 1);
@@ -549,6 +555,7 @@ int ryo_system_gravity(ryo_system* s, const ryo_params* prm, double* totalPot)
         }
         pot += ep;
     }
+    s->haveAcc = true;
     if (totalPot) *totalPot = pot;
     return RYO_OK;
 }
@@ -578,6 +585,40 @@ int ryo_system_tree(ryo_system* s, void** p)
                    s->i2l,  s->l2i,    s->centers, s->mp,    s->c64,      s->q64};
     for (int k = 0; k < 14; ++k)
         p[k] = v[k];
+    return RYO_OK;
+}
+
+int ryo_system_leapfrog(ryo_system* s, const ryo_params* prm, float dt, int steps, double* totalPot)
+{
+    if (!s || !prm || steps < 0) return RYO_ERR_ARG;
+    Context* ctx = s->ctx;
+    int rc;
+    double pot = 0;
+    if (!s->haveAcc)
+    {
+        if ((rc = ryo_system_build(s, prm)) || (rc = ryo_system_gravity(s, prm, &pot))) return rc;
+    }
+    for (int k = 0; k < steps; ++k)
+    {
+        float** A = s->a[s->cur];
+        if ((rc = kickDrift(ctx, A[0], A[1], A[2], A[5], A[6], A[7], s->ax, s->ay, s->az, s->n, 0.5f * dt, dt,
+                            s->periodic, s->limits)))
+            return rc;
+        if (!s->periodic && (rc = fitBox(ctx, A[0], A[1], A[2], s->n, s->limits))) return rc;
+        if ((rc = ryo_system_build(s, prm)) || (rc = ryo_system_gravity(s, prm, &pot))) return rc;
+        A = s->a[s->cur];
+        if ((rc = kickDrift(ctx, A[0], A[1], A[2], A[5], A[6], A[7], s->ax, s->ay, s->az, s->n, 0.5f * dt, 0.0f,
+                            s->periodic, s->limits)))
+            return rc;
+    }
+    if (totalPot) *totalPot = pot;
+    return RYO_OK;
+}
+
+int ryo_system_limits(ryo_system* s, float* limits)
+{
+    if (!s || !limits) return RYO_ERR_ARG;
+    std::memcpy(limits, s->limits, sizeof(s->limits));
     return RYO_OK;
 }
 

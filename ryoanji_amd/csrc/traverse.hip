@@ -104,6 +104,17 @@ struct WalkOut
     bool overflow, corrupt;
 };
 
+// Wave-uniform loads through the constant address space: the source data of one cell or body is the same
+// for all 64 lanes, so it is fetched with scalar memory instructions into SGPRs (read-only during the kernel).
+typedef float f4u __attribute__((ext_vector_type(4), aligned(4)));
+typedef float f8u __attribute__((ext_vector_type(8), aligned(16)));
+__device__ __forceinline__ f4u uload4(const float* p) { return *((const __attribute__((address_space(4))) f4u*)p); }
+__device__ __forceinline__ f8u uload8(const float* p) { return *((const __attribute__((address_space(4))) f8u*)p); }
+__device__ __forceinline__ float uload1(const float* p)
+{
+    return *((const __attribute__((address_space(4))) float*)p);
+}
+
 // apply all bodies of leaf ranges flagged in `dmask` (lane -> [rb, re)) to the wave's targets
 __device__ __forceinline__ unsigned applyLeaves(const TravArgs& a, uint64_t dmask, uint32_t rb, uint32_t re, Acc& acc,
                                                 float tx, float ty, float tz, float th)
@@ -115,8 +126,20 @@ __device__ __forceinline__ unsigned applyLeaves(const TravArgs& a, uint64_t dmas
         dmask &= dmask - 1;
         uint32_t b = readlane(rb, l), e = readlane(re, l);
         cnt += e - b;
-        for (uint32_t j = b; j < e; ++j)
-            p2p(acc, tx, ty, tz, th, a.xs[j], a.ys[j], a.zs[j], a.ms[j], a.hs[j]);
+        uint32_t j = b;
+        // four bodies per scalar load of each array (dword-aligned 16-byte scalar loads)
+        for (; j + 4 <= e; j += 4)
+        {
+            f4u X = uload4(a.xs + j), Y = uload4(a.ys + j), Z = uload4(a.zs + j), M = uload4(a.ms + j),
+                H = uload4(a.hs + j);
+            p2p(acc, tx, ty, tz, th, X.x, Y.x, Z.x, M.x, H.x);
+            p2p(acc, tx, ty, tz, th, X.y, Y.y, Z.y, M.y, H.y);
+            p2p(acc, tx, ty, tz, th, X.z, Y.z, Z.z, M.z, H.z);
+            p2p(acc, tx, ty, tz, th, X.w, Y.w, Z.w, M.w, H.w);
+        }
+        for (; j < e; ++j)
+            p2p(acc, tx, ty, tz, th, uload1(a.xs + j), uload1(a.ys + j), uload1(a.zs + j), uload1(a.ms + j),
+                uload1(a.hs + j));
     }
     return cnt;
 }
@@ -125,12 +148,17 @@ __device__ __forceinline__ unsigned applyCells(const TravArgs& a, uint64_t amask
                                                float ty, float tz)
 {
     unsigned cnt = __popcll(amask);
+    const float* cbase = reinterpret_cast<const float*>(a.centers);
+    const float* qbase = reinterpret_cast<const float*>(a.mp);
     while (amask)
     {
         int l = __ffsll((unsigned long long)amask) - 1;
         amask &= amask - 1;
-        int c = readlane(cell, l);
-        m2p(acc, tx, ty, tz, a.centers[c], a.mp[2 * c], a.mp[2 * c + 1]);
+        int c  = readlane(cell, l);
+        f4u C  = uload4(cbase + 4 * c);
+        f8u Q  = uload8(qbase + 8 * c);
+        m2p(acc, tx, ty, tz, make_float4(C.x, C.y, C.z, C.w), make_float4(Q.s0, Q.s1, Q.s2, Q.s3),
+            make_float4(Q.s4, Q.s5, Q.s6, Q.s7));
     }
     return cnt;
 }

@@ -1,0 +1,99 @@
+"""Serving: continuous batching gives each request the same result alone and in the mix; leapfrog state stays
+resident on the device; admission respects the HBM budget."""
+
+import numpy as np
+import pytest
+
+import ryoanji_amd as R
+from ryoanji_amd import serving as S
+from ryoanji_amd import workloads as W
+
+from gpu_util import dev, host
+
+pytestmark = pytest.mark.gpu
+
+
+def test_same_result_alone_and_in_the_mix(gpu):
+    systems, params = S.make_stream(10, rate_per_s=0, seed=3, lo_exp=10, hi_exp=15, steps=(1, 4))
+    mixed = S.Server(slots=4, max_n=1 << 15)
+    out_mix = mixed.run(systems, params, keep_state=True)
+    mixed.close()
+    alone = S.Server(slots=1, max_n=1 << 15)
+    out_alone = alone.run(systems, params, keep_state=True)
+    alone.close()
+    assert all(r["status"] == 0 for r in out_mix + out_alone)
+    assert len({r["slot"] for r in out_mix}) > 1  # requests really ran on several slots
+    for a, b in zip(out_alone, out_mix):
+        # identical kernels on identical data: bit-identical states (the stated tolerance is 1e-6 relative)
+        sa, sb = a["state"], b["state"]
+        assert np.array_equal(sa[6].view(np.int32), sb[6].view(np.int32))
+        err = np.abs(sa[:6] - sb[:6]).max() / (np.abs(sa[:6]).max() + 1e-30)
+        assert err <= 1e-6, err
+        assert abs(a["pot"] - b["pot"]) <= 1e-9 * abs(a["pot"])
+
+
+def test_leapfrog_matches_one_solve_per_step(ctx):
+    """the resident leapfrog equals kick-drift-kick done by hand with fresh solves"""
+    s = W.plummer(20000, seed=5)
+    dt = 1e-3
+    sysm = R.System(ctx, s.n)
+    sysm.load(*(dev(a) for a in s.arrays()), s.limits)
+    prm = R.default_params()
+    import ctypes
+
+    pot = ctypes.c_double()
+    ctx._check(R.lib().ryo_system_leapfrog(sysm.h, ctypes.byref(prm), dt, 2, ctypes.byref(pot)))
+    p, ax, ay, az, order = (host(t) for t in sysm.results())
+    x, y, z = (host(t) for t in sysm.arrays()[:3])
+    # by hand: one solve per step from numpy state
+    ref = R.System(ctx, s.n)
+    X = [s.x.astype(np.float32).copy(), s.y.copy(), s.z.copy()]
+    V = [np.zeros(s.n, np.float32) for _ in range(3)]
+    idx = np.arange(s.n)
+
+    def solve(X, lim):
+        ref.load(dev(X[0]), dev(X[1]), dev(X[2]), dev(s.m[idx]), dev(s.h[idx]), lim)
+        ref.build(prm)
+        ref.gravity(prm)
+        _, gx, gy, gz, o = (host(t) for t in ref.results())
+        A = [np.empty(s.n, np.float32) for _ in range(3)]
+        for d, g in enumerate((gx, gy, gz)):
+            A[d][o] = g
+        return A
+
+    A = solve(X, s.limits)
+    for _ in range(2):
+        for d in range(3):
+            V[d] = V[d] + np.float32(0.5 * dt) * A[d]
+            X[d] = X[d] + np.float32(dt) * V[d]
+        mn = np.array([X[d].min() for d in range(3)])
+        mx = np.array([X[d].max() for d in range(3)])
+        A = solve(X, _box(mn, mx))  # a tree on a slightly different box changes forces only at the BH-error level
+        for d in range(3):
+            V[d] = V[d] + np.float32(0.5 * dt) * A[d]
+    for d, got in enumerate((x, y, z)):
+        want = X[d][order]
+        assert np.allclose(got, want, rtol=0, atol=1e-5 * np.abs(want).max())
+
+
+def _box(mn, mx):
+    c = 0.5 * (mn + mx)
+    half = np.float32(0.5 * (mx - mn).max() * (1 + 1e-4) + 1e-30)
+    lo = (c - half).astype(np.float32)
+    L = np.float32(lo[0] + 2 * half) - lo[0]
+    return np.array([lo[0], lo[0] + L, lo[1], lo[1] + L, lo[2], lo[2] + L], np.float32)
+
+
+def test_admission_budget_limits_concurrency(gpu):
+    systems, params = S.make_stream(6, rate_per_s=0, seed=4, lo_exp=14, hi_exp=14.01, steps=(1, 1))
+    fp = R.lib().ryo_server_footprint(systems[0].n)
+    srv = S.Server(slots=4, max_n=1 << 15, hbm_budget=int(2.5 * fp))
+    out = srv.run(systems, params)
+    srv.close()
+    # at most two requests can be resident at once under a budget of 2.5 footprints
+    ev = sorted([(r["admit"], 1) for r in out] + [(r["finish"], -1) for r in out])
+    live = peak = 0
+    for _, d in ev:
+        live += d
+        peak = max(peak, live)
+    assert peak <= 2
