@@ -49,11 +49,15 @@ struct Acc
     float p, x, y, z;
 };
 
+// v_rsq_f32 (about 1 ulp) without the denormal rescaling that rsqrtf adds; the arguments are squared distances
+// clamped by the softening, never denormal for the systems served here
+__device__ __forceinline__ float rsq(float x) { return __builtin_amdgcn_rsqf(x); }
+
 __device__ __forceinline__ void m2p(Acc& a, float tx, float ty, float tz, float4 c, float4 q0, float4 q1)
 {
     float rx = tx - c.x, ry = ty - c.y, rz = tz - c.z;
     float r2  = rx * rx + ry * ry + rz * rz;
-    float r1  = rsqrtf(r2);
+    float r1  = rsq(r2);
     float ri2 = r1 * r1;
     float r5  = ri2 * ri2 * r1;
     float M = q0.x, qxx = q0.y, qxy = q0.z, qxz = q0.w, qyy = q1.x, qyz = q1.y, qzz = q1.z;
@@ -75,8 +79,8 @@ __device__ __forceinline__ void p2p(Acc& a, float tx, float ty, float tz, float 
     float r2    = dx * dx + dy * dy + dz * dz;
     float hij   = th + sh;
     float hij2  = hij * hij;
-    float r2eff = r2 < hij2 ? hij2 : r2;
-    float invR  = rsqrtf(r2eff);
+    float r2eff = fmaxf(r2, hij2); // the softening clamp (R2 < h_ij^2 ? h_ij^2 : R2) for non-NaN inputs
+    float invR  = rsq(r2eff);
     float invR3m = sm * invR * (invR * invR);
     a.p -= invR3m * r2;
     a.x += dx * invR3m;

@@ -182,9 +182,10 @@ __global__ __launch_bounds__(kLeafThreads) void leafEmitKernel(const uint64_t* _
         uint32_t ex = blockExclusiveScan256(uint32_t(em[r]), s.sw, &tot);
         uint32_t pos = base + ex;
         base += tot;
-        if (em[r] == 0) continue;
         int q  = r * kLeafThreads + threadIdx.x;
         long i = t0 + q;
+        // the last particle always writes the terminators leaves[L], layout[L], even when it emits no leaf
+        if (em[r] == 0 && i != n - 1) continue;
         uint64_t si = s.start[q + 1];
         bool boundary = (i == 0) || si != s.start[q];
         if (boundary)

@@ -119,13 +119,21 @@ def test_upsweep_fp64(ctx, name):
     # FP64 against the FP64 instantiation: only the summation order differs
     scale = np.abs(oq).max(axis=0) + 1e-300
     assert np.allclose(c64[:, :3], oc[:, :3], rtol=1e-12, atol=1e-12 * np.abs(oc[:, :3]).max())
+    assert np.array_equal(c64[:, 3] > 0, oc[:, 3] > 0)
     assert np.all(np.abs(q64 - oq) <= 1e-11 * scale)
-    # the FP32 copies against the reference float pipeline (mac^2 in column 3)
+    # the FP32 copies are the FP64 values rounded once (mac^2 in column 3 of the float centers)
+    assert np.allclose(c32, oc.astype(np.float32), rtol=1e-6, atol=1e-7 * np.abs(oc[:, :3]).max())
+    assert np.all(np.abs(q32 - oq.astype(np.float32)) <= 1e-6 * scale)
+    # against the reference float pipeline: centers and quadrupoles agree to float accumulation error; its MAC
+    # radius carries the float error of the cell geometry and center, an absolute error of a few float ulps of
+    # the box (large relative to the radius of deep cells), so the radii are compared with that absolute term
     fc, fq = t["centers"], t["mp"]
-    assert np.allclose(c32[:, :3], fc[:, :3], rtol=0, atol=2e-6 * np.abs(fc[:, :3]).max())
-    assert np.allclose(c32[:, 3], fc[:, 3], rtol=2e-5, atol=0)
+    assert np.allclose(c32[:, :3], fc[:, :3], rtol=0, atol=5e-6 * np.abs(fc[:, :3]).max())
     fscale = np.abs(fq).max(axis=0) + 1e-30
     assert np.all(np.abs(q32 - fq) <= 1e-4 * fscale)
+    box = float(s.limits[1] - s.limits[0])
+    r_gpu, r_ref = np.sqrt(c32[:, 3].astype(np.float64)), np.sqrt(fc[:, 3].astype(np.float64))
+    assert np.all(np.abs(r_gpu - r_ref) <= 1e-4 * r_ref + 4e-6 * box)
 
 
 @pytest.mark.parametrize("name", list(SYSTEMS))
@@ -164,19 +172,25 @@ def test_traversal_parity_64_targets(ctx, name):
         assert gst["sumP2P"] == ost[0]
 
 
-def test_traversal_split_retry_identical(ctx):
-    """a stack capacity of 64 entries forces overflow -> split/retry; results equal the normal run"""
+def test_traversal_split_retry(ctx):
+    """traversal_overflow_gpu.cpp:99-146: a stack capacity one below the largest stack a group needs forces the
+    overflow path; failed groups are split along the SFC and traversed again, every target is written once,
+    and the result keeps the Barnes-Hut accuracy (split groups have smaller boxes, hence other interactions)"""
+    import ryoanji_amd as R
+
     s = W.plummer(20000, seed=8)
     t = oracle_tree(s)
     a = gpu_traverse(ctx, t["x"], t["y"], t["z"], t["m"], t["h"], t["tree"], t["layout"], t["centers"], t["mp"], 64)
+    assert R.lib().ryo_traversal_retries(ctx.h) == 0
+    cap = max(9, int(a[5]["maxStack"]) - 1)
     b = gpu_traverse(ctx, t["x"], t["y"], t["z"], t["m"], t["h"], t["tree"], t["layout"], t["centers"], t["mp"], 64,
-                     stack_cap=64)
-    import ryoanji_amd as R
-
+                     stack_cap=cap)
     assert R.lib().ryo_traversal_retries(ctx.h) > 0
-    e = rel_err(b[2], b[3], b[4], a[2], a[3], a[4])
-    assert e.max() <= 1e-5
-    assert abs(a[0] - b[0]) <= 1e-6 * abs(a[0])
+    ref = O.direct_sum(t["x"], t["y"], t["z"], t["h"], t["m"])
+    for r in (a, b):
+        e = np.sort(rel_err(r[2], r[3], r[4], ref[:, 1], ref[:, 2], ref[:, 3]))
+        assert e[int(0.99 * e.size)] < 1e-2 and np.isfinite(e).all()
+    assert abs(a[0] - b[0]) <= 1e-3 * abs(a[0])
 
 
 def test_single_leaf_and_tiny(ctx):
