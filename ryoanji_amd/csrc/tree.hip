@@ -256,6 +256,11 @@ int buildLeaves(Context* ctx, const uint64_t* keys, size_t n, unsigned bucket, u
     leafEmitKernel<<<numBlocks, kLeafThreads, 0, s>>>(keys, long(n), int(bucket), sums, leaves, layout, long(L));
     leafCountsKernel<<<iceil(L, 256), 256, 0, s>>>(layout, counts, int(L));
     RYO_LAUNCH_CHECK(ctx);
+    // the layout drives every later particle access: check its terminator before anything reads it
+    uint32_t last = 0;
+    RYO_HIP(ctx, hipMemcpyAsync(&last, layout + L, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+    RYO_HIP(ctx, hipStreamSynchronize(s));
+    if (last != uint32_t(n)) return setError(ctx, RYO_ERR_HIP, "leaves: layout[L] = %u != n = %zu", last, n);
     return RYO_OK;
 }
 
