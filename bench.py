@@ -123,13 +123,51 @@ def accuracy(ctx, sysm, s, order, ax, ay, az, pot, num_targets=16384, seed=1):
     arrs = [torch.from_numpy(a[order]).cuda() for a in (s.x, s.y, s.z, s.h, s.m)]
     tgt = np.sort(np.random.default_rng(seed).choice(s.n, min(num_targets, s.n), replace=False)).astype(np.int64)
     out = torch.zeros(tgt.size * 4, dtype=torch.float64, device="cuda")
-    ctx.call("ryo_direct_sum_f64", *(R.ptr(a) for a in arrs), s.n, R.ptr(torch.from_numpy(tgt).cuda()), tgt.size,
-             1.0, None, 0, R.ptr(out))
+    dtgt = torch.from_numpy(tgt).cuda()  # bound to a name while the pointer is in use
+    ctx.call("ryo_direct_sum_f64", *(R.ptr(a) for a in arrs), s.n, R.ptr(dtgt), tgt.size, 1.0, None, 0, R.ptr(out))
     ref = out.cpu().numpy().reshape(-1, 4)
     d = np.sqrt((ax[tgt] - ref[:, 1]) ** 2 + (ay[tgt] - ref[:, 2]) ** 2 + (az[tgt] - ref[:, 3]) ** 2)
     e = np.sort(d / np.sqrt(ref[:, 1] ** 2 + ref[:, 2] ** 2 + ref[:, 3] ** 2))
     return {"targets": int(tgt.size), "err_p1": float(e[int(0.99 * e.size)]), "err_max": float(e[-1]),
             "err_median": float(e[e.size // 2])}
+
+
+def serving_bench(num=32, slots=8, seed=5):
+    """Served mixed-size stream (Plummer, N log-uniform in [2^12, 2^22], theta in {0.4,0.5,0.7}, 1-20 leapfrog
+    steps): continuous batching on `slots` slots vs one-at-a-time plain solves of the same stream. Throughput is
+    measured with all requests queued at t=0; latency percentiles at Poisson arrival rates below saturation."""
+    from ryoanji_amd import serving as S
+
+    systems, params = S.make_stream(num, rate_per_s=0, seed=seed)
+    max_n = int(max(s.n for s in systems))
+    out = {"requests": num, "slots": slots, "n_range": [int(min(s.n for s in systems)), max_n]}
+    plain = S.Server(1, max_n)
+    res1 = plain.run(systems, params)
+    plain.close()
+    mixed = S.Server(slots, max_n, hbm_budget=64 << 30)
+    resS = mixed.run(systems, params)
+    s1, sS = S.summarize(systems, params, res1), S.summarize(systems, params, resS)
+    out["plain_saturated"] = s1
+    out["batched_saturated"] = sS
+    out["throughput_ratio"] = sS["particle_solves_per_s"] / s1["particle_solves_per_s"]
+    # latency under Poisson arrivals at fractions of the plain solver's capacity
+    rates = []
+    cap_rps = s1["requests_per_s"]
+    for frac in (0.5, 2.0):
+        rate = frac * cap_rps
+        _, p = S.make_stream(num, rate_per_s=rate, seed=seed)
+        params["arrival_ms"] = p["arrival_ms"]
+        r = mixed.run(systems, params)
+        rates.append(dict(rate_per_s=rate, **{k: v for k, v in S.summarize(systems, params, r).items()
+                                              if k in ("p50_ms", "p99_ms", "requests_per_s")}))
+        r1 = S.Server(1, max_n)
+        rr = r1.run(systems, params)
+        r1.close()
+        rates[-1]["plain_p50_ms"] = S.summarize(systems, params, rr)["p50_ms"]
+        rates[-1]["plain_p99_ms"] = S.summarize(systems, params, rr)["p99_ms"]
+    mixed.close()
+    out["poisson"] = rates
+    return out
 
 
 def main():
@@ -142,6 +180,9 @@ def main():
     ap.add_argument("--theta", type=float, default=0.5)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-accuracy", action="store_true")
+    ap.add_argument("--no-serving", action="store_true")
+    ap.add_argument("--copy-calibration-mb", type=int, default=0,
+                    help="first run a plain device copy of this many MiB (reference kernel for HBM counters)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: this build has no multi-GPU decomposition (see DESIGN.md, "
@@ -154,6 +195,13 @@ def main():
     import ryoanji_amd as R
 
     torch.cuda.set_device(0)
+    if args.copy_calibration_mb:
+        src = torch.ones(args.copy_calibration_mb << 18, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
     s = make_system(args.workload, args.n)
     spec = WORKLOADS[args.workload]
     ctx = R.Context()
@@ -234,6 +282,10 @@ def main():
     }
     if not args.no_accuracy:
         out["accuracy_vs_fp64_direct"] = accuracy(ctx, sysm, s, order, ax, ay, az, pot)
+    sysm.close()
+    del dev
+    if not args.no_serving:
+        out["serving"] = serving_bench()
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(s)
     print(json.dumps(out))
