@@ -208,14 +208,16 @@ template<class T>
 using CartesianQuadrupole = std::array<T, 8>; // nbody/cartesian_qpole.hpp:27-51
 
 /*! fused FP64 replacement of computeLeafSourceCenterGpu + upsweepCentersGpu + computeLeafMultipoles +
- *  upsweepMultipoles + setMacGpu (upsweep_gpu.h:25,40; source_center_gpu.h:66-116) */
+ *  upsweepMultipoles + setMacGpu (upsweep_gpu.h:25,40; source_center_gpu.h:66-116). leafToInternal is the leaf
+ *  part of the map (numLeaves entries), as TreeBuilder::leafToInternal() and computeLeafMultipoles take it. */
 inline void upsweep(const float* x, const float* y, const float* z, const float* m, const LocalIndex* layout,
                     int numLeaves, const uint64_t* prefixes, const TreeNodeIndex* childOffsets,
                     const TreeNodeIndex* parents, const TreeNodeIndex* leafToInternal, const cstone::Box<float>& box,
                     float theta, Vec4<float>* centers, CartesianQuadrupole<float>* multipoles)
 {
+    const TreeNodeIndex* full = leafToInternal - (numLeaves - 1) / 7;
     ryoanji_amd::check(ryo_upsweep_f32(ryoanji_amd::ctx(), x, y, z, m, layout, numLeaves, prefixes, childOffsets,
-                                       parents, leafToInternal, box.limits(), 1.0f / theta,
+                                       parents, full, box.limits(), 1.0f / theta,
                                        reinterpret_cast<float*>(centers), reinterpret_cast<float*>(multipoles),
                                        nullptr, nullptr));
 }
