@@ -72,7 +72,7 @@ int ryo_sequence_i32(ryo_ctx* ctx, int32_t* out, size_t n, int32_t start);
 
 /* ---------------------------------------------------------------- a4: permutation */
 
-/* out_k[i] = in_k[perm[i]] for k < num_arrays (<= 8) -- the thrust::gather of TreeBuilder::update
+/* out_k[i] = in_k[perm[i]] for k < num_arrays (<= 10) -- the thrust::gather of TreeBuilder::update
  * (ryoanji/interface/treebuilder.cuh:74-79), applied to all arrays at once. `in`/`out` are HOST arrays of
  * device pointers. */
 int ryo_gather_f32(ryo_ctx* ctx, const int32_t* perm, size_t n, int num_arrays, const float* const* in,
@@ -101,18 +101,19 @@ int ryo_link_octree_u64(ryo_ctx* ctx, const uint64_t* leaves, int num_leaves, ui
 
 /* ---------------------------------------------------------------- a8-a12: centers, MAC, multipoles */
 
-/* Fused upsweep in FP64 (one launch, no per-level host loop): leaf mass centers and P2M
+/* Upsweep in FP64 (no host round trip): leaf mass centers and P2M
  * (cstone::computeLeafSourceCenterGpu, ryoanji::computeLeafMultipoles), the |m|-weighted center and M2M
  * combination of internal nodes up to the root (cstone::upsweepCentersGpu, ryoanji::upsweepMultipoles,
  * source_center_gpu.h:66-116, upsweep_gpu.h:25,40) and the vector-MAC radius (cstone::setMacGpu,
  * source_center_gpu.cu:284-310). Particles are SFC-ordered float x,y,z,m; tree arrays from
- * ryo_link_octree_u64 and the leaf layout. Outputs: centers float4[numNodes] {com, mac^2},
+ * ryo_link_octree_u64 (level_range is its device copy) and the leaf layout. Outputs: centers float4[numNodes] {com, mac^2},
  * multipoles float[numNodes][8]; centers64 double[numNodes][4] {com, sum|m|} and multipoles64
  * double[numNodes][8] are optional (NULL: kept internally). limits = host float[6]. */
 int ryo_upsweep_f32(ryo_ctx* ctx, const float* x, const float* y, const float* z, const float* m,
                     const uint32_t* layout, int num_leaves, const uint64_t* prefixes, const int32_t* child_offsets,
-                    const int32_t* parents, const int32_t* leaf_to_internal, const float* limits, float inv_theta,
-                    float* centers, float* multipoles, double* centers64, double* multipoles64);
+                    const int32_t* parents, const int32_t* leaf_to_internal, const int32_t* level_range,
+                    const float* limits, float inv_theta, float* centers, float* multipoles, double* centers64,
+                    double* multipoles64);
 
 /* ---------------------------------------------------------------- a13: target groups */
 

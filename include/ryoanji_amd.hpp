@@ -212,12 +212,13 @@ using CartesianQuadrupole = std::array<T, 8>; // nbody/cartesian_qpole.hpp:27-51
  *  part of the map (numLeaves entries), as TreeBuilder::leafToInternal() and computeLeafMultipoles take it. */
 inline void upsweep(const float* x, const float* y, const float* z, const float* m, const LocalIndex* layout,
                     int numLeaves, const uint64_t* prefixes, const TreeNodeIndex* childOffsets,
-                    const TreeNodeIndex* parents, const TreeNodeIndex* leafToInternal, const cstone::Box<float>& box,
-                    float theta, Vec4<float>* centers, CartesianQuadrupole<float>* multipoles)
+                    const TreeNodeIndex* parents, const TreeNodeIndex* leafToInternal, const TreeNodeIndex* d_levelRange,
+                    const cstone::Box<float>& box, float theta, Vec4<float>* centers,
+                    CartesianQuadrupole<float>* multipoles)
 {
     const TreeNodeIndex* full = leafToInternal - (numLeaves - 1) / 7;
     ryoanji_amd::check(ryo_upsweep_f32(ryoanji_amd::ctx(), x, y, z, m, layout, numLeaves, prefixes, childOffsets,
-                                       parents, full, box.limits(), 1.0f / theta,
+                                       parents, full, d_levelRange, box.limits(), 1.0f / theta,
                                        reinterpret_cast<float*>(centers), reinterpret_cast<float*>(multipoles),
                                        nullptr, nullptr));
 }
@@ -315,6 +316,7 @@ public:
     const TreeNodeIndex* childOffsets() const { return static_cast<const TreeNodeIndex*>(tree_[5]); }
     const TreeNodeIndex* parents() const { return static_cast<const TreeNodeIndex*>(tree_[6]); }
     const TreeNodeIndex* internalToLeaf() const { return static_cast<const TreeNodeIndex*>(tree_[8]); }
+    const TreeNodeIndex* d_levelRange() const { return static_cast<const TreeNodeIndex*>(tree_[7]); }
     const TreeNodeIndex* leafToInternal() const
     {
         return static_cast<const TreeNodeIndex*>(tree_[9]) + (info_[1] - 1) / 7;

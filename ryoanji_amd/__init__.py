@@ -91,7 +91,7 @@ SIGNATURES = {
     "ryo_gather_f32": (_I, [_P, _P, _SZ, _I, _P, _P]),
     "ryo_build_leaves_u64": (_I, [_P, _P, _SZ, _U, _P, _P, _P, _SZ, C.POINTER(_I)]),
     "ryo_link_octree_u64": (_I, [_P, _P, _I, _P, _P, _P, _P, _P, _P, _P]),
-    "ryo_upsweep_f32": (_I, [_P, _P, _P, _P, _P, _P, _I, _P, _P, _P, _P, _P, _F, _P, _P, _P, _P]),
+    "ryo_upsweep_f32": (_I, [_P, _P, _P, _P, _P, _P, _I, _P, _P, _P, _P, _P, _P, _F, _P, _P, _P, _P]),
     "ryo_fixed_groups": (_I, [_P, C.c_uint32, C.c_uint32, _U, _P, C.POINTER(_I)]),
     "ryo_traverse_f32": (_I, [_P, _I, _P, _P, _I] + [_P] * 10 + [_P, _P, _P, _P, _P, _F, _I, _P, _P, _P, _P, _P,
                                                                    _P]),

@@ -18,7 +18,8 @@ int gather(Context*, const int32_t*, size_t, int, const float* const*, float* co
 int buildLeaves(Context*, const uint64_t*, size_t, unsigned, uint64_t*, uint32_t*, uint32_t*, size_t, int*);
 int linkOctree(Context*, const uint64_t*, int, uint64_t*, int32_t*, int32_t*, int32_t*, int32_t*, int32_t*, int32_t*);
 int upsweep(Context*, const float*, const float*, const float*, const float*, const uint32_t*, int, const uint64_t*,
-            const int32_t*, const int32_t*, const int32_t*, const float*, float, float*, float*, double*, double*);
+            const int32_t*, const int32_t*, const int32_t*, const int32_t*, const float*, float, float*, float*,
+            double*, double*);
 int fixedGroups(Context*, uint32_t, uint32_t, unsigned, uint32_t*, int*);
 int ewald(Context*, const double*, const double*, uint32_t, uint32_t, const float*, const float*, const float*,
           const float*, const float*, float, const double*, float*, float*, float*, float*, double*);
@@ -188,12 +189,12 @@ int ryo_link_octree_u64(ryo_ctx* h, const uint64_t* leaves, int num_leaves, uint
 
 int ryo_upsweep_f32(ryo_ctx* h, const float* x, const float* y, const float* z, const float* m,
                     const uint32_t* layout, int num_leaves, const uint64_t* prefixes, const int32_t* child,
-                    const int32_t* parents, const int32_t* l2i, const float* limits, float inv_theta,
-                    float* centers, float* mp, double* c64, double* q64)
+                    const int32_t* parents, const int32_t* l2i, const int32_t* level_range, const float* limits,
+                    float inv_theta, float* centers, float* mp, double* c64, double* q64)
 {
     CTX_OR_FAIL(h);
-    return upsweep(ctx, x, y, z, m, layout, num_leaves, prefixes, child, parents, l2i, limits, inv_theta, centers,
-                   mp, c64, q64);
+    return upsweep(ctx, x, y, z, m, layout, num_leaves, prefixes, child, parents, l2i, level_range, limits,
+                   inv_theta, centers, mp, c64, q64);
 }
 
 int ryo_fixed_groups(ryo_ctx* h, uint32_t first, uint32_t last, unsigned group_size, uint32_t* groups,
@@ -289,7 +290,8 @@ struct ryo_system
     size_t groupCap = 0;
     uint32_t* groups = nullptr;
     int levelRangeHost[24] = {};
-    bool haveAcc = false; // accelerations of the current positions are in ax, ay, az
+    bool haveAcc = false;     // accelerations of the current positions are in ax, ay, az
+    bool integrating = false; // velocities are in use (leapfrog): the build permutes them too
     hipEvent_t ev[9] = {};
     double phaseMs[8] = {};
 };
@@ -418,6 +420,7 @@ int ryo_system_load(ryo_system* s, size_t n, const float* x, const float* y, con
     s->periodic = periodic;
     s->cur      = 0;
     s->haveAcc  = false;
+    s->integrating = false;
     const float* src[5] = {x, y, z, m, h};
     for (int k = 0; k < 5; ++k)
         RYO_HIP(ctx, hipMemcpyAsync(s->a[0][k], src[k], n * sizeof(float), hipMemcpyDefault, ctx->stream));
@@ -462,11 +465,15 @@ int ryo_system_build(ryo_system* s, const ryo_params* prm)
     if ((rc = sequence(ctx, s->perm, n, 0))) return rc;
     if ((rc = sortPairs(ctx, s->keys, s->perm, n, s->keyBuf, s->permBuf, s->sortTmp, s->sortTmpBytes))) return rc;
     mark(s, 2);
-    if ((rc = gather(ctx, s->perm, n, 8, const_cast<const float* const*>(A), B))) return rc;
     {
-        const float* in1[1] = {A[8]};
-        float* out1[1]      = {B[8]};
-        if ((rc = gather(ctx, s->perm, n, 1, in1, out1))) return rc;
+        // x, y, z, m, h and the original index always; velocities only once the system integrates
+        const int idx[9] = {0, 1, 2, 3, 4, 8, 5, 6, 7};
+        const int num    = s->integrating ? 9 : 6;
+        const float* in[9];
+        float* out[9];
+        for (int k = 0; k < num; ++k)
+            in[k] = A[idx[k]], out[k] = B[idx[k]];
+        if ((rc = gather(ctx, s->perm, n, num, in, out))) return rc;
     }
     s->cur ^= 1;
     A = s->a[s->cur];
@@ -490,8 +497,8 @@ int ryo_system_build(ryo_system* s, const ryo_params* prm)
                          nullptr)))
         return rc;
     mark(s, 5);
-    if ((rc = upsweep(ctx, A[0], A[1], A[2], A[3], s->layout, L, s->prefixes, s->child, s->parents, s->l2i, s->limits,
-                      1.0f / prm->theta, s->centers, s->mp, s->c64, s->q64)))
+    if ((rc = upsweep(ctx, A[0], A[1], A[2], A[3], s->layout, L, s->prefixes, s->child, s->parents, s->l2i,
+                      s->levelRange, s->limits, 1.0f / prm->theta, s->centers, s->mp, s->c64, s->q64)))
         return rc;
     mark(s, 6);
     int gsz = prm->group_size > 0 ? prm->group_size : 64;
@@ -594,6 +601,14 @@ int ryo_system_leapfrog(ryo_system* s, const ryo_params* prm, float dt, int step
     Context* ctx = s->ctx;
     int rc;
     double pot = 0;
+    if (!s->integrating)
+    {
+        // velocities are zero after load and were not carried by earlier builds: zero the current copies, then
+        // let every later permutation carry them
+        for (int k = 5; k < 8; ++k)
+            RYO_HIP(ctx, hipMemsetAsync(s->a[s->cur][k], 0, s->n * sizeof(float), ctx->stream));
+        s->integrating = true;
+    }
     if (!s->haveAcc)
     {
         if ((rc = ryo_system_build(s, prm)) || (rc = ryo_system_gravity(s, prm, &pot))) return rc;

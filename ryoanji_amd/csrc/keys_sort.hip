@@ -248,8 +248,8 @@ __global__ void iotaKernel(int32_t* out, size_t n, int32_t start)
 
 struct GatherArgs
 {
-    const float* in[8];
-    float* out[8];
+    const float* in[10];
+    float* out[10];
     int num;
 };
 
@@ -259,7 +259,7 @@ __global__ __launch_bounds__(256) void gatherKernel(const int32_t* __restrict__ 
     {
         int32_t j = perm[i];
 #pragma unroll
-        for (int k = 0; k < 8; ++k)
+        for (int k = 0; k < 10; ++k)
             if (k < a.num) a.out[k][i] = a.in[k][j];
     }
 }
@@ -342,7 +342,7 @@ int sequence(Context* ctx, int32_t* out, size_t n, int32_t start)
 
 int gather(Context* ctx, const int32_t* perm, size_t n, int num, const float* const* in, float* const* out)
 {
-    if (num < 0 || num > 8) return setError(ctx, RYO_ERR_ARG, "gather: at most 8 arrays");
+    if (num < 0 || num > 10) return setError(ctx, RYO_ERR_ARG, "gather: at most 10 arrays");
     if (n == 0 || num == 0) return RYO_OK;
     GatherArgs a{};
     a.num = num;

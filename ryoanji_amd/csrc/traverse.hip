@@ -323,12 +323,15 @@ __global__ __launch_bounds__(64 * kTravWaves) void traverseKernel(TravArgs a)
     }
 }
 
-// fixed-order FP64 sum (Neumaier) of v[0..n) into *out; one block
-__global__ __launch_bounds__(256) void sumKernel(const double* __restrict__ v, int n, double* __restrict__ out)
+// fixed-order FP64 sum (Neumaier) of v[0..n): block b reduces the contiguous chunk b of `chunk` values into out[b]
+// (one block with chunk = n gives the total); the chunking depends only on n, so the result is deterministic
+__global__ __launch_bounds__(256) void sumKernel(const double* __restrict__ v, int n, double* __restrict__ out,
+                                                 int chunk)
 {
     __shared__ double ss[256], sc[256];
     double s = 0, c = 0;
-    for (int i = threadIdx.x; i < n; i += 256)
+    const int lo = blockIdx.x * chunk, hi = min(n, lo + chunk);
+    for (int i = lo + threadIdx.x; i < hi; i += 256)
     {
         double x = v[i], t = s + x;
         c += fabs(s) >= fabs(x) ? (s - t) + x : (x - t) + s;
@@ -348,7 +351,16 @@ __global__ __launch_bounds__(256) void sumKernel(const double* __restrict__ v, i
         }
         __syncthreads();
     }
-    if (threadIdx.x == 0) *out = ss[0] + sc[0];
+    if (threadIdx.x == 0) out[blockIdx.x] = ss[0] + sc[0];
+}
+
+// deterministic two-stage FP64 sum of v[0..n) into *out (partials: scratch of 256 doubles)
+static void sumDeterministic(hipStream_t s, const double* v, int n, double* partials, double* out)
+{
+    const int blocks = 256;
+    const int chunk  = (n + blocks - 1) / blocks;
+    sumKernel<<<blocks, 256, 0, s>>>(v, n, partials, std::max(chunk, 1));
+    sumKernel<<<1, 256, 0, s>>>(partials, blocks, out, blocks);
 }
 
 __global__ void zeroKernelU(unsigned* a, int n)
@@ -385,10 +397,12 @@ int traverse(Context* ctx, TravArgs a, double* totalPot)
     for (int pass = 0;; ++pass)
     {
         size_t G  = size_t(a.numGroups);
-        auto* buf = static_cast<char*>(ctx->getScratch(kSlotTraverse, 12 * G + 256));
+        const size_t partOff = (12 * G + 255) & ~size_t(255);
+        auto* buf            = static_cast<char*>(ctx->getScratch(kSlotTraverse, partOff + 256 * sizeof(double)));
         if (!buf) return RYO_ERR_HIP;
-        a.groupPot = reinterpret_cast<double*>(buf);
-        a.failed   = reinterpret_cast<uint32_t*>(buf + 8 * G);
+        a.groupPot       = reinterpret_cast<double*>(buf);
+        a.failed         = reinterpret_cast<uint32_t*>(buf + 8 * G);
+        double* partials = reinterpret_cast<double*>(buf + partOff);
         if (pass > 0)
         {
             zeroKernelU<<<1, 64, 0, s>>>(work, 1);
@@ -401,7 +415,7 @@ int traverse(Context* ctx, TravArgs a, double* totalPot)
         traverseKernel<<<blocks, 64 * kTravWaves, 0, s>>>(a);
         if (ctx->timing && pass == 0) RYO_HIP(ctx, hipEventRecord(ctx->ev[1], s));
         RYO_LAUNCH_CHECK(ctx);
-        sumKernel<<<1, 256, 0, s>>>(a.groupPot, a.numGroups, potSum);
+        sumDeterministic(s, a.groupPot, a.numGroups, partials, potSum);
         double hp = 0;
         RYO_HIP(ctx, hipMemcpyAsync(h, stats, sizeof(h), hipMemcpyDeviceToHost, s));
         RYO_HIP(ctx, hipMemcpyAsync(&hp, potSum, sizeof(double), hipMemcpyDeviceToHost, s));

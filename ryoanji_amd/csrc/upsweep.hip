@@ -1,9 +1,11 @@
-// Fused FP64 upsweep (gfx950): leaf mass centers and P2M, |m|-weighted centers and M2M of internal nodes up to
-// the root, and the vector-MAC radius, in ONE launch without per-level host loops.
+// FP64 upsweep (gfx950): leaf mass centers and P2M, |m|-weighted centers and M2M of internal nodes up to the
+// root, and the vector-MAC radius.
 //
-// Each leaf is reduced by 8 lanes (particles strided by 8, shuffle reduction). The lane that finishes a node
-// increments its parent's arrival counter (acquire-release at agent scope); the 8th arrival combines the
-// 8 children and climbs on, so every internal node is computed exactly once, after all of its children.
+// Each leaf is reduced by 8 lanes (particles strided by 8, shuffle reduction). Internal nodes follow level by
+// level from the deepest to the root, 8 lanes per node (lane k reads child k, coalesced); each level launch
+// reads its node range from the device copy of the level ranges, so the sequence has no host round trip.
+// (Kernel boundaries order the levels; a single-launch bottom-up climb with device-scope acquire/release
+// atomics was 20x slower here, as every release writes back the L2.)
 //
 // Formulas: addBody/normalizeMass/CombineSourceCenter (cstone/focus/source_center.hpp:46-98),
 // P2M_add/P2M_finalize (ryoanji/nbody/cartesian_qpole.hpp:67-117), addQuadrupole (:187-210),
@@ -86,7 +88,6 @@ struct UpsweepArgs
     float* q32;
     double* c64; // [n][4] {com, sum|m|}
     double* q64; // [n][8]
-    int* arrivals;
     MacBox box;
 };
 
@@ -113,7 +114,8 @@ __device__ inline double red8(double v)
     return v;
 }
 
-__global__ __launch_bounds__(256) void upsweepKernel(UpsweepArgs a)
+// leaves: 8 lanes per leaf reduce its particles (|m|-weighted center, then the quadrupole about it)
+__global__ __launch_bounds__(256) void leafUpsweepKernel(UpsweepArgs a)
 {
     const int leaf = int((size_t(blockIdx.x) * blockDim.x + threadIdx.x) >> 3);
     const int sub  = threadIdx.x & 7;
@@ -157,80 +159,66 @@ __global__ __launch_bounds__(256) void upsweepKernel(UpsweepArgs a)
     q[6] = 3 * q[6] - tr;
     q[2] *= 3, q[3] *= 3, q[5] *= 3;
 
-    if (!ok || sub != 0) return;
-    int node = a.leafNode[leaf];
-    storeNode(a, node, c, q);
-
-    // climb: the last of 8 siblings to arrive computes the parent
-    while (node != 0)
-    {
-        int parent = a.parents[(node - 1) / 8];
-        int old    = __hip_atomic_fetch_add(&a.arrivals[parent], 1, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-        if (old != 7) break;
-        int ch      = a.child[parent];
-        double pc[4] = {0, 0, 0, 0};
-        for (int k = 0; k < 8; ++k)
-        {
-            const double* s = a.c64 + 4 * size_t(ch + k);
-            double w        = fabs(s[3]);
-            pc[0] += w * s[0];
-            pc[1] += w * s[1];
-            pc[2] += w * s[2];
-            pc[3] += w;
-        }
-        double pinv = pc[3] != 0.0 ? 1.0 / pc[3] : 1.0;
-        pc[0] *= pinv, pc[1] *= pinv, pc[2] *= pinv;
-        double pq[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-        for (int k = 0; k < 8; ++k)
-        {
-            const double* s  = a.c64 + 4 * size_t(ch + k);
-            const double* cq = a.q64 + 8 * size_t(ch + k);
-            double rx = pc[0] - s[0], ry = pc[1] - s[1], rz = pc[2] - s[2];
-            double rx2 = rx * rx, ry2 = ry * ry, rz2 = rz * rz;
-            double r2  = (rx2 + ry2 + rz2) * (1.0 / 3.0);
-            double ml  = cq[0] * 3;
-            pq[7] += cq[7] + ml * r2;
-            pq[0] += cq[0];
-            pq[1] += cq[1] + ml * (rx2 - r2);
-            pq[2] += cq[2] + ml * rx * ry;
-            pq[3] += cq[3] + ml * rx * rz;
-            pq[4] += cq[4] + ml * (ry2 - r2);
-            pq[5] += cq[5] + ml * ry * rz;
-            pq[6] += cq[6] + ml * (rz2 - r2);
-        }
-        storeNode(a, parent, pc, pq);
-        node = parent;
-    }
+    if (ok && sub == 0) storeNode(a, a.leafNode[leaf], c, q);
 }
 
-__global__ void zeroIntKernel(int* a, int n)
+// internal nodes of one level: lane k of an 8-lane group holds child k; center and M2M by 8-lane reductions
+__global__ __launch_bounds__(256) void levelUpsweepKernel(UpsweepArgs a, const int32_t* __restrict__ levelRange,
+                                                          int level)
 {
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
-        a[i] = 0;
+    const int first = levelRange[level], last = levelRange[level + 1];
+    const int sub   = threadIdx.x & 7;
+    const int groupsPerGrid = int(gridDim.x * blockDim.x) >> 3;
+    for (int node = first + int((size_t(blockIdx.x) * blockDim.x + threadIdx.x) >> 3); node < last;
+         node += groupsPerGrid)
+    {
+        const int ch = a.child[node]; // uniform over the 8 lanes of the node
+        if (ch == 0) continue;
+        const double* s  = a.c64 + 4 * size_t(ch + sub);
+        const double* cq = a.q64 + 8 * size_t(ch + sub);
+        double sx = s[0], sy = s[1], sz = s[2], w = fabs(s[3]);
+        double pc[4] = {red8(w * sx), red8(w * sy), red8(w * sz), red8(w)};
+        double pinv  = pc[3] != 0.0 ? 1.0 / pc[3] : 1.0;
+        pc[0] *= pinv, pc[1] *= pinv, pc[2] *= pinv;
+        double rx = pc[0] - sx, ry = pc[1] - sy, rz = pc[2] - sz;
+        double rx2 = rx * rx, ry2 = ry * ry, rz2 = rz * rz;
+        double r2  = (rx2 + ry2 + rz2) * (1.0 / 3.0);
+        double ml  = cq[0] * 3;
+        double pq[8];
+        pq[0] = red8(cq[0]);
+        pq[1] = red8(cq[1] + ml * (rx2 - r2));
+        pq[2] = red8(cq[2] + ml * rx * ry);
+        pq[3] = red8(cq[3] + ml * rx * rz);
+        pq[4] = red8(cq[4] + ml * (ry2 - r2));
+        pq[5] = red8(cq[5] + ml * ry * rz);
+        pq[6] = red8(cq[6] + ml * (rz2 - r2));
+        pq[7] = red8(cq[7] + ml * r2);
+        if (sub == 0) storeNode(a, node, pc, pq);
+    }
 }
 
 int upsweep(Context* ctx, const float* x, const float* y, const float* z, const float* m, const uint32_t* layout,
             int L, const uint64_t* prefixes, const int32_t* child, const int32_t* parents, const int32_t* l2i,
-            const float* limits, float invTheta, float* centers, float* mpoles, double* c64, double* q64)
+            const int32_t* levelRange, const float* limits, float invTheta, float* centers, float* mpoles, double* c64,
+            double* q64)
 {
     if (L < 1) return setError(ctx, RYO_ERR_ARG, "upsweep: need at least one leaf");
+    if (!levelRange) return setError(ctx, RYO_ERR_ARG, "upsweep: level ranges are required");
     int nI = (L - 1) / 7, n = L + nI;
-    size_t need = 0;
+    size_t need = 256;
     if (!c64) need += size_t(n) * 4 * sizeof(double);
     if (!q64) need += size_t(n) * 8 * sizeof(double);
-    need += size_t(n) * sizeof(int) + 256;
     auto* buf = static_cast<char*>(ctx->getScratch(kSlotUpsweep, need));
     if (!buf) return RYO_ERR_HIP;
     size_t off = 0;
     if (!c64) c64 = reinterpret_cast<double*>(buf + off), off += size_t(n) * 4 * sizeof(double);
     if (!q64) q64 = reinterpret_cast<double*>(buf + off), off += size_t(n) * 8 * sizeof(double);
-    int* arrivals = reinterpret_cast<int*>(buf + off);
 
     UpsweepArgs a;
     a.x = x, a.y = y, a.z = z, a.m = m;
     a.layout = layout, a.numLeaves = L;
     a.leafNode = l2i + nI, a.parents = parents, a.child = child, a.prefixes = prefixes;
-    a.c32 = reinterpret_cast<float4*>(centers), a.q32 = mpoles, a.c64 = c64, a.q64 = q64, a.arrivals = arrivals;
+    a.c32 = reinterpret_cast<float4*>(centers), a.q32 = mpoles, a.c64 = c64, a.q64 = q64;
     for (int d = 0; d < 3; ++d)
     {
         double lo = limits[2 * d], hi = limits[2 * d + 1];
@@ -239,8 +227,13 @@ int upsweep(Context* ctx, const float* x, const float* y, const float* z, const 
     }
     a.box.invTheta = invTheta;
     hipStream_t s = ctx->stream;
-    zeroIntKernel<<<std::min(iceil(n, 256), unsigned(ctx->numCUs * 8)), 256, 0, s>>>(arrivals, n);
-    upsweepKernel<<<iceil(size_t(L) * 8, 256), 256, 0, s>>>(a);
+    leafUpsweepKernel<<<iceil(size_t(L) * 8, 256), 256, 0, s>>>(a);
+    // levels from the deepest possible parent level to the root; each launch reads its node range on the device,
+    // so the sequence needs no host copy of the level ranges (and can be captured in a graph)
+    const unsigned levelBlocks = std::max(1u, std::min(iceil(size_t(nI) * 8, 256), unsigned(ctx->numCUs * 4)));
+    if (nI > 0)
+        for (int l = int(kMaxLevel) - 1; l >= 0; --l)
+            levelUpsweepKernel<<<levelBlocks, 256, 0, s>>>(a, levelRange, l);
     RYO_LAUNCH_CHECK(ctx);
     return RYO_OK;
 }

@@ -61,7 +61,7 @@ int main(int argc, char** argv)
     auto* centers    = devAlloc<Vec4<float>>(numNodes);
     auto* multipoles = devAlloc<CartesianQuadrupole<float>>(numNodes);
     upsweep(dx, dy, dz, dm, tb.layout(), tb.numLeafNodes(), tb.nodeKeys(), tb.childOffsets(), tb.parents(),
-            tb.leafToInternal(), box, theta, centers, multipoles);
+            tb.leafToInternal(), tb.d_levelRange(), box, theta, centers, multipoles);
 
     auto* groups = devAlloc<LocalIndex>(n / bhMaxTargetSize() + 2);
     GroupView grp = cstone::computeFixedGroups(cstone::execution::gpuDefaultStream, 0, LocalIndex(n),

@@ -97,13 +97,14 @@ def gpu_upsweep(ctx, x, y, z, m, layout, tree, limits, inv_theta):
 
     L = tree["numLeaves"] if "numLeaves" in tree else layout.size - 1
     n = L + (L - 1) // 7
-    d = {k: dev(tree[k]) for k in ("prefixes", "childOffsets", "parents", "leafToInternal")}
+    d = {k: dev(tree[k]) for k in ("prefixes", "childOffsets", "parents", "leafToInternal", "levelRange")}
     xs = [dev(a) for a in (x, y, z, m)]
     lay = dev(layout)
     c32, q32 = zeros(4 * n, torch.float32), zeros(8 * n, torch.float32)
     c64, q64 = zeros(4 * n, torch.float64), zeros(8 * n, torch.float64)
     ctx.call("ryo_upsweep_f32", *(R.ptr(a) for a in xs), R.ptr(lay), L, R.ptr(d["prefixes"]),
-             R.ptr(d["childOffsets"]), R.ptr(d["parents"]), R.ptr(d["leafToInternal"]), R.host_f32(limits),
+             R.ptr(d["childOffsets"]), R.ptr(d["parents"]), R.ptr(d["leafToInternal"]), R.ptr(d["levelRange"]),
+             R.host_f32(limits),
              float(inv_theta), R.ptr(c32), R.ptr(q32), R.ptr(c64), R.ptr(q64))
     return (host(c32).reshape(n, 4), host(q32).reshape(n, 8), host(c64).reshape(n, 4), host(q64).reshape(n, 8))
 
