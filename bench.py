@@ -185,8 +185,8 @@ def main():
                     help="first run a plain device copy of this many MiB (reference kernel for HBM counters)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
-        print("bench.py measures one GPU: this build has no multi-GPU decomposition (see DESIGN.md, "
-              "'How the path would shard'); run with --gpus 1", file=sys.stderr)
+        print("bench.py measures one GPU only: the library has no multi-GPU decomposition of the gravity pass; "
+              "run with --gpus 1", file=sys.stderr)
         sys.exit(2)
 
     import numpy as np

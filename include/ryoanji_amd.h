@@ -42,6 +42,7 @@ typedef struct ryo_ctx ryo_ctx;
 /* Replaces the reference's process-global device state (traversal_gpu.cu:351-356, csarray_gpu.cu:68,130,
  * ewald.cu:23) with per-context state. `stream` is a hipStream_t (NULL: the null stream). */
 int ryo_ctx_create(ryo_ctx** ctx, void* stream);
+/* Destroy every system created on a context before the context itself (systems use its stream and scratch). */
 int ryo_ctx_destroy(ryo_ctx* ctx);
 const char* ryo_last_error(const ryo_ctx* ctx);
 int ryo_ctx_sync(ryo_ctx* ctx);

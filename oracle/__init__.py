@@ -59,6 +59,8 @@ def lib(which: str = "orc") -> C.CDLL:
         sig.update({
             "compute_gravity_f32": (C.c_double, [_P, C.c_int, _P, _P, _P, _P, _P, _P, _P, _P, _P, C.c_float,
                                                  C.c_int, C.c_int, _P, _P, _P, _P, _P, C.c_int, _P]),
+            "compute_gravity_f64": (C.c_double, [_P, C.c_int, _P, _P, _P, _P, _P, _P, _P, _P, _P, C.c_double,
+                                                 C.c_int, C.c_int, _P, _P, _P, _P]),
             "direct_sum_f64": (C.c_int, [_P, _P, _P, _P, _P, C.c_size_t, _P, C.c_size_t, C.c_double, _P,
                                          C.c_int, _P]),
             "direct_sum_f32": (C.c_int, [_P, _P, _P, _P, _P, C.c_size_t, _P, C.c_size_t, C.c_double, _P,
@@ -203,6 +205,22 @@ def compute_gravity(tree_leaves, layout, centers, mp, x, y, z, h, m, limits, G=1
                                               *(_ptr(a) for a in arrs), _ptr(lim), float(G), num_shells, fl, ll,
                                               _ptr(ug), _ptr(ax), _ptr(ay), _ptr(az))
     return tot, ug, ax, ay, az, stats
+
+
+def compute_gravity_f64(tree_leaves, layout, centers, mp, x, y, z, h, m, limits, G=1.0, num_shells=0,
+                        group_size=16):
+    """The oracle traversal evaluated in double on the same (float) inputs -> (pot_total, ugrav, ax, ay, az)."""
+    leaves = _c(tree_leaves, np.uint64)
+    layout = _c(layout, np.uint32)
+    L = leaves.size - 1
+    arrs = [_c(a, np.float32) for a in (x, y, z, h, m)]
+    n = arrs[0].size
+    ug, ax, ay, az = (np.zeros(n, np.float64) for _ in range(4))
+    tot = _f("orc", "compute_gravity_f64")(_ptr(leaves), L, _ptr(layout), _ptr(_c(centers, np.float32)),
+                                          _ptr(_c(mp, np.float32)), *(_ptr(a) for a in arrs),
+                                          _ptr(_c(limits, np.float32)), float(G), num_shells, group_size, _ptr(ug),
+                                          _ptr(ax), _ptr(ay), _ptr(az))
+    return tot, ug, ax, ay, az
 
 
 def direct_sum(x, y, z, h, m, targets=None, G=1.0, box_l=None, num_shells=0, precision="f64"):

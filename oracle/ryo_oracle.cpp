@@ -567,13 +567,13 @@ inline void P2P(T* acc, const T* pi, T xj, T yj, T zj, T mj, T hi, T hj)
     acc[3] += dz * invR3m;
 }
 
-// cartesian_qpole.hpp:152-176
-template<class T>
+// cartesian_qpole.hpp:152-176 (Exact: a correctly rounded reciprocal square root, for error measurements)
+template<class T, bool Exact = false>
 inline void M2P(T* acc, const T* t, const T* c, const T* q)
 {
     T rx = t[0] - c[0], ry = t[1] - c[1], rz = t[2] - c[2];
     T r2  = rx * rx + (ry * ry + rz * rz);
-    T r1  = rsqrtFast(r2);
+    T r1  = Exact ? rsqrtExact(r2) : rsqrtFast(r2);
     T rm2 = r1 * r1;
     T rm5 = rm2 * rm2 * r1;
     T Qrx = rx * q[1] + ry * q[2] + rz * q[3];
@@ -611,7 +611,7 @@ struct TravStats
 
 /* ryoanji/src/ryoanji/nbody/traversal_cpu.hpp:67-141 with cstone traversal.hpp:26-69 (DFS from the root).
  * `nt` valid targets; the group bounding box uses the valid targets only. */
-template<class T>
+template<class T, bool Exact = false>
 void gravityGroup(const T (*tgt)[4], int nt, const Octree& o, const T* centers, const T* mp, const unsigned* layout,
                   const T* x, const T* y, const T* z, const T* h, const T* m, T (*acc)[4], TravStats* st)
 {
@@ -640,7 +640,7 @@ void gravityGroup(const T (*tgt)[4], int nt, const Octree& o, const T* centers, 
         if (!close || remote)
         {
             for (int k = 0; k < nt; ++k)
-                M2P(acc[k], tgt[k], com, mp + 8 * size_t(idx));
+                M2P<T, Exact>(acc[k], tgt[k], com, mp + 8 * size_t(idx));
             if (st) st->sumM2P += nt;
         }
         return close;
@@ -651,7 +651,7 @@ void gravityGroup(const T (*tgt)[4], int nt, const Octree& o, const T* centers, 
         unsigned b = layout[leaf], e = layout[leaf + 1];
         for (int k = 0; k < nt; ++k)
             for (unsigned s = b; s < e; ++s)
-                P2P<false>(acc[k], tgt[k], x[s], y[s], z[s], m[s], tgt[k][3], h[s]);
+                P2P<Exact>(acc[k], tgt[k], x[s], y[s], z[s], m[s], tgt[k][3], h[s]);
         if (st) st->sumP2P += uint64_t(nt) * (e - b);
     };
 
@@ -692,7 +692,7 @@ void gravityGroup(const T (*tgt)[4], int nt, const Octree& o, const T* centers, 
 
 /* traversal_cpu.hpp:172-230: groups of `groupSize` consecutive targets (the reference uses 16), all
  * (2s+1)^3 images, outputs added to ugrav/ax/ay/az, returns 0.5 * sum(G m phi). */
-template<class T>
+template<class T, bool Exact = false>
 double computeGravity(const Octree& o, const T* centers, const T* mp, const unsigned* layout, const T* x, const T* y,
                       const T* z, const T* h, const T* m, const Box<T>& box, float G, int numShells, int groupSize,
                       T* ugrav, T* ax, T* ay, T* az, const int* groupList, int numGroupList, TravStats* stats)
@@ -728,7 +728,7 @@ double computeGravity(const Octree& o, const T* centers, const T* mp, const unsi
                             shifted[k][d] = tgt[k][d] - sh[d];
                         shifted[k][3] = tgt[k][3] - T(0);
                     }
-                    gravityGroup<T>(shifted, nt, o, centers, mp, layout, x, y, z, h, m, acc, &st);
+                    gravityGroup<T, Exact>(shifted, nt, o, centers, mp, layout, x, y, z, h, m, acc, &st);
                 }
         for (int k = 0; k < nt; ++k)
         {
@@ -1029,6 +1029,26 @@ double orc_compute_gravity_f32(const uint64_t* leaves, int numLeaves, const unsi
                                        ugrav, ax, ay, az, groupList, numGroupList, &st);
     if (stats) stats[0] = st.sumP2P, stats[1] = st.sumM2P;
     return pot;
+}
+
+/* the same traversal evaluated in double with correctly rounded reciprocal square roots (inputs widened from
+ * float): the float path's rounding error is measured against it on the same tree, grouping and MAC inputs */
+double orc_compute_gravity_f64(const uint64_t* leaves, int numLeaves, const unsigned* layout, const float* centers,
+                               const float* mpoles, const float* x, const float* y, const float* z, const float* h,
+                               const float* m, const float* limits, double G, int numShells, int groupSize,
+                               double* ugrav, double* ax, double* ay, double* az)
+{
+    Octree o     = rebuild(leaves, numLeaves);
+    double lim[6];
+    for (int k = 0; k < 6; ++k)
+        lim[k] = limits[k];
+    auto box = makeBox<double>(lim, nullptr);
+    size_t n = layout[numLeaves], nn = size_t(o.numNodes);
+    std::vector<double> c(centers, centers + 4 * nn), q(mpoles, mpoles + 8 * nn), xd(x, x + n), yd(y, y + n),
+        zd(z, z + n), hd(h, h + n), md(m, m + n);
+    return computeGravity<double, true>(o, c.data(), q.data(), layout, xd.data(), yd.data(), zd.data(), hd.data(),
+                                  md.data(), box, float(G), numShells, groupSize, ugrav, ax, ay, az, nullptr, 0,
+                                  nullptr);
 }
 
 /* FP64 direct sum over all n sources for the listed targets (or all, if targets == NULL);

@@ -12,6 +12,7 @@ from __future__ import annotations
 
 import ctypes as C
 import os
+import weakref
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libryoanji_amd.so")
@@ -176,6 +177,7 @@ class Context:
         if stream is None:
             stream = torch.cuda.current_stream().cuda_stream
         self.h = _P()
+        self._systems = weakref.WeakSet()  # destroyed before the context (they use its stream and scratch)
         self._check(lib().ryo_ctx_create(C.byref(self.h), _P(stream)))
 
     def _check(self, rc):
@@ -189,6 +191,8 @@ class Context:
 
     def close(self):
         if self.h:
+            for s in list(self._systems):
+                s.close()
             lib().ryo_ctx_destroy(self.h)
             self.h = _P()
 
@@ -210,7 +214,10 @@ class System:
     def __init__(self, ctx: Context, capacity: int):
         self.ctx = ctx
         self.h = _P()
+        if not ctx.h:
+            raise RyoError(RYO_ERR_ARG, "system: the context is closed")
         ctx._check(lib().ryo_system_create(ctx.h, capacity, C.byref(self.h)))
+        ctx._systems.add(self)
         self.capacity = capacity
 
     def _check(self, rc):

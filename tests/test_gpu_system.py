@@ -126,9 +126,18 @@ def test_periodic_traversal_and_system(ctx):
     # the traversal alone, identical float inputs
     g = gpu_traverse(ctx, t["x"], t["y"], t["z"], t["m"], t["h"], t["tree"], t["layout"], t["centers"], t["mp"], 64,
                      num_shells=1, box_l=[1.0, 1.0, 1.0])
-    rms = np.sqrt(np.mean(oax.astype(np.float64) ** 2 + oay ** 2 + oaz ** 2))
+    # 27 float image sums: both float paths are measured against the same traversal evaluated in double; the
+    # GPU must be no noisier than the oracle's own float evaluation (p50/p99 within 1.5x of it)
+    _, _, dax, day, daz = O.compute_gravity_f64(t["leaves"], t["layout"], t["centers"], t["mp"], *args, s.limits,
+                                                group_size=64, num_shells=1)
+    rms = np.sqrt(np.mean(dax ** 2 + day ** 2 + daz ** 2))
+    d_orc = np.sqrt((oax - dax) ** 2 + (oay - day) ** 2 + (oaz - daz) ** 2) / rms
+    d_gpu = np.sqrt((g[2] - dax) ** 2 + (g[3] - day) ** 2 + (g[4] - daz) ** 2) / rms
+    for q in (0.5, 0.99):
+        assert np.quantile(d_gpu, q) <= 1.5 * np.quantile(d_orc, q), (q, np.quantile(d_gpu, q),
+                                                                      np.quantile(d_orc, q))
     d = np.sqrt((g[2] - oax) ** 2 + (g[3] - oay) ** 2 + (g[4] - oaz) ** 2) / rms
-    assert np.quantile(d, 0.99) <= 1e-5, np.quantile(d, 0.99)
+    assert np.quantile(d, 0.99) <= 2e-5, np.quantile(d, 0.99)
     assert abs(g[0] - otot) <= 1e-5 * abs(otot)
     # the full periodic system (FP64 upsweep, traversal with images, FP64 Ewald)
     sysm, pot, p, ax, ay, az, order = _solve(ctx, s, periodic=True, num_shells=1, ewald=1)
@@ -139,4 +148,5 @@ def test_periodic_traversal_and_system(ctx):
     rms = np.sqrt(np.mean(rx ** 2 + ry ** 2 + rz ** 2))
     d = np.sqrt((ax - rx) ** 2 + (ay - ry) ** 2 + (az - rz) ** 2) / rms
     assert np.quantile(d, 0.99) <= 1e-4, np.quantile(d, 0.99)
-    assert abs(pot - (otot + etot)) <= 1e-5 * abs(otot + etot)
+    # the tree (-10.45) and Ewald (+10.45) energies cancel to ~1e-3: bound the error by the terms' magnitude
+    assert abs(pot - (otot + etot)) <= 1e-6 * (abs(otot) + abs(etot))

@@ -220,6 +220,28 @@ def test_gravity_accuracy_vs_fp64_direct():
     assert abs(tot - 0.5 * ref[:, 0].sum()) / abs(0.5 * ref[:, 0].sum()) < 1e-2
 
 
+@pytest.mark.parametrize("shells", [0, 1])
+def test_double_traversal_opening_every_cell_is_the_direct_sum(shells):
+    """with a MAC radius far beyond the box every cell is opened, so the double-evaluated traversal reduces to
+    the FP64 direct sum over all sources (and their images); the float traversal stays close to it."""
+    s = W.uniform_cube(2048, seed=11, lo=0.0, hi=1.0, periodic=bool(shells))
+    keys = O.sfc_keys(s.x, s.y, s.z, s.limits)
+    sk, perm = O.sort_by_key(keys)
+    x, y, z, m, h = (a[perm] for a in s.arrays())
+    leaves, counts = O.compute_octree(sk, 32)
+    lay = O.layout_from_counts(counts)
+    c, q = O.upsweep(x, y, z, m, leaves, lay, s.limits, 1e4)
+    tot, ug, ax, ay, az = O.compute_gravity_f64(leaves, lay, c, q, x, y, z, h, m, s.limits, num_shells=shells)
+    ref = O.direct_sum(x, y, z, h, m, box_l=[1.0, 1.0, 1.0] if shells else None, num_shells=shells)
+    scale = np.sqrt(np.mean(np.sum(ref[:, 1:] ** 2, axis=1)))
+    for k, a in enumerate((ax, ay, az)):
+        assert np.abs(a - ref[:, k + 1]).max() <= 1e-11 * scale
+    assert abs(tot - 0.5 * ref[:, 0].sum()) <= 1e-11 * abs(0.5 * ref[:, 0].sum())
+    ftot, _, fax, fay, faz, st = O.compute_gravity(leaves, lay, c, q, x, y, z, h, m, s.limits, num_shells=shells)
+    assert st[1] == 0 and st[0] == 2048 * 2048 * (3 ** (3 * shells))
+    assert np.abs(fax - ax).max() <= 1e-4 * scale  # 2048 x 27 float terms per target
+
+
 # ---------------------------------------------------------------- the reference's own outputs
 
 @pytest.mark.parametrize("path", GOLDEN, ids=[os.path.basename(p) for p in GOLDEN])
