@@ -196,10 +196,11 @@ def main():
 
     torch.cuda.set_device(0)
     if args.copy_calibration_mb:
+        # an elementwise kernel that reads and writes exactly copy_calibration_mb MiB each
         src = torch.ones(args.copy_calibration_mb << 18, dtype=torch.float32, device="cuda")
         dst = torch.empty_like(src)
         for _ in range(3):
-            dst.copy_(src)
+            torch.mul(src, 2.0, out=dst)
         torch.cuda.synchronize()
         del src, dst
     s = make_system(args.workload, args.n)
