@@ -181,6 +181,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-accuracy", action="store_true")
     ap.add_argument("--no-serving", action="store_true")
+    ap.add_argument("--trav-stack", type=int, default=0, help="traversal LIFO entries per wave (0: default)")
+    ap.add_argument("--trav-blocks", type=int, default=0, help="traversal blocks per CU (0: default)")
     ap.add_argument("--copy-calibration-mb", type=int, default=0,
                     help="first run a plain device copy of this many MiB (reference kernel for HBM counters)")
     args = ap.parse_args()
@@ -207,6 +209,7 @@ def main():
     spec = WORKLOADS[args.workload]
     ctx = R.Context()
     R.lib().ryo_ctx_set_timing(ctx.h, 1)
+    ctx.call("ryo_ctx_set_traversal_shape", args.trav_stack, args.trav_blocks)
     sysm = R.System(ctx, s.n)
     dev = [torch.from_numpy(a).cuda() for a in s.arrays()]
     prm = R.default_params(theta=args.theta, group_size=64, bucket_size=64)
@@ -261,6 +264,7 @@ def main():
                         f"softening 1e-3, 64-target groups" + (", periodic 1 shell + Ewald" if spec["periodic"] else
                                                                ", open boundaries"),
             "n": s.n, "theta": args.theta, "bucket": 64, "group_size": 64,
+            "traversal_shape": {"stack_per_wave": args.trav_stack or "default", "blocks_per_cu": args.trav_blocks or "default"},
         },
         "roofline": {
             "bound": "valu",

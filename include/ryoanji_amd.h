@@ -49,6 +49,10 @@ int ryo_ctx_sync(ryo_ctx* ctx);
 /* enable HIP-event timing of the traversal kernel (ryo_last_traverse_ms) */
 int ryo_ctx_set_timing(ryo_ctx* ctx, int enable);
 double ryo_last_traverse_ms(const ryo_ctx* ctx);
+/* Launch shape of the traversal on this context: LIFO entries per wave held in LDS (9..1408; a walk that needs
+ * more takes the split/retry path) and blocks of 4 waves launched per CU; 0 keeps the default (1408, 6).
+ * Replaces the compile-time TravConfig (traversal_gpu.cu:45-64), for the autotuner. */
+int ryo_ctx_set_traversal_shape(ryo_ctx* ctx, int stack_entries_per_wave, int blocks_per_cu);
 
 /* ---------------------------------------------------------------- a2: SFC keys */
 

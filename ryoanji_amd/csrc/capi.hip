@@ -35,7 +35,7 @@ int traverseEntry(Context* ctx, int numGroups, const uint32_t* gs, const uint32_
                   const float* ys, const float* zs, const float* ms, const float* hs, const int32_t* child,
                   const int32_t* i2l, const uint32_t* layout, const float* centers, const float* mp, float G,
                   int numShells, const float* boxL, float* p, float* ax, float* ay, float* az, double* totalPot,
-                  int stackCap);
+                  int stackCap, size_t hCount);
 
 int setError(Context* ctx, int code, const char* fmt, ...)
 {
@@ -138,6 +138,17 @@ int ryo_ctx_set_timing(ryo_ctx* h, int enable)
     return RYO_OK;
 }
 
+int ryo_ctx_set_traversal_shape(ryo_ctx* h, int stack_entries_per_wave, int blocks_per_cu)
+{
+    CTX_OR_FAIL(h);
+    if (stack_entries_per_wave < 0 || (stack_entries_per_wave > 0 && stack_entries_per_wave < 9) ||
+        stack_entries_per_wave > 22 * 64 || blocks_per_cu < 0 || blocks_per_cu > 64)
+        return setError(ctx, RYO_ERR_ARG, "traversal shape: stack entries in [9, 1408], blocks per CU in [1, 64]");
+    ctx->travStack       = stack_entries_per_wave;
+    ctx->travBlocksPerCU = blocks_per_cu;
+    return RYO_OK;
+}
+
 double ryo_last_traverse_ms(const ryo_ctx* h) { return h ? h->c.lastTraverseMs : 0.0; }
 
 int ryo_sfc_keys_f32(ryo_ctx* h, const float* x, const float* y, const float* z, uint64_t* keys, size_t n,
@@ -216,7 +227,7 @@ int ryo_traverse_f32(ryo_ctx* h, int num_groups, const uint32_t* gs, const uint3
     if (init_node != 0 && init_node != 1) return setError(ctx, RYO_ERR_ARG, "traverse: init_node must be 0 or 1");
     if (num_shells < 0 || (num_shells > 0 && !box_l)) return setError(ctx, RYO_ERR_ARG, "traverse: bad shells");
     return traverseEntry(ctx, num_groups, gs, ge, init_node, xt, yt, zt, mt, ht, xs, ys, zs, ms, hs, child, i2l,
-                         layout, centers, mp, G, num_shells, box_l, p, ax, ay, az, total_pot, 0);
+                         layout, centers, mp, G, num_shells, box_l, p, ax, ay, az, total_pot, 0, 0);
 }
 
 /* test hook: the traversal with a reduced per-wave stack capacity (forces the split/retry path) */
@@ -229,7 +240,7 @@ int ryo_traverse_f32_stackcap(ryo_ctx* h, int stack_cap, int num_groups, const u
 {
     CTX_OR_FAIL(h);
     return traverseEntry(ctx, num_groups, gs, ge, init_node, xt, yt, zt, mt, ht, xs, ys, zs, ms, hs, child, i2l,
-                         layout, centers, mp, G, num_shells, box_l, p, ax, ay, az, total_pot, stack_cap);
+                         layout, centers, mp, G, num_shells, box_l, p, ax, ay, az, total_pot, stack_cap, 0);
 }
 
 int ryo_read_stats(ryo_ctx* h, uint64_t* stats)
@@ -537,7 +548,7 @@ int ryo_system_gravity(ryo_system* s, const ryo_params* prm, double* totalPot)
     double pot    = 0;
     rc = traverseEntry(ctx, s->G, s->groups, s->groups + 1, prm->init_node, A[0], A[1], A[2], A[3], A[4], A[0], A[1],
                        A[2], A[3], A[4], s->child, s->i2l, s->layout, s->centers, s->mp, prm->G, shells, boxL, s->p,
-                       s->ax, s->ay, s->az, &pot, 0);
+                       s->ax, s->ay, s->az, &pot, 0, s->n);
     if (rc) return rc;
     s->phaseMs[6] = ctx->lastTraverseMs;
     s->phaseMs[7] = 0;

@@ -131,6 +131,10 @@ struct Context
     unsigned long long hStats[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     uint64_t retriedGroups = 0;
 
+    // traversal launch shape: LIFO entries per wave (LDS) and resident blocks per CU (0: defaults)
+    int travStack       = 0;
+    int travBlocksPerCU = 0;
+
     // event timing of the last traversal kernel launch (ms), when enabled
     bool timing          = false;
     hipEvent_t ev[2]     = {nullptr, nullptr};

@@ -17,8 +17,16 @@
 namespace ryo
 {
 
-constexpr int kTravWaves = 4;              // waves per block
-constexpr int kStackCap  = 22 * 64;        // LIFO entries per wave (>= 21 levels x 64)
+constexpr int kTravWaves     = 4;       // waves per block
+constexpr int kMaxStack      = 22 * 64; // LIFO entries per wave that can never overflow (>= 21 levels x 64)
+constexpr int kDefaultStack  = kMaxStack; // default LIFO entries per wave (a smaller one may take the retry path)
+constexpr int kDefaultBlocks = 6;         // default blocks per CU: 6 waves per SIMD (the VGPR budget below)
+constexpr int kXcds          = 8;       // MI355X: 8 XCDs, workgroups dispatched round-robin over them
+
+// device control words (ctx->dStats, 32 x u64): u32 [0..7] per-XCD group counters, u64 [8..14] statistics
+// {sumP2P, maxP2P, sumM2P, maxM2P, maxStack, failed, corrupt}, u64 [16] FP64 potential sum, u32 [40..41]
+// softening range {min, max} as order-preserving integers
+constexpr int kCtlHRange = 40;
 
 struct TravArgs
 {
@@ -38,10 +46,12 @@ struct TravArgs
     float boxL[3];
     float *p, *ax, *ay, *az;
     double* groupPot;
-    unsigned* work;
+    unsigned* work;            // kXcds group counters
+    const unsigned* hRange;    // softening range of targets and sources (order-preserving integers)
     unsigned long long* stats;
     uint32_t* failed;
     int stackCap;
+    size_t hCount; // targets and sources are h[0..hCount) (resident systems), 0 when unknown
 };
 
 struct Acc
@@ -53,6 +63,7 @@ struct Acc
 // clamped by the softening, never denormal for the systems served here
 __device__ __forceinline__ float rsq(float x) { return __builtin_amdgcn_rsqf(x); }
 
+// quadrupole M2P (cartesian_qpole.hpp:152-176), with the products folded into fused multiply-adds
 __device__ __forceinline__ void m2p(Acc& a, float tx, float ty, float tz, float4 c, float4 q0, float4 q1)
 {
     float rx = tx - c.x, ry = ty - c.y, rz = tz - c.z;
@@ -65,22 +76,33 @@ __device__ __forceinline__ void m2p(Acc& a, float tx, float ty, float tz, float4
     float Qry  = rx * qxy + ry * qyy + rz * qyz;
     float Qrz  = rx * qxz + ry * qyz + rz * qzz;
     float rQr  = rx * Qrx + ry * Qry + rz * Qrz;
-    float mono = (-2.5f * rQr * r5 - M * r1) * ri2;
-    a.p -= M * r1 + 0.5f * r5 * rQr;
-    a.x += r5 * Qrx + mono * rx;
-    a.y += r5 * Qry + mono * ry;
-    a.z += r5 * Qrz + mono * rz;
+    float u    = M * r1;
+    float t    = rQr * r5;
+    float mono = __builtin_fmaf(-2.5f, t, -u) * ri2;
+    a.p = __builtin_fmaf(-0.5f, t, a.p - u);
+    a.x = __builtin_fmaf(r5, Qrx, __builtin_fmaf(mono, rx, a.x));
+    a.y = __builtin_fmaf(r5, Qry, __builtin_fmaf(mono, ry, a.y));
+    a.z = __builtin_fmaf(r5, Qrz, __builtin_fmaf(mono, rz, a.z));
 }
 
-__device__ __forceinline__ void p2p(Acc& a, float tx, float ty, float tz, float th, float sx, float sy, float sz,
-                                    float sm, float sh)
+// softened P2P (kernel.hpp:99-120). UniformH: every target and source has the same softening h, so
+// hij2 = (h + h)^2 is the value the general path computes for each pair.
+template<bool UniformH>
+__device__ __forceinline__ void p2p(Acc& a, float tx, float ty, float tz, float th, float hij2u, float sx, float sy,
+                                    float sz, float sm, float sh)
 {
     float dx = sx - tx, dy = sy - ty, dz = sz - tz;
-    float r2    = dx * dx + dy * dy + dz * dz;
-    float hij   = th + sh;
-    float hij2  = hij * hij;
-    float r2eff = fmaxf(r2, hij2); // the softening clamp (R2 < h_ij^2 ? h_ij^2 : R2) for non-NaN inputs
-    float invR  = rsq(r2eff);
+    float r2 = dx * dx + dy * dy + dz * dz;
+    float hij2;
+    if constexpr (UniformH)
+        hij2 = hij2u;
+    else
+    {
+        float hij = th + sh;
+        hij2      = hij * hij;
+    }
+    float r2eff  = fmaxf(r2, hij2); // the softening clamp (R2 < h_ij^2 ? h_ij^2 : R2) for non-NaN inputs
+    float invR   = rsq(r2eff);
     float invR3m = sm * invR * (invR * invR);
     a.p -= invR3m * r2;
     a.x += dx * invR3m;
@@ -119,9 +141,15 @@ __device__ __forceinline__ float uload1(const float* p)
     return *((const __attribute__((address_space(4))) float*)p);
 }
 
+struct Target
+{
+    float x, y, z, h, hij2u;
+};
+
 // apply all bodies of leaf ranges flagged in `dmask` (lane -> [rb, re)) to the wave's targets
+template<bool UH>
 __device__ __forceinline__ unsigned applyLeaves(const TravArgs& a, uint64_t dmask, uint32_t rb, uint32_t re, Acc& acc,
-                                                float tx, float ty, float tz, float th)
+                                                const Target& t)
 {
     unsigned cnt = 0;
     while (dmask)
@@ -134,22 +162,22 @@ __device__ __forceinline__ unsigned applyLeaves(const TravArgs& a, uint64_t dmas
         // four bodies per scalar load of each array (dword-aligned 16-byte scalar loads)
         for (; j + 4 <= e; j += 4)
         {
-            f4u X = uload4(a.xs + j), Y = uload4(a.ys + j), Z = uload4(a.zs + j), M = uload4(a.ms + j),
-                H = uload4(a.hs + j);
-            p2p(acc, tx, ty, tz, th, X.x, Y.x, Z.x, M.x, H.x);
-            p2p(acc, tx, ty, tz, th, X.y, Y.y, Z.y, M.y, H.y);
-            p2p(acc, tx, ty, tz, th, X.z, Y.z, Z.z, M.z, H.z);
-            p2p(acc, tx, ty, tz, th, X.w, Y.w, Z.w, M.w, H.w);
+            f4u X = uload4(a.xs + j), Y = uload4(a.ys + j), Z = uload4(a.zs + j), M = uload4(a.ms + j);
+            f4u H = UH ? f4u{0.f, 0.f, 0.f, 0.f} : uload4(a.hs + j);
+            p2p<UH>(acc, t.x, t.y, t.z, t.h, t.hij2u, X.x, Y.x, Z.x, M.x, H.x);
+            p2p<UH>(acc, t.x, t.y, t.z, t.h, t.hij2u, X.y, Y.y, Z.y, M.y, H.y);
+            p2p<UH>(acc, t.x, t.y, t.z, t.h, t.hij2u, X.z, Y.z, Z.z, M.z, H.z);
+            p2p<UH>(acc, t.x, t.y, t.z, t.h, t.hij2u, X.w, Y.w, Z.w, M.w, H.w);
         }
         for (; j < e; ++j)
-            p2p(acc, tx, ty, tz, th, uload1(a.xs + j), uload1(a.ys + j), uload1(a.zs + j), uload1(a.ms + j),
-                uload1(a.hs + j));
+            p2p<UH>(acc, t.x, t.y, t.z, t.h, t.hij2u, uload1(a.xs + j), uload1(a.ys + j), uload1(a.zs + j),
+                    uload1(a.ms + j), UH ? 0.f : uload1(a.hs + j));
     }
     return cnt;
 }
 
-__device__ __forceinline__ unsigned applyCells(const TravArgs& a, uint64_t amask, int cell, Acc& acc, float tx,
-                                               float ty, float tz)
+// apply the quadrupoles of the cells flagged in `amask` (lane -> cell) in ascending lane order
+__device__ __forceinline__ unsigned applyCells(const TravArgs& a, uint64_t amask, int cell, Acc& acc, const Target& t)
 {
     unsigned cnt = __popcll(amask);
     const float* cbase = reinterpret_cast<const float*>(a.centers);
@@ -158,20 +186,21 @@ __device__ __forceinline__ unsigned applyCells(const TravArgs& a, uint64_t amask
     {
         int l = __ffsll((unsigned long long)amask) - 1;
         amask &= amask - 1;
-        int c  = readlane(cell, l);
-        f4u C  = uload4(cbase + 4 * c);
-        f8u Q  = uload8(qbase + 8 * c);
-        m2p(acc, tx, ty, tz, make_float4(C.x, C.y, C.z, C.w), make_float4(Q.s0, Q.s1, Q.s2, Q.s3),
+        int c = readlane(cell, l);
+        f4u C = uload4(cbase + 4 * c);
+        f8u Q = uload8(qbase + 8 * c);
+        m2p(acc, t.x, t.y, t.z, make_float4(C.x, C.y, C.z, C.w), make_float4(Q.s0, Q.s1, Q.s2, Q.s3),
             make_float4(Q.s4, Q.s5, Q.s6, Q.s7));
     }
     return cnt;
 }
 
-__device__ WalkOut walk(const TravArgs& a, int* stack, float tx, float ty, float tz, float th, float tcx, float tcy,
-                        float tcz, float tsx, float tsy, float tsz, Acc& acc)
+template<bool UH>
+__device__ WalkOut walk(const TravArgs& a, int* stack, const Target& t, float tcx, float tcy, float tcz, float tsx,
+                        float tsy, float tsz, Acc& acc)
 {
-    const int lane      = threadIdx.x & 63;
-    const uint64_t lt   = (1ull << lane) - 1ull;
+    const int lane    = threadIdx.x & 63;
+    const uint64_t lt = (1ull << lane) - 1ull;
     WalkOut o{0, 0, 0, false, false};
     int sp = 0;
 
@@ -182,12 +211,12 @@ __device__ WalkOut walk(const TravArgs& a, int* stack, float tx, float ty, float
         bool close = macClose(c, tcx, tcy, tcz, tsx, tsy, tsz);
         if (!close)
         {
-            if (c.w != 0.0f) o.m2p += applyCells(a, 1ull, 0, acc, tx, ty, tz);
+            if (c.w != 0.0f) o.m2p += applyCells(a, 1ull, 0, acc, t);
             return o;
         }
         if (rootChild == 0)
         {
-            o.p2p += applyLeaves(a, 1ull, a.layout[0], a.layout[1], acc, tx, ty, tz, th);
+            o.p2p += applyLeaves<UH>(a, 1ull, a.layout[0], a.layout[1], acc, t);
             return o;
         }
     }
@@ -239,33 +268,105 @@ __device__ WalkOut walk(const TravArgs& a, int* stack, float tx, float ty, float
         sp += cnt;
         o.maxStack = max(o.maxStack, unsigned(sp));
 
-        o.m2p += applyCells(a, __ballot(approx), cell, acc, tx, ty, tz);
-        o.p2p += applyLeaves(a, __ballot(direct), rb, re, acc, tx, ty, tz, th);
+        o.m2p += applyCells(a, __ballot(approx), cell, acc, t);
+        o.p2p += applyLeaves<UH>(a, __ballot(direct), rb, re, acc, t);
     }
     return o;
 }
 
-__global__ __launch_bounds__(64 * kTravWaves) void traverseKernel(TravArgs a)
+__device__ __forceinline__ unsigned floatToOrdered(float f)
 {
-    __shared__ int stackMem[kTravWaves][kStackCap];
+    unsigned b = __float_as_uint(f);
+    return (b & 0x80000000u) ? ~b : (b | 0x80000000u);
+}
+__device__ __forceinline__ float orderedToFloat(unsigned o)
+{
+    return __uint_as_float((o & 0x80000000u) ? (o & 0x7fffffffu) : ~o);
+}
+
+// min and max softening over targets and sources (order-preserving integers; out = {~0u, 0} on entry)
+__global__ __launch_bounds__(256) void hRangeKernel(const float* __restrict__ h1, size_t n1,
+                                                    const float* __restrict__ h2, size_t n2, unsigned* out)
+{
+    unsigned lo = ~0u, hi = 0;
+    const size_t stride = size_t(gridDim.x) * blockDim.x;
+    for (size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n1 + n2; i += stride)
+    {
+        unsigned o = floatToOrdered(i < n1 ? h1[i] : h2[i - n1]);
+        lo = min(lo, o), hi = max(hi, o);
+    }
+    for (int off = 32; off > 0; off >>= 1)
+    {
+        lo = min(lo, unsigned(__shfl_xor(int(lo), off)));
+        hi = max(hi, unsigned(__shfl_xor(int(hi), off)));
+    }
+    if ((threadIdx.x & 63) == 0)
+    {
+        atomicMin(out, lo);
+        atomicMax(out + 1, hi);
+    }
+}
+
+// reset the control words: group counters, statistics and potential sum to 0, softening range to {~0u, 0}
+__global__ void travInitKernel(unsigned* ctl)
+{
+    int i = threadIdx.x;
+    if (i < 64) ctl[i] = (i == kCtlHRange) ? ~0u : 0u;
+}
+
+// 64 * kTravWaves threads, at least 6 waves per SIMD (<= 80 VGPRs; more waves would spill)
+__global__ __launch_bounds__(64 * kTravWaves, 6) void traverseKernel(TravArgs a)
+{
+    extern __shared__ int stackMem[];
     const int w    = threadIdx.x >> 6;
     const int lane = threadIdx.x & 63;
-    int* stack     = stackMem[w];
+    int* stack     = stackMem + w * a.stackCap;
 
+    // uniform softening: one hij^2 for every pair (the same value the general path computes)
+    const unsigned hlo = a.hRange[0], hhi = a.hRange[1];
+    const bool uniformH = hlo == hhi;
+    float hij = orderedToFloat(hlo);
+    hij       = hij + hij;
+    const float hij2u = hij * hij;
+
+    // XCD-aware work distribution: groups are split into kXcds contiguous ranges and the blocks of an XCD
+    // (blockIdx % kXcds under round-robin dispatch) take groups from their own range first, so the groups in
+    // flight on one XCD are SFC neighbours sharing source cells in that XCD's L2; exhausted ranges are then
+    // helped from in order
+    const int home = blockIdx.x % kXcds;
+    int src        = 0;
     for (;;)
     {
-        unsigned g = 0;
-        if (lane == 0) g = atomicAdd(a.work, 1u);
+        unsigned g = ~0u;
+        if (lane == 0)
+        {
+            while (src < kXcds)
+            {
+                const int x       = (home + src) % kXcds;
+                const unsigned lo = unsigned((unsigned long long)a.numGroups * x / kXcds);
+                const unsigned hi = unsigned((unsigned long long)a.numGroups * (x + 1) / kXcds);
+                if (lo < hi)
+                {
+                    const unsigned i = atomicAdd(a.work + x, 1u);
+                    if (i < hi - lo)
+                    {
+                        g = lo + i;
+                        break;
+                    }
+                }
+                ++src;
+            }
+        }
         g = __shfl(g, 0);
-        if (g >= unsigned(a.numGroups)) return;
+        if (g == ~0u) return;
 
         const uint32_t b = a.gStart[g], e = a.gEnd[g];
         const bool valid = b + lane < e;
         const uint32_t ti = valid ? b + lane : e - 1;
-        float tx = a.xt[ti], ty = a.yt[ti], tz = a.zt[ti], th = a.ht[ti];
+        Target t{a.xt[ti], a.yt[ti], a.zt[ti], a.ht[ti], hij2u};
 
-        float mnx = waveMin(tx), mny = waveMin(ty), mnz = waveMin(tz);
-        float mxx = waveMax(tx), mxy = waveMax(ty), mxz = waveMax(tz);
+        float mnx = waveMin(t.x), mny = waveMin(t.y), mnz = waveMin(t.z);
+        float mxx = waveMax(t.x), mxy = waveMax(t.y), mxz = waveMax(t.z);
         float tcx = (mxx + mnx) * 0.5f, tcy = (mxy + mny) * 0.5f, tcz = (mxz + mnz) * 0.5f;
         float tsx = (mxx - mnx) * 0.5f, tsy = (mxy - mny) * 0.5f, tsz = (mxz - mnz) * 0.5f;
 
@@ -278,8 +379,9 @@ __global__ __launch_bounds__(64 * kTravWaves) void traverseKernel(TravArgs a)
                 for (int ix = -S; ix <= S && !overflow; ++ix)
                 {
                     float sx = ix * a.boxL[0], sy = iy * a.boxL[1], sz = iz * a.boxL[2];
-                    WalkOut o = walk(a, stack, tx + sx, ty + sy, tz + sz, th, tcx + sx, tcy + sy, tcz + sz, tsx, tsy,
-                                     tsz, acc);
+                    Target ts{t.x + sx, t.y + sy, t.z + sz, t.h, t.hij2u};
+                    WalkOut o = uniformH ? walk<true>(a, stack, ts, tcx + sx, tcy + sy, tcz + sz, tsx, tsy, tsz, acc)
+                                         : walk<false>(a, stack, ts, tcx + sx, tcy + sy, tcz + sz, tsx, tsy, tsz, acc);
                     nM2P += o.m2p;
                     nP2P += o.p2p;
                     maxStack = max(maxStack, o.maxStack);
@@ -380,23 +482,34 @@ int traverse(Context* ctx, TravArgs a, double* totalPot)
         if (totalPot) *totalPot = 0;
         return RYO_OK;
     }
-    if (a.stackCap <= 8 || a.stackCap > kStackCap) a.stackCap = kStackCap;
-    // ctx->dStats: [0] work counter (u32), [8..13] stats u64 {sumP2P, maxP2P, sumM2P, maxM2P, maxStack, failed},
-    // [16] FP64 potential sum
-    unsigned* work            = reinterpret_cast<unsigned*>(ctx->dStats);
+    // LIFO capacity per wave (LDS) and resident blocks per CU: the context's launch shape, optionally reduced
+    // by the caller's stack capacity (the overflow path splits groups and retries them)
+    const int ldsStack = ctx->travStack > 8 ? std::min(ctx->travStack, kMaxStack) : kDefaultStack;
+    a.stackCap         = (a.stackCap > 8 && a.stackCap < ldsStack) ? a.stackCap : ldsStack;
+    const int perCU    = ctx->travBlocksPerCU > 0 ? ctx->travBlocksPerCU : kDefaultBlocks;
+    const size_t lds   = size_t(kTravWaves) * ldsStack * sizeof(int);
+
+    unsigned* ctl             = reinterpret_cast<unsigned*>(ctx->dStats);
     unsigned long long* stats = ctx->dStats + 8;
     double* potSum            = reinterpret_cast<double*>(ctx->dStats + 16);
-    a.work  = work;
-    a.stats = stats;
-    zeroKernelU<<<1, 64, 0, s>>>(work, 64);
+    a.work   = ctl;
+    a.stats  = stats;
+    a.hRange = ctl + kCtlHRange;
+    travInitKernel<<<1, 64, 0, s>>>(ctl);
+    // the softening range over all targets and sources selects the uniform-softening P2P; without the particle
+    // count (hCount = 0) the range stays empty and the general P2P runs
+    if (a.hCount > 0)
+        hRangeKernel<<<std::min(iceil(a.hCount, 256), 2048u), 256, 0, s>>>(a.ht, a.hCount,
+                                                                            a.hs == a.ht ? nullptr : a.hs,
+                                                                            a.hs == a.ht ? 0 : a.hCount,
+                                                                            ctl + kCtlHRange);
 
-    const int maxBlocks = ctx->numCUs * 8;
-    double pot          = 0;
+    double pot              = 0;
     unsigned long long h[6] = {0, 0, 0, 0, 0, 0};
     std::vector<uint32_t> ns, ne;
     for (int pass = 0;; ++pass)
     {
-        size_t G  = size_t(a.numGroups);
+        size_t G             = size_t(a.numGroups);
         const size_t partOff = (12 * G + 255) & ~size_t(255);
         auto* buf            = static_cast<char*>(ctx->getScratch(kSlotTraverse, partOff + 256 * sizeof(double)));
         if (!buf) return RYO_ERR_HIP;
@@ -405,14 +518,14 @@ int traverse(Context* ctx, TravArgs a, double* totalPot)
         double* partials = reinterpret_cast<double*>(buf + partOff);
         if (pass > 0)
         {
-            zeroKernelU<<<1, 64, 0, s>>>(work, 1);
+            zeroKernelU<<<1, 64, 0, s>>>(ctl, kXcds);
             zeroKernelU<<<1, 64, 0, s>>>(reinterpret_cast<unsigned*>(stats + 5), 4);
         }
         zeroKernelU<<<std::min(iceil(2 * G, 256), 4096u), 256, 0, s>>>(reinterpret_cast<unsigned*>(a.groupPot),
                                                                        int(2 * G));
-        int blocks = std::min(maxBlocks, int(iceil(G, kTravWaves)));
+        int blocks = std::min(ctx->numCUs * perCU, int(iceil(G, kTravWaves)));
         if (ctx->timing && pass == 0) RYO_HIP(ctx, hipEventRecord(ctx->ev[0], s));
-        traverseKernel<<<blocks, 64 * kTravWaves, 0, s>>>(a);
+        traverseKernel<<<blocks, 64 * kTravWaves, lds, s>>>(a);
         if (ctx->timing && pass == 0) RYO_HIP(ctx, hipEventRecord(ctx->ev[1], s));
         RYO_LAUNCH_CHECK(ctx);
         sumDeterministic(s, a.groupPot, a.numGroups, partials, potSum);
@@ -469,7 +582,7 @@ int traverseEntry(Context* ctx, int numGroups, const uint32_t* gs, const uint32_
                   const float* ys, const float* zs, const float* ms, const float* hs, const int32_t* child,
                   const int32_t* i2l, const uint32_t* layout, const float* centers, const float* mp, float G,
                   int numShells, const float* boxL, float* p, float* ax, float* ay, float* az, double* totalPot,
-                  int stackCap)
+                  int stackCap, size_t hCount)
 {
     if (numGroups > 0 && (!gs || !ge || !xt || !yt || !zt || !mt || !ht || !xs || !ys || !zs || !ms || !hs ||
                           !child || !i2l || !layout || !centers || !mp || !ax || !ay || !az))
@@ -487,6 +600,7 @@ int traverseEntry(Context* ctx, int numGroups, const uint32_t* gs, const uint32_
         a.boxL[d] = boxL ? boxL[d] : 0.0f;
     a.p = p, a.ax = ax, a.ay = ay, a.az = az;
     a.stackCap = stackCap;
+    a.hCount   = hCount;
     return traverse(ctx, a, totalPot);
 }
 
