@@ -50,9 +50,11 @@ int ryo_ctx_sync(ryo_ctx* ctx);
 int ryo_ctx_set_timing(ryo_ctx* ctx, int enable);
 double ryo_last_traverse_ms(const ryo_ctx* ctx);
 /* Launch shape of the traversal on this context: LIFO entries per wave held in LDS (9..1408; a walk that needs
- * more takes the split/retry path) and blocks of 4 waves launched per CU; 0 keeps the default (1408, 6).
+ * more takes the split/retry path), blocks of 4 waves launched per CU, and the number of contiguous group
+ * ranges with their own work counters (1..8; blocks start on range blockIdx % ranges, which under round-robin
+ * dispatch keeps each XCD's L2 on SFC-neighbouring groups). 0 keeps the default (1408, 6, 8).
  * Replaces the compile-time TravConfig (traversal_gpu.cu:45-64), for the autotuner. */
-int ryo_ctx_set_traversal_shape(ryo_ctx* ctx, int stack_entries_per_wave, int blocks_per_cu);
+int ryo_ctx_set_traversal_shape(ryo_ctx* ctx, int stack_entries_per_wave, int blocks_per_cu, int group_ranges);
 
 /* ---------------------------------------------------------------- a2: SFC keys */
 

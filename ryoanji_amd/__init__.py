@@ -84,7 +84,7 @@ SIGNATURES = {
     "ryo_last_error": (C.c_char_p, [_P]),
     "ryo_ctx_sync": (_I, [_P]),
     "ryo_ctx_set_timing": (_I, [_P, _I]),
-    "ryo_ctx_set_traversal_shape": (_I, [_P, _I, _I]),
+    "ryo_ctx_set_traversal_shape": (_I, [_P, _I, _I, _I]),
     "ryo_last_traverse_ms": (_D, [_P]),
     "ryo_sfc_keys_f32": (_I, [_P, _P, _P, _P, _P, _SZ, _P, _I, _I]),
     "ryo_sort_temp_bytes": (_SZ, [_SZ]),

@@ -138,14 +138,17 @@ int ryo_ctx_set_timing(ryo_ctx* h, int enable)
     return RYO_OK;
 }
 
-int ryo_ctx_set_traversal_shape(ryo_ctx* h, int stack_entries_per_wave, int blocks_per_cu)
+int ryo_ctx_set_traversal_shape(ryo_ctx* h, int stack_entries_per_wave, int blocks_per_cu, int group_ranges)
 {
     CTX_OR_FAIL(h);
     if (stack_entries_per_wave < 0 || (stack_entries_per_wave > 0 && stack_entries_per_wave < 9) ||
-        stack_entries_per_wave > 22 * 64 || blocks_per_cu < 0 || blocks_per_cu > 64)
-        return setError(ctx, RYO_ERR_ARG, "traversal shape: stack entries in [9, 1408], blocks per CU in [1, 64]");
+        stack_entries_per_wave > 22 * 64 || blocks_per_cu < 0 || blocks_per_cu > 64 || group_ranges < 0 ||
+        group_ranges > 8)
+        return setError(ctx, RYO_ERR_ARG,
+                        "traversal shape: stack entries in [9, 1408], blocks per CU in [1, 64], ranges in [1, 8]");
     ctx->travStack       = stack_entries_per_wave;
     ctx->travBlocksPerCU = blocks_per_cu;
+    ctx->travRanges      = group_ranges;
     return RYO_OK;
 }
 

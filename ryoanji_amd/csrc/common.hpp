@@ -134,6 +134,7 @@ struct Context
     // traversal launch shape: LIFO entries per wave (LDS) and resident blocks per CU (0: defaults)
     int travStack       = 0;
     int travBlocksPerCU = 0;
+    int travRanges      = 0; // group ranges with their own counters (0: one per XCD)
 
     // event timing of the last traversal kernel launch (ms), when enabled
     bool timing          = false;

@@ -52,6 +52,7 @@ struct TravArgs
     uint32_t* failed;
     int stackCap;
     size_t hCount; // targets and sources are h[0..hCount) (resident systems), 0 when unknown
+    int ranges;    // contiguous group ranges, one per XCD (1: a single global counter)
 };
 
 struct Acc
@@ -333,18 +334,19 @@ __global__ __launch_bounds__(64 * kTravWaves, 6) void traverseKernel(TravArgs a)
     // (blockIdx % kXcds under round-robin dispatch) take groups from their own range first, so the groups in
     // flight on one XCD are SFC neighbours sharing source cells in that XCD's L2; exhausted ranges are then
     // helped from in order
-    const int home = blockIdx.x % kXcds;
+    const int R    = a.ranges;
+    const int home = blockIdx.x % R;
     int src        = 0;
     for (;;)
     {
         unsigned g = ~0u;
         if (lane == 0)
         {
-            while (src < kXcds)
+            while (src < R)
             {
-                const int x       = (home + src) % kXcds;
-                const unsigned lo = unsigned((unsigned long long)a.numGroups * x / kXcds);
-                const unsigned hi = unsigned((unsigned long long)a.numGroups * (x + 1) / kXcds);
+                const int x       = (home + src) % R;
+                const unsigned lo = unsigned((unsigned long long)a.numGroups * x / R);
+                const unsigned hi = unsigned((unsigned long long)a.numGroups * (x + 1) / R);
                 if (lo < hi)
                 {
                     const unsigned i = atomicAdd(a.work + x, 1u);
@@ -487,6 +489,7 @@ int traverse(Context* ctx, TravArgs a, double* totalPot)
     const int ldsStack = ctx->travStack > 8 ? std::min(ctx->travStack, kMaxStack) : kDefaultStack;
     a.stackCap         = (a.stackCap > 8 && a.stackCap < ldsStack) ? a.stackCap : ldsStack;
     const int perCU    = ctx->travBlocksPerCU > 0 ? ctx->travBlocksPerCU : kDefaultBlocks;
+    a.ranges           = ctx->travRanges > 0 ? std::min(ctx->travRanges, kXcds) : kXcds;
     const size_t lds   = size_t(kTravWaves) * ldsStack * sizeof(int);
 
     unsigned* ctl             = reinterpret_cast<unsigned*>(ctx->dStats);
