@@ -68,6 +68,9 @@ def lib(which: str = "orc") -> C.CDLL:
             "ewald_f64": (C.c_double, [_P, _P, _P, _P, _P, _P, C.c_size_t, C.c_double, C.c_double, _P, _P, _P,
                                        _P, _P]),
             "hilbert3d": (C.c_uint64, [C.c_uint, C.c_uint, C.c_uint, C.c_int]),
+            "hilbert_mixd": (C.c_uint64, [C.c_uint, C.c_uint, C.c_uint, _P]),
+            "decode_hilbert_mixd": (C.c_int, [C.c_uint64, _P, _P]),
+            "box_bits": (None, [_P, _P]),
             "decode_hilbert3d": (None, [C.c_uint64, _P]),
             "p2p_f64": (None, [_P, _P, _P, C.c_double, C.c_double, C.c_double, C.c_int]),
             "m2p_f64": (None, [_P, _P, _P, _P]),
@@ -82,6 +85,10 @@ def lib(which: str = "orc") -> C.CDLL:
                                          _P]),
             "ewald_f64": (C.c_double, [_P, _P, _P, _P, _P, _P, C.c_size_t, _P, C.c_double, _P, _P, _P, _P, _P]),
             "ewald_f32": (C.c_double, [_P, _P, _P, _P, _P, _P, C.c_size_t, _P, C.c_double, _P, _P, _P, _P, _P]),
+            "hilbert_mixd": (C.c_uint64, [C.c_uint, C.c_uint, C.c_uint, _P]),
+            "decode_hilbert_mixd": (C.c_int, [C.c_uint64, _P, _P]),
+            "box_bits": (None, [_P, _P]),
+            "hilbert_ibox": (None, [C.c_uint64, C.c_uint, _P, _P]),
         })
     for name, (res, args) in sig.items():
         f = getattr(L, p + name)
@@ -269,6 +276,29 @@ def hilbert3d(ix, iy, iz, order=MAX_LEVEL):
 def decode_hilbert3d(key):
     out = np.zeros(3, np.uint32)
     _f("orc", "decode_hilbert3d")(int(key), _ptr(out))
+    return tuple(int(v) for v in out)
+
+
+def hilbert_mixd(ix, iy, iz, bits, lib="orc"):
+    return int(_f(lib, "hilbert_mixd")(ix, iy, iz, _ptr(_c(bits, np.uint32))))
+
+
+def decode_hilbert_mixd(key, bits, lib="orc"):
+    """(valid, (x, y, z)) of a mixed-dimension Hilbert key"""
+    out = np.zeros(3, np.uint32)
+    ok = _f(lib, "decode_hilbert_mixd")(int(key), _ptr(_c(bits, np.uint32)), _ptr(out))
+    return bool(ok), tuple(int(v) for v in out)
+
+
+def box_bits(limits, lib="orc"):
+    out = np.zeros(3, np.uint32)
+    _f(lib, "box_bits")(_ptr(_c(limits, np.float32)), _ptr(out))
+    return tuple(int(v) for v in out)
+
+
+def ref_hilbert_ibox(key, level, bits):
+    out = np.zeros(6, np.int32)
+    _f("ref", "hilbert_ibox")(int(key), int(level), _ptr(_c(bits, np.uint32)), _ptr(out))
     return tuple(int(v) for v in out)
 
 

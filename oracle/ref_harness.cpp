@@ -180,3 +180,35 @@ double ref_ewald_f32(const float* Mroot, const float* center, const float* x, co
 }
 
 } // extern "C"
+
+/* mixed-dimension Hilbert functions of the reference (hilbert.hpp:117-244, :343-585; box.hpp:160-198) */
+extern "C" {
+
+uint64_t ref_hilbert_mixd(unsigned ix, unsigned iy, unsigned iz, const unsigned* bits)
+{
+    return iHilbert<uint64_t>(ix, iy, iz, bits[0], bits[1], bits[2]);
+}
+
+int ref_decode_hilbert_mixd(uint64_t key, const unsigned* bits, unsigned* xyz)
+{
+    auto [x, y, z] = decodeHilbert<uint64_t>(key, bits[0], bits[1], bits[2]);
+    xyz[0] = unsigned(x), xyz[1] = unsigned(y), xyz[2] = unsigned(z);
+    return isValidHilbertMixDKey<uint64_t>(key, bits[0], bits[1], bits[2]) ? 1 : 0;
+}
+
+void ref_box_bits(const float* limits, unsigned* bits)
+{
+    auto b = box6(limits).getBoxDimBits(maxTreeLevel<uint64_t>{});
+    for (int d = 0; d < 3; ++d)
+        bits[d] = b[d];
+}
+
+/* integer box {xmin, xmax, ymin, ymax, zmin, zmax} of the cell at `level` with start key `key` */
+void ref_hilbert_ibox(uint64_t key, unsigned level, const unsigned* bits, int* ibox)
+{
+    IBox b = hilbertIBox<uint64_t>(key, level, bits[0], bits[1], bits[2]);
+    ibox[0] = b.xmin(), ibox[1] = b.xmax(), ibox[2] = b.ymin(), ibox[3] = b.ymax(), ibox[4] = b.zmin(),
+    ibox[5] = b.zmax();
+}
+
+} // extern "C"

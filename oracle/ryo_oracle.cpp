@@ -172,6 +172,96 @@ inline void decodeHilbert3D(uint64_t key, unsigned& ox, unsigned& oy, unsigned& 
     ox = px, oy = py, oz = pz;
 }
 
+/* Mixed-dimension Hilbert keys (hilbert.hpp:117-244), restated level by level: axes sorted by decreasing bit
+ * count (compare-and-swap (0,1), (0,2), (1,2), ties keep x, y, z); levels where only the longest axis has bits
+ * carry that bit; levels where two axes have bits follow the 2D Hilbert curve, whose swap/complement is applied
+ * to the whole remaining coordinates (so the 3D levels below see the rotated frame); the rest is 3D Hilbert. */
+inline void sortAxesDesc(const unsigned* bits, unsigned* sb, int* ax)
+{
+    for (int d = 0; d < 3; ++d)
+        sb[d] = bits[d], ax[d] = d;
+    const int pairs[3][2] = {{0, 1}, {0, 2}, {1, 2}};
+    for (auto& p : pairs)
+        if (sb[p[0]] < sb[p[1]])
+        {
+            std::swap(sb[p[0]], sb[p[1]]);
+            std::swap(ax[p[0]], ax[p[1]]);
+        }
+}
+
+inline uint64_t hilbertMixD(unsigned px, unsigned py, unsigned pz, const unsigned* bits)
+{
+    if (bits[0] == kMaxLevel && bits[1] == kMaxLevel && bits[2] == kMaxLevel) return hilbert3D(px, py, pz, kMaxLevel);
+    unsigned sb[3];
+    int ax[3];
+    sortAxesDesc(bits, sb, ax);
+    unsigned in[3] = {px, py, pz};
+    unsigned c0 = in[ax[0]], c1 = in[ax[1]], c2 = in[ax[2]];
+    uint64_t key = 0;
+    for (int L = int(sb[0]) - 1; L >= 0; --L)
+    {
+        if (L >= int(sb[1])) // one axis
+            key |= uint64_t((c0 >> L) & 1u) << (3 * L);
+        else if (L >= int(sb[2])) // two axes: 2D Hilbert digit, then orient the remaining coordinates
+        {
+            unsigned xi = (c0 >> L) & 1u, yi = (c1 >> L) & 1u;
+            key |= uint64_t((xi << 1) | (xi ^ yi)) << (3 * L);
+            if (yi == 0u)
+            {
+                std::swap(c0, c1);
+                if (xi) c0 = ~c0, c1 = ~c1;
+            }
+        }
+        else // three axes: the rest is a 3D Hilbert key of L + 1 levels in the current frame
+        {
+            unsigned m = (L + 1 >= 32) ? ~0u : ((1u << (L + 1)) - 1u);
+            key |= hilbert3D(c0 & m, c1 & m, c2 & m, L + 1);
+            break;
+        }
+    }
+    return key;
+}
+
+/* inverse of hilbertMixD, one coordinate bit per level from the top: at each level the bit combination whose
+ * encoding reproduces the key's digit there (digits depend on the coordinate bits at and above their level
+ * only). Returns false for a key no point encodes to (hilbert.hpp:517-548). */
+inline bool decodeHilbertMixD(uint64_t key, const unsigned* bits, unsigned& ox, unsigned& oy, unsigned& oz)
+{
+    if (bits[0] == kMaxLevel && bits[1] == kMaxLevel && bits[2] == kMaxLevel)
+    {
+        decodeHilbert3D(key, ox, oy, oz);
+        return true;
+    }
+    unsigned c[3] = {0, 0, 0};
+    for (int L = int(kMaxLevel) - 1; L >= 0; --L)
+    {
+        unsigned digit = unsigned(key >> (3 * L)) & 7u;
+        int act[3], na = 0;
+        for (int d = 0; d < 3; ++d)
+            if (int(bits[d]) > L) act[na++] = d;
+        if (na == 0)
+        {
+            if (digit != 0) return false;
+            continue;
+        }
+        bool found = false;
+        for (unsigned combo = 0; combo < (1u << na) && !found; ++combo)
+        {
+            unsigned t[3] = {c[0], c[1], c[2]};
+            for (int k = 0; k < na; ++k)
+                t[act[k]] |= ((combo >> k) & 1u) << L;
+            if (((hilbertMixD(t[0], t[1], t[2], bits) >> (3 * L)) & 7u) == digit)
+            {
+                c[0] = t[0], c[1] = t[1], c[2] = t[2];
+                found = true;
+            }
+        }
+        if (!found) return false;
+    }
+    ox = c[0], oy = c[1], oz = c[2];
+    return true;
+}
+
 // cstone/include/cstone/sfc/morton.hpp:94-110 (bit interleave, x most significant)
 inline uint64_t morton3D(unsigned ix, unsigned iy, unsigned iz)
 {
@@ -205,7 +295,7 @@ void sfcKeys(const T* x, const T* y, const T* z, uint64_t* keys, size_t n, const
         int ix    = std::min(cellIndex(x[i], box.lo[0], m[0]), mc[0]);
         int iy    = std::min(cellIndex(y[i], box.lo[1], m[1]), mc[1]);
         int iz    = std::min(cellIndex(z[i], box.lo[2], m[2]), mc[2]);
-        keys[i] = hilbert ? hilbert3D(ix, iy, iz, kMaxLevel) : morton3D(ix, iy, iz);
+        keys[i] = hilbert ? hilbertMixD(ix, iy, iz, box.bits) : morton3D(ix, iy, iz);
     }
 }
 
@@ -432,18 +522,21 @@ inline void nodeGeometry(uint64_t prefix, const Box<T>& box, T* center, T* size)
 {
     uint64_t key   = decodePlaceholderBit(prefix);
     unsigned level = decodePrefixLength(prefix) / 3;
-    unsigned ix, iy, iz;
-    decodeHilbert3D(key, ix, iy, iz);
-    unsigned cube = 1u << (kMaxLevel - level);
-    ix &= ~(cube - 1), iy &= ~(cube - 1), iz &= ~(cube - 1);
-    int lo[3] = {int(ix), int(iy), int(iz)};
+    unsigned ic[3] = {0, 0, 0};
+    int lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0}; // an unreachable key has the empty box at the origin
+    if (decodeHilbertMixD(key, box.bits, ic[0], ic[1], ic[2]))
+        for (int d = 0; d < 3; ++d)
+        {
+            unsigned cube = 1u << std::min(box.bits[d], kMaxLevel - level);
+            lo[d]         = int(ic[d] & ~(cube - 1));
+            hi[d]         = lo[d] + int(cube);
+        }
     constexpr T uL = T(1.) / T(1u << kMaxLevel);
     for (int d = 0; d < 3; ++d)
     {
         T half    = T(0.5) * uL * box.len[d] * T(1u << (kMaxLevel - box.bits[d]));
-        int hi    = lo[d] + int(cube);
-        center[d] = box.lo[d] + T(hi + lo[d]) * half;
-        size[d]   = T(hi - lo[d]) * half;
+        center[d] = box.lo[d] + T(hi[d] + lo[d]) * half;
+        size[d]   = T(hi[d] - lo[d]) * half;
     }
 }
 
@@ -935,7 +1028,6 @@ int orc_sfc_keys(const float* x, const float* y, const float* z, uint64_t* keys,
                  int hilbert)
 {
     auto box = makeBox<float>(limits, nullptr);
-    if (box.bits[0] != kMaxLevel || box.bits[1] != kMaxLevel || box.bits[2] != kMaxLevel) return -1; // cubic only
     sfcKeys(x, y, z, keys, n, box, hilbert != 0);
     return 0;
 }
@@ -944,7 +1036,6 @@ int orc_sfc_keys_f64(const double* x, const double* y, const double* z, uint64_t
                      const double* limits, int hilbert)
 {
     auto box = makeBox<double>(limits, nullptr);
-    if (box.bits[0] != kMaxLevel || box.bits[1] != kMaxLevel || box.bits[2] != kMaxLevel) return -1; // cubic only
     sfcKeys(x, y, z, keys, n, box, hilbert != 0);
     return 0;
 }
@@ -952,6 +1043,24 @@ int orc_sfc_keys_f64(const double* x, const double* y, const double* z, uint64_t
 uint64_t orc_hilbert3d(unsigned ix, unsigned iy, unsigned iz, int order) { return hilbert3D(ix, iy, iz, order); }
 
 void orc_decode_hilbert3d(uint64_t key, unsigned* xyz) { decodeHilbert3D(key, xyz[0], xyz[1], xyz[2]); }
+
+uint64_t orc_hilbert_mixd(unsigned ix, unsigned iy, unsigned iz, const unsigned* bits)
+{
+    return hilbertMixD(ix, iy, iz, bits);
+}
+
+/* returns 1 and the coordinates for a reachable key, 0 otherwise */
+int orc_decode_hilbert_mixd(uint64_t key, const unsigned* bits, unsigned* xyz)
+{
+    return decodeHilbertMixD(key, bits, xyz[0], xyz[1], xyz[2]) ? 1 : 0;
+}
+
+void orc_box_bits(const float* limits, unsigned* bits)
+{
+    auto box = makeBox<float>(limits, nullptr);
+    for (int d = 0; d < 3; ++d)
+        bits[d] = box.bits[d];
+}
 
 void orc_sort_by_key(uint64_t* keys, int32_t* values, size_t n) { sortByKey(keys, values, n); }
 

@@ -4,6 +4,7 @@
 // hilbert.hpp:43-94 (3D Hilbert), morton.hpp:94-110. Sort: the semantics of cub::DeviceRadixSort::SortPairs
 // as used by cstone::sortByKey (primitives_gpu.cu:310-358), i.e. std::stable_sort order on all 64 bits.
 #include "common.hpp"
+#include "sfc.hpp"
 
 #include <cstdarg>
 
@@ -21,34 +22,6 @@ __device__ __forceinline__ int cellIndex(float x, float xmin, float m)
     return int(__fsub_rn(fa, fb)) - int(__fsub_rn(a, fa) < __fsub_rn(b, fb));
 }
 
-// Hilbert key of integer coordinates, 21 levels (hilbert.hpp:43-94): octant digit via the Morton->Hilbert
-// map {0,1,3,2,7,6,4,5}, then the coordinate reflections and the axis rotation/swap of the next level.
-__device__ __forceinline__ uint64_t hilbertKey(unsigned px, unsigned py, unsigned pz)
-{
-    constexpr uint32_t toHilbert = (0u << 0) | (1u << 3) | (3u << 6) | (2u << 9) | (7u << 12) | (6u << 15) |
-                                   (4u << 18) | (5u << 21); // 3-bit table packed in a word
-    uint64_t key = 0;
-#pragma unroll
-    for (int level = int(kMaxLevel) - 1; level >= 0; --level)
-    {
-        unsigned xi = (px >> level) & 1u;
-        unsigned yi = (py >> level) & 1u;
-        unsigned zi = (pz >> level) & 1u;
-        unsigned oct = (xi << 2) | (yi << 1) | zi;
-        key = (key << 3) | ((toHilbert >> (3 * oct)) & 7u);
-        unsigned nyi = yi ^ 1u, nzi = zi ^ 1u;
-        px ^= 0u - (xi & (nyi | zi));
-        py ^= 0u - ((xi & (yi | zi)) | (yi & nzi));
-        pz ^= 0u - ((xi & nyi & nzi) | (yi & nzi));
-        unsigned tx = px, ty = py, tz = pz;
-        // zi: cyclic (x,y,z) <- (y,z,x); else !yi: swap x and z
-        px = zi ? ty : (nyi ? tz : tx);
-        py = zi ? tz : ty;
-        pz = zi ? tx : (nyi ? tx : tz);
-    }
-    return key;
-}
-
 __device__ __forceinline__ uint64_t spread3(unsigned v)
 {
     uint64_t x = v & 0x1fffffu;
@@ -63,21 +36,33 @@ __device__ __forceinline__ uint64_t spread3(unsigned v)
 struct KeyBox
 {
     float lo[3];
-    float m[3];
+    float m[3];  // 2^bits[d] / L_d
+    Bits3 bits;  // key bits per axis (21 on every axis of a cubic box)
 };
 
+// sfc.hpp:161-207: cell indices clamped to 2^bits - 1, then the Hilbert (mixed-dimension when bits differ) or
+// Morton key
+template<bool Cubic>
 __global__ __launch_bounds__(256) void sfcKeysKernel(const float* __restrict__ x, const float* __restrict__ y,
                                                      const float* __restrict__ z, uint64_t* __restrict__ keys,
                                                      size_t n, KeyBox box, int kind, int skipRemoved)
 {
-    const int mc = (1 << kMaxLevel) - 1;
+    const int mx = int((1u << box.bits.b[0]) - 1), my = int((1u << box.bits.b[1]) - 1),
+              mz = int((1u << box.bits.b[2]) - 1);
     for (size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += size_t(gridDim.x) * blockDim.x)
     {
         if (skipRemoved && keys[i] == kKeyEnd) continue;
-        int ix = min(cellIndex(x[i], box.lo[0], box.m[0]), mc);
-        int iy = min(cellIndex(y[i], box.lo[1], box.m[1]), mc);
-        int iz = min(cellIndex(z[i], box.lo[2], box.m[2]), mc);
-        keys[i] = kind == 0 ? hilbertKey(ix, iy, iz) : (spread3(ix) << 2) | (spread3(iy) << 1) | spread3(iz);
+        int ix = min(cellIndex(x[i], box.lo[0], box.m[0]), mx);
+        int iy = min(cellIndex(y[i], box.lo[1], box.m[1]), my);
+        int iz = min(cellIndex(z[i], box.lo[2], box.m[2]), mz);
+        uint64_t k;
+        if (kind != 0)
+            k = (spread3(ix) << 2) | (spread3(iy) << 1) | spread3(iz);
+        else if constexpr (Cubic)
+            k = hilbert3D(ix, iy, iz, int(kMaxLevel));
+        else
+            k = hilbertMixD(ix, iy, iz, box.bits);
+        keys[i] = k;
     }
 }
 
@@ -313,21 +298,20 @@ int sfcKeys(Context* ctx, const float* x, const float* y, const float* z, uint64
 {
     if (n == 0) return RYO_OK;
     KeyBox b;
-    float len[3];
+    b.bits = boxBits(limits); // box.hpp:160-198: an axis shorter than 0.75 x the longest one gets fewer bits
     for (int d = 0; d < 3; ++d)
     {
         float lo = limits[2 * d], hi = limits[2 * d + 1];
-        len[d]   = hi - lo;
+        if (!(hi > lo)) return setError(ctx, RYO_ERR_ARG, "sfc keys: empty box along axis %d", d);
         float il = 1.0f / (hi - lo); // Box<float>::inverseLengths_ (box.hpp:107), in float on the host
         b.lo[d]  = lo;
-        b.m[d]   = float(1u << kMaxLevel) * il;
+        b.m[d]   = float(1u << b.bits.b[d]) * il;
     }
-    // box.hpp:193-198: any axis shorter than 0.75 x the longest one changes the key layout (mixed-D keys)
-    float mx = std::max(std::max(len[0], len[1]), len[2]);
-    for (int d = 0; d < 3; ++d)
-        if (unsigned(std::floor(std::log2(mx / len[d]) + 0.5849625007211563f)) != 0)
-            return setError(ctx, RYO_ERR_UNSUPPORTED, "sfc keys: non-cubic (mixed-dimension) boxes are not supported");
-    sfcKeysKernel<<<gridFor(n, 256, ctx->numCUs * 16), 256, 0, ctx->stream>>>(x, y, z, keys, n, b, kind, skipRemoved);
+    const unsigned grid = gridFor(n, 256, ctx->numCUs * 16);
+    if (isCubic(b.bits))
+        sfcKeysKernel<true><<<grid, 256, 0, ctx->stream>>>(x, y, z, keys, n, b, kind, skipRemoved);
+    else
+        sfcKeysKernel<false><<<grid, 256, 0, ctx->stream>>>(x, y, z, keys, n, b, kind, skipRemoved);
     RYO_LAUNCH_CHECK(ctx);
     return RYO_OK;
 }

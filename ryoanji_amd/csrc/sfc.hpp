@@ -1,0 +1,321 @@
+// Hilbert keys for cubic and mixed-dimension boxes (gfx950 device code, also usable on the host).
+//
+// A box whose axes differ in length by a factor of 4/3 or more gets fewer key bits on the shorter axes
+// (cstone/include/cstone/sfc/box.hpp:160-198): bits[d] = 21 - floor(log2(Lmax / L_d) + 0.585). A key then
+// starts with 1D levels (only the longest axis has bits there), continues with 2D levels (Hilbert curve in the
+// two longest axes, whose orientation also rotates the coordinates of the levels below) and ends with 3D
+// Hilbert levels (hilbert.hpp:117-244 encode, :343-515 decode, :517-585 valid keys and cell boxes). With
+// bits = {21, 21, 21} everything reduces to the 21-level 3D Hilbert curve.
+#pragma once
+
+#include <algorithm>
+#include <cmath>
+
+#include "common.hpp"
+
+namespace ryo
+{
+
+struct Bits3
+{
+    unsigned b[3];
+};
+
+__host__ __device__ inline bool isCubic(const Bits3& a)
+{
+    return a.b[0] == kMaxLevel && a.b[1] == kMaxLevel && a.b[2] == kMaxLevel;
+}
+
+// 3D Hilbert key of `order` levels (hilbert.hpp:43-94): octant digit through the Morton->Hilbert map
+// {0,1,3,2,7,6,4,5}, then the reflections and the rotation/swap that orient the next level
+__host__ __device__ inline uint64_t hilbert3D(unsigned px, unsigned py, unsigned pz, int order)
+{
+    constexpr uint32_t toHilbert = (0u << 0) | (1u << 3) | (3u << 6) | (2u << 9) | (7u << 12) | (6u << 15) |
+                                   (4u << 18) | (5u << 21); // 3-bit table packed in a word
+    uint64_t key = 0;
+    for (int level = order - 1; level >= 0; --level)
+    {
+        unsigned xi = (px >> level) & 1u;
+        unsigned yi = (py >> level) & 1u;
+        unsigned zi = (pz >> level) & 1u;
+        unsigned oct = (xi << 2) | (yi << 1) | zi;
+        key = (key << 3) | ((toHilbert >> (3 * oct)) & 7u);
+        unsigned nyi = yi ^ 1u, nzi = zi ^ 1u;
+        px ^= 0u - (xi & (nyi | zi));
+        py ^= 0u - ((xi & (yi | zi)) | (yi & nzi));
+        pz ^= 0u - ((xi & nyi & nzi) | (yi & nzi));
+        unsigned tx = px, ty = py, tz = pz;
+        // zi: cyclic (x,y,z) <- (y,z,x); else !yi: swap x and z
+        px = zi ? ty : (nyi ? tz : tx);
+        py = zi ? tz : ty;
+        pz = zi ? tx : (nyi ? tx : tz);
+    }
+    return key;
+}
+
+// inverse of hilbert3D for `order` levels: corner coordinates of the cell with start key `key`
+__host__ __device__ inline void hilbert3DDecode(uint64_t key, int order, unsigned& ox, unsigned& oy, unsigned& oz)
+{
+    unsigned px = 0, py = 0, pz = 0;
+    for (int level = 0; level < order; ++level)
+    {
+        unsigned oct = unsigned(key >> (3 * level)) & 7u;
+        unsigned xi = oct >> 2u, yi = (oct >> 1u) & 1u, zi = oct & 1u;
+        unsigned tx = px, ty = py, tz = pz;
+        if (yi ^ zi) // inverse cyclic rotation
+        {
+            px = tz, pz = ty, py = tx;
+        }
+        else if ((xi | yi | zi) == 0u || (xi & yi & zi) == 1u) // swap x and z
+        {
+            px = tz, pz = tx;
+        }
+        unsigned mask = (1u << level) - 1u;
+        unsigned nxi = xi ^ 1u, nyi = yi ^ 1u, nzi = zi ^ 1u;
+        px ^= mask & (0u - (xi & (yi | zi)));
+        py ^= mask & (0u - ((xi & (nyi | nzi)) | (nxi & yi & zi)));
+        pz ^= mask & (0u - ((xi & nyi & nzi) | (yi & zi)));
+        px |= xi << level;
+        py |= (xi ^ yi) << level;
+        pz |= (yi ^ zi) << level;
+    }
+    ox = px, oy = py, oz = pz;
+}
+
+// 2D Hilbert key of `order` levels (hilbert.hpp:246-277): two bits per level, 2 xi + (xi ^ yi); when yi = 0 the
+// remaining coordinates are swapped and, if xi = 1, complemented
+__host__ __device__ inline uint64_t hilbert2D(unsigned px, unsigned py, int order)
+{
+    uint64_t key = 0;
+    for (int level = order - 1; level >= 0; --level)
+    {
+        unsigned xi = (px >> level) & 1u, yi = (py >> level) & 1u;
+        if (yi == 0u)
+        {
+            unsigned flip = 0u - xi, t = px;
+            px = py ^ flip;
+            py = t ^ flip;
+        }
+        key = (key << 2) | (xi << 1) | (xi ^ yi);
+    }
+    return key;
+}
+
+// inverse of hilbert2D for `order` >= 1 levels (hilbert.hpp:317-341)
+__host__ __device__ inline void hilbert2DDecode(uint64_t key, int order, unsigned& ox, unsigned& oy)
+{
+    unsigned x = 0, y = 0;
+    for (int level = 0; level < 2 * order; level += 2)
+    {
+        unsigned sa = unsigned(key >> (level + 1)) & 1u, sb = unsigned(key >> level) & 1u;
+        if ((sa ^ sb) == 0u)
+        {
+            unsigned flip = 0u - sa, t = x;
+            x = y ^ flip;
+            y = t ^ flip;
+        }
+        x = (x >> 1) | (sa << 31);
+        y = (y >> 1) | ((sa ^ sb) << 31);
+    }
+    ox = x >> (32 - order);
+    oy = y >> (32 - order);
+}
+
+// axes ordered by decreasing bit count; ties keep the x, y, z order (the three compare-and-swap steps of the
+// reference sort network)
+struct AxisOrder
+{
+    unsigned bits[3];
+    int axis[3];
+};
+
+__host__ __device__ inline AxisOrder sortAxes(const Bits3& a)
+{
+    AxisOrder o{{a.b[0], a.b[1], a.b[2]}, {0, 1, 2}};
+    auto cas = [&o](int i, int j)
+    {
+        if (o.bits[i] < o.bits[j])
+        {
+            unsigned tb = o.bits[i];
+            o.bits[i]   = o.bits[j];
+            o.bits[j]   = tb;
+            int ta      = o.axis[i];
+            o.axis[i]   = o.axis[j];
+            o.axis[j]   = ta;
+        }
+    };
+    cas(0, 1);
+    cas(0, 2);
+    cas(1, 2);
+    return o;
+}
+
+// the 2D levels decide the orientation in which the 3D part of the key is reached: apply (encode) the same swap
+// and complement to the low coordinates, from the coarsest 2D level to the finest
+__host__ __device__ inline void orient2D(uint64_t key2D, int n, uint64_t mask, uint64_t& u, uint64_t& v, bool forward)
+{
+    for (int i = 0; i < n; ++i)
+    {
+        int level      = forward ? n - 1 - i : i;
+        unsigned quad  = unsigned(key2D >> (2 * level)) & 3u;
+        unsigned xi    = (quad >> 1) & 1u;
+        unsigned yi    = xi ^ (quad & 1u);
+        if (yi == 0u)
+        {
+            uint64_t t = u;
+            u          = v;
+            v          = t;
+            if (xi == 1u)
+            {
+                u = mask - u;
+                v = mask - v;
+            }
+        }
+    }
+}
+
+// Hilbert key of integer coordinates with per-axis bit counts (hilbert.hpp:117-244)
+__host__ __device__ inline uint64_t hilbertMixD(unsigned px, unsigned py, unsigned pz, const Bits3& bits)
+{
+    if (isCubic(bits)) return hilbert3D(px, py, pz, int(kMaxLevel));
+    AxisOrder o    = sortAxes(bits);
+    unsigned in[3] = {px, py, pz};
+    uint64_t c[3]  = {in[o.axis[0]], in[o.axis[1]], in[o.axis[2]]};
+    unsigned b0 = o.bits[0], b1 = o.bits[1], b2 = o.bits[2];
+    uint64_t key = 0;
+    // 1D levels: the longest axis alone, one bit per 3-bit digit
+    for (unsigned bit = b1; bit < b0; ++bit)
+        key |= ((c[0] >> bit) & 1u) << (3 * bit);
+    c[0] &= (uint64_t(1) << b1) - 1;
+    // 2D levels of the two longest axes
+    if (b1 > b2)
+    {
+        const int n         = int(b1 - b2);
+        const uint64_t mask = (uint64_t(1) << b2) - 1;
+        uint64_t key2D      = hilbert2D(unsigned(c[0] >> b2), unsigned(c[1] >> b2), n);
+        uint64_t u = c[0] & mask, v = c[1] & mask;
+        orient2D(key2D, n, mask, u, v, true);
+        for (int i = 0; i < n; ++i)
+            key |= ((key2D >> (2 * (n - 1 - i))) & 3u) << (3 * (b1 - 1 - unsigned(i)));
+        c[0] = u, c[1] = v;
+    }
+    // 3D levels
+    return key | hilbert3D(unsigned(c[0]), unsigned(c[1]), unsigned(c[2]), int(b2));
+}
+
+// inverse of hilbertMixD (hilbert.hpp:343-515)
+__host__ __device__ inline void hilbertMixDDecode(uint64_t key, const Bits3& bits, unsigned& ox, unsigned& oy,
+                                                  unsigned& oz)
+{
+    if (isCubic(bits))
+    {
+        hilbert3DDecode(key, int(kMaxLevel), ox, oy, oz);
+        return;
+    }
+    AxisOrder o   = sortAxes(bits);
+    unsigned b0 = o.bits[0], b1 = o.bits[1], b2 = o.bits[2];
+    uint64_t c[3] = {0, 0, 0};
+    for (unsigned bit = b1; bit < b0; ++bit)
+        c[0] |= ((key >> (3 * bit)) & 1u) << bit;
+    key &= (uint64_t(1) << (3 * b1)) - 1;
+    uint64_t key2D = 0;
+    const int n    = b1 > b2 ? int(b1 - b2) : 0;
+    if (n > 0)
+    {
+        for (int i = 0; i < n; ++i)
+            key2D |= ((key >> (3 * (b1 - 1 - unsigned(i)))) & 3u) << (2 * (n - 1 - i));
+        unsigned hx, hy;
+        hilbert2DDecode(key2D, n, hx, hy);
+        const uint64_t nm = (uint64_t(1) << n) - 1;
+        c[0] |= (uint64_t(hx) & nm) << b2;
+        c[1] |= (uint64_t(hy) & nm) << b2;
+        key &= (uint64_t(1) << (3 * b2)) - 1;
+    }
+    unsigned x3, y3, z3;
+    hilbert3DDecode(key, int(b2), x3, y3, z3);
+    uint64_t u = x3, v = y3;
+    if (n > 0) orient2D(key2D, n, (uint64_t(1) << b2) - 1, u, v, false);
+    c[0] |= u, c[1] |= v, c[2] |= z3;
+    unsigned out[3];
+    out[o.axis[0]] = unsigned(c[0]);
+    out[o.axis[1]] = unsigned(c[1]);
+    out[o.axis[2]] = unsigned(c[2]);
+    ox = out[0], oy = out[1], oz = out[2];
+}
+
+// a key reachable by the mixed-dimension encoding: digits of 2D levels are < 4, digits of 1D levels < 2
+// (hilbert.hpp:517-548)
+__host__ __device__ inline bool validMixDKey(uint64_t key, const Bits3& bits)
+{
+    unsigned s[3] = {bits.b[0], bits.b[1], bits.b[2]}; // ascending
+    auto cas = [&s](int i, int j)
+    {
+        if (s[i] > s[j])
+        {
+            unsigned t = s[i];
+            s[i]       = s[j];
+            s[j]       = t;
+        }
+    };
+    cas(0, 1);
+    cas(0, 2);
+    cas(1, 2);
+    for (unsigned i = s[0] + 1; i <= kMaxLevel; ++i)
+    {
+        unsigned digit = unsigned(key >> (3 * (i - 1))) & 7u;
+        if (i <= s[1])
+        {
+            if (digit > 3u) return false;
+        }
+        else if (i <= s[2])
+        {
+            if (digit > 1u) return false;
+        }
+    }
+    return true;
+}
+
+// integer box [lo, lo + len) per axis of the cell at `level` with start key `key` (hilbertIBox,
+// hilbert.hpp:563-585); an unreachable key gives the empty box at the origin
+struct IBox3
+{
+    unsigned lo[3], len[3];
+};
+
+__host__ __device__ inline IBox3 cellIBox(uint64_t key, unsigned level, const Bits3& bits)
+{
+    IBox3 r{{0, 0, 0}, {0, 0, 0}};
+    if (!validMixDKey(key, bits)) return r;
+    const unsigned lfr = kMaxLevel - level;
+    unsigned c[3];
+    hilbertMixDDecode(key, bits, c[0], c[1], c[2]);
+    for (int d = 0; d < 3; ++d)
+    {
+        unsigned len = 1u << (bits.b[d] < lfr ? bits.b[d] : lfr);
+        r.lo[d]      = c[d] & ~(len - 1u);
+        r.len[d]     = len;
+    }
+    return r;
+}
+
+} // namespace ryo
+
+namespace ryo
+{
+
+// per-axis key bits of a box given as {xmin, xmax, ymin, ymax, zmin, zmax} (Box<float>::computeBoxDimBits and
+// getBoxDimBits, box.hpp:160-198, in float as the reference's Box<float> computes them)
+inline Bits3 boxBits(const float* limits)
+{
+    float len[3] = {limits[1] - limits[0], limits[3] - limits[2], limits[5] - limits[4]};
+    float mx     = std::max(std::max(len[0], len[1]), len[2]);
+    Bits3 r;
+    for (int d = 0; d < 3; ++d)
+    {
+        unsigned red = unsigned(std::floor(std::log2(mx / len[d]) + 0.5849625007211563f));
+        r.b[d]       = red >= kMaxLevel ? 0u : kMaxLevel - red;
+    }
+    return r;
+}
+
+} // namespace ryo

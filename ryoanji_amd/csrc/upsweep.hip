@@ -12,6 +12,7 @@
 // computeVecMacR2 (cstone/traversal/macs.hpp:64-82) with hilbertIBox/centerAndSize (hilbert.hpp:549-585,
 // box.hpp:357-381). All arithmetic is FP64; the traversal copies are rounded to FP32 once.
 #include "common.hpp"
+#include "sfc.hpp"
 
 namespace ryo
 {
@@ -19,54 +20,23 @@ namespace ryo
 struct MacBox
 {
     double lo[3];
-    double half[3]; // half of the smallest cell edge, per axis
+    double half[3]; // half of the smallest cell edge per axis: 0.5 L_d / 2^bits[d]
+    Bits3 bits;     // key bits per axis
     float invTheta;
 };
 
-// integer coordinates of the corner of the cell with Hilbert start key `key` (inverse of the encoding)
-__device__ inline void hilbertDecode(uint64_t key, unsigned& ox, unsigned& oy, unsigned& oz)
-{
-    unsigned px = 0, py = 0, pz = 0;
-    for (unsigned level = 0; level < kMaxLevel; ++level)
-    {
-        unsigned oct = unsigned(key >> (3 * level)) & 7u;
-        unsigned xi = oct >> 2u, yi = (oct >> 1u) & 1u, zi = oct & 1u;
-        unsigned tx = px, ty = py, tz = pz;
-        if (yi ^ zi) // inverse cyclic rotation
-        {
-            px = tz, pz = ty, py = tx;
-        }
-        else if ((xi | yi | zi) == 0u || (xi & yi & zi) == 1u) // swap x and z
-        {
-            px = tz, pz = tx;
-        }
-        unsigned mask = (1u << level) - 1u;
-        unsigned nxi = xi ^ 1u, nyi = yi ^ 1u, nzi = zi ^ 1u;
-        px ^= mask & (0u - (xi & (yi | zi)));
-        py ^= mask & (0u - ((xi & (nyi | nzi)) | (nxi & yi & zi)));
-        pz ^= mask & (0u - ((xi & nyi & nzi) | (yi & zi)));
-        px |= xi << level;
-        py |= (xi ^ yi) << level;
-        pz |= (yi ^ zi) << level;
-    }
-    ox = px, oy = py, oz = pz;
-}
-
-// squared vector-MAC radius (2 max(size)/theta + |com - geometric center|)^2 of the node with `prefix`
+// squared vector-MAC radius (2 max(size)/theta + |com - geometric center|)^2 of the node with `prefix`, the
+// geometric box from the node's integer cell box (hilbertIBox / centerAndSize, mixed-dimension aware)
 __device__ inline double macSquared(uint64_t prefix, double cx, double cy, double cz, const MacBox& b)
 {
     uint64_t key   = decodePlaceholder(prefix);
     unsigned level = prefixLength(prefix) / 3u;
-    unsigned ix, iy, iz;
-    hilbertDecode(key, ix, iy, iz);
-    unsigned cube = 1u << (kMaxLevel - level);
-    ix &= ~(cube - 1), iy &= ~(cube - 1), iz &= ~(cube - 1);
+    IBox3 ib       = cellIBox(key, level, b.bits);
     double gc[3], gs[3];
-    unsigned lo[3] = {ix, iy, iz};
     for (int d = 0; d < 3; ++d)
     {
-        gc[d] = b.lo[d] + double(2 * lo[d] + cube) * b.half[d];
-        gs[d] = double(cube) * b.half[d];
+        gc[d] = b.lo[d] + double(2 * ib.lo[d] + ib.len[d]) * b.half[d];
+        gs[d] = double(ib.len[d]) * b.half[d];
     }
     double dx = cx - gc[0], dy = cy - gc[1], dz = cz - gc[2];
     double s   = sqrt(dx * dx + dy * dy + dz * dz);
@@ -219,11 +189,13 @@ int upsweep(Context* ctx, const float* x, const float* y, const float* z, const 
     a.layout = layout, a.numLeaves = L;
     a.leafNode = l2i + nI, a.parents = parents, a.child = child, a.prefixes = prefixes;
     a.c32 = reinterpret_cast<float4*>(centers), a.q32 = mpoles, a.c64 = c64, a.q64 = q64;
+    const Bits3 bits = boxBits(limits);
+    a.box.bits       = bits;
     for (int d = 0; d < 3; ++d)
     {
         double lo = limits[2 * d], hi = limits[2 * d + 1];
         a.box.lo[d]   = lo;
-        a.box.half[d] = 0.5 * (hi - lo) / double(1u << kMaxLevel);
+        a.box.half[d] = 0.5 * (hi - lo) / double(1u << bits.b[d]);
     }
     a.box.invTheta = invTheta;
     hipStream_t s = ctx->stream;

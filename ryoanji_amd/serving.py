@@ -83,6 +83,63 @@ class Server:
             pass
 
 
+def _fit_box(x, y, z):
+    """cubic box around all particles, padded by a relative 1e-4 (the resident leapfrog's fitBox)"""
+    mn = np.array([x.min(), y.min(), z.min()], np.float64)
+    mx = np.array([x.max(), y.max(), z.max()], np.float64)
+    c = 0.5 * (mn + mx)
+    half = 0.5 * float((mx - mn).max()) * (1 + 1e-4) + 1e-30
+    lo = (c - half).astype(np.float32)
+    L = np.float32(2 * half)
+    return np.array([lo[0], lo[0] + L, lo[1], lo[1] + L, lo[2], lo[2] + L], np.float32)
+
+
+def plain_solves(systems, params):
+    """One-at-a-time plain solves of the same stream: each timestep of each request is one full pass from host
+    memory (upload x,y,z,m,h, keys -> sort -> tree -> upsweep -> traversal, download the accelerations) with the
+    kick-drift-kick on the host; nothing stays resident between steps and requests never overlap."""
+    import time
+
+    from . import Context, System, default_params
+
+    ctx = Context()
+    sysm = System(ctx, int(max(s.n for s in systems)))
+    out = []
+    t0 = time.perf_counter()
+    try:
+        for i, s in enumerate(systems):
+            prm = default_params(theta=float(params["theta"][i]), group_size=64, bucket_size=64)
+            dt = float(params["dt"])
+            X = [np.ascontiguousarray(a, np.float32) for a in (s.x, s.y, s.z)]
+            m, h = np.ascontiguousarray(s.m, np.float32), np.ascontiguousarray(s.h, np.float32)
+            V = [np.zeros(s.n, np.float32) for _ in range(3)]
+
+            def solve(lim):
+                sysm.load(X[0], X[1], X[2], m, h, lim)
+                sysm.build(prm)
+                pot = sysm.gravity(prm)
+                _, gx, gy, gz, o = (t.cpu().numpy() for t in sysm.results())
+                A = [np.empty(s.n, np.float32) for _ in range(3)]
+                for d, g in enumerate((gx, gy, gz)):
+                    A[d][o] = g
+                return A, pot
+
+            A, pot = solve(s.limits)
+            for _ in range(int(params["steps"][i])):
+                for d in range(3):
+                    V[d] += np.float32(0.5 * dt) * A[d]
+                    X[d] += np.float32(dt) * V[d]
+                A, pot = solve(_fit_box(*X))
+                for d in range(3):
+                    V[d] += np.float32(0.5 * dt) * A[d]
+            out.append(dict(arrival=0.0, admit=0.0, finish=(time.perf_counter() - t0) * 1e3, pot=pot, slot=0,
+                            status=0))
+    finally:
+        sysm.close()
+        ctx.close()
+    return out
+
+
 def summarize(systems, params, results):
     lat = np.array([r["finish"] - r["arrival"] for r in results])
     makespan = max(r["finish"] for r in results) - min(r["arrival"] for r in results)

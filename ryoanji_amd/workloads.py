@@ -33,7 +33,7 @@ def _stream(seed: int, k: int) -> int:
 
 
 class System:
-    """SoA float32 particle arrays plus the cubic box they live in."""
+    """SoA float32 particle arrays plus the box they live in ({xmin, xmax, ymin, ymax, zmin, zmax})."""
 
     def __init__(self, x, y, z, m, h, limits, periodic=False, name=""):
         self.x, self.y, self.z, self.m, self.h = (np.ascontiguousarray(a, np.float32) for a in (x, y, z, m, h))
@@ -56,6 +56,24 @@ def uniform_cube(n: int, seed: int = 42, lo: float = -1.0, hi: float = 1.0, soft
     m = np.full(n, 1.0 / n, np.float32)
     h = np.full(n, soft, np.float32)
     return System(*xyz, m, h, [lo, hi] * 3, periodic=periodic, name=f"uniform{n}")
+
+
+def uniform_box(n: int, seed: int = 42, extents=(2.0, 1.0, 0.25), soft: float = 1e-3, periodic=False):
+    """n particles uniform in the box [0, e_x) x [0, e_y) x [0, e_z), mass 1/n. Axes shorter than 3/4 of the
+    longest one get fewer key bits (mixed-dimension Hilbert keys)."""
+    xyz = [np.float32(e) * uniform01(_stream(seed, d), 0, n).astype(np.float32) for d, e in enumerate(extents)]
+    m = np.full(n, 1.0 / n, np.float32)
+    h = np.full(n, soft, np.float32)
+    lim = [0.0, extents[0], 0.0, extents[1], 0.0, extents[2]]
+    return System(*xyz, m, h, lim, periodic=periodic, name=f"box{n}")
+
+
+def plummer_disk(n: int, seed: int = 42, flattening: float = 0.2, rmax: float = 30.0, soft: float = 1e-3):
+    """A Plummer sphere squashed along z by `flattening`, in the box [-rmax, rmax]^2 x [-f rmax, f rmax]."""
+    s = plummer(n, seed, rmax, soft)
+    z = (s.z * np.float32(flattening)).astype(np.float32)
+    f = float(np.float32(flattening) * np.float32(rmax))
+    return System(s.x, s.y, z, s.m, s.h, [-rmax, rmax, -rmax, rmax, -f, f], name=f"disk{n}")
 
 
 def _plummer_radii(n: int, seed: int, rmax: float) -> np.ndarray:

@@ -20,6 +20,9 @@ SYSTEMS = {
     "uniform65536": lambda: W.uniform_cube(65536, seed=1),
     "plummer32768": lambda: W.plummer(32768, seed=2),
     "hernquist40000": lambda: W.hernquist_mix(40000, seed=3),
+    # non-cubic boxes: mixed-dimension Hilbert keys (bits 21/20/18 and 21/21/20)
+    "box40000": lambda: W.uniform_box(40000, seed=24, extents=(2.0, 1.0, 0.25)),
+    "disk32768": lambda: W.plummer_disk(32768, seed=25, flattening=0.4),
 }
 
 
@@ -131,7 +134,7 @@ def test_upsweep_fp64(ctx, name):
     assert np.allclose(c32[:, :3], fc[:, :3], rtol=0, atol=5e-6 * np.abs(fc[:, :3]).max())
     fscale = np.abs(fq).max(axis=0) + 1e-30
     assert np.all(np.abs(q32 - fq) <= 1e-4 * fscale)
-    box = float(s.limits[1] - s.limits[0])
+    box = float(max(s.limits[1] - s.limits[0], s.limits[3] - s.limits[2], s.limits[5] - s.limits[4]))
     r_gpu, r_ref = np.sqrt(c32[:, 3].astype(np.float64)), np.sqrt(fc[:, 3].astype(np.float64))
     assert np.all(np.abs(r_gpu - r_ref) <= 1e-4 * r_ref + 4e-6 * box)
 

@@ -43,7 +43,9 @@ def _direct(ctx, x, y, z, h, m, targets=None, box_l=None, shells=0):
 
 
 @pytest.mark.parametrize("name,mk", [("uniform", lambda: W.uniform_cube(65536, seed=1)),
-                                     ("plummer", lambda: W.plummer(65536, seed=2))])
+                                     ("plummer", lambda: W.plummer(65536, seed=2)),
+                                     ("box", lambda: W.uniform_box(65536, seed=3, extents=(0.5, 2.0, 1.0))),
+                                     ("disk", lambda: W.plummer_disk(65536, seed=4, flattening=0.1))])
 def test_system_tree_and_forces(ctx, name, mk):
     s = mk()
     sysm, pot, p, ax, ay, az, order = _solve(ctx, s)
