@@ -129,6 +129,20 @@ int ryo_upsweep_f32(ryo_ctx* ctx, const float* x, const float* y, const float* z
 int ryo_fixed_groups(ryo_ctx* ctx, uint32_t first, uint32_t last, unsigned group_size, uint32_t* groups,
                      int* num_groups);
 
+/* cstone::computeGroupSplits (traversal/groups_gpu.cu:121-151, groups_gpu.cuh:149-270) for (Tc, T) = (f, f):
+ * windows of group_size (= 64, bhMaxTargetSize) SFC-ordered targets [first, last) are split after target l
+ * when |x_{l+1} - x_l| > min(tol_factor * cbrt(smallest leaf volume in the window), 2 h_l) in units of the
+ * longest box edge (h may be NULL: radius 1), and windows longer than extent_ratio x tol_factor x the smallest
+ * leaf edge along an axis are cut into equal chunks (extent_ratio 0 disables that; the reference default is 8,
+ * CSTONE_GROUP_EXTENT_SPLIT). leaves/layout: the cornerstone leaf array ([num_leaves+1]) and its layout.
+ * groups (device, capacity entries) receives the G+1 group boundaries; *num_groups (host) = G. Synchronizes
+ * the stream (the group count is read back, as in the reference). RYO_ERR_CAPACITY when G+1 > capacity;
+ * RYO_ERR_UNSUPPORTED for group sizes other than 64. */
+int ryo_group_splits_f32(ryo_ctx* ctx, uint32_t first, uint32_t last, const float* x, const float* y, const float* z,
+                         const float* h, const uint64_t* leaves, int num_leaves, const uint32_t* layout,
+                         const float* limits, unsigned group_size, float tol_factor, float extent_ratio,
+                         uint32_t* groups, size_t capacity, int* num_groups);
+
 /* ---------------------------------------------------------------- a14-a17: traversal */
 
 /* ryoanji::traverse (nbody/traversal_gpu.h:51-56, traversal_gpu.cu:572-623) for the float instantiation

@@ -194,6 +194,21 @@ inline GroupView computeFixedGroups(execution::Gpu, LocalIndex first, LocalIndex
     ryoanji_amd::check(ryo_fixed_groups(ryoanji_amd::ctx(), first, last, groupSize, groups, &G));
     return GroupView{first, last, LocalIndex(G), groups, groups + 1};
 }
+
+/*! cstone::computeGroupSplits -- cstone/traversal/groups_gpu.cu:121-151 for (Tc, T) = (float, float): target
+ *  groups of up to groupSize (64) SFC-ordered particles, split where they are not compact. groups must hold
+ *  up to last - first + 1 entries (`capacity`); extentRatio as CSTONE_GROUP_EXTENT_SPLIT (default 8). */
+inline GroupView computeGroupSplits(execution::Gpu, LocalIndex first, LocalIndex last, const float* x, const float* y,
+                                    const float* z, const float* h, const uint64_t* leaves, TreeNodeIndex numLeaves,
+                                    const LocalIndex* layout, const Box<float>& box, unsigned groupSize,
+                                    float tolFactor, LocalIndex* groups, size_t capacity, float extentRatio = 8.0f)
+{
+    int G = 0;
+    ryoanji_amd::check(ryo_group_splits_f32(ryoanji_amd::ctx(), first, last, x, y, z, h, leaves, int(numLeaves),
+                                            layout, box.limits(), groupSize, tolFactor, extentRatio, groups,
+                                            capacity, &G));
+    return GroupView{first, last, LocalIndex(G), groups, groups + 1};
+}
 } // namespace cstone
 
 namespace ryoanji

@@ -71,6 +71,10 @@ def lib(which: str = "orc") -> C.CDLL:
             "hilbert_mixd": (C.c_uint64, [C.c_uint, C.c_uint, C.c_uint, _P]),
             "decode_hilbert_mixd": (C.c_int, [C.c_uint64, _P, _P]),
             "box_bits": (None, [_P, _P]),
+            "group_splits": (C.c_long, [C.c_uint, C.c_uint, _P, _P, _P, _P, _P, C.c_int, _P, _P, C.c_float,
+                                        C.c_float, _P, C.c_long]),
+            "compute_gravity_groups_f32": (C.c_double, [_P, C.c_int, _P, _P, _P, _P, _P, _P, _P, _P, _P,
+                                                        C.c_float, C.c_int, _P, C.c_long, _P, _P, _P, _P, _P]),
             "decode_hilbert3d": (None, [C.c_uint64, _P]),
             "p2p_f64": (None, [_P, _P, _P, C.c_double, C.c_double, C.c_double, C.c_int]),
             "m2p_f64": (None, [_P, _P, _P, _P]),
@@ -214,6 +218,23 @@ def compute_gravity(tree_leaves, layout, centers, mp, x, y, z, h, m, limits, G=1
     return tot, ug, ax, ay, az, stats
 
 
+def compute_gravity_groups(tree_leaves, layout, centers, mp, x, y, z, h, m, limits, bounds, G=1.0, num_shells=0):
+    """the oracle traversal over arbitrary target groups bounds[g]..bounds[g+1] -> (pot, ugrav, ax, ay, az, stats)"""
+    leaves = _c(tree_leaves, np.uint64)
+    layout = _c(layout, np.uint32)
+    arrs = [_c(a, np.float32) for a in (x, y, z, h, m)]
+    n = arrs[0].size
+    ug, ax, ay, az = (np.zeros(n, np.float32) for _ in range(4))
+    b = _c(bounds, np.uint32)
+    stats = np.zeros(2, np.uint64)
+    tot = _f("orc", "compute_gravity_groups_f32")(_ptr(leaves), leaves.size - 1, _ptr(layout),
+                                                 _ptr(_c(centers, np.float32)), _ptr(_c(mp, np.float32)),
+                                                 *(_ptr(a) for a in arrs), _ptr(_c(limits, np.float32)), float(G),
+                                                 num_shells, _ptr(b), b.size - 1, _ptr(ug), _ptr(ax), _ptr(ay),
+                                                 _ptr(az), _ptr(stats))
+    return tot, ug, ax, ay, az, stats
+
+
 def compute_gravity_f64(tree_leaves, layout, centers, mp, x, y, z, h, m, limits, G=1.0, num_shells=0,
                         group_size=16):
     """The oracle traversal evaluated in double on the same (float) inputs -> (pot_total, ugrav, ax, ay, az)."""
@@ -277,6 +298,20 @@ def decode_hilbert3d(key):
     out = np.zeros(3, np.uint32)
     _f("orc", "decode_hilbert3d")(int(key), _ptr(out))
     return tuple(int(v) for v in out)
+
+
+def group_splits(first, last, x, y, z, h, leaves, layout, limits, tol_factor=2.0, extent_ratio=8.0):
+    """computeGroupSplits with 64-target windows -> group boundaries (G+1 entries)"""
+    arrs = [_c(a, np.float32) for a in (x, y, z)]
+    hh = None if h is None else _c(h, np.float32)
+    lv = _c(leaves, np.uint64)
+    lay = _c(layout, np.uint32)
+    cap = int(last - first) + 2
+    out = np.zeros(cap, np.uint32)
+    G = _f("orc", "group_splits")(int(first), int(last), *(_ptr(a) for a in arrs), _ptr(hh), _ptr(lv), lv.size - 1,
+                                  _ptr(lay), _ptr(_c(limits, np.float32)), float(tol_factor), float(extent_ratio),
+                                  _ptr(out), cap)
+    return out[:G + 1].copy()
 
 
 def hilbert_mixd(ix, iy, iz, bits, lib="orc"):

@@ -788,10 +788,12 @@ void gravityGroup(const T (*tgt)[4], int nt, const Octree& o, const T* centers, 
 template<class T, bool Exact = false>
 double computeGravity(const Octree& o, const T* centers, const T* mp, const unsigned* layout, const T* x, const T* y,
                       const T* z, const T* h, const T* m, const Box<T>& box, float G, int numShells, int groupSize,
-                      T* ugrav, T* ax, T* ay, T* az, const int* groupList, int numGroupList, TravStats* stats)
+                      T* ugrav, T* ax, T* ay, T* az, const int* groupList, int numGroupList, TravStats* stats,
+                      const unsigned* bounds = nullptr, long numBounded = 0)
 {
+    // groups: fixed groupSize-target groups of [layout[0], layout[L]), or bounds[g]..bounds[g+1] when given
     const unsigned first = layout[0], last = layout[o.numLeaves];
-    long numGroups       = (long(last - first) + groupSize - 1) / groupSize;
+    long numGroups       = bounds ? numBounded : (long(last - first) + groupSize - 1) / groupSize;
     long work            = groupList ? numGroupList : numGroups;
     double pot           = 0;
     uint64_t p2p = 0, m2p = 0;
@@ -799,8 +801,8 @@ double computeGravity(const Octree& o, const T* centers, const T* mp, const unsi
     for (long gi = 0; gi < work; ++gi)
     {
         long g       = groupList ? groupList[gi] : gi;
-        unsigned i0  = first + unsigned(g) * groupSize;
-        int nt       = int(std::min<long>(groupSize, long(last) - i0));
+        unsigned i0  = bounds ? bounds[g] : first + unsigned(g) * groupSize;
+        int nt       = bounds ? int(bounds[g + 1] - bounds[g]) : int(std::min<long>(groupSize, long(last) - i0));
         std::vector<T> tbuf(4 * size_t(nt)), sbuf(4 * size_t(nt)), abuf(4 * size_t(nt), T(0));
         auto tgt     = reinterpret_cast<T(*)[4]>(tbuf.data());
         auto shifted = reinterpret_cast<T(*)[4]>(sbuf.data());
@@ -1140,6 +1142,22 @@ double orc_compute_gravity_f32(const uint64_t* leaves, int numLeaves, const unsi
     return pot;
 }
 
+/* computeGravity over arbitrary target groups bounds[g]..bounds[g+1] (g < numGroups), e.g. split groups */
+double orc_compute_gravity_groups_f32(const uint64_t* leaves, int numLeaves, const unsigned* layout,
+                                      const float* centers, const float* mpoles, const float* x, const float* y,
+                                      const float* z, const float* h, const float* m, const float* limits, float G,
+                                      int numShells, const unsigned* bounds, long numGroups, float* ugrav, float* ax,
+                                      float* ay, float* az, uint64_t* stats)
+{
+    Octree o = rebuild(leaves, numLeaves);
+    auto box = makeBox<float>(limits, nullptr);
+    TravStats st;
+    double pot = computeGravity<float>(o, centers, mpoles, layout, x, y, z, h, m, box, G, numShells, 64, ugrav, ax,
+                                       ay, az, nullptr, 0, &st, bounds, numGroups);
+    if (stats) stats[0] = st.sumP2P, stats[1] = st.sumM2P;
+    return pot;
+}
+
 /* the same traversal evaluated in double with correctly rounded reciprocal square roots (inputs widened from
  * float): the float path's rounding error is measured against it on the same tree, grouping and MAC inputs */
 double orc_compute_gravity_f64(const uint64_t* leaves, int numLeaves, const unsigned* layout, const float* centers,
@@ -1158,6 +1176,77 @@ double orc_compute_gravity_f64(const uint64_t* leaves, int numLeaves, const unsi
     return computeGravity<double, true>(o, c.data(), q.data(), layout, xd.data(), yd.data(), zd.data(), hd.data(),
                                   md.data(), box, float(G), numShells, groupSize, ugrav, ax, ay, az, nullptr, 0,
                                   nullptr);
+}
+
+/* computeGroupSplits (cstone/traversal/groups_gpu.cu:60-151, groups_gpu.cuh:29-270) for 64-target windows in
+ * float: returns the number of groups G and writes G+1 boundaries to groups (capacity entries), or -(G+1) when
+ * the capacity is too small. Windows are processed one after the other, target by target. */
+long orc_group_splits(unsigned first, unsigned last, const float* x, const float* y, const float* z, const float* h,
+                      const uint64_t* leaves, int numLeaves, const unsigned* layout, const float* limits,
+                      float tolFactor, float extentRatio, unsigned* groups, long capacity)
+{
+    const unsigned W = 64;
+    auto box         = makeBox<float>(limits, nullptr);
+    const float inv  = 1.0f / std::max(std::max(box.len[0], box.len[1]), box.len[2]);
+    std::vector<unsigned> out;
+    for (unsigned ws = first; ws < last; ws += W)
+    {
+        float P[64][4], volMin = 1.0f, edge[3] = {1.0f, 1.0f, 1.0f};
+        for (unsigned l = 0; l < W; ++l)
+        {
+            unsigned i = std::min(ws + l, last - 1);
+            // the leaf holding particle i: layout[leaf] <= i < layout[leaf + 1]
+            int leaf = int(std::upper_bound(layout, layout + numLeaves, i) - layout) - 1;
+            uint64_t k0 = leaves[leaf], k1 = leaves[leaf + 1];
+            unsigned level = kMaxLevel - unsigned(63 - clz64(k1 - k0)) / 3u;
+            // cell edge in the unit box (half-unit length 2^-22 on every axis) from the mixed-dimension cell box
+            unsigned c[3];
+            float sz[3] = {0, 0, 0};
+            if (decodeHilbertMixD(k0, box.bits, c[0], c[1], c[2]))
+                for (int d = 0; d < 3; ++d)
+                    sz[d] = float(1u << std::min(box.bits[d], kMaxLevel - level)) * (1.0f / float(1u << 22));
+            float vol = 8.0f * sz[0] * sz[1] * sz[2];
+            if (vol > 0.0f) volMin = std::min(vol, volMin);
+            for (int d = 0; d < 3; ++d)
+                edge[d] = std::min(edge[d], 2.0f * sz[d]);
+            P[l][0] = x[i] * inv, P[l][1] = y[i] * inv, P[l][2] = z[i] * inv;
+            P[l][3] = h ? 2.0f * h[i] * inv : 1.0f;
+        }
+        float crit = std::cbrt(volMin) * tolFactor, crit2 = crit * crit;
+        uint64_t mask = 0;
+        for (unsigned l = 0; l + 1 < W; ++l)
+        {
+            float dx = P[l + 1][0] - P[l][0], dy = P[l + 1][1] - P[l][1], dz = P[l + 1][2] - P[l][2];
+            float d2 = dx * dx + (dy * dy + dz * dz);
+            if (d2 > std::min(crit2, P[l][3] * P[l][3])) mask |= 1ull << l;
+        }
+        float ratio = 0.0f;
+        for (int d = 0; d < 3; ++d)
+        {
+            float lo = P[0][d], hi = P[0][d];
+            for (unsigned l = 1; l < W; ++l)
+                lo = std::min(lo, P[l][d]), hi = std::max(hi, P[l][d]);
+            float allowed = tolFactor * edge[d];
+            if (allowed > 0.0f) ratio = std::max(ratio, (hi - lo) / allowed);
+        }
+        if (extentRatio > 0.0f && ratio > extentRatio)
+        {
+            unsigned chunk = W;
+            while (chunk > 1 && float(chunk) * ratio > float(W) * extentRatio)
+                chunk /= 2;
+            unsigned inWindow = std::min(W, last - ws);
+            for (unsigned next = chunk; next < inWindow; next += chunk)
+                mask |= 1ull << (next - 1);
+        }
+        out.push_back(ws);
+        for (unsigned l = 0; l < W; ++l)
+            if (mask >> l & 1u) out.push_back(ws + l + 1);
+    }
+    out.push_back(last);
+    long G = long(out.size()) - 1;
+    if (long(out.size()) > capacity) return -long(out.size());
+    std::copy(out.begin(), out.end(), groups);
+    return G;
 }
 
 /* FP64 direct sum over all n sources for the listed targets (or all, if targets == NULL);

@@ -21,6 +21,10 @@ int upsweep(Context*, const float*, const float*, const float*, const float*, co
             const int32_t*, const int32_t*, const int32_t*, const int32_t*, const float*, float, float*, float*,
             double*, double*);
 int fixedGroups(Context*, uint32_t, uint32_t, unsigned, uint32_t*, int*);
+int groupSplits(Context* ctx, uint32_t first, uint32_t last, const float* x, const float* y, const float* z,
+                const float* h, const uint64_t* leaves, int numLeaves, const uint32_t* layout, const float* limits,
+                unsigned groupSize, float tolFactor, float extentRatio, uint32_t* groups, size_t capacity,
+                int* numGroups);
 int ewald(Context*, const double*, const double*, uint32_t, uint32_t, const float*, const float*, const float*,
           const float*, const float*, float, const double*, float*, float*, float*, float*, double*);
 int directSum(Context*, const float*, const float*, const float*, const float*, const float*, size_t,
@@ -217,6 +221,16 @@ int ryo_fixed_groups(ryo_ctx* h, uint32_t first, uint32_t last, unsigned group_s
     CTX_OR_FAIL(h);
     if (!groups || !num_groups) return setError(ctx, RYO_ERR_ARG, "groups: null pointer");
     return fixedGroups(ctx, first, last, group_size, groups, num_groups);
+}
+
+int ryo_group_splits_f32(ryo_ctx* hc, uint32_t first, uint32_t last, const float* x, const float* y, const float* z,
+                         const float* h, const uint64_t* leaves, int num_leaves, const uint32_t* layout,
+                         const float* limits, unsigned group_size, float tol_factor, float extent_ratio,
+                         uint32_t* groups, size_t capacity, int* num_groups)
+{
+    CTX_OR_FAIL(hc);
+    return groupSplits(ctx, first, last, x, y, z, h, leaves, num_leaves, layout, limits, group_size, tol_factor,
+                       extent_ratio, groups, capacity, num_groups);
 }
 
 int ryo_traverse_f32(ryo_ctx* h, int num_groups, const uint32_t* gs, const uint32_t* ge, int init_node,

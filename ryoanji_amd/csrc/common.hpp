@@ -158,6 +158,7 @@ enum ScratchSlot : int
     kSlotGroups,
     kSlotEwald,
     kSlotMisc,
+    kSlotSplits,
     kNumSlots
 };
 

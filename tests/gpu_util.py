@@ -110,12 +110,16 @@ def gpu_upsweep(ctx, x, y, z, m, layout, tree, limits, inv_theta):
 
 
 def gpu_traverse(ctx, x, y, z, m, h, tree, layout, centers, mp, group_size, G=1.0, num_shells=0, box_l=None,
-                 init_node=0, stack_cap=None, want_pot=True):
+                 init_node=0, stack_cap=None, want_pot=True, bounds=None):
+    """group_size-target groups, or the group boundaries `bounds` (G+1 entries) when given"""
     import torch
 
     n = x.size
-    ngroups = (n + group_size - 1) // group_size
-    groups = dev(np.minimum(np.arange(ngroups + 1, dtype=np.int64) * group_size, n).astype(np.uint32))
+    if bounds is None:
+        ngroups = (n + group_size - 1) // group_size
+        bounds = np.minimum(np.arange(ngroups + 1, dtype=np.int64) * group_size, n)
+    ngroups = len(bounds) - 1
+    groups = dev(np.asarray(bounds).astype(np.uint32))
     arrs = [dev(a) for a in (x, y, z, m, h)]
     child, i2l = dev(tree["childOffsets"]), dev(tree["internalToLeaf"])
     lay = dev(layout)
@@ -131,6 +135,21 @@ def gpu_traverse(ctx, x, y, z, m, h, tree, layout, centers, mp, group_size, G=1.
     else:
         ctx.call("ryo_traverse_f32_stackcap", stack_cap, ngroups, *common)
     return pot.value, host(p), host(ax), host(ay), host(az), ctx.stats()
+
+
+def gpu_group_splits(ctx, first, last, x, y, z, h, leaves, layout, limits, tol_factor=2.0, extent_ratio=8.0,
+                     group_size=64, capacity=None):
+    """ryo_group_splits_f32 on device copies of the inputs -> group boundaries (host)"""
+    cap = capacity if capacity is not None else int(last - first) + 2
+    arrs = [dev(np.ascontiguousarray(a, np.float32)) for a in (x, y, z)]
+    hh = None if h is None else dev(np.ascontiguousarray(h, np.float32))
+    lv, lay = dev(np.ascontiguousarray(leaves, np.uint64)), dev(np.ascontiguousarray(layout, np.uint32))
+    out = dev(np.zeros(cap, np.uint32))
+    G = C.c_int()
+    ctx.call("ryo_group_splits_f32", int(first), int(last), *(R.ptr(a) for a in arrs), R.ptr(hh), R.ptr(lv),
+             len(leaves) - 1, R.ptr(lay), R.host_f32(limits), group_size, float(tol_factor), float(extent_ratio),
+             R.ptr(out), cap, C.byref(G))
+    return host(out, np.uint32)[:G.value + 1]
 
 
 def rel_err(ax, ay, az, bx, by, bz):
