@@ -239,6 +239,34 @@ int ryo_system_limits(ryo_system* sys, float* limits);
 /* phase times (ms) of the last build+gravity: {keys, sort, permute, leaves, link, upsweep, traverse, ewald} */
 int ryo_system_phase_ms(ryo_system* sys, double* ms);
 
+/* ---------------------------------------------------------------- batches of systems */
+
+/* Many particle systems packed one after the other and solved together: every stage (keys, sort, leaves,
+ * linked octree, FP64 upsweep, groups, traversal) runs once for all of them, driven by per-system offset
+ * tables, with one host round trip per solve. Each system gets exactly the keys, tree, multipoles and forces it
+ * gets when solved on its own (GroupView offsets, cstone/traversal/groups.hpp:28-34, are the reference's
+ * per-batch table; the reference solves one system per call). Open boundaries. */
+typedef struct ryo_batch ryo_batch;
+int ryo_batch_create(ryo_ctx* ctx, size_t max_particles, int max_systems, ryo_batch** batch);
+int ryo_batch_destroy(ryo_batch* batch);
+/* counts[S] particles per system (each >= 1); x, y, z, m, h: the systems packed one after the other (device or
+ * host memory, copied); limits[6 S] the box of each system (cubic or not) */
+int ryo_batch_load(ryo_batch* batch, int num_systems, const uint32_t* counts, const float* x, const float* y,
+                   const float* z, const float* m, const float* h, const float* limits);
+/* one full pass for every system; theta[S] and G[S] per system (host; NULL: prm's value), pot[S] (host, may be
+ * NULL) receives each system's 0.5 G sum(m phi). prm: bucket_size and group_size (<= 64). */
+int ryo_batch_solve(ryo_batch* batch, const ryo_params* prm, const float* theta, const float* G, double* pot);
+/* device arrays, packed like the input but each system in its SFC order: out[0..4] = x, y, z, m, h,
+ * out[5..8] = p, ax, ay, az; *order = index into the loaded (packed) arrays */
+int ryo_batch_results(ryo_batch* batch, float** out, int32_t** order);
+/* host copies of the offset tables (num_systems + 1 entries each; NULL skips): particles P, leaf slots LO (each
+ * system's L leaves and its terminator), nodes NO, parents PO, groups GO */
+int ryo_batch_offsets(ryo_batch* batch, uint32_t* P, uint32_t* LO, uint32_t* NO, uint32_t* PO, uint32_t* GO);
+/* device pointers of the trees, indices local to each system's slice: {sorted keys, leaves, layout, counts,
+ * prefixes, childOffsets, parents, internalToLeaf, leafToInternal, levelRange (23 per system), centers,
+ * multipoles, centers64, multipoles64} */
+int ryo_batch_tree(ryo_batch* batch, void** arrays);
+
 /* ---------------------------------------------------------------- serving (no reference counterpart) */
 
 /* A served request: a particle system (host or device arrays) that runs `steps` leapfrog steps. */

@@ -115,6 +115,13 @@ SIGNATURES = {
     "ryo_system_info": (_I, [_P, _P]),
     "ryo_system_tree": (_I, [_P, _P]),
     "ryo_system_phase_ms": (_I, [_P, _P]),
+    "ryo_batch_create": (_I, [_P, _SZ, _I, C.POINTER(_P)]),
+    "ryo_batch_destroy": (_I, [_P]),
+    "ryo_batch_load": (_I, [_P, _I, _P, _P, _P, _P, _P, _P, _P]),
+    "ryo_batch_solve": (_I, [_P, C.POINTER(Params), _P, _P, _P]),
+    "ryo_batch_results": (_I, [_P, _P, _P]),
+    "ryo_batch_offsets": (_I, [_P, _P, _P, _P, _P, _P]),
+    "ryo_batch_tree": (_I, [_P, _P]),
 }
 # entry points that are not part of the public header (test hooks)
 EXTRA_SIGNATURES = {
@@ -305,6 +312,101 @@ class System:
             self.close()
         except Exception:
             pass
+
+
+class Batch:
+    """Many particle systems solved together (ryo_batch): one pass over all of them per solve."""
+
+    def __init__(self, ctx: Context, max_particles: int, max_systems: int):
+        self.ctx = ctx
+        self.h = _P()
+        if not ctx.h:
+            raise RyoError(RYO_ERR_ARG, "batch: the context is closed")
+        ctx._check(lib().ryo_batch_create(ctx.h, int(max_particles), int(max_systems), C.byref(self.h)))
+        ctx._systems.add(self)
+        self.S = 0
+
+    def _check(self, rc):
+        return self.ctx._check(rc)
+
+    def load(self, systems):
+        """systems: objects with x, y, z, m, h arrays (numpy float32) and limits[6]"""
+        import numpy as np
+
+        counts = np.array([s.n for s in systems], np.uint32)
+        cat = [np.ascontiguousarray(np.concatenate([getattr(s, k) for s in systems]), np.float32)
+               for k in ("x", "y", "z", "m", "h")]
+        lim = np.ascontiguousarray(np.concatenate([np.asarray(s.limits, np.float32) for s in systems]), np.float32)
+        self._keep = (counts, cat, lim)
+        self._check(lib().ryo_batch_load(self.h, len(systems), counts.ctypes.data, *(a.ctypes.data for a in cat),
+                                         lim.ctypes.data))
+        self.S = len(systems)
+
+    def solve(self, params: Params, theta=None, G=None):
+        import numpy as np
+
+        th = None if theta is None else np.ascontiguousarray(theta, np.float32)
+        g = None if G is None else np.ascontiguousarray(G, np.float32)
+        pot = np.zeros(self.S, np.float64)
+        self._check(lib().ryo_batch_solve(self.h, C.byref(params), None if th is None else th.ctypes.data,
+                                          None if g is None else g.ctypes.data, pot.ctypes.data))
+        return pot
+
+    def offsets(self):
+        import numpy as np
+
+        t = [np.zeros(self.S + 1, np.uint32) for _ in range(5)]
+        self._check(lib().ryo_batch_offsets(self.h, *(a.ctypes.data for a in t)))
+        return dict(zip(("P", "LO", "NO", "PO", "GO"), t))
+
+    def results(self):
+        """device copies (torch): x, y, z, m, h, p, ax, ay, az (packed, SFC order per system) and the order"""
+        import torch
+
+        ptrs = (_P * 9)()
+        order = _P()
+        self._check(lib().ryo_batch_results(self.h, ptrs, C.byref(order)))
+        n = int(self.offsets()["P"][-1])
+        out = [_tensor_from(ptrs[k], n, torch.float32) for k in range(9)]
+        return out, _tensor_from(order.value, n, torch.int32)
+
+    def tree(self):
+        """device copies (torch) of the packed tree arrays (local indices per system)"""
+        import torch
+
+        o = self.offsets()
+        slots, nodes, par = int(o["LO"][-1]), int(o["NO"][-1]), int(o["PO"][-1])
+        ptrs = (_P * 14)()
+        self._check(lib().ryo_batch_tree(self.h, ptrs))
+        n = int(o["P"][-1])
+        spec = [("keys", n, torch.int64), ("leaves", slots, torch.int64), ("layout", slots, torch.int32),
+                ("counts", slots, torch.int32), ("prefixes", nodes, torch.int64), ("childOffsets", nodes, torch.int32),
+                ("parents", par, torch.int32), ("internalToLeaf", nodes, torch.int32),
+                ("leafToInternal", nodes, torch.int32), ("levelRange", 23 * self.S, torch.int32),
+                ("centers", 4 * nodes, torch.float32), ("multipoles", 8 * nodes, torch.float32),
+                ("centers64", 4 * nodes, torch.float64), ("multipoles64", 8 * nodes, torch.float64)]
+        return {name: _tensor_from(ptrs[k], cnt, dt) for k, (name, cnt, dt) in enumerate(spec)}
+
+    def close(self):
+        if self.h:
+            lib().ryo_batch_destroy(self.h)
+            self.h = _P()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def _tensor_from(p, count, dtype):
+    """Copy `count` elements at device pointer p into a new torch tensor."""
+    import torch
+
+    out = torch.empty(count, dtype=dtype, device="cuda")
+    if count:
+        _memcpy_d2d(out.data_ptr(), p, out.element_size() * count)
+    return out
 
 
 def _memcpy_d2d(dst, src, nbytes):

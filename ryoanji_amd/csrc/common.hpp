@@ -113,6 +113,33 @@ __device__ inline uint32_t blockExclusiveScan256(uint32_t v, uint32_t* sWave /* 
     return pre + inc - v;
 }
 
+/* ------------------------------------------------------------------ batches of systems */
+
+// Offset tables of a batch of S particle systems packed one after the other (all on the device). System s owns
+// particles [P[s], P[s+1]), leaf slots [LO[s], LO[s+1]) (its L_s leaves plus the terminator slot), tree nodes
+// [NO[s], NO[s+1]) of which NIO[s+1] - NIO[s] are internal, parent slots from PO[s], level ranges LR[23 s ..]
+// (local node indices), per-level node prefixes LP[(S+1) l + s], target groups [GO[s], GO[s+1]) and leaf
+// tiles [TS[s], TS[s+1]). Every index stored inside a system's arrays is local to that system.
+struct BatchView
+{
+    int S;
+    const uint32_t *P, *LO, *NO, *NIO, *PO, *LP, *GO, *TS;
+    const int32_t* LR;
+};
+
+// the segment holding element i: the largest s with off[s] <= i (empty segments are skipped)
+__device__ inline int segmentOf(const uint32_t* off, int S, uint32_t i)
+{
+    int lo = 0, hi = S; // answer in [0, S)
+    while (hi - lo > 1)
+    {
+        int mid = (lo + hi) >> 1;
+        if (off[mid] <= i) lo = mid;
+        else hi = mid;
+    }
+    return lo;
+}
+
 /* ------------------------------------------------------------------ host side */
 
 struct Context

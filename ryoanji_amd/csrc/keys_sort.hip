@@ -33,12 +33,6 @@ __device__ __forceinline__ uint64_t spread3(unsigned v)
     return x;
 }
 
-struct KeyBox
-{
-    float lo[3];
-    float m[3];  // 2^bits[d] / L_d
-    Bits3 bits;  // key bits per axis (21 on every axis of a cubic box)
-};
 
 // sfc.hpp:161-207: cell indices clamped to 2^bits - 1, then the Hilbert (mixed-dimension when bits differ) or
 // Morton key
@@ -262,10 +256,12 @@ size_t sortTempBytes(size_t n)
     return 256 * sizeof(uint32_t) * (2 * kPasses) + size_t(kDigits) * numTiles * sizeof(uint32_t) + 256;
 }
 
-int sortPairs(Context* ctx, uint64_t* keys, int32_t* vals, size_t n, uint64_t* keyBuf, int32_t* valBuf, void* tmp,
-              size_t tmpBytes)
+// stable LSD sort on the lowest `digits` 8-bit digits of the keys; the result is in keys/vals
+int sortPairsDigits(Context* ctx, uint64_t* keys, int32_t* vals, size_t n, uint64_t* keyBuf, int32_t* valBuf,
+                    void* tmp, size_t tmpBytes, int digits)
 {
     if (n == 0) return RYO_OK;
+    if (digits < 1 || digits > kPasses) return setError(ctx, RYO_ERR_ARG, "sort: %d digits", digits);
     if (tmpBytes < sortTempBytes(n))
         return setError(ctx, RYO_ERR_ARG, "sort: temp storage too small (%zu < %zu)", tmpBytes, sortTempBytes(n));
     const int numTiles = int((n + kSortTile - 1) / kSortTile);
@@ -280,7 +276,7 @@ int sortPairs(Context* ctx, uint64_t* keys, int32_t* vals, size_t n, uint64_t* k
     uint64_t* kout = keyBuf;
     int32_t* vin = vals;
     int32_t* vout = valBuf;
-    for (int p = 0; p < kPasses; ++p)
+    for (int p = 0; p < digits; ++p)
     {
         int shift = 8 * p;
         tileHistKernel<<<numTiles, kSortThreads, 0, s>>>(kin, n, shift, tileHist, numTiles);
@@ -289,6 +285,105 @@ int sortPairs(Context* ctx, uint64_t* keys, int32_t* vals, size_t n, uint64_t* k
         std::swap(kin, kout);
         std::swap(vin, vout);
     }
+    if (kin != keys)
+    {
+        RYO_HIP(ctx, hipMemcpyAsync(keys, kin, n * sizeof(uint64_t), hipMemcpyDeviceToDevice, s));
+        RYO_HIP(ctx, hipMemcpyAsync(vals, vin, n * sizeof(int32_t), hipMemcpyDeviceToDevice, s));
+    }
+    RYO_LAUNCH_CHECK(ctx);
+    return RYO_OK;
+}
+
+int sortPairs(Context* ctx, uint64_t* keys, int32_t* vals, size_t n, uint64_t* keyBuf, int32_t* valBuf, void* tmp,
+              size_t tmpBytes)
+{
+    return sortPairsDigits(ctx, keys, vals, n, keyBuf, valBuf, tmp, tmpBytes, kPasses);
+}
+
+/* ============================================================================ batches of systems */
+
+// keys of packed systems: particle i of system s uses that system's box; vals = particle index
+__global__ __launch_bounds__(256) void batchKeysKernel(const float* __restrict__ x, const float* __restrict__ y,
+                                                       const float* __restrict__ z, size_t n,
+                                                       const uint32_t* __restrict__ P, int S,
+                                                       const KeyBox* __restrict__ boxes, uint64_t* __restrict__ keys,
+                                                       int32_t* __restrict__ vals)
+{
+    for (size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += size_t(gridDim.x) * blockDim.x)
+    {
+        const KeyBox& box = boxes[segmentOf(P, S, uint32_t(i))];
+        int ix = min(cellIndex(x[i], box.lo[0], box.m[0]), int((1u << box.bits.b[0]) - 1));
+        int iy = min(cellIndex(y[i], box.lo[1], box.m[1]), int((1u << box.bits.b[1]) - 1));
+        int iz = min(cellIndex(z[i], box.lo[2], box.m[2]), int((1u << box.bits.b[2]) - 1));
+        keys[i] = hilbertMixD(ix, iy, iz, box.bits);
+        vals[i] = int32_t(i);
+    }
+}
+
+// the segment of each (sorted) value as the key of a second, stable pass; pos = current position
+__global__ void segmentKeysKernel(const int32_t* __restrict__ vals, size_t n, const uint32_t* __restrict__ off, int S,
+                                  uint64_t* __restrict__ segKeys, int32_t* __restrict__ pos)
+{
+    for (size_t j = size_t(blockIdx.x) * blockDim.x + threadIdx.x; j < n; j += size_t(gridDim.x) * blockDim.x)
+    {
+        segKeys[j] = uint64_t(segmentOf(off, S, uint32_t(vals[j])));
+        pos[j]     = int32_t(j);
+    }
+}
+
+__global__ void applyPosKernel(const int32_t* __restrict__ pos, size_t n, const uint64_t* __restrict__ kin,
+                               const int32_t* __restrict__ vin, uint64_t* __restrict__ kout, int32_t* __restrict__ vout)
+{
+    for (size_t j = size_t(blockIdx.x) * blockDim.x + threadIdx.x; j < n; j += size_t(gridDim.x) * blockDim.x)
+    {
+        kout[j] = kin[pos[j]];
+        vout[j] = vin[pos[j]];
+    }
+}
+
+KeyBox makeKeyBox(const float* limits)
+{
+    KeyBox b;
+    b.bits = boxBits(limits);
+    for (int d = 0; d < 3; ++d)
+    {
+        float lo = limits[2 * d], hi = limits[2 * d + 1];
+        b.lo[d]  = lo;
+        b.m[d]   = float(1u << b.bits.b[d]) * (1.0f / (hi - lo));
+    }
+    return b;
+}
+
+int batchKeys(Context* ctx, const float* x, const float* y, const float* z, size_t n, const uint32_t* P, int S,
+              const KeyBox* boxes, uint64_t* keys, int32_t* vals)
+{
+    if (n == 0) return RYO_OK;
+    batchKeysKernel<<<gridFor(n, 256, ctx->numCUs * 16), 256, 0, ctx->stream>>>(x, y, z, n, P, S, boxes, keys,
+                                                                                 vals);
+    RYO_LAUNCH_CHECK(ctx);
+    return RYO_OK;
+}
+
+/* stable sort of (keys, vals) grouped by segment: a full sort on the keys, then a stable pass on the segment of
+ * each value (segment = the s with off[s] <= val < off[s+1]); keys ascend inside each segment and equal keys
+ * keep their order. keyBuf/valBuf/segKeys/segBuf/pos/posBuf: n entries each; tmp: sortTempBytes(n). */
+int segmentedSortPairs(Context* ctx, uint64_t* keys, int32_t* vals, size_t n, const uint32_t* off, int S,
+                       uint64_t* keyBuf, int32_t* valBuf, uint64_t* segKeys, uint64_t* segBuf, int32_t* pos,
+                       int32_t* posBuf, void* tmp, size_t tmpBytes)
+{
+    if (n == 0) return RYO_OK;
+    int rc = sortPairs(ctx, keys, vals, n, keyBuf, valBuf, tmp, tmpBytes);
+    if (rc) return rc;
+    if (S <= 1) return RYO_OK;
+    hipStream_t s = ctx->stream;
+    const unsigned grid = gridFor(n, 256, ctx->numCUs * 16);
+    segmentKeysKernel<<<grid, 256, 0, s>>>(vals, n, off, S, segKeys, pos);
+    const int digits = S <= 256 ? 1 : (S <= 65536 ? 2 : 3);
+    rc = sortPairsDigits(ctx, segKeys, pos, n, segBuf, posBuf, tmp, tmpBytes, digits);
+    if (rc) return rc;
+    applyPosKernel<<<grid, 256, 0, s>>>(pos, n, keys, vals, keyBuf, valBuf);
+    RYO_HIP(ctx, hipMemcpyAsync(keys, keyBuf, n * sizeof(uint64_t), hipMemcpyDeviceToDevice, s));
+    RYO_HIP(ctx, hipMemcpyAsync(vals, valBuf, n * sizeof(int32_t), hipMemcpyDeviceToDevice, s));
     RYO_LAUNCH_CHECK(ctx);
     return RYO_OK;
 }

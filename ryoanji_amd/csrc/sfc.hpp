@@ -303,6 +303,23 @@ __host__ __device__ inline IBox3 cellIBox(uint64_t key, unsigned level, const Bi
 namespace ryo
 {
 
+// a box as the key kernels use it
+struct KeyBox
+{
+    float lo[3];
+    float m[3]; // 2^bits[d] / L_d
+    Bits3 bits; // key bits per axis (21 on every axis of a cubic box)
+};
+
+// a box as the MAC computation uses it
+struct MacBox
+{
+    double lo[3];
+    double half[3]; // half of the smallest cell edge per axis: 0.5 L_d / 2^bits[d]
+    Bits3 bits;     // key bits per axis
+    float invTheta;
+};
+
 // per-axis key bits of a box given as {xmin, xmax, ymin, ymax, zmin, zmax} (Box<float>::computeBoxDimBits and
 // getBoxDimBits, box.hpp:160-198, in float as the reference's Box<float> computes them)
 inline Bits3 boxBits(const float* limits)

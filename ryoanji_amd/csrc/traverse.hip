@@ -60,6 +60,17 @@ struct Acc
     float p, x, y, z;
 };
 
+// the source tree and source particles a walk reads (one system's slices in a batch)
+struct TreeRef
+{
+    const int32_t* child;
+    const int32_t* i2l;
+    const uint32_t* layout;
+    const float4* centers;
+    const float4* mp;
+    const float *xs, *ys, *zs, *ms, *hs;
+};
+
 // v_rsq_f32 (about 1 ulp) without the denormal rescaling that rsqrtf adds; the arguments are squared distances
 // clamped by the softening, never denormal for the systems served here
 __device__ __forceinline__ float rsq(float x) { return __builtin_amdgcn_rsqf(x); }
@@ -149,7 +160,7 @@ struct Target
 
 // apply all bodies of leaf ranges flagged in `dmask` (lane -> [rb, re)) to the wave's targets
 template<bool UH>
-__device__ __forceinline__ unsigned applyLeaves(const TravArgs& a, uint64_t dmask, uint32_t rb, uint32_t re, Acc& acc,
+__device__ __forceinline__ unsigned applyLeaves(const TreeRef& a, uint64_t dmask, uint32_t rb, uint32_t re, Acc& acc,
                                                 const Target& t)
 {
     unsigned cnt = 0;
@@ -178,7 +189,7 @@ __device__ __forceinline__ unsigned applyLeaves(const TravArgs& a, uint64_t dmas
 }
 
 // apply the quadrupoles of the cells flagged in `amask` (lane -> cell) in ascending lane order
-__device__ __forceinline__ unsigned applyCells(const TravArgs& a, uint64_t amask, int cell, Acc& acc, const Target& t)
+__device__ __forceinline__ unsigned applyCells(const TreeRef& a, uint64_t amask, int cell, Acc& acc, const Target& t)
 {
     unsigned cnt = __popcll(amask);
     const float* cbase = reinterpret_cast<const float*>(a.centers);
@@ -197,8 +208,8 @@ __device__ __forceinline__ unsigned applyCells(const TravArgs& a, uint64_t amask
 }
 
 template<bool UH>
-__device__ WalkOut walk(const TravArgs& a, int* stack, const Target& t, float tcx, float tcy, float tcz, float tsx,
-                        float tsy, float tsz, Acc& acc)
+__device__ WalkOut walk(const TravArgs& ta, const TreeRef& a, int* stack, const Target& t, float tcx, float tcy,
+                        float tcz, float tsx, float tsy, float tsz, Acc& acc)
 {
     const int lane    = threadIdx.x & 63;
     const uint64_t lt = (1ull << lane) - 1ull;
@@ -206,7 +217,7 @@ __device__ WalkOut walk(const TravArgs& a, int* stack, const Target& t, float tc
     int sp = 0;
 
     const int rootChild = a.child[0];
-    if (a.initNode == 0 || rootChild == 0)
+    if (ta.initNode == 0 || rootChild == 0)
     {
         float4 c   = a.centers[0];
         bool close = macClose(c, tcx, tcy, tcz, tsx, tsy, tsz);
@@ -259,7 +270,7 @@ __device__ WalkOut walk(const TravArgs& a, int* stack, const Target& t, float tc
 
         uint64_t sm = __ballot(split);
         int cnt     = __popcll(sm);
-        if (sp + cnt > a.stackCap)
+        if (sp + cnt > ta.stackCap)
         {
             o.overflow = true;
             o.maxStack = max(o.maxStack, unsigned(sp + cnt));
@@ -315,8 +326,18 @@ __global__ void travInitKernel(unsigned* ctl)
     if (i < 64) ctl[i] = (i == kCtlHRange) ? ~0u : 0u;
 }
 
+// per-group source tree of a batch of systems: group g belongs to system gSys[g]; the tree arrays and the source
+// particles are the system's slices, the targets are global particle indices
+struct TravBatch
+{
+    BatchView bv;
+    const int32_t* gSys;
+    const float* G; // per system
+};
+
 // 64 * kTravWaves threads, at least 6 waves per SIMD (<= 80 VGPRs; more waves would spill)
-__global__ __launch_bounds__(64 * kTravWaves, 6) void traverseKernel(TravArgs a)
+template<bool Batched>
+__global__ __launch_bounds__(64 * kTravWaves, 6) void traverseKernelT(TravArgs a, TravBatch tb)
 {
     extern __shared__ int stackMem[];
     const int w    = threadIdx.x >> 6;
@@ -362,6 +383,16 @@ __global__ __launch_bounds__(64 * kTravWaves, 6) void traverseKernel(TravArgs a)
         g = __shfl(g, 0);
         if (g == ~0u) return;
 
+        TreeRef tr{a.child, a.i2l, a.layout, a.centers, a.mp, a.xs, a.ys, a.zs, a.ms, a.hs};
+        float Gs = a.G;
+        if constexpr (Batched)
+        {
+            const int s    = tb.gSys[g];
+            const size_t p = tb.bv.P[s], no = tb.bv.NO[s];
+            tr = TreeRef{a.child + no, a.i2l + no, a.layout + tb.bv.LO[s], a.centers + no, a.mp + 2 * no,
+                         a.xs + p, a.ys + p, a.zs + p, a.ms + p, a.hs + p};
+            Gs = tb.G[s];
+        }
         const uint32_t b = a.gStart[g], e = a.gEnd[g];
         const bool valid = b + lane < e;
         const uint32_t ti = valid ? b + lane : e - 1;
@@ -382,8 +413,9 @@ __global__ __launch_bounds__(64 * kTravWaves, 6) void traverseKernel(TravArgs a)
                 {
                     float sx = ix * a.boxL[0], sy = iy * a.boxL[1], sz = iz * a.boxL[2];
                     Target ts{t.x + sx, t.y + sy, t.z + sz, t.h, t.hij2u};
-                    WalkOut o = uniformH ? walk<true>(a, stack, ts, tcx + sx, tcy + sy, tcz + sz, tsx, tsy, tsz, acc)
-                                         : walk<false>(a, stack, ts, tcx + sx, tcy + sy, tcz + sz, tsx, tsy, tsz, acc);
+                    WalkOut o = uniformH
+                                    ? walk<true>(a, tr, stack, ts, tcx + sx, tcy + sy, tcz + sz, tsx, tsy, tsz, acc)
+                                    : walk<false>(a, tr, stack, ts, tcx + sx, tcy + sy, tcz + sz, tsx, tsy, tsz, acc);
                     nM2P += o.m2p;
                     nP2P += o.p2p;
                     maxStack = max(maxStack, o.maxStack);
@@ -407,10 +439,10 @@ __global__ __launch_bounds__(64 * kTravWaves, 6) void traverseKernel(TravArgs a)
         if (valid)
         {
             float m = a.mt[ti];
-            if (a.p) a.p[ti] += a.G * m * acc.p;
-            a.ax[ti] += a.G * acc.x;
-            a.ay[ti] += a.G * acc.y;
-            a.az[ti] += a.G * acc.z;
+            if (a.p) a.p[ti] += Gs * m * acc.p;
+            a.ax[ti] += Gs * acc.x;
+            a.ay[ti] += Gs * acc.y;
+            a.az[ti] += Gs * acc.z;
             myPot = double(m) * double(acc.p);
         }
         myPot = waveSum(myPot);
@@ -528,7 +560,7 @@ int traverse(Context* ctx, TravArgs a, double* totalPot)
                                                                        int(2 * G));
         int blocks = std::min(ctx->numCUs * perCU, int(iceil(G, kTravWaves)));
         if (ctx->timing && pass == 0) RYO_HIP(ctx, hipEventRecord(ctx->ev[0], s));
-        traverseKernel<<<blocks, 64 * kTravWaves, lds, s>>>(a);
+        traverseKernelT<false><<<blocks, 64 * kTravWaves, lds, s>>>(a, TravBatch{});
         if (ctx->timing && pass == 0) RYO_HIP(ctx, hipEventRecord(ctx->ev[1], s));
         RYO_LAUNCH_CHECK(ctx);
         sumDeterministic(s, a.groupPot, a.numGroups, partials, potSum);
@@ -578,6 +610,108 @@ int traverse(Context* ctx, TravArgs a, double* totalPot)
         ctx->hStats[k] = h[k];
     if (totalPot) *totalPot = 0.5 * double(a.G) * pot;
     return RYO_OK;
+}
+
+// per-system FP64 potential: block s sums the partials of groups [GO[s], GO[s+1]) in a fixed order (Neumaier)
+__global__ __launch_bounds__(256) void systemPotKernel(const double* __restrict__ groupPot, BatchView bv,
+                                                       const float* __restrict__ G, double* __restrict__ out)
+{
+    __shared__ double ss[256], sc[256];
+    const int sys = blockIdx.x;
+    double sum = 0, c = 0;
+    for (uint32_t i = bv.GO[sys] + threadIdx.x; i < bv.GO[sys + 1]; i += 256)
+    {
+        double x = groupPot[i], t = sum + x;
+        c += fabs(sum) >= fabs(x) ? (sum - t) + x : (x - t) + sum;
+        sum = t;
+    }
+    ss[threadIdx.x] = sum;
+    sc[threadIdx.x] = c;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1)
+    {
+        if (int(threadIdx.x) < o)
+        {
+            double x = ss[threadIdx.x + o], y = ss[threadIdx.x];
+            double t = x + y;
+            sc[threadIdx.x] += sc[threadIdx.x + o] + (fabs(y) >= fabs(x) ? (y - t) + x : (x - t) + y);
+            ss[threadIdx.x] = t;
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[sys] = 0.5 * double(G[sys]) * (ss[0] + sc[0]);
+}
+
+/* traversal of every system of a batch in one launch: a holds the batch-wide arrays (targets = sources = the
+ * packed particles, tree arrays, groups of all systems), tb the offset tables; sysPot (device, S entries)
+ * receives each system's 0.5 G sum(m phi). Open boundaries; the LIFO can never overflow at full depth. */
+int traverseBatch(Context* ctx, TravArgs a, const TravBatch& tb, double* sysPot)
+{
+    hipStream_t s = ctx->stream;
+    for (int k = 0; k < 6; ++k)
+        ctx->hStats[k] = 0;
+    if (a.numGroups <= 0) return RYO_OK;
+    a.stackCap      = kMaxStack;
+    const int perCU = ctx->travBlocksPerCU > 0 ? ctx->travBlocksPerCU : kDefaultBlocks;
+    a.ranges        = ctx->travRanges > 0 ? std::min(ctx->travRanges, kXcds) : kXcds;
+    const size_t lds = size_t(kTravWaves) * kMaxStack * sizeof(int);
+    unsigned* ctl             = reinterpret_cast<unsigned*>(ctx->dStats);
+    unsigned long long* stats = ctx->dStats + 8;
+    a.work   = ctl;
+    a.stats  = stats;
+    a.hRange = ctl + kCtlHRange;
+    travInitKernel<<<1, 64, 0, s>>>(ctl);
+    if (a.hCount > 0)
+        hRangeKernel<<<std::min(iceil(a.hCount, 256), 2048u), 256, 0, s>>>(a.ht, a.hCount, nullptr, 0,
+                                                                            ctl + kCtlHRange);
+    const size_t G = size_t(a.numGroups);
+    auto* buf      = static_cast<char*>(ctx->getScratch(kSlotTraverse, 12 * G + 256));
+    if (!buf) return RYO_ERR_HIP;
+    a.groupPot = reinterpret_cast<double*>(buf);
+    a.failed   = reinterpret_cast<uint32_t*>(buf + 8 * G);
+    zeroKernelU<<<std::min(iceil(2 * G, 256), 4096u), 256, 0, s>>>(reinterpret_cast<unsigned*>(a.groupPot),
+                                                                   int(2 * G));
+    int blocks = std::min(ctx->numCUs * perCU, int(iceil(G, kTravWaves)));
+    if (ctx->timing) RYO_HIP(ctx, hipEventRecord(ctx->ev[0], s));
+    traverseKernelT<true><<<blocks, 64 * kTravWaves, lds, s>>>(a, tb);
+    if (ctx->timing) RYO_HIP(ctx, hipEventRecord(ctx->ev[1], s));
+    RYO_LAUNCH_CHECK(ctx);
+    systemPotKernel<<<tb.bv.S, 256, 0, s>>>(a.groupPot, tb.bv, tb.G, sysPot);
+    unsigned long long h[7] = {0, 0, 0, 0, 0, 0, 0};
+    RYO_HIP(ctx, hipMemcpyAsync(h, stats, sizeof(h), hipMemcpyDeviceToHost, s));
+    RYO_HIP(ctx, hipStreamSynchronize(s));
+    if (ctx->timing)
+    {
+        float ms = 0;
+        RYO_HIP(ctx, hipEventElapsedTime(&ms, ctx->ev[0], ctx->ev[1]));
+        ctx->lastTraverseMs = ms;
+    }
+    if ((h[5] & 0xffffffffu) != 0 || h[6] != 0)
+        return setError(ctx, RYO_ERR_TRAVERSAL, "batch traversal: %llu groups failed", h[5] & 0xffffffffu);
+    for (int k = 0; k < 5; ++k)
+        ctx->hStats[k] = h[k];
+    return RYO_OK;
+}
+
+int batchTraverse(Context* ctx, int numGroups, const uint32_t* gStart, const uint32_t* gEnd, const int32_t* gSys,
+                  const BatchView& bv, const float* G, const float* x, const float* y, const float* z,
+                  const float* m, const float* h, size_t n, const int32_t* child, const int32_t* i2l,
+                  const uint32_t* layout, const float* centers, const float* mp, float* p, float* ax, float* ay,
+                  float* az, double* sysPot, int initNode)
+{
+    TravArgs a{};
+    a.numGroups = numGroups;
+    a.gStart = gStart, a.gEnd = gEnd, a.initNode = initNode;
+    a.xt = x, a.yt = y, a.zt = z, a.mt = m, a.ht = h;
+    a.xs = x, a.ys = y, a.zs = z, a.ms = m, a.hs = h;
+    a.child = child, a.i2l = i2l, a.layout = layout;
+    a.centers = reinterpret_cast<const float4*>(centers);
+    a.mp      = reinterpret_cast<const float4*>(mp);
+    a.G = 1.0f, a.numShells = 0;
+    a.p = p, a.ax = ax, a.ay = ay, a.az = az;
+    a.hCount = n;
+    TravBatch tb{bv, gSys, G};
+    return traverseBatch(ctx, a, tb, sysPot);
 }
 
 int traverseEntry(Context* ctx, int numGroups, const uint32_t* gs, const uint32_t* ge, int initNode, const float* xt,

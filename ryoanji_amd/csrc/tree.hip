@@ -164,18 +164,28 @@ __global__ __launch_bounds__(256) void scanBlocksKernel(uint32_t* __restrict__ s
     if (threadIdx.x == 0) sums[numBlocks] = carry;
 }
 
+// leaves of the tile starting at particle t0, written from position `base` on
+__device__ void leafEmitTile(const uint64_t* __restrict__ k, long n, int B, long t0, uint32_t base,
+                             uint64_t* __restrict__ leaves, uint32_t* __restrict__ layout, long capacity,
+                             LeafTileSmem& s);
+
 __global__ __launch_bounds__(kLeafThreads) void leafEmitKernel(const uint64_t* __restrict__ k, long n, int B,
                                                               const uint32_t* __restrict__ blockOffsets,
                                                               uint64_t* __restrict__ leaves,
                                                               uint32_t* __restrict__ layout, long capacity)
 {
     __shared__ LeafTileSmem s;
+    leafEmitTile(k, n, B, long(blockIdx.x) * kLeafTile, blockOffsets[blockIdx.x], leaves, layout, capacity, s);
+}
+
+__device__ void leafEmitTile(const uint64_t* __restrict__ k, long n, int B, long t0, uint32_t base,
+                             uint64_t* __restrict__ leaves, uint32_t* __restrict__ layout, long capacity,
+                             LeafTileSmem& s)
+{
     int em[4];
-    long t0 = long(blockIdx.x) * kLeafTile;
     leafTile(k, n, B, t0, s, em);
     // thread-contiguous order: item r of thread t is particle t0 + r*256 + t -> scan in that order
     // (the emission order must follow particle order, so scan over q = r*256 + t by rows)
-    uint32_t base = blockOffsets[blockIdx.x];
     for (int r = 0; r < kLeafTile / kLeafThreads; ++r)
     {
         uint32_t tot;
@@ -388,6 +398,246 @@ int linkOctree(Context* ctx, const uint64_t* leaves, int L, uint64_t* prefixes, 
                                     hipMemcpyDeviceToHost, s));
         RYO_HIP(ctx, hipStreamSynchronize(s));
     }
+    return RYO_OK;
+}
+
+/* ============================================================================ batches of systems */
+
+// Each kernel below runs the single-system computation of one stage for every system of a batch in one launch:
+// a block (or thread, or node) finds its system in the offset tables and works on that system's slice with
+// local indices, so every system gets exactly the arrays it would get on its own.
+
+// leaf tiles: block b is tile b - TS[s] of system s; the last tile of a system also counts its terminator slot
+__global__ __launch_bounds__(kLeafThreads) void batchLeafCountKernel(const uint64_t* __restrict__ sk, BatchView bv,
+                                                                    int B, uint32_t* __restrict__ blockSums)
+{
+    __shared__ LeafTileSmem s;
+    const int sys = segmentOf(bv.TS, bv.S, blockIdx.x);
+    const uint32_t t = blockIdx.x - bv.TS[sys];
+    const uint32_t p0 = bv.P[sys];
+    int em[4];
+    leafTile(sk + p0, long(bv.P[sys + 1] - p0), B, long(t) * kLeafTile, s, em);
+    uint32_t v = em[0] + em[1] + em[2] + em[3], tot;
+    blockExclusiveScan256(v, s.sw, &tot);
+    if (threadIdx.x == 0) blockSums[blockIdx.x] = tot + (t + 1 == bv.TS[sys + 1] - bv.TS[sys] ? 1u : 0u);
+}
+
+__global__ __launch_bounds__(kLeafThreads) void batchLeafEmitKernel(const uint64_t* __restrict__ sk, BatchView bv,
+                                                                   int B, const uint32_t* __restrict__ blockOffsets,
+                                                                   uint64_t* __restrict__ leaves,
+                                                                   uint32_t* __restrict__ layout, long capacity)
+{
+    __shared__ LeafTileSmem s;
+    const int sys = segmentOf(bv.TS, bv.S, blockIdx.x);
+    const uint32_t t = blockIdx.x - bv.TS[sys];
+    const uint32_t p0 = bv.P[sys];
+    leafEmitTile(sk + p0, long(bv.P[sys + 1] - p0), B, long(t) * kLeafTile, blockOffsets[blockIdx.x], leaves, layout,
+                 capacity, s);
+}
+
+// per-system leaf-slot offsets LO[s] = offset of the system's first tile (LO[S] = the total)
+__global__ void batchLeafOffsetsKernel(BatchView bv, const uint32_t* __restrict__ blockOffsets, int numBlocks,
+                                       uint32_t* __restrict__ LO)
+{
+    for (int s = blockIdx.x * blockDim.x + threadIdx.x; s <= bv.S; s += gridDim.x * blockDim.x)
+        LO[s] = s < bv.S ? blockOffsets[bv.TS[s]] : blockOffsets[numBlocks];
+}
+
+// particle counts per leaf slot (0 at the terminator slots)
+__global__ void batchLeafCountsKernel(const uint32_t* __restrict__ layout, BatchView bv, uint32_t* __restrict__ counts)
+{
+    const uint32_t total = bv.LO[bv.S];
+    for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < total; j += gridDim.x * blockDim.x)
+    {
+        int s     = segmentOf(bv.LO, bv.S, j);
+        counts[j] = j + 1 == bv.LO[s + 1] ? 0u : layout[j + 1] - layout[j];
+    }
+}
+
+// unsorted node layout of every system; the values are global node slots (NO[s] + local unsorted index)
+__global__ void batchUnsortedLayoutKernel(const uint64_t* __restrict__ leaves, BatchView bv,
+                                          uint64_t* __restrict__ pre, int32_t* __restrict__ idx)
+{
+    const uint32_t total = bv.LO[bv.S];
+    for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < total; j += gridDim.x * blockDim.x)
+    {
+        const int s    = segmentOf(bv.LO, bv.S, j);
+        const int L    = int(bv.LO[s + 1] - bv.LO[s]) - 1;
+        const int t    = int(j - bv.LO[s]);
+        if (t >= L) continue; // terminator slot
+        const int nI   = int(bv.NIO[s + 1] - bv.NIO[s]);
+        const int base = int(bv.NO[s]);
+        const uint64_t* lv = leaves + bv.LO[s];
+        uint64_t key = lv[t], next = lv[t + 1];
+        unsigned lvl = treeLevel(next - key);
+        pre[base + t + nI] = encodePlaceholder(key, 3 * lvl);
+        idx[base + t + nI] = base + t + nI;
+        int pl = commonPrefix(key, next);
+        if (pl % 3 == 0 && t < L - 1)
+        {
+            unsigned level = unsigned(pl) / 3u;
+            int w          = 0;
+            for (unsigned l = 1; l <= level + 1; ++l)
+                w += digitWeight(int(octalDigit(key, l)));
+            int o = (t + w) / 7;
+            if (o >= 0 && o < nI)
+            {
+                pre[base + o] = encodePlaceholder(key, pl);
+                idx[base + o] = base + o;
+            }
+        }
+    }
+}
+
+// sorted slot j of system s: l2i (local) and i2l (local, minus the internal count) from the global unsorted slot
+__global__ void batchInvertKernel(const int32_t* __restrict__ slot, BatchView bv, int32_t* __restrict__ i2l,
+                                  int32_t* __restrict__ l2i)
+{
+    const uint32_t total = bv.NO[bv.S];
+    for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < total; j += gridDim.x * blockDim.x)
+    {
+        const int s  = segmentOf(bv.NO, bv.S, j);
+        const int nn = int(bv.NO[s + 1] - bv.NO[s]);
+        const int nI = int(bv.NIO[s + 1] - bv.NIO[s]);
+        const int u  = slot[j] - int(bv.NO[s]);
+        if (u >= 0 && u < nn) l2i[bv.NO[s] + u] = int(j - bv.NO[s]);
+        i2l[j] = u - nI;
+    }
+}
+
+// level ranges of system blockIdx.x (local node indices)
+__global__ void batchLevelRangeKernel(const uint64_t* __restrict__ pre, BatchView bv, int32_t* __restrict__ LR)
+{
+    const int s = blockIdx.x, l = threadIdx.x;
+    const int nn = int(bv.NO[s + 1] - bv.NO[s]);
+    const uint64_t* p = pre + bv.NO[s];
+    if (l <= int(kMaxLevel)) LR[23 * s + l] = lowerBound(p, 0, nn, encodePlaceholder(0, 3 * l));
+    if (l == int(kMaxLevel) + 1) LR[23 * s + l] = nn;
+}
+
+// per level l (block l): LP[(S+1) l + s] = number of level-l nodes in systems before s
+__global__ __launch_bounds__(256) void batchLevelPrefixKernel(const int32_t* __restrict__ LR, int S,
+                                                              uint32_t* __restrict__ LP)
+{
+    __shared__ uint32_t sw[4];
+    const int l    = blockIdx.x;
+    uint32_t carry = 0;
+    for (int s0 = 0; s0 < S; s0 += 256)
+    {
+        int s       = s0 + threadIdx.x;
+        uint32_t v  = s < S ? uint32_t(LR[23 * s + l + 1] - LR[23 * s + l]) : 0u, tot;
+        uint32_t ex = blockExclusiveScan256(v, sw, &tot);
+        if (s < S) LP[(S + 1) * l + s] = carry + ex;
+        carry += tot;
+    }
+    if (threadIdx.x == 0) LP[(S + 1) * l + S] = carry;
+}
+
+__global__ void batchLinkKernel(const uint64_t* __restrict__ pre, BatchView bv, const int32_t* __restrict__ l2i,
+                                int32_t* __restrict__ child, int32_t* __restrict__ parents)
+{
+    const uint32_t total = bv.NIO[bv.S];
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < total; i += gridDim.x * blockDim.x)
+    {
+        const int s       = segmentOf(bv.NIO, bv.S, i);
+        const int base    = int(bv.NO[s]);
+        const int nn      = int(bv.NO[s + 1] - bv.NO[s]);
+        const int32_t* lr = bv.LR + 23 * s;
+        int a = l2i[base + int(i - bv.NIO[s])];
+        if (a < 0 || a >= nn) continue;
+        const uint64_t* p = pre + base;
+        uint64_t prefix = p[a];
+        unsigned pl     = prefixLength(prefix);
+        unsigned level  = pl / 3;
+        if (level >= kMaxLevel) continue;
+        uint64_t cp = encodePlaceholder(decodePlaceholder(prefix), int(pl) + 3);
+        int lo = lr[level + 1], hi = lr[level + 2];
+        int c  = lowerBound(p, lo, hi, cp);
+        if (c != hi && p[c] == cp)
+        {
+            child[base + a]                  = c;
+            parents[bv.PO[s] + (c - 1) / 8] = a;
+        }
+    }
+}
+
+// fixed groups of every system: group g of system s covers its particles [64 (g - GO[s]), ...), as global
+// particle indices, and remembers its system
+__global__ void batchGroupsKernel(BatchView bv, unsigned gs, uint32_t* __restrict__ gStart,
+                                  uint32_t* __restrict__ gEnd, int32_t* __restrict__ gSys)
+{
+    const uint32_t total = bv.GO[bv.S];
+    for (uint32_t g = blockIdx.x * blockDim.x + threadIdx.x; g < total; g += gridDim.x * blockDim.x)
+    {
+        const int s = segmentOf(bv.GO, bv.S, g);
+        uint32_t b  = bv.P[s] + (g - bv.GO[s]) * gs;
+        gStart[g]   = b;
+        gEnd[g]     = min(b + gs, bv.P[s + 1]);
+        gSys[g]     = s;
+    }
+}
+
+// host side of the batch stages (bv's device tables must hold what each stage reads)
+int batchLeavesCount(Context* ctx, const uint64_t* sk, const BatchView& bv, int numTiles, int B,
+                     uint32_t* blockSums, uint32_t* LO)
+{
+    if (B < 1 || B > kMaxBucket) return setError(ctx, RYO_ERR_UNSUPPORTED, "batch leaves: bucket size %d", B);
+    hipStream_t s = ctx->stream;
+    batchLeafCountKernel<<<numTiles, kLeafThreads, 0, s>>>(sk, bv, B, blockSums);
+    scanBlocksKernel<<<1, 256, 0, s>>>(blockSums, numTiles);
+    batchLeafOffsetsKernel<<<iceil(bv.S + 1, 256), 256, 0, s>>>(bv, blockSums, numTiles, LO);
+    RYO_LAUNCH_CHECK(ctx);
+    return RYO_OK;
+}
+
+int batchLeavesEmit(Context* ctx, const uint64_t* sk, const BatchView& bv, int numTiles, int B,
+                    const uint32_t* blockSums, uint64_t* leaves, uint32_t* layout, uint32_t* counts,
+                    uint32_t totalSlots)
+{
+    hipStream_t s = ctx->stream;
+    batchLeafEmitKernel<<<numTiles, kLeafThreads, 0, s>>>(sk, bv, B, blockSums, leaves, layout, long(totalSlots));
+    batchLeafCountsKernel<<<std::max(1u, std::min(iceil(totalSlots, 256), unsigned(ctx->numCUs * 8))), 256, 0, s>>>(
+        layout, bv, counts);
+    RYO_LAUNCH_CHECK(ctx);
+    return RYO_OK;
+}
+
+int batchUnsortedLayout(Context* ctx, const uint64_t* leaves, const BatchView& bv, uint32_t totalSlots,
+                        uint64_t* pre, int32_t* idx)
+{
+    batchUnsortedLayoutKernel<<<std::max(1u, std::min(iceil(totalSlots, 256), unsigned(ctx->numCUs * 8))), 256, 0,
+                                ctx->stream>>>(leaves, bv, pre, idx);
+    RYO_LAUNCH_CHECK(ctx);
+    return RYO_OK;
+}
+
+/* after the segmented sort of the node prefixes (slot = global unsorted slot of each sorted node): i2l, l2i,
+ * level ranges LR (S x 23), per-level prefixes LP (22 x (S+1)), child links and parents */
+int batchLinkFinish(Context* ctx, const uint64_t* pre, const int32_t* slot, const BatchView& bv, uint32_t totalNodes,
+                    uint32_t totalInternal, uint32_t totalParents, int32_t* i2l, int32_t* l2i, int32_t* LR,
+                    uint32_t* LP, int32_t* child, int32_t* parents)
+{
+    hipStream_t s = ctx->stream;
+    const unsigned g = std::max(1u, std::min(iceil(totalNodes, 256), unsigned(ctx->numCUs * 8)));
+    batchInvertKernel<<<g, 256, 0, s>>>(slot, bv, i2l, l2i);
+    batchLevelRangeKernel<<<bv.S, 64, 0, s>>>(pre, bv, LR);
+    batchLevelPrefixKernel<<<int(kMaxLevel) + 1, 256, 0, s>>>(LR, bv.S, LP);
+    zeroI32Kernel<<<g, 256, 0, s>>>(child, int(totalNodes) + 1);
+    zeroI32Kernel<<<std::max(1u, iceil(totalParents, 256)), 256, 0, s>>>(parents, int(std::max(1u, totalParents)));
+    if (totalInternal > 0)
+        batchLinkKernel<<<std::max(1u, std::min(iceil(totalInternal, 256), unsigned(ctx->numCUs * 8))), 256, 0, s>>>(
+            pre, bv, l2i, child, parents);
+    RYO_LAUNCH_CHECK(ctx);
+    return RYO_OK;
+}
+
+int batchGroups(Context* ctx, const BatchView& bv, uint32_t totalGroups, unsigned gs, uint32_t* gStart,
+                uint32_t* gEnd, int32_t* gSys)
+{
+    if (totalGroups == 0) return RYO_OK;
+    batchGroupsKernel<<<std::min(iceil(totalGroups, 256), unsigned(ctx->numCUs * 8)), 256, 0, ctx->stream>>>(
+        bv, gs, gStart, gEnd, gSys);
+    RYO_LAUNCH_CHECK(ctx);
     return RYO_OK;
 }
 

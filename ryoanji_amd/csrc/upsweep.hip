@@ -17,14 +17,6 @@
 namespace ryo
 {
 
-struct MacBox
-{
-    double lo[3];
-    double half[3]; // half of the smallest cell edge per axis: 0.5 L_d / 2^bits[d]
-    Bits3 bits;     // key bits per axis
-    float invTheta;
-};
-
 // squared vector-MAC radius (2 max(size)/theta + |com - geometric center|)^2 of the node with `prefix`, the
 // geometric box from the node's integer cell box (hilbertIBox / centerAndSize, mixed-dimension aware)
 __device__ inline double macSquared(uint64_t prefix, double cx, double cy, double cz, const MacBox& b)
@@ -84,12 +76,10 @@ __device__ inline double red8(double v)
     return v;
 }
 
-// leaves: 8 lanes per leaf reduce its particles (|m|-weighted center, then the quadrupole about it)
-__global__ __launch_bounds__(256) void leafUpsweepKernel(UpsweepArgs a)
+// leaves: 8 lanes per leaf reduce its particles (|m|-weighted center, then the quadrupole about it); all 8 lanes
+// of a group call with the same leaf and `ok`
+__device__ inline void leafUpsweepOne(const UpsweepArgs& a, int leaf, bool ok, int sub)
 {
-    const int leaf = int((size_t(blockIdx.x) * blockDim.x + threadIdx.x) >> 3);
-    const int sub  = threadIdx.x & 7;
-    const bool ok  = leaf < a.numLeaves;
     uint32_t b = 0, e = 0;
     if (ok) b = a.layout[leaf], e = a.layout[leaf + 1];
 
@@ -132,18 +122,18 @@ __global__ __launch_bounds__(256) void leafUpsweepKernel(UpsweepArgs a)
     if (ok && sub == 0) storeNode(a, a.leafNode[leaf], c, q);
 }
 
-// internal nodes of one level: lane k of an 8-lane group holds child k; center and M2M by 8-lane reductions
-__global__ __launch_bounds__(256) void levelUpsweepKernel(UpsweepArgs a, const int32_t* __restrict__ levelRange,
-                                                          int level)
+__global__ __launch_bounds__(256) void leafUpsweepKernel(UpsweepArgs a)
 {
-    const int first = levelRange[level], last = levelRange[level + 1];
-    const int sub   = threadIdx.x & 7;
-    const int groupsPerGrid = int(gridDim.x * blockDim.x) >> 3;
-    for (int node = first + int((size_t(blockIdx.x) * blockDim.x + threadIdx.x) >> 3); node < last;
-         node += groupsPerGrid)
+    const int leaf = int((size_t(blockIdx.x) * blockDim.x + threadIdx.x) >> 3);
+    leafUpsweepOne(a, leaf, leaf < a.numLeaves, threadIdx.x & 7);
+}
+
+// one internal node: lane k of the 8-lane group holds child k; center and M2M by 8-lane reductions
+__device__ inline void levelNodeOne(const UpsweepArgs& a, int node, int sub)
+{
     {
         const int ch = a.child[node]; // uniform over the 8 lanes of the node
-        if (ch == 0) continue;
+        if (ch == 0) return;
         const double* s  = a.c64 + 4 * size_t(ch + sub);
         const double* cq = a.q64 + 8 * size_t(ch + sub);
         double sx = s[0], sy = s[1], sz = s[2], w = fabs(s[3]);
@@ -165,6 +155,105 @@ __global__ __launch_bounds__(256) void levelUpsweepKernel(UpsweepArgs a, const i
         pq[7] = red8(cq[7] + ml * r2);
         if (sub == 0) storeNode(a, node, pc, pq);
     }
+}
+
+// internal nodes of one level
+__global__ __launch_bounds__(256) void levelUpsweepKernel(UpsweepArgs a, const int32_t* __restrict__ levelRange,
+                                                          int level)
+{
+    const int first = levelRange[level], last = levelRange[level + 1];
+    const int groupsPerGrid = int(gridDim.x * blockDim.x) >> 3;
+    for (int node = first + int((size_t(blockIdx.x) * blockDim.x + threadIdx.x) >> 3); node < last;
+         node += groupsPerGrid)
+        levelNodeOne(a, node, threadIdx.x & 7);
+}
+
+/* ---------------------------------------------------------------- batches of systems */
+
+// the arguments of system s: the batch-wide arrays offset to the system's slices (local indices inside)
+__device__ inline UpsweepArgs systemArgs(const UpsweepArgs& g, const BatchView& bv, const MacBox* boxes, int s)
+{
+    UpsweepArgs a  = g;
+    const size_t p = bv.P[s], no = bv.NO[s];
+    a.x = g.x + p, a.y = g.y + p, a.z = g.z + p, a.m = g.m + p;
+    a.layout    = g.layout + bv.LO[s];
+    a.numLeaves = int(bv.LO[s + 1] - bv.LO[s]) - 1;
+    a.leafNode  = g.leafNode + no + (bv.NIO[s + 1] - bv.NIO[s]);
+    a.parents   = g.parents + bv.PO[s];
+    a.child     = g.child + no;
+    a.prefixes  = g.prefixes + no;
+    a.c32 = g.c32 + no, a.q32 = g.q32 + 8 * no, a.c64 = g.c64 + 4 * no, a.q64 = g.q64 + 8 * no;
+    a.box = boxes[s];
+    return a;
+}
+
+// 8 lanes per leaf slot of every system (terminator slots idle)
+__global__ __launch_bounds__(256) void batchLeafUpsweepKernel(UpsweepArgs g, BatchView bv,
+                                                              const MacBox* __restrict__ boxes)
+{
+    const uint32_t j = uint32_t((size_t(blockIdx.x) * blockDim.x + threadIdx.x) >> 3);
+    const uint32_t total = bv.LO[bv.S];
+    const int s = segmentOf(bv.LO, bv.S, j < total ? j : total - 1);
+    UpsweepArgs a = systemArgs(g, bv, boxes, s);
+    const int t   = int(j - bv.LO[s]);
+    leafUpsweepOne(a, t, j < total && t < a.numLeaves, threadIdx.x & 7);
+}
+
+// level `level` of every system: node group q is node LR[s][level] + q - LP[level][s] of system s
+__global__ __launch_bounds__(256) void batchLevelUpsweepKernel(UpsweepArgs g, BatchView bv,
+                                                               const MacBox* __restrict__ boxes, int level)
+{
+    const uint32_t* lp   = bv.LP + size_t(bv.S + 1) * level;
+    const uint32_t total = lp[bv.S];
+    const uint32_t per   = uint32_t(gridDim.x * blockDim.x) >> 3;
+    for (uint32_t q = uint32_t((size_t(blockIdx.x) * blockDim.x + threadIdx.x) >> 3); q < total; q += per)
+    {
+        const int s   = segmentOf(lp, bv.S, q);
+        UpsweepArgs a = systemArgs(g, bv, boxes, s);
+        levelNodeOne(a, bv.LR[23 * s + level] + int(q - lp[s]), threadIdx.x & 7);
+    }
+}
+
+/* upsweep of every system of a batch: g holds the batch-wide arrays (particles, leaf slots, node arrays),
+ * boxes the per-system MAC boxes (device) */
+int batchUpsweep(Context* ctx, const UpsweepArgs& g, const BatchView& bv, const MacBox* boxes, uint32_t totalSlots,
+                 uint32_t totalNodes)
+{
+    hipStream_t s = ctx->stream;
+    if (totalSlots == 0) return RYO_OK;
+    batchLeafUpsweepKernel<<<iceil(size_t(totalSlots) * 8, 256), 256, 0, s>>>(g, bv, boxes);
+    const unsigned blocks = std::max(1u, std::min(iceil(size_t(totalNodes) * 8, 256), unsigned(ctx->numCUs * 4)));
+    for (int l = int(kMaxLevel) - 1; l >= 0; --l)
+        batchLevelUpsweepKernel<<<blocks, 256, 0, s>>>(g, bv, boxes, l);
+    RYO_LAUNCH_CHECK(ctx);
+    return RYO_OK;
+}
+
+int batchUpsweepEntry(Context* ctx, const float* x, const float* y, const float* z, const float* m,
+                      const uint32_t* layout, const uint64_t* prefixes, const int32_t* child, const int32_t* parents,
+                      const int32_t* l2i, float* centers, float* mp, double* c64, double* q64, const BatchView& bv,
+                      const MacBox* boxes, uint32_t totalSlots, uint32_t totalNodes)
+{
+    UpsweepArgs g{};
+    g.x = x, g.y = y, g.z = z, g.m = m;
+    g.layout = layout, g.numLeaves = 0;
+    g.leafNode = l2i, g.parents = parents, g.child = child, g.prefixes = prefixes;
+    g.c32 = reinterpret_cast<float4*>(centers), g.q32 = mp, g.c64 = c64, g.q64 = q64;
+    return batchUpsweep(ctx, g, bv, boxes, totalSlots, totalNodes);
+}
+
+MacBox makeMacBox(const float* limits, float invTheta)
+{
+    MacBox b;
+    b.bits = boxBits(limits);
+    for (int d = 0; d < 3; ++d)
+    {
+        double lo = limits[2 * d], hi = limits[2 * d + 1];
+        b.lo[d]   = lo;
+        b.half[d] = 0.5 * (hi - lo) / double(1u << b.bits.b[d]);
+    }
+    b.invTheta = invTheta;
+    return b;
 }
 
 int upsweep(Context* ctx, const float* x, const float* y, const float* z, const float* m, const uint32_t* layout,

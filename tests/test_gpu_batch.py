@@ -1,0 +1,94 @@
+"""Batches of systems (ryo_batch): every system of a batch gets exactly the keys, tree, multipoles and forces
+it gets when solved on its own (ryo_system), including single-leaf, tiny and non-cubic systems."""
+
+import numpy as np
+import pytest
+
+import ryoanji_amd as R
+from ryoanji_amd import workloads as W
+
+from gpu_util import dev, host
+
+pytestmark = pytest.mark.gpu
+
+
+def _systems():
+    out = []
+    kinds = [lambda n, s: W.uniform_cube(n, seed=s), lambda n, s: W.plummer(n, seed=s),
+             lambda n, s: W.uniform_box(n, seed=s, extents=(2.0, 1.0, 0.25)),
+             lambda n, s: W.plummer_disk(n, seed=s, flattening=0.3)]
+    sizes = [1, 2, 63, 64, 65, 1000, 4095, 4096, 4097, 20000, 3, 777, 1025, 9000, 64, 12345]
+    for i, n in enumerate(sizes):
+        out.append(kinds[i % len(kinds)](n, 100 + i))
+    return out
+
+
+def _alone(ctx, s, prm):
+    sysm = R.System(ctx, s.n)
+    arrs = [dev(a) for a in s.arrays()]
+    sysm.load(*arrs, s.limits)
+    sysm.build(prm)
+    pot = sysm.gravity(prm)
+    res = [host(t) for t in sysm.results()]
+    tr = {k: host(v) for k, v in sysm.tree().items()}
+    info = sysm.info()
+    sysm.close()
+    return pot, res, tr, info
+
+
+@pytest.mark.parametrize("group_size", [64, 32])
+def test_batch_equals_systems_alone(ctx, group_size):
+    systems = _systems()
+    thetas = np.array([0.4, 0.5, 0.7, 0.6] * 4, np.float32)[: len(systems)]
+    Gs = np.linspace(0.5, 2.0, len(systems)).astype(np.float32)
+    b = R.Batch(ctx, sum(s.n for s in systems), len(systems))
+    b.load(systems)
+    prm = R.default_params(group_size=group_size)
+    pots = b.solve(prm, theta=thetas, G=Gs)
+    off = b.offsets()
+    (x, y, z, m, h, p, ax, ay, az), order = b.results()
+    arrs = [host(t) for t in (p, ax, ay, az)]
+    order = host(order)
+    tr = {k: host(v) for k, v in b.tree().items()}
+    for s, sysd in enumerate(systems):
+        prm_s = R.default_params(theta=float(thetas[s]), G=float(Gs[s]), group_size=group_size)
+        pot, res, t1, info = _alone(ctx, sysd, prm_s)
+        p0, p1 = int(off["P"][s]), int(off["P"][s + 1])
+        l0, l1 = int(off["LO"][s]), int(off["LO"][s + 1])
+        n0, n1 = int(off["NO"][s]), int(off["NO"][s + 1])
+        q0 = int(off["PO"][s])
+        L, nn = info["numLeaves"], info["numNodes"]
+        assert l1 - l0 == L + 1 and n1 - n0 == nn, s
+        assert np.array_equal(tr["keys"][p0:p1], t1["keys"]), s
+        assert np.array_equal(tr["leaves"][l0:l1], t1["leaves"]), s
+        assert np.array_equal(tr["layout"][l0:l1], t1["layout"]), s
+        assert np.array_equal(tr["counts"][l0:l1 - 1], t1["counts"]), s
+        for k in ("prefixes", "childOffsets", "internalToLeaf", "leafToInternal"):
+            assert np.array_equal(tr[k][n0:n1], t1[k]), (s, k)
+        assert np.array_equal(tr["parents"][q0:q0 + t1["parents"].size], t1["parents"]), s
+        assert np.array_equal(tr["levelRange"][23 * s:23 * s + 23], t1["levelRange"]), s
+        for k, w in (("centers", 4), ("multipoles", 8), ("centers64", 4), ("multipoles64", 8)):
+            assert np.array_equal(tr[k][w * n0:w * n1], t1[k]), (s, k)
+        assert np.array_equal(order[p0:p1] - p0, res[4]), s
+        for k in range(4):
+            assert np.array_equal(arrs[k][p0:p1], res[k]), (s, k)
+        assert abs(pots[s] - pot) <= 1e-12 * abs(pot), (s, pots[s], pot)
+    b.close()
+
+
+def test_batch_reuse_and_capacity(ctx):
+    """a batch object serves batches of different composition; loading beyond its capacity is an error"""
+    b = R.Batch(ctx, 30000, 8)
+    a = [W.plummer(5000, seed=1), W.uniform_cube(7000, seed=2)]
+    c = [W.uniform_cube(100, seed=3), W.plummer(20000, seed=4), W.uniform_box(900, seed=5)]
+    prm = R.default_params()
+    b.load(a)
+    pa = b.solve(prm)
+    b.load(c)
+    pc = b.solve(prm)
+    b.load(a)
+    pa2 = b.solve(prm)
+    assert np.array_equal(pa, pa2) and pc.size == 3
+    with pytest.raises(R.RyoError):
+        b.load([W.plummer(40000, seed=6)])
+    b.close()
