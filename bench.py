@@ -181,6 +181,54 @@ def serving_bench(num=32, slots=8, seed=5):
     return out
 
 
+def batching_bench(classes=(12, 14, 16), total_exp=22, seed=9):
+    """Many small Plummer systems (2^total_exp particles in all, systems of 2^c particles for each size class c):
+    one batched solve (one launch sequence for all systems) against solving them one at a time on a resident
+    system; the batched launch configuration (group size, traversal blocks per CU, XCD ranges) is tuned per size
+    class by the autotuner. Inputs are resident on the device for both."""
+    import numpy as np
+    import torch
+
+    import ryoanji_amd as R
+    from ryoanji_amd import autotune as A
+    from ryoanji_amd import workloads as W
+
+    ctx = R.Context()
+    tuner = A.Autotuner(ctx)
+    rows = []
+    for c in classes:
+        n, K = 1 << c, 1 << (total_exp - c)
+        systems = [W.plummer(n, seed=seed * 100000 + i) for i in range(K)]
+        prm = R.default_params()
+        sysm = R.System(ctx, n)
+        devs = [[torch.from_numpy(a).cuda() for a in s.arrays()] for s in systems]
+        for rep in range(2):  # the first round warms up
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            for s, d in zip(systems, devs):
+                sysm.load(*d, s.limits)
+                sysm.build(prm)
+                sysm.gravity(prm)
+            torch.cuda.synchronize()
+            t_one = time.perf_counter() - t0
+        sysm.close()
+        del devs
+        b = R.Batch(ctx, n * K, K)
+        b.load(systems)
+        t_default = A.time_solve(b, prm)
+        best, timings = tuner.tune(systems, b)
+        t_best = min(t for _, t in timings)
+        b.close()
+        rows.append(dict(size_class=f"2^{c}", systems=K, particles=n * K,
+                         one_at_a_time_ms=t_one * 1e3, batched_default_ms=t_default * 1e3,
+                         batched_tuned_ms=t_best * 1e3,
+                         tuned={"group_size": best[0], "blocks_per_cu": best[1], "group_ranges": best[2]},
+                         particles_per_s_one_at_a_time=n * K / t_one, particles_per_s_batched=n * K / t_best,
+                         speedup=t_one / t_best))
+    ctx.close()
+    return rows
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -192,6 +240,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-accuracy", action="store_true")
     ap.add_argument("--no-serving", action="store_true")
+    ap.add_argument("--no-batching", action="store_true")
     ap.add_argument("--trav-stack", type=int, default=0, help="traversal LIFO entries per wave (0: default)")
     ap.add_argument("--trav-blocks", type=int, default=0, help="traversal blocks per CU (0: default)")
     ap.add_argument("--trav-ranges", type=int, default=0, help="traversal group ranges (0: one per XCD)")
@@ -315,6 +364,8 @@ def main():
     del dev
     if not args.no_serving:
         out["serving"] = serving_bench()
+    if not args.no_batching:
+        out["batching"] = batching_bench()
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(s)
     print(json.dumps(out))

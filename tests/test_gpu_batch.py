@@ -92,3 +92,22 @@ def test_batch_reuse_and_capacity(ctx):
     with pytest.raises(R.RyoError):
         b.load([W.plummer(40000, seed=6)])
     b.close()
+
+
+def test_autotuner_picks_a_candidate(ctx):
+    from ryoanji_amd import autotune as A
+
+    systems = [W.plummer(2048, seed=40 + i) for i in range(16)]
+    tuner = A.Autotuner(ctx, candidates=[(64, 6, 8), (32, 6, 8), (64, 4, 1)], reps=1)
+    best, timings = tuner.tune(systems)
+    assert best in [c for c, _ in timings] and len(timings) == 3
+    assert tuner.config_for(systems) == best
+    # the tuned configuration changes the launch, not the results of its group size
+    prm = tuner.apply(best)
+    b = R.Batch(ctx, sum(s.n for s in systems), len(systems))
+    b.load(systems)
+    pots = b.solve(prm)
+    tuner.reset()
+    pots2 = b.solve(prm)
+    assert np.array_equal(pots, pots2)
+    b.close()
