@@ -161,6 +161,13 @@ def serving_bench(num=32, slots=8, seed=5):
     out["batched_saturated"] = sS
     out["throughput_ratio"] = sS["particle_solves_per_s"] / sP["particle_solves_per_s"]
     out["throughput_ratio_vs_resident_one_slot"] = sS["particle_solves_per_s"] / s1["particle_solves_per_s"]
+    # precision report of the served results: every request's final accelerations against the FP64 direct sum
+    # (256 sampled targets each; a separate run, so the timings above carry no checking work)
+    racc = mixed.run(systems, params, accuracy_targets=256)
+    worst = {th: max(r["err_p99"] for r, t in zip(racc, params["theta"]) if t == th) for th in set(params["theta"])}
+    out["served_accuracy"] = {"targets_per_request": 256,
+                              "max_err_p99_by_theta": {f"{k:.1f}": v for k, v in sorted(worst.items())},
+                              "max_err_median": max(r["err_median"] for r in racc)}
     # latency under Poisson arrivals at fractions of the one-slot server's capacity
     rates = []
     cap_rps = s1["requests_per_s"]

@@ -239,6 +239,13 @@ int ryo_system_limits(ryo_system* sys, float* limits);
 /* phase times (ms) of the last build+gravity: {keys, sort, permute, leaves, link, upsweep, traverse, ewald} */
 int ryo_system_phase_ms(ryo_system* sys, double* ms);
 
+/* Accuracy of the last ryo_system_gravity of `sys` against the FP64 direct sum (P2P<Exact>,
+ * traversal_cpu.hpp:232-268, the reference's accuracy oracle) on num_targets targets drawn from the system
+ * (splitmix64 of seed, with replacement; over the same replica images for a periodic system, whose last gravity
+ * must not include the Ewald term). out[4] = {median, 99th percentile, maximum} of |a - a_ref| / |a_ref| and
+ * the relative error of the sampled potential sum. Synchronizes the stream. */
+int ryo_system_accuracy(ryo_system* sys, int num_targets, uint64_t seed, double* out);
+
 /* ---------------------------------------------------------------- batches of systems */
 
 /* Many particle systems packed one after the other and solved together: every stage (keys, sort, leaves,
@@ -281,12 +288,14 @@ typedef struct
     unsigned bucket_size; /* 0: 64 */
     int group_size;       /* 0: 64 */
     double arrival_ms;    /* relative to the start of ryo_server_run; requests sorted by arrival */
+    int accuracy_targets; /* > 0: check the final accelerations against the FP64 direct sum on this many targets */
 } ryo_request;
 
 typedef struct
 {
     double arrival_ms, admit_ms, finish_ms, total_pot;
     int status, slot;
+    double err_median, err_p99, err_max; /* ryo_system_accuracy of the final state, when requested (else 0) */
 } ryo_result;
 
 typedef struct ryo_server ryo_server;

@@ -61,6 +61,7 @@ class Request(C.Structure):
         ("bucket_size", C.c_uint),
         ("group_size", C.c_int),
         ("arrival_ms", C.c_double),
+        ("accuracy_targets", C.c_int),
     ]
 
 
@@ -68,7 +69,8 @@ class Result(C.Structure):
     """ryo_result: timing and outcome of one served request."""
 
     _fields_ = [("arrival_ms", C.c_double), ("admit_ms", C.c_double), ("finish_ms", C.c_double),
-                ("total_pot", C.c_double), ("status", C.c_int), ("slot", C.c_int)]
+                ("total_pot", C.c_double), ("status", C.c_int), ("slot", C.c_int),
+                ("err_median", C.c_double), ("err_p99", C.c_double), ("err_max", C.c_double)]
 
 
 # exported symbol -> (restype, argtypes)
@@ -115,6 +117,7 @@ SIGNATURES = {
     "ryo_system_info": (_I, [_P, _P]),
     "ryo_system_tree": (_I, [_P, _P]),
     "ryo_system_phase_ms": (_I, [_P, _P]),
+    "ryo_system_accuracy": (_I, [_P, _I, C.c_uint64, _P]),
     "ryo_batch_create": (_I, [_P, _SZ, _I, C.POINTER(_P)]),
     "ryo_batch_destroy": (_I, [_P]),
     "ryo_batch_load": (_I, [_P, _I, _P, _P, _P, _P, _P, _P, _P]),
@@ -251,6 +254,12 @@ class System:
         v = (C.c_int64 * 4)()
         self._check(lib().ryo_system_info(self.h, v))
         return dict(zip(("n", "numLeaves", "numNodes", "numGroups"), list(v)))
+
+    def accuracy(self, num_targets=4096, seed=1):
+        """errors of the last gravity call vs the FP64 direct sum on sampled targets"""
+        v = (C.c_double * 4)()
+        self._check(lib().ryo_system_accuracy(self.h, int(num_targets), int(seed), v))
+        return dict(zip(("err_median", "err_p99", "err_max", "pot_rel_err"), list(v)))
 
     def phase_ms(self):
         v = (C.c_double * 8)()

@@ -319,6 +319,8 @@ struct ryo_system
     uint32_t* groups = nullptr;
     int levelRangeHost[24] = {};
     bool haveAcc = false;     // accelerations of the current positions are in ax, ay, az
+    float lastG = 1.0f;       // G, replica shells and Ewald use of the last gravity call
+    int lastShells = 0, lastEwald = 0;
     bool integrating = false; // velocities are in use (leapfrog): the build permutes them too
     hipEvent_t ev[9] = {};
     double phaseMs[8] = {};
@@ -590,8 +592,72 @@ int ryo_system_gravity(ryo_system* s, const ryo_params* prm, double* totalPot)
         }
         pot += ep;
     }
-    s->haveAcc = true;
+    s->haveAcc    = true;
+    s->lastG      = prm->G;
+    s->lastShells = shells;
+    s->lastEwald  = (s->periodic && prm->ewald) ? 1 : 0;
     if (totalPot) *totalPot = pot;
+    return RYO_OK;
+}
+
+int ryo_system_accuracy(ryo_system* s, int num_targets, uint64_t seed, double* out)
+{
+    if (!s || !out || num_targets < 1) return RYO_ERR_ARG;
+    Context* ctx = s->ctx;
+    if (!s->haveAcc) return setError(ctx, RYO_ERR_ARG, "accuracy: no gravity results to check");
+    if (s->lastEwald)
+        return setError(ctx, RYO_ERR_UNSUPPORTED, "accuracy: the FP64 reference has no Ewald term (solve without it)");
+    const size_t n  = s->n;
+    const size_t nt = std::min(size_t(num_targets), n);
+    // targets: splitmix64 draws of the seed, with replacement
+    std::vector<int64_t> tg(nt);
+    for (size_t k = 0; k < nt; ++k)
+    {
+        uint64_t z = seed + (k + 1) * 0x9E3779B97F4A7C15ull;
+        z          = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+        z          = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+        tg[k]      = int64_t((z ^ (z >> 31)) % n);
+    }
+    const size_t refBytes = nt * 4 * sizeof(double), tgBytes = nt * sizeof(int64_t);
+    auto* buf = static_cast<char*>(ctx->getScratch(kSlotAccuracy, refBytes + tgBytes + nt * 4 * (sizeof(float) + 4) + 1024));
+    if (!buf) return RYO_ERR_HIP;
+    auto* ref  = reinterpret_cast<double*>(buf);
+    auto* dtg  = reinterpret_cast<int64_t*>(buf + refBytes);
+    auto* idx  = reinterpret_cast<int32_t*>(buf + refBytes + tgBytes);
+    auto* got  = reinterpret_cast<float*>(buf + refBytes + tgBytes + ((nt * 4 + 255) & ~size_t(255)));
+    std::vector<int32_t> ti(tg.begin(), tg.end());
+    hipStream_t st = ctx->stream;
+    RYO_HIP(ctx, hipMemcpyAsync(dtg, tg.data(), tgBytes, hipMemcpyHostToDevice, st));
+    RYO_HIP(ctx, hipMemcpyAsync(idx, ti.data(), nt * 4, hipMemcpyHostToDevice, st));
+    float** A = s->a[s->cur];
+    double boxL[3] = {double(s->limits[1] - s->limits[0]), double(s->limits[3] - s->limits[2]),
+                      double(s->limits[5] - s->limits[4])};
+    int rc = directSum(ctx, A[0], A[1], A[2], A[4], A[3], n, dtg, nt, double(s->lastG), s->lastShells ? boxL : nullptr,
+                       s->lastShells, ref);
+    if (rc) return rc;
+    const float* src[4] = {s->p, s->ax, s->ay, s->az};
+    float* dst[4]       = {got, got + nt, got + 2 * nt, got + 3 * nt};
+    if ((rc = gather(ctx, idx, nt, 4, src, dst))) return rc;
+    std::vector<double> hr(nt * 4);
+    std::vector<float> hg(nt * 4);
+    RYO_HIP(ctx, hipMemcpyAsync(hr.data(), ref, refBytes, hipMemcpyDeviceToHost, st));
+    RYO_HIP(ctx, hipMemcpyAsync(hg.data(), got, nt * 4 * sizeof(float), hipMemcpyDeviceToHost, st));
+    RYO_HIP(ctx, hipStreamSynchronize(st));
+    std::vector<double> e(nt);
+    double potRef = 0, potGot = 0;
+    for (size_t k = 0; k < nt; ++k)
+    {
+        double dx = hg[nt + k] - hr[4 * k + 1], dy = hg[2 * nt + k] - hr[4 * k + 2], dz = hg[3 * nt + k] - hr[4 * k + 3];
+        double r  = std::sqrt(hr[4 * k + 1] * hr[4 * k + 1] + hr[4 * k + 2] * hr[4 * k + 2] + hr[4 * k + 3] * hr[4 * k + 3]);
+        e[k]      = std::sqrt(dx * dx + dy * dy + dz * dz) / std::max(r, 1e-300);
+        potRef += hr[4 * k];
+        potGot += hg[k];
+    }
+    std::sort(e.begin(), e.end());
+    out[0] = e[nt / 2];
+    out[1] = e[std::min(nt - 1, size_t(0.99 * double(nt)))];
+    out[2] = e[nt - 1];
+    out[3] = std::abs(potGot - potRef) / std::max(std::abs(potRef), 1e-300);
     return RYO_OK;
 }
 

@@ -186,6 +186,7 @@ enum ScratchSlot : int
     kSlotEwald,
     kSlotMisc,
     kSlotSplits,
+    kSlotAccuracy,
     kNumSlots
 };
 

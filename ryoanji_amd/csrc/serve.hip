@@ -129,6 +129,8 @@ int ryo_server_run(ryo_server* s, int num, const ryo_request* req, ryo_result* r
             double pot = 0;
             int rc = ryo_system_load(slot.sys, r.n, r.x, r.y, r.z, r.m, r.h, r.limits, r.periodic);
             if (!rc) rc = ryo_system_leapfrog(slot.sys, &p, r.dt, r.steps, &pot);
+            double acc[4] = {0, 0, 0, 0};
+            if (!rc && r.accuracy_targets > 0) rc = ryo_system_accuracy(slot.sys, r.accuracy_targets, uint64_t(j), acc);
             if (!rc && out_state && out_state[j])
             {
                 float* a[8];
@@ -147,6 +149,7 @@ int ryo_server_run(ryo_server* s, int num, const ryo_request* req, ryo_result* r
                 std::lock_guard<std::mutex> lk(mu);
                 res[j].finish_ms = now();
                 res[j].total_pot = pot;
+                res[j].err_median = acc[0], res[j].err_p99 = acc[1], res[j].err_max = acc[2];
                 res[j].status    = rc;
                 res[j].slot      = k;
                 if (rc && !firstError) firstError = rc;

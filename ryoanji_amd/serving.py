@@ -35,7 +35,7 @@ class Server:
             raise RuntimeError(f"ryo_server_create failed ({rc})")
         self.slots = slots
 
-    def run(self, systems, params, keep_state=False, arrival_scale=1.0):
+    def run(self, systems, params, keep_state=False, arrival_scale=1.0, accuracy_targets=0):
         num = len(systems)
         reqs = (Request * num)()
         keep = []
@@ -55,6 +55,7 @@ class Server:
             r.bucket_size = 64
             r.group_size = 64
             r.arrival_ms = float(params["arrival_ms"][i]) * arrival_scale
+            r.accuracy_targets = int(accuracy_targets)
         res = (Result * num)()
         states = None
         if keep_state:
@@ -64,7 +65,7 @@ class Server:
         if rc:
             raise RuntimeError(f"ryo_server_run failed ({rc})")
         out = [dict(arrival=r.arrival_ms, admit=r.admit_ms, finish=r.finish_ms, pot=r.total_pot, slot=r.slot,
-                    status=r.status) for r in res]
+                    status=r.status, err_median=r.err_median, err_p99=r.err_p99, err_max=r.err_max) for r in res]
         if keep_state:
             for i, b in enumerate(bufs):
                 n = systems[i].n

@@ -84,6 +84,18 @@ def _box(mn, mx):
     return np.array([lo[0], lo[0] + L, lo[1], lo[1] + L, lo[2], lo[2] + L], np.float32)
 
 
+def test_served_results_report_their_accuracy(gpu):
+    systems, params = S.make_stream(6, rate_per_s=0, seed=6, lo_exp=11, hi_exp=14, steps=(1, 3))
+    params["theta"][:] = 0.5
+    srv = S.Server(slots=3, max_n=1 << 14)
+    out = srv.run(systems, params, accuracy_targets=256)
+    srv.close()
+    for r in out:
+        assert r["status"] == 0
+        assert 0 < r["err_median"] <= r["err_p99"] <= r["err_max"]
+        assert r["err_p99"] < 2e-3 and r["err_max"] < 3e-2
+
+
 def test_admission_budget_limits_concurrency(gpu):
     systems, params = S.make_stream(6, rate_per_s=0, seed=4, lo_exp=14, hi_exp=14.01, steps=(1, 1))
     fp = R.lib().ryo_server_footprint(systems[0].n)

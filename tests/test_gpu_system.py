@@ -152,3 +152,20 @@ def test_periodic_traversal_and_system(ctx):
     assert np.quantile(d, 0.99) <= 1e-4, np.quantile(d, 0.99)
     # the tree (-10.45) and Ewald (+10.45) energies cancel to ~1e-3: bound the error by the terms' magnitude
     assert abs(pot - (otot + etot)) <= 1e-6 * (abs(otot) + abs(etot))
+
+
+def test_system_accuracy_report(ctx):
+    """ryo_system_accuracy: its statistics are those of the FP64 direct sum on the same sampled targets"""
+    s = W.plummer(30000, seed=9)
+    sysm, pot, p, ax, ay, az, order = _solve(ctx, s)
+    rep = sysm.accuracy(num_targets=700, seed=5)
+    tgt = (W.splitmix64(5, 0, 700) % np.uint64(s.n)).astype(np.int64)
+    x, y, z, m, h = (a[order] for a in s.arrays())
+    ref = _direct(ctx, x, y, z, h, m, targets=tgt)
+    e = np.sort(rel_err(ax[tgt], ay[tgt], az[tgt], ref[:, 1], ref[:, 2], ref[:, 3]))
+    assert np.isclose(rep["err_median"], e[e.size // 2], rtol=1e-9)
+    assert np.isclose(rep["err_p99"], e[min(e.size - 1, int(0.99 * e.size))], rtol=1e-9)
+    assert np.isclose(rep["err_max"], e[-1], rtol=1e-9)
+    assert rep["err_p99"] < 2e-3 and rep["err_max"] < 3e-2
+    prel = abs(p[tgt].astype(np.float64).sum() - ref[:, 0].sum()) / abs(ref[:, 0].sum())
+    assert np.isclose(rep["pot_rel_err"], prel, rtol=1e-6, atol=1e-12)
