@@ -15,6 +15,7 @@ size_t sortTempBytes(size_t n);
 int sortPairs(Context*, uint64_t*, int32_t*, size_t, uint64_t*, int32_t*, void*, size_t);
 int sequence(Context*, int32_t*, size_t, int32_t);
 int gather(Context*, const int32_t*, size_t, int, const float* const*, float* const*);
+int gatherPacked6(Context*, const int32_t*, size_t, const float* const*, float* const*);
 int buildLeaves(Context*, const uint64_t*, size_t, unsigned, uint64_t*, uint32_t*, uint32_t*, size_t, int*);
 int linkOctree(Context*, const uint64_t*, int, uint64_t*, int32_t*, int32_t*, int32_t*, int32_t*, int32_t*, int32_t*);
 int upsweep(Context*, const float*, const float*, const float*, const float*, const uint32_t*, int, const uint64_t*,
@@ -319,6 +320,7 @@ struct ryo_system
     uint32_t* groups = nullptr;
     int levelRangeHost[24] = {};
     bool haveAcc = false;     // accelerations of the current positions are in ax, ay, az
+    bool freshOrder = false;  // the particle arrays are in the order they were loaded in
     float lastG = 1.0f;       // G, replica shells and Ewald use of the last gravity call
     int lastShells = 0, lastEwald = 0;
     bool integrating = false; // velocities are in use (leapfrog): the build permutes them too
@@ -451,6 +453,7 @@ int ryo_system_load(ryo_system* s, size_t n, const float* x, const float* y, con
     s->cur      = 0;
     s->haveAcc  = false;
     s->integrating = false;
+    s->freshOrder  = true;
     const float* src[5] = {x, y, z, m, h};
     for (int k = 0; k < 5; ++k)
         RYO_HIP(ctx, hipMemcpyAsync(s->a[0][k], src[k], n * sizeof(float), hipMemcpyDefault, ctx->stream));
@@ -503,7 +506,12 @@ int ryo_system_build(ryo_system* s, const ryo_params* prm)
         float* out[9];
         for (int k = 0; k < num; ++k)
             in[k] = A[idx[k]], out[k] = B[idx[k]];
-        if ((rc = gather(ctx, s->perm, n, num, in, out))) return rc;
+        // arrays in the caller's order (a fresh load): one 32-byte record per particle; a time-stepping system is
+        // already close to SFC order, where the plain gather is cheaper
+        const bool packed = s->freshOrder && num == 6;
+        if ((rc = packed ? gatherPacked6(ctx, s->perm, n, in, out) : gather(ctx, s->perm, n, num, in, out)))
+            return rc;
+        s->freshOrder = false;
     }
     s->cur ^= 1;
     A = s->a[s->cur];

@@ -187,6 +187,7 @@ enum ScratchSlot : int
     kSlotMisc,
     kSlotSplits,
     kSlotAccuracy,
+    kSlotPack,
     kNumSlots
 };
 

@@ -243,6 +243,35 @@ __global__ __launch_bounds__(256) void gatherKernel(const int32_t* __restrict__ 
     }
 }
 
+// the six arrays of a freshly loaded system packed into one 32-byte record per particle, so that the gather of a
+// random permutation fetches one line per particle instead of one per array and particle
+__global__ __launch_bounds__(256) void packRecordsKernel(const float* __restrict__ x, const float* __restrict__ y,
+                                                         const float* __restrict__ z, const float* __restrict__ m,
+                                                         const float* __restrict__ h, const float* __restrict__ o,
+                                                         size_t n, float4* __restrict__ rec)
+{
+    for (size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += size_t(gridDim.x) * blockDim.x)
+    {
+        rec[2 * i]     = make_float4(x[i], y[i], z[i], m[i]);
+        rec[2 * i + 1] = make_float4(h[i], o[i], 0.0f, 0.0f);
+    }
+}
+
+__global__ __launch_bounds__(256) void gatherRecordsKernel(const int32_t* __restrict__ perm, size_t n,
+                                                           const float4* __restrict__ rec, float* __restrict__ x,
+                                                           float* __restrict__ y, float* __restrict__ z,
+                                                           float* __restrict__ m, float* __restrict__ h,
+                                                           float* __restrict__ o)
+{
+    for (size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += size_t(gridDim.x) * blockDim.x)
+    {
+        const size_t j = size_t(perm[i]);
+        const float4 a = rec[2 * j], b = rec[2 * j + 1];
+        x[i] = a.x, y[i] = a.y, z[i] = a.z, m[i] = a.w;
+        h[i] = b.x, o[i] = b.y;
+    }
+}
+
 static unsigned gridFor(size_t n, int threads, int maxBlocks)
 {
     size_t b = (n + threads - 1) / threads;
@@ -415,6 +444,21 @@ int sequence(Context* ctx, int32_t* out, size_t n, int32_t start)
 {
     if (n == 0) return RYO_OK;
     iotaKernel<<<gridFor(n, 256, ctx->numCUs * 16), 256, 0, ctx->stream>>>(out, n, start);
+    RYO_LAUNCH_CHECK(ctx);
+    return RYO_OK;
+}
+
+/* out[k][i] = in[k][perm[i]] for the six arrays x, y, z, m, h, order of a system in random order, through
+ * 32-byte records (scratch: 32 n bytes) */
+int gatherPacked6(Context* ctx, const int32_t* perm, size_t n, const float* const* in, float* const* out)
+{
+    if (n == 0) return RYO_OK;
+    auto* rec = static_cast<float4*>(ctx->getScratch(kSlotPack, 32 * n));
+    if (!rec) return RYO_ERR_HIP;
+    const unsigned grid = gridFor(n, 256, ctx->numCUs * 16);
+    packRecordsKernel<<<grid, 256, 0, ctx->stream>>>(in[0], in[1], in[2], in[3], in[4], in[5], n, rec);
+    gatherRecordsKernel<<<grid, 256, 0, ctx->stream>>>(perm, n, rec, out[0], out[1], out[2], out[3], out[4],
+                                                       out[5]);
     RYO_LAUNCH_CHECK(ctx);
     return RYO_OK;
 }

@@ -21,8 +21,9 @@ using Clock = std::chrono::steady_clock;
 
 size_t footprint(size_t n)
 {
-    // particles (2 x 9 arrays), keys x2, permutation x2, outputs, sort tiles, and ~n/6 tree nodes x ~230 B
-    return n * (2 * 9 * 4 + 16 + 8 + 16) + (n / 6 + 64) * 230 + (64u << 10);
+    // particles (2 x 9 arrays), keys x2, permutation x2, outputs, 32-byte gather records, and ~n/6 tree nodes x
+    // ~230 B
+    return n * (2 * 9 * 4 + 16 + 8 + 16 + 32) + (n / 6 + 64) * 230 + (64u << 10);
 }
 } // namespace
 
