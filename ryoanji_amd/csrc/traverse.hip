@@ -296,17 +296,40 @@ __device__ __forceinline__ float orderedToFloat(unsigned o)
     return __uint_as_float((o & 0x80000000u) ? (o & 0x7fffffffu) : ~o);
 }
 
-// min and max softening over targets and sources (order-preserving integers; out = {~0u, 0} on entry)
+// min and max softening over targets and sources (order-preserving integers; out = {~0u, 0} on entry); the
+// arrays are read as float4 where 16-byte aligned, the rest element by element
+__device__ inline void hRangeOf(const float* __restrict__ h, size_t n, unsigned& lo, unsigned& hi)
+{
+    const size_t stride = size_t(gridDim.x) * blockDim.x, t0 = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    size_t head = (reinterpret_cast<uintptr_t>(h) & 15u) ? (16u - (reinterpret_cast<uintptr_t>(h) & 15u)) / 4u : 0u;
+    head = head < n ? head : n;
+    for (size_t i = t0; i < head; i += stride)
+    {
+        unsigned o = floatToOrdered(h[i]);
+        lo = min(lo, o), hi = max(hi, o);
+    }
+    const float4* h4 = reinterpret_cast<const float4*>(h + head);
+    const size_t n4  = (n - head) / 4;
+    for (size_t i = t0; i < n4; i += stride)
+    {
+        float4 v = h4[i];
+        unsigned a = floatToOrdered(v.x), b = floatToOrdered(v.y), c = floatToOrdered(v.z), d = floatToOrdered(v.w);
+        lo = min(lo, min(min(a, b), min(c, d)));
+        hi = max(hi, max(max(a, b), max(c, d)));
+    }
+    for (size_t i = head + 4 * n4 + t0; i < n; i += stride)
+    {
+        unsigned o = floatToOrdered(h[i]);
+        lo = min(lo, o), hi = max(hi, o);
+    }
+}
+
 __global__ __launch_bounds__(256) void hRangeKernel(const float* __restrict__ h1, size_t n1,
                                                     const float* __restrict__ h2, size_t n2, unsigned* out)
 {
     unsigned lo = ~0u, hi = 0;
-    const size_t stride = size_t(gridDim.x) * blockDim.x;
-    for (size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n1 + n2; i += stride)
-    {
-        unsigned o = floatToOrdered(i < n1 ? h1[i] : h2[i - n1]);
-        lo = min(lo, o), hi = max(hi, o);
-    }
+    hRangeOf(h1, n1, lo, hi);
+    if (h2) hRangeOf(h2, n2, lo, hi);
     for (int off = 32; off > 0; off >>= 1)
     {
         lo = min(lo, unsigned(__shfl_xor(int(lo), off)));
@@ -534,7 +557,7 @@ int traverse(Context* ctx, TravArgs a, double* totalPot)
     // the softening range over all targets and sources selects the uniform-softening P2P; without the particle
     // count (hCount = 0) the range stays empty and the general P2P runs
     if (a.hCount > 0)
-        hRangeKernel<<<std::min(iceil(a.hCount, 256), 2048u), 256, 0, s>>>(a.ht, a.hCount,
+        hRangeKernel<<<std::max(1u, std::min(iceil(a.hCount, 1024), unsigned(ctx->numCUs * 8))), 256, 0, s>>>(a.ht, a.hCount,
                                                                             a.hs == a.ht ? nullptr : a.hs,
                                                                             a.hs == a.ht ? 0 : a.hCount,
                                                                             ctl + kCtlHRange);
@@ -662,7 +685,7 @@ int traverseBatch(Context* ctx, TravArgs a, const TravBatch& tb, double* sysPot)
     a.hRange = ctl + kCtlHRange;
     travInitKernel<<<1, 64, 0, s>>>(ctl);
     if (a.hCount > 0)
-        hRangeKernel<<<std::min(iceil(a.hCount, 256), 2048u), 256, 0, s>>>(a.ht, a.hCount, nullptr, 0,
+        hRangeKernel<<<std::max(1u, std::min(iceil(a.hCount, 1024), unsigned(ctx->numCUs * 8))), 256, 0, s>>>(a.ht, a.hCount, nullptr, 0,
                                                                             ctl + kCtlHRange);
     const size_t G = size_t(a.numGroups);
     auto* buf      = static_cast<char*>(ctx->getScratch(kSlotTraverse, 12 * G + 256));
