@@ -191,8 +191,9 @@ def serving_bench(num=32, slots=8, seed=5):
 def batching_bench(classes=(12, 14, 16), total_exp=22, seed=9):
     """Many small Plummer systems (2^total_exp particles in all, systems of 2^c particles for each size class c):
     one batched solve (one launch sequence for all systems) against solving them one at a time on a resident
-    system; the batched launch configuration (group size, traversal blocks per CU, XCD ranges) is tuned per size
-    class by the autotuner. Inputs are resident on the device for both."""
+    system. The batched solve runs eagerly (one host round trip sizes the node arrays) and as one captured HIP
+    graph; the graph's launch configuration (group size, traversal blocks per CU, XCD ranges, LIFO depth) is
+    tuned per size class by the autotuner. Inputs are resident on the device for all of them."""
     import numpy as np
     import torch
 
@@ -222,14 +223,18 @@ def batching_bench(classes=(12, 14, 16), total_exp=22, seed=9):
         del devs
         b = R.Batch(ctx, n * K, K)
         b.load(systems)
-        t_default = A.time_solve(b, prm)
+        t_eager = A.time_solve(b, prm, graph=False)
+        t_default = A.time_solve(b, prm, graph=True)
         best, timings = tuner.tune(systems, b)
         t_best = min(t for _, t in timings)
+        graph_stats = b.graph_stats()
         b.close()
         rows.append(dict(size_class=f"2^{c}", systems=K, particles=n * K,
-                         one_at_a_time_ms=t_one * 1e3, batched_default_ms=t_default * 1e3,
-                         batched_tuned_ms=t_best * 1e3,
-                         tuned={"group_size": best[0], "blocks_per_cu": best[1], "group_ranges": best[2]},
+                         one_at_a_time_ms=t_one * 1e3, batched_eager_ms=t_eager * 1e3,
+                         batched_graph_default_ms=t_default * 1e3, batched_graph_tuned_ms=t_best * 1e3,
+                         tuned={"group_size": best[0], "blocks_per_cu": best[1], "group_ranges": best[2],
+                                "lifo_entries_per_wave": best[3] or 22 * 64},
+                         graph_stats=graph_stats,
                          particles_per_s_one_at_a_time=n * K / t_one, particles_per_s_batched=n * K / t_best,
                          speedup=t_one / t_best))
     ctx.close()

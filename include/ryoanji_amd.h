@@ -250,9 +250,9 @@ int ryo_system_accuracy(ryo_system* sys, int num_targets, uint64_t seed, double*
 
 /* Many particle systems packed one after the other and solved together: every stage (keys, sort, leaves,
  * linked octree, FP64 upsweep, groups, traversal) runs once for all of them, driven by per-system offset
- * tables, with one host round trip per solve. Each system gets exactly the keys, tree, multipoles and forces it
- * gets when solved on its own (GroupView offsets, cstone/traversal/groups.hpp:28-34, are the reference's
- * per-batch table; the reference solves one system per call). Open boundaries. */
+ * tables. Each system gets exactly the keys, tree, multipoles and forces it gets when solved on its own
+ * (GroupView offsets, cstone/traversal/groups.hpp:28-34, are the reference's per-batch table; the reference
+ * solves one system per call). Open boundaries. */
 typedef struct ryo_batch ryo_batch;
 int ryo_batch_create(ryo_ctx* ctx, size_t max_particles, int max_systems, ryo_batch** batch);
 int ryo_batch_destroy(ryo_batch* batch);
@@ -263,6 +263,18 @@ int ryo_batch_load(ryo_batch* batch, int num_systems, const uint32_t* counts, co
 /* one full pass for every system; theta[S] and G[S] per system (host; NULL: prm's value), pot[S] (host, may be
  * NULL) receives each system's 0.5 G sum(m phi). prm: bucket_size and group_size (<= 64). */
 int ryo_batch_solve(ryo_batch* batch, const ryo_params* prm, const float* theta, const float* G, double* pot);
+/* The same solve as ryo_batch_solve, bit for bit, run as one captured HIP graph (keys -> sort -> leaves ->
+ * octrees -> upsweep -> groups -> traversal, no host round trip inside; ryo_batch_solve reads the leaf counts
+ * back once to size the node arrays). The graph is captured on the first call and again when the system
+ * counts, bucket or group size, init node, traversal shape or a buffer changed; positions, boxes, theta and G
+ * may change between replays. Its node arrays hold the capturing solve's totals plus a margin; a replay whose
+ * trees outgrow them is detected on the device and solved again exactly (counted in the statistics). The
+ * batch's stream must not be capturing. Synchronizes. */
+int ryo_batch_solve_graph(ryo_batch* batch, const ryo_params* prm, const float* theta, const float* G,
+                          double* pot);
+/* out[4] = {graphs captured, replays, replays solved again because the trees outgrew the graph's arrays,
+ * leaf-slot capacity of the current graph} */
+int ryo_batch_graph_stats(ryo_batch* batch, uint64_t* out);
 /* device arrays, packed like the input but each system in its SFC order: out[0..4] = x, y, z, m, h,
  * out[5..8] = p, ax, ay, az; *order = index into the loaded (packed) arrays */
 int ryo_batch_results(ryo_batch* batch, float** out, int32_t** order);

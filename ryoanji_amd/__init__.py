@@ -122,6 +122,8 @@ SIGNATURES = {
     "ryo_batch_destroy": (_I, [_P]),
     "ryo_batch_load": (_I, [_P, _I, _P, _P, _P, _P, _P, _P, _P]),
     "ryo_batch_solve": (_I, [_P, C.POINTER(Params), _P, _P, _P]),
+    "ryo_batch_solve_graph": (_I, [_P, C.POINTER(Params), _P, _P, _P]),
+    "ryo_batch_graph_stats": (_I, [_P, _P]),
     "ryo_batch_results": (_I, [_P, _P, _P]),
     "ryo_batch_offsets": (_I, [_P, _P, _P, _P, _P, _P]),
     "ryo_batch_tree": (_I, [_P, _P]),
@@ -351,15 +353,27 @@ class Batch:
                                          lim.ctypes.data))
         self.S = len(systems)
 
-    def solve(self, params: Params, theta=None, G=None):
+    def solve(self, params: Params, theta=None, G=None, graph: bool = False):
+        """one pass over every system; graph=True replays the pipeline as one captured HIP graph (the same
+        results, bit for bit)"""
         import numpy as np
 
         th = None if theta is None else np.ascontiguousarray(theta, np.float32)
         g = None if G is None else np.ascontiguousarray(G, np.float32)
         pot = np.zeros(self.S, np.float64)
-        self._check(lib().ryo_batch_solve(self.h, C.byref(params), None if th is None else th.ctypes.data,
-                                          None if g is None else g.ctypes.data, pot.ctypes.data))
+        fn = lib().ryo_batch_solve_graph if graph else lib().ryo_batch_solve
+        self._check(fn(self.h, C.byref(params), None if th is None else th.ctypes.data,
+                       None if g is None else g.ctypes.data, pot.ctypes.data))
         return pot
+
+    def graph_stats(self):
+        """{captures, replays, fallbacks, capacity}: graphs captured, replays, replays solved again exactly
+        because the trees outgrew the captured arrays, leaf-slot capacity of the current graph"""
+        import numpy as np
+
+        out = np.zeros(4, np.uint64)
+        self._check(lib().ryo_batch_graph_stats(self.h, out.ctypes.data))
+        return dict(zip(("captures", "replays", "fallbacks", "capacity"), (int(v) for v in out)))
 
     def offsets(self):
         import numpy as np

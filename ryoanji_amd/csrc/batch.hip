@@ -5,11 +5,18 @@
 //
 // Per step: keys with each system's box -> one stable radix sort of all keys followed by a stable pass on the
 // system of each particle (grouping by system, keys ascending inside) -> gather -> leaves of every system (leaf
-// tiles never straddle systems) -> the leaf-slot offsets are read back (the one host round trip, which sizes the
-// node arrays) -> linked octrees (the node prefixes sorted the same segmented way) -> FP64 upsweep, level by
-// level across all systems -> fixed 64-target groups per system -> one traversal launch over all groups ->
-// per-system FP64 potentials. Every index stored in a system's slice is local to that system, so each system
-// gets exactly the keys, tree, multipoles and forces it gets when solved on its own.
+// tiles never straddle systems) -> per-system leaf-slot, node and parent offsets computed on the device ->
+// linked octrees (the node prefixes sorted the same segmented way) -> FP64 upsweep, level by level across all
+// systems -> fixed 64-target groups per system -> one traversal launch over all groups -> per-system FP64
+// potentials. Every index stored in a system's slice is local to that system, so each system gets exactly the
+// keys, tree, multipoles and forces it gets when solved on its own.
+//
+// Two ways to run it. ryo_batch_solve reads the leaf-slot offsets back once to size the node arrays exactly.
+// ryo_batch_solve_graph runs the whole sequence as one captured HIP graph with no host round trip inside: node
+// arrays are sized for a capacity (the totals of the solve that captured the graph plus a margin), the node
+// stages read their counts from the device tables, and a replay whose trees outgrow the capacity is detected
+// on the device (every later stage then sees empty trees) and solved again the first way; the graph is
+// captured again for the grown arrays on the next call.
 #include "common.hpp"
 #include "sfc.hpp"
 
@@ -26,7 +33,11 @@ int batchKeys(Context* ctx, const float* x, const float* y, const float* z, size
               const KeyBox* boxes, uint64_t* keys, int32_t* vals);
 int segmentedSortPairs(Context* ctx, uint64_t* keys, int32_t* vals, size_t n, const uint32_t* off, int S,
                        uint64_t* keyBuf, int32_t* valBuf, uint64_t* segKeys, uint64_t* segBuf, int32_t* pos,
-                       int32_t* posBuf, void* tmp, size_t tmpBytes);
+                       int32_t* posBuf, void* tmp, size_t tmpBytes, const uint32_t* dN);
+int copyWords(Context* ctx, void* dst, const void* src, size_t words);
+int zeroWords(Context* ctx, void* dst, size_t words);
+int batchNodeTables(Context* ctx, const BatchView& bv, uint32_t capSlots, uint32_t capNodes, uint32_t capParents,
+                    uint32_t* overflow);
 int batchLeavesCount(Context* ctx, const uint64_t* sk, const BatchView& bv, int numTiles, int B,
                      uint32_t* blockSums, uint32_t* LO);
 int batchLeavesEmit(Context* ctx, const uint64_t* sk, const BatchView& bv, int numTiles, int B,
@@ -43,7 +54,8 @@ int batchTraverse(Context* ctx, int numGroups, const uint32_t* gStart, const uin
                   const BatchView& bv, const float* G, const float* x, const float* y, const float* z,
                   const float* m, const float* h, size_t n, const int32_t* child, const int32_t* i2l,
                   const uint32_t* layout, const float* centers, const float* mp, float* p, float* ax, float* ay,
-                  float* az, double* sysPot, int initNode);
+                  float* az, double* sysPot, int initNode, void* work, const uint32_t* abort, int mode);
+size_t batchTraverseWorkBytes(size_t numGroups);
 
 int batchUpsweepEntry(Context* ctx, const float* x, const float* y, const float* z, const float* m,
                       const uint32_t* layout, const uint64_t* prefixes, const int32_t* child, const int32_t* parents,
@@ -105,9 +117,19 @@ struct ryo_batch
     DevBuf leaves, layout, counts, prefixes, child, parents, i2l, l2i, centers, mp, c64, q64;
     DevBuf nodeIdx, nodeKeyBuf, nodeValBuf, nodeSeg, nodeSegBuf, nodePos, nodePosBuf, nodeTmp;
     DevBuf gStart, gEnd, gSys;
+    DevBuf travWork, flags; // traversal work arrays; flags[0]: the trees outgrew the node arrays
     std::vector<double> hPot;
     uint32_t totalSlots = 0, totalNodes = 0, totalInternal = 0, totalParents = 0, totalGroups = 0;
     bool solved = false;
+
+    // the captured pipeline: the graph, what it was captured for, the capacities its node arrays have
+    hipStream_t capStream = nullptr;
+    hipGraph_t graph      = nullptr;
+    hipGraphExec_t exec   = nullptr;
+    std::vector<uint64_t> graphKey;
+    uint64_t graphBufs = 0;
+    uint32_t capSlots = 0, capNodes = 0, capParents = 0;
+    uint64_t captures = 0, replays = 0, fallbacks = 0;
 
     uint32_t* T(int k) const { return tab.as<uint32_t>() + size_t(k) * (capS + 1); }
     BatchView view() const
@@ -134,7 +156,7 @@ void allBufs(ryo_batch* b, std::vector<DevBuf*>& out)
                       &b->prefixes, &b->child,    &b->parents,  &b->i2l,        &b->l2i,        &b->centers,
                       &b->mp,       &b->c64,      &b->q64,      &b->nodeIdx,    &b->nodeKeyBuf, &b->nodeValBuf,
                       &b->nodeSeg,  &b->nodeSegBuf, &b->nodePos, &b->nodePosBuf, &b->nodeTmp,   &b->gStart,
-                      &b->gEnd,     &b->gSys};
+                      &b->gEnd,     &b->gSys,     &b->travWork, &b->flags};
     for (DevBuf* d : rest) out.push_back(d);
 }
 
@@ -144,6 +166,295 @@ std::vector<uint32_t> prefix(const std::vector<uint32_t>& v)
     for (size_t i = 0; i < v.size(); ++i)
         r[i + 1] = r[i] + v[i];
     return r;
+}
+
+// what a solve needs from the host: launch sizes and the per-system parameters that were uploaded
+struct Shape
+{
+    int B          = 64;
+    unsigned gs    = 64;
+    int initNode   = 1;
+    int numTiles   = 0;
+};
+
+// host tables known before the build (particle offsets P, leaf tiles TS, groups GO) and the per-system key and
+// MAC boxes and G, uploaded to the device; validates the parameters
+int uploadTables(ryo_batch* b, const ryo_params* prm, const float* theta, const float* Gs, Shape& sh)
+{
+    Context* ctx   = b->ctx;
+    hipStream_t st = ctx->stream;
+    const int S    = b->S;
+    sh.B  = int(prm->bucket_size ? prm->bucket_size : 64);
+    sh.gs = prm->group_size ? prm->group_size : 64;
+    sh.initNode = prm->init_node;
+    if (sh.gs > 64) return setError(ctx, RYO_ERR_UNSUPPORTED, "batch: group size %u > 64", sh.gs);
+    if (prm->init_node != 0 && prm->init_node != 1) return setError(ctx, RYO_ERR_ARG, "batch: init_node 0 or 1");
+    if (prm->num_shells != 0 || prm->ewald != 0)
+        return setError(ctx, RYO_ERR_UNSUPPORTED, "batch: open boundaries only");
+    std::vector<uint32_t> tiles(S), ngroups(S);
+    std::vector<KeyBox> kb(S);
+    std::vector<MacBox> mb(S);
+    std::vector<float> gv(S);
+    for (int s = 0; s < S; ++s)
+    {
+        uint32_t n = b->hP[s + 1] - b->hP[s];
+        tiles[s]   = (n + kBatchLeafTile - 1) / kBatchLeafTile;
+        ngroups[s] = (n + sh.gs - 1) / sh.gs;
+        float th   = theta ? theta[s] : prm->theta;
+        if (!(th > 0)) return setError(ctx, RYO_ERR_ARG, "batch: theta of system %d", s);
+        kb[s] = makeKeyBox(&b->limits[6 * s]);
+        mb[s] = makeMacBox(&b->limits[6 * s], 1.0f / th);
+        gv[s] = Gs ? Gs[s] : prm->G;
+    }
+    b->hTS = prefix(tiles);
+    b->hGO = prefix(ngroups);
+    sh.numTiles = int(b->hTS[S]);
+    b->totalGroups = b->hGO[S];
+    int rc = b->blockSums.ensure(ctx, (size_t(sh.numTiles) + 1) * 4);
+    if (!rc) rc = b->flags.ensure(ctx, 256);
+    for (DevBuf* d : {&b->gStart, &b->gEnd, &b->gSys})
+        if (!rc) rc = d->ensure(ctx, (size_t(b->totalGroups) + 1) * 4);
+    if (!rc) rc = b->travWork.ensure(ctx, batchTraverseWorkBytes(b->totalGroups));
+    if (rc) return rc;
+    RYO_HIP(ctx, hipMemcpyAsync(b->T(0), b->hP.data(), (S + 1) * 4, hipMemcpyHostToDevice, st));
+    RYO_HIP(ctx, hipMemcpyAsync(b->T(1), b->hTS.data(), (S + 1) * 4, hipMemcpyHostToDevice, st));
+    RYO_HIP(ctx, hipMemcpyAsync(b->T(6), b->hGO.data(), (S + 1) * 4, hipMemcpyHostToDevice, st));
+    RYO_HIP(ctx, hipMemcpyAsync(b->keyBoxes.p, kb.data(), S * sizeof(KeyBox), hipMemcpyHostToDevice, st));
+    RYO_HIP(ctx, hipMemcpyAsync(b->macBoxes.p, mb.data(), S * sizeof(MacBox), hipMemcpyHostToDevice, st));
+    RYO_HIP(ctx, hipMemcpyAsync(b->G.p, gv.data(), S * 4, hipMemcpyHostToDevice, st));
+    return RYO_OK;
+}
+
+// keys, segmented sort, gather, leaf counts and offsets, node tables (device); no host synchronization
+int runFront(ryo_batch* b, const Shape& sh, uint32_t capSlots, uint32_t capNodes, uint32_t capParents)
+{
+    Context* ctx = b->ctx;
+    const int S = b->S;
+    const size_t N = b->N;
+    BatchView bv = b->view();
+    auto* keys = b->keys.as<uint64_t>();
+    auto* vals = b->vals.as<int32_t>();
+    const float* in[5] = {b->in[0].as<float>(), b->in[1].as<float>(), b->in[2].as<float>(), b->in[3].as<float>(),
+                          b->in[4].as<float>()};
+    float* out[5] = {b->srt[0].as<float>(), b->srt[1].as<float>(), b->srt[2].as<float>(), b->srt[3].as<float>(),
+                     b->srt[4].as<float>()};
+    int rc;
+    if ((rc = batchKeys(ctx, in[0], in[1], in[2], N, bv.P, S, b->keyBoxes.as<KeyBox>(), keys, vals))) return rc;
+    if ((rc = segmentedSortPairs(ctx, keys, vals, N, bv.P, S, b->keyBuf.as<uint64_t>(), b->valBuf.as<int32_t>(),
+                                 b->segKeys.as<uint64_t>(), b->segBuf.as<uint64_t>(), b->pos.as<int32_t>(),
+                                 b->posBuf.as<int32_t>(), b->sortTmp.p, b->sortTmp.size, nullptr)))
+        return rc;
+    if ((rc = gather(ctx, vals, N, 5, in, out))) return rc;
+    if ((rc = copyWords(ctx, b->srt[5].p, vals, N))) return rc;
+    if ((rc = batchLeavesCount(ctx, keys, bv, sh.numTiles, sh.B, b->blockSums.as<uint32_t>(), b->T(2)))) return rc;
+    return batchNodeTables(ctx, bv, capSlots, capNodes, capParents, b->flags.as<uint32_t>());
+}
+
+// node arrays for the given totals (leaf slots, nodes, parents)
+int ensureNodes(ryo_batch* b, uint32_t slots1, uint32_t nodes1, uint32_t parents1)
+{
+    Context* ctx = b->ctx;
+    const size_t slots = size_t(slots1) + 1, nn = size_t(nodes1) + 1, np = size_t(parents1) + 1;
+    int rc = RYO_OK;
+    if (!rc) rc = b->leaves.ensure(ctx, slots * 8);
+    if (!rc) rc = b->layout.ensure(ctx, slots * 4);
+    if (!rc) rc = b->counts.ensure(ctx, slots * 4);
+    for (DevBuf* d : {&b->prefixes, &b->nodeKeyBuf, &b->nodeSeg, &b->nodeSegBuf})
+        if (!rc) rc = d->ensure(ctx, nn * 8);
+    for (DevBuf* d : {&b->child, &b->i2l, &b->l2i, &b->nodeIdx, &b->nodeValBuf, &b->nodePos, &b->nodePosBuf})
+        if (!rc) rc = d->ensure(ctx, nn * 4);
+    if (!rc) rc = b->parents.ensure(ctx, np * 4);
+    if (!rc) rc = b->centers.ensure(ctx, nn * 16);
+    if (!rc) rc = b->mp.ensure(ctx, nn * 32);
+    if (!rc) rc = b->c64.ensure(ctx, nn * 32);
+    if (!rc) rc = b->q64.ensure(ctx, nn * 64);
+    if (!rc) rc = b->nodeTmp.ensure(ctx, sortTempBytes(nn));
+    return rc;
+}
+
+// the batch traversal with the batch's arrays (mode 0: launch and finish, 1: launch only, 2: finish only)
+int traversal(ryo_batch* b, const Shape& sh, int mode, const uint32_t* abort)
+{
+    const float* srt[5] = {b->srt[0].as<float>(), b->srt[1].as<float>(), b->srt[2].as<float>(),
+                           b->srt[3].as<float>(), b->srt[4].as<float>()};
+    return batchTraverse(b->ctx, int(b->totalGroups), b->gStart.as<uint32_t>(), b->gEnd.as<uint32_t>(),
+                         b->gSys.as<int32_t>(), b->view(), b->G.as<float>(), srt[0], srt[1], srt[2], srt[3], srt[4],
+                         b->N, b->child.as<int32_t>(), b->i2l.as<int32_t>(), b->layout.as<uint32_t>(),
+                         b->centers.as<float>(), b->mp.as<float>(), b->p.as<float>(), b->ax.as<float>(),
+                         b->ay.as<float>(), b->az.as<float>(), b->sysPot.as<double>(), sh.initNode, b->travWork.p,
+                         abort, mode);
+}
+
+// leaves, linked octrees, upsweep, groups, zeroed outputs and the traversal. slots/nodes/internal/parents:
+// exact totals (eager) or capacities (captured; the node sort then reads its count on the device)
+int runBack(ryo_batch* b, const Shape& sh, uint32_t slots, uint32_t nodes, uint32_t internal, uint32_t parents,
+            bool captured)
+{
+    Context* ctx = b->ctx;
+    const int S = b->S;
+    BatchView bv = b->view();
+    auto* keys = b->keys.as<uint64_t>();
+    int rc;
+    if ((rc = batchLeavesEmit(ctx, keys, bv, sh.numTiles, sh.B, b->blockSums.as<uint32_t>(), b->leaves.as<uint64_t>(),
+                              b->layout.as<uint32_t>(), b->counts.as<uint32_t>(), slots)))
+        return rc;
+    auto* pre = b->prefixes.as<uint64_t>();
+    auto* idx = b->nodeIdx.as<int32_t>();
+    if ((rc = batchUnsortedLayout(ctx, b->leaves.as<uint64_t>(), bv, slots, pre, idx))) return rc;
+    if ((rc = segmentedSortPairs(ctx, pre, idx, nodes, bv.NO, S, b->nodeKeyBuf.as<uint64_t>(),
+                                 b->nodeValBuf.as<int32_t>(), b->nodeSeg.as<uint64_t>(),
+                                 b->nodeSegBuf.as<uint64_t>(), b->nodePos.as<int32_t>(),
+                                 b->nodePosBuf.as<int32_t>(), b->nodeTmp.p, b->nodeTmp.size,
+                                 captured ? bv.NO + S : nullptr)))
+        return rc;
+    if ((rc = batchLinkFinish(ctx, pre, idx, bv, nodes, internal, parents, b->i2l.as<int32_t>(),
+                              b->l2i.as<int32_t>(), b->LR.as<int32_t>(), b->LP.as<uint32_t>(), b->child.as<int32_t>(),
+                              b->parents.as<int32_t>())))
+        return rc;
+    const float* srt[4] = {b->srt[0].as<float>(), b->srt[1].as<float>(), b->srt[2].as<float>(),
+                           b->srt[3].as<float>()};
+    if ((rc = batchUpsweepEntry(ctx, srt[0], srt[1], srt[2], srt[3], b->layout.as<uint32_t>(), pre,
+                                b->child.as<int32_t>(), b->parents.as<int32_t>(), b->l2i.as<int32_t>(),
+                                b->centers.as<float>(), b->mp.as<float>(), b->c64.as<double>(), b->q64.as<double>(),
+                                bv, b->macBoxes.as<MacBox>(), slots, nodes)))
+        return rc;
+    if ((rc = batchGroups(ctx, bv, b->totalGroups, sh.gs, b->gStart.as<uint32_t>(), b->gEnd.as<uint32_t>(),
+                          b->gSys.as<int32_t>())))
+        return rc;
+    for (DevBuf* d : {&b->p, &b->ax, &b->ay, &b->az})
+        if ((rc = zeroWords(ctx, d->p, b->N))) return rc;
+    return traversal(b, sh, captured ? 1 : 0, captured ? b->flags.as<uint32_t>() : nullptr);
+}
+
+// node, internal-node and parent totals of every system from its leaf-slot offsets (the formula of the
+// device's node-table kernel)
+void hostNodeTables(ryo_batch* b)
+{
+    const int S = b->S;
+    std::vector<uint32_t> nodes(S), internal(S), parentsN(S);
+    for (int s = 0; s < S; ++s)
+    {
+        uint32_t L  = b->hLO[s + 1] - b->hLO[s] - 1;
+        internal[s] = (L - 1) / 7;
+        nodes[s]    = L + internal[s];
+        parentsN[s] = std::max(1u, (nodes[s] - 1) / 8);
+    }
+    b->hNO  = prefix(nodes);
+    b->hNIO = prefix(internal);
+    b->hPO  = prefix(parentsN);
+    b->totalSlots    = b->hLO[S];
+    b->totalNodes    = b->hNO[S];
+    b->totalInternal = b->hNIO[S];
+    b->totalParents  = b->hPO[S];
+}
+
+// the exact solve: the leaf-slot offsets are read back once to size the node arrays
+int solveEager(ryo_batch* b, const Shape& sh)
+{
+    Context* ctx = b->ctx;
+    const int S  = b->S;
+    int rc = runFront(b, sh, ~0u, ~0u, ~0u);
+    if (rc) return rc;
+    b->hLO.resize(S + 1);
+    RYO_HIP(ctx, hipMemcpyAsync(b->hLO.data(), b->T(2), (S + 1) * 4, hipMemcpyDeviceToHost, ctx->stream));
+    RYO_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    hostNodeTables(b);
+    if ((rc = ensureNodes(b, b->totalSlots, b->totalNodes, b->totalParents))) return rc;
+    return runBack(b, sh, b->totalSlots, b->totalNodes, b->totalInternal, b->totalParents, false);
+}
+
+// per-system potentials to the host
+int finishSolve(ryo_batch* b, const Shape&, double* pot)
+{
+    Context* ctx = b->ctx;
+    b->hPot.resize(b->S);
+    RYO_HIP(ctx, hipMemcpyAsync(b->hPot.data(), b->sysPot.p, b->S * 8, hipMemcpyDeviceToHost, ctx->stream));
+    RYO_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    if (pot) std::memcpy(pot, b->hPot.data(), b->S * sizeof(double));
+    b->solved = true;
+    return RYO_OK;
+}
+
+// what a captured graph depends on: sizes and launch parameters (theta, G, boxes and positions are read from
+// device memory at replay)
+std::vector<uint64_t> graphKey(const ryo_batch* b, const Shape& sh)
+{
+    const Context* ctx = b->ctx;
+    std::vector<uint64_t> k = {uint64_t(b->S), uint64_t(b->N), uint64_t(sh.B), uint64_t(sh.gs),
+                               uint64_t(sh.initNode), uint64_t(ctx->travStack), uint64_t(ctx->travBlocksPerCU),
+                               uint64_t(ctx->travRanges)};
+    k.insert(k.end(), b->hP.begin(), b->hP.end());
+    return k;
+}
+
+// the device addresses a graph holds (any reallocation makes it stale)
+uint64_t bufferSignature(ryo_batch* b)
+{
+    std::vector<DevBuf*> bufs;
+    allBufs(b, bufs);
+    uint64_t h = 1469598103934665603ull;
+    for (DevBuf* d : bufs)
+        h = (h ^ uint64_t(reinterpret_cast<uintptr_t>(d->p))) * 1099511628211ull;
+    h = (h ^ uint64_t(reinterpret_cast<uintptr_t>(b->ctx->dStats))) * 1099511628211ull;
+    return h;
+}
+
+void dropGraph(ryo_batch* b)
+{
+    if (b->exec) hipGraphExecDestroy(b->exec);
+    if (b->graph) hipGraphDestroy(b->graph);
+    b->exec = nullptr, b->graph = nullptr;
+    b->graphKey.clear();
+}
+
+// capture the whole solve (after the host tables) as one graph, with node arrays sized for the last exact
+// solve's totals plus a quarter and 8 slots per system
+int capture(ryo_batch* b, const Shape& sh)
+{
+    Context* ctx = b->ctx;
+    dropGraph(b);
+    const uint64_t slots64 = uint64_t(b->totalSlots) + b->totalSlots / 4 + 8ull * uint64_t(b->S) + 64;
+    if (slots64 > 0x7fffffffull) return setError(ctx, RYO_ERR_CAPACITY, "batch graph: too many leaf slots");
+    const uint32_t capSlots = uint32_t(slots64);
+    const uint32_t capNodes = capSlots + capSlots / 7 + uint32_t(b->S);
+    const uint32_t capParents = capNodes / 8 + uint32_t(b->S);
+    int rc = ensureNodes(b, capSlots, capNodes, capParents);
+    if (rc) return rc;
+    if (!b->capStream)
+        RYO_HIP(ctx, hipStreamCreateWithFlags(&b->capStream, hipStreamNonBlocking));
+    RYO_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    hipStream_t user = ctx->stream;
+    ctx->stream      = b->capStream;
+    hipError_t e     = hipStreamBeginCapture(b->capStream, hipStreamCaptureModeThreadLocal);
+    if (e != hipSuccess)
+    {
+        ctx->stream = user;
+        return setError(ctx, RYO_ERR_HIP, "batch graph: begin capture: %s", hipGetErrorString(e));
+    }
+    rc = runFront(b, sh, capSlots, capNodes, capParents);
+    if (!rc)
+        rc = runBack(b, sh, capSlots, capNodes, capSlots / 7 + 1, capParents, true);
+    hipGraph_t g = nullptr;
+    e            = hipStreamEndCapture(b->capStream, &g);
+    ctx->stream  = user;
+    if (rc || e != hipSuccess || !g)
+    {
+        if (g) hipGraphDestroy(g);
+        return rc ? rc : setError(ctx, RYO_ERR_HIP, "batch graph: end capture: %s", hipGetErrorString(e));
+    }
+    e = hipGraphInstantiate(&b->exec, g, nullptr, nullptr, 0);
+    if (e != hipSuccess)
+    {
+        hipGraphDestroy(g);
+        b->exec = nullptr;
+        return setError(ctx, RYO_ERR_HIP, "batch graph: instantiate: %s", hipGetErrorString(e));
+    }
+    b->graph     = g;
+    b->graphKey  = graphKey(b, sh);
+    b->graphBufs = bufferSignature(b);
+    b->capSlots = capSlots, b->capNodes = capNodes, b->capParents = capParents;
+    ++b->captures;
+    return RYO_OK;
 }
 } // namespace
 
@@ -189,6 +500,8 @@ int ryo_batch_destroy(ryo_batch* b)
 {
     if (!b) return RYO_OK;
     hipStreamSynchronize(b->ctx->stream);
+    dropGraph(b);
+    if (b->capStream) hipStreamDestroy(b->capStream);
     std::vector<DevBuf*> bufs;
     allBufs(b, bufs);
     for (DevBuf* d : bufs)
@@ -225,147 +538,62 @@ int ryo_batch_load(ryo_batch* b, int num_systems, const uint32_t* counts, const 
     return RYO_OK;
 }
 
-/* theta and G per system (host arrays, NULL: prm's value for every system) */
 int ryo_batch_solve(ryo_batch* b, const ryo_params* prm, const float* theta, const float* Gs, double* pot)
 {
     if (!b || !prm || b->S < 1) return RYO_ERR_ARG;
-    Context* ctx    = b->ctx;
-    hipStream_t st  = ctx->stream;
-    const int S     = b->S;
-    const size_t N  = b->N;
-    const int B     = int(prm->bucket_size ? prm->bucket_size : 64);
-    const unsigned gs = prm->group_size ? prm->group_size : 64;
-    if (gs > 64) return setError(ctx, RYO_ERR_UNSUPPORTED, "batch: group size %u > 64", gs);
-    if (prm->init_node != 0 && prm->init_node != 1) return setError(ctx, RYO_ERR_ARG, "batch: init_node 0 or 1");
-    if (prm->num_shells != 0 || prm->ewald != 0)
-        return setError(ctx, RYO_ERR_UNSUPPORTED, "batch: open boundaries only");
-    int rc;
+    Shape sh;
+    int rc = uploadTables(b, prm, theta, Gs, sh);
+    if (!rc) rc = solveEager(b, sh);
+    if (!rc) rc = finishSolve(b, sh, pot);
+    return rc;
+}
 
-    // per-system tables known before the build: particle offsets, leaf tiles, boxes, G
-    std::vector<uint32_t> tiles(S), ngroups(S);
-    std::vector<KeyBox> kb(S);
-    std::vector<MacBox> mb(S);
-    std::vector<float> gv(S);
-    for (int s = 0; s < S; ++s)
-    {
-        uint32_t n = b->hP[s + 1] - b->hP[s];
-        tiles[s]   = (n + kBatchLeafTile - 1) / kBatchLeafTile;
-        ngroups[s] = (n + gs - 1) / gs;
-        float th   = theta ? theta[s] : prm->theta;
-        if (!(th > 0)) return setError(ctx, RYO_ERR_ARG, "batch: theta of system %d", s);
-        kb[s] = makeKeyBox(&b->limits[6 * s]);
-        mb[s] = makeMacBox(&b->limits[6 * s], 1.0f / th);
-        gv[s] = Gs ? Gs[s] : prm->G;
-    }
-    b->hTS = prefix(tiles);
-    b->hGO = prefix(ngroups);
-    const int numTiles = int(b->hTS[S]);
-    if ((rc = b->blockSums.ensure(ctx, (size_t(numTiles) + 1) * 4))) return rc;
-    RYO_HIP(ctx, hipMemcpyAsync(b->T(0), b->hP.data(), (S + 1) * 4, hipMemcpyHostToDevice, st));
-    RYO_HIP(ctx, hipMemcpyAsync(b->T(1), b->hTS.data(), (S + 1) * 4, hipMemcpyHostToDevice, st));
-    RYO_HIP(ctx, hipMemcpyAsync(b->T(6), b->hGO.data(), (S + 1) * 4, hipMemcpyHostToDevice, st));
-    RYO_HIP(ctx, hipMemcpyAsync(b->keyBoxes.p, kb.data(), S * sizeof(KeyBox), hipMemcpyHostToDevice, st));
-    RYO_HIP(ctx, hipMemcpyAsync(b->macBoxes.p, mb.data(), S * sizeof(MacBox), hipMemcpyHostToDevice, st));
-    RYO_HIP(ctx, hipMemcpyAsync(b->G.p, gv.data(), S * 4, hipMemcpyHostToDevice, st));
-    BatchView bv = b->view();
-
-    // keys, segmented sort, gather
-    auto* keys = b->keys.as<uint64_t>();
-    auto* vals = b->vals.as<int32_t>();
-    const float* in[5] = {b->in[0].as<float>(), b->in[1].as<float>(), b->in[2].as<float>(), b->in[3].as<float>(),
-                          b->in[4].as<float>()};
-    if ((rc = batchKeys(ctx, in[0], in[1], in[2], N, bv.P, S, b->keyBoxes.as<KeyBox>(), keys, vals))) return rc;
-    if ((rc = segmentedSortPairs(ctx, keys, vals, N, bv.P, S, b->keyBuf.as<uint64_t>(), b->valBuf.as<int32_t>(),
-                                 b->segKeys.as<uint64_t>(), b->segBuf.as<uint64_t>(), b->pos.as<int32_t>(),
-                                 b->posBuf.as<int32_t>(), b->sortTmp.p, b->sortTmp.size)))
-        return rc;
-    float* out[5] = {b->srt[0].as<float>(), b->srt[1].as<float>(), b->srt[2].as<float>(), b->srt[3].as<float>(),
-                     b->srt[4].as<float>()};
-    if ((rc = gather(ctx, vals, N, 5, in, out))) return rc;
-    RYO_HIP(ctx, hipMemcpyAsync(b->srt[5].p, vals, N * 4, hipMemcpyDeviceToDevice, st));
-
-    // leaves; the slot offsets come back to size the node arrays
-    if ((rc = batchLeavesCount(ctx, keys, bv, numTiles, B, b->blockSums.as<uint32_t>(), b->T(2)))) return rc;
-    b->hLO.resize(S + 1);
-    RYO_HIP(ctx, hipMemcpyAsync(b->hLO.data(), b->T(2), (S + 1) * 4, hipMemcpyDeviceToHost, st));
-    RYO_HIP(ctx, hipStreamSynchronize(st));
-    std::vector<uint32_t> nodes(S), internal(S), parentsN(S);
-    for (int s = 0; s < S; ++s)
-    {
-        uint32_t L  = b->hLO[s + 1] - b->hLO[s] - 1;
-        internal[s] = (L - 1) / 7;
-        nodes[s]    = L + internal[s];
-        parentsN[s] = std::max(1u, (nodes[s] - 1) / 8);
-    }
-    b->hNO  = prefix(nodes);
-    b->hNIO = prefix(internal);
-    b->hPO  = prefix(parentsN);
-    b->totalSlots    = b->hLO[S];
-    b->totalNodes    = b->hNO[S];
-    b->totalInternal = b->hNIO[S];
-    b->totalParents  = b->hPO[S];
-    b->totalGroups   = b->hGO[S];
-    RYO_HIP(ctx, hipMemcpyAsync(b->T(3), b->hNO.data(), (S + 1) * 4, hipMemcpyHostToDevice, st));
-    RYO_HIP(ctx, hipMemcpyAsync(b->T(4), b->hNIO.data(), (S + 1) * 4, hipMemcpyHostToDevice, st));
-    RYO_HIP(ctx, hipMemcpyAsync(b->T(5), b->hPO.data(), (S + 1) * 4, hipMemcpyHostToDevice, st));
-
-    const size_t slots = b->totalSlots + 1, nn = b->totalNodes + 1, np = b->totalParents + 1, G = b->totalGroups + 1;
-    rc = RYO_OK;
-    if (!rc) rc = b->leaves.ensure(ctx, slots * 8);
-    if (!rc) rc = b->layout.ensure(ctx, slots * 4);
-    if (!rc) rc = b->counts.ensure(ctx, slots * 4);
-    for (DevBuf* d : {&b->prefixes, &b->nodeKeyBuf, &b->nodeSeg, &b->nodeSegBuf})
-        if (!rc) rc = d->ensure(ctx, nn * 8);
-    for (DevBuf* d : {&b->child, &b->i2l, &b->l2i, &b->nodeIdx, &b->nodeValBuf, &b->nodePos, &b->nodePosBuf})
-        if (!rc) rc = d->ensure(ctx, nn * 4);
-    if (!rc) rc = b->parents.ensure(ctx, np * 4);
-    if (!rc) rc = b->centers.ensure(ctx, nn * 16);
-    if (!rc) rc = b->mp.ensure(ctx, nn * 32);
-    if (!rc) rc = b->c64.ensure(ctx, nn * 32);
-    if (!rc) rc = b->q64.ensure(ctx, nn * 64);
-    if (!rc) rc = b->nodeTmp.ensure(ctx, sortTempBytes(nn));
-    for (DevBuf* d : {&b->gStart, &b->gEnd, &b->gSys})
-        if (!rc) rc = d->ensure(ctx, G * 4);
+int ryo_batch_solve_graph(ryo_batch* b, const ryo_params* prm, const float* theta, const float* Gs, double* pot)
+{
+    if (!b || !prm || b->S < 1) return RYO_ERR_ARG;
+    Context* ctx = b->ctx;
+    Shape sh;
+    int rc = uploadTables(b, prm, theta, Gs, sh);
     if (rc) return rc;
+    const bool current = b->exec && b->graphKey == graphKey(b, sh) && b->graphBufs == bufferSignature(b);
+    if (!current)
+    {
+        // an exact solve sizes the capacities, then the graph is captured for them and replayed below
+        if ((rc = solveEager(b, sh))) return rc;
+        if ((rc = capture(b, sh))) return rc;
+    }
+    RYO_HIP(ctx, hipGraphLaunch(b->exec, ctx->stream));
+    ++b->replays;
+    uint32_t overflow = 0;
+    RYO_HIP(ctx, hipMemcpyAsync(&overflow, b->flags.p, sizeof(uint32_t), hipMemcpyDeviceToHost, ctx->stream));
+    RYO_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    if (overflow)
+    {
+        // the trees outgrew the captured capacities: solve exactly (the arrays grow, so the next call captures
+        // again)
+        ++b->fallbacks;
+        if ((rc = solveEager(b, sh))) return rc;
+        return finishSolve(b, sh, pot);
+    }
+    // host copies of the device-computed tables (the exact solve reads them before building)
+    const size_t S1 = size_t(b->S) + 1;
+    b->hLO.resize(S1), b->hNO.resize(S1), b->hNIO.resize(S1), b->hPO.resize(S1);
+    RYO_HIP(ctx, hipMemcpyAsync(b->hLO.data(), b->T(2), S1 * 4, hipMemcpyDeviceToHost, ctx->stream));
+    RYO_HIP(ctx, hipMemcpyAsync(b->hNO.data(), b->T(3), S1 * 4, hipMemcpyDeviceToHost, ctx->stream));
+    RYO_HIP(ctx, hipMemcpyAsync(b->hNIO.data(), b->T(4), S1 * 4, hipMemcpyDeviceToHost, ctx->stream));
+    RYO_HIP(ctx, hipMemcpyAsync(b->hPO.data(), b->T(5), S1 * 4, hipMemcpyDeviceToHost, ctx->stream));
+    if ((rc = traversal(b, sh, 2, nullptr))) return rc;
+    b->totalSlots = b->hLO[b->S], b->totalNodes = b->hNO[b->S];
+    b->totalInternal = b->hNIO[b->S], b->totalParents = b->hPO[b->S];
+    return finishSolve(b, sh, pot);
+}
 
-    if ((rc = batchLeavesEmit(ctx, keys, bv, numTiles, B, b->blockSums.as<uint32_t>(), b->leaves.as<uint64_t>(),
-                              b->layout.as<uint32_t>(), b->counts.as<uint32_t>(), b->totalSlots)))
-        return rc;
-    // linked octrees
-    auto* pre = b->prefixes.as<uint64_t>();
-    auto* idx = b->nodeIdx.as<int32_t>();
-    if ((rc = batchUnsortedLayout(ctx, b->leaves.as<uint64_t>(), bv, b->totalSlots, pre, idx))) return rc;
-    if ((rc = segmentedSortPairs(ctx, pre, idx, b->totalNodes, bv.NO, S, b->nodeKeyBuf.as<uint64_t>(),
-                                 b->nodeValBuf.as<int32_t>(), b->nodeSeg.as<uint64_t>(),
-                                 b->nodeSegBuf.as<uint64_t>(), b->nodePos.as<int32_t>(),
-                                 b->nodePosBuf.as<int32_t>(), b->nodeTmp.p, b->nodeTmp.size)))
-        return rc;
-    if ((rc = batchLinkFinish(ctx, pre, idx, bv, b->totalNodes, b->totalInternal, b->totalParents,
-                              b->i2l.as<int32_t>(), b->l2i.as<int32_t>(), b->LR.as<int32_t>(), b->LP.as<uint32_t>(),
-                              b->child.as<int32_t>(), b->parents.as<int32_t>())))
-        return rc;
-    // upsweep, groups, traversal
-    if ((rc = batchUpsweepEntry(ctx, out[0], out[1], out[2], out[3], b->layout.as<uint32_t>(), pre,
-                                b->child.as<int32_t>(), b->parents.as<int32_t>(), b->l2i.as<int32_t>(),
-                                b->centers.as<float>(), b->mp.as<float>(), b->c64.as<double>(), b->q64.as<double>(),
-                                bv, b->macBoxes.as<MacBox>(), b->totalSlots, b->totalNodes)))
-        return rc;
-    if ((rc = batchGroups(ctx, bv, b->totalGroups, gs, b->gStart.as<uint32_t>(), b->gEnd.as<uint32_t>(),
-                          b->gSys.as<int32_t>())))
-        return rc;
-    for (DevBuf* d : {&b->p, &b->ax, &b->ay, &b->az})
-        RYO_HIP(ctx, hipMemsetAsync(d->p, 0, N * 4, st));
-    if ((rc = batchTraverse(ctx, int(b->totalGroups), b->gStart.as<uint32_t>(), b->gEnd.as<uint32_t>(),
-                            b->gSys.as<int32_t>(), bv, b->G.as<float>(), out[0], out[1], out[2], out[3], out[4], N,
-                            b->child.as<int32_t>(), b->i2l.as<int32_t>(), b->layout.as<uint32_t>(),
-                            b->centers.as<float>(), b->mp.as<float>(), b->p.as<float>(), b->ax.as<float>(),
-                            b->ay.as<float>(), b->az.as<float>(), b->sysPot.as<double>(), prm->init_node)))
-        return rc;
-    b->hPot.resize(S);
-    RYO_HIP(ctx, hipMemcpyAsync(b->hPot.data(), b->sysPot.p, S * 8, hipMemcpyDeviceToHost, st));
-    RYO_HIP(ctx, hipStreamSynchronize(st));
-    if (pot) std::memcpy(pot, b->hPot.data(), S * sizeof(double));
-    b->solved = true;
+/* {graphs captured, replays, replays solved again exactly because the trees outgrew the captured arrays,
+ * leaf-slot capacity of the current graph} */
+int ryo_batch_graph_stats(ryo_batch* b, uint64_t* out)
+{
+    if (!b || !out) return RYO_ERR_ARG;
+    out[0] = b->captures, out[1] = b->replays, out[2] = b->fallbacks, out[3] = b->capSlots;
     return RYO_OK;
 }
 

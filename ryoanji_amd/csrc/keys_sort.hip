@@ -68,11 +68,26 @@ constexpr int kSortTile    = kSortThreads * kSortItems; // 4096 keys per tile
 constexpr int kDigits      = 256;
 constexpr int kPasses      = 8;
 
+// element count of a sort: n, or the count held in device memory (at most n) when dN is given, so that a
+// sequence whose sizes are produced on the device can run without a host round trip
+__device__ inline size_t deviceCount(const uint32_t* dN, size_t n)
+{
+    return dN ? min(size_t(*dN), n) : n;
+}
+
+__global__ void zeroWordsKernel(uint32_t* a, int n)
+{
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+        a[i] = 0;
+}
+
 // all 8 digit histograms in one read of the keys
 __global__ __launch_bounds__(256) void histAllKernel(const uint64_t* __restrict__ keys, size_t n,
+                                                    const uint32_t* __restrict__ dN,
                                                     uint32_t* __restrict__ hist /* [8][256] */)
 {
     __shared__ uint32_t h[kPasses * kDigits];
+    n = deviceCount(dN, n);
     for (int i = threadIdx.x; i < kPasses * kDigits; i += blockDim.x)
         h[i] = 0;
     __syncthreads();
@@ -98,10 +113,12 @@ __global__ __launch_bounds__(256) void digitBaseKernel(const uint32_t* __restric
 }
 
 // per-tile digit counts of one pass: tileHist[d * numTiles + tile]
-__global__ __launch_bounds__(kSortThreads) void tileHistKernel(const uint64_t* __restrict__ keys, size_t n, int shift,
+__global__ __launch_bounds__(kSortThreads) void tileHistKernel(const uint64_t* __restrict__ keys, size_t n,
+                                                              const uint32_t* __restrict__ dN, int shift,
                                                               uint32_t* __restrict__ tileHist, int numTiles)
 {
     __shared__ uint32_t h[kDigits];
+    n = deviceCount(dN, n); // tiles past the count record empty histograms
     h[threadIdx.x] = 0;
     __syncthreads();
     size_t base = size_t(blockIdx.x) * kSortTile;
@@ -139,9 +156,12 @@ __global__ __launch_bounds__(256) void tileScanKernel(uint32_t* __restrict__ til
 __global__ __launch_bounds__(kSortThreads) void scatterKernel(const uint64_t* __restrict__ keysIn,
                                                              const int32_t* __restrict__ valsIn,
                                                              uint64_t* __restrict__ keysOut,
-                                                             int32_t* __restrict__ valsOut, size_t n, int shift,
+                                                             int32_t* __restrict__ valsOut, size_t n,
+                                                             const uint32_t* __restrict__ dN, int shift,
                                                              const uint32_t* __restrict__ tileOffsets, int numTiles)
 {
+    n = deviceCount(dN, n);
+    if (size_t(blockIdx.x) * kSortTile >= n) return; // a tile past the count (the whole block leaves)
     __shared__ uint64_t sk[kSortTile];
     __shared__ int32_t sv[kSortTile];
     __shared__ uint32_t wcnt[4][kDigits];
@@ -219,6 +239,15 @@ __global__ __launch_bounds__(kSortThreads) void scatterKernel(const uint64_t* __
     }
 }
 
+__global__ void copyPairsKernel(const uint64_t* __restrict__ kin, const int32_t* __restrict__ vin, size_t n,
+                                const uint32_t* __restrict__ dN, uint64_t* __restrict__ kout,
+                                int32_t* __restrict__ vout)
+{
+    n = deviceCount(dN, n);
+    for (size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += size_t(gridDim.x) * blockDim.x)
+        kout[i] = kin[i], vout[i] = vin[i];
+}
+
 __global__ void iotaKernel(int32_t* out, size_t n, int32_t start)
 {
     for (size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += size_t(gridDim.x) * blockDim.x)
@@ -285,9 +314,11 @@ size_t sortTempBytes(size_t n)
     return 256 * sizeof(uint32_t) * (2 * kPasses) + size_t(kDigits) * numTiles * sizeof(uint32_t) + 256;
 }
 
-// stable LSD sort on the lowest `digits` 8-bit digits of the keys; the result is in keys/vals
+// stable LSD sort on the lowest `digits` 8-bit digits of the keys; the result is in keys/vals. With dN, n is the
+// capacity the launches are sized for and *dN (device memory) the number of elements sorted; the sequence then
+// has no host dependence on the count and can be captured in a graph.
 int sortPairsDigits(Context* ctx, uint64_t* keys, int32_t* vals, size_t n, uint64_t* keyBuf, int32_t* valBuf,
-                    void* tmp, size_t tmpBytes, int digits)
+                    void* tmp, size_t tmpBytes, int digits, const uint32_t* dN)
 {
     if (n == 0) return RYO_OK;
     if (digits < 1 || digits > kPasses) return setError(ctx, RYO_ERR_ARG, "sort: %d digits", digits);
@@ -298,8 +329,8 @@ int sortPairsDigits(Context* ctx, uint64_t* keys, int32_t* vals, size_t n, uint6
     uint32_t* dbase    = hist + kPasses * kDigits;
     uint32_t* tileHist = dbase + kPasses * kDigits;
     hipStream_t s = ctx->stream;
-    RYO_HIP(ctx, hipMemsetAsync(hist, 0, kPasses * kDigits * sizeof(uint32_t), s));
-    histAllKernel<<<gridFor(n, 256, ctx->numCUs * 4), 256, 0, s>>>(keys, n, hist);
+    zeroWordsKernel<<<8, 256, 0, s>>>(hist, kPasses * kDigits); // a kernel, not a memset node, under capture
+    histAllKernel<<<gridFor(n, 256, ctx->numCUs * 4), 256, 0, s>>>(keys, n, dN, hist);
     digitBaseKernel<<<kPasses, 256, 0, s>>>(hist, dbase);
     uint64_t* kin = keys;
     uint64_t* kout = keyBuf;
@@ -308,17 +339,14 @@ int sortPairsDigits(Context* ctx, uint64_t* keys, int32_t* vals, size_t n, uint6
     for (int p = 0; p < digits; ++p)
     {
         int shift = 8 * p;
-        tileHistKernel<<<numTiles, kSortThreads, 0, s>>>(kin, n, shift, tileHist, numTiles);
+        tileHistKernel<<<numTiles, kSortThreads, 0, s>>>(kin, n, dN, shift, tileHist, numTiles);
         tileScanKernel<<<kDigits, 256, 0, s>>>(tileHist, numTiles, dbase + p * kDigits);
-        scatterKernel<<<numTiles, kSortThreads, 0, s>>>(kin, vin, kout, vout, n, shift, tileHist, numTiles);
+        scatterKernel<<<numTiles, kSortThreads, 0, s>>>(kin, vin, kout, vout, n, dN, shift, tileHist, numTiles);
         std::swap(kin, kout);
         std::swap(vin, vout);
     }
     if (kin != keys)
-    {
-        RYO_HIP(ctx, hipMemcpyAsync(keys, kin, n * sizeof(uint64_t), hipMemcpyDeviceToDevice, s));
-        RYO_HIP(ctx, hipMemcpyAsync(vals, vin, n * sizeof(int32_t), hipMemcpyDeviceToDevice, s));
-    }
+        copyPairsKernel<<<gridFor(n, 256, ctx->numCUs * 16), 256, 0, s>>>(kin, vin, n, dN, keys, vals);
     RYO_LAUNCH_CHECK(ctx);
     return RYO_OK;
 }
@@ -326,7 +354,7 @@ int sortPairsDigits(Context* ctx, uint64_t* keys, int32_t* vals, size_t n, uint6
 int sortPairs(Context* ctx, uint64_t* keys, int32_t* vals, size_t n, uint64_t* keyBuf, int32_t* valBuf, void* tmp,
               size_t tmpBytes)
 {
-    return sortPairsDigits(ctx, keys, vals, n, keyBuf, valBuf, tmp, tmpBytes, kPasses);
+    return sortPairsDigits(ctx, keys, vals, n, keyBuf, valBuf, tmp, tmpBytes, kPasses, nullptr);
 }
 
 /* ============================================================================ batches of systems */
@@ -350,9 +378,11 @@ __global__ __launch_bounds__(256) void batchKeysKernel(const float* __restrict__
 }
 
 // the segment of each (sorted) value as the key of a second, stable pass; pos = current position
-__global__ void segmentKeysKernel(const int32_t* __restrict__ vals, size_t n, const uint32_t* __restrict__ off, int S,
-                                  uint64_t* __restrict__ segKeys, int32_t* __restrict__ pos)
+__global__ void segmentKeysKernel(const int32_t* __restrict__ vals, size_t n, const uint32_t* __restrict__ dN,
+                                  const uint32_t* __restrict__ off, int S, uint64_t* __restrict__ segKeys,
+                                  int32_t* __restrict__ pos)
 {
+    n = deviceCount(dN, n);
     for (size_t j = size_t(blockIdx.x) * blockDim.x + threadIdx.x; j < n; j += size_t(gridDim.x) * blockDim.x)
     {
         segKeys[j] = uint64_t(segmentOf(off, S, uint32_t(vals[j])));
@@ -360,9 +390,11 @@ __global__ void segmentKeysKernel(const int32_t* __restrict__ vals, size_t n, co
     }
 }
 
-__global__ void applyPosKernel(const int32_t* __restrict__ pos, size_t n, const uint64_t* __restrict__ kin,
-                               const int32_t* __restrict__ vin, uint64_t* __restrict__ kout, int32_t* __restrict__ vout)
+__global__ void applyPosKernel(const int32_t* __restrict__ pos, size_t n, const uint32_t* __restrict__ dN,
+                               const uint64_t* __restrict__ kin, const int32_t* __restrict__ vin,
+                               uint64_t* __restrict__ kout, int32_t* __restrict__ vout)
 {
+    n = deviceCount(dN, n);
     for (size_t j = size_t(blockIdx.x) * blockDim.x + threadIdx.x; j < n; j += size_t(gridDim.x) * blockDim.x)
     {
         kout[j] = kin[pos[j]];
@@ -395,24 +427,24 @@ int batchKeys(Context* ctx, const float* x, const float* y, const float* z, size
 
 /* stable sort of (keys, vals) grouped by segment: a full sort on the keys, then a stable pass on the segment of
  * each value (segment = the s with off[s] <= val < off[s+1]); keys ascend inside each segment and equal keys
- * keep their order. keyBuf/valBuf/segKeys/segBuf/pos/posBuf: n entries each; tmp: sortTempBytes(n). */
+ * keep their order. keyBuf/valBuf/segKeys/segBuf/pos/posBuf: n entries each; tmp: sortTempBytes(n). With dN, n is
+ * the capacity and *dN (device) the element count, as in sortPairsDigits. */
 int segmentedSortPairs(Context* ctx, uint64_t* keys, int32_t* vals, size_t n, const uint32_t* off, int S,
                        uint64_t* keyBuf, int32_t* valBuf, uint64_t* segKeys, uint64_t* segBuf, int32_t* pos,
-                       int32_t* posBuf, void* tmp, size_t tmpBytes)
+                       int32_t* posBuf, void* tmp, size_t tmpBytes, const uint32_t* dN)
 {
     if (n == 0) return RYO_OK;
-    int rc = sortPairs(ctx, keys, vals, n, keyBuf, valBuf, tmp, tmpBytes);
+    int rc = sortPairsDigits(ctx, keys, vals, n, keyBuf, valBuf, tmp, tmpBytes, kPasses, dN);
     if (rc) return rc;
     if (S <= 1) return RYO_OK;
     hipStream_t s = ctx->stream;
     const unsigned grid = gridFor(n, 256, ctx->numCUs * 16);
-    segmentKeysKernel<<<grid, 256, 0, s>>>(vals, n, off, S, segKeys, pos);
+    segmentKeysKernel<<<grid, 256, 0, s>>>(vals, n, dN, off, S, segKeys, pos);
     const int digits = S <= 256 ? 1 : (S <= 65536 ? 2 : 3);
-    rc = sortPairsDigits(ctx, segKeys, pos, n, segBuf, posBuf, tmp, tmpBytes, digits);
+    rc = sortPairsDigits(ctx, segKeys, pos, n, segBuf, posBuf, tmp, tmpBytes, digits, dN);
     if (rc) return rc;
-    applyPosKernel<<<grid, 256, 0, s>>>(pos, n, keys, vals, keyBuf, valBuf);
-    RYO_HIP(ctx, hipMemcpyAsync(keys, keyBuf, n * sizeof(uint64_t), hipMemcpyDeviceToDevice, s));
-    RYO_HIP(ctx, hipMemcpyAsync(vals, valBuf, n * sizeof(int32_t), hipMemcpyDeviceToDevice, s));
+    applyPosKernel<<<grid, 256, 0, s>>>(pos, n, dN, keys, vals, keyBuf, valBuf);
+    copyPairsKernel<<<grid, 256, 0, s>>>(keyBuf, valBuf, n, dN, keys, vals);
     RYO_LAUNCH_CHECK(ctx);
     return RYO_OK;
 }
@@ -436,6 +468,37 @@ int sfcKeys(Context* ctx, const float* x, const float* y, const float* z, uint64
         sfcKeysKernel<true><<<grid, 256, 0, ctx->stream>>>(x, y, z, keys, n, b, kind, skipRemoved);
     else
         sfcKeysKernel<false><<<grid, 256, 0, ctx->stream>>>(x, y, z, keys, n, b, kind, skipRemoved);
+    RYO_LAUNCH_CHECK(ctx);
+    return RYO_OK;
+}
+
+__global__ void copyWordsKernel(uint32_t* __restrict__ dst, const uint32_t* __restrict__ src, size_t n)
+{
+    for (size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += size_t(gridDim.x) * blockDim.x)
+        dst[i] = src[i];
+}
+
+__global__ void zeroWordsKernelN(uint32_t* __restrict__ dst, size_t n)
+{
+    for (size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += size_t(gridDim.x) * blockDim.x)
+        dst[i] = 0;
+}
+
+// 4-byte word copies and fills as kernels (capturable in a graph without memcpy or memset nodes)
+int copyWords(Context* ctx, void* dst, const void* src, size_t words)
+{
+    if (words == 0) return RYO_OK;
+    copyWordsKernel<<<gridFor(words, 256, ctx->numCUs * 16), 256, 0, ctx->stream>>>(
+        static_cast<uint32_t*>(dst), static_cast<const uint32_t*>(src), words);
+    RYO_LAUNCH_CHECK(ctx);
+    return RYO_OK;
+}
+
+int zeroWords(Context* ctx, void* dst, size_t words)
+{
+    if (words == 0) return RYO_OK;
+    zeroWordsKernelN<<<gridFor(words, 256, ctx->numCUs * 16), 256, 0, ctx->stream>>>(static_cast<uint32_t*>(dst),
+                                                                                    words);
     RYO_LAUNCH_CHECK(ctx);
     return RYO_OK;
 }

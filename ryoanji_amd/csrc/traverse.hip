@@ -50,6 +50,7 @@ struct TravArgs
     const unsigned* hRange;    // softening range of targets and sources (order-preserving integers)
     unsigned long long* stats;
     uint32_t* failed;
+    const uint32_t* groupList; // when set, work item i is group groupList[i] (a retry of overflowed groups)
     int stackCap;
     size_t hCount; // targets and sources are h[0..hCount) (resident systems), 0 when unknown
     int ranges;    // contiguous group ranges, one per XCD (1: a single global counter)
@@ -335,8 +336,14 @@ __global__ __launch_bounds__(256) void hRangeKernel(const float* __restrict__ h1
         lo = min(lo, unsigned(__shfl_xor(int(lo), off)));
         hi = max(hi, unsigned(__shfl_xor(int(hi), off)));
     }
-    if ((threadIdx.x & 63) == 0)
+    // one atomic pair per block: thousands of atomics on the same two words serialize in L2
+    __shared__ unsigned wlo[4], whi[4];
+    if ((threadIdx.x & 63) == 0) wlo[threadIdx.x >> 6] = lo, whi[threadIdx.x >> 6] = hi;
+    __syncthreads();
+    if (threadIdx.x == 0)
     {
+        for (int w = 1; w < int(blockDim.x >> 6); ++w)
+            lo = min(lo, wlo[w]), hi = max(hi, whi[w]);
         atomicMin(out, lo);
         atomicMax(out + 1, hi);
     }
@@ -355,7 +362,8 @@ struct TravBatch
 {
     BatchView bv;
     const int32_t* gSys;
-    const float* G; // per system
+    const float* G;         // per system
+    const uint32_t* abort;  // nonzero: the trees were not built (arrays too small), traverse nothing
 };
 
 // 64 * kTravWaves threads, at least 6 waves per SIMD (<= 80 VGPRs; more waves would spill)
@@ -366,6 +374,8 @@ __global__ __launch_bounds__(64 * kTravWaves, 6) void traverseKernelT(TravArgs a
     const int w    = threadIdx.x >> 6;
     const int lane = threadIdx.x & 63;
     int* stack     = stackMem + w * a.stackCap;
+    if constexpr (Batched)
+        if (tb.abort && *tb.abort) return;
 
     // uniform softening: one hij^2 for every pair (the same value the general path computes)
     const unsigned hlo = a.hRange[0], hhi = a.hRange[1];
@@ -405,6 +415,7 @@ __global__ __launch_bounds__(64 * kTravWaves, 6) void traverseKernelT(TravArgs a
         }
         g = __shfl(g, 0);
         if (g == ~0u) return;
+        if (a.groupList) g = a.groupList[g];
 
         TreeRef tr{a.child, a.i2l, a.layout, a.centers, a.mp, a.xs, a.ys, a.zs, a.ms, a.hs};
         float Gs = a.G;
@@ -557,7 +568,7 @@ int traverse(Context* ctx, TravArgs a, double* totalPot)
     // the softening range over all targets and sources selects the uniform-softening P2P; without the particle
     // count (hCount = 0) the range stays empty and the general P2P runs
     if (a.hCount > 0)
-        hRangeKernel<<<std::max(1u, std::min(iceil(a.hCount, 1024), unsigned(ctx->numCUs * 8))), 256, 0, s>>>(a.ht, a.hCount,
+        hRangeKernel<<<std::max(1u, std::min(iceil(a.hCount, 4096), unsigned(ctx->numCUs * 2))), 256, 0, s>>>(a.ht, a.hCount,
                                                                             a.hs == a.ht ? nullptr : a.hs,
                                                                             a.hs == a.ht ? 0 : a.hCount,
                                                                             ctl + kCtlHRange);
@@ -665,62 +676,120 @@ __global__ __launch_bounds__(256) void systemPotKernel(const double* __restrict_
     if (threadIdx.x == 0) out[sys] = 0.5 * double(G[sys]) * (ss[0] + sc[0]);
 }
 
-/* traversal of every system of a batch in one launch: a holds the batch-wide arrays (targets = sources = the
- * packed particles, tree arrays, groups of all systems), tb the offset tables; sysPot (device, S entries)
- * receives each system's 0.5 G sum(m phi). Open boundaries; the LIFO can never overflow at full depth. */
-int traverseBatch(Context* ctx, TravArgs a, const TravBatch& tb, double* sysPot)
+// device memory a batch traversal of numGroups groups works in: per-group potentials (8 B), the overflow list
+// (4 B) and the retry list (4 B)
+size_t batchTraverseWorkBytes(size_t numGroups) { return 16 * numGroups + 256; }
+
+namespace
+{
+// launch shape and work arrays of a batch traversal (the same for its launch and its finish)
+void batchShape(Context* ctx, TravArgs& a, void* work)
+{
+    a.stackCap    = ctx->travStack > 8 ? std::min(ctx->travStack, kMaxStack) : kMaxStack;
+    a.ranges      = ctx->travRanges > 0 ? std::min(ctx->travRanges, kXcds) : kXcds;
+    unsigned* ctl = reinterpret_cast<unsigned*>(ctx->dStats);
+    a.work        = ctl;
+    a.stats       = ctx->dStats + 8;
+    a.hRange      = ctl + kCtlHRange;
+    auto* buf     = static_cast<char*>(work);
+    a.groupPot    = reinterpret_cast<double*>(buf);
+    a.failed      = reinterpret_cast<uint32_t*>(buf + 8 * size_t(a.numGroups));
+    a.groupList   = nullptr;
+}
+
+int launchBlocks(Context* ctx, size_t groups)
+{
+    const int perCU = ctx->travBlocksPerCU > 0 ? ctx->travBlocksPerCU : kDefaultBlocks;
+    return std::min(ctx->numCUs * perCU, int(iceil(groups, kTravWaves)));
+}
+} // namespace
+
+// the capturable part: init, softening range, zeroed partials, traversal, per-system potentials
+int traverseBatchLaunch(Context* ctx, const TravArgs& a, const TravBatch& tb, double* sysPot, bool events)
+{
+    hipStream_t s = ctx->stream;
+    if (a.numGroups <= 0) return RYO_OK;
+    unsigned* ctl = reinterpret_cast<unsigned*>(ctx->dStats);
+    travInitKernel<<<1, 64, 0, s>>>(ctl);
+    if (a.hCount > 0)
+        hRangeKernel<<<std::max(1u, std::min(iceil(a.hCount, 4096), unsigned(ctx->numCUs * 2))), 256, 0, s>>>(
+            a.ht, a.hCount, nullptr, 0, ctl + kCtlHRange);
+    const size_t G = size_t(a.numGroups);
+    zeroKernelU<<<std::min(iceil(2 * G, 256), 4096u), 256, 0, s>>>(reinterpret_cast<unsigned*>(a.groupPot),
+                                                                   int(2 * G));
+    if (events && ctx->timing) RYO_HIP(ctx, hipEventRecord(ctx->ev[0], s));
+    traverseKernelT<true><<<launchBlocks(ctx, G), 64 * kTravWaves, size_t(kTravWaves) * a.stackCap * sizeof(int),
+                            s>>>(a, tb);
+    if (events && ctx->timing) RYO_HIP(ctx, hipEventRecord(ctx->ev[1], s));
+    RYO_LAUNCH_CHECK(ctx);
+    systemPotKernel<<<tb.bv.S, 256, 0, s>>>(a.groupPot, tb.bv, tb.G, sysPot);
+    RYO_LAUNCH_CHECK(ctx);
+    return RYO_OK;
+}
+
+// statistics of a finished batch traversal (one host synchronization); groups that overflowed the LIFO wrote
+// nothing and are traversed again at full depth, after which the per-system potentials are summed again
+int traverseBatchFinish(Context* ctx, TravArgs a, const TravBatch& tb, double* sysPot, bool events)
 {
     hipStream_t s = ctx->stream;
     for (int k = 0; k < 6; ++k)
         ctx->hStats[k] = 0;
+    ctx->retriedGroups = 0;
     if (a.numGroups <= 0) return RYO_OK;
-    a.stackCap      = kMaxStack;
-    const int perCU = ctx->travBlocksPerCU > 0 ? ctx->travBlocksPerCU : kDefaultBlocks;
-    a.ranges        = ctx->travRanges > 0 ? std::min(ctx->travRanges, kXcds) : kXcds;
-    const size_t lds = size_t(kTravWaves) * kMaxStack * sizeof(int);
-    unsigned* ctl             = reinterpret_cast<unsigned*>(ctx->dStats);
-    unsigned long long* stats = ctx->dStats + 8;
-    a.work   = ctl;
-    a.stats  = stats;
-    a.hRange = ctl + kCtlHRange;
-    travInitKernel<<<1, 64, 0, s>>>(ctl);
-    if (a.hCount > 0)
-        hRangeKernel<<<std::max(1u, std::min(iceil(a.hCount, 1024), unsigned(ctx->numCUs * 8))), 256, 0, s>>>(a.ht, a.hCount, nullptr, 0,
-                                                                            ctl + kCtlHRange);
-    const size_t G = size_t(a.numGroups);
-    auto* buf      = static_cast<char*>(ctx->getScratch(kSlotTraverse, 12 * G + 256));
-    if (!buf) return RYO_ERR_HIP;
-    a.groupPot = reinterpret_cast<double*>(buf);
-    a.failed   = reinterpret_cast<uint32_t*>(buf + 8 * G);
-    zeroKernelU<<<std::min(iceil(2 * G, 256), 4096u), 256, 0, s>>>(reinterpret_cast<unsigned*>(a.groupPot),
-                                                                   int(2 * G));
-    int blocks = std::min(ctx->numCUs * perCU, int(iceil(G, kTravWaves)));
-    if (ctx->timing) RYO_HIP(ctx, hipEventRecord(ctx->ev[0], s));
-    traverseKernelT<true><<<blocks, 64 * kTravWaves, lds, s>>>(a, tb);
-    if (ctx->timing) RYO_HIP(ctx, hipEventRecord(ctx->ev[1], s));
-    RYO_LAUNCH_CHECK(ctx);
-    systemPotKernel<<<tb.bv.S, 256, 0, s>>>(a.groupPot, tb.bv, tb.G, sysPot);
     unsigned long long h[7] = {0, 0, 0, 0, 0, 0, 0};
-    RYO_HIP(ctx, hipMemcpyAsync(h, stats, sizeof(h), hipMemcpyDeviceToHost, s));
+    RYO_HIP(ctx, hipMemcpyAsync(h, a.stats, sizeof(h), hipMemcpyDeviceToHost, s));
     RYO_HIP(ctx, hipStreamSynchronize(s));
-    if (ctx->timing)
+    if (events && ctx->timing)
     {
         float ms = 0;
         RYO_HIP(ctx, hipEventElapsedTime(&ms, ctx->ev[0], ctx->ev[1]));
         ctx->lastTraverseMs = ms;
     }
-    if ((h[5] & 0xffffffffu) != 0 || h[6] != 0)
-        return setError(ctx, RYO_ERR_TRAVERSAL, "batch traversal: %llu groups failed", h[5] & 0xffffffffu);
+    if (h[6] != 0) return setError(ctx, RYO_ERR_TRAVERSAL, "batch traversal: step limit exceeded (corrupt tree?)");
+    const unsigned numFailed = unsigned(h[5] & 0xffffffffu);
+    if (numFailed > 0)
+    {
+        if (a.stackCap >= kMaxStack)
+            return setError(ctx, RYO_ERR_TRAVERSAL, "batch traversal: %u groups overflowed a full-depth LIFO",
+                            numFailed);
+        ctx->retriedGroups = numFailed;
+        // the retry reads its groups from a copy of the overflow list (it records overflows into the list)
+        auto* list = reinterpret_cast<uint32_t*>(reinterpret_cast<char*>(a.groupPot) + 12 * size_t(a.numGroups));
+        RYO_HIP(ctx, hipMemcpyAsync(list, a.failed, numFailed * sizeof(uint32_t), hipMemcpyDeviceToDevice, s));
+        zeroKernelU<<<1, 64, 0, s>>>(a.work, kXcds);
+        zeroKernelU<<<1, 64, 0, s>>>(reinterpret_cast<unsigned*>(a.stats + 5), 4);
+        a.groupList = list;
+        a.numGroups = int(numFailed);
+        a.stackCap  = kMaxStack;
+        traverseKernelT<true><<<launchBlocks(ctx, numFailed), 64 * kTravWaves,
+                                size_t(kTravWaves) * kMaxStack * sizeof(int), s>>>(a, tb);
+        RYO_LAUNCH_CHECK(ctx);
+        systemPotKernel<<<tb.bv.S, 256, 0, s>>>(a.groupPot, tb.bv, tb.G, sysPot);
+        RYO_LAUNCH_CHECK(ctx);
+        unsigned long long r[7] = {0, 0, 0, 0, 0, 0, 0};
+        RYO_HIP(ctx, hipMemcpyAsync(r, a.stats, sizeof(r), hipMemcpyDeviceToHost, s));
+        RYO_HIP(ctx, hipStreamSynchronize(s));
+        if ((r[5] & 0xffffffffu) != 0 || r[6] != 0)
+            return setError(ctx, RYO_ERR_TRAVERSAL, "batch traversal: retry at full depth failed");
+        h[0] = r[0], h[2] = r[2]; // sums now include the retried groups
+        for (int k : {1, 3, 4})
+            h[k] = std::max(h[k], r[k]);
+    }
     for (int k = 0; k < 5; ++k)
         ctx->hStats[k] = h[k];
     return RYO_OK;
 }
 
+/* traversal of every system of a batch: the batch-wide arrays (targets = sources = the packed particles, tree
+ * arrays, groups of all systems) and the offset tables; sysPot (device, S entries) receives each system's
+ * 0.5 G sum(m phi); work: batchTraverseWorkBytes(numGroups) bytes of device memory; abort (device, optional):
+ * nonzero when the trees were not built. Open boundaries. mode 0: launch and finish; 1: launch only (no host
+ * synchronization, e.g. under graph capture); 2: finish only, after a mode-1 sequence with the same arguments. */
 int batchTraverse(Context* ctx, int numGroups, const uint32_t* gStart, const uint32_t* gEnd, const int32_t* gSys,
                   const BatchView& bv, const float* G, const float* x, const float* y, const float* z,
                   const float* m, const float* h, size_t n, const int32_t* child, const int32_t* i2l,
                   const uint32_t* layout, const float* centers, const float* mp, float* p, float* ax, float* ay,
-                  float* az, double* sysPot, int initNode)
+                  float* az, double* sysPot, int initNode, void* work, const uint32_t* abort, int mode)
 {
     TravArgs a{};
     a.numGroups = numGroups;
@@ -733,8 +802,12 @@ int batchTraverse(Context* ctx, int numGroups, const uint32_t* gStart, const uin
     a.G = 1.0f, a.numShells = 0;
     a.p = p, a.ax = ax, a.ay = ay, a.az = az;
     a.hCount = n;
-    TravBatch tb{bv, gSys, G};
-    return traverseBatch(ctx, a, tb, sysPot);
+    batchShape(ctx, a, work);
+    TravBatch tb{bv, gSys, G, abort};
+    int rc = RYO_OK;
+    if (mode != 2) rc = traverseBatchLaunch(ctx, a, tb, sysPot, mode == 0);
+    if (!rc && mode != 1) rc = traverseBatchFinish(ctx, a, tb, sysPot, mode == 0);
+    return rc;
 }
 
 int traverseEntry(Context* ctx, int numGroups, const uint32_t* gs, const uint32_t* ge, int initNode, const float* xt,

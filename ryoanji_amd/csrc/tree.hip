@@ -443,6 +443,43 @@ __global__ void batchLeafOffsetsKernel(BatchView bv, const uint32_t* __restrict_
         LO[s] = s < bv.S ? blockOffsets[bv.TS[s]] : blockOffsets[numBlocks];
 }
 
+// node, internal-node and parent-slot offsets of every system from its leaf-slot offsets (a system with L leaves
+// has L + 1 slots, (L - 1) / 7 internal nodes, L + (L - 1) / 7 nodes and max(1, (nodes - 1) / 8) parent slots).
+// *overflow = whether a total exceeds the capacity the arrays were sized for; on overflow every table is
+// cleared, so each later stage sees empty trees and stays inside its arrays (the traversal then does nothing).
+__global__ __launch_bounds__(256) void batchNodeTablesKernel(int S, uint32_t* __restrict__ LO, uint32_t* __restrict__ NO,
+                                                             uint32_t* __restrict__ NIO, uint32_t* __restrict__ PO,
+                                                             uint32_t capSlots, uint32_t capNodes, uint32_t capParents,
+                                                             uint32_t* __restrict__ overflow)
+{
+    __shared__ uint32_t sw[4];
+    __shared__ uint32_t over;
+    uint32_t cN = 0, cI = 0, cP = 0;
+    for (int s0 = 0; s0 < S; s0 += 256)
+    {
+        const int s      = s0 + int(threadIdx.x);
+        const uint32_t L = s < S ? LO[s + 1] - LO[s] - 1 : 1u;
+        uint32_t nI = s < S ? (L - 1) / 7 : 0u, nn = s < S ? L + nI : 0u;
+        uint32_t np = s < S ? max(1u, (nn - 1) / 8) : 0u;
+        uint32_t tN, tI, tP;
+        const uint32_t eN = blockExclusiveScan256(nn, sw, &tN);
+        const uint32_t eI = blockExclusiveScan256(nI, sw, &tI);
+        const uint32_t eP = blockExclusiveScan256(np, sw, &tP);
+        if (s < S) NO[s] = cN + eN, NIO[s] = cI + eI, PO[s] = cP + eP;
+        cN += tN, cI += tI, cP += tP;
+    }
+    if (threadIdx.x == 0)
+    {
+        NO[S] = cN, NIO[S] = cI, PO[S] = cP;
+        over      = (LO[S] > capSlots || cN > capNodes || cP > capParents) ? 1u : 0u;
+        *overflow = over;
+    }
+    __syncthreads();
+    if (over)
+        for (int s = threadIdx.x; s <= S; s += blockDim.x)
+            LO[s] = 0, NO[s] = 0, NIO[s] = 0, PO[s] = 0;
+}
+
 // particle counts per leaf slot (0 at the terminator slots)
 __global__ void batchLeafCountsKernel(const uint32_t* __restrict__ layout, BatchView bv, uint32_t* __restrict__ counts)
 {
@@ -586,6 +623,17 @@ int batchLeavesCount(Context* ctx, const uint64_t* sk, const BatchView& bv, int 
     batchLeafCountKernel<<<numTiles, kLeafThreads, 0, s>>>(sk, bv, B, blockSums);
     scanBlocksKernel<<<1, 256, 0, s>>>(blockSums, numTiles);
     batchLeafOffsetsKernel<<<iceil(bv.S + 1, 256), 256, 0, s>>>(bv, blockSums, numTiles, LO);
+    RYO_LAUNCH_CHECK(ctx);
+    return RYO_OK;
+}
+
+int batchNodeTables(Context* ctx, const BatchView& bv, uint32_t capSlots, uint32_t capNodes, uint32_t capParents,
+                    uint32_t* overflow)
+{
+    auto* t = const_cast<uint32_t*>(bv.LO);
+    batchNodeTablesKernel<<<1, 256, 0, ctx->stream>>>(bv.S, t, const_cast<uint32_t*>(bv.NO),
+                                                      const_cast<uint32_t*>(bv.NIO), const_cast<uint32_t*>(bv.PO),
+                                                      capSlots, capNodes, capParents, overflow);
     RYO_LAUNCH_CHECK(ctx);
     return RYO_OK;
 }

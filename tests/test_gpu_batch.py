@@ -111,3 +111,118 @@ def test_autotuner_picks_a_candidate(ctx):
     pots2 = b.solve(prm)
     assert np.array_equal(pots, pots2)
     b.close()
+
+
+def _batch_outputs(b):
+    (x, y, z, m, h, p, ax, ay, az), order = b.results()
+    out = {"arrays": [host(t) for t in (x, y, z, m, h, p, ax, ay, az)], "order": host(order)}
+    out.update({k: host(v) for k, v in b.tree().items()})
+    out.update({"off_" + k: v.copy() for k, v in b.offsets().items()})
+    return out
+
+
+def _assert_same(a, b):
+    assert a.keys() == b.keys()
+    for k in a:
+        if k == "arrays":
+            for i, (u, v) in enumerate(zip(a[k], b[k])):
+                assert np.array_equal(u.view(np.int32), v.view(np.int32)), (k, i)
+        else:
+            assert np.array_equal(a[k], b[k]), k
+
+
+def _clumped(sizes, seed):
+    """systems whose particles sit in two clumps 1e-5 wide at opposite corners: deep trees, many leaf slots"""
+    out = []
+    for i, n in enumerate(sizes):
+        u = [W.uniform01(W._stream(seed + i, d), 0, n) for d in range(3)]
+        side = np.where(np.arange(n) % 2 == 0, -0.9, 0.9)
+        xyz = [(side + 1e-5 * c).astype(np.float32) for c in u]
+        out.append(W.System(*xyz, np.full(n, 1.0 / n, np.float32), np.full(n, 1e-3, np.float32),
+                            [-1, 1, -1, 1, -1, 1]))
+    return out
+
+
+def test_graph_solve_equals_eager(ctx):
+    """the captured pipeline gives the eager solve's arrays bit for bit, on capture, on replay, and on a replay
+    after new positions of the same counts were loaded"""
+    systems = _systems()
+    b = R.Batch(ctx, sum(s.n for s in systems), len(systems))
+    prm = R.default_params()
+    thetas = np.array([0.4, 0.5, 0.7, 0.6] * 4, np.float32)[: len(systems)]
+    b.load(systems)
+    pe = b.solve(prm, theta=thetas)
+    eager = _batch_outputs(b)
+    pg = b.solve(prm, theta=thetas, graph=True)  # exact solve, capture, replay
+    _assert_same(eager, _batch_outputs(b))
+    assert np.array_equal(pe, pg)
+    pg2 = b.solve(prm, theta=thetas, graph=True)  # replay
+    _assert_same(eager, _batch_outputs(b))
+    assert np.array_equal(pe, pg2)
+    st = b.graph_stats()
+    assert st["captures"] == 1 and st["replays"] == 2 and st["fallbacks"] == 0, st
+    # new positions, same counts: the graph is replayed with them (and another theta)
+    moved = [W.System(s.x * np.float32(0.5), s.y, s.z, s.m, s.h, s.limits) for s in systems]
+    b.load(moved)
+    pg3 = b.solve(prm, theta=thetas[::-1].copy(), graph=True)
+    graph3 = _batch_outputs(b)
+    assert b.graph_stats()["captures"] == 1
+    pe3 = b.solve(prm, theta=thetas[::-1].copy())
+    _assert_same(_batch_outputs(b), graph3)
+    assert np.array_equal(pe3, pg3)
+    b.close()
+
+
+def test_graph_solve_outgrowing_its_arrays(ctx):
+    """trees that outgrow the captured arrays are detected on the device and solved exactly; the next call
+    captures again for the grown arrays"""
+    sizes = [4096] * 12
+    flat = [W.uniform_cube(n, seed=70 + i) for i, n in enumerate(sizes)]
+    deep = _clumped(sizes, 90)
+    b = R.Batch(ctx, sum(sizes), len(sizes))
+    prm = R.default_params()
+    b.load(flat)
+    b.solve(prm, graph=True)
+    cap = b.graph_stats()["capacity"]
+    b.load(deep)
+    pg = b.solve(prm, graph=True)  # replayed, outgrows the arrays, solved exactly
+    st = b.graph_stats()
+    assert st["fallbacks"] == 1 and st["captures"] == 1 and st["replays"] == 2, st
+    fallback = _batch_outputs(b)
+    pe = b.solve(prm)
+    eager = _batch_outputs(b)
+    assert int(eager["off_LO"][-1]) > cap  # the deep trees need more leaf slots than the graph holds
+    _assert_same(eager, fallback)
+    assert np.array_equal(pe, pg)
+    pg2 = b.solve(prm, graph=True)  # captured again for the grown arrays, then replayed
+    _assert_same(eager, _batch_outputs(b))
+    assert np.array_equal(pe, pg2)
+    st = b.graph_stats()
+    assert st["captures"] == 2 and st["fallbacks"] == 1, st
+    b.close()
+
+
+def test_shallow_lifo_retries_at_full_depth(ctx):
+    """with a LIFO too shallow for some groups, those groups are traversed again at full depth and every
+    result equals the full-depth solve, in both ways of solving"""
+    systems = [W.plummer(20000, seed=7), W.uniform_cube(9000, seed=8), W.plummer(3000, seed=9)]
+    b = R.Batch(ctx, sum(s.n for s in systems), len(systems))
+    prm = R.default_params()
+    b.load(systems)
+    pe = b.solve(prm)
+    full = _batch_outputs(b)
+    depth = ctx.stats()["maxStack"]  # the deepest LIFO any group needed
+    assert depth > 18
+    try:
+        ctx.call("ryo_ctx_set_traversal_shape", int(depth // 2), 0, 0)
+        ps = b.solve(prm)
+        assert R.lib().ryo_traversal_retries(ctx.h) > 0
+        _assert_same(full, _batch_outputs(b))
+        assert np.array_equal(pe, ps)
+        pg = b.solve(prm, graph=True)
+        assert R.lib().ryo_traversal_retries(ctx.h) > 0
+        _assert_same(full, _batch_outputs(b))
+        assert np.array_equal(pe, pg)
+    finally:
+        ctx.call("ryo_ctx_set_traversal_shape", 0, 0, 0)
+    b.close()
