@@ -192,8 +192,8 @@ def batching_bench(classes=(12, 14, 16), total_exp=22, seed=9):
     """Many small Plummer systems (2^total_exp particles in all, systems of 2^c particles for each size class c):
     one batched solve (one launch sequence for all systems) against solving them one at a time on a resident
     system. The batched solve runs eagerly (one host round trip sizes the node arrays) and as one captured HIP
-    graph; the graph's launch configuration (group size, traversal blocks per CU, XCD ranges, LIFO depth) is
-    tuned per size class by the autotuner. Inputs are resident on the device for all of them."""
+    graph; the graph's launch configuration (group size, traversal block size, waves per CU, XCD ranges, LIFO
+    depth) is tuned per size class by the autotuner. Inputs are resident on the device for all of them."""
     import numpy as np
     import torch
 
@@ -232,8 +232,7 @@ def batching_bench(classes=(12, 14, 16), total_exp=22, seed=9):
         rows.append(dict(size_class=f"2^{c}", systems=K, particles=n * K,
                          one_at_a_time_ms=t_one * 1e3, batched_eager_ms=t_eager * 1e3,
                          batched_graph_default_ms=t_default * 1e3, batched_graph_tuned_ms=t_best * 1e3,
-                         tuned={"group_size": best[0], "blocks_per_cu": best[1], "group_ranges": best[2],
-                                "lifo_entries_per_wave": best[3] or 22 * 64},
+                         tuned={**best, "lifo": best["lifo"] or 22 * 64}, configurations_timed=len(timings),
                          graph_stats=graph_stats,
                          particles_per_s_one_at_a_time=n * K / t_one, particles_per_s_batched=n * K / t_best,
                          speedup=t_one / t_best))

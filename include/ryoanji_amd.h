@@ -52,9 +52,12 @@ double ryo_last_traverse_ms(const ryo_ctx* ctx);
 /* Launch shape of the traversal on this context: LIFO entries per wave held in LDS (9..1408; a walk that needs
  * more takes the split/retry path), blocks of 4 waves launched per CU, and the number of contiguous group
  * ranges with their own work counters (1..8; blocks start on range blockIdx % ranges, which under round-robin
- * dispatch keeps each XCD's L2 on SFC-neighbouring groups). 0 keeps the default (1408, 6, 8).
+ * dispatch keeps each XCD's L2 on SFC-neighbouring groups). 0 keeps the default (1408, 24 waves per CU, 8).
  * Replaces the compile-time TravConfig (traversal_gpu.cu:45-64), for the autotuner. */
 int ryo_ctx_set_traversal_shape(ryo_ctx* ctx, int stack_entries_per_wave, int blocks_per_cu, int group_ranges);
+/* Waves per traversal block: 2, 4 or 8 (0: 4). Every block size keeps 6 waves per SIMD resident; the default
+ * blocks per CU follow (12, 6, 3). The block size of the reference's TravConfig (traversal_gpu.cu:45-64). */
+int ryo_ctx_set_traversal_waves(ryo_ctx* ctx, int waves_per_block);
 
 /* ---------------------------------------------------------------- a2: SFC keys */
 

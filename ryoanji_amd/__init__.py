@@ -123,6 +123,7 @@ SIGNATURES = {
     "ryo_batch_load": (_I, [_P, _I, _P, _P, _P, _P, _P, _P, _P]),
     "ryo_batch_solve": (_I, [_P, C.POINTER(Params), _P, _P, _P]),
     "ryo_batch_solve_graph": (_I, [_P, C.POINTER(Params), _P, _P, _P]),
+    "ryo_ctx_set_traversal_waves": (_I, [_P, _I]),
     "ryo_batch_graph_stats": (_I, [_P, _P]),
     "ryo_batch_results": (_I, [_P, _P, _P]),
     "ryo_batch_offsets": (_I, [_P, _P, _P, _P, _P, _P]),

@@ -382,7 +382,7 @@ std::vector<uint64_t> graphKey(const ryo_batch* b, const Shape& sh)
     const Context* ctx = b->ctx;
     std::vector<uint64_t> k = {uint64_t(b->S), uint64_t(b->N), uint64_t(sh.B), uint64_t(sh.gs),
                                uint64_t(sh.initNode), uint64_t(ctx->travStack), uint64_t(ctx->travBlocksPerCU),
-                               uint64_t(ctx->travRanges)};
+                               uint64_t(ctx->travRanges), uint64_t(ctx->travWaves)};
     k.insert(k.end(), b->hP.begin(), b->hP.end());
     return k;
 }

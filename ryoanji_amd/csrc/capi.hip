@@ -157,6 +157,15 @@ int ryo_ctx_set_traversal_shape(ryo_ctx* h, int stack_entries_per_wave, int bloc
     return RYO_OK;
 }
 
+int ryo_ctx_set_traversal_waves(ryo_ctx* h, int waves_per_block)
+{
+    CTX_OR_FAIL(h);
+    if (waves_per_block != 0 && waves_per_block != 2 && waves_per_block != 4 && waves_per_block != 8)
+        return setError(ctx, RYO_ERR_ARG, "traversal waves per block: 2, 4 or 8 (0: 4)");
+    ctx->travWaves = waves_per_block;
+    return RYO_OK;
+}
+
 double ryo_last_traverse_ms(const ryo_ctx* h) { return h ? h->c.lastTraverseMs : 0.0; }
 
 int ryo_sfc_keys_f32(ryo_ctx* h, const float* x, const float* y, const float* z, uint64_t* keys, size_t n,

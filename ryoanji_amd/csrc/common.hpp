@@ -162,6 +162,7 @@ struct Context
     int travStack       = 0;
     int travBlocksPerCU = 0;
     int travRanges      = 0; // group ranges with their own counters (0: one per XCD)
+    int travWaves       = 0; // waves per traversal block (2, 4 or 8; 0: 4)
 
     // event timing of the last traversal kernel launch (ms), when enabled
     bool timing          = false;

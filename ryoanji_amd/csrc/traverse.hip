@@ -17,10 +17,10 @@
 namespace ryo
 {
 
-constexpr int kTravWaves     = 4;       // waves per block
+constexpr int kTravWaves     = 4;       // default waves per block (2, 4 or 8: ryo_ctx_set_traversal_waves)
 constexpr int kMaxStack      = 22 * 64; // LIFO entries per wave that can never overflow (>= 21 levels x 64)
 constexpr int kDefaultStack  = kMaxStack; // default LIFO entries per wave (a smaller one may take the retry path)
-constexpr int kDefaultBlocks = 6;         // default blocks per CU: 6 waves per SIMD (the VGPR budget below)
+constexpr int kWavesPerCU    = 24;        // default resident waves per CU: 6 per SIMD (the VGPR budget below)
 constexpr int kXcds          = 8;       // MI355X: 8 XCDs, workgroups dispatched round-robin over them
 
 // device control words (ctx->dStats, 32 x u64): u32 [0..7] per-XCD group counters, u64 [8..14] statistics
@@ -366,9 +366,9 @@ struct TravBatch
     const uint32_t* abort;  // nonzero: the trees were not built (arrays too small), traverse nothing
 };
 
-// 64 * kTravWaves threads, at least 6 waves per SIMD (<= 80 VGPRs; more waves would spill)
-template<bool Batched>
-__global__ __launch_bounds__(64 * kTravWaves, 6) void traverseKernelT(TravArgs a, TravBatch tb)
+// 64 * Waves threads; the second bound is the minimum number of waves per SIMD: 6 (<= 80 VGPRs; more would spill)
+template<bool Batched, int Waves>
+__global__ __launch_bounds__(64 * Waves, kWavesPerCU / 4) void traverseKernelT(TravArgs a, TravBatch tb)
 {
     extern __shared__ int stackMem[];
     const int w    = threadIdx.x >> 6;
@@ -413,7 +413,11 @@ __global__ __launch_bounds__(64 * kTravWaves, 6) void traverseKernelT(TravArgs a
                 ++src;
             }
         }
-        g = __shfl(g, 0);
+        // batched: lane 0 is the first active lane here, so the group index becomes a scalar value and the
+        // group's system and that system's tree pointers are scalar loads and scalar registers (as vector values
+        // the ten pointers spill); a single system keeps the broadcast, measured 0.25 % faster there
+        if constexpr (Batched) g = unsigned(__builtin_amdgcn_readfirstlane(int(g)));
+        else g = __shfl(g, 0);
         if (g == ~0u) return;
         if (a.groupList) g = a.groupList[g];
 
@@ -493,6 +497,28 @@ __global__ __launch_bounds__(64 * kTravWaves, 6) void traverseKernelT(TravArgs a
     }
 }
 
+// waves per block of this context's traversal launches, resident blocks per CU, and the launch itself
+int travWaves(const Context* ctx) { return ctx->travWaves == 2 || ctx->travWaves == 8 ? ctx->travWaves : kTravWaves; }
+
+int travBlocks(const Context* ctx, size_t groups)
+{
+    const int W     = travWaves(ctx);
+    const int perCU = ctx->travBlocksPerCU > 0 ? ctx->travBlocksPerCU : kWavesPerCU / W;
+    return std::max(1, std::min(ctx->numCUs * perCU, int(iceil(groups, W))));
+}
+
+template<bool Batched>
+void launchTraversal(const Context* ctx, size_t groups, int stackCap, hipStream_t s, const TravArgs& a,
+                     const TravBatch& tb)
+{
+    const int W       = travWaves(ctx);
+    const int blocks  = travBlocks(ctx, groups);
+    const size_t lds  = size_t(W) * stackCap * sizeof(int);
+    if (W == 2) traverseKernelT<Batched, 2><<<blocks, 128, lds, s>>>(a, tb);
+    else if (W == 8) traverseKernelT<Batched, 8><<<blocks, 512, lds, s>>>(a, tb);
+    else traverseKernelT<Batched, 4><<<blocks, 256, lds, s>>>(a, tb);
+}
+
 // fixed-order FP64 sum (Neumaier) of v[0..n): block b reduces the contiguous chunk b of `chunk` values into out[b]
 // (one block with chunk = n gives the total); the chunking depends only on n, so the result is deterministic
 __global__ __launch_bounds__(256) void sumKernel(const double* __restrict__ v, int n, double* __restrict__ out,
@@ -554,9 +580,7 @@ int traverse(Context* ctx, TravArgs a, double* totalPot)
     // by the caller's stack capacity (the overflow path splits groups and retries them)
     const int ldsStack = ctx->travStack > 8 ? std::min(ctx->travStack, kMaxStack) : kDefaultStack;
     a.stackCap         = (a.stackCap > 8 && a.stackCap < ldsStack) ? a.stackCap : ldsStack;
-    const int perCU    = ctx->travBlocksPerCU > 0 ? ctx->travBlocksPerCU : kDefaultBlocks;
     a.ranges           = ctx->travRanges > 0 ? std::min(ctx->travRanges, kXcds) : kXcds;
-    const size_t lds   = size_t(kTravWaves) * ldsStack * sizeof(int);
 
     unsigned* ctl             = reinterpret_cast<unsigned*>(ctx->dStats);
     unsigned long long* stats = ctx->dStats + 8;
@@ -592,9 +616,8 @@ int traverse(Context* ctx, TravArgs a, double* totalPot)
         }
         zeroKernelU<<<std::min(iceil(2 * G, 256), 4096u), 256, 0, s>>>(reinterpret_cast<unsigned*>(a.groupPot),
                                                                        int(2 * G));
-        int blocks = std::min(ctx->numCUs * perCU, int(iceil(G, kTravWaves)));
         if (ctx->timing && pass == 0) RYO_HIP(ctx, hipEventRecord(ctx->ev[0], s));
-        traverseKernelT<false><<<blocks, 64 * kTravWaves, lds, s>>>(a, TravBatch{});
+        launchTraversal<false>(ctx, G, ldsStack, s, a, TravBatch{});
         if (ctx->timing && pass == 0) RYO_HIP(ctx, hipEventRecord(ctx->ev[1], s));
         RYO_LAUNCH_CHECK(ctx);
         sumDeterministic(s, a.groupPot, a.numGroups, partials, potSum);
@@ -697,11 +720,6 @@ void batchShape(Context* ctx, TravArgs& a, void* work)
     a.groupList   = nullptr;
 }
 
-int launchBlocks(Context* ctx, size_t groups)
-{
-    const int perCU = ctx->travBlocksPerCU > 0 ? ctx->travBlocksPerCU : kDefaultBlocks;
-    return std::min(ctx->numCUs * perCU, int(iceil(groups, kTravWaves)));
-}
 } // namespace
 
 // the capturable part: init, softening range, zeroed partials, traversal, per-system potentials
@@ -718,8 +736,7 @@ int traverseBatchLaunch(Context* ctx, const TravArgs& a, const TravBatch& tb, do
     zeroKernelU<<<std::min(iceil(2 * G, 256), 4096u), 256, 0, s>>>(reinterpret_cast<unsigned*>(a.groupPot),
                                                                    int(2 * G));
     if (events && ctx->timing) RYO_HIP(ctx, hipEventRecord(ctx->ev[0], s));
-    traverseKernelT<true><<<launchBlocks(ctx, G), 64 * kTravWaves, size_t(kTravWaves) * a.stackCap * sizeof(int),
-                            s>>>(a, tb);
+    launchTraversal<true>(ctx, G, a.stackCap, s, a, tb);
     if (events && ctx->timing) RYO_HIP(ctx, hipEventRecord(ctx->ev[1], s));
     RYO_LAUNCH_CHECK(ctx);
     systemPotKernel<<<tb.bv.S, 256, 0, s>>>(a.groupPot, tb.bv, tb.G, sysPot);
@@ -761,8 +778,7 @@ int traverseBatchFinish(Context* ctx, TravArgs a, const TravBatch& tb, double* s
         a.groupList = list;
         a.numGroups = int(numFailed);
         a.stackCap  = kMaxStack;
-        traverseKernelT<true><<<launchBlocks(ctx, numFailed), 64 * kTravWaves,
-                                size_t(kTravWaves) * kMaxStack * sizeof(int), s>>>(a, tb);
+        launchTraversal<true>(ctx, numFailed, kMaxStack, s, a, tb);
         RYO_LAUNCH_CHECK(ctx);
         systemPotKernel<<<tb.bv.S, 256, 0, s>>>(a.groupPot, tb.bv, tb.G, sysPot);
         RYO_LAUNCH_CHECK(ctx);

@@ -98,18 +98,65 @@ def test_autotuner_picks_a_candidate(ctx):
     from ryoanji_amd import autotune as A
 
     systems = [W.plummer(2048, seed=40 + i) for i in range(16)]
-    tuner = A.Autotuner(ctx, candidates=[(64, 6, 8), (32, 6, 8), (64, 4, 1)], reps=1)
+    tuner = A.Autotuner(ctx, candidates=[(64, 6, 8), (32, 6, 8), (64, 4, 1), (64, 12, 8, 512, 2)], reps=1)
     best, timings = tuner.tune(systems)
-    assert best in [c for c, _ in timings] and len(timings) == 3
+    assert best in [c for c, _ in timings] and len(timings) == 4
     assert tuner.config_for(systems) == best
     # the tuned configuration changes the launch, not the results of its group size
     prm = tuner.apply(best)
     b = R.Batch(ctx, sum(s.n for s in systems), len(systems))
     b.load(systems)
-    pots = b.solve(prm)
+    pots = b.solve(prm, graph=True)
     tuner.reset()
     pots2 = b.solve(prm)
     assert np.array_equal(pots, pots2)
+    b.close()
+
+
+def test_autotuner_coordinate_descent(ctx):
+    """the default search visits every value of every dimension once and keeps a configuration it timed"""
+    from ryoanji_amd import autotune as A
+
+    systems = [W.plummer(1024, seed=60 + i) for i in range(8)]
+    tuner = A.Autotuner(ctx, reps=1)
+    best, timings = tuner.tune(systems)
+    seen = {A._key(c) for c, _ in timings}
+    for dim, values in A.DIMENSIONS.items():
+        assert {c[dim] for c, _ in timings} == set(values), dim
+    assert A._key(best) in seen and len(seen) <= sum(len(v) for v in A.DIMENSIONS.values())
+
+
+@pytest.mark.parametrize("waves", [2, 8])
+def test_block_size_keeps_results(ctx, waves):
+    """2 or 8 waves per traversal block: the same results as the default 4, single system and batch"""
+    s = W.plummer(30000, seed=12)
+    prm = R.default_params()
+
+    def solve_system():
+        sysm = R.System(ctx, s.n)
+        sysm.load(*(dev(a) for a in s.arrays()), s.limits)
+        sysm.build(prm)
+        pot = sysm.gravity(prm)
+        out = [host(t) for t in sysm.results()]
+        sysm.close()
+        return pot, out
+
+    systems = [W.plummer(5000, seed=13 + i) for i in range(5)]
+    b = R.Batch(ctx, sum(q.n for q in systems), len(systems))
+    b.load(systems)
+    ref_sys = solve_system()
+    ref_batch = b.solve(prm)
+    ref_out = _batch_outputs(b)
+    try:
+        ctx.call("ryo_ctx_set_traversal_waves", waves)
+        got = solve_system()
+        assert got[0] == ref_sys[0]
+        for u, v in zip(got[1], ref_sys[1]):
+            assert np.array_equal(u, v)
+        assert np.array_equal(b.solve(prm, graph=True), ref_batch)
+        _assert_same(ref_out, _batch_outputs(b))
+    finally:
+        ctx.call("ryo_ctx_set_traversal_waves", 0)
     b.close()
 
 
