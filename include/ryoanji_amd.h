@@ -192,6 +192,17 @@ int ryo_direct_sum_f64(ryo_ctx* ctx, const float* x, const float* y, const float
                        const float* m, size_t n, const int64_t* targets, size_t num_targets, double G,
                        const double* box_l, int num_shells, double* out);
 
+/* The exact periodic potential and acceleration of the softened particles in a cubic box of side box_length,
+ * in FP64, for a list of targets: the Ewald sum over all n sources and all their images (the convention of
+ * ewald.hpp:201-327: neutralizing background), with pair images closer than h_i + h_j interacting by the
+ * softened P2P<Exact> law as in the traversal. No reference counterpart: the reference has no periodic FP64
+ * reference (global_forces_gpu.cpp:45-47); this is the accuracy oracle of traversal + Ewald correction.
+ * alpha_scale = alpha L in (5, 16] (0: 6.5); the result does not depend on it beyond ~1e-12.
+ * out = double[num_targets][4] {G m_t phi_t, G a}. */
+int ryo_ewald_pair_sum_f64(ryo_ctx* ctx, const float* x, const float* y, const float* z, const float* h,
+                           const float* m, size_t n, const int64_t* targets, size_t num_targets, double G,
+                           double box_length, double alpha_scale, double* out);
+
 /* ---------------------------------------------------------------- system pipeline */
 
 /* One resident particle system: the ryoanji::TreeBuilder (interface/treebuilder.cuh:35-127) and
@@ -244,9 +255,10 @@ int ryo_system_phase_ms(ryo_system* sys, double* ms);
 
 /* Accuracy of the last ryo_system_gravity of `sys` against the FP64 direct sum (P2P<Exact>,
  * traversal_cpu.hpp:232-268, the reference's accuracy oracle) on num_targets targets drawn from the system
- * (splitmix64 of seed, with replacement; over the same replica images for a periodic system, whose last gravity
- * must not include the Ewald term). out[4] = {median, 99th percentile, maximum} of |a - a_ref| / |a_ref| and
- * the relative error of the sampled potential sum. Synchronizes the stream. */
+ * (splitmix64 of seed, with replacement). For a periodic system the reference is the direct sum over the same
+ * replica images when the last gravity had no Ewald term, and the FP64 Ewald pair sum
+ * (ryo_ewald_pair_sum_f64, cubic boxes) when it had. out[4] = {median, 99th percentile, maximum} of
+ * |a - a_ref| / |a_ref| and the relative error of the sampled potential sum. Synchronizes the stream. */
 int ryo_system_accuracy(ryo_system* sys, int num_targets, uint64_t seed, double* out);
 
 /* ---------------------------------------------------------------- batches of systems */

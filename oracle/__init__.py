@@ -67,6 +67,8 @@ def lib(which: str = "orc") -> C.CDLL:
                                          C.c_int, _P]),
             "ewald_f64": (C.c_double, [_P, _P, _P, _P, _P, _P, C.c_size_t, C.c_double, C.c_double, _P, _P, _P,
                                        _P, _P]),
+            "ewald_pair_sum_f64": (C.c_int, [_P, _P, _P, _P, _P, C.c_size_t, _P, C.c_size_t, C.c_double, C.c_double,
+                                             C.c_double, _P]),
             "hilbert3d": (C.c_uint64, [C.c_uint, C.c_uint, C.c_uint, C.c_int]),
             "hilbert_mixd": (C.c_uint64, [C.c_uint, C.c_uint, C.c_uint, _P]),
             "decode_hilbert_mixd": (C.c_int, [C.c_uint64, _P, _P]),
@@ -262,6 +264,20 @@ def direct_sum(x, y, z, h, m, targets=None, G=1.0, box_l=None, num_shells=0, pre
     bl = None if box_l is None else _c(box_l, np.float64)
     _f("orc", "direct_sum_" + precision)(*(_ptr(a) for a in arrs), n, _ptr(t), nt, float(G), _ptr(bl),
                                          num_shells, _ptr(out))
+    return out
+
+
+def ewald_pair_sum(x, y, z, h, m, L, targets=None, G=1.0, alpha_scale=6.5):
+    """exact periodic potential/acceleration (FP64 Ewald pair sum, softened pairs) -> [nt, 4] {G m phi, G a}"""
+    arrs = [_c(a, np.float32) for a in (x, y, z, h, m)]
+    n = arrs[0].size
+    t = None if targets is None else _c(targets, np.int64)
+    nt = n if t is None else t.size
+    out = np.zeros((nt, 4), np.float64)
+    rc = _f("orc", "ewald_pair_sum_f64")(*(_ptr(a) for a in arrs), n, _ptr(t), nt, float(G), float(L),
+                                         float(alpha_scale), _ptr(out))
+    if rc:
+        raise ValueError("ewald_pair_sum: alpha_scale must exceed 5")
     return out
 
 

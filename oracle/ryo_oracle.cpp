@@ -20,10 +20,14 @@
  *   P2P / M2P       : reference goldens (kernel.cpp, cartesian_qpole.cpp), oracle/_ref
  *   gravity         : FP64 direct-sum accuracy bounds + oracle/_ref (16-target groups)
  *   ewald           : analytic single-particle energy goldens, disabled==0, oracle/_ref
+ *   Ewald pair sum  : Madelung constant of the simple cubic lattice (one particle, eight on a
+ *                     sub-lattice), alpha independence, traversal + Ewald correction where the
+ *                     root multipole is exact (tests/test_oracle_ewald_pair.py)
  */
 
 #include <algorithm>
 #include <cmath>
+#include <complex>
 #include <cstdint>
 #include <cstring>
 #include <numeric>
@@ -1266,6 +1270,97 @@ int orc_direct_sum_f32(const float* x, const float* y, const float* z, const flo
 {
     double zero[3] = {0, 0, 0};
     directSum<float, float>(x, y, z, h, m, n, targets, numTargets, G, boxL ? boxL : zero, numShells, out);
+    return 0;
+}
+
+/* Exact periodic potential and acceleration of softened particles in a cubic box of side L, in FP64, for the
+ * listed targets (all, if targets == NULL): the Ewald sum over every source and every image, in the potential
+ * convention of ewald.hpp:201-327 (-erfc(aR)/R real space, -(4 pi/L^3) exp(-k^2/4a^2)/k^2 cos(k.r) k-space,
+ * +pi M/(a^2 L^3) background), except that a pair image closer than h_i + h_j contributes the softened law of
+ * P2P<Exact> (kernel.hpp:99-120) in place of the point mass, as in the traversal. Written as the plain sum:
+ * explicit images |n| <= 2 within rc = 5/a, the point-mass real-space term with the point mass swapped for the
+ * softened law by difference, the self term as the R -> 0 limit 2a/sqrt(pi), and the k-space sum over the full
+ * lattice |h| <= ceil(5.5 a L / pi) through complex structure factors. out[4 i] = {G m phi, G a}.
+ * Pinned by the simple-cubic Madelung constant (one particle; eight on a sub-lattice), by independence of
+ * alpha, and by the traversal + Ewald correction in the limit where the root multipole is exact. */
+int orc_ewald_pair_sum_f64(const float* x, const float* y, const float* z, const float* h, const float* m, size_t n,
+                           const int64_t* targets, size_t numTargets, double G, double L, double alphaScale,
+                           double* out)
+{
+    if (!(alphaScale > 5.0)) return -1;
+    const double a = alphaScale / L, rc = 5.0 / a, pi = 3.14159265358979323846;
+    const int hmax = int(std::ceil(5.5 * alphaScale / pi));
+    struct Kv
+    {
+        double k[3], c;
+        std::complex<double> S;
+    };
+    std::vector<Kv> kv;
+    double M = 0;
+    for (size_t j = 0; j < n; ++j)
+        M += double(m[j]);
+    for (int hx = -hmax; hx <= hmax; ++hx)
+        for (int hy = -hmax; hy <= hmax; ++hy)
+            for (int hz = -hmax; hz <= hmax; ++hz)
+            {
+                int h2 = hx * hx + hy * hy + hz * hz;
+                if (h2 == 0 || h2 > hmax * hmax) continue;
+                Kv e;
+                e.k[0] = 2 * pi * hx / L, e.k[1] = 2 * pi * hy / L, e.k[2] = 2 * pi * hz / L;
+                double k2 = e.k[0] * e.k[0] + e.k[1] * e.k[1] + e.k[2] * e.k[2];
+                e.c = 4 * pi / (L * L * L) * std::exp(-k2 / (4 * a * a)) / k2;
+                e.S = 0;
+                for (size_t j = 0; j < n; ++j)
+                    e.S += double(m[j]) * std::exp(std::complex<double>(0, e.k[0] * x[j] + e.k[1] * y[j] + e.k[2] * z[j]));
+                kv.push_back(e);
+            }
+    const size_t nt = targets ? numTargets : n;
+#pragma omp parallel for
+    for (size_t q = 0; q < nt; ++q)
+    {
+        const size_t t = targets ? size_t(targets[q]) : q;
+        const double p[3] = {x[t], y[t], z[t]};
+        double phi = pi * M / (a * a * L * L * L), acc[3] = {0, 0, 0};
+        for (size_t j = 0; j < n; ++j)
+        {
+            const double hij = double(h[t]) + double(h[j]), mj = m[j];
+            for (int ix = -2; ix <= 2; ++ix)
+                for (int iy = -2; iy <= 2; ++iy)
+                    for (int iz = -2; iz <= 2; ++iz)
+                    {
+                        double r[3] = {x[j] + ix * L - p[0], y[j] + iy * L - p[1], z[j] + iz * L - p[2]};
+                        double R2   = r[0] * r[0] + r[1] * r[1] + r[2] * r[2];
+                        if (R2 >= rc * rc) continue;
+                        if (R2 == 0)
+                        {
+                            phi += mj * 2 * a / std::sqrt(pi); // the self term (and exact duplicates)
+                            continue;
+                        }
+                        double R = std::sqrt(R2);
+                        double fpot = -std::erfc(a * R) / R;
+                        double ffac = std::erfc(a * R) / (R2 * R) + 2 * a / std::sqrt(pi) * std::exp(-a * a * R2) / R2;
+                        if (R < hij) // the softened law in place of the point mass: + (-R^2/h^3) - (-1/R)
+                        {
+                            fpot += -R2 / (hij * hij * hij) + 1.0 / R;
+                            ffac += 1.0 / (hij * hij * hij) - 1.0 / (R2 * R);
+                        }
+                        phi += mj * fpot;
+                        for (int d = 0; d < 3; ++d)
+                            acc[d] += mj * ffac * r[d];
+                    }
+        }
+        for (const Kv& e : kv)
+        {
+            std::complex<double> v = e.S * std::exp(std::complex<double>(0, -(e.k[0] * p[0] + e.k[1] * p[1] + e.k[2] * p[2])));
+            phi -= e.c * v.real();
+            for (int d = 0; d < 3; ++d)
+                acc[d] += e.c * e.k[d] * v.imag();
+        }
+        out[4 * q]     = G * double(m[t]) * phi;
+        out[4 * q + 1] = G * acc[0];
+        out[4 * q + 2] = G * acc[1];
+        out[4 * q + 3] = G * acc[2];
+    }
     return 0;
 }
 

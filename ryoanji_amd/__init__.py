@@ -105,6 +105,7 @@ SIGNATURES = {
     "ryo_traversal_retries": (C.c_uint64, [_P]),
     "ryo_ewald_f32": (_I, [_P, _P, _P, C.c_uint32, C.c_uint32, _P, _P, _P, _P, _P, _F, _P, _P, _P, _P, _P, _P]),
     "ryo_direct_sum_f64": (_I, [_P, _P, _P, _P, _P, _P, _SZ, _P, _SZ, _D, _P, _I, _P]),
+    "ryo_ewald_pair_sum_f64": (_I, [_P, _P, _P, _P, _P, _P, _SZ, _P, _SZ, _D, _D, _D, _P]),
     "ryo_default_params": (None, [C.POINTER(Params)]),
     "ryo_system_create": (_I, [_P, _SZ, C.POINTER(_P)]),
     "ryo_system_destroy": (_I, [_P]),

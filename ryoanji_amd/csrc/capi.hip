@@ -30,6 +30,8 @@ int ewald(Context*, const double*, const double*, uint32_t, uint32_t, const floa
           const float*, const float*, float, const double*, float*, float*, float*, float*, double*);
 int directSum(Context*, const float*, const float*, const float*, const float*, const float*, size_t,
               const int64_t*, size_t, double, const double*, int, double*);
+int ewaldPairSum(Context* ctx, const float* x, const float* y, const float* z, const float* h, const float* m,
+                 size_t n, const int64_t* targets, size_t nt, double G, double L, double alphaScale, double* out);
 int kickDrift(Context*, float*, float*, float*, float*, float*, float*, const float*, const float*, const float*,
               size_t, float, float, int, const float*);
 int fitBox(Context*, const float*, const float*, const float*, size_t, float*);
@@ -294,6 +296,15 @@ int ryo_direct_sum_f64(ryo_ctx* h, const float* x, const float* y, const float* 
 {
     CTX_OR_FAIL(h);
     return directSum(ctx, x, y, z, hh, m, n, targets, nt, G, box_l, num_shells, out);
+}
+
+int ryo_ewald_pair_sum_f64(ryo_ctx* h, const float* x, const float* y, const float* z, const float* hh,
+                           const float* m, size_t n, const int64_t* targets, size_t nt, double G, double box_length,
+                           double alpha_scale, double* out)
+{
+    CTX_OR_FAIL(h);
+    if (nt && (!x || !y || !z || !hh || !m || !out)) return setError(ctx, RYO_ERR_ARG, "Ewald pair sum: null pointer");
+    return ewaldPairSum(ctx, x, y, z, hh, m, n, targets, nt, G, box_length, alpha_scale, out);
 }
 
 } // extern "C"
@@ -622,8 +633,6 @@ int ryo_system_accuracy(ryo_system* s, int num_targets, uint64_t seed, double* o
     if (!s || !out || num_targets < 1) return RYO_ERR_ARG;
     Context* ctx = s->ctx;
     if (!s->haveAcc) return setError(ctx, RYO_ERR_ARG, "accuracy: no gravity results to check");
-    if (s->lastEwald)
-        return setError(ctx, RYO_ERR_UNSUPPORTED, "accuracy: the FP64 reference has no Ewald term (solve without it)");
     const size_t n  = s->n;
     const size_t nt = std::min(size_t(num_targets), n);
     // targets: splitmix64 draws of the seed, with replacement
@@ -649,7 +658,16 @@ int ryo_system_accuracy(ryo_system* s, int num_targets, uint64_t seed, double* o
     float** A = s->a[s->cur];
     double boxL[3] = {double(s->limits[1] - s->limits[0]), double(s->limits[3] - s->limits[2]),
                       double(s->limits[5] - s->limits[4])};
-    int rc = directSum(ctx, A[0], A[1], A[2], A[4], A[3], n, dtg, nt, double(s->lastG), s->lastShells ? boxL : nullptr,
+    int rc;
+    if (s->lastEwald)
+    {
+        // the full periodic result (images + Ewald correction) against the FP64 Ewald pair sum
+        if (!(boxL[0] == boxL[1] && boxL[1] == boxL[2]))
+            return setError(ctx, RYO_ERR_UNSUPPORTED, "accuracy: the Ewald reference needs a cubic box");
+        rc = ewaldPairSum(ctx, A[0], A[1], A[2], A[4], A[3], n, dtg, nt, double(s->lastG), boxL[0], 0.0, ref);
+    }
+    else
+        rc = directSum(ctx, A[0], A[1], A[2], A[4], A[3], n, dtg, nt, double(s->lastG), s->lastShells ? boxL : nullptr,
                        s->lastShells, ref);
     if (rc) return rc;
     const float* src[4] = {s->p, s->ax, s->ay, s->az};

@@ -341,4 +341,246 @@ int directSum(Context* ctx, const float* x, const float* y, const float* z, cons
     return RYO_OK;
 }
 
+/* ============================================================================ Ewald pair sum */
+
+// The exact periodic potential of the softened particles, in FP64, on sampled targets: the accuracy reference of
+// the periodic pipeline (images + Ewald correction). Every source j contributes m_j psi(r_ij) with the Ewald
+// point-mass potential
+//   psi(r) = -[ sum_n erfc(a|r+nL|)/|r+nL| + 4 pi / L^3 sum_{k!=0} exp(-k^2 / 4a^2) / k^2 cos(k.r) - pi/(a^2 L^3) ]
+// (the convention of ewald.hpp:201-327: neutralizing background, 0 at k = 0), except that a pair image closer
+// than h_i + h_j interacts by the softened law of P2P<Exact> (kernel.hpp:99-120: -m R^2/h^3, force m r/h^3),
+// as it does in the traversal; the self pair (R = 0) is such a pair. The real-space part runs over the images
+// within rc = 5/a (erfc(5) = 1.5e-12), the k-space part over |h| <= ceil(5.5 a L / pi) (weights below 1e-13)
+// through structure factors S(k) = sum_j m_j exp(i k.x_j).
+
+struct EwaldK
+{
+    double kx, ky, kz, w; // k = 2 pi h / L; w = 2 * 4 pi / L^3 * exp(-k^2 / 4a^2) / k^2 (the +-k pair)
+};
+
+constexpr double kTwoOverSqrtPi = 1.1283791670955126;
+
+// part[chunk * K + k] = sum over the chunk's sources of m (cos k.x, sin k.x)
+__global__ __launch_bounds__(256) void structureKernel(const float* __restrict__ x, const float* __restrict__ y,
+                                                       const float* __restrict__ z, const float* __restrict__ m,
+                                                       size_t n, const EwaldK* __restrict__ kv, int K,
+                                                       size_t chunkLen, double2* __restrict__ part)
+{
+    __shared__ double sx[256], sy[256], sz[256], sm[256];
+    const int k      = blockIdx.x * 256 + threadIdx.x;
+    const size_t s0  = size_t(blockIdx.y) * chunkLen;
+    const size_t s1  = std::min(n, s0 + chunkLen);
+    double kx = 0, ky = 0, kz = 0;
+    if (k < K) kx = kv[k].kx, ky = kv[k].ky, kz = kv[k].kz;
+    double c = 0, sn = 0;
+    for (size_t b = s0; b < s1; b += 256)
+    {
+        const size_t j = b + threadIdx.x;
+        __syncthreads();
+        if (j < s1) sx[threadIdx.x] = x[j], sy[threadIdx.x] = y[j], sz[threadIdx.x] = z[j], sm[threadIdx.x] = m[j];
+        __syncthreads();
+        const int cnt = int(std::min<size_t>(256, s1 - b));
+        for (int q = 0; q < cnt; ++q)
+        {
+            double sv, cv;
+            sincos(kx * sx[q] + ky * sy[q] + kz * sz[q], &sv, &cv);
+            c += sm[q] * cv;
+            sn += sm[q] * sv;
+        }
+    }
+    if (k < K) part[size_t(blockIdx.y) * K + k] = make_double2(c, sn);
+}
+
+__global__ void structureReduceKernel(const double2* __restrict__ part, int chunks, int K, double2* __restrict__ S)
+{
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= K) return;
+    double c = 0, s = 0;
+    for (int q = 0; q < chunks; ++q)
+        c += part[size_t(q) * K + k].x, s += part[size_t(q) * K + k].y;
+    S[k] = make_double2(c, s);
+}
+
+// real-space partial sums of one source chunk for a block of targets: part[chunk][target][4] = {phi, a}
+__global__ __launch_bounds__(256) void ewaldPairKernel(const float* __restrict__ x, const float* __restrict__ y,
+                                                       const float* __restrict__ z, const float* __restrict__ h,
+                                                       const float* __restrict__ m, size_t n,
+                                                       const int64_t* __restrict__ targets, size_t nt, double L,
+                                                       double alpha, double rc, size_t chunkLen,
+                                                       double* __restrict__ part)
+{
+    __shared__ float sx[256], sy[256], sz[256], sm[256], shh[256];
+    const size_t ti = size_t(blockIdx.x) * 256 + threadIdx.x;
+    const size_t s0 = size_t(blockIdx.y) * chunkLen;
+    const size_t s1 = std::min(n, s0 + chunkLen);
+    double px = 0, py = 0, pz = 0, th = 0;
+    if (ti < nt)
+    {
+        const size_t t = targets ? size_t(targets[ti]) : ti;
+        px = x[t], py = y[t], pz = z[t], th = h[t];
+    }
+    const double rc2 = rc * rc, invL = 1.0 / L, a2 = alpha * alpha, a3 = a2 * alpha;
+    double phi = 0, ax = 0, ay = 0, az = 0;
+    for (size_t b = s0; b < s1; b += 256)
+    {
+        const size_t j = b + threadIdx.x;
+        __syncthreads();
+        if (j < s1) sx[threadIdx.x] = x[j], sy[threadIdx.x] = y[j], sz[threadIdx.x] = z[j], sm[threadIdx.x] = m[j],
+                    shh[threadIdx.x] = h[j];
+        __syncthreads();
+        const int cnt = int(std::min<size_t>(256, s1 - b));
+        for (int q = 0; q < cnt; ++q)
+        {
+            // minimum image, then the images one box away along the axes where that can come within rc < L
+            double d0[3] = {double(sx[q]) - px, double(sy[q]) - py, double(sz[q]) - pz};
+            double sh[3];
+            for (int c = 0; c < 3; ++c)
+            {
+                d0[c] -= L * rint(d0[c] * invL);
+                sh[c] = d0[c] > 0 ? -L : L;
+            }
+            const double hij = th + double(shh[q]), hij2 = hij * hij, mj = double(sm[q]);
+            for (int im = 0; im < 8; ++im)
+            {
+                const double dx = d0[0] + ((im & 1) ? sh[0] : 0.0), dy = d0[1] + ((im & 2) ? sh[1] : 0.0),
+                             dz = d0[2] + ((im & 4) ? sh[2] : 0.0);
+                const double R2 = dx * dx + dy * dy + dz * dz;
+                if (R2 >= rc2) continue;
+                double fac;
+                if (R2 < hij2)
+                {
+                    // softened pair: -m R^2/h^3 in place of the point mass's -m/R, plus the smooth part erf(aR)/R
+                    // of the Ewald real-space term; g = erf(u)/u, f = ((2/sqrt pi) u exp(-u^2) - erf(u)) / u^3
+                    const double R = sqrt(R2), u = alpha * R, u2 = u * u;
+                    double g, f;
+                    if (u < 0.02)
+                    {
+                        g = kTwoOverSqrtPi * (1.0 + u2 * (-1.0 / 3.0 + u2 * (0.1 - u2 / 42.0)));
+                        f = kTwoOverSqrtPi * (-2.0 / 3.0 + u2 * (0.4 + u2 * (-1.0 / 7.0 + u2 / 27.0)));
+                    }
+                    else
+                    {
+                        const double er = erf(u);
+                        g = er / u;
+                        f = (kTwoOverSqrtPi * u * exp(-u2) - er) / (u2 * u);
+                    }
+                    const double h3 = hij2 * hij;
+                    phi += mj * (alpha * g - R2 / h3);
+                    fac = mj * (1.0 / h3 + a3 * f);
+                }
+                else
+                {
+                    const double R = sqrt(R2), ec = erfc(alpha * R);
+                    phi -= mj * ec / R;
+                    fac = mj * (ec / (R2 * R) + kTwoOverSqrtPi * alpha * exp(-a2 * R2) / R2);
+                }
+                ax += fac * dx;
+                ay += fac * dy;
+                az += fac * dz;
+            }
+        }
+    }
+    if (ti < nt)
+    {
+        double* o = part + (size_t(blockIdx.y) * nt + ti) * 4;
+        o[0] = phi, o[1] = ax, o[2] = ay, o[3] = az;
+    }
+}
+
+// chunk partials in a fixed order, the k-space sum from the structure factors and the background term; out[4 t] =
+// {G m_t phi, G a}
+__global__ void ewaldPairReduceKernel(const double* __restrict__ part, int chunks, size_t nt,
+                                      const int64_t* __restrict__ targets, const float* __restrict__ x,
+                                      const float* __restrict__ y, const float* __restrict__ z,
+                                      const float* __restrict__ m, const EwaldK* __restrict__ kv,
+                                      const double2* __restrict__ S, int K, double background, double G,
+                                      double* __restrict__ out)
+{
+    const size_t ti = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (ti >= nt) return;
+    double a[4] = {0, 0, 0, 0};
+    for (int c = 0; c < chunks; ++c)
+        for (int q = 0; q < 4; ++q)
+            a[q] += part[(size_t(c) * nt + ti) * 4 + q];
+    const size_t t = targets ? size_t(targets[ti]) : ti;
+    const double px = x[t], py = y[t], pz = z[t];
+    for (int k = 0; k < K; ++k)
+    {
+        double sv, cv;
+        sincos(kv[k].kx * px + kv[k].ky * py + kv[k].kz * pz, &sv, &cv);
+        const double w = kv[k].w;
+        a[0] -= w * (S[k].x * cv + S[k].y * sv);
+        const double f = w * (S[k].y * cv - S[k].x * sv);
+        a[1] += f * kv[k].kx, a[2] += f * kv[k].ky, a[3] += f * kv[k].kz;
+    }
+    a[0] += background;
+    out[4 * ti + 0] = G * double(m[t]) * a[0];
+    for (int q = 1; q < 4; ++q)
+        out[4 * ti + q] = G * a[q];
+}
+
+int ewaldPairSum(Context* ctx, const float* x, const float* y, const float* z, const float* h, const float* m,
+                 size_t n, const int64_t* targets, size_t nt, double G, double L, double alphaScale, double* out)
+{
+    if (nt == 0 || n == 0) return RYO_OK;
+    if (!(L > 0)) return setError(ctx, RYO_ERR_ARG, "Ewald pair sum: box length must be positive");
+    if (alphaScale == 0) alphaScale = 6.5;
+    if (!(alphaScale > 5.0 && alphaScale <= 16.0))
+        return setError(ctx, RYO_ERR_ARG, "Ewald pair sum: alpha L in (5, 16]");
+    const double alpha = alphaScale / L, rc = 5.0 / alpha;
+    const int hmax     = int(std::ceil(5.5 * alphaScale / M_PI));
+    std::vector<EwaldK> kv;
+    const double c = 8.0 * M_PI / (L * L * L);
+    for (int hx = 0; hx <= hmax; ++hx)
+        for (int hy = -hmax; hy <= hmax; ++hy)
+            for (int hz = -hmax; hz <= hmax; ++hz)
+            {
+                const bool half = hx > 0 || (hx == 0 && (hy > 0 || (hy == 0 && hz > 0)));
+                const int h2    = hx * hx + hy * hy + hz * hz;
+                if (!half || h2 > hmax * hmax) continue;
+                EwaldK e;
+                e.kx = 2 * M_PI * hx / L, e.ky = 2 * M_PI * hy / L, e.kz = 2 * M_PI * hz / L;
+                const double k2 = e.kx * e.kx + e.ky * e.ky + e.kz * e.kz;
+                e.w = c * std::exp(-k2 / (4 * alpha * alpha)) / k2;
+                kv.push_back(e);
+            }
+    // k = 0 last: its structure factor's real part is the total mass (the background term)
+    kv.push_back(EwaldK{0, 0, 0, 0});
+    const int K = int(kv.size());
+    const unsigned tb = iceil(nt, 256);
+    const int pairChunks = int(std::max<size_t>(1, std::min<size_t>((size_t(ctx->numCUs) * 8 + tb - 1) / tb,
+                                                                   (n + 4095) / 4096)));
+    const size_t pairLen = (n + pairChunks - 1) / pairChunks;
+    const unsigned kb    = iceil(K, 256);
+    const int sChunks    = int(std::max<size_t>(1, std::min<size_t>((size_t(ctx->numCUs) * 8 + kb - 1) / kb,
+                                                                   (n + 4095) / 4096)));
+    const size_t sLen    = (n + sChunks - 1) / sChunks;
+    const size_t kvBytes = (K * sizeof(EwaldK) + 255) & ~size_t(255);
+    const size_t sBytes  = (K * sizeof(double2) + 255) & ~size_t(255);
+    const size_t spBytes = (size_t(sChunks) * K * sizeof(double2) + 255) & ~size_t(255);
+    const size_t ppBytes = size_t(pairChunks) * nt * 4 * sizeof(double);
+    auto* buf = static_cast<char*>(ctx->getScratch(kSlotMisc, kvBytes + sBytes + spBytes + ppBytes));
+    if (!buf) return RYO_ERR_HIP;
+    auto* dkv   = reinterpret_cast<EwaldK*>(buf);
+    auto* dS    = reinterpret_cast<double2*>(buf + kvBytes);
+    auto* spart = reinterpret_cast<double2*>(buf + kvBytes + sBytes);
+    auto* ppart = reinterpret_cast<double*>(buf + kvBytes + sBytes + spBytes);
+    hipStream_t s = ctx->stream;
+    RYO_HIP(ctx, hipMemcpyAsync(dkv, kv.data(), K * sizeof(EwaldK), hipMemcpyHostToDevice, s));
+    RYO_HIP(ctx, hipStreamSynchronize(s)); // kv is host memory that ends with this call
+    structureKernel<<<dim3(kb, sChunks), 256, 0, s>>>(x, y, z, m, n, dkv, K, sLen, spart);
+    structureReduceKernel<<<kb, 256, 0, s>>>(spart, sChunks, K, dS);
+    ewaldPairKernel<<<dim3(tb, pairChunks), 256, 0, s>>>(x, y, z, h, m, n, targets, nt, L, alpha, rc, pairLen, ppart);
+    RYO_LAUNCH_CHECK(ctx);
+    // the total mass for the background term: the k = 0 structure factor
+    double2 S0;
+    RYO_HIP(ctx, hipMemcpyAsync(&S0, dS + (K - 1), sizeof(double2), hipMemcpyDeviceToHost, s));
+    RYO_HIP(ctx, hipStreamSynchronize(s));
+    const double background = M_PI * S0.x / (alpha * alpha * L * L * L);
+    ewaldPairReduceKernel<<<iceil(nt, 256), 256, 0, s>>>(ppart, pairChunks, nt, targets, x, y, z, m, dkv, dS, K - 1,
+                                                          background, G, out);
+    RYO_LAUNCH_CHECK(ctx);
+    return RYO_OK;
+}
+
 } // namespace ryo
