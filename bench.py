@@ -113,6 +113,20 @@ def cpu_baseline(s, sample_frac=1 / 128, seed=0):
     }
 
 
+def build_roofline(phase, n, nodes):
+    """the key -> sort -> build -> upsweep phase against the HBM peak: algorithmic bytes B_K = 112 N + 320 n (keys
+    20, sort pairs 24, permute of x, y, z, m, h 44, leaf counting 8, centers + leaf P2M 16 per particle; about
+    320 per tree node) over the phase time from keys through the upsweep (HIP events per phase)"""
+    import numpy as np
+
+    keys = ("keys", "sort", "permute", "leaves", "link", "upsweep")
+    t = sum(float(np.mean([ph[k] for ph in phase])) for k in keys) / 1e3
+    b = 112.0 * n + 320.0 * nodes
+    gbs = b / t / 1e9
+    return {"bound": "hbm", "achieved": gbs, "peak": PEAK_HBM_GBS, "unit": "GB/s", "frac": gbs / PEAK_HBM_GBS,
+            "algorithmic_bytes": b, "phase_ms": t * 1e3, "phases": list(keys)}
+
+
 def accuracy(ctx, sysm, s, order, ax, ay, az, pot, num_targets=16384, seed=1, box_l=None, shells=0, ewald_l=None,
              p=None):
     """1st-percentile and max relative acceleration error vs an FP64 direct sum on sampled targets (with the same
@@ -365,6 +379,7 @@ def main():
             "flops_per_launch": flops_ns,
             "mean_launch_ms": t_trav * 1e3,
         },
+        "build_roofline": build_roofline(phase, s.n, info["numNodes"]),
         "tflops_code_count": flops_code / t_trav / 1e12,
         "particles_per_s_traversal": s.n / t_trav,
         "interactions_per_particle": {"p2p": stats["sumP2P"] / s.n, "m2p": stats["sumM2P"] / s.n,

@@ -12,7 +12,8 @@ import pytest
 import oracle as O
 from ryoanji_amd import workloads as W
 
-from gpu_util import (gpu_keys, gpu_leaves, gpu_link, gpu_sort, gpu_traverse, gpu_upsweep, oracle_tree, rel_err)
+from gpu_util import (dev, gpu_keys, gpu_leaves, gpu_link, gpu_sort, gpu_traverse, gpu_upsweep, oracle_tree,
+                      rel_err)
 
 pytestmark = pytest.mark.gpu
 
@@ -206,3 +207,45 @@ def test_single_leaf_and_tiny(ctx):
         _, _, gax, gay, gaz, _ = gpu_traverse(ctx, t["x"], t["y"], t["z"], t["m"], t["h"], t["tree"], t["layout"],
                                               t["centers"], t["mp"], 16, init_node=1)
         assert np.allclose(gax, oax, rtol=1e-5, atol=1e-6 * np.abs(oax).max() + 1e-30)
+
+
+def _faces(limits):
+    """particles on the lower and upper faces, one ulp inside them and at the middle, along every axis (125)"""
+    lim = np.asarray(limits, np.float32)
+    vals = []
+    for d in range(3):
+        lo, hi = lim[2 * d], lim[2 * d + 1]
+        vals.append(np.array([lo, np.nextafter(lo, hi), np.float32(0.5) * (lo + hi), np.nextafter(hi, lo), hi],
+                             np.float32))
+    g = np.array(np.meshgrid(*vals, indexing="ij")).reshape(3, -1)
+    n = g.shape[1]
+    return W.System(g[0], g[1], g[2], np.full(n, 1.0 / n, np.float32), np.full(n, 1e-3, np.float32), lim)
+
+
+@pytest.mark.parametrize("limits", [[-1, 1, -1, 1, -1, 1], [0, 2, -0.5, 0.5, 0, 0.25], [-30, 30, -30, 30, -30, 30]])
+def test_keys_on_box_faces_bit_exact(ctx, limits):
+    """cell indices at the box faces (the upper face clamps into the last cell): keys equal the oracle's"""
+    s = _faces(limits)
+    for kind, hil in ((0, True), (1, False)):
+        assert np.array_equal(gpu_keys(ctx, s, kind=kind), O.sfc_keys(s.x, s.y, s.z, s.limits, hilbert=hil)), kind
+
+
+def test_empty_inputs(ctx):
+    """no particles: keys and sort do nothing, the leaves are the root alone with no particles (the reference's
+    computeOctree of an empty key range), a system refuses to build"""
+    import ryoanji_amd as R
+
+    e = np.zeros(0, np.float32)
+    s = W.System(e, e, e, e, e, [-1, 1, -1, 1, -1, 1])
+    assert gpu_keys(ctx, s).size == 0
+    k, v = gpu_sort(ctx, np.zeros(0, np.uint64))
+    assert k.size == 0 and v.size == 0
+    leaves, counts, layout = gpu_leaves(ctx, np.zeros(0, np.uint64), 64)
+    ol, oc = O.compute_octree(np.zeros(0, np.uint64), 64)
+    assert np.array_equal(leaves, ol) and np.array_equal(counts, oc) and layout.tolist() == [0, 0]
+    assert leaves.tolist() == [0, 1 << 63]
+    sysm = R.System(ctx, 16)
+    sysm.load(*(dev(a) for a in s.arrays()), s.limits)
+    with pytest.raises(R.RyoError):
+        sysm.build(R.default_params())
+    sysm.close()

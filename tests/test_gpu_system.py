@@ -222,3 +222,35 @@ def test_periodic_accuracy_against_the_ewald_pair_sum(ctx):
     rms = np.sqrt(np.mean(np.sum(ref[:, 1:] ** 2, axis=1)))
     d = np.sqrt((ax[tgt] - ref[:, 1]) ** 2 + (ay[tgt] - ref[:, 2]) ** 2 + (az[tgt] - ref[:, 3]) ** 2) / rms
     assert np.quantile(d, 0.99) < 1.5e-2, np.quantile(d, 0.99)
+
+
+@pytest.mark.parametrize("name,mk", [("plummer_2^24", lambda: W.plummer(1 << 24, seed=42)),
+                                     ("uniform_2^26", lambda: W.uniform_cube(1 << 26, seed=42))])
+def test_full_size_invariants(ctx, name, mk):
+    """the benchmark sizes: the sorted keys are the oracle's keys of the input in stable sorted order (sorted, a
+    bijection, equal keys in input order), the leaves and counts are the oracle's, the layout ends at N, and the
+    accelerations meet the accuracy bound on sampled targets"""
+    s = mk()
+    n = s.n
+    sysm = R.System(ctx, n)
+    sysm.load(*(dev(a) for a in s.arrays()), s.limits)
+    prm = R.default_params()
+    sysm.build(prm)
+    sysm.gravity(prm)
+    order = host(sysm.results()[4])
+    t = sysm.tree()
+    keys = host(t["keys"], np.uint64)
+    leaves, counts, layout = host(t["leaves"], np.uint64), host(t["counts"], np.uint32), host(t["layout"])
+    del t
+    assert np.bincount(order, minlength=n).max() == 1  # a permutation of 0..n-1
+    ok = O.sfc_keys(s.x, s.y, s.z, s.limits)
+    assert np.array_equal(keys, ok[order])
+    assert np.all(keys[1:] >= keys[:-1])
+    assert np.all((keys[1:] != keys[:-1]) | (order[1:] > order[:-1]))  # stable
+    del ok
+    lo, co = O.compute_octree(keys, 64)
+    assert np.array_equal(leaves, lo) and np.array_equal(counts, co)
+    assert layout[0] == 0 and layout[-1] == n and np.array_equal(np.diff(layout.astype(np.int64)), co)
+    rep = sysm.accuracy(num_targets=2048, seed=7)
+    assert rep["err_p99"] < 1.5e-3, rep
+    sysm.close()

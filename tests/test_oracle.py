@@ -298,3 +298,18 @@ def test_oracle_equals_reference_harness(mk):
     gr = O.compute_gravity(lo, lay, c, q, x, y, z, h, m, s.limits, lib="ref")
     for i in (1, 2, 3, 4):
         assert np.array_equal(go[i], gr[i])
+
+
+@pytest.mark.skipif(not O.have_ref(), reason="reference harness not built (no reference checkout)")
+@pytest.mark.parametrize("limits", [[-1, 1, -1, 1, -1, 1], [0, 2, -0.5, 0.5, 0, 0.25], [-30, 30, -30, 30, -30, 30]])
+def test_keys_on_box_faces_equal_reference(limits):
+    """particles on the faces of the box, one ulp inside and in the middle: the upper face clamps into the last
+    cell as in the reference's sfcCellIndex"""
+    lim = np.asarray(limits, np.float32)
+    vals = [np.array([lim[2 * d], np.nextafter(lim[2 * d], lim[2 * d + 1]), np.float32(0.5) * (lim[2 * d] + lim[2 * d + 1]),
+                      np.nextafter(lim[2 * d + 1], lim[2 * d]), lim[2 * d + 1]], np.float32) for d in range(3)]
+    x, y, z = np.array(np.meshgrid(*vals, indexing="ij")).reshape(3, -1)
+    for hil in (True, False):
+        k = O.sfc_keys(x, y, z, lim, hilbert=hil)
+        assert np.array_equal(k, O.sfc_keys(x, y, z, lim, hilbert=hil, lib="ref"))
+        assert k.max() < (1 << 63)
