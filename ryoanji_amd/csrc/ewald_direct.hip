@@ -15,12 +15,20 @@ namespace ryo
 
 constexpr int kMaxH = 343;
 
+// the root quadrupole in the form its evaluation uses: M, the traceless components plus trace over 3, and half
+// the trace (ewald.hpp:97-108)
+struct EwaldQ
+{
+    double M, qxx, qyy, qzz, qxy, qxz, qyz, Qtr;
+};
+
 struct EwaldTable
 {
     double Mroot[8];
+    EwaldQ q; // derived from Mroot once, on the host
     double center[3];
-    int numReplicaShells, numEwaldShells, numH;
-    double L, lCut, alphaScale, smallR;
+    int numReplicaShells, numEwaldShells, numH, hReps;
+    double L, lCut, alphaScale, smallR, hCut2;
     double h[kMaxH][5]; // {2 pi h / L (3), hfac_cos, hfac_sin}
 };
 
@@ -40,9 +48,35 @@ __host__ __device__ inline void ewaldEval(double* pa, const double* r, const dou
         pa[1 + d] += g[2] * Qr[d] - r[d] * f;
 }
 
+// exp(i j theta) for j in [-3, 3] from the powers p1 = exp(i theta), p2, p3 (j is wave-uniform)
+__device__ inline double2 phasePow(int j, double2 p1, double2 p2, double2 p3)
+{
+    const int a = j < 0 ? -j : j;
+    double2 v   = a == 0 ? make_double2(1.0, 0.0) : (a == 1 ? p1 : (a == 2 ? p2 : p3));
+    if (j < 0) v.y = -v.y;
+    return v;
+}
+
+__device__ inline double2 cmul(double2 a, double2 b) { return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x); }
+
+// the same evaluation with the derived quadrupole (wave-uniform, so its values stay in scalar registers)
+__device__ inline void ewaldEvalQ(double* pa, const double* r, const double* g, const EwaldQ& q)
+{
+    double Qr[3] = {r[0] * q.qxx + r[1] * q.qxy + r[2] * q.qxz, r[0] * q.qxy + r[1] * q.qyy + r[2] * q.qyz,
+                    r[0] * q.qxz + r[1] * q.qyz + r[2] * q.qzz};
+    double rQr = 0.5 * (r[0] * Qr[0] + r[1] * Qr[1] + r[2] * Qr[2]);
+    pa[0] += -g[0] * q.M + g[1] * q.Qtr - g[2] * rQr;
+    double f = g[1] * q.M - g[2] * q.Qtr + g[3] * rQr;
+    for (int d = 0; d < 3; ++d)
+        pa[1 + d] += g[2] * Qr[d] - r[d] * f;
+}
+
 void ewaldInitTable(EwaldTable& t, const double* Mroot, const double* center, int numReplicaShells, double L,
                     double lCut, double hCut, double alphaScale, double smallR)
 {
+    t.q = EwaldQ{Mroot[0],       (Mroot[1] + Mroot[7]) / 3.0, (Mroot[4] + Mroot[7]) / 3.0,
+                 (Mroot[6] + Mroot[7]) / 3.0, Mroot[2] / 3.0,  Mroot[3] / 3.0,
+                 Mroot[5] / 3.0, 0.5 * Mroot[7]};
     if (lCut == 0 && hCut == 0 && alphaScale == 0) numReplicaShells = 0;
     for (int k = 0; k < 8; ++k)
         t.Mroot[k] = Mroot[k];
@@ -54,6 +88,8 @@ void ewaldInitTable(EwaldTable& t, const double* Mroot, const double* center, in
     t.numH = 0;
     if (t.numEwaldShells == 0) return;
     const int hReps    = int(std::ceil(hCut));
+    t.hReps            = hReps;
+    t.hCut2            = hCut * hCut;
     const double alpha = alphaScale / L;
     const double k4    = M_PI * M_PI / (alpha * alpha * L * L);
     for (int hx = -hReps; hx <= hReps; hx++)
@@ -81,25 +117,23 @@ void ewaldInitTable(EwaldTable& t, const double* Mroot, const double* center, in
             }
 }
 
-__global__ __launch_bounds__(256) void ewaldKernel(const EwaldTable* __restrict__ tab, uint32_t first, uint32_t last,
+__global__ __launch_bounds__(256, 3) void ewaldKernel(const EwaldTable* __restrict__ tab, uint32_t first, uint32_t last,
                                                   const float* __restrict__ x, const float* __restrict__ y,
                                                   const float* __restrict__ z, const float* __restrict__ m, float G,
                                                   float* ugrav, float* ax, float* ay, float* az,
                                                   double* __restrict__ blockPot)
 {
-    __shared__ double sh[kMaxH][5];
-    __shared__ double sMp[8];
+    // the table is read with wave-uniform addresses (scalar loads), so the quadrupole and the k-space
+    // coefficients live in scalar registers
     __shared__ double sPot[256];
-    const int numH = tab->numH;
-    for (int i = threadIdx.x; i < numH * 5; i += blockDim.x)
-        (&sh[0][0])[i] = (&tab->h[0][0])[i];
-    if (threadIdx.x < 8) sMp[threadIdx.x] = tab->Mroot[threadIdx.x];
-    __syncthreads();
+    const int numH  = tab->numH;
+    const EwaldQ q  = tab->q;
 
     const double L = tab->L, lCut = tab->lCut;
     const double lCut2  = lCut * lCut * L * L;
     const double alpha  = tab->alphaScale / L, alpha2 = alpha * alpha;
     const double k1     = M_PI / (alpha2 * L * L * L);
+    const double M      = tab->Mroot[0];
     const double ka     = 2.0 * alpha / sqrt(M_PI);
     const double smallR2 = tab->smallR * L * L;
     const int ns = tab->numEwaldShells, nr = tab->numReplicaShells;
@@ -110,7 +144,7 @@ __global__ __launch_bounds__(256) void ewaldKernel(const EwaldTable* __restrict_
     {
         double px = x[i], py = y[i], pz = z[i];
         double r[3] = {px - tab->center[0], py - tab->center[1], pz - tab->center[2]};
-        double pa[4] = {k1 * sMp[0], 0, 0, 0};
+        double pa[4] = {k1 * M, 0, 0, 0};
         for (int ix = -ns; ix <= ns; ix++)
             for (int iy = -ns; iy <= ns; iy++)
                 for (int iz = -ns; iz <= ns; iz++)
@@ -152,20 +186,48 @@ __global__ __launch_bounds__(256) void ewaldKernel(const EwaldTable* __restrict_
                         an *= 2 * alpha2;
                         g[5] = 9 * g[4] * invR2 + an * a;
                     }
-                    ewaldEval(pa, R, g, sMp);
+                    ewaldEvalQ(pa, R, g, q);
                 }
+        // k space: exp(i 2 pi h.r / L) for the table's lattice vectors, visited in the table's order (hx, hy, hz
+        // ascending, 0 < |h|^2 <= hCut^2), as products of the axis phases exp(i 2 pi h_d r_d / L) (|h_d| <= 3)
+        // instead of one sincos per vector
         double kp[4] = {0, 0, 0, 0};
-        for (int k = 0; k < numH; ++k)
+        double2 qx[3], qy[3], qz[3]; // exp(i j 2 pi r_d / L), j = 1, 2, 3
         {
-            double hx = sh[k][0] * r[0] + sh[k][1] * r[1] + sh[k][2] * r[2];
+            const double w = 2.0 * M_PI / L;
             double s, c;
-            sincos(hx, &s, &c);
-            double sum = sh[k][3] * c + sh[k][4] * s;
-            double dif = sh[k][3] * s - sh[k][4] * c;
-            kp[0] -= sum;
-            kp[1] += dif * sh[k][0];
-            kp[2] += dif * sh[k][1];
-            kp[3] += dif * sh[k][2];
+            sincos(w * r[0], &s, &c);
+            qx[0] = make_double2(c, s);
+            sincos(w * r[1], &s, &c);
+            qy[0] = make_double2(c, s);
+            sincos(w * r[2], &s, &c);
+            qz[0] = make_double2(c, s);
+            qx[1] = cmul(qx[0], qx[0]), qy[1] = cmul(qy[0], qy[0]), qz[1] = cmul(qz[0], qz[0]);
+            qx[2] = cmul(qx[1], qx[0]), qy[2] = cmul(qy[1], qy[0]), qz[2] = cmul(qz[1], qz[0]);
+        }
+        const int hr = tab->hReps;
+        const double hc2 = tab->hCut2;
+        int k = 0;
+        for (int hx = -hr; hx <= hr; ++hx)
+        {
+            const double2 ex = phasePow(hx, qx[0], qx[1], qx[2]);
+            for (int hy = -hr; hy <= hr; ++hy)
+            {
+                const double2 exy = cmul(ex, phasePow(hy, qy[0], qy[1], qy[2]));
+                for (int hz = -hr; hz <= hr; ++hz)
+                {
+                    const int h2 = hx * hx + hy * hy + hz * hz;
+                    if (h2 == 0 || double(h2) > hc2 || k >= numH) continue;
+                    const double2 ph = cmul(exy, phasePow(hz, qz[0], qz[1], qz[2]));
+                    const double* e  = tab->h[k++];
+                    double sum = e[3] * ph.x + e[4] * ph.y;
+                    double dif = e[3] * ph.y - e[4] * ph.x;
+                    kp[0] -= sum;
+                    kp[1] += dif * e[0];
+                    kp[2] += dif * e[1];
+                    kp[3] += dif * e[2];
+                }
+            }
         }
         double pot = pa[0] + kp[0];
         double mi  = m[i];
@@ -207,6 +269,9 @@ int ewald(Context* ctx, const double* Mroot, const double* center, uint32_t firs
 {
     float lx = limits[1] - limits[0], ly = limits[3] - limits[2], lz = limits[5] - limits[4];
     if (!(lx == ly && ly == lz)) return setError(ctx, RYO_ERR_UNSUPPORTED, "Ewald gravity requires cubic boxes");
+    if (std::ceil(settings[2]) > 3.0)
+        return setError(ctx, RYO_ERR_UNSUPPORTED, "Ewald: hCut %g > 3 (the reference's maxCeilHcut, ewald.hpp:53)",
+                        settings[2]);
     EwaldTable* t = new EwaldTable();
     ewaldInitTable(*t, Mroot, center, int(settings[0]), double(lx), settings[1], settings[2], settings[3],
                    settings[4]);

@@ -63,8 +63,8 @@ __global__ __launch_bounds__(256) void sfcKeysKernel(const float* __restrict__ x
 /* ============================================================================ radix sort */
 
 constexpr int kSortThreads = 256;
-constexpr int kSortItems   = 16;
-constexpr int kSortTile    = kSortThreads * kSortItems; // 4096 keys per tile
+constexpr int kSortItems   = 8;  // 2048-key tiles: 30 KB of LDS and 74 VGPRs in the scatter, 5 waves per SIMD
+constexpr int kSortTile    = kSortThreads * kSortItems; // keys per tile
 constexpr int kDigits      = 256;
 constexpr int kPasses      = 8;
 
