@@ -7,6 +7,7 @@
 // Direct sum: P2P<Exact> (kernel.hpp:99-120) over all sources and images, accumulated in FP64, for a
 // list of sampled targets (the accuracy oracle; direct.cuh:36-107 is its FP32 reference counterpart).
 #include "common.hpp"
+#include "erfcx_table.hpp"
 
 #include <cmath>
 
@@ -14,6 +15,7 @@ namespace ryo
 {
 
 constexpr int kMaxH = 343;
+constexpr double kTwoOverSqrtPi = 1.1283791670955126;
 
 // the root quadrupole in the form its evaluation uses: M, the traceless components plus trace over 3, and half
 // the trace (ewald.hpp:97-108)
@@ -46,6 +48,41 @@ __host__ __device__ inline void ewaldEval(double* pa, const double* r, const dou
     double f = g[1] * M - g[2] * Qtr + g[3] * rQr;
     for (int d = 0; d < 3; ++d)
         pa[1 + d] += g[2] * Qr[d] - r[d] * f;
+}
+
+// erfcx(x) = exp(x^2) erfc(x) for x >= 0: on [0, 8] the degree-11 piece of x's interval of width 1/4
+// (erfcx_table.hpp, within 2.5e-16 of erfcx), beyond 8 the first three terms of the asymptotic series (there
+// erfc(x) < 1.2e-29, so its relative accuracy of 1e-5 is immaterial). The intervals present among the active
+// lanes are evaluated one at a time with wave-uniform coefficients (scalar loads); a wave's targets are SFC
+// neighbours that see an image at nearly the same distance, so there is usually one.
+__device__ inline double erfcxPos(double x)
+{
+    const int j = x < 8.0 ? int(x * 4.0) : kErfcxIntervals;
+    double r    = 0;
+    for (uint64_t todo = __ballot(true); todo;)
+    {
+        const int ju    = __builtin_amdgcn_readlane(j, __builtin_ctzll(todo));
+        const bool mine = j == ju;
+        if (mine)
+        {
+            if (ju == kErfcxIntervals)
+            {
+                const double u = 1.0 / (x * x);
+                r = (0.5 * kTwoOverSqrtPi / x) * (1.0 + u * (-0.5 + 0.75 * u));
+            }
+            else
+            {
+                const double* c = kErfcx[ju];
+                const double t  = (x - (ju + 0.5) * 0.25) * 8.0;
+                double v        = c[kErfcxTerms - 1];
+                for (int k = kErfcxTerms - 2; k >= 0; --k)
+                    v = fma(v, t, c[k]);
+                r = v;
+            }
+        }
+        todo &= ~__ballot(mine);
+    }
+    return r;
 }
 
 // exp(i j theta) for j in [-3, 3] from the powers p1 = exp(i theta), p2, p3 (j is wave-uniform)
@@ -117,7 +154,7 @@ void ewaldInitTable(EwaldTable& t, const double* Mroot, const double* center, in
             }
 }
 
-__global__ __launch_bounds__(256, 3) void ewaldKernel(const EwaldTable* __restrict__ tab, uint32_t first, uint32_t last,
+__global__ __launch_bounds__(256, 4) void ewaldKernel(const EwaldTable* __restrict__ tab, uint32_t first, uint32_t last,
                                                   const float* __restrict__ x, const float* __restrict__ y,
                                                   const float* __restrict__ z, const float* __restrict__ m, float G,
                                                   float* ugrav, float* ax, float* ay, float* az,
@@ -172,9 +209,12 @@ __global__ __launch_bounds__(256, 3) void ewaldKernel(const EwaldTable* __restri
                     else
                     {
                         double Rm = sqrt(R2), invR = 1.0 / Rm, invR2 = invR * invR;
-                        double a  = exp(-R2 * alpha2) * ka * invR2;
+                        double e2 = exp(-R2 * alpha2);
+                        double a  = e2 * ka * invR2;
                         double an = 1.0;
-                        double fn = inner ? -erf(alpha * Rm) : erfc(alpha * Rm);
+                        // erfc(aR) = exp(-a^2 R^2) erfcx(aR), reusing the exponential above; -erf = erfc - 1
+                        double ec = e2 * erfcxPos(alpha * Rm);
+                        double fn = inner ? ec - 1.0 : ec;
                         g[0] = fn * invR;
                         g[1] = g[0] * invR2 + a;
                         an *= 2 * alpha2;
@@ -422,8 +462,6 @@ struct EwaldK
 {
     double kx, ky, kz, w; // k = 2 pi h / L; w = 2 * 4 pi / L^3 * exp(-k^2 / 4a^2) / k^2 (the +-k pair)
 };
-
-constexpr double kTwoOverSqrtPi = 1.1283791670955126;
 
 // part[chunk * K + k] = sum over the chunk's sources of m (cos k.x, sin k.x)
 __global__ __launch_bounds__(256) void structureKernel(const float* __restrict__ x, const float* __restrict__ y,
