@@ -8,8 +8,8 @@ Default workload: N = 2^24 Plummer sphere (scale radius 1, truncated at r = 30),
 softening 1e-3, theta 0.5, Cartesian quadrupoles, bucket 64, open boundaries.
 
 Prints ONE JSON line. value = N / (mean wall time of a pass) in particles/s. The roofline object is for the
-dominant kernel (the traversal): algorithmic flops per launch (23 per P2P + 128 per M2P, the north-star
-count) over the kernel's mean duration from HIP events on the library's stream, against the FP32 vector peak.
+dominant kernel (the traversal): algorithmic flops per launch (23 per P2P + 128 per M2P, the count of the
+reference's README) over the kernel's mean duration from HIP events on the library's stream, against the FP32 vector peak.
 """
 
 from __future__ import annotations
