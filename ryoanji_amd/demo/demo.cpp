@@ -3,8 +3,11 @@
  * counterpart of ryoanji/test/demo.cu: build the octree of a cube of bodies, upsweep, traverse in 64-target
  * groups, and compare with an FP64 direct sum on sampled targets.
  *
- *   demo [log2 N = 16] [theta = 0.5]
+ *   demo [log2 N = 16] [theta = 0.5] [dump file]
  * prints one line: N cells p2p/target m2p/target pot err_p50 err_p1 err_max
+ * With a dump file it also writes, for checking against another implementation: int64 {N, nodes, leaves},
+ * float32 x, y, z as generated, float32 x, y, z, m, h in SFC order, float32 p, ax, ay, az, uint64 node keys
+ * (prefixes), int32 childOffsets, uint32 layout.
  */
 #include <hip/hip_runtime.h>
 
@@ -12,6 +15,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
+#include <fstream>
 #include <random>
 #include <vector>
 
@@ -48,6 +52,7 @@ int main(int argc, char** argv)
     std::vector<float> x(n), y(n), z(n), m(n, 1.0f / n), h(n, 1e-3f);
     for (size_t i = 0; i < n; ++i)
         x[i] = u(rng), y[i] = u(rng), z[i] = u(rng);
+    const std::vector<float> x0 = x, y0 = y, z0 = z;
 
     float *dx = devAlloc<float>(n), *dy = devAlloc<float>(n), *dz = devAlloc<float>(n), *dm = devAlloc<float>(n),
           *dh = devAlloc<float>(n);
@@ -99,6 +104,32 @@ int main(int argc, char** argv)
         err[i]    = std::sqrt(dX * dX + dY * dY + dZ * dZ) / r;
     }
     std::sort(err.begin(), err.end());
+    if (argc > 3)
+    {
+        std::ofstream f(argv[3], std::ios::binary);
+        const int64_t head[3] = {int64_t(n), int64_t(numNodes), int64_t(tb.numLeafNodes())};
+        f.write(reinterpret_cast<const char*>(head), sizeof(head));
+        for (const auto* v : {&x0, &y0, &z0})
+            f.write(reinterpret_cast<const char*>(v->data()), n * sizeof(float));
+        std::vector<float> hbuf(n);
+        for (const float* d : {(const float*)dx, (const float*)dy, (const float*)dz, (const float*)dm,
+                               (const float*)dh, (const float*)p, (const float*)ax, (const float*)ay,
+                               (const float*)az})
+        {
+            HIPCHECK(hipMemcpy(hbuf.data(), d, n * sizeof(float), hipMemcpyDeviceToHost));
+            f.write(reinterpret_cast<const char*>(hbuf.data()), n * sizeof(float));
+        }
+        std::vector<uint64_t> keys(numNodes);
+        std::vector<int32_t> child(numNodes);
+        std::vector<uint32_t> layout(tb.numLeafNodes() + 1);
+        HIPCHECK(hipMemcpy(keys.data(), tb.nodeKeys(), numNodes * sizeof(uint64_t), hipMemcpyDeviceToHost));
+        HIPCHECK(hipMemcpy(child.data(), tb.childOffsets(), numNodes * sizeof(int32_t), hipMemcpyDeviceToHost));
+        HIPCHECK(hipMemcpy(layout.data(), tb.layout(), layout.size() * sizeof(uint32_t), hipMemcpyDeviceToHost));
+        f.write(reinterpret_cast<const char*>(keys.data()), keys.size() * sizeof(uint64_t));
+        f.write(reinterpret_cast<const char*>(child.data()), child.size() * sizeof(int32_t));
+        f.write(reinterpret_cast<const char*>(layout.data()), layout.size() * sizeof(uint32_t));
+        if (!f) return 2;
+    }
     std::printf("%zu %d %.1f %.1f %.9f %.3e %.3e %.3e\n", n, numNodes, double(stats[0]) / n, double(stats[2]) / n, pot,
                 err[nt / 2], err[size_t(0.99 * nt)], err[nt - 1]);
     return 0;
