@@ -217,7 +217,10 @@ typedef struct
     float G;              /* gravitational constant                                     */
     int group_size;       /* targets per traversal group: 64 (one wave) or 16 (parity)  */
     int num_shells;       /* periodic image shells in the traversal (0 = open)          */
-    int ewald;            /* add the Ewald correction (periodic boxes)                  */
+    int ewald;            /* periodic boxes: 0 no Ewald correction; 1 the reference's correction
+                           * (computeGravityEwald, ewald.hpp:356-389); 2 the same correction with its k-space
+                           * potential consistent with its k-space acceleration (ewald.hpp:320 subtracts that
+                           * potential term with the opposite sign; accelerations are identical to 1) */
     double ewald_settings[5];
     int init_node;        /* 0: traversal starts at the root, 1: at its children       */
 } ryo_params;

@@ -67,6 +67,8 @@ def lib(which: str = "orc") -> C.CDLL:
                                          C.c_int, _P]),
             "ewald_f64": (C.c_double, [_P, _P, _P, _P, _P, _P, C.c_size_t, C.c_double, C.c_double, _P, _P, _P,
                                        _P, _P]),
+            "ewald_consistent_f64": (C.c_double, [_P, _P, _P, _P, _P, _P, C.c_size_t, C.c_double, C.c_double, _P,
+                                                  _P, _P, _P, _P]),
             "ewald_pair_sum_f64": (C.c_int, [_P, _P, _P, _P, _P, C.c_size_t, _P, C.c_size_t, C.c_double, C.c_double,
                                              C.c_double, _P]),
             "hilbert3d": (C.c_uint64, [C.c_uint, C.c_uint, C.c_uint, C.c_int]),
@@ -284,12 +286,14 @@ def ewald_pair_sum(x, y, z, h, m, L, targets=None, G=1.0, alpha_scale=6.5):
 EWALD_DEFAULT = (1, 2.6, 2.8, 2.0, 3.0e-3)
 
 
-def ewald(mroot, center, x, y, z, m, L, G=1.0, settings=EWALD_DEFAULT):
+def ewald(mroot, center, x, y, z, m, L, G=1.0, settings=EWALD_DEFAULT, consistent=False):
+    """the reference's Ewald correction; consistent=True takes the k-space potential consistent with the k-space
+    acceleration (the reference's potential has the opposite sign there; accelerations are the same)"""
     arrs = [_c(a, np.float32) for a in (x, y, z, m)]
     n = arrs[0].size
     ug, ax, ay, az = (np.zeros(n, np.float64) for _ in range(4))
     s = _c(settings, np.float64)
-    tot = _f("orc", "ewald_f64")(_ptr(_c(mroot, np.float64)), _ptr(_c(center, np.float64)), *(_ptr(a) for a in arrs),
+    tot = _f("orc", "ewald_consistent_f64" if consistent else "ewald_f64")(_ptr(_c(mroot, np.float64)), _ptr(_c(center, np.float64)), *(_ptr(a) for a in arrs),
                                  n, float(L), float(G), _ptr(s), _ptr(ug), _ptr(ax), _ptr(ay), _ptr(az))
     return tot, ug, ax, ay, az
 

@@ -952,7 +952,7 @@ void ewaldInit(EwaldParams<T>& p, const T* Mroot, const T* center, int numReplic
 
 // ewald.hpp:201-327
 template<class T>
-void ewaldParticle(const EwaldParams<T>& p, const T* pos, T* out)
+void ewaldParticle(const EwaldParams<T>& p, const T* pos, T* out, int kSign)
 {
     const T lCut = p.lCut, L = p.L;
     const T lCut2 = lCut * lCut * L * L;
@@ -1014,7 +1014,9 @@ void ewaldParticle(const EwaldParams<T>& p, const T* pos, T* out)
         T c = std::cos(hx), s = std::sin(hx);
         T sum = p.hfacCos[i] * c + p.hfacSin[i] * s;
         T dif = p.hfacCos[i] * s - p.hfacSin[i] * c;
-        k[0] -= sum;
+        // kSign -1: the reference's k-space potential (ewald.hpp:320); +1: the potential whose negative gradient
+        // is the k-space acceleration below
+        k[0] += T(kSign) * sum;
         for (int d = 0; d < 3; ++d)
             k[1 + d] += dif * p.hrScaled[i][d];
     }
@@ -1366,9 +1368,9 @@ int orc_ewald_pair_sum_f64(const float* x, const float* y, const float* z, const
 
 /* Ewald correction (ewald.hpp:356-389) in T = double: adds G*m*phi (ugrav), G*a; returns 0.5 G sum(m phi).
  * settings = {numReplicaShells, lCut, hCut, alphaScale, smallR}. */
-double orc_ewald_f64(const double* Mroot, const double* center, const float* x, const float* y, const float* z,
+static double ewaldAll(const double* Mroot, const double* center, const float* x, const float* y, const float* z,
                      const float* m, size_t n, double L, double G, const double* settings, double* ugrav, double* ax,
-                     double* ay, double* az)
+                     double* ay, double* az, int kSign)
 {
     auto* p = new EwaldParams<double>();
     ewaldInit(*p, Mroot, center, int(settings[0]), L, settings[1], settings[2], settings[3], settings[4]);
@@ -1379,7 +1381,7 @@ double orc_ewald_f64(const double* Mroot, const double* center, const float* x, 
         for (size_t i = 0; i < n; ++i)
         {
             double pos[3] = {x[i], y[i], z[i]}, out[4];
-            ewaldParticle(*p, pos, out);
+            ewaldParticle(*p, pos, out, kSign);
             if (ugrav) ugrav[i] += G * out[0] * m[i];
             U += out[0] * m[i];
             ax[i] += G * out[1];
@@ -1389,6 +1391,22 @@ double orc_ewald_f64(const double* Mroot, const double* center, const float* x, 
     }
     delete p;
     return 0.5 * G * U;
+}
+
+double orc_ewald_f64(const double* Mroot, const double* center, const float* x, const float* y, const float* z,
+                     const float* m, size_t n, double L, double G, const double* settings, double* ugrav, double* ax,
+                     double* ay, double* az)
+{
+    return ewaldAll(Mroot, center, x, y, z, m, n, L, G, settings, ugrav, ax, ay, az, -1);
+}
+
+/* the same correction with the k-space potential taken consistent with the k-space acceleration (the opposite
+ * sign of ewald.hpp:320 for the potential term; accelerations unchanged) */
+double orc_ewald_consistent_f64(const double* Mroot, const double* center, const float* x, const float* y,
+                                const float* z, const float* m, size_t n, double L, double G, const double* settings,
+                                double* ugrav, double* ax, double* ay, double* az)
+{
+    return ewaldAll(Mroot, center, x, y, z, m, n, L, G, settings, ugrav, ax, ay, az, +1);
 }
 
 /* single-interaction kernels, for golden checks */

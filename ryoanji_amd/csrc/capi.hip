@@ -27,7 +27,7 @@ int groupSplits(Context* ctx, uint32_t first, uint32_t last, const float* x, con
                 unsigned groupSize, float tolFactor, float extentRatio, uint32_t* groups, size_t capacity,
                 int* numGroups);
 int ewald(Context*, const double*, const double*, uint32_t, uint32_t, const float*, const float*, const float*,
-          const float*, const float*, float, const double*, float*, float*, float*, float*, double*);
+          const float*, const float*, float, const double*, float*, float*, float*, float*, double*, bool);
 int directSum(Context*, const float*, const float*, const float*, const float*, const float*, size_t,
               const int64_t*, size_t, double, const double*, int, double*);
 int ewaldPairSum(Context* ctx, const float* x, const float* y, const float* z, const float* h, const float* m,
@@ -287,7 +287,8 @@ int ryo_ewald_f32(ryo_ctx* h, const double* Mroot, const double* center, uint32_
                   const double* settings, float* ugrav, float* ax, float* ay, float* az, double* total_pot)
 {
     CTX_OR_FAIL(h);
-    return ewald(ctx, Mroot, center, first, last, x, y, z, m, limits, G, settings, ugrav, ax, ay, az, total_pot);
+    return ewald(ctx, Mroot, center, first, last, x, y, z, m, limits, G, settings, ugrav, ax, ay, az, total_pot,
+                 false);
 }
 
 int ryo_direct_sum_f64(ryo_ctx* h, const float* x, const float* y, const float* z, const float* hh, const float* m,
@@ -585,6 +586,7 @@ int ryo_system_gravity(ryo_system* s, const ryo_params* prm, double* totalPot)
 {
     if (!s || !prm) return RYO_ERR_ARG;
     Context* ctx = s->ctx;
+    if (prm->ewald < 0 || prm->ewald > 2) return setError(ctx, RYO_ERR_ARG, "system: ewald must be 0, 1 or 2");
     const size_t n = s->n;
     float** A = s->a[s->cur];
     int rc;
@@ -608,7 +610,7 @@ int ryo_system_gravity(ryo_system* s, const ryo_params* prm, double* totalPot)
         double ep = 0;
         mark(s, 7);
         if ((rc = ewald(ctx, root, rc64, 0, uint32_t(n), A[0], A[1], A[2], A[3], s->limits, prm->G,
-                        prm->ewald_settings, s->p, s->ax, s->ay, s->az, &ep)))
+                        prm->ewald_settings, s->p, s->ax, s->ay, s->az, &ep, prm->ewald == 2)))
             return rc;
         mark(s, 8);
         if (ctx->timing)

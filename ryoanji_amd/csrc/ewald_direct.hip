@@ -31,6 +31,7 @@ struct EwaldTable
     double center[3];
     int numReplicaShells, numEwaldShells, numH, hReps;
     double L, lCut, alphaScale, smallR, hCut2;
+    double kPotSign; // -1: the reference's k-space potential (ewald.hpp:320); +1: consistent with the acceleration
     double h[kMaxH][5]; // {2 pi h / L (3), hfac_cos, hfac_sin}
 };
 
@@ -262,7 +263,7 @@ __global__ __launch_bounds__(256, 4) void ewaldKernel(const EwaldTable* __restri
                     const double* e  = tab->h[k++];
                     double sum = e[3] * ph.x + e[4] * ph.y;
                     double dif = e[3] * ph.y - e[4] * ph.x;
-                    kp[0] -= sum;
+                    kp[0] += tab->kPotSign * sum;
                     kp[1] += dif * e[0];
                     kp[2] += dif * e[1];
                     kp[3] += dif * e[2];
@@ -305,7 +306,7 @@ __global__ __launch_bounds__(256) void sumFixedKernel(const double* __restrict__
 
 int ewald(Context* ctx, const double* Mroot, const double* center, uint32_t first, uint32_t last, const float* x,
           const float* y, const float* z, const float* m, const float* limits, float G, const double* settings,
-          float* ugrav, float* ax, float* ay, float* az, double* totalPot)
+          float* ugrav, float* ax, float* ay, float* az, double* totalPot, bool consistentPotential)
 {
     float lx = limits[1] - limits[0], ly = limits[3] - limits[2], lz = limits[5] - limits[4];
     if (!(lx == ly && ly == lz)) return setError(ctx, RYO_ERR_UNSUPPORTED, "Ewald gravity requires cubic boxes");
@@ -315,6 +316,7 @@ int ewald(Context* ctx, const double* Mroot, const double* center, uint32_t firs
     EwaldTable* t = new EwaldTable();
     ewaldInitTable(*t, Mroot, center, int(settings[0]), double(lx), settings[1], settings[2], settings[3],
                    settings[4]);
+    t->kPotSign = consistentPotential ? 1.0 : -1.0;
     if (t->numEwaldShells == 0 || last <= first)
     {
         delete t;

@@ -254,3 +254,38 @@ def test_full_size_invariants(ctx, name, mk):
     rep = sysm.accuracy(num_targets=2048, seed=7)
     assert rep["err_p99"] < 1.5e-3, rep
     sysm.close()
+
+
+def test_consistent_ewald_potential(ctx):
+    """ewald = 2: accelerations bit-identical to the reference's correction (ewald = 1), potentials equal to the
+    oracle's consistent correction and much closer to the exact Ewald sum"""
+    s = W.uniform_cube(16384, seed=23, lo=0.0, hi=1.0, periodic=True)
+    sys1, pot1, p1, ax1, ay1, az1, order = _solve(ctx, s, periodic=True, num_shells=1, ewald=1)
+    rep1 = sys1.accuracy(num_targets=400, seed=4)
+    sys2, pot2, p2, ax2, ay2, az2, order2 = _solve(ctx, s, periodic=True, num_shells=1, ewald=2)
+    rep2 = sys2.accuracy(num_targets=400, seed=4)
+    assert np.array_equal(order, order2)
+    for a, b in ((ax1, ax2), (ay1, ay2), (az1, az2)):
+        assert np.array_equal(a.view(np.int32), b.view(np.int32))
+    # the potentials differ from ewald = 1 by the k-space potential term only; against the oracle
+    t = oracle_tree(s)
+    c64, q64 = O.upsweep(t["x"], t["y"], t["z"], t["m"], t["leaves"], t["layout"], s.limits.astype(np.float64), 2.0,
+                         precision="f64")
+    _, e1, *_ = O.ewald(q64[0], c64[0, :3], t["x"], t["y"], t["z"], t["m"], 1.0)
+    _, e2, *_ = O.ewald(q64[0], c64[0, :3], t["x"], t["y"], t["z"], t["m"], 1.0, consistent=True)
+    d = (p2.astype(np.float64) - p1) - (e2 - e1)
+    assert np.abs(d).max() <= 1e-5 * np.abs(e2 - e1).max()
+    assert rep2["err_p99"] == rep1["err_p99"]
+    # where the root quadrupole represents the box well (a compact clump at its center) the consistent potential
+    # follows the exact Ewald sum to the traversal's accuracy, the reference's is off by twice the k-space term
+    n = 4096
+    u = [W.uniform01(W._stream(31, d), 0, n) for d in range(4)]
+    r = 0.02 * np.cbrt(u[0])
+    cth, phi = 1 - 2 * u[1], 2 * np.pi * u[2]
+    sth = np.sqrt(1 - cth * cth)
+    xyz = [0.5 + r * sth * np.cos(phi), 0.5 + r * sth * np.sin(phi), 0.5 + r * cth]
+    clump = W.System(*(a.astype(np.float32) for a in xyz), np.full(n, 1.0 / n, np.float32),
+                     np.full(n, 1e-5, np.float32), [0, 1, 0, 1, 0, 1], periodic=True)
+    c1 = _solve(ctx, clump, periodic=True, num_shells=1, ewald=1)[0].accuracy(num_targets=300, seed=5)
+    c2 = _solve(ctx, clump, periodic=True, num_shells=1, ewald=2)[0].accuracy(num_targets=300, seed=5)
+    assert c2["pot_rel_err"] < 1e-4 and c1["pot_rel_err"] > 1e-3, (c1, c2)

@@ -83,16 +83,17 @@ def test_softened_pairs_follow_the_traversal_law():
     assert np.abs((b[0, 1:] - a[0, 1:]) - (fb - fa)).max() < 1e-3 * np.abs(fb - fa).max()
 
 
-def _pipeline(x, y, z, h, m, L):
+def _pipeline(x, y, z, h, m, L, consistent=False):
     """the reference's periodic path in the oracle: traversal over 1 replica shell (opening every cell: one leaf)
-    plus the Ewald correction from the root multipole (default settings: alpha 2/L, hCut 2.8)"""
+    plus the Ewald correction from the root multipole (default settings: alpha 2/L, hCut 2.8); consistent=True
+    takes the k-space potential consistent with the k-space acceleration"""
     lim = np.array([-L / 2, L / 2] * 3, np.float32)
     leaves = np.array([0, 1 << 63], np.uint64)
     lay = np.array([0, x.size], np.uint32)
     c, q = O.upsweep(x, y, z, m, leaves, lay, lim, 2.0)
     _, ug, ax, ay, az = O.compute_gravity_f64(leaves, lay, c, q, x, y, z, h, m, lim, num_shells=1)
     c64, q64 = O.upsweep(x, y, z, m, leaves, lay, lim.astype(np.float64), 2.0, precision="f64")
-    _, eug, eax, eay, eaz = O.ewald(q64[0], c64[0, :3], x, y, z, m, L)
+    _, eug, eax, eay, eaz = O.ewald(q64[0], c64[0, :3], x, y, z, m, L, consistent=consistent)
     return ug + eug, np.stack([ax + eax, ay + eay, az + eaz], axis=1)
 
 
@@ -100,7 +101,7 @@ def _pipeline(x, y, z, h, m, L):
 def test_single_particle_traversal_plus_ewald(L):
     """one particle: the root multipole is the particle itself, so the traversal over the replica images plus the
     Ewald correction is the exact periodic sum up to the correction's cutoffs, except for one term: the
-    reference's k-space potential (ewald.hpp:315-319 with hfac_cos = -g0 M, :111 and :180) has the opposite sign
+    reference's k-space potential (ewald.hpp:315-320 with hfac_cos = -g0 M, :111 and :180) has the opposite sign
     of its k-space acceleration, so the path's potential exceeds the exact one by twice the k-space sum
     2 M sum_{0 < |h| <= 2.8} exp(-pi^2 h^2 / (alpha L)^2) / (pi h^2 L) (alpha L = 2); its accelerations are exact"""
     x = np.array([0.1 * L], np.float32)
@@ -133,3 +134,35 @@ def test_close_pair_accelerations_equal_traversal_plus_ewald():
     # and the periodic part matters at that level: without the images the accelerations differ by far more
     iso = O.direct_sum(x, y, z, h, m)
     assert np.abs(iso[:, 1:] - out[:, 1:]).max() > 1e-6 * scale
+
+
+@pytest.mark.parametrize("L", [1.0, 2.0])
+def test_consistent_potential_is_exact_for_one_particle(L):
+    """with the k-space potential consistent with the k-space acceleration, the path's potential of one particle is
+    the exact periodic potential (the Madelung constant) and its acceleration is unchanged"""
+    x = np.array([0.1 * L], np.float32)
+    y = np.array([-0.2 * L], np.float32)
+    z = np.array([0.05 * L], np.float32)
+    m = np.array([1.0], np.float32)
+    h = np.array([1e-3], np.float32)
+    pot, acc = _pipeline(x, y, z, h, m, L, consistent=True)
+    pot_ref, acc_ref = _pipeline(x, y, z, h, m, L)
+    out = O.ewald_pair_sum(x, y, z, h, m, L)
+    assert abs(pot[0] - out[0, 0]) < 1e-9 * abs(out[0, 0])
+    assert abs(pot[0] - 2.8372974794806 / L) < 5e-9  # the correction's k-space cut (hCut 2.8) leaves ~3e-9
+    assert np.array_equal(acc, acc_ref)
+
+
+def test_consistent_potential_of_a_close_pair():
+    """two particles: the consistent potential matches the exact sum to the root quadrupole's accuracy, the
+    reference's does not; accelerations are the same in both"""
+    d = np.array([0.003, 0.0018, -0.0012])
+    x, y, z = (np.array([0.02 + d[k], 0.02 - d[k]], np.float32) for k in range(3))
+    m = np.array([0.5, 0.5], np.float32)
+    h = np.array([1e-4, 1e-4], np.float32)
+    pc, ac = _pipeline(x, y, z, h, m, 1.0, consistent=True)
+    pr, ar = _pipeline(x, y, z, h, m, 1.0)
+    out = O.ewald_pair_sum(x, y, z, h, m, 1.0)
+    assert np.array_equal(ac, ar)
+    assert np.abs(pc - out[:, 0]).max() < 1e-8 * np.abs(out[:, 0]).max()
+    assert np.abs(pr - out[:, 0]).max() > 1e-3 * np.abs(out[:, 0]).max()
