@@ -67,6 +67,8 @@ constexpr int kSortItems   = 8;  // 2048-key tiles: 30 KB of LDS and 74 VGPRs in
 constexpr int kSortTile    = kSortThreads * kSortItems; // keys per tile
 constexpr int kDigits      = 256;
 constexpr int kPasses      = 8;
+constexpr int kScanChunk   = 32; // tiles per chunk of the tile-offset scan
+constexpr int kSmallScanTiles = 1024; // up to this many tiles the offsets take one launch
 
 // element count of a sort: n, or the count held in device memory (at most n) when dN is given, so that a
 // sequence whose sizes are produced on the device can run without a host round trip
@@ -112,7 +114,8 @@ __global__ __launch_bounds__(256) void digitBaseKernel(const uint32_t* __restric
     base[p * kDigits + d] = blockExclusiveScan256(hist[p * kDigits + d], sw, &tot);
 }
 
-// per-tile digit counts of one pass: tileHist[d * numTiles + tile]
+// per-tile digit counts of one pass, one contiguous row per tile: tileHist[tile * 256 + d] (a coalesced 1 KB write
+// per tile, read back the same way by the offset scan and the scatter)
 __global__ __launch_bounds__(kSortThreads) void tileHistKernel(const uint64_t* __restrict__ keys, size_t n,
                                                               const uint32_t* __restrict__ dN, int shift,
                                                               uint32_t* __restrict__ tileHist, int numTiles)
@@ -129,25 +132,70 @@ __global__ __launch_bounds__(kSortThreads) void tileHistKernel(const uint64_t* _
         if (i < n) atomicAdd(&h[(keys[i] >> shift) & 255u], 1u);
     }
     __syncthreads();
-    tileHist[size_t(threadIdx.x) * numTiles + blockIdx.x] = h[threadIdx.x];
+    tileHist[size_t(blockIdx.x) * kDigits + threadIdx.x] = h[threadIdx.x];
 }
 
-// per digit (one block each): exclusive scan over tiles plus the digit base -> global tile offsets
-__global__ __launch_bounds__(256) void tileScanKernel(uint32_t* __restrict__ tileHist, int numTiles,
-                                                     const uint32_t* __restrict__ digitBase)
+// tile offsets in three coalesced steps: (1) the digit sums of each chunk of kScanChunk tiles, (2) per digit an
+// exclusive scan over the chunks from the digit's base, (3) per chunk the running offsets of its tiles, written in
+// place of the counts: tileHist[tile * 256 + d] = base[d] + (keys of digit d in earlier tiles)
+__global__ __launch_bounds__(256) void tileChunkSumKernel(const uint32_t* __restrict__ tileHist, int numTiles,
+                                                     uint32_t* __restrict__ chunkSum)
+{
+    const int c = blockIdx.x, d = threadIdx.x;
+    const int t1 = min(numTiles, (c + 1) * kScanChunk);
+    uint32_t sum = 0;
+    for (int t = c * kScanChunk; t < t1; ++t)
+        sum += tileHist[size_t(t) * kDigits + d];
+    chunkSum[size_t(c) * kDigits + d] = sum;
+}
+
+__global__ __launch_bounds__(256) void tileChunkScanKernel(uint32_t* __restrict__ chunkSum, int numChunks,
+                                                      const uint32_t* __restrict__ digitBase)
 {
     __shared__ uint32_t sw[4];
     const int d = blockIdx.x;
-    uint32_t* row = tileHist + size_t(d) * numTiles;
+    uint32_t carry = digitBase[d];
+    for (int c0 = 0; c0 < numChunks; c0 += 256)
+    {
+        const int c = c0 + threadIdx.x;
+        const uint32_t v = c < numChunks ? chunkSum[size_t(c) * kDigits + d] : 0u;
+        uint32_t tot;
+        const uint32_t ex = blockExclusiveScan256(v, sw, &tot);
+        if (c < numChunks) chunkSum[size_t(c) * kDigits + d] = carry + ex;
+        carry += tot;
+    }
+}
+
+// the same offsets in one launch for few tiles (one block per digit; its reads are strided, which costs little
+// when there are few tiles and saves two launches per pass)
+__global__ __launch_bounds__(256) void tileScanSmallKernel(uint32_t* __restrict__ tileHist, int numTiles,
+                                                          const uint32_t* __restrict__ digitBase)
+{
+    __shared__ uint32_t sw[4];
+    const int d = blockIdx.x;
     uint32_t carry = digitBase[d];
     for (int t0 = 0; t0 < numTiles; t0 += 256)
     {
-        int t = t0 + threadIdx.x;
-        uint32_t v = t < numTiles ? row[t] : 0;
+        const int t = t0 + threadIdx.x;
+        const uint32_t v = t < numTiles ? tileHist[size_t(t) * kDigits + d] : 0u;
         uint32_t tot;
-        uint32_t ex = blockExclusiveScan256(v, sw, &tot);
-        if (t < numTiles) row[t] = carry + ex;
+        const uint32_t ex = blockExclusiveScan256(v, sw, &tot);
+        if (t < numTiles) tileHist[size_t(t) * kDigits + d] = carry + ex;
         carry += tot;
+    }
+}
+
+__global__ __launch_bounds__(256) void tileOffsetKernel(uint32_t* __restrict__ tileHist, int numTiles,
+                                                       const uint32_t* __restrict__ chunkSum)
+{
+    const int c = blockIdx.x, d = threadIdx.x;
+    const int t1 = min(numTiles, (c + 1) * kScanChunk);
+    uint32_t carry = chunkSum[size_t(c) * kDigits + d];
+    for (int t = c * kScanChunk; t < t1; ++t)
+    {
+        const uint32_t v = tileHist[size_t(t) * kDigits + d];
+        tileHist[size_t(t) * kDigits + d] = carry;
+        carry += v;
     }
 }
 
@@ -212,7 +260,7 @@ __global__ __launch_bounds__(kSortThreads) void scatterKernel(const uint64_t* __
         wcnt[2][d] = c0 + c1;
         wcnt[3][d] = c0 + c1 + c2;
         tstart[d] = start;
-        gbase[d] = tileOffsets[size_t(d) * numTiles + tile];
+        gbase[d] = tileOffsets[size_t(tile) * kDigits + d];
     }
     __syncthreads();
 #pragma unroll
@@ -310,8 +358,9 @@ static unsigned gridFor(size_t n, int threads, int maxBlocks)
 
 size_t sortTempBytes(size_t n)
 {
-    size_t numTiles = (n + kSortTile - 1) / kSortTile;
-    return 256 * sizeof(uint32_t) * (2 * kPasses) + size_t(kDigits) * numTiles * sizeof(uint32_t) + 256;
+    size_t numTiles  = (n + kSortTile - 1) / kSortTile;
+    size_t numChunks = (numTiles + kScanChunk - 1) / kScanChunk;
+    return 256 * sizeof(uint32_t) * (2 * kPasses) + size_t(kDigits) * (numTiles + numChunks) * sizeof(uint32_t) + 256;
 }
 
 // stable LSD sort on the lowest `digits` 8-bit digits of the keys; the result is in keys/vals. With dN, n is the
@@ -328,6 +377,8 @@ int sortPairsDigits(Context* ctx, uint64_t* keys, int32_t* vals, size_t n, uint6
     uint32_t* hist     = reinterpret_cast<uint32_t*>(tmp);
     uint32_t* dbase    = hist + kPasses * kDigits;
     uint32_t* tileHist = dbase + kPasses * kDigits;
+    const int numChunks = (numTiles + kScanChunk - 1) / kScanChunk;
+    uint32_t* chunkSum  = tileHist + size_t(kDigits) * numTiles;
     hipStream_t s = ctx->stream;
     zeroWordsKernel<<<8, 256, 0, s>>>(hist, kPasses * kDigits); // a kernel, not a memset node, under capture
     histAllKernel<<<gridFor(n, 256, ctx->numCUs * 4), 256, 0, s>>>(keys, n, dN, hist);
@@ -340,7 +391,14 @@ int sortPairsDigits(Context* ctx, uint64_t* keys, int32_t* vals, size_t n, uint6
     {
         int shift = 8 * p;
         tileHistKernel<<<numTiles, kSortThreads, 0, s>>>(kin, n, dN, shift, tileHist, numTiles);
-        tileScanKernel<<<kDigits, 256, 0, s>>>(tileHist, numTiles, dbase + p * kDigits);
+        if (numTiles <= kSmallScanTiles)
+            tileScanSmallKernel<<<kDigits, 256, 0, s>>>(tileHist, numTiles, dbase + p * kDigits);
+        else
+        {
+            tileChunkSumKernel<<<numChunks, 256, 0, s>>>(tileHist, numTiles, chunkSum);
+            tileChunkScanKernel<<<kDigits, 256, 0, s>>>(chunkSum, numChunks, dbase + p * kDigits);
+            tileOffsetKernel<<<numChunks, 256, 0, s>>>(tileHist, numTiles, chunkSum);
+        }
         scatterKernel<<<numTiles, kSortThreads, 0, s>>>(kin, vin, kout, vout, n, dN, shift, tileHist, numTiles);
         std::swap(kin, kout);
         std::swap(vin, vout);
