@@ -223,6 +223,9 @@ typedef struct
                            * potential term with the opposite sign; accelerations are identical to 1) */
     double ewald_settings[5];
     int init_node;        /* 0: traversal starts at the root, 1: at its children       */
+    int soft_mac;         /* 0: the reference's MAC (macs.hpp:64-82). 1: also open every cell nearer than
+                           * its radius plus twice the largest softening, so that no softened pair is
+                           * approximated by a multipole (resident systems; costs a softening-range read-back) */
 } ryo_params;
 
 void ryo_default_params(ryo_params* p);

@@ -45,6 +45,7 @@ class Params(C.Structure):
         ("ewald", C.c_int),
         ("ewald_settings", C.c_double * 5),
         ("init_node", C.c_int),
+        ("soft_mac", C.c_int),
     ]
 
 

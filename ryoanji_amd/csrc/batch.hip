@@ -191,6 +191,7 @@ int uploadTables(ryo_batch* b, const ryo_params* prm, const float* theta, const 
     if (prm->init_node != 0 && prm->init_node != 1) return setError(ctx, RYO_ERR_ARG, "batch: init_node 0 or 1");
     if (prm->num_shells != 0 || prm->ewald != 0)
         return setError(ctx, RYO_ERR_UNSUPPORTED, "batch: open boundaries only");
+    if (prm->soft_mac != 0) return setError(ctx, RYO_ERR_UNSUPPORTED, "batch: the reference MAC only");
     std::vector<uint32_t> tiles(S), ngroups(S);
     std::vector<KeyBox> kb(S);
     std::vector<MacBox> mb(S);

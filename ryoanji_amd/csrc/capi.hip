@@ -20,7 +20,7 @@ int buildLeaves(Context*, const uint64_t*, size_t, unsigned, uint64_t*, uint32_t
 int linkOctree(Context*, const uint64_t*, int, uint64_t*, int32_t*, int32_t*, int32_t*, int32_t*, int32_t*, int32_t*);
 int upsweep(Context*, const float*, const float*, const float*, const float*, const uint32_t*, int, const uint64_t*,
             const int32_t*, const int32_t*, const int32_t*, const int32_t*, const float*, float, float*, float*,
-            double*, double*);
+            double*, double*, double);
 int fixedGroups(Context*, uint32_t, uint32_t, unsigned, uint32_t*, int*);
 int groupSplits(Context* ctx, uint32_t first, uint32_t last, const float* x, const float* y, const float* z,
                 const float* h, const uint64_t* leaves, int numLeaves, const uint32_t* layout, const float* limits,
@@ -28,6 +28,7 @@ int groupSplits(Context* ctx, uint32_t first, uint32_t last, const float* x, con
                 int* numGroups);
 int ewald(Context*, const double*, const double*, uint32_t, uint32_t, const float*, const float*, const float*,
           const float*, const float*, float, const double*, float*, float*, float*, float*, double*, bool);
+int softeningRange(Context* ctx, const float* h, size_t n, float* hmin, float* hmax);
 int directSum(Context*, const float*, const float*, const float*, const float*, const float*, size_t,
               const int64_t*, size_t, double, const double*, int, double*);
 int ewaldPairSum(Context* ctx, const float* x, const float* y, const float* z, const float* h, const float* m,
@@ -224,7 +225,7 @@ int ryo_upsweep_f32(ryo_ctx* h, const float* x, const float* y, const float* z, 
 {
     CTX_OR_FAIL(h);
     return upsweep(ctx, x, y, z, m, layout, num_leaves, prefixes, child, parents, l2i, level_range, limits,
-                   inv_theta, centers, mp, c64, q64);
+                   inv_theta, centers, mp, c64, q64, 0.0);
 }
 
 int ryo_fixed_groups(ryo_ctx* h, uint32_t first, uint32_t last, unsigned group_size, uint32_t* groups,
@@ -401,6 +402,7 @@ void ryo_default_params(ryo_params* p)
     for (int k = 0; k < 5; ++k)
         p->ewald_settings[k] = def[k];
     p->init_node = 0;
+    p->soft_mac  = 0;
 }
 
 int ryo_system_create(ryo_ctx* h, size_t capacity, ryo_system** out)
@@ -556,8 +558,16 @@ int ryo_system_build(ryo_system* s, const ryo_params* prm)
                          nullptr)))
         return rc;
     mark(s, 5);
+    double softReach = 0;
+    if (prm->soft_mac)
+    {
+        // the softening-safe MAC needs the largest softening of the system (one read-back)
+        float hmin = 0, hmax = 0;
+        if ((rc = softeningRange(ctx, A[4], n, &hmin, &hmax))) return rc;
+        softReach = 2.0 * double(hmax);
+    }
     if ((rc = upsweep(ctx, A[0], A[1], A[2], A[3], s->layout, L, s->prefixes, s->child, s->parents, s->l2i,
-                      s->levelRange, s->limits, 1.0f / prm->theta, s->centers, s->mp, s->c64, s->q64)))
+                      s->levelRange, s->limits, 1.0f / prm->theta, s->centers, s->mp, s->c64, s->q64, softReach)))
         return rc;
     mark(s, 6);
     int gsz = prm->group_size > 0 ? prm->group_size : 64;

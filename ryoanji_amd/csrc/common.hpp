@@ -189,6 +189,7 @@ enum ScratchSlot : int
     kSlotSplits,
     kSlotAccuracy,
     kSlotPack,
+    kSlotSoft,
     kNumSlots
 };
 

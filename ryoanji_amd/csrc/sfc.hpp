@@ -318,6 +318,7 @@ struct MacBox
     double half[3]; // half of the smallest cell edge per axis: 0.5 L_d / 2^bits[d]
     Bits3 bits;     // key bits per axis
     float invTheta;
+    double softReach = 0; // > 0: cells also open within their radius plus this distance (2 x the largest h)
 };
 
 // per-axis key bits of a box given as {xmin, xmax, ymin, ymax, zmin, zmax} (Box<float>::computeBoxDimBits and

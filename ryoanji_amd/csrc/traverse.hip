@@ -14,6 +14,8 @@
 // with scalar loads and broadcast to all 64 targets.
 #include "common.hpp"
 
+#include <cstring>
+
 namespace ryo
 {
 
@@ -354,6 +356,31 @@ __global__ void travInitKernel(unsigned* ctl)
 {
     int i = threadIdx.x;
     if (i < 64) ctl[i] = (i == kCtlHRange) ? ~0u : 0u;
+}
+
+// smallest and largest softening of h[0..n) (one read-back; synchronizes the stream)
+int softeningRange(Context* ctx, const float* h, size_t n, float* hmin, float* hmax)
+{
+    if (n == 0) return setError(ctx, RYO_ERR_ARG, "softening range: no particles");
+    auto* w = static_cast<unsigned*>(ctx->getScratch(kSlotSoft, 64 * sizeof(unsigned)));
+    if (!w) return RYO_ERR_HIP;
+    hipStream_t s = ctx->stream;
+    travInitKernel<<<1, 64, 0, s>>>(w); // w[kCtlHRange..+1] = {~0u, 0}
+    hRangeKernel<<<std::max(1u, std::min(iceil(n, 4096), unsigned(ctx->numCUs * 2))), 256, 0, s>>>(
+        h, n, nullptr, 0, w + kCtlHRange);
+    RYO_LAUNCH_CHECK(ctx);
+    unsigned o[2];
+    RYO_HIP(ctx, hipMemcpyAsync(o, w + kCtlHRange, sizeof(o), hipMemcpyDeviceToHost, s));
+    RYO_HIP(ctx, hipStreamSynchronize(s));
+    // the order-preserving map back to float (the inverse of floatToOrdered)
+    for (int k = 0; k < 2; ++k)
+    {
+        unsigned u = (o[k] & 0x80000000u) ? (o[k] ^ 0x80000000u) : ~o[k];
+        float f;
+        std::memcpy(&f, &u, sizeof(f));
+        (k == 0 ? *hmin : *hmax) = f;
+    }
+    return RYO_OK;
 }
 
 // per-group source tree of a batch of systems: group g belongs to system gSys[g]; the tree arrays and the source

@@ -34,6 +34,10 @@ __device__ inline double macSquared(uint64_t prefix, double cx, double cy, doubl
     double s   = sqrt(dx * dx + dy * dy + dz * dz);
     double l   = 2.0 * fmax(fmax(gs[0], gs[1]), gs[2]);
     double mac = l * double(b.invTheta) + s;
+    // softening-safe variant: every particle of the cell lies within s + the half diagonal of its box from the
+    // center of mass, so a cell accepted beyond that radius plus 2 h_max interacts with no target inside the
+    // softening length
+    if (b.softReach > 0) mac = fmax(mac, s + sqrt(gs[0] * gs[0] + gs[1] * gs[1] + gs[2] * gs[2]) + b.softReach);
     return mac * mac;
 }
 
@@ -259,7 +263,7 @@ MacBox makeMacBox(const float* limits, float invTheta)
 int upsweep(Context* ctx, const float* x, const float* y, const float* z, const float* m, const uint32_t* layout,
             int L, const uint64_t* prefixes, const int32_t* child, const int32_t* parents, const int32_t* l2i,
             const int32_t* levelRange, const float* limits, float invTheta, float* centers, float* mpoles, double* c64,
-            double* q64)
+            double* q64, double softReach)
 {
     if (L < 1) return setError(ctx, RYO_ERR_ARG, "upsweep: need at least one leaf");
     if (!levelRange) return setError(ctx, RYO_ERR_ARG, "upsweep: level ranges are required");
@@ -286,7 +290,8 @@ int upsweep(Context* ctx, const float* x, const float* y, const float* z, const 
         a.box.lo[d]   = lo;
         a.box.half[d] = 0.5 * (hi - lo) / double(1u << bits.b[d]);
     }
-    a.box.invTheta = invTheta;
+    a.box.invTheta  = invTheta;
+    a.box.softReach = softReach;
     hipStream_t s = ctx->stream;
     leafUpsweepKernel<<<iceil(size_t(L) * 8, 256), 256, 0, s>>>(a);
     // levels from the deepest possible parent level to the root; each launch reads its node range on the device,
