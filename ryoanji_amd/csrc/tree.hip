@@ -151,17 +151,22 @@ __global__ __launch_bounds__(kLeafThreads) void leafCountKernel(const uint64_t* 
 // exclusive scan of the block sums in place (one block); total into blockSums[numBlocks]
 __global__ __launch_bounds__(256) void scanBlocksKernel(uint32_t* __restrict__ sums, int numBlocks)
 {
+    // thread t scans the contiguous run [t k, (t + 1) k): independent loads, one block-wide scan of the run sums
     __shared__ uint32_t sw[4];
-    uint32_t carry = 0;
-    for (int b0 = 0; b0 < numBlocks; b0 += 256)
+    const int k  = (numBlocks + 255) / 256;
+    const int b0 = int(threadIdx.x) * k, b1 = min(numBlocks, b0 + k);
+    uint32_t run = 0;
+    for (int b = b0; b < b1; ++b)
+        run += sums[b];
+    uint32_t tot;
+    uint32_t carry = blockExclusiveScan256(run, sw, &tot);
+    for (int b = b0; b < b1; ++b)
     {
-        int b      = b0 + threadIdx.x;
-        uint32_t v = b < numBlocks ? sums[b] : 0, tot;
-        uint32_t ex = blockExclusiveScan256(v, sw, &tot);
-        if (b < numBlocks) sums[b] = carry + ex;
-        carry += tot;
+        const uint32_t v = sums[b];
+        sums[b] = carry;
+        carry += v;
     }
-    if (threadIdx.x == 0) sums[numBlocks] = carry;
+    if (threadIdx.x == 0) sums[numBlocks] = tot;
 }
 
 // leaves of the tile starting at particle t0, written from position `base` on
