@@ -226,6 +226,11 @@ typedef struct
     int soft_mac;         /* 0: the reference's MAC (macs.hpp:64-82). 1: also open every cell nearer than
                            * its radius plus twice the largest softening, so that no softened pair is
                            * approximated by a multipole (resident systems; costs a softening-range read-back) */
+    float group_split_tol; /* resident systems: 0 fixed groups of group_size (computeFixedGroups, as the
+                            * single-GPU demo, ryoanji/test/demo.cu:180); > 0 spatial groups, computeGroupSplits
+                            * with this tolerance factor, the particles' h and extent ratio 8, as
+                            * MultipoleHolder::computeSpatialGroups (multipole_holder.cu:36-53, factor 2);
+                            * needs group_size 64 (one group-count read-back) */
 } ryo_params;
 
 void ryo_default_params(ryo_params* p);

@@ -46,6 +46,7 @@ class Params(C.Structure):
         ("ewald_settings", C.c_double * 5),
         ("init_node", C.c_int),
         ("soft_mac", C.c_int),
+        ("group_split_tol", C.c_float),
     ]
 
 

@@ -192,6 +192,7 @@ int uploadTables(ryo_batch* b, const ryo_params* prm, const float* theta, const 
     if (prm->num_shells != 0 || prm->ewald != 0)
         return setError(ctx, RYO_ERR_UNSUPPORTED, "batch: open boundaries only");
     if (prm->soft_mac != 0) return setError(ctx, RYO_ERR_UNSUPPORTED, "batch: the reference MAC only");
+    if (prm->group_split_tol != 0.0f) return setError(ctx, RYO_ERR_UNSUPPORTED, "batch: fixed target groups only");
     std::vector<uint32_t> tiles(S), ngroups(S);
     std::vector<KeyBox> kb(S);
     std::vector<MacBox> mb(S);

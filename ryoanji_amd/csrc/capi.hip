@@ -401,8 +401,9 @@ void ryo_default_params(ryo_params* p)
     const double def[5] = {1, 2.6, 2.8, 2.0, 3.0e-3};
     for (int k = 0; k < 5; ++k)
         p->ewald_settings[k] = def[k];
-    p->init_node = 0;
-    p->soft_mac  = 0;
+    p->init_node       = 0;
+    p->soft_mac        = 0;
+    p->group_split_tol = 0.0f;
 }
 
 int ryo_system_create(ryo_ctx* h, size_t capacity, ryo_system** out)
@@ -572,13 +573,32 @@ int ryo_system_build(ryo_system* s, const ryo_params* prm)
     mark(s, 6);
     int gsz = prm->group_size > 0 ? prm->group_size : 64;
     if (gsz > 64) return setError(ctx, RYO_ERR_ARG, "system: group size must be <= 64");
+    if (!(prm->group_split_tol >= 0.0f))
+        return setError(ctx, RYO_ERR_ARG, "system: group_split_tol must be >= 0");
+    const bool spatial = prm->group_split_tol > 0.0f;
+    if (spatial && gsz != 64)
+        return setError(ctx, RYO_ERR_UNSUPPORTED, "system: spatial groups split windows of 64 targets");
     size_t G = (n + gsz - 1) / gsz;
-    if (G + 1 > s->groupCap)
+    for (int attempt = 0;; ++attempt)
     {
-        if ((rc = devAlloc(s, &s->groups, G + 1 + G / 4))) return rc;
-        s->groupCap = G + 1 + G / 4;
+        if (G + 1 > s->groupCap)
+        {
+            s->groupCap = 0;
+            if ((rc = devAlloc(s, &s->groups, G + 1 + G / 4))) return rc;
+            s->groupCap = G + 1 + G / 4;
+        }
+        if (!spatial)
+        {
+            rc = fixedGroups(ctx, 0, uint32_t(n), unsigned(gsz), s->groups, &s->G);
+            break;
+        }
+        // the split count is known only after the split kernel: grow and split again when it exceeds capacity
+        rc = groupSplits(ctx, 0, uint32_t(n), A[0], A[1], A[2], A[4], s->leaves, L, s->layout, s->limits, 64u,
+                         prm->group_split_tol, 8.0f, s->groups, s->groupCap, &s->G);
+        if (rc != RYO_ERR_CAPACITY || attempt > 0) break;
+        G = size_t(s->G);
     }
-    if ((rc = fixedGroups(ctx, 0, uint32_t(n), unsigned(gsz), s->groups, &s->G))) return rc;
+    if (rc) return rc;
     if (ctx->timing)
     {
         RYO_HIP(ctx, hipEventSynchronize(s->ev[6]));

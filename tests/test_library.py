@@ -43,3 +43,16 @@ def test_library_is_built_for_gfx950():
 
 def test_sort_temp_bytes_is_host_only():
     assert R.lib().ryo_sort_temp_bytes(1 << 20) > 0
+
+
+def test_params_mirror_the_header():
+    """ryoanji_amd.Params lists ryo_params' fields in the header's order, and ryo_default_params (host only) fills
+    the reference demo's settings"""
+    src = re.sub(r"/\*.*?\*/", "", open(os.path.join(ROOT, "include", "ryoanji_amd.h")).read(), flags=re.S)
+    body = re.search(r"typedef struct\s*\{([^}]*)\}\s*ryo_params;", src).group(1)
+    fields = [re.match(r"\s*[\w\s]+?\b(\w+)\s*(?:\[\d+\])?\s*$", d).group(1) for d in body.split(";") if d.strip()]
+    assert fields == [f[0] for f in R.Params._fields_]
+    p = R.default_params()
+    assert (p.bucket_size, p.theta, p.G, p.group_size, p.num_shells, p.ewald) == (64, 0.5, 1.0, 64, 0, 0)
+    assert list(p.ewald_settings) == [1, 2.6, 2.8, 2.0, 3.0e-3]
+    assert (p.init_node, p.soft_mac, p.group_split_tol) == (0, 0, 0.0)
