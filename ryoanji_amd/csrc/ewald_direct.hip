@@ -16,6 +16,8 @@ namespace ryo
 
 constexpr int kMaxH = 343;
 constexpr double kTwoOverSqrtPi = 1.1283791670955126;
+constexpr int kMaxCullShells = 8; // image rows are culled and exponentials factored for up to 8 shells
+constexpr int kMaxRows       = (2 * kMaxCullShells + 1) * (2 * kMaxCullShells + 1);
 
 // the root quadrupole in the form its evaluation uses: M, the traceless components plus trace over 3, and half
 // the trace (ewald.hpp:97-108)
@@ -33,7 +35,29 @@ struct EwaldTable
     double L, lCut, alphaScale, smallR, hCut2;
     double kPotSign; // -1: the reference's k-space potential (ewald.hpp:320); +1: consistent with the acceleration
     double h[kMaxH][5]; // {2 pi h / L (3), hfac_cos, hfac_sin}
+    // real-space images: row (ix, iy) evaluates iz in [rowLo, rowHi] (empty when rowLo > rowHi); every image
+    // outside can be neither an inner replica nor within lCut of any particle of the box
+    int cullRows;
+    int8_t rowLo[kMaxRows], rowHi[kMaxRows];
+    // exp(-a^2 |r + nL|^2) = exp(-a^2 |r|^2) prod_d exp(-2 a^2 L r_d)^n_d exp(-a^2 L^2 n_d^2): expLat[i] is
+    // exp(-a^2 L^2 i^2); used when the factors stay far from overflow (productExp)
+    int productExp;
+    double expLat[kMaxCullShells + 1];
+    // k space: row (hx, hy) holds the vectors hz in [-kRowHz, kRowHz] (none when -1), the origin excluded
+    int8_t kRowHz[49];
 };
+
+// 1/sqrt(x) for positive normal x: v_rsq_f64 (relative error below 2^-22) and two Newton steps, within about
+// 1 ulp
+__device__ inline double rsqrt64(double x)
+{
+    double y       = __builtin_amdgcn_rsq(x);
+    const double h = 0.5 * x;
+    double e       = fma(-h * y, y, 0.5);
+    y              = fma(y, e, y);
+    e              = fma(-h * y, y, 0.5);
+    return fma(y, e, y);
+}
 
 // gamma-weighted quadrupole evaluation (ewaldEvalMultipoleComplete, ewald.hpp:90-115)
 __host__ __device__ inline void ewaldEval(double* pa, const double* r, const double* g, const double* mp)
@@ -51,39 +75,36 @@ __host__ __device__ inline void ewaldEval(double* pa, const double* r, const dou
         pa[1 + d] += g[2] * Qr[d] - r[d] * f;
 }
 
-// erfcx(x) = exp(x^2) erfc(x) for x >= 0: on [0, 8] the degree-11 piece of x's interval of width 1/4
-// (erfcx_table.hpp, within 2.5e-16 of erfcx), beyond 8 the first three terms of the asymptotic series (there
-// erfc(x) < 1.2e-29, so its relative accuracy of 1e-5 is immaterial). The intervals present among the active
-// lanes are evaluated one at a time with wave-uniform coefficients (scalar loads); a wave's targets are SFC
-// neighbours that see an image at nearly the same distance, so there is usually one.
-__device__ inline double erfcxPos(double x)
+// erfcx(x) = exp(x^2) erfc(x) for x >= 0 (invX = 1/x): on [0, 8] the degree-11 piece of x's interval of width
+// 1/4 (erfcx_table.hpp, within 2.5e-16 of erfcx), beyond 8 the first three terms of the asymptotic series (there
+// erfc(x) < 1.2e-29, so its relative accuracy of 1e-5 is immaterial). Each lane reads its interval's
+// coefficients from the copy of the table in LDS (c2: pairs of coefficients, constant term first); a wave's
+// targets are SFC neighbours that see an image at nearly the same distance, so the reads are mostly broadcasts.
+constexpr int kErfcxPairs = kErfcxTerms / 2;
+static_assert(kErfcxTerms % 2 == 0, "coefficients are read in pairs");
+
+__device__ inline double erfcxLds(double x, double invX, const double2* __restrict__ c2)
 {
-    const int j = x < 8.0 ? int(x * 4.0) : kErfcxIntervals;
-    double r    = 0;
-    for (uint64_t todo = __ballot(true); todo;)
+    const bool far   = !(x < 8.0);
+    const int j      = far ? kErfcxIntervals - 1 : int(x * 4.0);
+    const double2* c = c2 + j * kErfcxPairs;
+    double2 p[kErfcxPairs];
+    for (int k = 0; k < kErfcxPairs; ++k)
+        p[k] = c[k];
+    const double t = (x - (j + 0.5) * 0.25) * 8.0;
+    double v       = p[kErfcxPairs - 1].y;
+    v              = fma(v, t, p[kErfcxPairs - 1].x);
+    for (int k = kErfcxPairs - 2; k >= 0; --k)
     {
-        const int ju    = __builtin_amdgcn_readlane(j, __builtin_ctzll(todo));
-        const bool mine = j == ju;
-        if (mine)
-        {
-            if (ju == kErfcxIntervals)
-            {
-                const double u = 1.0 / (x * x);
-                r = (0.5 * kTwoOverSqrtPi / x) * (1.0 + u * (-0.5 + 0.75 * u));
-            }
-            else
-            {
-                const double* c = kErfcx[ju];
-                const double t  = (x - (ju + 0.5) * 0.25) * 8.0;
-                double v        = c[kErfcxTerms - 1];
-                for (int k = kErfcxTerms - 2; k >= 0; --k)
-                    v = fma(v, t, c[k]);
-                r = v;
-            }
-        }
-        todo &= ~__ballot(mine);
+        v = fma(v, t, p[k].y);
+        v = fma(v, t, p[k].x);
     }
-    return r;
+    if (far)
+    {
+        const double u = invX * invX;
+        v = (0.5 * kTwoOverSqrtPi * invX) * (1.0 + u * (-0.5 + 0.75 * u));
+    }
+    return v;
 }
 
 // exp(i j theta) for j in [-3, 3] from the powers p1 = exp(i theta), p2, p3 (j is wave-uniform)
@@ -107,6 +128,45 @@ __device__ inline void ewaldEvalQ(double* pa, const double* r, const double* g, 
     double f = g[1] * q.M - g[2] * q.Qtr + g[3] * rQr;
     for (int d = 0; d < 3; ++d)
         pa[1 + d] += g[2] * Qr[d] - r[d] * f;
+}
+
+// image rows and the exponential factorization for particles in the box `limits` (with a margin of 5 % of L)
+void ewaldInitCulling(EwaldTable& t, const float* limits)
+{
+    const int ns = t.numEwaldShells, nr = t.numReplicaShells, w = 2 * ns + 1;
+    const double L = t.L;
+    t.cullRows = ns <= kMaxCullShells;
+    const double a2 = t.alphaScale * t.alphaScale;
+    t.productExp = t.cullRows && 2.0 * a2 * ns * 1.1 <= 200.0 && a2 * ns * ns <= 600.0;
+    for (int i = 0; i <= kMaxCullShells; ++i)
+        t.expLat[i] = i <= ns ? std::exp(-a2 * i * i) : 0.0;
+    if (!t.cullRows) return;
+    double lo[3], hi[3];
+    for (int d = 0; d < 3; ++d)
+    {
+        lo[d] = double(limits[2 * d]) - t.center[d] - 0.05 * L;
+        hi[d] = double(limits[2 * d + 1]) - t.center[d] + 0.05 * L;
+    }
+    // smallest |r_d + i L| over r_d in [lo, hi]
+    auto dmin = [&](int d, int i) {
+        const double a = lo[d] + i * L, b = hi[d] + i * L;
+        return (a <= 0 && 0 <= b) ? 0.0 : std::min(std::fabs(a), std::fabs(b));
+    };
+    const double cut2 = t.lCut * t.lCut * L * L * (1 + 1e-9);
+    for (int ix = -ns; ix <= ns; ++ix)
+        for (int iy = -ns; iy <= ns; ++iy)
+        {
+            const int row = (ix + ns) * w + (iy + ns);
+            const double dx = dmin(0, ix), dy = dmin(1, iy);
+            int l = ns + 1, h = -ns - 1;
+            for (int iz = -ns; iz <= ns; ++iz)
+            {
+                const bool inner = std::abs(ix) <= nr && std::abs(iy) <= nr && std::abs(iz) <= nr;
+                const double dz  = dmin(2, iz);
+                if (inner || dx * dx + dy * dy + dz * dz <= cut2) l = std::min(l, iz), h = std::max(h, iz);
+            }
+            t.rowLo[row] = int8_t(l), t.rowHi[row] = int8_t(h);
+        }
 }
 
 void ewaldInitTable(EwaldTable& t, const double* Mroot, const double* center, int numReplicaShells, double L,
@@ -153,6 +213,14 @@ void ewaldInitTable(EwaldTable& t, const double* Mroot, const double* center, in
                 e[3] = a[0];
                 e[4] = b[0];
             }
+    for (int hx = -hReps; hx <= hReps; hx++)
+        for (int hy = -hReps; hy <= hReps; hy++)
+        {
+            int top = -1;
+            for (int hz = 0; hz <= hReps; ++hz)
+                if (double(hx * hx + hy * hy + hz * hz) <= hCut * hCut) top = hz;
+            t.kRowHz[(hx + hReps) * (2 * hReps + 1) + (hy + hReps)] = int8_t(top);
+        }
 }
 
 __global__ __launch_bounds__(256, 4) void ewaldKernel(const EwaldTable* __restrict__ tab, uint32_t first, uint32_t last,
@@ -164,12 +232,16 @@ __global__ __launch_bounds__(256, 4) void ewaldKernel(const EwaldTable* __restri
     // the table is read with wave-uniform addresses (scalar loads), so the quadrupole and the k-space
     // coefficients live in scalar registers
     __shared__ double sPot[256];
-    const int numH  = tab->numH;
+    __shared__ double2 sErfcx[kErfcxIntervals * kErfcxPairs];
+    for (int k = threadIdx.x; k < kErfcxIntervals * kErfcxPairs; k += blockDim.x)
+        sErfcx[k] = make_double2(kErfcx[k / kErfcxPairs][2 * (k % kErfcxPairs)],
+                                 kErfcx[k / kErfcxPairs][2 * (k % kErfcxPairs) + 1]);
+    __syncthreads();
     const EwaldQ q  = tab->q;
 
     const double L = tab->L, lCut = tab->lCut;
     const double lCut2  = lCut * lCut * L * L;
-    const double alpha  = tab->alphaScale / L, alpha2 = alpha * alpha;
+    const double alpha  = tab->alphaScale / L, alpha2 = alpha * alpha, invAlpha = L / tab->alphaScale;
     const double k1     = M_PI / (alpha2 * L * L * L);
     const double M      = tab->Mroot[0];
     const double ka     = 2.0 * alpha / sqrt(M_PI);
@@ -183,11 +255,33 @@ __global__ __launch_bounds__(256, 4) void ewaldKernel(const EwaldTable* __restri
         double px = x[i], py = y[i], pz = z[i];
         double r[3] = {px - tab->center[0], py - tab->center[1], pz - tab->center[2]};
         double pa[4] = {k1 * M, 0, 0, 0};
-        for (int ix = -ns; ix <= ns; ix++)
-            for (int iy = -ns; iy <= ns; iy++)
-                for (int iz = -ns; iz <= ns; iz++)
+        const bool cull = tab->cullRows, prodExp = tab->productExp;
+        // the exponential factors of the images (productExp): px_ = exp(-2 a^2 L r_x)^ix along the loop
+        double e0 = 0, fx = 0, fy = 0, fz = 0, px_ = 0, py0 = 0, pz0 = 0;
+        if (prodExp)
+        {
+            const double c = 2.0 * alpha2 * L;
+            e0  = exp(-alpha2 * (r[0] * r[0] + r[1] * r[1] + r[2] * r[2]));
+            fx  = exp(-c * r[0]), fy = exp(-c * r[1]), fz = exp(-c * r[2]);
+            px_ = exp(c * ns * r[0]), py0 = exp(c * ns * r[1]), pz0 = exp(c * ns * r[2]);
+        }
+        for (int ix = -ns; ix <= ns; ix++, px_ *= fx)
+        {
+            const double ex = prodExp ? e0 * px_ * tab->expLat[ix < 0 ? -ix : ix] : 0.0;
+            double py_      = py0;
+            for (int iy = -ns; iy <= ns; iy++, py_ *= fy)
+            {
+                const int row = (ix + ns) * (2 * ns + 1) + (iy + ns);
+                const int zlo = cull ? tab->rowLo[row] : -ns, zhi = cull ? tab->rowHi[row] : ns;
+                if (zlo > zhi) continue;
+                const double exy = prodExp ? ex * py_ * tab->expLat[iy < 0 ? -iy : iy] : 0.0;
+                double pz_       = pz0;
+                for (int iz = -ns; iz < zlo; ++iz)
+                    pz_ *= fz;
+                const bool innerRow = (-nr <= ix && ix <= nr) && (-nr <= iy && iy <= nr);
+                for (int iz = zlo; iz <= zhi; iz++, pz_ *= fz)
                 {
-                    bool inner = (-nr <= ix && ix <= nr) && (-nr <= iy && iy <= nr) && (-nr <= iz && iz <= nr);
+                    const bool inner = innerRow && (-nr <= iz && iz <= nr);
                     double R[3] = {r[0] + ix * L, r[1] + iy * L, r[2] + iz * L};
                     double R2   = R[0] * R[0] + R[1] * R[1] + R[2] * R[2];
                     if (R2 > lCut2 && !inner) continue;
@@ -209,12 +303,12 @@ __global__ __launch_bounds__(256, 4) void ewaldKernel(const EwaldTable* __restri
                     }
                     else
                     {
-                        double Rm = sqrt(R2), invR = 1.0 / Rm, invR2 = invR * invR;
-                        double e2 = exp(-R2 * alpha2);
+                        double invR = rsqrt64(R2), Rm = R2 * invR, invR2 = invR * invR;
+                        double e2 = prodExp ? exy * (pz_ * tab->expLat[iz < 0 ? -iz : iz]) : exp(-R2 * alpha2);
                         double a  = e2 * ka * invR2;
                         double an = 1.0;
                         // erfc(aR) = exp(-a^2 R^2) erfcx(aR), reusing the exponential above; -erf = erfc - 1
-                        double ec = e2 * erfcxPos(alpha * Rm);
+                        double ec = e2 * erfcxLds(alpha * Rm, invR * invAlpha, sErfcx);
                         double fn = inner ? ec - 1.0 : ec;
                         g[0] = fn * invR;
                         g[1] = g[0] * invR2 + a;
@@ -229,6 +323,8 @@ __global__ __launch_bounds__(256, 4) void ewaldKernel(const EwaldTable* __restri
                     }
                     ewaldEvalQ(pa, R, g, q);
                 }
+            }
+        }
         // k space: exp(i 2 pi h.r / L) for the table's lattice vectors, visited in the table's order (hx, hy, hz
         // ascending, 0 < |h|^2 <= hCut^2), as products of the axis phases exp(i 2 pi h_d r_d / L) (|h_d| <= 3)
         // instead of one sincos per vector
@@ -247,18 +343,18 @@ __global__ __launch_bounds__(256, 4) void ewaldKernel(const EwaldTable* __restri
             qx[2] = cmul(qx[1], qx[0]), qy[2] = cmul(qy[1], qy[0]), qz[2] = cmul(qz[1], qz[0]);
         }
         const int hr = tab->hReps;
-        const double hc2 = tab->hCut2;
         int k = 0;
         for (int hx = -hr; hx <= hr; ++hx)
         {
             const double2 ex = phasePow(hx, qx[0], qx[1], qx[2]);
             for (int hy = -hr; hy <= hr; ++hy)
             {
+                const int hzm = tab->kRowHz[(hx + hr) * (2 * hr + 1) + (hy + hr)];
+                if (hzm < 0) continue;
                 const double2 exy = cmul(ex, phasePow(hy, qy[0], qy[1], qy[2]));
-                for (int hz = -hr; hz <= hr; ++hz)
+                for (int hz = -hzm; hz <= hzm; ++hz)
                 {
-                    const int h2 = hx * hx + hy * hy + hz * hz;
-                    if (h2 == 0 || double(h2) > hc2 || k >= numH) continue;
+                    if (hx == 0 && hy == 0 && hz == 0) continue;
                     const double2 ph = cmul(exy, phasePow(hz, qz[0], qz[1], qz[2]));
                     const double* e  = tab->h[k++];
                     double sum = e[3] * ph.x + e[4] * ph.y;
@@ -317,6 +413,7 @@ int ewald(Context* ctx, const double* Mroot, const double* center, uint32_t firs
     ewaldInitTable(*t, Mroot, center, int(settings[0]), double(lx), settings[1], settings[2], settings[3],
                    settings[4]);
     t->kPotSign = consistentPotential ? 1.0 : -1.0;
+    ewaldInitCulling(*t, limits);
     if (t->numEwaldShells == 0 || last <= first)
     {
         delete t;

@@ -91,7 +91,7 @@ def test_direct_sum_matches_oracle(ctx):
     assert np.allclose(g1, o1, rtol=1e-11, atol=1e-13 * np.abs(o1).max())
 
 
-def _ewald_gpu(ctx, root, center, t, limits):
+def _ewald_gpu(ctx, root, center, t, limits, settings=O.EWALD_DEFAULT):
     import torch
 
     n = t["x"].size
@@ -99,18 +99,26 @@ def _ewald_gpu(ctx, root, center, t, limits):
     ug, ax, ay, az = (torch.zeros(n, dtype=torch.float32, device="cuda") for _ in range(4))
     tot = ctypes.c_double()
     ctx.call("ryo_ewald_f32", R.host_f64(root), R.host_f64(center), 0, n, *(R.ptr(a) for a in pos),
-             R.host_f32(limits), 1.0, R.host_f64(O.EWALD_DEFAULT), R.ptr(ug), R.ptr(ax), R.ptr(ay), R.ptr(az),
+             R.host_f32(limits), 1.0, R.host_f64(settings), R.ptr(ug), R.ptr(ax), R.ptr(ay), R.ptr(az),
              ctypes.byref(tot))
     return tot.value, host(ug), host(ax), host(ay), host(az)
 
 
-def test_ewald_matches_oracle(ctx):
-    s = W.uniform_cube(20000, seed=6, lo=0.0, hi=1.0, periodic=True)
+@pytest.mark.parametrize("settings,lo,hi", [
+    (O.EWALD_DEFAULT, 0.0, 1.0),
+    # a box that is not centred on the origin, larger real-space cut (4 image shells, culled rows)
+    ((1, 3.4, 2.8, 2.0, 3.0e-3), -0.7, 0.3),
+    # a large splitting parameter: the image exponentials are evaluated directly instead of factored
+    ((1, 2.6, 2.8, 7.0, 3.0e-3), 0.0, 1.0),
+    # 2 replica shells
+    ((2, 2.6, 2.8, 2.0, 3.0e-3), 0.0, 1.0)])
+def test_ewald_matches_oracle(ctx, settings, lo, hi):
+    s = W.uniform_cube(20000, seed=6, lo=lo, hi=hi, periodic=True)
     t = oracle_tree(s)
     c64, q64 = O.upsweep(t["x"], t["y"], t["z"], t["m"], t["leaves"], t["layout"], s.limits.astype(np.float64), 2.0,
                          precision="f64")
-    otot, oug, oax, oay, oaz = O.ewald(q64[0], c64[0, :3], t["x"], t["y"], t["z"], t["m"], 1.0)
-    gtot, gug, gax, gay, gaz = _ewald_gpu(ctx, q64[0], c64[0, :3], t, s.limits)
+    otot, oug, oax, oay, oaz = O.ewald(q64[0], c64[0, :3], t["x"], t["y"], t["z"], t["m"], 1.0, settings=settings)
+    gtot, gug, gax, gay, gaz = _ewald_gpu(ctx, q64[0], c64[0, :3], t, s.limits, settings)
     # both FP64 with the same formulas; the device stores float outputs
     e = rel_err(gax, gay, gaz, oax, oay, oaz)
     assert np.quantile(e, 0.99) < 1e-6
