@@ -36,11 +36,24 @@ __device__ __forceinline__ uint64_t spread3(unsigned v)
 
 // sfc.hpp:161-207: cell indices clamped to 2^bits - 1, then the Hilbert (mixed-dimension when bits differ) or
 // Morton key
+__constant__ HilbertLut kHilbertLutDev = detail::makeHilbertLut();
+
 template<bool Cubic>
 __global__ __launch_bounds__(256) void sfcKeysKernel(const float* __restrict__ x, const float* __restrict__ y,
                                                      const float* __restrict__ z, uint64_t* __restrict__ keys,
                                                      size_t n, KeyBox box, int kind, int skipRemoved)
 {
+    // cubic boxes: the Hilbert state tables in LDS (6.9 KB), read with per-lane indices
+    __shared__ HilbertLut lut;
+    if constexpr (Cubic)
+    {
+        static_assert(sizeof(HilbertLut) % 4 == 0, "tables are copied in words");
+        auto* dst       = reinterpret_cast<uint32_t*>(&lut);
+        const auto* src = reinterpret_cast<const uint32_t*>(&kHilbertLutDev);
+        for (int w = threadIdx.x; w < int(sizeof(HilbertLut) / 4); w += blockDim.x)
+            dst[w] = src[w];
+        __syncthreads();
+    }
     const int mx = int((1u << box.bits.b[0]) - 1), my = int((1u << box.bits.b[1]) - 1),
               mz = int((1u << box.bits.b[2]) - 1);
     for (size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += size_t(gridDim.x) * blockDim.x)
@@ -53,7 +66,7 @@ __global__ __launch_bounds__(256) void sfcKeysKernel(const float* __restrict__ x
         if (kind != 0)
             k = (spread3(ix) << 2) | (spread3(iy) << 1) | spread3(iz);
         else if constexpr (Cubic)
-            k = hilbert3D(ix, iy, iz, int(kMaxLevel));
+            k = hilbert3DLut(ix, iy, iz, lut.two, lut.one);
         else
             k = hilbertMixD(ix, iy, iz, box.bits);
         keys[i] = k;

@@ -28,7 +28,7 @@ __host__ __device__ inline bool isCubic(const Bits3& a)
 
 // 3D Hilbert key of `order` levels (hilbert.hpp:43-94): octant digit through the Morton->Hilbert map
 // {0,1,3,2,7,6,4,5}, then the reflections and the rotation/swap that orient the next level
-__host__ __device__ inline uint64_t hilbert3D(unsigned px, unsigned py, unsigned pz, int order)
+__host__ __device__ constexpr uint64_t hilbert3D(unsigned px, unsigned py, unsigned pz, int order)
 {
     constexpr uint32_t toHilbert = (0u << 0) | (1u << 3) | (3u << 6) | (2u << 9) | (7u << 12) | (6u << 15) |
                                    (4u << 18) | (5u << 21); // 3-bit table packed in a word
@@ -52,6 +52,112 @@ __host__ __device__ inline uint64_t hilbert3D(unsigned px, unsigned py, unsigned
     }
     return key;
 }
+
+// The same 21-level curve as a state machine. hilbert3D complements and permutes the remaining coordinate bits
+// after every level; their composition is a signed permutation of the axes, so the coordinates it reads at a
+// level are raw bits of the input permuted and complemented by one of 48 states (permutation index * 8 + three
+// complement flags). kHilbertLut.two[state][raw bits of two levels] holds both octal digits and the next state,
+// so a 21-level key takes one single-level step and ten two-level table reads instead of 21 transform steps.
+struct HilbertLut
+{
+    uint16_t two[48][64]; // index (x1 x0 y1 y0 z1 z0), entry (next state << 6) | (digit1 << 3) | digit0
+    uint16_t one[48][8];  // index (x y z), entry (next state << 3) | digit
+};
+
+namespace detail
+{
+constexpr int kPerms[6][3] = {{0, 1, 2}, {0, 2, 1}, {1, 0, 2}, {1, 2, 0}, {2, 0, 1}, {2, 1, 0}};
+
+constexpr int permIndex(int a, int b)
+{
+    for (int k = 0; k < 6; ++k)
+        if (kPerms[k][0] == a && kPerms[k][1] == b) return k;
+    return -1;
+}
+
+// one level of hilbert3D in state form: raw bits (rx, ry, rz) -> digit and next state
+constexpr unsigned hilbertStep(int state, unsigned rx, unsigned ry, unsigned rz, int& next)
+{
+    constexpr uint32_t toHilbert = (0u << 0) | (1u << 3) | (3u << 6) | (2u << 9) | (7u << 12) | (6u << 15) |
+                                   (4u << 18) | (5u << 21);
+    const int* p    = kPerms[state >> 3];
+    unsigned f[3]   = {unsigned(state >> 2) & 1u, unsigned(state >> 1) & 1u, unsigned(state) & 1u};
+    unsigned raw[3] = {rx, ry, rz};
+    unsigned xi = raw[p[0]] ^ f[0], yi = raw[p[1]] ^ f[1], zi = raw[p[2]] ^ f[2];
+    unsigned digit = (toHilbert >> (3 * ((xi << 2) | (yi << 1) | zi))) & 7u;
+    unsigned nyi = yi ^ 1u, nzi = zi ^ 1u;
+    f[0] ^= xi & (nyi | zi);
+    f[1] ^= (xi & (yi | zi)) | (yi & nzi);
+    f[2] ^= (xi & nyi & nzi) | (yi & nzi);
+    int q[3]        = {p[0], p[1], p[2]};
+    unsigned g[3]   = {f[0], f[1], f[2]};
+    if (zi) // coordinates (x, y, z) <- (y, z, x)
+        q[0] = p[1], q[1] = p[2], q[2] = p[0], g[0] = f[1], g[1] = f[2], g[2] = f[0];
+    else if (nyi) // swap x and z
+        q[0] = p[2], q[2] = p[0], g[0] = f[2], g[2] = f[0];
+    next = permIndex(q[0], q[1]) * 8 + int((g[0] << 2) | (g[1] << 1) | g[2]);
+    return digit;
+}
+
+constexpr HilbertLut makeHilbertLut()
+{
+    HilbertLut t{};
+    for (int s = 0; s < 48; ++s)
+    {
+        for (unsigned o = 0; o < 8; ++o)
+        {
+            int n = 0;
+            unsigned d = hilbertStep(s, o >> 2, (o >> 1) & 1u, o & 1u, n);
+            t.one[s][o] = uint16_t((n << 3) | d);
+        }
+        for (unsigned i = 0; i < 64; ++i)
+        {
+            int s1 = 0, s2 = 0;
+            unsigned d1 = hilbertStep(s, (i >> 5) & 1u, (i >> 3) & 1u, (i >> 1) & 1u, s1);
+            unsigned d0 = hilbertStep(s1, (i >> 4) & 1u, (i >> 2) & 1u, i & 1u, s2);
+            t.two[s][i] = uint16_t((s2 << 6) | (d1 << 3) | d0);
+        }
+    }
+    return t;
+}
+} // namespace detail
+
+constexpr HilbertLut kHilbertLut = detail::makeHilbertLut();
+
+// 21-level 3D Hilbert key from the tables (equal to hilbert3D(px, py, pz, 21))
+__host__ __device__ constexpr uint64_t hilbert3DLut(unsigned px, unsigned py, unsigned pz, const uint16_t (*two)[64],
+                                                    const uint16_t (*one)[8])
+{
+    unsigned e   = one[0][((px >> 18) & 4u) | ((py >> 19) & 2u) | ((pz >> 20) & 1u)];
+    uint64_t key = e & 7u;
+    unsigned s   = e >> 3;
+    for (int level = 18; level >= 0; level -= 2)
+    {
+        const unsigned i = (((px >> level) & 3u) << 4) | (((py >> level) & 3u) << 2) | ((pz >> level) & 3u);
+        e   = two[s][i];
+        key = (key << 6) | (e & 63u);
+        s   = e >> 6;
+    }
+    return key;
+}
+
+namespace detail
+{
+// compile-time check of the tables against hilbert3D on a pseudo-random sample of 21-bit coordinates
+constexpr bool hilbertLutMatches()
+{
+    uint64_t r = 0x9E3779B97F4A7C15ull;
+    for (int k = 0; k < 2000; ++k)
+    {
+        r ^= r << 13, r ^= r >> 7, r ^= r << 17;
+        const unsigned x = unsigned(r) & 0x1FFFFFu, y = unsigned(r >> 21) & 0x1FFFFFu, z = unsigned(r >> 42) & 0x1FFFFFu;
+        if (hilbert3DLut(x, y, z, kHilbertLut.two, kHilbertLut.one) != hilbert3D(x, y, z, 21))
+            return false;
+    }
+    return true;
+}
+static_assert(hilbertLutMatches(), "Hilbert tables differ from hilbert3D");
+} // namespace detail
 
 // inverse of hilbert3D for `order` levels: corner coordinates of the cell with start key `key`
 __host__ __device__ inline void hilbert3DDecode(uint64_t key, int order, unsigned& ox, unsigned& oy, unsigned& oz)
