@@ -15,7 +15,8 @@ size_t sortTempBytes(size_t n);
 int sortPairs(Context*, uint64_t*, int32_t*, size_t, uint64_t*, int32_t*, void*, size_t);
 int sequence(Context*, int32_t*, size_t, int32_t);
 int gather(Context*, const int32_t*, size_t, int, const float* const*, float* const*);
-int gatherPacked6(Context*, const int32_t*, size_t, const float* const*, float* const*);
+int sfcKeysPacked(Context*, const float* const*, uint64_t*, int32_t*, size_t, const float*, float4**);
+int gatherRecords(Context*, const int32_t*, size_t, const float4*, float* const*);
 int buildLeaves(Context*, const uint64_t*, size_t, unsigned, uint64_t*, uint32_t*, uint32_t*, size_t, int*);
 int linkOctree(Context*, const uint64_t*, int, uint64_t*, int32_t*, int32_t*, int32_t*, int32_t*, int32_t*, int32_t*);
 int upsweep(Context*, const float*, const float*, const float*, const float*, const uint32_t*, int, const uint64_t*,
@@ -516,27 +517,28 @@ int ryo_system_build(ryo_system* s, const ryo_params* prm)
     float** B = s->a[s->cur ^ 1];
     int rc;
     s->haveAcc = false;
+    // x, y, z, m, h and the original index always; velocities only once the system integrates
+    const int idx[9] = {0, 1, 2, 3, 4, 8, 5, 6, 7};
+    const int num    = s->integrating ? 9 : 6;
+    const float* in[9];
+    float* out[9];
+    for (int k = 0; k < num; ++k)
+        in[k] = A[idx[k]], out[k] = B[idx[k]];
+    // arrays in the caller's order (a fresh load): the keys pass also writes the identity permutation and one
+    // 32-byte record per particle, which the gather after the sort reads with one cache line per particle; a
+    // time-stepping system is already close to SFC order, where the plain gather of the arrays is cheaper
+    const bool packed = s->freshOrder && num == 6;
+    float4* rec = nullptr;
     mark(s, 0);
-    if ((rc = sfcKeys(ctx, A[0], A[1], A[2], s->keys, n, s->limits, 0, 0))) return rc;
+    if ((rc = packed ? sfcKeysPacked(ctx, in, s->keys, s->perm, n, s->limits, &rec)
+                     : sfcKeys(ctx, A[0], A[1], A[2], s->keys, n, s->limits, 0, 0)))
+        return rc;
     mark(s, 1);
-    if ((rc = sequence(ctx, s->perm, n, 0))) return rc;
+    if (!packed && (rc = sequence(ctx, s->perm, n, 0))) return rc;
     if ((rc = sortPairs(ctx, s->keys, s->perm, n, s->keyBuf, s->permBuf, s->sortTmp, s->sortTmpBytes))) return rc;
     mark(s, 2);
-    {
-        // x, y, z, m, h and the original index always; velocities only once the system integrates
-        const int idx[9] = {0, 1, 2, 3, 4, 8, 5, 6, 7};
-        const int num    = s->integrating ? 9 : 6;
-        const float* in[9];
-        float* out[9];
-        for (int k = 0; k < num; ++k)
-            in[k] = A[idx[k]], out[k] = B[idx[k]];
-        // arrays in the caller's order (a fresh load): one 32-byte record per particle; a time-stepping system is
-        // already close to SFC order, where the plain gather is cheaper
-        const bool packed = s->freshOrder && num == 6;
-        if ((rc = packed ? gatherPacked6(ctx, s->perm, n, in, out) : gather(ctx, s->perm, n, num, in, out)))
-            return rc;
-        s->freshOrder = false;
-    }
+    if ((rc = packed ? gatherRecords(ctx, s->perm, n, rec, out) : gather(ctx, s->perm, n, num, in, out))) return rc;
+    s->freshOrder = false;
     s->cur ^= 1;
     A = s->a[s->cur];
     mark(s, 3);

@@ -38,10 +38,19 @@ __device__ __forceinline__ uint64_t spread3(unsigned v)
 // Morton key
 __constant__ HilbertLut kHilbertLutDev = detail::makeHilbertLut();
 
-template<bool Cubic>
+// a freshly loaded system also writes, in the same pass, the identity permutation the sort starts from and one
+// 32-byte record {x, y, z, m}, {h, o, 0, 0} per particle for the permutation gather
+struct PackOut
+{
+    const float *m, *h, *o;
+    int32_t* perm;
+    float4* rec;
+};
+
+template<bool Cubic, bool Pack>
 __global__ __launch_bounds__(256) void sfcKeysKernel(const float* __restrict__ x, const float* __restrict__ y,
                                                      const float* __restrict__ z, uint64_t* __restrict__ keys,
-                                                     size_t n, KeyBox box, int kind, int skipRemoved)
+                                                     size_t n, KeyBox box, int kind, int skipRemoved, PackOut pk)
 {
     // cubic boxes: the Hilbert state tables in LDS (6.9 KB), read with per-lane indices
     __shared__ HilbertLut lut;
@@ -58,10 +67,11 @@ __global__ __launch_bounds__(256) void sfcKeysKernel(const float* __restrict__ x
               mz = int((1u << box.bits.b[2]) - 1);
     for (size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += size_t(gridDim.x) * blockDim.x)
     {
-        if (skipRemoved && keys[i] == kKeyEnd) continue;
-        int ix = min(cellIndex(x[i], box.lo[0], box.m[0]), mx);
-        int iy = min(cellIndex(y[i], box.lo[1], box.m[1]), my);
-        int iz = min(cellIndex(z[i], box.lo[2], box.m[2]), mz);
+        if (!Pack && skipRemoved && keys[i] == kKeyEnd) continue;
+        const float px = x[i], py = y[i], pz = z[i];
+        int ix = min(cellIndex(px, box.lo[0], box.m[0]), mx);
+        int iy = min(cellIndex(py, box.lo[1], box.m[1]), my);
+        int iz = min(cellIndex(pz, box.lo[2], box.m[2]), mz);
         uint64_t k;
         if (kind != 0)
             k = (spread3(ix) << 2) | (spread3(iy) << 1) | spread3(iz);
@@ -70,6 +80,12 @@ __global__ __launch_bounds__(256) void sfcKeysKernel(const float* __restrict__ x
         else
             k = hilbertMixD(ix, iy, iz, box.bits);
         keys[i] = k;
+        if constexpr (Pack)
+        {
+            pk.perm[i]       = int32_t(i);
+            pk.rec[2 * i]     = make_float4(px, py, pz, pk.m[i]);
+            pk.rec[2 * i + 1] = make_float4(pk.h[i], pk.o[i], 0.0f, 0.0f);
+        }
     }
 }
 
@@ -333,20 +349,8 @@ __global__ __launch_bounds__(256) void gatherKernel(const int32_t* __restrict__ 
     }
 }
 
-// the six arrays of a freshly loaded system packed into one 32-byte record per particle, so that the gather of a
-// random permutation fetches one line per particle instead of one per array and particle
-__global__ __launch_bounds__(256) void packRecordsKernel(const float* __restrict__ x, const float* __restrict__ y,
-                                                         const float* __restrict__ z, const float* __restrict__ m,
-                                                         const float* __restrict__ h, const float* __restrict__ o,
-                                                         size_t n, float4* __restrict__ rec)
-{
-    for (size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += size_t(gridDim.x) * blockDim.x)
-    {
-        rec[2 * i]     = make_float4(x[i], y[i], z[i], m[i]);
-        rec[2 * i + 1] = make_float4(h[i], o[i], 0.0f, 0.0f);
-    }
-}
-
+// the six arrays of a freshly loaded system from the 32-byte records the keys pass wrote (sfcKeysPacked): the
+// gather of a random permutation fetches one line per particle instead of one per array and particle
 __global__ __launch_bounds__(256) void gatherRecordsKernel(const int32_t* __restrict__ perm, size_t n,
                                                            const float4* __restrict__ rec, float* __restrict__ x,
                                                            float* __restrict__ y, float* __restrict__ z,
@@ -520,11 +524,8 @@ int segmentedSortPairs(Context* ctx, uint64_t* keys, int32_t* vals, size_t n, co
     return RYO_OK;
 }
 
-int sfcKeys(Context* ctx, const float* x, const float* y, const float* z, uint64_t* keys, size_t n,
-            const float* limits, int kind, int skipRemoved)
+static int makeKeyBox(Context* ctx, const float* limits, KeyBox& b)
 {
-    if (n == 0) return RYO_OK;
-    KeyBox b;
     b.bits = boxBits(limits); // box.hpp:160-198: an axis shorter than 0.75 x the longest one gets fewer bits
     for (int d = 0; d < 3; ++d)
     {
@@ -534,12 +535,44 @@ int sfcKeys(Context* ctx, const float* x, const float* y, const float* z, uint64
         b.lo[d]  = lo;
         b.m[d]   = float(1u << b.bits.b[d]) * il;
     }
+    return RYO_OK;
+}
+
+int sfcKeys(Context* ctx, const float* x, const float* y, const float* z, uint64_t* keys, size_t n,
+            const float* limits, int kind, int skipRemoved)
+{
+    if (n == 0) return RYO_OK;
+    KeyBox b;
+    if (int rc = makeKeyBox(ctx, limits, b)) return rc;
     const unsigned grid = gridFor(n, 256, ctx->numCUs * 16);
     if (isCubic(b.bits))
-        sfcKeysKernel<true><<<grid, 256, 0, ctx->stream>>>(x, y, z, keys, n, b, kind, skipRemoved);
+        sfcKeysKernel<true, false><<<grid, 256, 0, ctx->stream>>>(x, y, z, keys, n, b, kind, skipRemoved, PackOut{});
     else
-        sfcKeysKernel<false><<<grid, 256, 0, ctx->stream>>>(x, y, z, keys, n, b, kind, skipRemoved);
+        sfcKeysKernel<false, false><<<grid, 256, 0, ctx->stream>>>(x, y, z, keys, n, b, kind, skipRemoved,
+                                                                   PackOut{});
     RYO_LAUNCH_CHECK(ctx);
+    return RYO_OK;
+}
+
+// Hilbert keys of a freshly loaded system together with the identity permutation and the gather records of
+// in = {x, y, z, m, h, o} (the records stay in the context's pack scratch until gatherRecords)
+int sfcKeysPacked(Context* ctx, const float* const* in, uint64_t* keys, int32_t* perm, size_t n,
+                  const float* limits, float4** rec)
+{
+    *rec = nullptr;
+    if (n == 0) return RYO_OK;
+    KeyBox b;
+    if (int rc = makeKeyBox(ctx, limits, b)) return rc;
+    auto* r = static_cast<float4*>(ctx->getScratch(kSlotPack, 32 * n));
+    if (!r) return RYO_ERR_HIP;
+    const PackOut pk{in[3], in[4], in[5], perm, r};
+    const unsigned grid = gridFor(n, 256, ctx->numCUs * 16);
+    if (isCubic(b.bits))
+        sfcKeysKernel<true, true><<<grid, 256, 0, ctx->stream>>>(in[0], in[1], in[2], keys, n, b, 0, 0, pk);
+    else
+        sfcKeysKernel<false, true><<<grid, 256, 0, ctx->stream>>>(in[0], in[1], in[2], keys, n, b, 0, 0, pk);
+    RYO_LAUNCH_CHECK(ctx);
+    *rec = r;
     return RYO_OK;
 }
 
@@ -584,15 +617,12 @@ int sequence(Context* ctx, int32_t* out, size_t n, int32_t start)
 
 /* out[k][i] = in[k][perm[i]] for the six arrays x, y, z, m, h, order of a system in random order, through
  * 32-byte records (scratch: 32 n bytes) */
-int gatherPacked6(Context* ctx, const int32_t* perm, size_t n, const float* const* in, float* const* out)
+// out[k][i] = field k of record perm[i] (records from sfcKeysPacked)
+int gatherRecords(Context* ctx, const int32_t* perm, size_t n, const float4* rec, float* const* out)
 {
     if (n == 0) return RYO_OK;
-    auto* rec = static_cast<float4*>(ctx->getScratch(kSlotPack, 32 * n));
-    if (!rec) return RYO_ERR_HIP;
-    const unsigned grid = gridFor(n, 256, ctx->numCUs * 16);
-    packRecordsKernel<<<grid, 256, 0, ctx->stream>>>(in[0], in[1], in[2], in[3], in[4], in[5], n, rec);
-    gatherRecordsKernel<<<grid, 256, 0, ctx->stream>>>(perm, n, rec, out[0], out[1], out[2], out[3], out[4],
-                                                       out[5]);
+    gatherRecordsKernel<<<gridFor(n, 256, ctx->numCUs * 16), 256, 0, ctx->stream>>>(perm, n, rec, out[0], out[1],
+                                                                                     out[2], out[3], out[4], out[5]);
     RYO_LAUNCH_CHECK(ctx);
     return RYO_OK;
 }
