@@ -49,18 +49,22 @@ __device__ inline int spanCount(uint64_t a, uint64_t b)
     return c;
 }
 
+// LDS of one leaf tile for buckets up to MaxB (21 KB for MaxB = 64, so 7 blocks fit a CU; 45 KB for 1024)
+template<int MaxB>
 struct LeafTileSmem
 {
-    uint64_t keys[kLeafTile + 2 * kMaxBucket + 2];
-    int c[2][kLeafTile + kMaxBucket + 2];
-    uint64_t start[kLeafTile + 1]; // leaf start key of particles t0-1 .. t1-1
+    uint64_t keys[kLeafTile + 2 * MaxB + 2];
+    int8_t c[2][kLeafTile + MaxB + 2]; // common levels -1 .. 21
+    uint64_t start[kLeafTile + 1];     // leaf start key of particles t0-1 .. t1-1
     unsigned char level[kLeafTile + 1];
     uint32_t sw[4];
 };
+constexpr int kSmallBucket = 64;
 
 /* Per tile [t0, t1): leaf start/level of particles t0-1..t1-1 into s.start/s.level, and the number of
  * leaves each particle of the tile emits (returned for the first kLeafTile/256 items of each thread). */
-__device__ void leafTile(const uint64_t* __restrict__ k, long n, int B, long t0, LeafTileSmem& s, int em[4])
+template<int MaxB>
+__device__ void leafTile(const uint64_t* __restrict__ k, long n, int B, long t0, LeafTileSmem<MaxB>& s, int em[4])
 {
     const long t1    = min(t0 + long(kLeafTile), n);
     const long kbase = t0 - 1 - B; // s.keys[j - kbase] = k[j]
@@ -83,7 +87,7 @@ __device__ void leafTile(const uint64_t* __restrict__ k, long n, int B, long t0,
             int cp = commonPrefix(s.keys[j], s.keys[j + B]);
             v      = min(cp / 3, int(kMaxLevel));
         }
-        s.c[0][j] = v;
+        s.c[0][j] = int8_t(v);
     }
     __syncthreads();
     // doubling: c[P][j] = max(c[j .. j+P-1]) for the largest power of two P <= B+1
@@ -96,7 +100,7 @@ __device__ void leafTile(const uint64_t* __restrict__ k, long n, int B, long t0,
         {
             int a = s.c[cur][j];
             int b = (j + P < nc) ? s.c[cur][j + P] : -1;
-            s.c[cur ^ 1][j] = max(a, b);
+            s.c[cur ^ 1][j] = int8_t(max(a, b));
         }
         __syncthreads();
         cur ^= 1;
@@ -136,10 +140,11 @@ __device__ void leafTile(const uint64_t* __restrict__ k, long n, int B, long t0,
     }
 }
 
+template<int MaxB>
 __global__ __launch_bounds__(kLeafThreads) void leafCountKernel(const uint64_t* __restrict__ k, long n, int B,
                                                                uint32_t* __restrict__ blockSums)
 {
-    __shared__ LeafTileSmem s;
+    __shared__ LeafTileSmem<MaxB> s;
     int em[4];
     long t0 = long(blockIdx.x) * kLeafTile;
     leafTile(k, n, B, t0, s, em);
@@ -170,22 +175,25 @@ __global__ __launch_bounds__(256) void scanBlocksKernel(uint32_t* __restrict__ s
 }
 
 // leaves of the tile starting at particle t0, written from position `base` on
+template<int MaxB>
 __device__ void leafEmitTile(const uint64_t* __restrict__ k, long n, int B, long t0, uint32_t base,
                              uint64_t* __restrict__ leaves, uint32_t* __restrict__ layout, long capacity,
-                             LeafTileSmem& s);
+                             LeafTileSmem<MaxB>& s);
 
+template<int MaxB>
 __global__ __launch_bounds__(kLeafThreads) void leafEmitKernel(const uint64_t* __restrict__ k, long n, int B,
                                                               const uint32_t* __restrict__ blockOffsets,
                                                               uint64_t* __restrict__ leaves,
                                                               uint32_t* __restrict__ layout, long capacity)
 {
-    __shared__ LeafTileSmem s;
+    __shared__ LeafTileSmem<MaxB> s;
     leafEmitTile(k, n, B, long(blockIdx.x) * kLeafTile, blockOffsets[blockIdx.x], leaves, layout, capacity, s);
 }
 
+template<int MaxB>
 __device__ void leafEmitTile(const uint64_t* __restrict__ k, long n, int B, long t0, uint32_t base,
                              uint64_t* __restrict__ leaves, uint32_t* __restrict__ layout, long capacity,
-                             LeafTileSmem& s)
+                             LeafTileSmem<MaxB>& s)
 {
     int em[4];
     leafTile(k, n, B, t0, s, em);
@@ -260,7 +268,10 @@ int buildLeaves(Context* ctx, const uint64_t* keys, size_t n, unsigned bucket, u
     int numBlocks = int((n + kLeafTile - 1) / kLeafTile);
     auto* sums    = static_cast<uint32_t*>(ctx->getScratch(kSlotLeafBlocks, (numBlocks + 1) * sizeof(uint32_t)));
     if (!sums) return RYO_ERR_HIP;
-    leafCountKernel<<<numBlocks, kLeafThreads, 0, s>>>(keys, long(n), int(bucket), sums);
+    if (bucket <= unsigned(kSmallBucket))
+        leafCountKernel<kSmallBucket><<<numBlocks, kLeafThreads, 0, s>>>(keys, long(n), int(bucket), sums);
+    else
+        leafCountKernel<kMaxBucket><<<numBlocks, kLeafThreads, 0, s>>>(keys, long(n), int(bucket), sums);
     scanBlocksKernel<<<1, 256, 0, s>>>(sums, numBlocks);
     RYO_LAUNCH_CHECK(ctx);
     uint32_t L = 0;
@@ -268,7 +279,12 @@ int buildLeaves(Context* ctx, const uint64_t* keys, size_t n, unsigned bucket, u
     RYO_HIP(ctx, hipStreamSynchronize(s));
     *numLeaves = int(L);
     if (size_t(L) > capacity) return setError(ctx, RYO_ERR_CAPACITY, "leaves: %u leaves > capacity %zu", L, capacity);
-    leafEmitKernel<<<numBlocks, kLeafThreads, 0, s>>>(keys, long(n), int(bucket), sums, leaves, layout, long(L));
+    if (bucket <= unsigned(kSmallBucket))
+        leafEmitKernel<kSmallBucket><<<numBlocks, kLeafThreads, 0, s>>>(keys, long(n), int(bucket), sums, leaves,
+                                                                         layout, long(L));
+    else
+        leafEmitKernel<kMaxBucket><<<numBlocks, kLeafThreads, 0, s>>>(keys, long(n), int(bucket), sums, leaves,
+                                                                       layout, long(L));
     leafCountsKernel<<<iceil(L, 256), 256, 0, s>>>(layout, counts, int(L));
     RYO_LAUNCH_CHECK(ctx);
     // the layout drives every later particle access: check its terminator before anything reads it
@@ -413,10 +429,11 @@ int linkOctree(Context* ctx, const uint64_t* leaves, int L, uint64_t* prefixes, 
 // local indices, so every system gets exactly the arrays it would get on its own.
 
 // leaf tiles: block b is tile b - TS[s] of system s; the last tile of a system also counts its terminator slot
+template<int MaxB>
 __global__ __launch_bounds__(kLeafThreads) void batchLeafCountKernel(const uint64_t* __restrict__ sk, BatchView bv,
                                                                     int B, uint32_t* __restrict__ blockSums)
 {
-    __shared__ LeafTileSmem s;
+    __shared__ LeafTileSmem<MaxB> s;
     const int sys = segmentOf(bv.TS, bv.S, blockIdx.x);
     const uint32_t t = blockIdx.x - bv.TS[sys];
     const uint32_t p0 = bv.P[sys];
@@ -427,12 +444,13 @@ __global__ __launch_bounds__(kLeafThreads) void batchLeafCountKernel(const uint6
     if (threadIdx.x == 0) blockSums[blockIdx.x] = tot + (t + 1 == bv.TS[sys + 1] - bv.TS[sys] ? 1u : 0u);
 }
 
+template<int MaxB>
 __global__ __launch_bounds__(kLeafThreads) void batchLeafEmitKernel(const uint64_t* __restrict__ sk, BatchView bv,
                                                                    int B, const uint32_t* __restrict__ blockOffsets,
                                                                    uint64_t* __restrict__ leaves,
                                                                    uint32_t* __restrict__ layout, long capacity)
 {
-    __shared__ LeafTileSmem s;
+    __shared__ LeafTileSmem<MaxB> s;
     const int sys = segmentOf(bv.TS, bv.S, blockIdx.x);
     const uint32_t t = blockIdx.x - bv.TS[sys];
     const uint32_t p0 = bv.P[sys];
@@ -625,7 +643,10 @@ int batchLeavesCount(Context* ctx, const uint64_t* sk, const BatchView& bv, int 
 {
     if (B < 1 || B > kMaxBucket) return setError(ctx, RYO_ERR_UNSUPPORTED, "batch leaves: bucket size %d", B);
     hipStream_t s = ctx->stream;
-    batchLeafCountKernel<<<numTiles, kLeafThreads, 0, s>>>(sk, bv, B, blockSums);
+    if (B <= kSmallBucket)
+        batchLeafCountKernel<kSmallBucket><<<numTiles, kLeafThreads, 0, s>>>(sk, bv, B, blockSums);
+    else
+        batchLeafCountKernel<kMaxBucket><<<numTiles, kLeafThreads, 0, s>>>(sk, bv, B, blockSums);
     scanBlocksKernel<<<1, 256, 0, s>>>(blockSums, numTiles);
     batchLeafOffsetsKernel<<<iceil(bv.S + 1, 256), 256, 0, s>>>(bv, blockSums, numTiles, LO);
     RYO_LAUNCH_CHECK(ctx);
@@ -648,7 +669,12 @@ int batchLeavesEmit(Context* ctx, const uint64_t* sk, const BatchView& bv, int n
                     uint32_t totalSlots)
 {
     hipStream_t s = ctx->stream;
-    batchLeafEmitKernel<<<numTiles, kLeafThreads, 0, s>>>(sk, bv, B, blockSums, leaves, layout, long(totalSlots));
+    if (B <= kSmallBucket)
+        batchLeafEmitKernel<kSmallBucket><<<numTiles, kLeafThreads, 0, s>>>(sk, bv, B, blockSums, leaves, layout,
+                                                                             long(totalSlots));
+    else
+        batchLeafEmitKernel<kMaxBucket><<<numTiles, kLeafThreads, 0, s>>>(sk, bv, B, blockSums, leaves, layout,
+                                                                           long(totalSlots));
     batchLeafCountsKernel<<<std::max(1u, std::min(iceil(totalSlots, 256), unsigned(ctx->numCUs * 8))), 256, 0, s>>>(
         layout, bv, counts);
     RYO_LAUNCH_CHECK(ctx);
