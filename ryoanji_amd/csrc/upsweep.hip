@@ -72,11 +72,23 @@ __device__ inline void storeNode(const UpsweepArgs& a, int node, const double c[
         q32[k] = float(q[k]);
 }
 
+// value of another lane of the same 8-lane group through DPP (both halves of the double); the 8 lanes of a
+// group are always active or inactive together, so every lane read is active
+template<int Ctrl>
+__device__ __forceinline__ double dppMove(double v)
+{
+    const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), Ctrl, 0xF, 0xF, true);
+    const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), Ctrl, 0xF, 0xF, true);
+    return __hiloint2double(hi, lo);
+}
+
+// sum over the 8-lane group, in every lane: pairs (quad_perm [1,0,3,2]), quads ([2,3,0,1]), then the other quad
+// of the group (row_half_mirror: lane l reads 7 - l); the same additions as a butterfly over lane ^ 1, ^ 2, ^ 4
 __device__ inline double red8(double v)
 {
-    v += __shfl_xor(v, 1);
-    v += __shfl_xor(v, 2);
-    v += __shfl_xor(v, 4);
+    v += dppMove<0xB1>(v);
+    v += dppMove<0x4E>(v);
+    v += dppMove<0x141>(v);
     return v;
 }
 
