@@ -165,6 +165,31 @@ int ryo_traverse_f32(ryo_ctx* ctx, int num_groups, const uint32_t* group_start, 
                      const float* box_l /* host float[3] */, float* p, float* ax, float* ay, float* az,
                      double* total_pot);
 
+/* ryoanji::traverse for the (Tc,Th,Tm,Ta,Tf) = (d,d,d,d,d) instantiation with CartesianQuadrupole<double>
+ * (traversal_gpu.cu:635-641): double coordinates, softenings, masses and outputs; centers double[n][4]
+ * {center of mass, mac^2} (SourceCenterType<double>), multipoles double[n][8]. Every interaction is evaluated and
+ * accumulated in double except 1/R, which the reference's inverseSquareRoot(double) takes in single precision
+ * (kernel.hpp:39-53); same walk, MAC, grouping and output convention as ryo_traverse_f32 (init_node,
+ * outputs ADDED, box_l host double[3] when num_shells > 0, *total_pot = 0.5 G sum m phi in FP64). */
+int ryo_traverse_f64(ryo_ctx* ctx, int num_groups, const uint32_t* group_start, const uint32_t* group_end,
+                     int init_node, const double* xt, const double* yt, const double* zt, const double* mt,
+                     const double* ht, const double* xs, const double* ys, const double* zs, const double* ms,
+                     const double* hs, const int32_t* child_offsets, const int32_t* internal_to_leaf,
+                     const uint32_t* layout, const double* centers, const double* multipoles, double G,
+                     int num_shells, const double* box_l, double* p, double* ax, double* ay, double* az,
+                     double* total_pot);
+
+/* ryoanji::traverse for the (d,f,f,f,d) instantiation with CartesianQuadrupole<float>: double coordinates and
+ * centers, float softenings, masses, multipoles and outputs; interactions in double (the reference accumulates in
+ * Tf = double), outputs rounded to float. Otherwise as ryo_traverse_f64. */
+int ryo_traverse_f64_f32(ryo_ctx* ctx, int num_groups, const uint32_t* group_start, const uint32_t* group_end,
+                         int init_node, const double* xt, const double* yt, const double* zt, const float* mt,
+                         const float* ht, const double* xs, const double* ys, const double* zs, const float* ms,
+                         const float* hs, const int32_t* child_offsets, const int32_t* internal_to_leaf,
+                         const uint32_t* layout, const double* centers, const float* multipoles, double G,
+                         int num_shells, const double* box_l, float* p, float* ax, float* ay, float* az,
+                         double* total_pot);
+
 /* ryoanji::readBhStats (traversal_gpu.cu:688-700): host uint64[5] {sumP2P, maxP2P, sumM2P, maxM2P, maxStack}
  * of the last ryo_traverse_f32 call (sums are interactions summed over targets). */
 int ryo_read_stats(ryo_ctx* ctx, uint64_t* stats);

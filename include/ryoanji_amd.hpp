@@ -254,6 +254,38 @@ inline double traverse(GroupView grp, int initNodeIdx, const float* xt, const fl
     return pot;
 }
 
+//! ryoanji::traverse, (Tc,Th,Tm,Ta,Tf) = (d,d,d,d,d) with CartesianQuadrupole<double> (traversal_gpu.cu:635-641)
+inline double traverse(GroupView grp, int initNodeIdx, const double* xt, const double* yt, const double* zt,
+                       const double* mt, const double* ht, const double* xs, const double* ys, const double* zs,
+                       const double* ms, const double* hs, const TreeNodeIndex* childOffsets,
+                       const TreeNodeIndex* internalToLeaf, const LocalIndex* layout, const Vec4<double>* sourceCenter,
+                       const CartesianQuadrupole<double>* multipoles, double G, int numShells, Vec3<double> boxL,
+                       double* p, double* ax, double* ay, double* az, int* /*gmPool*/ = nullptr)
+{
+    double pot = 0;
+    ryoanji_amd::check(ryo_traverse_f64(
+        ryoanji_amd::ctx(), int(grp.numGroups), grp.groupStart, grp.groupEnd, initNodeIdx, xt, yt, zt, mt, ht, xs, ys,
+        zs, ms, hs, childOffsets, internalToLeaf, layout, reinterpret_cast<const double*>(sourceCenter),
+        reinterpret_cast<const double*>(multipoles), G, numShells, boxL.data(), p, ax, ay, az, &pot));
+    return pot;
+}
+
+//! ryoanji::traverse, (Tc,Th,Tm,Ta,Tf) = (d,f,f,f,d) with CartesianQuadrupole<float> (traversal_gpu.cu:635-641)
+inline double traverse(GroupView grp, int initNodeIdx, const double* xt, const double* yt, const double* zt,
+                       const float* mt, const float* ht, const double* xs, const double* ys, const double* zs,
+                       const float* ms, const float* hs, const TreeNodeIndex* childOffsets,
+                       const TreeNodeIndex* internalToLeaf, const LocalIndex* layout, const Vec4<double>* sourceCenter,
+                       const CartesianQuadrupole<float>* multipoles, double G, int numShells, Vec3<double> boxL,
+                       float* p, float* ax, float* ay, float* az, int* /*gmPool*/ = nullptr)
+{
+    double pot = 0;
+    ryoanji_amd::check(ryo_traverse_f64_f32(
+        ryoanji_amd::ctx(), int(grp.numGroups), grp.groupStart, grp.groupEnd, initNodeIdx, xt, yt, zt, mt, ht, xs, ys,
+        zs, ms, hs, childOffsets, internalToLeaf, layout, reinterpret_cast<const double*>(sourceCenter),
+        reinterpret_cast<const float*>(multipoles), G, numShells, boxL.data(), p, ax, ay, az, &pot));
+    return pot;
+}
+
 //! ryoanji::bhMaxTargetSize -- traversal_gpu.cu:643 (one wavefront of targets)
 inline int bhMaxTargetSize() { return 64; }
 

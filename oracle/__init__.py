@@ -89,6 +89,10 @@ def lib(which: str = "orc") -> C.CDLL:
         sig.update({
             "compute_gravity_f32": (C.c_double, [_P, C.c_int, _P, _P, _P, _P, _P, _P, _P, _P, _P, C.c_float,
                                                  C.c_int, C.c_int, C.c_int, _P, _P, _P, _P]),
+            "compute_gravity_f64": (C.c_double, [_P, C.c_int, _P, _P, _P, _P, _P, _P, _P, _P, _P, C.c_float,
+                                                 C.c_int, C.c_int, C.c_int, _P, _P, _P, _P]),
+            "compute_gravity_df": (C.c_double, [_P, C.c_int, _P, _P, _P, _P, _P, _P, _P, _P, _P, C.c_float,
+                                                C.c_int, C.c_int, C.c_int, _P, _P, _P, _P]),
             "direct_sum_f64": (C.c_int, [_P, _P, _P, _P, _P, C.c_size_t, C.c_double, _P, C.c_int, _P, _P, _P,
                                          _P]),
             "ewald_f64": (C.c_double, [_P, _P, _P, _P, _P, _P, C.c_size_t, _P, C.c_double, _P, _P, _P, _P, _P]),
@@ -252,6 +256,25 @@ def compute_gravity_f64(tree_leaves, layout, centers, mp, x, y, z, h, m, limits,
                                           _ptr(_c(mp, np.float32)), *(_ptr(a) for a in arrs),
                                           _ptr(_c(limits, np.float32)), float(G), num_shells, group_size, _ptr(ug),
                                           _ptr(ax), _ptr(ay), _ptr(az))
+    return tot, ug, ax, ay, az
+
+
+def ref_compute_gravity64(tree_leaves, layout, centers, mp, x, y, z, h, m, limits, G=1.0, num_shells=0, mixed=False):
+    """The reference computeGravity (16-target groups) with double coordinates and centers: mixed=False is the
+    all-double instantiation (CartesianQuadrupole<double>, double outputs), mixed=True has float h, m, multipoles
+    and outputs. -> (pot_total, ugrav, ax, ay, az)"""
+    leaves = _c(tree_leaves, np.uint64)
+    layout = _c(layout, np.uint32)
+    L = leaves.size - 1
+    ft = np.float32 if mixed else np.float64
+    xyz = [_c(a, np.float64) for a in (x, y, z)]
+    hm = [_c(a, ft) for a in (h, m)]
+    n = xyz[0].size
+    ug, ax, ay, az = (np.zeros(n, ft) for _ in range(4))
+    tot = _f("ref", "compute_gravity_df" if mixed else "compute_gravity_f64")(
+        _ptr(leaves), L, _ptr(layout), _ptr(_c(centers, np.float64)), _ptr(_c(mp, ft)), *(_ptr(a) for a in xyz),
+        *(_ptr(a) for a in hm), _ptr(_c(limits, np.float64)), float(G), num_shells, 0, L, _ptr(ug), _ptr(ax), _ptr(ay),
+        _ptr(az))
     return tot, ug, ax, ay, az
 
 

@@ -103,6 +103,10 @@ SIGNATURES = {
                                   C.c_size_t, C.POINTER(_I)]),
     "ryo_traverse_f32": (_I, [_P, _I, _P, _P, _I] + [_P] * 10 + [_P, _P, _P, _P, _P, _F, _I, _P, _P, _P, _P, _P,
                                                                    _P]),
+    "ryo_traverse_f64": (_I, [_P, _I, _P, _P, _I] + [_P] * 10 + [_P, _P, _P, _P, _P, _D, _I, _P, _P, _P, _P, _P,
+                                                                   _P]),
+    "ryo_traverse_f64_f32": (_I, [_P, _I, _P, _P, _I] + [_P] * 10 + [_P, _P, _P, _P, _P, _D, _I, _P, _P, _P, _P,
+                                                                       _P, _P]),
     "ryo_read_stats": (_I, [_P, _P]),
     "ryo_traversal_retries": (C.c_uint64, [_P]),
     "ryo_ewald_f32": (_I, [_P, _P, _P, C.c_uint32, C.c_uint32, _P, _P, _P, _P, _P, _F, _P, _P, _P, _P, _P, _P]),

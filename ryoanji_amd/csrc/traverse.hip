@@ -578,7 +578,7 @@ __global__ __launch_bounds__(256) void sumKernel(const double* __restrict__ v, i
 }
 
 // deterministic two-stage FP64 sum of v[0..n) into *out (partials: scratch of 256 doubles)
-static void sumDeterministic(hipStream_t s, const double* v, int n, double* partials, double* out)
+void sumDeterministic(hipStream_t s, const double* v, int n, double* partials, double* out)
 {
     const int blocks = 256;
     const int chunk  = (n + blocks - 1) / blocks;
