@@ -67,6 +67,11 @@ int ryo_ctx_set_traversal_waves(ryo_ctx* ctx, int waves_per_block);
  * with 0 the keys array is write-only. Cubic boxes only (RYO_ERR_UNSUPPORTED otherwise). */
 int ryo_sfc_keys_f32(ryo_ctx* ctx, const float* x, const float* y, const float* z, uint64_t* keys, size_t n,
                      const float* limits, int kind, int skip_removed);
+/* cstone::computeSfcKeys for double coordinates (the Box<double> instantiations, sfc_gpu.cu:46-62): cell indices
+ * and box bits in double as Box<double> and sfcCellIndex<double> compute them (box.hpp:160-198, sfc.hpp:153-159);
+ * limits = host double[6]; otherwise as ryo_sfc_keys_f32 (mixed-dimension keys for non-cubic boxes included). */
+int ryo_sfc_keys_f64(ryo_ctx* ctx, const double* x, const double* y, const double* z, uint64_t* keys, size_t n,
+                     const double* limits, int kind, int skip_removed);
 
 /* ---------------------------------------------------------------- a3: stable radix sort */
 
@@ -87,6 +92,9 @@ int ryo_sequence_i32(ryo_ctx* ctx, int32_t* out, size_t n, int32_t start);
  * device pointers. */
 int ryo_gather_f32(ryo_ctx* ctx, const int32_t* perm, size_t n, int num_arrays, const float* const* in,
                    float* const* out);
+/* the same gather for double arrays (TreeBuilder over double coordinates) */
+int ryo_gather_f64(ryo_ctx* ctx, const int32_t* perm, size_t n, int num_arrays, const double* const* in,
+                   double* const* out);
 
 /* ---------------------------------------------------------------- a5+a7: cornerstone leaves */
 
@@ -124,6 +132,14 @@ int ryo_upsweep_f32(ryo_ctx* ctx, const float* x, const float* y, const float* z
                     const int32_t* parents, const int32_t* leaf_to_internal, const int32_t* level_range,
                     const float* limits, float inv_theta, float* centers, float* multipoles, double* centers64,
                     double* multipoles64);
+/* The same upsweep for double particles and a Box<double> (computeLeafMultipoles / upsweepMultipoles / setMac for
+ * T = double, upsweep_gpu.cu:134-143): centers double[numNodes][4] {com, mac^2} (SourceCenterType<double>),
+ * multipoles double[numNodes][8] (CartesianQuadrupole<double>), multipoles32 (optional, NULL: not written) the
+ * float copy for the (d,f,f,f,d) traversal. limits = host double[6]. */
+int ryo_upsweep_f64(ryo_ctx* ctx, const double* x, const double* y, const double* z, const double* m,
+                    const uint32_t* layout, int num_leaves, const uint64_t* prefixes, const int32_t* child_offsets,
+                    const int32_t* parents, const int32_t* leaf_to_internal, const int32_t* level_range,
+                    const double* limits, float inv_theta, double* centers, double* multipoles, float* multipoles32);
 
 /* ---------------------------------------------------------------- a13: target groups */
 

@@ -108,6 +108,8 @@ public:
             f_[k] = float(lim_[k]);
         return f_;
     }
+    //! the limits in the box's own type (host T[6]), for the double entry points
+    const T* limitsT() const { return lim_; }
 
 private:
     T lim_[6];
@@ -143,6 +145,13 @@ inline void computeSfcKeys(execution::Gpu, const float* x, const float* y, const
                            const Box<float>& box)
 {
     ryoanji_amd::check(ryo_sfc_keys_f32(ryoanji_amd::ctx(), x, y, z, keys, n, box.limits(), 0, 1));
+}
+
+//! cstone::computeSfcKeys for double coordinates and Box<double> (sfc_gpu.cu:46-62)
+inline void computeSfcKeys(execution::Gpu, const double* x, const double* y, const double* z, uint64_t* keys,
+                           size_t n, const Box<double>& box)
+{
+    ryoanji_amd::check(ryo_sfc_keys_f64(ryoanji_amd::ctx(), x, y, z, keys, n, box.limitsT(), 0, 1));
 }
 
 //! cstone::sortByKeyTempStorage -- cstone/primitives/primitives_gpu.h:120-124

@@ -91,6 +91,7 @@ def lib(which: str = "orc") -> C.CDLL:
                                                  C.c_int, C.c_int, C.c_int, _P, _P, _P, _P]),
             "compute_gravity_f64": (C.c_double, [_P, C.c_int, _P, _P, _P, _P, _P, _P, _P, _P, _P, C.c_float,
                                                  C.c_int, C.c_int, C.c_int, _P, _P, _P, _P]),
+            "sfc_keys_f64": (C.c_int, [_P, _P, _P, _P, C.c_size_t, _P, C.c_int]),
             "compute_gravity_df": (C.c_double, [_P, C.c_int, _P, _P, _P, _P, _P, _P, _P, _P, _P, C.c_float,
                                                 C.c_int, C.c_int, C.c_int, _P, _P, _P, _P]),
             "direct_sum_f64": (C.c_int, [_P, _P, _P, _P, _P, C.c_size_t, C.c_double, _P, C.c_int, _P, _P, _P,
@@ -257,6 +258,15 @@ def compute_gravity_f64(tree_leaves, layout, centers, mp, x, y, z, h, m, limits,
                                           _ptr(_c(limits, np.float32)), float(G), num_shells, group_size, _ptr(ug),
                                           _ptr(ax), _ptr(ay), _ptr(az))
     return tot, ug, ax, ay, az
+
+
+def ref_sfc_keys64(x, y, z, limits, hilbert=True):
+    """the reference computeSfcKeys for double coordinates and Box<double> (64-bit Hilbert or Morton keys)"""
+    x, y, z = (_c(a, np.float64) for a in (x, y, z))
+    keys = np.zeros(x.size, np.uint64)
+    _f("ref", "sfc_keys_f64")(_ptr(x), _ptr(y), _ptr(z), _ptr(keys), x.size, _ptr(_c(limits, np.float64)),
+                              int(hilbert))
+    return keys
 
 
 def ref_compute_gravity64(tree_leaves, layout, centers, mp, x, y, z, h, m, limits, G=1.0, num_shells=0, mixed=False):

@@ -53,6 +53,15 @@ int ref_sfc_keys(const float* x, const float* y, const float* z, uint64_t* keys,
     return 0;
 }
 
+int ref_sfc_keys_f64(const double* x, const double* y, const double* z, uint64_t* keys, size_t n,
+                     const double* limits, int hilbert)
+{
+    auto box = box6(limits);
+    if (hilbert) computeSfcKeys(x, y, z, reinterpret_cast<HilbertKey<uint64_t>*>(keys), n, box);
+    else computeSfcKeys(x, y, z, reinterpret_cast<MortonKey<uint64_t>*>(keys), n, box);
+    return 0;
+}
+
 void ref_sort_by_key(uint64_t* keys, int32_t* values, size_t n) { sort_by_key(keys, keys + n, values); }
 
 long ref_compute_octree(const uint64_t* sortedKeys, size_t n, unsigned bucket, uint64_t* leaves, unsigned* counts,

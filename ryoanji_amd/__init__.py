@@ -91,13 +91,16 @@ SIGNATURES = {
     "ryo_ctx_set_traversal_shape": (_I, [_P, _I, _I, _I]),
     "ryo_last_traverse_ms": (_D, [_P]),
     "ryo_sfc_keys_f32": (_I, [_P, _P, _P, _P, _P, _SZ, _P, _I, _I]),
+    "ryo_sfc_keys_f64": (_I, [_P, _P, _P, _P, _P, _SZ, _P, _I, _I]),
     "ryo_sort_temp_bytes": (_SZ, [_SZ]),
     "ryo_sort_pairs_u64_i32": (_I, [_P, _P, _P, _SZ, _P, _P, _P, _SZ]),
     "ryo_sequence_i32": (_I, [_P, _P, _SZ, C.c_int32]),
     "ryo_gather_f32": (_I, [_P, _P, _SZ, _I, _P, _P]),
+    "ryo_gather_f64": (_I, [_P, _P, _SZ, _I, _P, _P]),
     "ryo_build_leaves_u64": (_I, [_P, _P, _SZ, _U, _P, _P, _P, _SZ, C.POINTER(_I)]),
     "ryo_link_octree_u64": (_I, [_P, _P, _I, _P, _P, _P, _P, _P, _P, _P]),
     "ryo_upsweep_f32": (_I, [_P, _P, _P, _P, _P, _P, _I, _P, _P, _P, _P, _P, _P, _F, _P, _P, _P, _P]),
+    "ryo_upsweep_f64": (_I, [_P, _P, _P, _P, _P, _P, _I, _P, _P, _P, _P, _P, _P, _F, _P, _P, _P]),
     "ryo_fixed_groups": (_I, [_P, C.c_uint32, C.c_uint32, _U, _P, C.POINTER(_I)]),
     "ryo_group_splits_f32": (_I, [_P, C.c_uint32, C.c_uint32, _P, _P, _P, _P, _P, _I, _P, _P, _U, _F, _F, _P,
                                   C.c_size_t, C.POINTER(_I)]),

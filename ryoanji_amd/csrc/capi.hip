@@ -11,10 +11,12 @@ namespace ryo
 
 // stage implementations (keys_sort.hip, tree.hip, upsweep.hip, traverse.hip, ewald_direct.hip)
 int sfcKeys(Context*, const float*, const float*, const float*, uint64_t*, size_t, const float*, int, int);
+int sfcKeys64(Context*, const double*, const double*, const double*, uint64_t*, size_t, const double*, int, int);
 size_t sortTempBytes(size_t n);
 int sortPairs(Context*, uint64_t*, int32_t*, size_t, uint64_t*, int32_t*, void*, size_t);
 int sequence(Context*, int32_t*, size_t, int32_t);
 int gather(Context*, const int32_t*, size_t, int, const float* const*, float* const*);
+int gather64(Context*, const int32_t*, size_t, int, const double* const*, double* const*);
 int sfcKeysPacked(Context*, const float* const*, uint64_t*, int32_t*, size_t, const float*, float4**);
 int gatherRecords(Context*, const int32_t*, size_t, const float4*, float* const*);
 int buildLeaves(Context*, const uint64_t*, size_t, unsigned, uint64_t*, uint32_t*, uint32_t*, size_t, int*);
@@ -22,6 +24,9 @@ int linkOctree(Context*, const uint64_t*, int, uint64_t*, int32_t*, int32_t*, in
 int upsweep(Context*, const float*, const float*, const float*, const float*, const uint32_t*, int, const uint64_t*,
             const int32_t*, const int32_t*, const int32_t*, const int32_t*, const float*, float, float*, float*,
             double*, double*, double);
+int upsweep64(Context*, const double*, const double*, const double*, const double*, const uint32_t*, int,
+              const uint64_t*, const int32_t*, const int32_t*, const int32_t*, const int32_t*, const double*, float,
+              double*, double*, float*);
 int fixedGroups(Context*, uint32_t, uint32_t, unsigned, uint32_t*, int*);
 int groupSplits(Context* ctx, uint32_t first, uint32_t last, const float* x, const float* y, const float* z,
                 const float* h, const uint64_t* leaves, int numLeaves, const uint32_t* layout, const float* limits,
@@ -180,6 +185,14 @@ int ryo_sfc_keys_f32(ryo_ctx* h, const float* x, const float* y, const float* z,
     return sfcKeys(ctx, x, y, z, keys, n, limits, kind, skip_removed);
 }
 
+int ryo_sfc_keys_f64(ryo_ctx* h, const double* x, const double* y, const double* z, uint64_t* keys, size_t n,
+                     const double* limits, int kind, int skip_removed)
+{
+    CTX_OR_FAIL(h);
+    if (!limits || (n && (!x || !y || !z || !keys))) return setError(ctx, RYO_ERR_ARG, "sfc keys: null pointer");
+    return sfcKeys64(ctx, x, y, z, keys, n, limits, kind, skip_removed);
+}
+
 size_t ryo_sort_temp_bytes(size_t n) { return sortTempBytes(n); }
 
 int ryo_sort_pairs_u64_i32(ryo_ctx* h, uint64_t* keys, int32_t* values, size_t n, uint64_t* keyBuf,
@@ -202,6 +215,13 @@ int ryo_gather_f32(ryo_ctx* h, const int32_t* perm, size_t n, int num_arrays, co
 {
     CTX_OR_FAIL(h);
     return gather(ctx, perm, n, num_arrays, in, out);
+}
+
+int ryo_gather_f64(ryo_ctx* h, const int32_t* perm, size_t n, int num_arrays, const double* const* in,
+                   double* const* out)
+{
+    CTX_OR_FAIL(h);
+    return gather64(ctx, perm, n, num_arrays, in, out);
 }
 
 int ryo_build_leaves_u64(ryo_ctx* h, const uint64_t* sorted_keys, size_t n, unsigned bucket_size, uint64_t* leaves,
@@ -227,6 +247,17 @@ int ryo_upsweep_f32(ryo_ctx* h, const float* x, const float* y, const float* z, 
     CTX_OR_FAIL(h);
     return upsweep(ctx, x, y, z, m, layout, num_leaves, prefixes, child, parents, l2i, level_range, limits,
                    inv_theta, centers, mp, c64, q64, 0.0);
+}
+
+int ryo_upsweep_f64(ryo_ctx* h, const double* x, const double* y, const double* z, const double* m,
+                    const uint32_t* layout, int num_leaves, const uint64_t* prefixes, const int32_t* child,
+                    const int32_t* parents, const int32_t* l2i, const int32_t* level_range, const double* limits,
+                    float inv_theta, double* centers, double* mp, float* mp32)
+{
+    CTX_OR_FAIL(h);
+    if (!limits) return setError(ctx, RYO_ERR_ARG, "upsweep: null limits");
+    return upsweep64(ctx, x, y, z, m, layout, num_leaves, prefixes, child, parents, l2i, level_range, limits,
+                     inv_theta, centers, mp, mp32);
 }
 
 int ryo_fixed_groups(ryo_ctx* h, uint32_t first, uint32_t last, unsigned group_size, uint32_t* groups,

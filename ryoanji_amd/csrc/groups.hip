@@ -50,6 +50,7 @@ __device__ inline int leafOf(const uint32_t* layout, int numLeaves, uint32_t i)
 
 __global__ __launch_bounds__(256) void groupSplitsKernel(SplitArgs a)
 {
+#pragma clang fp contract(off) // the split criteria round every operation as the reference CPU does
     const int w    = int((size_t(blockIdx.x) * blockDim.x + threadIdx.x) >> 6);
     const int lane = threadIdx.x & 63;
     if (w >= a.numWindows) return; // whole waves exit together
@@ -65,38 +66,38 @@ __global__ __launch_bounds__(256) void groupSplitsKernel(SplitArgs a)
     float sz[3];
     for (int d = 0; d < 3; ++d)
         sz[d] = float(ib.len[d]) * hul;
-    float vol    = __fmul_rn(__fmul_rn(__fmul_rn(8.0f, sz[0]), sz[1]), sz[2]);
+    float vol    = ((8.0f * sz[0]) * sz[1]) * sz[2];
     float volMin = waveMin(vol > 0.0f ? fminf(vol, 1.0f) : 1.0f);
     float edge[3];
     for (int d = 0; d < 3; ++d)
         edge[d] = waveMin(fminf(1.0f, 2.0f * sz[d]));
-    const float distCrit   = __fmul_rn(cbrtf(volMin), a.tolFactor);
-    const float distCritSq = __fmul_rn(distCrit, distCrit);
+    const float distCrit   = (cbrtf(volMin) * a.tolFactor);
+    const float distCritSq = (distCrit * distCrit);
 
     // positions and interaction radii in units of the longest box edge
-    const float px = __fmul_rn(a.x[body], a.invMaxExtent), py = __fmul_rn(a.y[body], a.invMaxExtent),
-                pz = __fmul_rn(a.z[body], a.invMaxExtent);
-    const float pr = a.h ? __fmul_rn(__fmul_rn(2.0f, a.h[body]), a.invMaxExtent) : 1.0f;
+    const float px = (a.x[body] * a.invMaxExtent), py = (a.y[body] * a.invMaxExtent),
+                pz = (a.z[body] * a.invMaxExtent);
+    const float pr = a.h ? ((2.0f * a.h[body]) * a.invMaxExtent) : 1.0f;
 
     // consecutive-distance criterion (the last lane compares with itself: never a split)
     const float nx = __shfl_down(px, 1), ny = __shfl_down(py, 1), nz = __shfl_down(pz, 1);
-    const float dx = __fsub_rn(nx, px), dy = __fsub_rn(ny, py), dz = __fsub_rn(nz, pz);
-    const float d2 = __fadd_rn(__fmul_rn(dx, dx), __fadd_rn(__fmul_rn(dy, dy), __fmul_rn(dz, dz)));
-    uint64_t mask  = __ballot(d2 > fminf(distCritSq, __fmul_rn(pr, pr)));
+    const float dx = (nx - px), dy = (ny - py), dz = (nz - pz);
+    const float d2 = dx * dx + (dy * dy + dz * dz);
+    uint64_t mask  = __ballot(d2 > fminf(distCritSq, (pr * pr)));
 
     // window-extent criterion
     float ratio       = 0.0f;
     const float pos[3] = {px, py, pz};
     for (int d = 0; d < 3; ++d)
     {
-        const float allowed = __fmul_rn(a.tolFactor, edge[d]);
-        const float extent  = __fsub_rn(waveMax(pos[d]), waveMin(pos[d]));
-        if (allowed > 0.0f) ratio = fmaxf(ratio, __fdiv_rn(extent, allowed));
+        const float allowed = (a.tolFactor * edge[d]);
+        const float extent  = (waveMax(pos[d]) - waveMin(pos[d]));
+        if (allowed > 0.0f) ratio = fmaxf(ratio, (extent / allowed));
     }
     if (a.extentRatio > 0.0f && ratio > a.extentRatio)
     {
         unsigned chunk = kWindow;
-        while (chunk > 1 && __fmul_rn(float(chunk), ratio) > __fmul_rn(float(kWindow), a.extentRatio))
+        while (chunk > 1 && (float(chunk) * ratio) > (float(kWindow) * a.extentRatio))
             chunk >>= 1;
         const uint32_t inWindow = min(uint32_t(kWindow), a.last - ws);
         const uint32_t next     = uint32_t(lane) + 1;

@@ -14,12 +14,21 @@ namespace ryo
 /* ============================================================================ keys */
 
 // Cell index floor((x - xmin) * m) with x*m and xmin*m rounded once each and split into integer and
-// fraction (sfc.hpp:153-159). The explicit _rn intrinsics keep the compiler from fusing a*b - c.
+// fraction (sfc.hpp:153-159). Contraction is off: fused into a*b - c the products would not be rounded, and a
+// point whose exact product lies just below an integer would get the cell below the reference's.
 __device__ __forceinline__ int cellIndex(float x, float xmin, float m)
 {
-    const float a = __fmul_rn(x, m), b = __fmul_rn(xmin, m);
+#pragma clang fp contract(off)
+    const float a = x * m, b = xmin * m;
     const float fa = floorf(a), fb = floorf(b);
-    return int(__fsub_rn(fa, fb)) - int(__fsub_rn(a, fa) < __fsub_rn(b, fb));
+    return int(fa - fb) - int(a - fa < b - fb);
+}
+__device__ __forceinline__ int cellIndex(double x, double xmin, double m)
+{
+#pragma clang fp contract(off)
+    const double a = x * m, b = xmin * m;
+    const double fa = floor(a), fb = floor(b);
+    return int(fa - fb) - int(a - fa < b - fb);
 }
 
 __device__ __forceinline__ uint64_t spread3(unsigned v)
@@ -47,11 +56,12 @@ struct PackOut
     float4* rec;
 };
 
-template<bool Cubic, bool Pack>
-__global__ __launch_bounds__(256) void sfcKeysKernel(const float* __restrict__ x, const float* __restrict__ y,
-                                                     const float* __restrict__ z, uint64_t* __restrict__ keys,
-                                                     size_t n, KeyBox box, int kind, int skipRemoved, PackOut pk)
+template<class T, bool Cubic, bool Pack>
+__global__ __launch_bounds__(256) void sfcKeysKernel(const T* __restrict__ x, const T* __restrict__ y,
+                                                     const T* __restrict__ z, uint64_t* __restrict__ keys,
+                                                     size_t n, KeyBoxT<T> box, int kind, int skipRemoved, PackOut pk)
 {
+    static_assert(!Pack || sizeof(T) == 4, "the gather records hold float coordinates");
     // cubic boxes: the Hilbert state tables in LDS (6.9 KB), read with per-lane indices
     __shared__ HilbertLut lut;
     if constexpr (Cubic)
@@ -68,7 +78,7 @@ __global__ __launch_bounds__(256) void sfcKeysKernel(const float* __restrict__ x
     for (size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += size_t(gridDim.x) * blockDim.x)
     {
         if (!Pack && skipRemoved && keys[i] == kKeyEnd) continue;
-        const float px = x[i], py = y[i], pz = z[i];
+        const T px = x[i], py = y[i], pz = z[i];
         int ix = min(cellIndex(px, box.lo[0], box.m[0]), mx);
         int iy = min(cellIndex(py, box.lo[1], box.m[1]), my);
         int iz = min(cellIndex(pz, box.lo[2], box.m[2]), mz);
@@ -83,7 +93,7 @@ __global__ __launch_bounds__(256) void sfcKeysKernel(const float* __restrict__ x
         if constexpr (Pack)
         {
             pk.perm[i]       = int32_t(i);
-            pk.rec[2 * i]     = make_float4(px, py, pz, pk.m[i]);
+            pk.rec[2 * i]     = make_float4(float(px), float(py), float(pz), pk.m[i]);
             pk.rec[2 * i + 1] = make_float4(pk.h[i], pk.o[i], 0.0f, 0.0f);
         }
     }
@@ -331,14 +341,17 @@ __global__ void iotaKernel(int32_t* out, size_t n, int32_t start)
         out[i] = start + int32_t(i);
 }
 
-struct GatherArgs
+template<class T>
+struct GatherArgsT
 {
-    const float* in[10];
-    float* out[10];
+    const T* in[10];
+    T* out[10];
     int num;
 };
+using GatherArgs = GatherArgsT<float>;
 
-__global__ __launch_bounds__(256) void gatherKernel(const int32_t* __restrict__ perm, size_t n, GatherArgs a)
+template<class T>
+__global__ __launch_bounds__(256) void gatherKernel(const int32_t* __restrict__ perm, size_t n, GatherArgsT<T> a)
 {
     for (size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += size_t(gridDim.x) * blockDim.x)
     {
@@ -524,34 +537,49 @@ int segmentedSortPairs(Context* ctx, uint64_t* keys, int32_t* vals, size_t n, co
     return RYO_OK;
 }
 
-static int makeKeyBox(Context* ctx, const float* limits, KeyBox& b)
+template<class T>
+static int makeKeyBox(Context* ctx, const T* limits, KeyBoxT<T>& b)
 {
     b.bits = boxBits(limits); // box.hpp:160-198: an axis shorter than 0.75 x the longest one gets fewer bits
     for (int d = 0; d < 3; ++d)
     {
-        float lo = limits[2 * d], hi = limits[2 * d + 1];
+        T lo = limits[2 * d], hi = limits[2 * d + 1];
         if (!(hi > lo)) return setError(ctx, RYO_ERR_ARG, "sfc keys: empty box along axis %d", d);
-        float il = 1.0f / (hi - lo); // Box<float>::inverseLengths_ (box.hpp:107), in float on the host
-        b.lo[d]  = lo;
-        b.m[d]   = float(1u << b.bits.b[d]) * il;
+        T il    = T(1) / (hi - lo); // Box<T>::inverseLengths_ (box.hpp:107), in T on the host
+        b.lo[d] = lo;
+        b.m[d]  = T(1u << b.bits.b[d]) * il;
     }
+    return RYO_OK;
+}
+
+template<class T>
+int sfcKeysT(Context* ctx, const T* x, const T* y, const T* z, uint64_t* keys, size_t n, const T* limits, int kind,
+             int skipRemoved)
+{
+    if (n == 0) return RYO_OK;
+    KeyBoxT<T> b;
+    if (int rc = makeKeyBox(ctx, limits, b)) return rc;
+    const unsigned grid = gridFor(n, 256, ctx->numCUs * 16);
+    if (isCubic(b.bits))
+        sfcKeysKernel<T, true, false><<<grid, 256, 0, ctx->stream>>>(x, y, z, keys, n, b, kind, skipRemoved,
+                                                                      PackOut{});
+    else
+        sfcKeysKernel<T, false, false><<<grid, 256, 0, ctx->stream>>>(x, y, z, keys, n, b, kind, skipRemoved,
+                                                                       PackOut{});
+    RYO_LAUNCH_CHECK(ctx);
     return RYO_OK;
 }
 
 int sfcKeys(Context* ctx, const float* x, const float* y, const float* z, uint64_t* keys, size_t n,
             const float* limits, int kind, int skipRemoved)
 {
-    if (n == 0) return RYO_OK;
-    KeyBox b;
-    if (int rc = makeKeyBox(ctx, limits, b)) return rc;
-    const unsigned grid = gridFor(n, 256, ctx->numCUs * 16);
-    if (isCubic(b.bits))
-        sfcKeysKernel<true, false><<<grid, 256, 0, ctx->stream>>>(x, y, z, keys, n, b, kind, skipRemoved, PackOut{});
-    else
-        sfcKeysKernel<false, false><<<grid, 256, 0, ctx->stream>>>(x, y, z, keys, n, b, kind, skipRemoved,
-                                                                   PackOut{});
-    RYO_LAUNCH_CHECK(ctx);
-    return RYO_OK;
+    return sfcKeysT(ctx, x, y, z, keys, n, limits, kind, skipRemoved);
+}
+
+int sfcKeys64(Context* ctx, const double* x, const double* y, const double* z, uint64_t* keys, size_t n,
+              const double* limits, int kind, int skipRemoved)
+{
+    return sfcKeysT(ctx, x, y, z, keys, n, limits, kind, skipRemoved);
 }
 
 // Hilbert keys of a freshly loaded system together with the identity permutation and the gather records of
@@ -568,9 +596,9 @@ int sfcKeysPacked(Context* ctx, const float* const* in, uint64_t* keys, int32_t*
     const PackOut pk{in[3], in[4], in[5], perm, r};
     const unsigned grid = gridFor(n, 256, ctx->numCUs * 16);
     if (isCubic(b.bits))
-        sfcKeysKernel<true, true><<<grid, 256, 0, ctx->stream>>>(in[0], in[1], in[2], keys, n, b, 0, 0, pk);
+        sfcKeysKernel<float, true, true><<<grid, 256, 0, ctx->stream>>>(in[0], in[1], in[2], keys, n, b, 0, 0, pk);
     else
-        sfcKeysKernel<false, true><<<grid, 256, 0, ctx->stream>>>(in[0], in[1], in[2], keys, n, b, 0, 0, pk);
+        sfcKeysKernel<float, false, true><<<grid, 256, 0, ctx->stream>>>(in[0], in[1], in[2], keys, n, b, 0, 0, pk);
     RYO_LAUNCH_CHECK(ctx);
     *rec = r;
     return RYO_OK;
@@ -627,17 +655,28 @@ int gatherRecords(Context* ctx, const int32_t* perm, size_t n, const float4* rec
     return RYO_OK;
 }
 
-int gather(Context* ctx, const int32_t* perm, size_t n, int num, const float* const* in, float* const* out)
+template<class T>
+static int gatherT(Context* ctx, const int32_t* perm, size_t n, int num, const T* const* in, T* const* out)
 {
     if (num < 0 || num > 10) return setError(ctx, RYO_ERR_ARG, "gather: at most 10 arrays");
     if (n == 0 || num == 0) return RYO_OK;
-    GatherArgs a{};
+    GatherArgsT<T> a{};
     a.num = num;
     for (int k = 0; k < num; ++k)
         a.in[k] = in[k], a.out[k] = out[k];
     gatherKernel<<<gridFor(n, 256, ctx->numCUs * 16), 256, 0, ctx->stream>>>(perm, n, a);
     RYO_LAUNCH_CHECK(ctx);
     return RYO_OK;
+}
+
+int gather(Context* ctx, const int32_t* perm, size_t n, int num, const float* const* in, float* const* out)
+{
+    return gatherT(ctx, perm, n, num, in, out);
+}
+
+int gather64(Context* ctx, const int32_t* perm, size_t n, int num, const double* const* in, double* const* out)
+{
+    return gatherT(ctx, perm, n, num, in, out);
 }
 
 } // namespace ryo

@@ -409,13 +409,15 @@ __host__ __device__ inline IBox3 cellIBox(uint64_t key, unsigned level, const Bi
 namespace ryo
 {
 
-// a box as the key kernels use it
-struct KeyBox
+// a box as the key kernels use it, in the coordinates' type
+template<class T>
+struct KeyBoxT
 {
-    float lo[3];
-    float m[3]; // 2^bits[d] / L_d
+    T lo[3];
+    T m[3];     // 2^bits[d] / L_d
     Bits3 bits; // key bits per axis (21 on every axis of a cubic box)
 };
+using KeyBox = KeyBoxT<float>;
 
 // a box as the MAC computation uses it
 struct MacBox
@@ -427,16 +429,17 @@ struct MacBox
     double softReach = 0; // > 0: cells also open within their radius plus this distance (2 x the largest h)
 };
 
-// per-axis key bits of a box given as {xmin, xmax, ymin, ymax, zmin, zmax} (Box<float>::computeBoxDimBits and
-// getBoxDimBits, box.hpp:160-198, in float as the reference's Box<float> computes them)
-inline Bits3 boxBits(const float* limits)
+// per-axis key bits of a box given as {xmin, xmax, ymin, ymax, zmin, zmax} (Box<T>::computeBoxDimBits and
+// getBoxDimBits, box.hpp:160-198, in the box's type T as the reference's Box<T> computes them)
+template<class T>
+inline Bits3 boxBits(const T* limits)
 {
-    float len[3] = {limits[1] - limits[0], limits[3] - limits[2], limits[5] - limits[4]};
-    float mx     = std::max(std::max(len[0], len[1]), len[2]);
+    T len[3] = {limits[1] - limits[0], limits[3] - limits[2], limits[5] - limits[4]};
+    T mx     = std::max(std::max(len[0], len[1]), len[2]);
     Bits3 r;
     for (int d = 0; d < 3; ++d)
     {
-        unsigned red = unsigned(std::floor(std::log2(mx / len[d]) + 0.5849625007211563f));
+        unsigned red = unsigned(std::floor(std::log2(mx / len[d]) + T(0.5849625007211563)));
         r.b[d]       = red >= kMaxLevel ? 0u : kMaxLevel - red;
     }
     return r;

@@ -129,13 +129,14 @@ __device__ __forceinline__ void p2p(Acc& a, float tx, float ty, float tz, float 
 // compared with mac^2. Unfused operations so that equal inputs give the CPU's decision.
 __device__ __forceinline__ bool macClose(float4 c, float tcx, float tcy, float tcz, float tsx, float tsy, float tsz)
 {
-    float dx = __fsub_rn(fabsf(__fsub_rn(tcx, c.x)), tsx);
-    float dy = __fsub_rn(fabsf(__fsub_rn(tcy, c.y)), tsy);
-    float dz = __fsub_rn(fabsf(__fsub_rn(tcz, c.z)), tsz);
-    dx = __fmul_rn(__fadd_rn(dx, fabsf(dx)), 0.5f);
-    dy = __fmul_rn(__fadd_rn(dy, fabsf(dy)), 0.5f);
-    dz = __fmul_rn(__fadd_rn(dz, fabsf(dz)), 0.5f);
-    float R2 = __fadd_rn(__fmul_rn(dx, dx), __fadd_rn(__fmul_rn(dy, dy), __fmul_rn(dz, dz)));
+#pragma clang fp contract(off)
+    float dx = fabsf(tcx - c.x) - tsx;
+    float dy = fabsf(tcy - c.y) - tsy;
+    float dz = fabsf(tcz - c.z) - tsz;
+    dx = (dx + fabsf(dx)) * 0.5f;
+    dy = (dy + fabsf(dy)) * 0.5f;
+    dz = (dz + fabsf(dz)) * 0.5f;
+    float R2 = dx * dx + (dy * dy + dz * dz);
     return R2 < fabsf(c.w);
 }
 

@@ -66,13 +66,14 @@ __device__ inline double waveMax64(double v)
 __device__ inline bool macClose64(const double* c, double tcx, double tcy, double tcz, double tsx, double tsy,
                                   double tsz)
 {
-    double dx = __dsub_rn(fabs(__dsub_rn(tcx, c[0])), tsx);
-    double dy = __dsub_rn(fabs(__dsub_rn(tcy, c[1])), tsy);
-    double dz = __dsub_rn(fabs(__dsub_rn(tcz, c[2])), tsz);
-    dx = __dmul_rn(__dadd_rn(dx, fabs(dx)), 0.5);
-    dy = __dmul_rn(__dadd_rn(dy, fabs(dy)), 0.5);
-    dz = __dmul_rn(__dadd_rn(dz, fabs(dz)), 0.5);
-    const double R2 = __dadd_rn(__dmul_rn(dx, dx), __dadd_rn(__dmul_rn(dy, dy), __dmul_rn(dz, dz)));
+#pragma clang fp contract(off)
+    double dx = fabs(tcx - c[0]) - tsx;
+    double dy = fabs(tcy - c[1]) - tsy;
+    double dz = fabs(tcz - c[2]) - tsz;
+    dx = (dx + fabs(dx)) * 0.5;
+    dy = (dy + fabs(dy)) * 0.5;
+    dz = (dz + fabs(dz)) * 0.5;
+    const double R2 = dx * dx + (dy * dy + dz * dz);
     return R2 < fabs(c[3]);
 }
 

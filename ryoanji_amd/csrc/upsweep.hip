@@ -41,9 +41,12 @@ __device__ inline double macSquared(uint64_t prefix, double cx, double cy, doubl
     return mac * mac;
 }
 
-struct UpsweepArgs
+// particles of type Tin (float, or double for the double-coordinate entry point); outputs c32, q32 and cmac are
+// optional (NULL: not written)
+template<class Tin>
+struct UpsweepArgsT
 {
-    const float *x, *y, *z, *m;
+    const Tin *x, *y, *z, *m;
     const uint32_t* layout;
     int numLeaves;
     const int32_t* leafNode; // leafToInternal + numInternal
@@ -55,9 +58,12 @@ struct UpsweepArgs
     double* c64; // [n][4] {com, sum|m|}
     double* q64; // [n][8]
     MacBox box;
+    double4* cmac = nullptr; // [n] {com, mac^2} in double (SourceCenterType<double>)
 };
+using UpsweepArgs = UpsweepArgsT<float>;
 
-__device__ inline void storeNode(const UpsweepArgs& a, int node, const double c[4], const double q[8])
+template<class Tin>
+__device__ inline void storeNode(const UpsweepArgsT<Tin>& a, int node, const double c[4], const double q[8])
 {
     double* c64 = a.c64 + 4 * size_t(node);
     double* q64 = a.q64 + 8 * size_t(node);
@@ -66,10 +72,14 @@ __device__ inline void storeNode(const UpsweepArgs& a, int node, const double c[
     for (int k = 0; k < 8; ++k)
         q64[k] = q[k];
     double mac2 = c[3] != 0.0 ? macSquared(a.prefixes[node], c[0], c[1], c[2], a.box) : 0.0;
-    a.c32[node] = make_float4(float(c[0]), float(c[1]), float(c[2]), float(mac2));
-    float* q32  = a.q32 + 8 * size_t(node);
-    for (int k = 0; k < 8; ++k)
-        q32[k] = float(q[k]);
+    if (a.c32) a.c32[node] = make_float4(float(c[0]), float(c[1]), float(c[2]), float(mac2));
+    if (a.q32)
+    {
+        float* q32 = a.q32 + 8 * size_t(node);
+        for (int k = 0; k < 8; ++k)
+            q32[k] = float(q[k]);
+    }
+    if (a.cmac) a.cmac[node] = make_double4(c[0], c[1], c[2], mac2);
 }
 
 // value of another lane of the same 8-lane group through DPP (both halves of the double); the 8 lanes of a
@@ -94,7 +104,8 @@ __device__ inline double red8(double v)
 
 // leaves: 8 lanes per leaf reduce its particles (|m|-weighted center, then the quadrupole about it); all 8 lanes
 // of a group call with the same leaf and `ok`
-__device__ inline void leafUpsweepOne(const UpsweepArgs& a, int leaf, bool ok, int sub)
+template<class Tin>
+__device__ inline void leafUpsweepOne(const UpsweepArgsT<Tin>& a, int leaf, bool ok, int sub)
 {
     uint32_t b = 0, e = 0;
     if (ok) b = a.layout[leaf], e = a.layout[leaf + 1];
@@ -138,14 +149,16 @@ __device__ inline void leafUpsweepOne(const UpsweepArgs& a, int leaf, bool ok, i
     if (ok && sub == 0) storeNode(a, a.leafNode[leaf], c, q);
 }
 
-__global__ __launch_bounds__(256) void leafUpsweepKernel(UpsweepArgs a)
+template<class Tin>
+__global__ __launch_bounds__(256) void leafUpsweepKernel(UpsweepArgsT<Tin> a)
 {
     const int leaf = int((size_t(blockIdx.x) * blockDim.x + threadIdx.x) >> 3);
     leafUpsweepOne(a, leaf, leaf < a.numLeaves, threadIdx.x & 7);
 }
 
 // one internal node: lane k of the 8-lane group holds child k; center and M2M by 8-lane reductions
-__device__ inline void levelNodeOne(const UpsweepArgs& a, int node, int sub)
+template<class Tin>
+__device__ inline void levelNodeOne(const UpsweepArgsT<Tin>& a, int node, int sub)
 {
     {
         const int ch = a.child[node]; // uniform over the 8 lanes of the node
@@ -174,7 +187,8 @@ __device__ inline void levelNodeOne(const UpsweepArgs& a, int node, int sub)
 }
 
 // internal nodes of one level
-__global__ __launch_bounds__(256) void levelUpsweepKernel(UpsweepArgs a, const int32_t* __restrict__ levelRange,
+template<class Tin>
+__global__ __launch_bounds__(256) void levelUpsweepKernel(UpsweepArgsT<Tin> a, const int32_t* __restrict__ levelRange,
                                                           int level)
 {
     const int first = levelRange[level], last = levelRange[level + 1];
@@ -272,28 +286,25 @@ MacBox makeMacBox(const float* limits, float invTheta)
     return b;
 }
 
-int upsweep(Context* ctx, const float* x, const float* y, const float* z, const float* m, const uint32_t* layout,
-            int L, const uint64_t* prefixes, const int32_t* child, const int32_t* parents, const int32_t* l2i,
-            const int32_t* levelRange, const float* limits, float invTheta, float* centers, float* mpoles, double* c64,
-            double* q64, double softReach)
+// leaf kernel, then one kernel per level from the deepest possible parent level to the root; each launch reads
+// its node range on the device, so the sequence needs no host copy of the level ranges (and can be captured)
+template<class Tin, class Tl>
+static int upsweepT(Context* ctx, UpsweepArgsT<Tin> a, int L, const int32_t* levelRange, const Tl* limits,
+                    float invTheta, double softReach)
 {
     if (L < 1) return setError(ctx, RYO_ERR_ARG, "upsweep: need at least one leaf");
     if (!levelRange) return setError(ctx, RYO_ERR_ARG, "upsweep: level ranges are required");
     int nI = (L - 1) / 7, n = L + nI;
     size_t need = 256;
-    if (!c64) need += size_t(n) * 4 * sizeof(double);
-    if (!q64) need += size_t(n) * 8 * sizeof(double);
+    if (!a.c64) need += size_t(n) * 4 * sizeof(double);
+    if (!a.q64) need += size_t(n) * 8 * sizeof(double);
     auto* buf = static_cast<char*>(ctx->getScratch(kSlotUpsweep, need));
     if (!buf) return RYO_ERR_HIP;
     size_t off = 0;
-    if (!c64) c64 = reinterpret_cast<double*>(buf + off), off += size_t(n) * 4 * sizeof(double);
-    if (!q64) q64 = reinterpret_cast<double*>(buf + off), off += size_t(n) * 8 * sizeof(double);
-
-    UpsweepArgs a;
-    a.x = x, a.y = y, a.z = z, a.m = m;
-    a.layout = layout, a.numLeaves = L;
-    a.leafNode = l2i + nI, a.parents = parents, a.child = child, a.prefixes = prefixes;
-    a.c32 = reinterpret_cast<float4*>(centers), a.q32 = mpoles, a.c64 = c64, a.q64 = q64;
+    if (!a.c64) a.c64 = reinterpret_cast<double*>(buf + off), off += size_t(n) * 4 * sizeof(double);
+    if (!a.q64) a.q64 = reinterpret_cast<double*>(buf + off), off += size_t(n) * 8 * sizeof(double);
+    a.numLeaves = L;
+    a.leafNode += nI; // leafToInternal + numInternal
     const Bits3 bits = boxBits(limits);
     a.box.bits       = bits;
     for (int d = 0; d < 3; ++d)
@@ -306,14 +317,41 @@ int upsweep(Context* ctx, const float* x, const float* y, const float* z, const 
     a.box.softReach = softReach;
     hipStream_t s = ctx->stream;
     leafUpsweepKernel<<<iceil(size_t(L) * 8, 256), 256, 0, s>>>(a);
-    // levels from the deepest possible parent level to the root; each launch reads its node range on the device,
-    // so the sequence needs no host copy of the level ranges (and can be captured in a graph)
     const unsigned levelBlocks = std::max(1u, std::min(iceil(size_t(nI) * 8, 256), unsigned(ctx->numCUs * 4)));
     if (nI > 0)
         for (int l = int(kMaxLevel) - 1; l >= 0; --l)
             levelUpsweepKernel<<<levelBlocks, 256, 0, s>>>(a, levelRange, l);
     RYO_LAUNCH_CHECK(ctx);
     return RYO_OK;
+}
+
+int upsweep(Context* ctx, const float* x, const float* y, const float* z, const float* m, const uint32_t* layout,
+            int L, const uint64_t* prefixes, const int32_t* child, const int32_t* parents, const int32_t* l2i,
+            const int32_t* levelRange, const float* limits, float invTheta, float* centers, float* mpoles, double* c64,
+            double* q64, double softReach)
+{
+    UpsweepArgs a{};
+    a.x = x, a.y = y, a.z = z, a.m = m;
+    a.layout = layout;
+    a.leafNode = l2i, a.parents = parents, a.child = child, a.prefixes = prefixes;
+    a.c32 = reinterpret_cast<float4*>(centers), a.q32 = mpoles, a.c64 = c64, a.q64 = q64;
+    return upsweepT(ctx, a, L, levelRange, limits, invTheta, softReach);
+}
+
+// double particles and box: centers {com, mac^2} and multipoles in double (mp32: optional float copy)
+int upsweep64(Context* ctx, const double* x, const double* y, const double* z, const double* m,
+              const uint32_t* layout, int L, const uint64_t* prefixes, const int32_t* child, const int32_t* parents,
+              const int32_t* l2i, const int32_t* levelRange, const double* limits, float invTheta, double* centers,
+              double* mpoles, float* mp32)
+{
+    if (!centers || !mpoles) return setError(ctx, RYO_ERR_ARG, "upsweep: null output");
+    UpsweepArgsT<double> a{};
+    a.x = x, a.y = y, a.z = z, a.m = m;
+    a.layout = layout;
+    a.leafNode = l2i, a.parents = parents, a.child = child, a.prefixes = prefixes;
+    a.c32 = nullptr, a.q32 = mp32, a.c64 = nullptr, a.q64 = mpoles;
+    a.cmac = reinterpret_cast<double4*>(centers);
+    return upsweepT(ctx, a, L, levelRange, limits, invTheta, 0.0);
 }
 
 } // namespace ryo

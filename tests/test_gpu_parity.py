@@ -10,10 +10,11 @@ import numpy as np
 import pytest
 
 import oracle as O
+import ryoanji_amd as R
 from ryoanji_amd import workloads as W
 
-from gpu_util import (dev, gpu_keys, gpu_leaves, gpu_link, gpu_sort, gpu_traverse, gpu_upsweep, oracle_tree,
-                      rel_err)
+from gpu_util import (dev, gpu_keys, gpu_leaves, gpu_link, gpu_sort, gpu_traverse, gpu_upsweep, host, oracle_tree,
+                      rel_err, zeros)
 
 pytestmark = pytest.mark.gpu
 
@@ -249,3 +250,26 @@ def test_empty_inputs(ctx):
     with pytest.raises(R.RyoError):
         sysm.build(R.default_params())
     sysm.close()
+
+
+@pytest.mark.parametrize("name", list(SYSTEMS))
+def test_keys_f64_bit_exact(ctx, name):
+    """keys of double coordinates (Box<double>) equal the reference's computeSfcKeys<double> (oracle/_ref), for
+    coordinates that need double precision and for points exactly on the box faces"""
+    import torch
+
+    s = SYSTEMS[name]()
+    rng = np.random.default_rng(11)
+    lim = s.limits.astype(np.float64)
+    xyz = []
+    for d, a in enumerate((s.x, s.y, s.z)):
+        v = a.astype(np.float64) + rng.uniform(-1e-9, 1e-9, s.n) * (lim[2 * d + 1] - lim[2 * d])
+        v[:3] = (lim[2 * d], lim[2 * d + 1], 0.5 * (lim[2 * d] + lim[2 * d + 1]))
+        xyz.append(np.clip(v, lim[2 * d], lim[2 * d + 1]))
+    for kind, hil in ((0, True), (1, False)):
+        pos = [dev(a) for a in xyz]
+        k = zeros(s.n, torch.int64)
+        ctx.call("ryo_sfc_keys_f64", *(R.ptr(a) for a in pos), R.ptr(k), s.n, R.host_f64(lim), kind, 0)
+        g = host(k, np.uint64)
+        o = O.ref_sfc_keys64(*xyz, lim, hilbert=hil)
+        assert np.array_equal(g, o), f"{name} kind {kind}: {np.count_nonzero(g != o)} keys differ"

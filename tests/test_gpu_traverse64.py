@@ -114,3 +114,70 @@ def test_double_traversal_64_target_groups(ctx):
     _, _, bx, by, bz, _ = _gpu64(ctx, t, xyz, h, m, c64, q64, 16, mixed=False)
     assert np.quantile(rel_err(ax, ay, az, bx, by, bz), 0.99) < 2e-3
     assert st["maxStack"] <= 22 * 64
+
+
+@pytest.mark.parametrize("name,mk", [("plummer", lambda: W.plummer(30000, seed=47)),
+                                     ("box", lambda: W.uniform_box(30000, seed=48, extents=(0.5, 2.0, 1.0)))])
+def test_double_pipeline_equals_reference(ctx, name, mk):
+    """the whole path on double particles through the C ABI (keys, sort, gather, leaves, linked tree, FP64 upsweep,
+    traversal) against the reference's own pipeline on the same doubles (oracle/_ref for keys, sort, upsweep and
+    gravity): integer arrays bit-exact, centers and multipoles to FP64 rounding, forces to FP64 rounding"""
+    import torch
+
+    from gpu_util import gpu_leaves, gpu_link, gpu_sort
+
+    s = mk()
+    rng = np.random.default_rng(5)
+    lim = s.limits.astype(np.float64)
+    xyz = [np.clip(a.astype(np.float64) + rng.uniform(-1e-9, 1e-9, s.n) * (lim[2 * d + 1] - lim[2 * d]),
+                   lim[2 * d], lim[2 * d + 1]) for d, a in enumerate((s.x, s.y, s.z))]
+    m, h = s.m.astype(np.float64), s.h.astype(np.float64)
+    # the reference
+    rkeys = O.ref_sfc_keys64(*xyz, lim)
+    rsk, rperm = O.sort_by_key(rkeys, lib="ref")
+    rx = [a[rperm] for a in (*xyz, m, h)]
+    leaves, counts = O.compute_octree(rsk, 64, lib="ref")
+    layout = O.layout_from_counts(counts)
+    tree = O.link_octree(leaves, lib="ref")
+    c64, q64 = O.upsweep(*rx[:4], leaves, layout, lim, 2.0, precision="f64", lib="ref")
+    rtot, rug, rax, ray, raz = O.ref_compute_gravity64(leaves, layout, c64, q64, *rx[:3], rx[4], rx[3], lim)
+    # the device path
+    pos = [dev(a) for a in xyz]
+    k = zeros(s.n, torch.int64)
+    ctx.call("ryo_sfc_keys_f64", *(R.ptr(a) for a in pos), R.ptr(k), s.n, R.host_f64(lim), 0, 0)
+    gkeys = host(k, np.uint64)
+    assert np.array_equal(gkeys, rkeys)
+    gsk, gperm = gpu_sort(ctx, gkeys)
+    assert np.array_equal(gsk, rsk) and np.array_equal(gperm, rperm)
+    src = [dev(a) for a in (*xyz, m, h)]
+    dst = [zeros(s.n, torch.float64) for _ in range(5)]
+    perm = dev(gperm)
+    ins, outs = (C.c_void_p * 5)(*(R.ptr(a) for a in src)), (C.c_void_p * 5)(*(R.ptr(a) for a in dst))
+    ctx.call("ryo_gather_f64", R.ptr(perm), s.n, 5, ins, outs)
+    gx = [host(a) for a in dst]
+    for a, b in zip(gx, rx):
+        assert np.array_equal(a, b)
+    gl, gc, glay = gpu_leaves(ctx, gsk, 64)
+    assert np.array_equal(gl, leaves) and np.array_equal(glay, layout)
+    gt = gpu_link(ctx, gl)
+    for key in ("childOffsets", "internalToLeaf", "leafToInternal"):
+        assert np.array_equal(gt[key], tree[key]), key
+    L = gl.size - 1
+    n = L + (L - 1) // 7
+    d = {key: dev(gt[key]) for key in ("prefixes", "childOffsets", "parents", "leafToInternal", "levelRange")}
+    gdx = [dev(a) for a in gx[:4]]
+    lay = dev(glay)
+    cen, mp, mp32 = zeros(4 * n, torch.float64), zeros(8 * n, torch.float64), zeros(8 * n, torch.float32)
+    ctx.call("ryo_upsweep_f64", *(R.ptr(a) for a in gdx), R.ptr(lay), L, R.ptr(d["prefixes"]),
+             R.ptr(d["childOffsets"]), R.ptr(d["parents"]), R.ptr(d["leafToInternal"]), R.ptr(d["levelRange"]),
+             R.host_f64(lim), 2.0, R.ptr(cen), R.ptr(mp), R.ptr(mp32))
+    gcen, gmp = host(cen).reshape(n, 4), host(mp).reshape(n, 8)
+    assert np.allclose(gcen[:, :3], c64[:, :3], rtol=1e-12, atol=1e-14 * np.abs(c64[:, :3]).max())
+    assert np.allclose(gcen[:, 3], c64[:, 3], rtol=1e-12, atol=0)
+    assert np.allclose(gmp, q64, rtol=1e-10, atol=1e-13 * np.abs(q64).max())
+    assert np.array_equal(host(mp32).reshape(n, 8), gmp.astype(np.float32))
+    t = {"tree": {"childOffsets": gt["childOffsets"], "internalToLeaf": gt["internalToLeaf"]}, "layout": glay}
+    gtot, gp, gax, gay, gaz, _ = _gpu64(ctx, t, gx[:3], gx[4], gx[3], gcen, gmp, 16, mixed=False)
+    e = np.sort(rel_err(gax, gay, gaz, rax, ray, raz))
+    assert e[int(0.99 * e.size)] <= 1e-10, e[int(0.99 * e.size)]
+    assert abs(gtot - rtot) <= 1e-10 * abs(rtot)
