@@ -247,6 +247,21 @@ inline void upsweep(const float* x, const float* y, const float* z, const float*
                                        nullptr, nullptr));
 }
 
+//! the same fused FP64 upsweep for double particles and Box<double>: SourceCenterType<double> centers and
+//! CartesianQuadrupole<double> multipoles (upsweep_gpu.cu:134-143)
+inline void upsweep(const double* x, const double* y, const double* z, const double* m, const LocalIndex* layout,
+                    int numLeaves, const uint64_t* prefixes, const TreeNodeIndex* childOffsets,
+                    const TreeNodeIndex* parents, const TreeNodeIndex* leafToInternal, const TreeNodeIndex* d_levelRange,
+                    const cstone::Box<double>& box, float theta, Vec4<double>* centers,
+                    CartesianQuadrupole<double>* multipoles)
+{
+    const TreeNodeIndex* full = leafToInternal - (numLeaves - 1) / 7;
+    ryoanji_amd::check(ryo_upsweep_f64(ryoanji_amd::ctx(), x, y, z, m, layout, numLeaves, prefixes, childOffsets,
+                                       parents, full, d_levelRange, box.limitsT(), 1.0f / theta,
+                                       reinterpret_cast<double*>(centers), reinterpret_cast<double*>(multipoles),
+                                       nullptr));
+}
+
 //! ryoanji::traverse -- nbody/traversal_gpu.h:51-56, float instantiation; gmPool is not needed (LDS stacks)
 inline double traverse(GroupView grp, int initNodeIdx, const float* xt, const float* yt, const float* zt,
                        const float* mt, const float* ht, const float* xs, const float* ys, const float* zs,
