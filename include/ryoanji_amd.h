@@ -72,6 +72,13 @@ int ryo_sfc_keys_f32(ryo_ctx* ctx, const float* x, const float* y, const float* 
  * limits = host double[6]; otherwise as ryo_sfc_keys_f32 (mixed-dimension keys for non-cubic boxes included). */
 int ryo_sfc_keys_f64(ryo_ctx* ctx, const double* x, const double* y, const double* z, uint64_t* keys, size_t n,
                      const double* limits, int kind, int skip_removed);
+/* computeSfcKeys with 32-bit keys (HilbertKey<uint32_t> / MortonKey<uint32_t>, sfc_gpu.cu:46-62): 10 levels,
+ * box bits reduced from maxTreeLevel<uint32_t> = 10 (sfc.hpp:89-96), removeKey 2^30 for skip_removed; float or
+ * double coordinates with a box of the same type. */
+int ryo_sfc_keys_u32_f32(ryo_ctx* ctx, const float* x, const float* y, const float* z, uint32_t* keys, size_t n,
+                         const float* limits, int kind, int skip_removed);
+int ryo_sfc_keys_u32_f64(ryo_ctx* ctx, const double* x, const double* y, const double* z, uint32_t* keys, size_t n,
+                         const double* limits, int kind, int skip_removed);
 
 /* ---------------------------------------------------------------- a3: stable radix sort */
 

@@ -147,6 +147,18 @@ inline void computeSfcKeys(execution::Gpu, const float* x, const float* y, const
     ryoanji_amd::check(ryo_sfc_keys_f32(ryoanji_amd::ctx(), x, y, z, keys, n, box.limits(), 0, 1));
 }
 
+//! cstone::computeSfcKeys with 32-bit Hilbert keys (10 levels) for float and double boxes (sfc_gpu.cu:46-62)
+inline void computeSfcKeys(execution::Gpu, const float* x, const float* y, const float* z, uint32_t* keys, size_t n,
+                           const Box<float>& box)
+{
+    ryoanji_amd::check(ryo_sfc_keys_u32_f32(ryoanji_amd::ctx(), x, y, z, keys, n, box.limitsT(), 0, 1));
+}
+inline void computeSfcKeys(execution::Gpu, const double* x, const double* y, const double* z, uint32_t* keys,
+                           size_t n, const Box<double>& box)
+{
+    ryoanji_amd::check(ryo_sfc_keys_u32_f64(ryoanji_amd::ctx(), x, y, z, keys, n, box.limitsT(), 0, 1));
+}
+
 //! cstone::computeSfcKeys for double coordinates and Box<double> (sfc_gpu.cu:46-62)
 inline void computeSfcKeys(execution::Gpu, const double* x, const double* y, const double* z, uint64_t* keys,
                            size_t n, const Box<double>& box)

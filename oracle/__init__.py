@@ -92,6 +92,8 @@ def lib(which: str = "orc") -> C.CDLL:
             "compute_gravity_f64": (C.c_double, [_P, C.c_int, _P, _P, _P, _P, _P, _P, _P, _P, _P, C.c_float,
                                                  C.c_int, C.c_int, C.c_int, _P, _P, _P, _P]),
             "sfc_keys_f64": (C.c_int, [_P, _P, _P, _P, C.c_size_t, _P, C.c_int]),
+            "sfc_keys_u32": (C.c_int, [_P, _P, _P, _P, C.c_size_t, _P, C.c_int]),
+            "sfc_keys_u32_f64": (C.c_int, [_P, _P, _P, _P, C.c_size_t, _P, C.c_int]),
             "compute_gravity_df": (C.c_double, [_P, C.c_int, _P, _P, _P, _P, _P, _P, _P, _P, _P, C.c_float,
                                                 C.c_int, C.c_int, C.c_int, _P, _P, _P, _P]),
             "direct_sum_f64": (C.c_int, [_P, _P, _P, _P, _P, C.c_size_t, C.c_double, _P, C.c_int, _P, _P, _P,
@@ -266,6 +268,16 @@ def ref_sfc_keys64(x, y, z, limits, hilbert=True):
     keys = np.zeros(x.size, np.uint64)
     _f("ref", "sfc_keys_f64")(_ptr(x), _ptr(y), _ptr(z), _ptr(keys), x.size, _ptr(_c(limits, np.float64)),
                               int(hilbert))
+    return keys
+
+
+def ref_sfc_keys32(x, y, z, limits, hilbert=True, precision="f32"):
+    """the reference computeSfcKeys with 32-bit keys (10 levels) for float or double coordinates and box"""
+    dt = np.float32 if precision == "f32" else np.float64
+    x, y, z = (_c(a, dt) for a in (x, y, z))
+    keys = np.zeros(x.size, np.uint32)
+    _f("ref", "sfc_keys_u32" if precision == "f32" else "sfc_keys_u32_f64")(
+        _ptr(x), _ptr(y), _ptr(z), _ptr(keys), x.size, _ptr(_c(limits, dt)), int(hilbert))
     return keys
 
 

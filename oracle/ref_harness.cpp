@@ -62,6 +62,25 @@ int ref_sfc_keys_f64(const double* x, const double* y, const double* z, uint64_t
     return 0;
 }
 
+/* 32-bit keys (HilbertKey<uint32_t> / MortonKey<uint32_t>) of float or double coordinates */
+int ref_sfc_keys_u32(const float* x, const float* y, const float* z, uint32_t* keys, size_t n, const float* limits,
+                     int hilbert)
+{
+    auto box = box6(limits);
+    if (hilbert) computeSfcKeys(x, y, z, reinterpret_cast<HilbertKey<uint32_t>*>(keys), n, box);
+    else computeSfcKeys(x, y, z, reinterpret_cast<MortonKey<uint32_t>*>(keys), n, box);
+    return 0;
+}
+
+int ref_sfc_keys_u32_f64(const double* x, const double* y, const double* z, uint32_t* keys, size_t n,
+                         const double* limits, int hilbert)
+{
+    auto box = box6(limits);
+    if (hilbert) computeSfcKeys(x, y, z, reinterpret_cast<HilbertKey<uint32_t>*>(keys), n, box);
+    else computeSfcKeys(x, y, z, reinterpret_cast<MortonKey<uint32_t>*>(keys), n, box);
+    return 0;
+}
+
 void ref_sort_by_key(uint64_t* keys, int32_t* values, size_t n) { sort_by_key(keys, keys + n, values); }
 
 long ref_compute_octree(const uint64_t* sortedKeys, size_t n, unsigned bucket, uint64_t* leaves, unsigned* counts,

@@ -91,6 +91,8 @@ SIGNATURES = {
     "ryo_ctx_set_traversal_shape": (_I, [_P, _I, _I, _I]),
     "ryo_last_traverse_ms": (_D, [_P]),
     "ryo_sfc_keys_f32": (_I, [_P, _P, _P, _P, _P, _SZ, _P, _I, _I]),
+    "ryo_sfc_keys_u32_f32": (_I, [_P, _P, _P, _P, _P, _SZ, _P, _I, _I]),
+    "ryo_sfc_keys_u32_f64": (_I, [_P, _P, _P, _P, _P, _SZ, _P, _I, _I]),
     "ryo_sfc_keys_f64": (_I, [_P, _P, _P, _P, _P, _SZ, _P, _I, _I]),
     "ryo_sort_temp_bytes": (_SZ, [_SZ]),
     "ryo_sort_pairs_u64_i32": (_I, [_P, _P, _P, _SZ, _P, _P, _P, _SZ]),

@@ -12,6 +12,8 @@ namespace ryo
 // stage implementations (keys_sort.hip, tree.hip, upsweep.hip, traverse.hip, ewald_direct.hip)
 int sfcKeys(Context*, const float*, const float*, const float*, uint64_t*, size_t, const float*, int, int);
 int sfcKeys64(Context*, const double*, const double*, const double*, uint64_t*, size_t, const double*, int, int);
+template<class T>
+int sfcKeys32(Context*, const T*, const T*, const T*, uint32_t*, size_t, const T*, int, int);
 size_t sortTempBytes(size_t n);
 int sortPairs(Context*, uint64_t*, int32_t*, size_t, uint64_t*, int32_t*, void*, size_t);
 int sequence(Context*, int32_t*, size_t, int32_t);
@@ -183,6 +185,22 @@ int ryo_sfc_keys_f32(ryo_ctx* h, const float* x, const float* y, const float* z,
     CTX_OR_FAIL(h);
     if (!limits || (n && (!x || !y || !z || !keys))) return setError(ctx, RYO_ERR_ARG, "sfc keys: null pointer");
     return sfcKeys(ctx, x, y, z, keys, n, limits, kind, skip_removed);
+}
+
+int ryo_sfc_keys_u32_f32(ryo_ctx* h, const float* x, const float* y, const float* z, uint32_t* keys, size_t n,
+                         const float* limits, int kind, int skip_removed)
+{
+    CTX_OR_FAIL(h);
+    if (!limits || (n && (!x || !y || !z || !keys))) return setError(ctx, RYO_ERR_ARG, "sfc keys: null pointer");
+    return sfcKeys32(ctx, x, y, z, keys, n, limits, kind, skip_removed);
+}
+
+int ryo_sfc_keys_u32_f64(ryo_ctx* h, const double* x, const double* y, const double* z, uint32_t* keys, size_t n,
+                         const double* limits, int kind, int skip_removed)
+{
+    CTX_OR_FAIL(h);
+    if (!limits || (n && (!x || !y || !z || !keys))) return setError(ctx, RYO_ERR_ARG, "sfc keys: null pointer");
+    return sfcKeys32(ctx, x, y, z, keys, n, limits, kind, skip_removed);
 }
 
 int ryo_sfc_keys_f64(ryo_ctx* h, const double* x, const double* y, const double* z, uint64_t* keys, size_t n,

@@ -56,12 +56,16 @@ struct PackOut
     float4* rec;
 };
 
-template<class T, bool Cubic, bool Pack>
+// KeyT: 64-bit keys (21 levels) or 32-bit keys (10 levels, HilbertKey<uint32_t> / MortonKey<uint32_t>); the
+// removal sentinel is 2^(3 levels) (removeKey, tree/definitions.h:71-75)
+template<class T, bool Cubic, bool Pack, class KeyT = uint64_t>
 __global__ __launch_bounds__(256) void sfcKeysKernel(const T* __restrict__ x, const T* __restrict__ y,
-                                                     const T* __restrict__ z, uint64_t* __restrict__ keys,
+                                                     const T* __restrict__ z, KeyT* __restrict__ keys,
                                                      size_t n, KeyBoxT<T> box, int kind, int skipRemoved, PackOut pk)
 {
     static_assert(!Pack || sizeof(T) == 4, "the gather records hold float coordinates");
+    static_assert(!Cubic || sizeof(KeyT) == 8, "the Hilbert tables encode 21 levels");
+    constexpr KeyT kRemove = KeyT(1) << (sizeof(KeyT) == 8 ? 63 : 30);
     // cubic boxes: the Hilbert state tables in LDS (6.9 KB), read with per-lane indices
     __shared__ HilbertLut lut;
     if constexpr (Cubic)
@@ -77,7 +81,7 @@ __global__ __launch_bounds__(256) void sfcKeysKernel(const T* __restrict__ x, co
               mz = int((1u << box.bits.b[2]) - 1);
     for (size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += size_t(gridDim.x) * blockDim.x)
     {
-        if (!Pack && skipRemoved && keys[i] == kKeyEnd) continue;
+        if (!Pack && skipRemoved && keys[i] == kRemove) continue;
         const T px = x[i], py = y[i], pz = z[i];
         int ix = min(cellIndex(px, box.lo[0], box.m[0]), mx);
         int iy = min(cellIndex(py, box.lo[1], box.m[1]), my);
@@ -89,7 +93,7 @@ __global__ __launch_bounds__(256) void sfcKeysKernel(const T* __restrict__ x, co
             k = hilbert3DLut(ix, iy, iz, lut.two, lut.one);
         else
             k = hilbertMixD(ix, iy, iz, box.bits);
-        keys[i] = k;
+        keys[i] = KeyT(k);
         if constexpr (Pack)
         {
             pk.perm[i]       = int32_t(i);
@@ -538,9 +542,9 @@ int segmentedSortPairs(Context* ctx, uint64_t* keys, int32_t* vals, size_t n, co
 }
 
 template<class T>
-static int makeKeyBox(Context* ctx, const T* limits, KeyBoxT<T>& b)
+static int makeKeyBox(Context* ctx, const T* limits, KeyBoxT<T>& b, unsigned maxLevel = kMaxLevel)
 {
-    b.bits = boxBits(limits); // box.hpp:160-198: an axis shorter than 0.75 x the longest one gets fewer bits
+    b.bits = boxBits(limits, maxLevel); // box.hpp:160-198: an axis shorter than 0.75 x the longest gets fewer bits
     for (int d = 0; d < 3; ++d)
     {
         T lo = limits[2 * d], hi = limits[2 * d + 1];
@@ -581,6 +585,24 @@ int sfcKeys64(Context* ctx, const double* x, const double* y, const double* z, u
 {
     return sfcKeysT(ctx, x, y, z, keys, n, limits, kind, skipRemoved);
 }
+
+// 32-bit keys: 10 levels, the box bits reduced from 10 (maxTreeLevel<uint32_t>, sfc.hpp:89-96)
+template<class T>
+int sfcKeys32(Context* ctx, const T* x, const T* y, const T* z, uint32_t* keys, size_t n, const T* limits, int kind,
+              int skipRemoved)
+{
+    if (n == 0) return RYO_OK;
+    KeyBoxT<T> b;
+    if (int rc = makeKeyBox(ctx, limits, b, 10u)) return rc;
+    sfcKeysKernel<T, false, false, uint32_t><<<gridFor(n, 256, ctx->numCUs * 16), 256, 0, ctx->stream>>>(
+        x, y, z, keys, n, b, kind, skipRemoved, PackOut{});
+    RYO_LAUNCH_CHECK(ctx);
+    return RYO_OK;
+}
+template int sfcKeys32<float>(Context*, const float*, const float*, const float*, uint32_t*, size_t, const float*, int,
+                              int);
+template int sfcKeys32<double>(Context*, const double*, const double*, const double*, uint32_t*, size_t,
+                               const double*, int, int);
 
 // Hilbert keys of a freshly loaded system together with the identity permutation and the gather records of
 // in = {x, y, z, m, h, o} (the records stay in the context's pack scratch until gatherRecords)

@@ -432,7 +432,7 @@ struct MacBox
 // per-axis key bits of a box given as {xmin, xmax, ymin, ymax, zmin, zmax} (Box<T>::computeBoxDimBits and
 // getBoxDimBits, box.hpp:160-198, in the box's type T as the reference's Box<T> computes them)
 template<class T>
-inline Bits3 boxBits(const T* limits)
+inline Bits3 boxBits(const T* limits, unsigned maxLevel = kMaxLevel)
 {
     T len[3] = {limits[1] - limits[0], limits[3] - limits[2], limits[5] - limits[4]};
     T mx     = std::max(std::max(len[0], len[1]), len[2]);
@@ -440,7 +440,7 @@ inline Bits3 boxBits(const T* limits)
     for (int d = 0; d < 3; ++d)
     {
         unsigned red = unsigned(std::floor(std::log2(mx / len[d]) + T(0.5849625007211563)));
-        r.b[d]       = red >= kMaxLevel ? 0u : kMaxLevel - red;
+        r.b[d]       = red >= maxLevel ? 0u : maxLevel - red;
     }
     return r;
 }

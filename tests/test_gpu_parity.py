@@ -273,3 +273,39 @@ def test_keys_f64_bit_exact(ctx, name):
         g = host(k, np.uint64)
         o = O.ref_sfc_keys64(*xyz, lim, hilbert=hil)
         assert np.array_equal(g, o), f"{name} kind {kind}: {np.count_nonzero(g != o)} keys differ"
+
+
+@pytest.mark.parametrize("name", list(SYSTEMS))
+@pytest.mark.parametrize("precision", ["f32", "f64"])
+def test_keys_u32_bit_exact(ctx, name, precision):
+    """32-bit keys (10 levels) of float or double coordinates equal the reference's computeSfcKeys with
+    HilbertKey<uint32_t> / MortonKey<uint32_t> (oracle/_ref), box faces included; keys equal to removeKey (2^30)
+    are left untouched with skip_removed"""
+    import torch
+
+    s = SYSTEMS[name]()
+    dt = np.float32 if precision == "f32" else np.float64
+    lim = s.limits.astype(dt)
+    xyz = []
+    for d, a in enumerate((s.x, s.y, s.z)):
+        v = a.astype(dt).copy()
+        v[:3] = (lim[2 * d], lim[2 * d + 1], dt(0.5) * (lim[2 * d] + lim[2 * d + 1]))
+        xyz.append(v)
+    host_lim = R.host_f32(lim) if precision == "f32" else R.host_f64(lim)
+    for kind, hil in ((0, True), (1, False)):
+        pos = [dev(a) for a in xyz]
+        k = zeros(s.n, torch.int32)
+        ctx.call(f"ryo_sfc_keys_u32_{precision}", *(R.ptr(a) for a in pos), R.ptr(k), s.n, host_lim, kind, 0)
+        g = host(k, np.uint32)
+        o = O.ref_sfc_keys32(*xyz, lim, hilbert=hil, precision=precision)
+        assert np.array_equal(g, o), f"{name} {precision} kind {kind}: {np.count_nonzero(g != o)} keys differ"
+    # removeKey<uint32_t> entries are skipped
+    pre = np.zeros(s.n, np.uint32)
+    pre[::5] = np.uint32(1 << 30)
+    k = dev(pre)
+    ctx.call(f"ryo_sfc_keys_u32_{precision}", *(R.ptr(a) for a in pos), R.ptr(k), s.n, host_lim, 0, 1)
+    g = host(k, np.uint32)
+    assert np.all(g[::5] == np.uint32(1 << 30))
+    mask = np.ones(s.n, bool)
+    mask[::5] = False
+    assert np.array_equal(g[mask], O.ref_sfc_keys32(*xyz, lim, precision=precision)[mask])
