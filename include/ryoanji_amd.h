@@ -139,6 +139,15 @@ int ryo_upsweep_f32(ryo_ctx* ctx, const float* x, const float* y, const float* z
                     const int32_t* parents, const int32_t* leaf_to_internal, const int32_t* level_range,
                     const float* limits, float inv_theta, float* centers, float* multipoles, double* centers64,
                     double* multipoles64);
+/* The upsweep with CartesianMDQpole<float> multipoles (P2M / addQuadrupole / M2M for CartesianMDQpole,
+ * cartesian_qpole.hpp:237-417): multipoles float[numNodes][12] {M, qxx, qxy, qxz, qyy, qyz, qzz, trace, px, py,
+ * pz, 0}, the dipole p = sum m (r - com) taken about the |m|-weighted center (nonzero with signed masses);
+ * multipoles64 (optional) double[numNodes][12]. Otherwise as ryo_upsweep_f32. */
+int ryo_upsweep_mdq_f32(ryo_ctx* ctx, const float* x, const float* y, const float* z, const float* m,
+                        const uint32_t* layout, int num_leaves, const uint64_t* prefixes, const int32_t* child_offsets,
+                        const int32_t* parents, const int32_t* leaf_to_internal, const int32_t* level_range,
+                        const float* limits, float inv_theta, float* centers, float* multipoles, double* centers64,
+                        double* multipoles64);
 /* The same upsweep for double particles and a Box<double> (computeLeafMultipoles / upsweepMultipoles / setMac for
  * T = double, upsweep_gpu.cu:134-143): centers double[numNodes][4] {com, mac^2} (SourceCenterType<double>),
  * multipoles double[numNodes][8] (CartesianQuadrupole<double>), multipoles32 (optional, NULL: not written) the
@@ -187,6 +196,17 @@ int ryo_traverse_f32(ryo_ctx* ctx, int num_groups, const uint32_t* group_start, 
                      const uint32_t* layout, const float* centers, const float* multipoles, float G, int num_shells,
                      const float* box_l /* host float[3] */, float* p, float* ax, float* ay, float* az,
                      double* total_pot);
+
+/* ryoanji::traverse with CartesianMDQpole<float> multipoles (float[n][12] from ryo_upsweep_mdq_f32): M2P adds the
+ * dipole terms (M2P for CartesianMDQpole, cartesian_qpole.hpp:320-357); otherwise as ryo_traverse_f32
+ * (no batched form). */
+int ryo_traverse_mdq_f32(ryo_ctx* ctx, int num_groups, const uint32_t* group_start, const uint32_t* group_end,
+                         int init_node, const float* xt, const float* yt, const float* zt, const float* mt,
+                         const float* ht, const float* xs, const float* ys, const float* zs, const float* ms,
+                         const float* hs, const int32_t* child_offsets, const int32_t* internal_to_leaf,
+                         const uint32_t* layout, const float* centers, const float* multipoles, float G,
+                         int num_shells, const float* box_l, float* p, float* ax, float* ay, float* az,
+                         double* total_pot);
 
 /* ryoanji::traverse for the (Tc,Th,Tm,Ta,Tf) = (d,d,d,d,d) instantiation with CartesianQuadrupole<double>
  * (traversal_gpu.cu:635-641): double coordinates, softenings, masses and outputs; centers double[n][4]

@@ -93,6 +93,10 @@ def lib(which: str = "orc") -> C.CDLL:
                                                  C.c_int, C.c_int, C.c_int, _P, _P, _P, _P]),
             "sfc_keys_f64": (C.c_int, [_P, _P, _P, _P, C.c_size_t, _P, C.c_int]),
             "sfc_keys_u32": (C.c_int, [_P, _P, _P, _P, C.c_size_t, _P, C.c_int]),
+            "upsweep_mdq_f32": (C.c_int, [_P, _P, _P, _P, _P, C.c_int, _P, _P, C.c_float, _P, _P]),
+            "upsweep_mdq_f64": (C.c_int, [_P, _P, _P, _P, _P, C.c_int, _P, _P, C.c_float, _P, _P]),
+            "compute_gravity_mdq_f32": (C.c_double, [_P, C.c_int, _P, _P, _P, _P, _P, _P, _P, _P, _P, C.c_float,
+                                                     C.c_int, C.c_int, C.c_int, _P, _P, _P, _P]),
             "sfc_keys_u32_f64": (C.c_int, [_P, _P, _P, _P, C.c_size_t, _P, C.c_int]),
             "compute_gravity_df": (C.c_double, [_P, C.c_int, _P, _P, _P, _P, _P, _P, _P, _P, _P, C.c_float,
                                                 C.c_int, C.c_int, C.c_int, _P, _P, _P, _P]),
@@ -279,6 +283,37 @@ def ref_sfc_keys32(x, y, z, limits, hilbert=True, precision="f32"):
     _f("ref", "sfc_keys_u32" if precision == "f32" else "sfc_keys_u32_f64")(
         _ptr(x), _ptr(y), _ptr(z), _ptr(keys), x.size, _ptr(_c(limits, dt)), int(hilbert))
     return keys
+
+
+def ref_upsweep_mdq(x, y, z, m, leaves, layout, limits, inv_theta, precision="f32"):
+    """the reference's upsweep with CartesianMDQpole (monopole, dipole, quadrupole): centers [n,4] (x,y,z,mac^2) and
+    multipoles [n,12] (M, qxx, qxy, qxz, qyy, qyz, qzz, trace, px, py, pz, 0)"""
+    dt = np.float32 if precision == "f32" else np.float64
+    leaves = _c(leaves, np.uint64)
+    layout = _c(layout, np.uint32)
+    L = leaves.size - 1
+    n = L + (L - 1) // 7
+    xs = [_c(a, dt) for a in (x, y, z, m)]
+    centers = np.zeros((n, 4), dt)
+    mp = np.zeros((n, 12), dt)
+    _f("ref", "upsweep_mdq_" + precision)(*(_ptr(a) for a in xs), _ptr(leaves), L, _ptr(layout),
+                                          _ptr(_c(limits, dt)), float(inv_theta), _ptr(centers), _ptr(mp))
+    return centers, mp
+
+
+def ref_compute_gravity_mdq(tree_leaves, layout, centers, mp, x, y, z, h, m, limits, G=1.0, num_shells=0):
+    """the reference computeGravity (16-target groups) with CartesianMDQpole<float> -> (pot, ugrav, ax, ay, az)"""
+    leaves = _c(tree_leaves, np.uint64)
+    layout = _c(layout, np.uint32)
+    L = leaves.size - 1
+    arrs = [_c(a, np.float32) for a in (x, y, z, h, m)]
+    n = arrs[0].size
+    ug, ax, ay, az = (np.zeros(n, np.float32) for _ in range(4))
+    tot = _f("ref", "compute_gravity_mdq_f32")(_ptr(leaves), L, _ptr(layout), _ptr(_c(centers, np.float32)),
+                                               _ptr(_c(mp, np.float32)), *(_ptr(a) for a in arrs),
+                                               _ptr(_c(limits, np.float32)), float(G), num_shells, 0, L, _ptr(ug),
+                                               _ptr(ax), _ptr(ay), _ptr(az))
+    return tot, ug, ax, ay, az
 
 
 def ref_compute_gravity64(tree_leaves, layout, centers, mp, x, y, z, h, m, limits, G=1.0, num_shells=0, mixed=False):

@@ -110,7 +110,7 @@ int ref_link_octree(const uint64_t* leaves, int numLeaves, uint64_t* prefixes, i
 
 } // extern "C"
 
-template<class T>
+template<class T, class MType = ryoanji::CartesianQuadrupole<T>>
 static void upsweepImpl(const T* x, const T* y, const T* z, const T* m, const uint64_t* leaves, int numLeaves,
                         const unsigned* layout, const T* limits, float invTheta, T* centers, T* mpoles)
 {
@@ -119,7 +119,7 @@ static void upsweepImpl(const T* x, const T* y, const T* z, const T* m, const ui
     auto box = box6(limits);
     size_t n = layout[numLeaves];
     auto c   = reinterpret_cast<SourceCenterType<T>*>(centers);
-    auto mp  = reinterpret_cast<ryoanji::CartesianQuadrupole<T>*>(mpoles);
+    auto mp  = reinterpret_cast<MType*>(mpoles);
     computeLeafMassCenter<T, T, T>(std::span<const T>(x, n), std::span<const T>(y, n), std::span<const T>(z, n),
                                    std::span<const T>(m, n), leafToInternal(o), layout, c);
     upsweep(std::span<const TreeNodeIndex>(o.levelRange), o.childOffsets.data(), c, CombineSourceCenter<T>{});
@@ -145,6 +145,43 @@ int ref_upsweep_f64(const double* x, const double* y, const double* z, const dou
 {
     upsweepImpl<double>(x, y, z, m, leaves, numLeaves, layout, limits, invTheta, centers, mpoles);
     return 0;
+}
+
+/* the same with CartesianMDQpole<T> (monopole, dipole, quadrupole: 12 values per node, cartesian_qpole.hpp:237-380) */
+int ref_upsweep_mdq_f32(const float* x, const float* y, const float* z, const float* m, const uint64_t* leaves,
+                        int numLeaves, const unsigned* layout, const float* limits, float invTheta, float* centers,
+                        float* mpoles)
+{
+    upsweepImpl<float, ryoanji::CartesianMDQpole<float>>(x, y, z, m, leaves, numLeaves, layout, limits, invTheta,
+                                                         centers, mpoles);
+    return 0;
+}
+
+int ref_upsweep_mdq_f64(const double* x, const double* y, const double* z, const double* m, const uint64_t* leaves,
+                        int numLeaves, const unsigned* layout, const double* limits, float invTheta, double* centers,
+                        double* mpoles)
+{
+    upsweepImpl<double, ryoanji::CartesianMDQpole<double>>(x, y, z, m, leaves, numLeaves, layout, limits, invTheta,
+                                                           centers, mpoles);
+    return 0;
+}
+
+/* computeGravity with CartesianMDQpole<float> multipoles, all float */
+double ref_compute_gravity_mdq_f32(const uint64_t* leaves, int numLeaves, const unsigned* layout,
+                                   const float* centers, const float* mpoles, const float* x, const float* y,
+                                   const float* z, const float* h, const float* m, const float* limits, float G,
+                                   int numShells, int firstLeaf, int lastLeaf, float* ugrav, float* ax, float* ay,
+                                   float* az)
+{
+    Tree t(leaves, numLeaves);
+    auto& o  = t.o;
+    auto box = box6(limits);
+    float tot = 0;
+    ryoanji::computeGravity(o.childOffsets.data(), o.parents.data(), o.internalToLeaf.data(),
+                            reinterpret_cast<const SourceCenterType<float>*>(centers),
+                            reinterpret_cast<const ryoanji::CartesianMDQpole<float>*>(mpoles), layout, firstLeaf,
+                            lastLeaf, x, y, z, h, m, box, G, ugrav, ax, ay, az, &tot, numShells);
+    return tot;
 }
 
 /* computeGravity (traversal_cpu.hpp:172-230) over the leaf range [firstLeaf, lastLeaf); the reference groups

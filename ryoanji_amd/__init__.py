@@ -102,11 +102,14 @@ SIGNATURES = {
     "ryo_build_leaves_u64": (_I, [_P, _P, _SZ, _U, _P, _P, _P, _SZ, C.POINTER(_I)]),
     "ryo_link_octree_u64": (_I, [_P, _P, _I, _P, _P, _P, _P, _P, _P, _P]),
     "ryo_upsweep_f32": (_I, [_P, _P, _P, _P, _P, _P, _I, _P, _P, _P, _P, _P, _P, _F, _P, _P, _P, _P]),
+    "ryo_upsweep_mdq_f32": (_I, [_P, _P, _P, _P, _P, _P, _I, _P, _P, _P, _P, _P, _P, _F, _P, _P, _P, _P]),
     "ryo_upsweep_f64": (_I, [_P, _P, _P, _P, _P, _P, _I, _P, _P, _P, _P, _P, _P, _F, _P, _P, _P]),
     "ryo_fixed_groups": (_I, [_P, C.c_uint32, C.c_uint32, _U, _P, C.POINTER(_I)]),
     "ryo_group_splits_f32": (_I, [_P, C.c_uint32, C.c_uint32, _P, _P, _P, _P, _P, _I, _P, _P, _U, _F, _F, _P,
                                   C.c_size_t, C.POINTER(_I)]),
     "ryo_traverse_f32": (_I, [_P, _I, _P, _P, _I] + [_P] * 10 + [_P, _P, _P, _P, _P, _F, _I, _P, _P, _P, _P, _P,
+                                                                   _P]),
+    "ryo_traverse_mdq_f32": (_I, [_P, _I, _P, _P, _I] + [_P] * 10 + [_P, _P, _P, _P, _P, _F, _I, _P, _P, _P, _P, _P,
                                                                    _P]),
     "ryo_traverse_f64": (_I, [_P, _I, _P, _P, _I] + [_P] * 10 + [_P, _P, _P, _P, _P, _D, _I, _P, _P, _P, _P, _P,
                                                                    _P]),

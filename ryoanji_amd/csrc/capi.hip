@@ -26,6 +26,9 @@ int linkOctree(Context*, const uint64_t*, int, uint64_t*, int32_t*, int32_t*, in
 int upsweep(Context*, const float*, const float*, const float*, const float*, const uint32_t*, int, const uint64_t*,
             const int32_t*, const int32_t*, const int32_t*, const int32_t*, const float*, float, float*, float*,
             double*, double*, double);
+int upsweepMDQ(Context*, const float*, const float*, const float*, const float*, const uint32_t*, int,
+               const uint64_t*, const int32_t*, const int32_t*, const int32_t*, const int32_t*, const float*, float,
+               float*, float*, double*, double*);
 int upsweep64(Context*, const double*, const double*, const double*, const double*, const uint32_t*, int,
               const uint64_t*, const int32_t*, const int32_t*, const int32_t*, const int32_t*, const double*, float,
               double*, double*, float*);
@@ -51,7 +54,7 @@ int traverseEntry(Context* ctx, int numGroups, const uint32_t* gs, const uint32_
                   const float* ys, const float* zs, const float* ms, const float* hs, const int32_t* child,
                   const int32_t* i2l, const uint32_t* layout, const float* centers, const float* mp, float G,
                   int numShells, const float* boxL, float* p, float* ax, float* ay, float* az, double* totalPot,
-                  int stackCap, size_t hCount);
+                  int stackCap, size_t hCount, bool mdq = false);
 
 int setError(Context* ctx, int code, const char* fmt, ...)
 {
@@ -267,6 +270,17 @@ int ryo_upsweep_f32(ryo_ctx* h, const float* x, const float* y, const float* z, 
                    inv_theta, centers, mp, c64, q64, 0.0);
 }
 
+int ryo_upsweep_mdq_f32(ryo_ctx* h, const float* x, const float* y, const float* z, const float* m,
+                        const uint32_t* layout, int num_leaves, const uint64_t* prefixes, const int32_t* child,
+                        const int32_t* parents, const int32_t* l2i, const int32_t* level_range, const float* limits,
+                        float inv_theta, float* centers, float* mp, double* c64, double* q64)
+{
+    CTX_OR_FAIL(h);
+    if (!limits || !centers || !mp) return setError(ctx, RYO_ERR_ARG, "upsweep: null pointer");
+    return upsweepMDQ(ctx, x, y, z, m, layout, num_leaves, prefixes, child, parents, l2i, level_range, limits,
+                      inv_theta, centers, mp, c64, q64);
+}
+
 int ryo_upsweep_f64(ryo_ctx* h, const double* x, const double* y, const double* z, const double* m,
                     const uint32_t* layout, int num_leaves, const uint64_t* prefixes, const int32_t* child,
                     const int32_t* parents, const int32_t* l2i, const int32_t* level_range, const double* limits,
@@ -308,6 +322,20 @@ int ryo_traverse_f32(ryo_ctx* h, int num_groups, const uint32_t* gs, const uint3
     if (num_shells < 0 || (num_shells > 0 && !box_l)) return setError(ctx, RYO_ERR_ARG, "traverse: bad shells");
     return traverseEntry(ctx, num_groups, gs, ge, init_node, xt, yt, zt, mt, ht, xs, ys, zs, ms, hs, child, i2l,
                          layout, centers, mp, G, num_shells, box_l, p, ax, ay, az, total_pot, 0, 0);
+}
+
+int ryo_traverse_mdq_f32(ryo_ctx* h, int num_groups, const uint32_t* gs, const uint32_t* ge, int init_node,
+                         const float* xt, const float* yt, const float* zt, const float* mt, const float* ht,
+                         const float* xs, const float* ys, const float* zs, const float* ms, const float* hs,
+                         const int32_t* child, const int32_t* i2l, const uint32_t* layout, const float* centers,
+                         const float* mp, float G, int num_shells, const float* box_l, float* p, float* ax, float* ay,
+                         float* az, double* total_pot)
+{
+    CTX_OR_FAIL(h);
+    if (init_node != 0 && init_node != 1) return setError(ctx, RYO_ERR_ARG, "traverse: init_node must be 0 or 1");
+    if (num_shells < 0 || (num_shells > 0 && !box_l)) return setError(ctx, RYO_ERR_ARG, "traverse: bad shells");
+    return traverseEntry(ctx, num_groups, gs, ge, init_node, xt, yt, zt, mt, ht, xs, ys, zs, ms, hs, child, i2l,
+                         layout, centers, mp, G, num_shells, box_l, p, ax, ay, az, total_pot, 0, 0, true);
 }
 
 /* test hook: the traversal with a reduced per-wave stack capacity (forces the split/retry path) */

@@ -56,6 +56,7 @@ struct TravArgs
     int stackCap;
     size_t hCount; // targets and sources are h[0..hCount) (resident systems), 0 when unknown
     int ranges;    // contiguous group ranges, one per XCD (1: a single global counter)
+    bool mdq;      // multipoles are CartesianMDQpole (12 floats per node: the quadrupole's 8, then px, py, pz, 0)
 };
 
 struct Acc
@@ -98,6 +99,30 @@ __device__ __forceinline__ void m2p(Acc& a, float tx, float ty, float tz, float4
     a.x = __builtin_fmaf(r5, Qrx, __builtin_fmaf(mono, rx, a.x));
     a.y = __builtin_fmaf(r5, Qry, __builtin_fmaf(mono, ry, a.y));
     a.z = __builtin_fmaf(r5, Qrz, __builtin_fmaf(mono, rz, a.z));
+}
+
+// M2P of a monopole-dipole-quadrupole (CartesianMDQpole, cartesian_qpole.hpp:318-344): the quadrupole M2P plus
+// the dipole p about the expansion center, potential -(p.r) / r^3 and field (p r^2 - 3 (p.r) r) / r^5
+__device__ __forceinline__ void m2pMDQ(Acc& a, float tx, float ty, float tz, float4 c, float4 q0, float4 q1,
+                                       float4 d)
+{
+    float rx = tx - c.x, ry = ty - c.y, rz = tz - c.z;
+    float r2  = rx * rx + ry * ry + rz * rz;
+    float r1  = rsq(r2);
+    float ri2 = r1 * r1;
+    float r5  = ri2 * ri2 * r1;
+    float M = q0.x, qxx = q0.y, qxy = q0.z, qxz = q0.w, qyy = q1.x, qyz = q1.y, qzz = q1.z;
+    float Qrx  = rx * qxx + ry * qxy + rz * qxz;
+    float Qry  = rx * qxy + ry * qyy + rz * qyz;
+    float Qrz  = rx * qxz + ry * qyz + rz * qzz;
+    float rQr  = rx * Qrx + ry * Qry + rz * Qrz;
+    float PR   = rx * d.x + ry * d.y + rz * d.z;
+    float mono = (-2.5f * rQr * r5 - M * r1) * ri2;
+    float p3   = 3.0f * PR;
+    a.p += (-M - PR * ri2) * r1 - 0.5f * r5 * rQr;
+    a.x += r5 * Qrx + mono * rx + r5 * (d.x * r2 - p3 * rx);
+    a.y += r5 * Qry + mono * ry + r5 * (d.y * r2 - p3 * ry);
+    a.z += r5 * Qrz + mono * rz + r5 * (d.z * r2 - p3 * rz);
 }
 
 // softened P2P (kernel.hpp:99-120). UniformH: every target and source has the same softening h, so
@@ -192,26 +217,35 @@ __device__ __forceinline__ unsigned applyLeaves(const TreeRef& a, uint64_t dmask
     return cnt;
 }
 
-// apply the quadrupoles of the cells flagged in `amask` (lane -> cell) in ascending lane order
+// apply the multipoles of the cells flagged in `amask` (lane -> cell) in ascending lane order
+template<bool MDQ>
 __device__ __forceinline__ unsigned applyCells(const TreeRef& a, uint64_t amask, int cell, Acc& acc, const Target& t)
 {
     unsigned cnt = __popcll(amask);
     const float* cbase = reinterpret_cast<const float*>(a.centers);
     const float* qbase = reinterpret_cast<const float*>(a.mp);
+    constexpr int stride = MDQ ? 12 : 8;
     while (amask)
     {
         int l = __ffsll((unsigned long long)amask) - 1;
         amask &= amask - 1;
         int c = readlane(cell, l);
         f4u C = uload4(cbase + 4 * c);
-        f8u Q = uload8(qbase + 8 * c);
-        m2p(acc, t.x, t.y, t.z, make_float4(C.x, C.y, C.z, C.w), make_float4(Q.s0, Q.s1, Q.s2, Q.s3),
-            make_float4(Q.s4, Q.s5, Q.s6, Q.s7));
+        f8u Q = uload8(qbase + stride * c);
+        if constexpr (MDQ)
+        {
+            f4u D = uload4(qbase + stride * c + 8);
+            m2pMDQ(acc, t.x, t.y, t.z, make_float4(C.x, C.y, C.z, C.w), make_float4(Q.s0, Q.s1, Q.s2, Q.s3),
+                   make_float4(Q.s4, Q.s5, Q.s6, Q.s7), make_float4(D.x, D.y, D.z, D.w));
+        }
+        else
+            m2p(acc, t.x, t.y, t.z, make_float4(C.x, C.y, C.z, C.w), make_float4(Q.s0, Q.s1, Q.s2, Q.s3),
+                make_float4(Q.s4, Q.s5, Q.s6, Q.s7));
     }
     return cnt;
 }
 
-template<bool UH>
+template<bool UH, bool MDQ>
 __device__ WalkOut walk(const TravArgs& ta, const TreeRef& a, int* stack, const Target& t, float tcx, float tcy,
                         float tcz, float tsx, float tsy, float tsz, Acc& acc)
 {
@@ -227,7 +261,7 @@ __device__ WalkOut walk(const TravArgs& ta, const TreeRef& a, int* stack, const 
         bool close = macClose(c, tcx, tcy, tcz, tsx, tsy, tsz);
         if (!close)
         {
-            if (c.w != 0.0f) o.m2p += applyCells(a, 1ull, 0, acc, t);
+            if (c.w != 0.0f) o.m2p += applyCells<MDQ>(a, 1ull, 0, acc, t);
             return o;
         }
         if (rootChild == 0)
@@ -284,7 +318,7 @@ __device__ WalkOut walk(const TravArgs& ta, const TreeRef& a, int* stack, const 
         sp += cnt;
         o.maxStack = max(o.maxStack, unsigned(sp));
 
-        o.m2p += applyCells(a, __ballot(approx), cell, acc, t);
+        o.m2p += applyCells<MDQ>(a, __ballot(approx), cell, acc, t);
         o.p2p += applyLeaves<UH>(a, __ballot(direct), rb, re, acc, t);
     }
     return o;
@@ -395,7 +429,7 @@ struct TravBatch
 };
 
 // 64 * Waves threads; the second bound is the minimum number of waves per SIMD: 6 (<= 80 VGPRs; more would spill)
-template<bool Batched, int Waves>
+template<bool Batched, int Waves, bool MDQ = false>
 __global__ __launch_bounds__(64 * Waves, kWavesPerCU / 4) void traverseKernelT(TravArgs a, TravBatch tb)
 {
     extern __shared__ int stackMem[];
@@ -480,8 +514,10 @@ __global__ __launch_bounds__(64 * Waves, kWavesPerCU / 4) void traverseKernelT(T
                     float sx = ix * a.boxL[0], sy = iy * a.boxL[1], sz = iz * a.boxL[2];
                     Target ts{t.x + sx, t.y + sy, t.z + sz, t.h, t.hij2u};
                     WalkOut o = uniformH
-                                    ? walk<true>(a, tr, stack, ts, tcx + sx, tcy + sy, tcz + sz, tsx, tsy, tsz, acc)
-                                    : walk<false>(a, tr, stack, ts, tcx + sx, tcy + sy, tcz + sz, tsx, tsy, tsz, acc);
+                                    ? walk<true, MDQ>(a, tr, stack, ts, tcx + sx, tcy + sy, tcz + sz, tsx, tsy, tsz,
+                                                      acc)
+                                    : walk<false, MDQ>(a, tr, stack, ts, tcx + sx, tcy + sy, tcz + sz, tsx, tsy, tsz,
+                                                       acc);
                     nM2P += o.m2p;
                     nP2P += o.p2p;
                     maxStack = max(maxStack, o.maxStack);
@@ -528,6 +564,12 @@ __global__ __launch_bounds__(64 * Waves, kWavesPerCU / 4) void traverseKernelT(T
 // waves per block of this context's traversal launches, resident blocks per CU, and the launch itself
 int travWaves(const Context* ctx) { return ctx->travWaves == 2 || ctx->travWaves == 8 ? ctx->travWaves : kTravWaves; }
 
+int travBlocks4(const Context* ctx, size_t groups)
+{
+    const int perCU = ctx->travBlocksPerCU > 0 ? ctx->travBlocksPerCU : kWavesPerCU / 4;
+    return std::max(1, std::min(ctx->numCUs * perCU, int(iceil(groups, 4))));
+}
+
 int travBlocks(const Context* ctx, size_t groups)
 {
     const int W     = travWaves(ctx);
@@ -542,6 +584,14 @@ void launchTraversal(const Context* ctx, size_t groups, int stackCap, hipStream_
     const int W       = travWaves(ctx);
     const int blocks  = travBlocks(ctx, groups);
     const size_t lds  = size_t(W) * stackCap * sizeof(int);
+    if constexpr (!Batched)
+        if (a.mdq)
+        {
+            // monopole-dipole-quadrupole multipoles: one block shape (4 waves)
+            const size_t lds4 = size_t(4) * stackCap * sizeof(int);
+            traverseKernelT<false, 4, true><<<travBlocks4(ctx, groups), 256, lds4, s>>>(a, tb);
+            return;
+        }
     if (W == 2) traverseKernelT<Batched, 2><<<blocks, 128, lds, s>>>(a, tb);
     else if (W == 8) traverseKernelT<Batched, 8><<<blocks, 512, lds, s>>>(a, tb);
     else traverseKernelT<Batched, 4><<<blocks, 256, lds, s>>>(a, tb);
@@ -859,7 +909,7 @@ int traverseEntry(Context* ctx, int numGroups, const uint32_t* gs, const uint32_
                   const float* ys, const float* zs, const float* ms, const float* hs, const int32_t* child,
                   const int32_t* i2l, const uint32_t* layout, const float* centers, const float* mp, float G,
                   int numShells, const float* boxL, float* p, float* ax, float* ay, float* az, double* totalPot,
-                  int stackCap, size_t hCount)
+                  int stackCap, size_t hCount, bool mdq)
 {
     if (numGroups > 0 && (!gs || !ge || !xt || !yt || !zt || !mt || !ht || !xs || !ys || !zs || !ms || !hs ||
                           !child || !i2l || !layout || !centers || !mp || !ax || !ay || !az))
@@ -878,6 +928,7 @@ int traverseEntry(Context* ctx, int numGroups, const uint32_t* gs, const uint32_
     a.p = p, a.ax = ax, a.ay = ay, a.az = az;
     a.stackCap = stackCap;
     a.hCount   = hCount;
+    a.mdq      = mdq;
     return traverse(ctx, a, totalPot);
 }
 

@@ -60,23 +60,27 @@ struct UpsweepArgsT
     MacBox box;
     double4* cmac = nullptr; // [n] {com, mac^2} in double (SourceCenterType<double>)
 };
+// multipole values per node: 8 (CartesianQuadrupole), 12 (CartesianMDQpole: the quadrupole's 8, px, py, pz, 0)
+template<bool MDQ>
+constexpr int kMpN = MDQ ? 12 : 8;
 using UpsweepArgs = UpsweepArgsT<float>;
 
-template<class Tin>
-__device__ inline void storeNode(const UpsweepArgsT<Tin>& a, int node, const double c[4], const double q[8])
+template<bool MDQ = false, class Tin>
+__device__ inline void storeNode(const UpsweepArgsT<Tin>& a, int node, const double c[4], const double* q)
 {
+    constexpr int N = kMpN<MDQ>;
     double* c64 = a.c64 + 4 * size_t(node);
-    double* q64 = a.q64 + 8 * size_t(node);
+    double* q64 = a.q64 + N * size_t(node);
     for (int k = 0; k < 4; ++k)
         c64[k] = c[k];
-    for (int k = 0; k < 8; ++k)
+    for (int k = 0; k < N; ++k)
         q64[k] = q[k];
     double mac2 = c[3] != 0.0 ? macSquared(a.prefixes[node], c[0], c[1], c[2], a.box) : 0.0;
     if (a.c32) a.c32[node] = make_float4(float(c[0]), float(c[1]), float(c[2]), float(mac2));
     if (a.q32)
     {
-        float* q32 = a.q32 + 8 * size_t(node);
-        for (int k = 0; k < 8; ++k)
+        float* q32 = a.q32 + N * size_t(node);
+        for (int k = 0; k < N; ++k)
             q32[k] = float(q[k]);
     }
     if (a.cmac) a.cmac[node] = make_double4(c[0], c[1], c[2], mac2);
@@ -102,9 +106,10 @@ __device__ inline double red8(double v)
     return v;
 }
 
-// leaves: 8 lanes per leaf reduce its particles (|m|-weighted center, then the quadrupole about it); all 8 lanes
-// of a group call with the same leaf and `ok`
-template<class Tin>
+// leaves: 8 lanes per leaf reduce its particles (|m|-weighted center, then the quadrupole about it; MDQ adds the
+// dipole sum m r about the center, P2M for CartesianMDQpole, cartesian_qpole.hpp:237-295); all 8 lanes of a group
+// call with the same leaf and `ok`
+template<bool MDQ = false, class Tin>
 __device__ inline void leafUpsweepOne(const UpsweepArgsT<Tin>& a, int leaf, bool ok, int sub)
 {
     uint32_t b = 0, e = 0;
@@ -124,7 +129,7 @@ __device__ inline void leafUpsweepOne(const UpsweepArgsT<Tin>& a, int leaf, bool
     double inv = c[3] != 0.0 ? 1.0 / c[3] : 1.0;
     c[0] *= inv, c[1] *= inv, c[2] *= inv;
 
-    double q[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    double q[12] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
     for (uint32_t p = b + sub; p < e; p += 8)
     {
         double rx = double(a.x[p]) - c[0], ry = double(a.y[p]) - c[1], rz = double(a.z[p]) - c[2];
@@ -136,9 +141,13 @@ __device__ inline void leafUpsweepOne(const UpsweepArgsT<Tin>& a, int leaf, bool
         q[4] += ry * ry * mi;
         q[5] += ry * rz * mi;
         q[6] += rz * rz * mi;
+        if constexpr (MDQ) q[8] += mi * rx, q[9] += mi * ry, q[10] += mi * rz;
     }
     for (int k = 0; k < 7; ++k)
         q[k] = red8(q[k]);
+    if constexpr (MDQ)
+        for (int k = 8; k < 11; ++k)
+            q[k] = red8(q[k]);
     double tr = q[1] + q[4] + q[6];
     q[7] = tr;
     q[1] = 3 * q[1] - tr;
@@ -146,25 +155,28 @@ __device__ inline void leafUpsweepOne(const UpsweepArgsT<Tin>& a, int leaf, bool
     q[6] = 3 * q[6] - tr;
     q[2] *= 3, q[3] *= 3, q[5] *= 3;
 
-    if (ok && sub == 0) storeNode(a, a.leafNode[leaf], c, q);
+    if (ok && sub == 0) storeNode<MDQ>(a, a.leafNode[leaf], c, q);
 }
 
-template<class Tin>
+template<class Tin, bool MDQ = false>
 __global__ __launch_bounds__(256) void leafUpsweepKernel(UpsweepArgsT<Tin> a)
 {
     const int leaf = int((size_t(blockIdx.x) * blockDim.x + threadIdx.x) >> 3);
-    leafUpsweepOne(a, leaf, leaf < a.numLeaves, threadIdx.x & 7);
+    leafUpsweepOne<MDQ>(a, leaf, leaf < a.numLeaves, threadIdx.x & 7);
 }
 
-// one internal node: lane k of the 8-lane group holds child k; center and M2M by 8-lane reductions
-template<class Tin>
+// one internal node: lane k of the 8-lane group holds child k; center and M2M by 8-lane reductions. MDQ follows
+// addQuadrupole for CartesianMDQpole (cartesian_qpole.hpp:359-403): the child's dipole p about its own center
+// shifts the trace by -2 p.dX, the diagonal by -6 dX_i p_i + 2 p.dX, the off-diagonal by -3 (dX_i p_j + dX_j p_i),
+// and the parent dipole collects p - M dX (dX = parent center - child center)
+template<bool MDQ = false, class Tin>
 __device__ inline void levelNodeOne(const UpsweepArgsT<Tin>& a, int node, int sub)
 {
     {
         const int ch = a.child[node]; // uniform over the 8 lanes of the node
         if (ch == 0) return;
         const double* s  = a.c64 + 4 * size_t(ch + sub);
-        const double* cq = a.q64 + 8 * size_t(ch + sub);
+        const double* cq = a.q64 + kMpN<MDQ> * size_t(ch + sub);
         double sx = s[0], sy = s[1], sz = s[2], w = fabs(s[3]);
         double pc[4] = {red8(w * sx), red8(w * sy), red8(w * sz), red8(w)};
         double pinv  = pc[3] != 0.0 ? 1.0 / pc[3] : 1.0;
@@ -173,21 +185,41 @@ __device__ inline void levelNodeOne(const UpsweepArgsT<Tin>& a, int node, int su
         double rx2 = rx * rx, ry2 = ry * ry, rz2 = rz * rz;
         double r2  = (rx2 + ry2 + rz2) * (1.0 / 3.0);
         double ml  = cq[0] * 3;
-        double pq[8];
-        pq[0] = red8(cq[0]);
-        pq[1] = red8(cq[1] + ml * (rx2 - r2));
-        pq[2] = red8(cq[2] + ml * rx * ry);
-        pq[3] = red8(cq[3] + ml * rx * rz);
-        pq[4] = red8(cq[4] + ml * (ry2 - r2));
-        pq[5] = red8(cq[5] + ml * ry * rz);
-        pq[6] = red8(cq[6] + ml * (rz2 - r2));
-        pq[7] = red8(cq[7] + ml * r2);
-        if (sub == 0) storeNode(a, node, pc, pq);
+        double pq[12];
+        if constexpr (!MDQ)
+        {
+            pq[0] = red8(cq[0]);
+            pq[1] = red8(cq[1] + ml * (rx2 - r2));
+            pq[2] = red8(cq[2] + ml * rx * ry);
+            pq[3] = red8(cq[3] + ml * rx * rz);
+            pq[4] = red8(cq[4] + ml * (ry2 - r2));
+            pq[5] = red8(cq[5] + ml * ry * rz);
+            pq[6] = red8(cq[6] + ml * (rz2 - r2));
+            pq[7] = red8(cq[7] + ml * r2);
+        }
+        else
+        {
+            const double px = cq[8], py = cq[9], pz = cq[10];
+            const double dip = rx * px + ry * py + rz * pz;
+            pq[0]  = red8(cq[0]);
+            pq[1]  = red8(cq[1] + ml * (rx2 - r2) - 6.0 * rx * px + 2.0 * dip);
+            pq[2]  = red8(cq[2] + ml * rx * ry - 3.0 * (rx * py + ry * px));
+            pq[3]  = red8(cq[3] + ml * rx * rz - 3.0 * (rx * pz + rz * px));
+            pq[4]  = red8(cq[4] + ml * (ry2 - r2) - 6.0 * ry * py + 2.0 * dip);
+            pq[5]  = red8(cq[5] + ml * ry * rz - 3.0 * (ry * pz + rz * py));
+            pq[6]  = red8(cq[6] + ml * (rz2 - r2) - 6.0 * rz * pz + 2.0 * dip);
+            pq[7]  = red8(cq[7] + ml * r2 - 2.0 * dip);
+            pq[8]  = red8(px - cq[0] * rx);
+            pq[9]  = red8(py - cq[0] * ry);
+            pq[10] = red8(pz - cq[0] * rz);
+            pq[11] = 0.0;
+        }
+        if (sub == 0) storeNode<MDQ>(a, node, pc, pq);
     }
 }
 
 // internal nodes of one level
-template<class Tin>
+template<class Tin, bool MDQ = false>
 __global__ __launch_bounds__(256) void levelUpsweepKernel(UpsweepArgsT<Tin> a, const int32_t* __restrict__ levelRange,
                                                           int level)
 {
@@ -195,7 +227,7 @@ __global__ __launch_bounds__(256) void levelUpsweepKernel(UpsweepArgsT<Tin> a, c
     const int groupsPerGrid = int(gridDim.x * blockDim.x) >> 3;
     for (int node = first + int((size_t(blockIdx.x) * blockDim.x + threadIdx.x) >> 3); node < last;
          node += groupsPerGrid)
-        levelNodeOne(a, node, threadIdx.x & 7);
+        levelNodeOne<MDQ>(a, node, threadIdx.x & 7);
 }
 
 /* ---------------------------------------------------------------- batches of systems */
@@ -288,7 +320,7 @@ MacBox makeMacBox(const float* limits, float invTheta)
 
 // leaf kernel, then one kernel per level from the deepest possible parent level to the root; each launch reads
 // its node range on the device, so the sequence needs no host copy of the level ranges (and can be captured)
-template<class Tin, class Tl>
+template<class Tin, class Tl, bool MDQ = false>
 static int upsweepT(Context* ctx, UpsweepArgsT<Tin> a, int L, const int32_t* levelRange, const Tl* limits,
                     float invTheta, double softReach)
 {
@@ -297,12 +329,12 @@ static int upsweepT(Context* ctx, UpsweepArgsT<Tin> a, int L, const int32_t* lev
     int nI = (L - 1) / 7, n = L + nI;
     size_t need = 256;
     if (!a.c64) need += size_t(n) * 4 * sizeof(double);
-    if (!a.q64) need += size_t(n) * 8 * sizeof(double);
+    if (!a.q64) need += size_t(n) * kMpN<MDQ> * sizeof(double);
     auto* buf = static_cast<char*>(ctx->getScratch(kSlotUpsweep, need));
     if (!buf) return RYO_ERR_HIP;
     size_t off = 0;
     if (!a.c64) a.c64 = reinterpret_cast<double*>(buf + off), off += size_t(n) * 4 * sizeof(double);
-    if (!a.q64) a.q64 = reinterpret_cast<double*>(buf + off), off += size_t(n) * 8 * sizeof(double);
+    if (!a.q64) a.q64 = reinterpret_cast<double*>(buf + off), off += size_t(n) * kMpN<MDQ> * sizeof(double);
     a.numLeaves = L;
     a.leafNode += nI; // leafToInternal + numInternal
     const Bits3 bits = boxBits(limits);
@@ -316,11 +348,11 @@ static int upsweepT(Context* ctx, UpsweepArgsT<Tin> a, int L, const int32_t* lev
     a.box.invTheta  = invTheta;
     a.box.softReach = softReach;
     hipStream_t s = ctx->stream;
-    leafUpsweepKernel<<<iceil(size_t(L) * 8, 256), 256, 0, s>>>(a);
+    leafUpsweepKernel<Tin, MDQ><<<iceil(size_t(L) * 8, 256), 256, 0, s>>>(a);
     const unsigned levelBlocks = std::max(1u, std::min(iceil(size_t(nI) * 8, 256), unsigned(ctx->numCUs * 4)));
     if (nI > 0)
         for (int l = int(kMaxLevel) - 1; l >= 0; --l)
-            levelUpsweepKernel<<<levelBlocks, 256, 0, s>>>(a, levelRange, l);
+            levelUpsweepKernel<Tin, MDQ><<<levelBlocks, 256, 0, s>>>(a, levelRange, l);
     RYO_LAUNCH_CHECK(ctx);
     return RYO_OK;
 }
@@ -336,6 +368,20 @@ int upsweep(Context* ctx, const float* x, const float* y, const float* z, const 
     a.leafNode = l2i, a.parents = parents, a.child = child, a.prefixes = prefixes;
     a.c32 = reinterpret_cast<float4*>(centers), a.q32 = mpoles, a.c64 = c64, a.q64 = q64;
     return upsweepT(ctx, a, L, levelRange, limits, invTheta, softReach);
+}
+
+// CartesianMDQpole multipoles ([n][12] float, optional [n][12] double copy) from float particles
+int upsweepMDQ(Context* ctx, const float* x, const float* y, const float* z, const float* m, const uint32_t* layout,
+               int L, const uint64_t* prefixes, const int32_t* child, const int32_t* parents, const int32_t* l2i,
+               const int32_t* levelRange, const float* limits, float invTheta, float* centers, float* mpoles,
+               double* c64, double* q64)
+{
+    UpsweepArgs a{};
+    a.x = x, a.y = y, a.z = z, a.m = m;
+    a.layout = layout;
+    a.leafNode = l2i, a.parents = parents, a.child = child, a.prefixes = prefixes;
+    a.c32 = reinterpret_cast<float4*>(centers), a.q32 = mpoles, a.c64 = c64, a.q64 = q64;
+    return upsweepT<float, float, true>(ctx, a, L, levelRange, limits, invTheta, 0.0);
 }
 
 // double particles and box: centers {com, mac^2} and multipoles in double (mp32: optional float copy)

@@ -92,7 +92,8 @@ def gpu_link(ctx, leaves):
     return res
 
 
-def gpu_upsweep(ctx, x, y, z, m, layout, tree, limits, inv_theta):
+def gpu_upsweep(ctx, x, y, z, m, layout, tree, limits, inv_theta, mdq=False):
+    """float upsweep; mdq=True: CartesianMDQpole multipoles (12 per node)"""
     import torch
 
     L = tree["numLeaves"] if "numLeaves" in tree else layout.size - 1
@@ -100,18 +101,20 @@ def gpu_upsweep(ctx, x, y, z, m, layout, tree, limits, inv_theta):
     d = {k: dev(tree[k]) for k in ("prefixes", "childOffsets", "parents", "leafToInternal", "levelRange")}
     xs = [dev(a) for a in (x, y, z, m)]
     lay = dev(layout)
-    c32, q32 = zeros(4 * n, torch.float32), zeros(8 * n, torch.float32)
-    c64, q64 = zeros(4 * n, torch.float64), zeros(8 * n, torch.float64)
-    ctx.call("ryo_upsweep_f32", *(R.ptr(a) for a in xs), R.ptr(lay), L, R.ptr(d["prefixes"]),
+    w = 12 if mdq else 8
+    c32, q32 = zeros(4 * n, torch.float32), zeros(w * n, torch.float32)
+    c64, q64 = zeros(4 * n, torch.float64), zeros(w * n, torch.float64)
+    ctx.call("ryo_upsweep_mdq_f32" if mdq else "ryo_upsweep_f32", *(R.ptr(a) for a in xs), R.ptr(lay), L, R.ptr(d["prefixes"]),
              R.ptr(d["childOffsets"]), R.ptr(d["parents"]), R.ptr(d["leafToInternal"]), R.ptr(d["levelRange"]),
              R.host_f32(limits),
              float(inv_theta), R.ptr(c32), R.ptr(q32), R.ptr(c64), R.ptr(q64))
-    return (host(c32).reshape(n, 4), host(q32).reshape(n, 8), host(c64).reshape(n, 4), host(q64).reshape(n, 8))
+    return (host(c32).reshape(n, 4), host(q32).reshape(n, w), host(c64).reshape(n, 4), host(q64).reshape(n, w))
 
 
 def gpu_traverse(ctx, x, y, z, m, h, tree, layout, centers, mp, group_size, G=1.0, num_shells=0, box_l=None,
-                 init_node=0, stack_cap=None, want_pot=True, bounds=None):
-    """group_size-target groups, or the group boundaries `bounds` (G+1 entries) when given"""
+                 init_node=0, stack_cap=None, want_pot=True, bounds=None, mdq=False):
+    """group_size-target groups, or the group boundaries `bounds` (G+1 entries) when given; mdq=True: the
+    CartesianMDQpole traversal (mp with 12 values per node)"""
     import torch
 
     n = x.size
@@ -130,7 +133,9 @@ def gpu_traverse(ctx, x, y, z, m, h, tree, layout, centers, mp, group_size, G=1.
     common = [R.ptr(groups), R.ptr(groups) + 4, init_node, *(R.ptr(a) for a in arrs), *(R.ptr(a) for a in arrs),
               R.ptr(child), R.ptr(i2l), R.ptr(lay), R.ptr(c), R.ptr(q), float(G), num_shells, bl,
               R.ptr(p) if want_pot else None, R.ptr(ax), R.ptr(ay), R.ptr(az), C.byref(pot)]
-    if stack_cap is None:
+    if mdq:
+        ctx.call("ryo_traverse_mdq_f32", ngroups, *common)
+    elif stack_cap is None:
         ctx.call("ryo_traverse_f32", ngroups, *common)
     else:
         ctx.call("ryo_traverse_f32_stackcap", stack_cap, ngroups, *common)

@@ -313,3 +313,49 @@ def test_keys_on_box_faces_equal_reference(limits):
         k = O.sfc_keys(x, y, z, lim, hilbert=hil)
         assert np.array_equal(k, O.sfc_keys(x, y, z, lim, hilbert=hil, lib="ref"))
         assert k.max() < (1 << 63)
+
+
+def _mdq_direct(x, y, z, m, c):
+    """CartesianMDQpole of all particles about c, straight from the definition (traceless 3 sum m r r - tr,
+    trace, dipole sum m r)"""
+    r = np.stack([x, y, z], 1).astype(np.float64) - np.asarray(c, np.float64)
+    m = m.astype(np.float64)
+    S = np.einsum("i,ij,ik->jk", m, r, r)
+    tr = np.trace(S)
+    Q = 3 * S - tr * np.eye(3)
+    p = m @ r
+    return np.array([m.sum(), Q[0, 0], Q[0, 1], Q[0, 2], Q[1, 1], Q[1, 2], Q[2, 2], tr, p[0], p[1], p[2], 0.0])
+
+
+def test_mdq_root_equals_direct_expansion():
+    """signed masses: the reference MDQpole upsweep's root (P2M at the leaves, then M2M with the dipole shifts of
+    the parallel-axis theorem) equals the expansion of all particles about the root center; a dropped or
+    mis-signed dipole term in M2M fails this"""
+    s = W.plummer(6000, seed=41)
+    rng = np.random.default_rng(5)
+    s.m = np.where(rng.random(s.m.size) < 0.4, -s.m, s.m).astype(s.m.dtype)
+    sk, perm = O.sort_by_key(O.sfc_keys(s.x, s.y, s.z, s.limits))
+    x, y, z, m = (a[perm] for a in (s.x, s.y, s.z, s.m))
+    leaves, counts = O.compute_octree(sk, 16)
+    c, q = O.ref_upsweep_mdq(x, y, z, m, leaves, O.layout_from_counts(counts), s.limits.astype(np.float64), 2.0,
+                             precision="f64")
+    want = _mdq_direct(x, y, z, m, c[0, :3])
+    assert np.abs(want[8:11]).max() > 1e-2 * np.abs(want[1:7]).max()  # the dipole is not negligible
+    scale = np.abs(want).max()
+    assert np.allclose(q[0], want, rtol=0, atol=1e-10 * scale), (q[0], want)
+
+
+def test_mdq_positive_masses_reduce_to_quadrupole():
+    """positive masses: the dipole about the center of mass vanishes and the MDQpole equals the quadrupole"""
+    s = W.uniform_cube(4000, seed=42)
+    sk, perm = O.sort_by_key(O.sfc_keys(s.x, s.y, s.z, s.limits))
+    x, y, z, m = (a[perm] for a in (s.x, s.y, s.z, s.m))
+    leaves, counts = O.compute_octree(sk, 16)
+    lay = O.layout_from_counts(counts)
+    lim = s.limits.astype(np.float64)
+    cm, qm = O.ref_upsweep_mdq(x, y, z, m, leaves, lay, lim, 2.0, precision="f64")
+    cq, qq = O.upsweep(x, y, z, m, leaves, lay, lim, 2.0, precision="f64")
+    scale = np.abs(qq).max(axis=0) + 1e-300
+    assert np.array_equal(cm, cq)
+    assert np.all(np.abs(qm[:, :8] - qq) <= 1e-12 * scale)
+    assert np.abs(qm[:, 8:11]).max() <= 1e-12 * np.abs(qq[:, 1:7]).max()
