@@ -222,6 +222,31 @@ int ryo_traverse_f64(ryo_ctx* ctx, int num_groups, const uint32_t* group_start, 
                      int num_shells, const double* box_l, double* p, double* ax, double* ay, double* az,
                      double* total_pot);
 
+/* The double-particle upsweep with CartesianMDQpole<double> multipoles double[numNodes][12] (the layout of
+ * ryo_upsweep_mdq_f32), multipoles32 (optional) the CartesianMDQpole<float> copy; otherwise as ryo_upsweep_f64. */
+int ryo_upsweep_mdq_f64(ryo_ctx* ctx, const double* x, const double* y, const double* z, const double* m,
+                        const uint32_t* layout, int num_leaves, const uint64_t* prefixes, const int32_t* child_offsets,
+                        const int32_t* parents, const int32_t* leaf_to_internal, const int32_t* level_range,
+                        const double* limits, float inv_theta, double* centers, double* multipoles,
+                        float* multipoles32);
+/* ryoanji::traverse for (d,d,d,d,d) with CartesianMDQpole<double> and (d,f,f,f,d) with CartesianMDQpole<float>
+ * (traversal_gpu.cu:635-642): the arguments of ryo_traverse_f64 / ryo_traverse_f64_f32 with 12 values per
+ * node; M2P adds the dipole terms (cartesian_qpole.hpp:320-357). */
+int ryo_traverse_mdq_f64(ryo_ctx* ctx, int num_groups, const uint32_t* group_start, const uint32_t* group_end,
+                         int init_node, const double* xt, const double* yt, const double* zt, const double* mt,
+                         const double* ht, const double* xs, const double* ys, const double* zs, const double* ms,
+                         const double* hs, const int32_t* child_offsets, const int32_t* internal_to_leaf,
+                         const uint32_t* layout, const double* centers, const double* multipoles, double G,
+                         int num_shells, const double* box_l, double* p, double* ax, double* ay, double* az,
+                         double* total_pot);
+int ryo_traverse_mdq_f64_f32(ryo_ctx* ctx, int num_groups, const uint32_t* group_start, const uint32_t* group_end,
+                             int init_node, const double* xt, const double* yt, const double* zt, const float* mt,
+                             const float* ht, const double* xs, const double* ys, const double* zs, const float* ms,
+                             const float* hs, const int32_t* child_offsets, const int32_t* internal_to_leaf,
+                             const uint32_t* layout, const double* centers, const float* multipoles, double G,
+                             int num_shells, const double* box_l, float* p, float* ax, float* ay, float* az,
+                             double* total_pot);
+
 /* ryoanji::traverse for the (d,f,f,f,d) instantiation with CartesianQuadrupole<float>: double coordinates and
  * centers, float softenings, masses, multipoles and outputs; interactions in double (the reference accumulates in
  * Tf = double), outputs rounded to float. Otherwise as ryo_traverse_f64. */

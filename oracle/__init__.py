@@ -97,6 +97,10 @@ def lib(which: str = "orc") -> C.CDLL:
             "upsweep_mdq_f64": (C.c_int, [_P, _P, _P, _P, _P, C.c_int, _P, _P, C.c_float, _P, _P]),
             "compute_gravity_mdq_f32": (C.c_double, [_P, C.c_int, _P, _P, _P, _P, _P, _P, _P, _P, _P, C.c_float,
                                                      C.c_int, C.c_int, C.c_int, _P, _P, _P, _P]),
+            "compute_gravity_mdq_f64": (C.c_double, [_P, C.c_int, _P, _P, _P, _P, _P, _P, _P, _P, _P, C.c_float,
+                                                     C.c_int, C.c_int, C.c_int, _P, _P, _P, _P]),
+            "compute_gravity_mdq_df": (C.c_double, [_P, C.c_int, _P, _P, _P, _P, _P, _P, _P, _P, _P, C.c_float,
+                                                    C.c_int, C.c_int, C.c_int, _P, _P, _P, _P]),
             "sfc_keys_u32_f64": (C.c_int, [_P, _P, _P, _P, C.c_size_t, _P, C.c_int]),
             "compute_gravity_df": (C.c_double, [_P, C.c_int, _P, _P, _P, _P, _P, _P, _P, _P, _P, C.c_float,
                                                 C.c_int, C.c_int, C.c_int, _P, _P, _P, _P]),
@@ -316,10 +320,11 @@ def ref_compute_gravity_mdq(tree_leaves, layout, centers, mp, x, y, z, h, m, lim
     return tot, ug, ax, ay, az
 
 
-def ref_compute_gravity64(tree_leaves, layout, centers, mp, x, y, z, h, m, limits, G=1.0, num_shells=0, mixed=False):
+def ref_compute_gravity64(tree_leaves, layout, centers, mp, x, y, z, h, m, limits, G=1.0, num_shells=0, mixed=False,
+                          mdq=False):
     """The reference computeGravity (16-target groups) with double coordinates and centers: mixed=False is the
     all-double instantiation (CartesianQuadrupole<double>, double outputs), mixed=True has float h, m, multipoles
-    and outputs. -> (pot_total, ugrav, ax, ay, az)"""
+    and outputs; mdq=True takes CartesianMDQpole multipoles ([n,12]). -> (pot_total, ugrav, ax, ay, az)"""
     leaves = _c(tree_leaves, np.uint64)
     layout = _c(layout, np.uint32)
     L = leaves.size - 1
@@ -328,7 +333,8 @@ def ref_compute_gravity64(tree_leaves, layout, centers, mp, x, y, z, h, m, limit
     hm = [_c(a, ft) for a in (h, m)]
     n = xyz[0].size
     ug, ax, ay, az = (np.zeros(n, ft) for _ in range(4))
-    tot = _f("ref", "compute_gravity_df" if mixed else "compute_gravity_f64")(
+    name = ("compute_gravity_mdq_" if mdq else "compute_gravity_") + ("df" if mixed else "f64")
+    tot = _f("ref", name)(
         _ptr(leaves), L, _ptr(layout), _ptr(_c(centers, np.float64)), _ptr(_c(mp, ft)), *(_ptr(a) for a in xyz),
         *(_ptr(a) for a in hm), _ptr(_c(limits, np.float64)), float(G), num_shells, 0, L, _ptr(ug), _ptr(ax), _ptr(ay),
         _ptr(az))

@@ -238,6 +238,42 @@ double ref_compute_gravity_df(const uint64_t* leaves, int numLeaves, const unsig
     return tot;
 }
 
+/* the two double-coordinate computeGravity instantiations with CartesianMDQpole multipoles: (d,d,d,d,d) with
+ * CartesianMDQpole<double>, and (d,f,f,f,d) with CartesianMDQpole<float> */
+double ref_compute_gravity_mdq_f64(const uint64_t* leaves, int numLeaves, const unsigned* layout,
+                                   const double* centers, const double* mpoles, const double* x, const double* y,
+                                   const double* z, const double* h, const double* m, const double* limits, float G,
+                                   int numShells, int firstLeaf, int lastLeaf, double* ugrav, double* ax, double* ay,
+                                   double* az)
+{
+    Tree t(leaves, numLeaves);
+    auto& o  = t.o;
+    auto box = box6(limits);
+    double tot = 0;
+    ryoanji::computeGravity(o.childOffsets.data(), o.parents.data(), o.internalToLeaf.data(),
+                            reinterpret_cast<const SourceCenterType<double>*>(centers),
+                            reinterpret_cast<const ryoanji::CartesianMDQpole<double>*>(mpoles), layout, firstLeaf,
+                            lastLeaf, x, y, z, h, m, box, G, ugrav, ax, ay, az, &tot, numShells);
+    return tot;
+}
+
+double ref_compute_gravity_mdq_df(const uint64_t* leaves, int numLeaves, const unsigned* layout,
+                                  const double* centers, const float* mpoles, const double* x, const double* y,
+                                  const double* z, const float* h, const float* m, const double* limits, float G,
+                                  int numShells, int firstLeaf, int lastLeaf, float* ugrav, float* ax, float* ay,
+                                  float* az)
+{
+    Tree t(leaves, numLeaves);
+    auto& o  = t.o;
+    auto box = box6(limits);
+    double tot = 0;
+    ryoanji::computeGravity(o.childOffsets.data(), o.parents.data(), o.internalToLeaf.data(),
+                            reinterpret_cast<const SourceCenterType<double>*>(centers),
+                            reinterpret_cast<const ryoanji::CartesianMDQpole<float>*>(mpoles), layout, firstLeaf,
+                            lastLeaf, x, y, z, h, m, box, G, ugrav, ax, ay, az, &tot, numShells);
+    return tot;
+}
+
 /* directSum (traversal_cpu.hpp:232-268) over all particles, T = double */
 int ref_direct_sum_f64(const double* x, const double* y, const double* z, const double* h, const double* m, size_t n,
                        double G, const double* boxL, int numShells, double* ugrav, double* ax, double* ay, double* az)

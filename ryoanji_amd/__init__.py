@@ -104,6 +104,7 @@ SIGNATURES = {
     "ryo_upsweep_f32": (_I, [_P, _P, _P, _P, _P, _P, _I, _P, _P, _P, _P, _P, _P, _F, _P, _P, _P, _P]),
     "ryo_upsweep_mdq_f32": (_I, [_P, _P, _P, _P, _P, _P, _I, _P, _P, _P, _P, _P, _P, _F, _P, _P, _P, _P]),
     "ryo_upsweep_f64": (_I, [_P, _P, _P, _P, _P, _P, _I, _P, _P, _P, _P, _P, _P, _F, _P, _P, _P]),
+    "ryo_upsweep_mdq_f64": (_I, [_P, _P, _P, _P, _P, _P, _I, _P, _P, _P, _P, _P, _P, _F, _P, _P, _P]),
     "ryo_fixed_groups": (_I, [_P, C.c_uint32, C.c_uint32, _U, _P, C.POINTER(_I)]),
     "ryo_group_splits_f32": (_I, [_P, C.c_uint32, C.c_uint32, _P, _P, _P, _P, _P, _I, _P, _P, _U, _F, _F, _P,
                                   C.c_size_t, C.POINTER(_I)]),
@@ -113,7 +114,11 @@ SIGNATURES = {
                                                                    _P]),
     "ryo_traverse_f64": (_I, [_P, _I, _P, _P, _I] + [_P] * 10 + [_P, _P, _P, _P, _P, _D, _I, _P, _P, _P, _P, _P,
                                                                    _P]),
+    "ryo_traverse_mdq_f64": (_I, [_P, _I, _P, _P, _I] + [_P] * 10 + [_P, _P, _P, _P, _P, _D, _I, _P, _P, _P, _P, _P,
+                                                                   _P]),
     "ryo_traverse_f64_f32": (_I, [_P, _I, _P, _P, _I] + [_P] * 10 + [_P, _P, _P, _P, _P, _D, _I, _P, _P, _P, _P,
+                                                                       _P, _P]),
+    "ryo_traverse_mdq_f64_f32": (_I, [_P, _I, _P, _P, _I] + [_P] * 10 + [_P, _P, _P, _P, _P, _D, _I, _P, _P, _P, _P,
                                                                        _P, _P]),
     "ryo_read_stats": (_I, [_P, _P]),
     "ryo_traversal_retries": (C.c_uint64, [_P]),

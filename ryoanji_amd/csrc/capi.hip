@@ -29,6 +29,9 @@ int upsweep(Context*, const float*, const float*, const float*, const float*, co
 int upsweepMDQ(Context*, const float*, const float*, const float*, const float*, const uint32_t*, int,
                const uint64_t*, const int32_t*, const int32_t*, const int32_t*, const int32_t*, const float*, float,
                float*, float*, double*, double*);
+int upsweepMDQ64(Context*, const double*, const double*, const double*, const double*, const uint32_t*, int,
+                 const uint64_t*, const int32_t*, const int32_t*, const int32_t*, const int32_t*, const double*,
+                 float, double*, double*, float*);
 int upsweep64(Context*, const double*, const double*, const double*, const double*, const uint32_t*, int,
               const uint64_t*, const int32_t*, const int32_t*, const int32_t*, const int32_t*, const double*, float,
               double*, double*, float*);
@@ -290,6 +293,17 @@ int ryo_upsweep_f64(ryo_ctx* h, const double* x, const double* y, const double* 
     if (!limits) return setError(ctx, RYO_ERR_ARG, "upsweep: null limits");
     return upsweep64(ctx, x, y, z, m, layout, num_leaves, prefixes, child, parents, l2i, level_range, limits,
                      inv_theta, centers, mp, mp32);
+}
+
+int ryo_upsweep_mdq_f64(ryo_ctx* h, const double* x, const double* y, const double* z, const double* m,
+                        const uint32_t* layout, int num_leaves, const uint64_t* prefixes, const int32_t* child,
+                        const int32_t* parents, const int32_t* l2i, const int32_t* level_range, const double* limits,
+                        float inv_theta, double* centers, double* mp, float* mp32)
+{
+    CTX_OR_FAIL(h);
+    if (!limits) return setError(ctx, RYO_ERR_ARG, "upsweep: null limits");
+    return upsweepMDQ64(ctx, x, y, z, m, layout, num_leaves, prefixes, child, parents, l2i, level_range, limits,
+                        inv_theta, centers, mp, mp32);
 }
 
 int ryo_fixed_groups(ryo_ctx* h, uint32_t first, uint32_t last, unsigned group_size, uint32_t* groups,

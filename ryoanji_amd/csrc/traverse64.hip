@@ -40,7 +40,8 @@ struct Trav64Args
     const int32_t* i2l;
     const uint32_t* layout;
     const double* centers; // [n][4] {center of mass, mac^2}
-    const Tq* mp;          // [n][8] {M, qxx, qxy, qxz, qyy, qyz, qzz, trace}
+    const Tq* mp;          // [n][8] {M, qxx, qxy, qxz, qyy, qyz, qzz, trace}, or [n][12] with px, py, pz, 0 (mdq)
+    bool mdq;              // multipoles are CartesianMDQpole
     double G;
     int numShells;
     double boxL[3];
@@ -128,6 +129,28 @@ __device__ inline void m2p64(Acc64& a, double tx, double ty, double tz, const do
     a.z += r5 * Qrz + mono * rz;
 }
 
+// monopole-dipole-quadrupole M2P (cartesian_qpole.hpp:320-357) in double: the quadrupole terms plus the dipole
+// p about the expansion center, potential -(p.r) / r^3 and field (p r^2 - 3 (p.r) r) / r^5
+template<class Tq>
+__device__ inline void m2pMDQ64(Acc64& a, double tx, double ty, double tz, const double* c, const Tq* q)
+{
+    const double rx = tx - c[0], ry = ty - c[1], rz = tz - c[2];
+    const double r2 = rx * rx + ry * ry + rz * rz;
+    const double r1 = invSqrtRef(r2), ri2 = r1 * r1, r5 = ri2 * ri2 * r1;
+    const double M = q[0], qxx = q[1], qxy = q[2], qxz = q[3], qyy = q[4], qyz = q[5], qzz = q[6];
+    const double px = q[8], py = q[9], pz = q[10];
+    const double Qrx = rx * qxx + ry * qxy + rz * qxz;
+    const double Qry = rx * qxy + ry * qyy + rz * qyz;
+    const double Qrz = rx * qxz + ry * qyz + rz * qzz;
+    const double rQr  = rx * Qrx + ry * Qry + rz * Qrz;
+    const double mono = (-2.5 * rQr * r5 - M * r1) * ri2;
+    const double PR   = rx * px + ry * py + rz * pz;
+    a.p += (-M - PR * ri2) * r1 - 0.5 * r5 * rQr;
+    a.x += r5 * Qrx + mono * rx + r5 * (px * r2 - 3.0 * PR * rx);
+    a.y += r5 * Qry + mono * ry + r5 * (py * r2 - 3.0 * PR * ry);
+    a.z += r5 * Qrz + mono * rz + r5 * (pz * r2 - 3.0 * PR * rz);
+}
+
 struct Walk64
 {
     unsigned m2p, p2p, maxStack;
@@ -149,7 +172,10 @@ __device__ Walk64 walk64(const Trav64Args<Tc, Th, Tm, Ta, Tq>& a, int* stack, Tc
             const int l = __ffsll((unsigned long long)amask) - 1;
             amask &= amask - 1;
             const int c = __builtin_amdgcn_readlane(cell, l);
-            m2p64(acc, double(tx), double(ty), double(tz), a.centers + 4 * size_t(c), a.mp + 8 * size_t(c));
+            if (a.mdq)
+                m2pMDQ64(acc, double(tx), double(ty), double(tz), a.centers + 4 * size_t(c), a.mp + 12 * size_t(c));
+            else
+                m2p64(acc, double(tx), double(ty), double(tz), a.centers + 4 * size_t(c), a.mp + 8 * size_t(c));
         }
     };
     auto applyLeaves = [&](uint64_t dmask, uint32_t rb, uint32_t re) {
@@ -364,7 +390,7 @@ Trav64Args<Tc, Th, Tm, Ta, Tq> makeArgs(int numGroups, const uint32_t* gs, const
                                         const Tc* xs, const Tc* ys, const Tc* zs, const Tm* ms, const Th* hs,
                                         const int32_t* child, const int32_t* i2l, const uint32_t* layout,
                                         const double* centers, const Tq* mp, double G, int numShells,
-                                        const double* boxL, Ta* p, Ta* ax, Ta* ay, Ta* az)
+                                        const double* boxL, Ta* p, Ta* ax, Ta* ay, Ta* az, bool mdq = false)
 {
     Trav64Args<Tc, Th, Tm, Ta, Tq> a{};
     a.numGroups = numGroups, a.gs = gs, a.ge = ge, a.initNode = initNode;
@@ -375,6 +401,7 @@ Trav64Args<Tc, Th, Tm, Ta, Tq> makeArgs(int numGroups, const uint32_t* gs, const
     for (int d = 0; d < 3; ++d)
         a.boxL[d] = boxL ? boxL[d] : 0.0;
     a.p = p, a.ax = ax, a.ay = ay, a.az = az;
+    a.mdq = mdq;
     return a;
 }
 
@@ -419,6 +446,42 @@ int ryo_traverse_f64_f32(ryo_ctx* h, int num_groups, const uint32_t* group_start
                       makeArgs(num_groups, group_start, group_end, init_node, xt, yt, zt, mt, ht, xs, ys, zs, ms, hs,
                                child_offsets, internal_to_leaf, layout, centers, multipoles, G, num_shells, box_l, p,
                                ax, ay, az),
+                      total_pot);
+}
+
+int ryo_traverse_mdq_f64(ryo_ctx* h, int num_groups, const uint32_t* group_start, const uint32_t* group_end,
+                         int init_node, const double* xt, const double* yt, const double* zt, const double* mt,
+                         const double* ht, const double* xs, const double* ys, const double* zs, const double* ms,
+                         const double* hs, const int32_t* child_offsets, const int32_t* internal_to_leaf,
+                         const uint32_t* layout, const double* centers, const double* multipoles, double G,
+                         int num_shells, const double* box_l, double* p, double* ax, double* ay, double* az,
+                         double* total_pot)
+{
+    if (!h) return RYO_ERR_ARG;
+    Context* ctx = &h->c;
+    if (num_shells < 0 || (num_shells > 0 && !box_l)) return setError(ctx, RYO_ERR_ARG, "traverse: box lengths");
+    return traverse64(ctx,
+                      makeArgs(num_groups, group_start, group_end, init_node, xt, yt, zt, mt, ht, xs, ys, zs, ms, hs,
+                               child_offsets, internal_to_leaf, layout, centers, multipoles, G, num_shells, box_l, p,
+                               ax, ay, az, true),
+                      total_pot);
+}
+
+int ryo_traverse_mdq_f64_f32(ryo_ctx* h, int num_groups, const uint32_t* group_start, const uint32_t* group_end,
+                             int init_node, const double* xt, const double* yt, const double* zt, const float* mt,
+                             const float* ht, const double* xs, const double* ys, const double* zs, const float* ms,
+                             const float* hs, const int32_t* child_offsets, const int32_t* internal_to_leaf,
+                             const uint32_t* layout, const double* centers, const float* multipoles, double G,
+                             int num_shells, const double* box_l, float* p, float* ax, float* ay, float* az,
+                             double* total_pot)
+{
+    if (!h) return RYO_ERR_ARG;
+    Context* ctx = &h->c;
+    if (num_shells < 0 || (num_shells > 0 && !box_l)) return setError(ctx, RYO_ERR_ARG, "traverse: box lengths");
+    return traverse64(ctx,
+                      makeArgs(num_groups, group_start, group_end, init_node, xt, yt, zt, mt, ht, xs, ys, zs, ms, hs,
+                               child_offsets, internal_to_leaf, layout, centers, multipoles, G, num_shells, box_l, p,
+                               ax, ay, az, true),
                       total_pot);
 }
 

@@ -400,4 +400,20 @@ int upsweep64(Context* ctx, const double* x, const double* y, const double* z, c
     return upsweepT(ctx, a, L, levelRange, limits, invTheta, 0.0);
 }
 
+// the same with CartesianMDQpole multipoles: mpoles double[n][12], mp32 (optional) float[n][12]
+int upsweepMDQ64(Context* ctx, const double* x, const double* y, const double* z, const double* m,
+                 const uint32_t* layout, int L, const uint64_t* prefixes, const int32_t* child, const int32_t* parents,
+                 const int32_t* l2i, const int32_t* levelRange, const double* limits, float invTheta,
+                 double* centers, double* mpoles, float* mp32)
+{
+    if (!centers || !mpoles) return setError(ctx, RYO_ERR_ARG, "upsweep: null output");
+    UpsweepArgsT<double> a{};
+    a.x = x, a.y = y, a.z = z, a.m = m;
+    a.layout = layout;
+    a.leafNode = l2i, a.parents = parents, a.child = child, a.prefixes = prefixes;
+    a.c32 = nullptr, a.q32 = mp32, a.c64 = nullptr, a.q64 = mpoles;
+    a.cmac = reinterpret_cast<double4*>(centers);
+    return upsweepT<double, double, true>(ctx, a, L, levelRange, limits, invTheta, 0.0);
+}
+
 } // namespace ryo

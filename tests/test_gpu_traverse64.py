@@ -29,7 +29,7 @@ def _double_system(s, seed):
     return t, xyz, lim, c64, q64
 
 
-def _gpu64(ctx, t, xyz, h, m, c64, mp, group_size, mixed, num_shells=0, box_l=None):
+def _gpu64(ctx, t, xyz, h, m, c64, mp, group_size, mixed, num_shells=0, box_l=None, mdq=False):
     import torch
 
     n = xyz[0].size
@@ -47,7 +47,8 @@ def _gpu64(ctx, t, xyz, h, m, c64, mp, group_size, mixed, num_shells=0, box_l=No
     args = [ng, R.ptr(bounds), R.ptr(bounds) + 4, 0, *(R.ptr(a) for a in pos), R.ptr(hm[1]), R.ptr(hm[0]),
             *(R.ptr(a) for a in pos), R.ptr(hm[1]), R.ptr(hm[0]), R.ptr(child), R.ptr(i2l), R.ptr(lay), R.ptr(c),
             R.ptr(q), 1.0, num_shells, bl, R.ptr(p), R.ptr(ax), R.ptr(ay), R.ptr(az), C.byref(pot)]
-    ctx.call("ryo_traverse_f64_f32" if mixed else "ryo_traverse_f64", *args)
+    name = "ryo_traverse_mdq_f64" if mdq else "ryo_traverse_f64"
+    ctx.call(name + "_f32" if mixed else name, *args)
     return pot.value, host(p), host(ax), host(ay), host(az), ctx.stats()
 
 
@@ -181,3 +182,51 @@ def test_double_pipeline_equals_reference(ctx, name, mk):
     e = np.sort(rel_err(gax, gay, gaz, rax, ray, raz))
     assert e[int(0.99 * e.size)] <= 1e-10, e[int(0.99 * e.size)]
     assert abs(gtot - rtot) <= 1e-10 * abs(rtot)
+
+
+@pytest.mark.parametrize("mixed", [False, True])
+def test_double_mdq_upsweep_and_traversal(ctx, mixed):
+    """CartesianMDQpole with double coordinates (signed masses, so the dipoles are nonzero): the GPU upsweep against
+    the reference's MDQpole upsweep on the same double particles, then (d,d,d,d,d) / (d,f,f,f,d) traversal against
+    the reference's computeGravity with the same multipoles"""
+    import torch
+
+    s = W.plummer(20000, seed=46)
+    rng = np.random.default_rng(9)
+    s.m = np.where(rng.random(s.n) < 1 / 3, -s.m, s.m).astype(s.m.dtype)
+    t, xyz, lim, _, _ = _double_system(s, 10)
+    m64 = t["m"].astype(np.float64)
+    cm, qm = O.ref_upsweep_mdq(*xyz, m64, t["leaves"], t["layout"], lim, 2.0, precision="f64")
+    L = t["leaves"].size - 1
+    n = L + (L - 1) // 7
+    tr = t["tree"]
+    d = {key: dev(tr[key]) for key in ("prefixes", "childOffsets", "parents", "leafToInternal", "levelRange")}
+    gdx = [dev(a) for a in (*xyz, m64)]
+    lay = dev(t["layout"])
+    cen, mp, mp32 = zeros(4 * n, torch.float64), zeros(12 * n, torch.float64), zeros(12 * n, torch.float32)
+    ctx.call("ryo_upsweep_mdq_f64", *(R.ptr(a) for a in gdx), R.ptr(lay), L, R.ptr(d["prefixes"]),
+             R.ptr(d["childOffsets"]), R.ptr(d["parents"]), R.ptr(d["leafToInternal"]), R.ptr(d["levelRange"]),
+             R.host_f64(lim), 2.0, R.ptr(cen), R.ptr(mp), R.ptr(mp32))
+    gcen, gmp = host(cen).reshape(n, 4), host(mp).reshape(n, 12)
+    assert np.abs(qm[:, 8:11]).max() > 1e-3 * np.abs(qm[:, 1:7]).max()
+    assert np.allclose(gcen[:, :3], cm[:, :3], rtol=1e-12, atol=1e-14 * np.abs(cm[:, :3]).max())
+    assert np.allclose(gcen[:, 3], cm[:, 3], rtol=1e-12, atol=0)
+    scale = np.abs(qm).max(axis=0) + 1e-300
+    assert np.all(np.abs(gmp - qm) <= 1e-11 * scale)
+    assert np.array_equal(host(mp32).reshape(n, 12), gmp.astype(np.float32))
+
+    ft = np.float32 if mixed else np.float64
+    q = qm.astype(ft)
+    h, m = t["h"].astype(ft), t["m"].astype(ft)
+    rtot, rug, rax, ray, raz = O.ref_compute_gravity64(t["leaves"], t["layout"], cm, q, *xyz, h, m, lim,
+                                                       mixed=mixed, mdq=True)
+    gtot, gp, gax, gay, gaz, _ = _gpu64(ctx, t, xyz, h, m, cm, q, 16, mixed=mixed, mdq=True)
+    e = np.sort(rel_err(gax, gay, gaz, rax, ray, raz))
+    rms = np.sqrt(np.mean(rax.astype(np.float64) ** 2 + ray ** 2 + raz ** 2))
+    dd = np.sqrt((gax - rax).astype(np.float64) ** 2 + (gay - ray) ** 2 + (gaz - raz) ** 2) / rms
+    if mixed:
+        assert e[int(0.99 * e.size)] <= 2e-7, e[int(0.99 * e.size)]
+    else:
+        assert e[int(0.99 * e.size)] <= 1e-11 and dd.max() <= 1e-10, (e[int(0.99 * e.size)], dd.max())
+    # signed masses: the total may cancel, so the bound also scales with the sum of |potential| terms
+    assert abs(gtot - rtot) <= 1e-11 * abs(rtot) + 1e-12 * float(np.abs(gp.astype(np.float64)).sum())
