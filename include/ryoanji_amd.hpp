@@ -242,6 +242,8 @@ using cstone::Vec4;
 
 template<class T>
 using CartesianQuadrupole = std::array<T, 8>; // nbody/cartesian_qpole.hpp:27-51
+template<class T>
+using CartesianMDQpole = std::array<T, 12>; // nbody/cartesian_qpole.hpp:31-32
 
 /*! fused FP64 replacement of computeLeafSourceCenterGpu + upsweepCentersGpu + computeLeafMultipoles +
  *  upsweepMultipoles + setMacGpu (upsweep_gpu.h:25,40; source_center_gpu.h:66-116). leafToInternal is the leaf
@@ -269,6 +271,34 @@ inline void upsweep(const double* x, const double* y, const double* z, const dou
 {
     const TreeNodeIndex* full = leafToInternal - (numLeaves - 1) / 7;
     ryoanji_amd::check(ryo_upsweep_f64(ryoanji_amd::ctx(), x, y, z, m, layout, numLeaves, prefixes, childOffsets,
+                                       parents, full, d_levelRange, box.limitsT(), 1.0f / theta,
+                                       reinterpret_cast<double*>(centers), reinterpret_cast<double*>(multipoles),
+                                       nullptr));
+}
+
+//! the fused upsweep with CartesianMDQpole<float> multipoles (P2M / M2M for CartesianMDQpole)
+inline void upsweep(const float* x, const float* y, const float* z, const float* m, const LocalIndex* layout,
+                    int numLeaves, const uint64_t* prefixes, const TreeNodeIndex* childOffsets,
+                    const TreeNodeIndex* parents, const TreeNodeIndex* leafToInternal, const TreeNodeIndex* d_levelRange,
+                    const cstone::Box<float>& box, float theta, Vec4<float>* centers,
+                    CartesianMDQpole<float>* multipoles)
+{
+    const TreeNodeIndex* full = leafToInternal - (numLeaves - 1) / 7;
+    ryoanji_amd::check(ryo_upsweep_mdq_f32(ryoanji_amd::ctx(), x, y, z, m, layout, numLeaves, prefixes, childOffsets,
+                                       parents, full, d_levelRange, box.limits(), 1.0f / theta,
+                                       reinterpret_cast<float*>(centers), reinterpret_cast<float*>(multipoles),
+                                       nullptr, nullptr));
+}
+
+//! the FP64 upsweep for double particles with CartesianMDQpole<double> multipoles
+inline void upsweep(const double* x, const double* y, const double* z, const double* m, const LocalIndex* layout,
+                    int numLeaves, const uint64_t* prefixes, const TreeNodeIndex* childOffsets,
+                    const TreeNodeIndex* parents, const TreeNodeIndex* leafToInternal, const TreeNodeIndex* d_levelRange,
+                    const cstone::Box<double>& box, float theta, Vec4<double>* centers,
+                    CartesianMDQpole<double>* multipoles)
+{
+    const TreeNodeIndex* full = leafToInternal - (numLeaves - 1) / 7;
+    ryoanji_amd::check(ryo_upsweep_mdq_f64(ryoanji_amd::ctx(), x, y, z, m, layout, numLeaves, prefixes, childOffsets,
                                        parents, full, d_levelRange, box.limitsT(), 1.0f / theta,
                                        reinterpret_cast<double*>(centers), reinterpret_cast<double*>(multipoles),
                                        nullptr));
@@ -316,6 +346,52 @@ inline double traverse(GroupView grp, int initNodeIdx, const double* xt, const d
 {
     double pot = 0;
     ryoanji_amd::check(ryo_traverse_f64_f32(
+        ryoanji_amd::ctx(), int(grp.numGroups), grp.groupStart, grp.groupEnd, initNodeIdx, xt, yt, zt, mt, ht, xs, ys,
+        zs, ms, hs, childOffsets, internalToLeaf, layout, reinterpret_cast<const double*>(sourceCenter),
+        reinterpret_cast<const float*>(multipoles), G, numShells, boxL.data(), p, ax, ay, az, &pot));
+    return pot;
+}
+
+//! ryoanji::traverse with CartesianMDQpole multipoles, for each of the three (Tc,Th,Tm,Ta,Tf) instantiations
+inline double traverse(GroupView grp, int initNodeIdx, const float* xt, const float* yt, const float* zt,
+                       const float* mt, const float* ht, const float* xs, const float* ys, const float* zs,
+                       const float* ms, const float* hs, const TreeNodeIndex* childOffsets,
+                       const TreeNodeIndex* internalToLeaf, const LocalIndex* layout, const Vec4<float>* sourceCenter,
+                       const CartesianMDQpole<float>* multipoles, float G, int numShells, Vec3<float> boxL,
+                       float* p, float* ax, float* ay, float* az, int* /*gmPool*/ = nullptr)
+{
+    double pot = 0;
+    ryoanji_amd::check(ryo_traverse_mdq_f32(
+        ryoanji_amd::ctx(), int(grp.numGroups), grp.groupStart, grp.groupEnd, initNodeIdx, xt, yt, zt, mt, ht, xs, ys,
+        zs, ms, hs, childOffsets, internalToLeaf, layout, reinterpret_cast<const float*>(sourceCenter),
+        reinterpret_cast<const float*>(multipoles), G, numShells, boxL.data(), p, ax, ay, az, &pot));
+    return pot;
+}
+
+inline double traverse(GroupView grp, int initNodeIdx, const double* xt, const double* yt, const double* zt,
+                       const double* mt, const double* ht, const double* xs, const double* ys, const double* zs,
+                       const double* ms, const double* hs, const TreeNodeIndex* childOffsets,
+                       const TreeNodeIndex* internalToLeaf, const LocalIndex* layout, const Vec4<double>* sourceCenter,
+                       const CartesianMDQpole<double>* multipoles, double G, int numShells, Vec3<double> boxL,
+                       double* p, double* ax, double* ay, double* az, int* /*gmPool*/ = nullptr)
+{
+    double pot = 0;
+    ryoanji_amd::check(ryo_traverse_mdq_f64(
+        ryoanji_amd::ctx(), int(grp.numGroups), grp.groupStart, grp.groupEnd, initNodeIdx, xt, yt, zt, mt, ht, xs, ys,
+        zs, ms, hs, childOffsets, internalToLeaf, layout, reinterpret_cast<const double*>(sourceCenter),
+        reinterpret_cast<const double*>(multipoles), G, numShells, boxL.data(), p, ax, ay, az, &pot));
+    return pot;
+}
+
+inline double traverse(GroupView grp, int initNodeIdx, const double* xt, const double* yt, const double* zt,
+                       const float* mt, const float* ht, const double* xs, const double* ys, const double* zs,
+                       const float* ms, const float* hs, const TreeNodeIndex* childOffsets,
+                       const TreeNodeIndex* internalToLeaf, const LocalIndex* layout, const Vec4<double>* sourceCenter,
+                       const CartesianMDQpole<float>* multipoles, double G, int numShells, Vec3<double> boxL,
+                       float* p, float* ax, float* ay, float* az, int* /*gmPool*/ = nullptr)
+{
+    double pot = 0;
+    ryoanji_amd::check(ryo_traverse_mdq_f64_f32(
         ryoanji_amd::ctx(), int(grp.numGroups), grp.groupStart, grp.groupEnd, initNodeIdx, xt, yt, zt, mt, ht, xs, ys,
         zs, ms, hs, childOffsets, internalToLeaf, layout, reinterpret_cast<const double*>(sourceCenter),
         reinterpret_cast<const float*>(multipoles), G, numShells, boxL.data(), p, ax, ay, az, &pot));
