@@ -156,6 +156,13 @@ int ryo_upsweep_f64(ryo_ctx* ctx, const double* x, const double* y, const double
                     const uint32_t* layout, int num_leaves, const uint64_t* prefixes, const int32_t* child_offsets,
                     const int32_t* parents, const int32_t* leaf_to_internal, const int32_t* level_range,
                     const double* limits, float inv_theta, double* centers, double* multipoles, float* multipoles32);
+/* The double-particle upsweep with CartesianMDQpole<double> multipoles double[numNodes][12] (the layout of
+ * ryo_upsweep_mdq_f32), multipoles32 (optional) the CartesianMDQpole<float> copy; otherwise as ryo_upsweep_f64. */
+int ryo_upsweep_mdq_f64(ryo_ctx* ctx, const double* x, const double* y, const double* z, const double* m,
+                        const uint32_t* layout, int num_leaves, const uint64_t* prefixes, const int32_t* child_offsets,
+                        const int32_t* parents, const int32_t* leaf_to_internal, const int32_t* level_range,
+                        const double* limits, float inv_theta, double* centers, double* multipoles,
+                        float* multipoles32);
 
 /* ---------------------------------------------------------------- a13: target groups */
 
@@ -222,13 +229,6 @@ int ryo_traverse_f64(ryo_ctx* ctx, int num_groups, const uint32_t* group_start, 
                      int num_shells, const double* box_l, double* p, double* ax, double* ay, double* az,
                      double* total_pot);
 
-/* The double-particle upsweep with CartesianMDQpole<double> multipoles double[numNodes][12] (the layout of
- * ryo_upsweep_mdq_f32), multipoles32 (optional) the CartesianMDQpole<float> copy; otherwise as ryo_upsweep_f64. */
-int ryo_upsweep_mdq_f64(ryo_ctx* ctx, const double* x, const double* y, const double* z, const double* m,
-                        const uint32_t* layout, int num_leaves, const uint64_t* prefixes, const int32_t* child_offsets,
-                        const int32_t* parents, const int32_t* leaf_to_internal, const int32_t* level_range,
-                        const double* limits, float inv_theta, double* centers, double* multipoles,
-                        float* multipoles32);
 /* ryoanji::traverse for (d,d,d,d,d) with CartesianMDQpole<double> and (d,f,f,f,d) with CartesianMDQpole<float>
  * (traversal_gpu.cu:635-642): the arguments of ryo_traverse_f64 / ryo_traverse_f64_f32 with 12 values per
  * node; M2P adds the dipole terms (cartesian_qpole.hpp:320-357). */
