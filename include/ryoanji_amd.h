@@ -258,6 +258,24 @@ int ryo_traverse_f64_f32(ryo_ctx* ctx, int num_groups, const uint32_t* group_sta
                          int num_shells, const double* box_l, float* p, float* ax, float* ay, float* az,
                          double* total_pot);
 
+/* No counterpart in the reference. ryo_traverse_f32 for callers that cannot allocate device memory (code under HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc): its work memory (per-group FP64
+ * potentials, overflow list, control words, reduction partials) is a fixed array that the library holds on each
+ * GPU for the life of the process (12.0 MiB), so it serves at most RYO_NOALLOC_MAX_GROUPS target groups
+ * (RYO_ERR_CAPACITY beyond). It allocates nothing. It always walks with the full-depth LIFO, which no group can
+ * overflow, so there is no split/retry; where ryo_traverse_f32 needs none (its default LIFO depth) the results,
+ * statistics and total potential are the same bit for bit. Every call on one GPU uses that GPU's one array:
+ * calls that overlap on one GPU (other contexts or streams) race, so the caller must serialize them.
+ * Synchronizes the context's stream (statistics and potential read-back). */
+#define RYO_NOALLOC_MAX_GROUPS 1048576
+int ryo_traverse_f32_noalloc(ryo_ctx* ctx, int num_groups, const uint32_t* group_start, const uint32_t* group_end,
+                             int init_node, const float* xt, const float* yt, const float* zt, const float* mt,
+                             const float* ht, const float* xs, const float* ys, const float* zs, const float* ms,
+                             const float* hs, const int32_t* child_offsets, const int32_t* internal_to_leaf,
+                             const uint32_t* layout, const float* centers, const float* multipoles, float G,
+                             int num_shells, const float* box_l, float* p, float* ax, float* ay, float* az,
+                             double* total_pot);
+
 /* ryoanji::readBhStats (traversal_gpu.cu:688-700): host uint64[5] {sumP2P, maxP2P, sumM2P, maxM2P, maxStack}
  * of the last ryo_traverse_f32 call (sums are interactions summed over targets). */
 int ryo_read_stats(ryo_ctx* ctx, uint64_t* stats);

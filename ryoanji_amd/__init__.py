@@ -110,6 +110,8 @@ SIGNATURES = {
                                   C.c_size_t, C.POINTER(_I)]),
     "ryo_traverse_f32": (_I, [_P, _I, _P, _P, _I] + [_P] * 10 + [_P, _P, _P, _P, _P, _F, _I, _P, _P, _P, _P, _P,
                                                                    _P]),
+    "ryo_traverse_f32_noalloc": (_I, [_P, _I, _P, _P, _I] + [_P] * 10 + [_P, _P, _P, _P, _P, _F, _I, _P, _P, _P, _P,
+                                                                       _P, _P]),
     "ryo_traverse_mdq_f32": (_I, [_P, _I, _P, _P, _I] + [_P] * 10 + [_P, _P, _P, _P, _P, _F, _I, _P, _P, _P, _P, _P,
                                                                    _P]),
     "ryo_traverse_f64": (_I, [_P, _I, _P, _P, _I] + [_P] * 10 + [_P, _P, _P, _P, _P, _D, _I, _P, _P, _P, _P, _P,

@@ -57,7 +57,8 @@ int traverseEntry(Context* ctx, int numGroups, const uint32_t* gs, const uint32_
                   const float* ys, const float* zs, const float* ms, const float* hs, const int32_t* child,
                   const int32_t* i2l, const uint32_t* layout, const float* centers, const float* mp, float G,
                   int numShells, const float* boxL, float* p, float* ax, float* ay, float* az, double* totalPot,
-                  int stackCap, size_t hCount, bool mdq = false);
+                  int stackCap, size_t hCount, bool mdq = false, bool noAlloc = false);
+int initNoAllocScratch(Context* ctx);
 
 int setError(Context* ctx, int code, const char* fmt, ...)
 {
@@ -124,6 +125,13 @@ int ryo_ctx_create(ryo_ctx** out, void* stream)
         return RYO_ERR_HIP;
     }
     hipMemset(c->dStats, 0, 256);
+    // this device's work array of the allocation-free traversal (once per device)
+    if (initNoAllocScratch(c) != RYO_OK)
+    {
+        hipFree(c->dStats);
+        delete h;
+        return RYO_ERR_HIP;
+    }
     hipEventCreate(&c->ev[0]);
     hipEventCreate(&c->ev[1]);
     *out = h;
@@ -350,6 +358,20 @@ int ryo_traverse_mdq_f32(ryo_ctx* h, int num_groups, const uint32_t* gs, const u
     if (num_shells < 0 || (num_shells > 0 && !box_l)) return setError(ctx, RYO_ERR_ARG, "traverse: bad shells");
     return traverseEntry(ctx, num_groups, gs, ge, init_node, xt, yt, zt, mt, ht, xs, ys, zs, ms, hs, child, i2l,
                          layout, centers, mp, G, num_shells, box_l, p, ax, ay, az, total_pot, 0, 0, true);
+}
+
+int ryo_traverse_f32_noalloc(ryo_ctx* h, int num_groups, const uint32_t* gs, const uint32_t* ge, int init_node,
+                             const float* xt, const float* yt, const float* zt, const float* mt, const float* ht,
+                             const float* xs, const float* ys, const float* zs, const float* ms, const float* hs,
+                             const int32_t* child, const int32_t* i2l, const uint32_t* layout, const float* centers,
+                             const float* mp, float G, int num_shells, const float* box_l, float* p, float* ax,
+                             float* ay, float* az, double* total_pot)
+{
+    CTX_OR_FAIL(h);
+    if (init_node != 0 && init_node != 1) return setError(ctx, RYO_ERR_ARG, "traverse: init_node must be 0 or 1");
+    if (num_shells < 0 || (num_shells > 0 && !box_l)) return setError(ctx, RYO_ERR_ARG, "traverse: bad shells");
+    return traverseEntry(ctx, num_groups, gs, ge, init_node, xt, yt, zt, mt, ht, xs, ys, zs, ms, hs, child, i2l,
+                         layout, centers, mp, G, num_shells, box_l, p, ax, ay, az, total_pot, 0, 0, false, true);
 }
 
 /* test hook: the traversal with a reduced per-wave stack capacity (forces the split/retry path) */

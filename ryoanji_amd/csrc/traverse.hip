@@ -15,6 +15,7 @@
 #include "common.hpp"
 
 #include <cstring>
+#include <mutex>
 
 namespace ryo
 {
@@ -747,6 +748,117 @@ int traverse(Context* ctx, TravArgs a, double* totalPot)
     return RYO_OK;
 }
 
+/* ------------------------------------------------------------------ allocation-free traversal */
+
+// Work memory of ryo_traverse_f32_noalloc, one copy per GPU for the life of the process: 256 bytes of control
+// words (the layout of ctx->dStats: group counters, statistics, potential sum, softening range), 256 FP64
+// reduction partials, then per-group FP64 potentials and the overflow list for up to kNoAllocMaxGroups groups
+// (12 MiB in all). In host code `scratch` only names the symbol; its device address comes from
+// hipGetSymbolAddress, once per device (initNoAllocScratch), and only that address is passed to kernels.
+constexpr size_t kNoAllocMaxGroups = size_t(RYO_NOALLOC_MAX_GROUPS);
+constexpr size_t kNoAllocPartials  = 256;
+constexpr size_t kNoAllocPotOff    = 256 + kNoAllocPartials * sizeof(double);
+constexpr size_t kNoAllocFailedOff = kNoAllocPotOff + kNoAllocMaxGroups * sizeof(double);
+constexpr size_t kNoAllocBytes     = kNoAllocFailedOff + kNoAllocMaxGroups * sizeof(uint32_t);
+constexpr int kNoAllocMaxDevices   = 64;
+
+static __device__ __align__(256) unsigned char scratch[kNoAllocBytes];
+
+namespace
+{
+std::mutex g_noAllocMutex;
+void* g_noAllocPtr[kNoAllocMaxDevices] = {};
+} // namespace
+
+// resolve this device's copy of `scratch` (the context's device is current: ryo_ctx_create calls this)
+int initNoAllocScratch(Context* ctx)
+{
+    if (ctx->device < 0 || ctx->device >= kNoAllocMaxDevices)
+        return setError(ctx, RYO_ERR_UNSUPPORTED, "device %d: at most %d devices", ctx->device, kNoAllocMaxDevices);
+    std::lock_guard<std::mutex> lock(g_noAllocMutex);
+    if (g_noAllocPtr[ctx->device]) return RYO_OK;
+    void* ptr    = nullptr;
+    hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !ptr)
+        return setError(ctx, RYO_ERR_HIP, "hipGetSymbolAddress(scratch) on device %d: %s", ctx->device,
+                        hipGetErrorString(e));
+    g_noAllocPtr[ctx->device] = ptr;
+    return RYO_OK;
+}
+
+// the traversal of traverse() with every work array in this device's `scratch`: no allocation, no split/retry
+// (the full-depth LIFO cannot overflow), one read-back of statistics and potential
+int traverseNoAlloc(Context* ctx, TravArgs a, double* totalPot)
+{
+    hipStream_t s = ctx->stream;
+    for (int k = 0; k < 6; ++k)
+        ctx->hStats[k] = 0;
+    ctx->retriedGroups = 0;
+    if (a.numGroups <= 0)
+    {
+        if (totalPot) *totalPot = 0;
+        return RYO_OK;
+    }
+    if (size_t(a.numGroups) > kNoAllocMaxGroups)
+        return setError(ctx, RYO_ERR_CAPACITY, "allocation-free traversal: %d groups, at most %zu", a.numGroups,
+                        kNoAllocMaxGroups);
+    char* base = nullptr;
+    {
+        std::lock_guard<std::mutex> lock(g_noAllocMutex);
+        if (ctx->device >= 0 && ctx->device < kNoAllocMaxDevices)
+            base = static_cast<char*>(g_noAllocPtr[ctx->device]);
+    }
+    if (!base) return setError(ctx, RYO_ERR_HIP, "allocation-free traversal: scratch of device %d not resolved",
+                               ctx->device);
+
+    a.stackCap                = kMaxStack;
+    a.ranges                  = ctx->travRanges > 0 ? std::min(ctx->travRanges, kXcds) : kXcds;
+    unsigned* ctl             = reinterpret_cast<unsigned*>(base);
+    unsigned long long* stats = reinterpret_cast<unsigned long long*>(base) + 8;
+    double* potSum            = reinterpret_cast<double*>(base) + 16;
+    double* partials          = reinterpret_cast<double*>(base + 256);
+    a.work                    = ctl;
+    a.stats                   = stats;
+    a.hRange                  = ctl + kCtlHRange;
+    a.groupPot                = reinterpret_cast<double*>(base + kNoAllocPotOff);
+    a.failed                  = reinterpret_cast<uint32_t*>(base + kNoAllocFailedOff);
+    a.groupList               = nullptr;
+
+    const size_t G = size_t(a.numGroups);
+    travInitKernel<<<1, 64, 0, s>>>(ctl); // counters, statistics and potential 0; softening range empty
+    if (a.hCount > 0)
+        hRangeKernel<<<std::max(1u, std::min(iceil(a.hCount, 4096), unsigned(ctx->numCUs * 2))), 256, 0, s>>>(
+            a.ht, a.hCount, a.hs == a.ht ? nullptr : a.hs, a.hs == a.ht ? 0 : a.hCount, ctl + kCtlHRange);
+    zeroKernelU<<<std::min(iceil(2 * G, 256), 4096u), 256, 0, s>>>(reinterpret_cast<unsigned*>(a.groupPot),
+                                                                   int(2 * G));
+    if (ctx->timing) RYO_HIP(ctx, hipEventRecord(ctx->ev[0], s));
+    launchTraversal<false>(ctx, G, kMaxStack, s, a, TravBatch{});
+    if (ctx->timing) RYO_HIP(ctx, hipEventRecord(ctx->ev[1], s));
+    RYO_LAUNCH_CHECK(ctx);
+    sumDeterministic(s, a.groupPot, a.numGroups, partials, potSum);
+    RYO_LAUNCH_CHECK(ctx);
+
+    unsigned long long h[7] = {0, 0, 0, 0, 0, 0, 0};
+    double pot              = 0;
+    RYO_HIP(ctx, hipMemcpyAsync(h, stats, sizeof(h), hipMemcpyDeviceToHost, s));
+    RYO_HIP(ctx, hipMemcpyAsync(&pot, potSum, sizeof(double), hipMemcpyDeviceToHost, s));
+    RYO_HIP(ctx, hipStreamSynchronize(s));
+    if (ctx->timing)
+    {
+        float ms = 0;
+        RYO_HIP(ctx, hipEventElapsedTime(&ms, ctx->ev[0], ctx->ev[1]));
+        ctx->lastTraverseMs = ms;
+    }
+    if (h[6] != 0) return setError(ctx, RYO_ERR_TRAVERSAL, "traversal: step limit exceeded (corrupt tree?)");
+    if ((h[5] & 0xffffffffu) != 0)
+        return setError(ctx, RYO_ERR_TRAVERSAL, "allocation-free traversal: %u groups overflowed the full LIFO",
+                        unsigned(h[5] & 0xffffffffu));
+    for (int k = 0; k < 5; ++k)
+        ctx->hStats[k] = h[k];
+    if (totalPot) *totalPot = 0.5 * double(a.G) * pot;
+    return RYO_OK;
+}
+
 // per-system FP64 potential: block s sums the partials of groups [GO[s], GO[s+1]) in a fixed order (Neumaier)
 __global__ __launch_bounds__(256) void systemPotKernel(const double* __restrict__ groupPot, BatchView bv,
                                                        const float* __restrict__ G, double* __restrict__ out)
@@ -909,7 +1021,7 @@ int traverseEntry(Context* ctx, int numGroups, const uint32_t* gs, const uint32_
                   const float* ys, const float* zs, const float* ms, const float* hs, const int32_t* child,
                   const int32_t* i2l, const uint32_t* layout, const float* centers, const float* mp, float G,
                   int numShells, const float* boxL, float* p, float* ax, float* ay, float* az, double* totalPot,
-                  int stackCap, size_t hCount, bool mdq)
+                  int stackCap, size_t hCount, bool mdq, bool noAlloc)
 {
     if (numGroups > 0 && (!gs || !ge || !xt || !yt || !zt || !mt || !ht || !xs || !ys || !zs || !ms || !hs ||
                           !child || !i2l || !layout || !centers || !mp || !ax || !ay || !az))
@@ -929,7 +1041,7 @@ int traverseEntry(Context* ctx, int numGroups, const uint32_t* gs, const uint32_
     a.stackCap = stackCap;
     a.hCount   = hCount;
     a.mdq      = mdq;
-    return traverse(ctx, a, totalPot);
+    return noAlloc ? traverseNoAlloc(ctx, a, totalPot) : traverse(ctx, a, totalPot);
 }
 
 } // namespace ryo
