@@ -2,14 +2,19 @@
 
 One step = one full pass over one particle system already resident in HBM:
 SFC keys -> stable radix sort -> permute -> cornerstone leaves -> linked octree -> FP64 upsweep (centers, P2M,
-M2M, MAC) -> fixed 64-target groups -> traversal (M2P + P2P), plus the Ewald correction for periodic workloads.
+M2M, MAC) -> fixed 64-target groups -> traversal (M2P + P2P), plus the FP64 Ewald correction for periodic
+workloads.
 
-Default workload: N = 2^24 Plummer sphere (scale radius 1, truncated at r = 30), equal masses 1/N, fp32,
-softening 1e-3, theta 0.5, Cartesian quadrupoles, bucket 64, open boundaries.
+Default workload (the headline configuration): N = 2^26 particles uniform in a unit periodic cube, equal masses
+1/N, fp32, softening 1e-3, theta 0.5, Cartesian quadrupoles, bucket 64, traversal over 1 replica shell (27
+images) and the Ewald correction (1 replica shell, real-space cut 2.6 L, k-space cut 2.8, alpha 2/L) in FP64.
+Beside it, the same particles with open boundaries at theta 0.5 and 0.4 (`open_boundary`).
 
 Prints ONE JSON line. value = N / (mean wall time of a pass) in particles/s. The roofline object is for the
 dominant kernel (the traversal): algorithmic flops per launch (23 per P2P + 128 per M2P, the count of the
-reference's README) over the kernel's mean duration from HIP events on the library's stream, against the FP32 vector peak.
+reference's README) over the kernel's mean duration from HIP events on the library's stream, against the FP32
+vector peak. `ewald_roofline` prices the Ewald kernel against the FP64 vector peak, `build_roofline` the
+key -> sort -> build -> upsweep phase against the HBM peak.
 """
 
 from __future__ import annotations
@@ -24,15 +29,23 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 PEAK_FP32_TFLOPS = 157.3  # MI355X FP32 vector peak (AMD published)
+PEAK_FP64_TFLOPS = 78.6   # MI355X FP64 vector peak (AMD published)
 PEAK_HBM_GBS = 8000.0
+EWALD_SETTINGS = (1, 2.6, 2.8, 2.0, 3.0e-3)  # numReplicaShells, lCut, hCut, alpha_scale, small_R_scale_factor
 METRIC = "particles/s + P2P/M2P TFLOP/s at θ=0.5; served p50/p99 vs plain solves"
 
 WORKLOADS = {
-    "plummer24": dict(kind="plummer", n=1 << 24, periodic=False),
+    "uniform26p": dict(kind="uniform_periodic", n=1 << 26, periodic=True),  # the headline (default)
     "uniform26": dict(kind="uniform", n=1 << 26, periodic=False),
-    "uniform26p": dict(kind="uniform_periodic", n=1 << 26, periodic=True),
+    "plummer24": dict(kind="plummer", n=1 << 24, periodic=False),
     "hernquist24": dict(kind="hernquist", n=1 << 24, periodic=False),
 }
+DEFAULT_WORKLOAD = "uniform26p"
+
+
+def log(msg):
+    """a progress line on stderr (the JSON result is the only line on stdout)"""
+    print(f"[bench {time.strftime('%H:%M:%S')}] {msg}", file=sys.stderr, flush=True)
 
 
 def make_system(name, n=None, seed=42):
@@ -61,10 +74,23 @@ def traffic_from_profiles(kernel="traverseKernel", workload="plummer24"):
         return None
 
 
-def cpu_baseline(s, sample_frac=1 / 128, seed=0):
+def cpu_model():
+    """the host CPU's model name (first processor in /proc/cpuinfo)"""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(s, sample_frac=1 / 128, seed=0, num_shells=0, ewald_settings=None, budget_s=15.0):
     """Reference CPU path (oracle/_ref, the reference's OpenMP code) when it was built, else the restatement:
-    keys, stable sort, permutation, leaves, linked tree and upsweep on the full system; the traversal on a seeded
-    sample of leaf ranges, scaled to all particles."""
+    keys, stable sort, permutation, leaves, linked tree and upsweep on the full system; the traversal (over the
+    replica images when num_shells > 0) on a seeded sample of leaf ranges and, for a periodic system, the Ewald
+    correction (ewald_settings) of the same sampled particles, both scaled to all particles."""
     import numpy as np
 
     import oracle as O
@@ -87,30 +113,81 @@ def cpu_baseline(s, sample_frac=1 / 128, seed=0):
     starts = np.sort(rng.choice(max(1, L - span), nchunks, replace=False))
     t1 = time.perf_counter()
     nsample = 0
-    if kind == "reference":
-        for a in starts:
-            O.compute_gravity(leaves, layout, c, q, x, y, z, h, m, s.limits, lib="ref",
+    sample_idx = []
+    used = 0
+    for a in starts:  # chunk by chunk, stopping once the traversal sample has taken `budget_s`
+        if kind == "reference":
+            O.compute_gravity(leaves, layout, c, q, x, y, z, h, m, s.limits, lib="ref", num_shells=num_shells,
                               leaf_range=(int(a), int(a + span)), want_pot=False)
             nsample += int(layout[a + span] - layout[a])
-    else:
-        groups = []
-        for a in starts:
+            sample_idx.append(np.arange(layout[a], layout[a + span]))
+        else:
             g0, g1 = int(layout[a]) // 16, (int(layout[a + span]) + 15) // 16
-            groups.extend(range(g0, g1))
-        groups = np.unique(np.array(groups, np.int32))
-        O.compute_gravity(leaves, layout, c, q, x, y, z, h, m, s.limits, group_list=groups, want_pot=False)
-        nsample = int(groups.size * 16)
+            groups = np.arange(g0, g1, dtype=np.int32)
+            O.compute_gravity(leaves, layout, c, q, x, y, z, h, m, s.limits, group_list=groups, want_pot=False,
+                              num_shells=num_shells)
+            nsample += int(groups.size * 16)
+            sample_idx.append(np.arange(g0 * 16, min(g1 * 16, s.n)))
+        used += 1
+        if time.perf_counter() - t1 > budget_s:
+            break
+    nchunks = used
     t_trav = time.perf_counter() - t1
-    t_full = t_build + t_trav * s.n / max(nsample, 1)
+    t_ewald, ewald_note = 0.0, ""
+    if ewald_settings is not None:
+        idx = np.concatenate(sample_idx)
+        mroot, center = q[0].astype(np.float64), c[0, :3].astype(np.float64)
+        t2 = time.perf_counter()
+        if kind == "reference":
+            O.ref_ewald(mroot, center, *(a[idx].astype(np.float64) for a in (x, y, z, m)), s.limits,
+                        settings=ewald_settings, precision="f64")
+        else:
+            O.ewald(mroot, center, x[idx], y[idx], z[idx], m[idx], float(s.limits[1] - s.limits[0]),
+                    settings=ewald_settings)
+        t_ewald = time.perf_counter() - t2
+        ewald_note = f", Ewald correction (FP64) of the same {idx.size} particles ({t_ewald:.2f} s)"
+    t_full = t_build + (t_trav + t_ewald) * s.n / max(nsample, 1)
+    images = f" over {(2 * num_shells + 1) ** 3} images" if num_shells else ""
     return {
         "value": s.n / t_full,
         "unit": "particles/s",
         "cores": threads,
+        "cpu_model": cpu_model(),
         "kind": kind,
-        "sample": (f"full keys/sort/permute/leaves/tree/upsweep of the same N={s.n} system ({t_build:.2f} s), "
-                   f"traversal (16-target groups) timed on {nsample} particles in {nchunks} seeded leaf ranges "
-                   f"({t_trav:.2f} s) and scaled by N/{nsample}"),
+        "sample": (f"extrapolated: full keys/sort/permute/leaves/tree/upsweep of the same N={s.n} system "
+                   f"({t_build:.2f} s), traversal (16-target groups{images}) timed on {nsample} particles in "
+                   f"{nchunks} seeded leaf ranges ({t_trav:.2f} s){ewald_note}, scaled by N/{nsample}"),
     }
+
+
+def ewald_flops(s, center, settings=EWALD_SETTINGS, sample=65536, seed=3):
+    """Algorithmic FP64 flops of the Ewald correction per particle (ewald.hpp:201-327), counted from the terms the
+    reference evaluates: every real-space image with |r - c + n L| <= lCut L or inside the replica shells costs
+    77 flops (image offset 6, |R|^2 5, 1/R and 1/R^2 3, the Gaussian factor 4, erf/erfc 2, gamma_0..3 13,
+    quadrupole evaluation and accumulation 44), every k-space vector 20 (h.r 5, cos and sin 2, the two
+    combinations 6, accumulation 7), plus 10 per particle; exp, erf, erfc, sqrt, cos and sin count as one flop
+    each. The image count is averaged over `sample` seeded particles."""
+    import itertools
+    import math
+
+    import numpy as np
+
+    nr, lcut, hcut = int(settings[0]), float(settings[1]), float(settings[2])
+    L = float(s.limits[1] - s.limits[0])
+    ns = max(int(math.ceil(lcut)), nr)
+    idx = np.random.default_rng(seed).choice(s.n, min(sample, s.n), replace=False)
+    r = np.stack([s.x[idx], s.y[idx], s.z[idx]], axis=1).astype(np.float64) - np.asarray(center, np.float64)
+    cut2 = lcut * lcut * L * L
+    count = np.zeros(idx.size)
+    for ix, iy, iz in itertools.product(range(-ns, ns + 1), repeat=3):
+        R = r + np.array([ix, iy, iz], np.float64) * L
+        inner = max(abs(ix), abs(iy), abs(iz)) <= nr
+        count += (np.einsum("ij,ij->i", R, R) <= cut2) | inner
+    hr = int(math.ceil(hcut))
+    nk = sum(1 for hx, hy, hz in itertools.product(range(-hr, hr + 1), repeat=3)
+             if 0 < hx * hx + hy * hy + hz * hz <= hcut * hcut)
+    images = float(count.mean())
+    return 77.0 * images + 20.0 * nk + 10.0, images, nk
 
 
 def build_roofline(phase, n, nodes):
@@ -268,16 +345,34 @@ def batching_bench(classes=(12, 14, 16), total_exp=22, seed=9):
     return rows
 
 
+def trav_summary(s, stats, t_trav_s):
+    """traversal rates of one pass: TFLOP/s by the 23/128 count (north star, README) and by the 23/65 count (the
+    quadrupole actually computed), particles/s of the traversal alone, interactions per particle"""
+    flops_ns = 23.0 * stats["sumP2P"] + 128.0 * stats["sumM2P"]
+    flops_code = 23.0 * stats["sumP2P"] + 65.0 * stats["sumM2P"]
+    return {
+        "traversal_ms": t_trav_s * 1e3,
+        "tflops": flops_ns / t_trav_s / 1e12,
+        "frac_of_fp32_peak": flops_ns / t_trav_s / 1e12 / PEAK_FP32_TFLOPS,
+        "tflops_code_count": flops_code / t_trav_s / 1e12,
+        "flops_per_launch": flops_ns,
+        "particles_per_s_traversal": s.n / t_trav_s,
+        "interactions_per_particle": {"p2p": stats["sumP2P"] / s.n, "m2p": stats["sumM2P"] / s.n,
+                                      "max_stack": stats["maxStack"]},
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--workload", default="plummer24", choices=list(WORKLOADS))
+    ap.add_argument("--workload", default=DEFAULT_WORKLOAD, choices=list(WORKLOADS))
     ap.add_argument("--n", type=int, default=None, help="override the particle count")
     ap.add_argument("--theta", type=float, default=0.5)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-accuracy", action="store_true")
+    ap.add_argument("--no-open", action="store_true", help="skip the open-boundary passes beside a periodic run")
     ap.add_argument("--no-serving", action="store_true")
     ap.add_argument("--no-batching", action="store_true")
     ap.add_argument("--trav-stack", type=int, default=0, help="traversal LIFO entries per wave (0: default)")
@@ -305,6 +400,7 @@ def main():
             torch.mul(src, 2.0, out=dst)
         torch.cuda.synchronize()
         del src, dst
+    log(f"workload {args.workload}: generating particles")
     s = make_system(args.workload, args.n)
     spec = WORKLOADS[args.workload]
     ctx = R.Context()
@@ -312,13 +408,18 @@ def main():
     ctx.call("ryo_ctx_set_traversal_shape", args.trav_stack, args.trav_blocks, args.trav_ranges)
     sysm = R.System(ctx, s.n)
     dev = [torch.from_numpy(a).cuda() for a in s.arrays()]
-    prm = R.default_params(theta=args.theta, group_size=64, bucket_size=64)
-    if spec["periodic"]:
-        prm.num_shells, prm.ewald = 1, 1
 
-    def one_pass():
+    def params(theta, periodic):
+        prm = R.default_params(theta=theta, group_size=64, bucket_size=64)
+        if periodic:
+            prm.num_shells, prm.ewald = 1, 1
+            for k, v in enumerate(EWALD_SETTINGS):
+                prm.ewald_settings[k] = v
+        return prm
+
+    def one_pass(prm, periodic):
         # inputs resident in HBM: reset to the unsorted initial state (device-to-device), then a full pass
-        sysm.load(*dev, s.limits, periodic=spec["periodic"])
+        sysm.load(*dev, s.limits, periodic=periodic)
         torch.cuda.synchronize()
         t = time.perf_counter()
         sysm.build(prm)
@@ -326,26 +427,30 @@ def main():
         torch.cuda.synchronize()
         return time.perf_counter() - t, pot
 
-    for _ in range(args.warmup):
-        one_pass()
-    times, trav_ms, phase = [], [], []
-    stats = None
-    pot = 0.0
-    for _ in range(args.steps):
-        dt, pot = one_pass()
-        times.append(dt)
-        ph = sysm.phase_ms()
-        phase.append(ph)
-        trav_ms.append(ph["traverse"])
-        stats = ctx.stats()
-    t_pass = float(np.mean(times))
-    t_trav = float(np.mean(trav_ms)) / 1e3
+    def run(prm, periodic, steps, warmup):
+        for _ in range(warmup):
+            one_pass(prm, periodic)
+        times, phase = [], []
+        pot, stats = 0.0, None
+        for _ in range(steps):
+            dt, pot = one_pass(prm, periodic)
+            times.append(dt)
+            phase.append(sysm.phase_ms())
+            stats = ctx.stats()
+        return float(np.mean(times)), phase, stats, pot
+
+    periodic = spec["periodic"]
+    prm = params(args.theta, periodic)
+    log(f"N={s.n}: {args.warmup} warmup + {args.steps} timed passes")
+    t_pass, phase, stats, pot = run(prm, periodic, args.steps, args.warmup)
+    log(f"pass {t_pass * 1e3:.1f} ms ({s.n / t_pass / 1e6:.1f} M particles/s)")
+    t_trav = float(np.mean([ph["traverse"] for ph in phase])) / 1e3
     info = sysm.info()
-    flops_ns = 23.0 * stats["sumP2P"] + 128.0 * stats["sumM2P"]
-    flops_code = 23.0 * stats["sumP2P"] + 65.0 * stats["sumM2P"]
-    achieved = flops_ns / t_trav / 1e12
+    trav = trav_summary(s, stats, t_trav)
     p, ax, ay, az, order = (t.cpu().numpy() for t in sysm.results())
 
+    bc = "periodic, traversal over 1 replica shell (27 images) + FP64 Ewald correction (1 replica shell, lCut 2.6, " \
+         "hCut 2.8, alpha 2/L)" if periodic else "open boundaries"
     out = {
         "metric": METRIC,
         "value": s.n / t_pass,
@@ -360,81 +465,110 @@ def main():
         "dtype": "f32",
         "data": "synthetic",
         "config": {
-            "workload": f"{args.workload}: N={s.n}, theta={args.theta}, cartesian quadrupole, bucket 64, "
-                        f"softening 1e-3, 64-target groups" + (", periodic 1 shell + Ewald" if spec["periodic"] else
-                                                               ", open boundaries"),
-            "n": s.n, "theta": args.theta, "bucket": 64, "group_size": 64,
+            "workload": f"{args.workload}: N={s.n} {spec['kind']} particles, theta={args.theta}, cartesian "
+                        f"quadrupole, bucket 64, softening 1e-3, 64-target groups, {bc}",
+            "n": s.n, "theta": args.theta, "bucket": 64, "group_size": 64, "periodic": periodic,
             "traversal_shape": {"stack_per_wave": args.trav_stack or "default",
                                 "blocks_per_cu": args.trav_blocks or "default",
                                 "group_ranges": args.trav_ranges or "default"},
         },
         "roofline": {
             "bound": "valu",
-            "achieved": achieved,
+            "achieved": trav["tflops"],
             "peak": PEAK_FP32_TFLOPS,
             "unit": "TFLOP/s",
-            "frac": achieved / PEAK_FP32_TFLOPS,
+            "frac": trav["frac_of_fp32_peak"],
             "traffic": traffic_from_profiles(workload=args.workload),
             "kernel": "traverseKernel",
-            "flops_per_launch": flops_ns,
+            "flops_per_launch": trav["flops_per_launch"],
             "mean_launch_ms": t_trav * 1e3,
         },
         "build_roofline": build_roofline(phase, s.n, info["numNodes"]),
-        "tflops_code_count": flops_code / t_trav / 1e12,
-        "particles_per_s_traversal": s.n / t_trav,
-        "interactions_per_particle": {"p2p": stats["sumP2P"] / s.n, "m2p": stats["sumM2P"] / s.n,
-                                      "max_stack": stats["maxStack"]},
+        "tflops_code_count": trav["tflops_code_count"],
+        "particles_per_s_traversal": trav["particles_per_s_traversal"],
+        "interactions_per_particle": trav["interactions_per_particle"],
         "phases_ms": {k: float(np.mean([ph[k] for ph in phase])) for k in phase[0]},
         "tree": info,
         "total_potential": pot,
     }
+    if periodic:
+        t_ew = out["phases_ms"]["ewald"] / 1e3
+        center = sysm.tree()["centers64"][:3].cpu().numpy()
+        per, images, nk = ewald_flops(s, center)
+        out["ewald_roofline"] = {"bound": "fp64 valu", "achieved": per * s.n / t_ew / 1e12, "peak": PEAK_FP64_TFLOPS,
+                                 "unit": "TFLOP/s", "frac": per * s.n / t_ew / 1e12 / PEAK_FP64_TFLOPS,
+                                 "kernel": "ewaldKernel", "mean_launch_ms": t_ew * 1e3,
+                                 "flops_per_particle": per, "images_per_particle": images, "k_vectors": nk,
+                                 "particles_per_s": s.n / t_ew}
     if not args.no_accuracy:
-        if spec["periodic"]:
+        log("accuracy against FP64 references")
+        if periodic:
+            L = float(s.limits[1] - s.limits[0])
             # the full periodic result (traversal over the replica images + Ewald correction) against the FP64
             # Ewald pair sum over all images; its potentials carry the reference's k-space potential sign, so
             # pot_rel_err there is that convention, not an accuracy figure
-            L = [float(s.limits[1] - s.limits[0]), float(s.limits[3] - s.limits[2]), float(s.limits[5] - s.limits[4])]
             out["accuracy_vs_fp64_ewald"] = accuracy(ctx, sysm, s, order, ax, ay, az, pot, num_targets=1024,
-                                                     ewald_l=L[0], p=p)
+                                                     ewald_l=L, p=p)
             # the same solve with the k-space potential consistent with its acceleration (ewald = 2): the
             # accelerations are identical, the potentials follow the exact sum
             prm.ewald = 2
             sysm.gravity(prm)
             p2, ax2, ay2, az2, ord2 = (t.cpu().numpy() for t in sysm.results())
             out["accuracy_vs_fp64_ewald_consistent_potential"] = accuracy(ctx, sysm, s, ord2, ax2, ay2, az2, pot,
-                                                                          num_targets=1024, ewald_l=L[0], p=p2)
+                                                                          num_targets=1024, ewald_l=L, p=p2)
             # the traversal over the replica images alone (no Ewald term) against the FP64 direct sum over the
             # same images; the Ewald term is FP64 in both and is reported separately in phases_ms
             prm.ewald = 0
             sysm.gravity(prm)
-            _, tax, tay, taz, tord = (t.cpu().numpy() for t in sysm.results())
-            L = [float(s.limits[1] - s.limits[0]), float(s.limits[3] - s.limits[2]), float(s.limits[5] - s.limits[4])]
+            p0, tax, tay, taz, tord = (t.cpu().numpy() for t in sysm.results())
             out["accuracy_vs_fp64_direct"] = accuracy(ctx, sysm, s, tord, tax, tay, taz, pot, num_targets=1024,
-                                                      box_l=L, shells=1)
+                                                      box_l=[L, L, L], shells=1, p=p0)
+            prm.ewald = 1
         else:
-            out["accuracy_vs_fp64_direct"] = accuracy(ctx, sysm, s, order, ax, ay, az, pot)
+            out["accuracy_vs_fp64_direct"] = accuracy(ctx, sysm, s, order, ax, ay, az, pot, p=p)
+    if periodic and not args.no_open:
+        # the same particles with open boundaries (the north star's 2^26 uniform throughput and accuracy target),
+        # at theta 0.5 and 0.4
+        rows = []
+        for th in (args.theta, 0.4):
+            po = params(th, False)
+            log(f"open boundaries, theta {th}")
+            t_o, ph_o, st_o, pot_o = run(po, False, max(3, args.steps // 4), 1)
+            t_tr = float(np.mean([ph["traverse"] for ph in ph_o])) / 1e3
+            row = {"theta": th, "ms_per_step": t_o * 1e3, "particles_per_s": s.n / t_o,
+                   "phases_ms": {k: float(np.mean([ph[k] for ph in ph_o])) for k in ph_o[0]},
+                   **trav_summary(s, st_o, t_tr), "total_potential": pot_o}
+            if not args.no_accuracy:
+                po_, oax, oay, oaz, oord = (t.cpu().numpy() for t in sysm.results())
+                row["accuracy_vs_fp64_direct"] = accuracy(ctx, sysm, s, oord, oax, oay, oaz, pot_o, p=po_)
+            rows.append(row)
+        out["open_boundary"] = rows
     if spec["kind"] == "hernquist" and not args.no_accuracy:
         # the precision stress: the same pass with the softening-safe MAC (cells nearer than their radius plus
         # 2 h_max are opened, so no softened pair is approximated by a multipole)
         prm.soft_mac = 1
         sm_t = []
         for _ in range(2):
-            dt, _pot = one_pass()
+            dt, _pot = one_pass(prm, periodic)
             sm_t.append(dt)
         sst = ctx.stats()
         p3, ax3, ay3, az3, ord3 = (t.cpu().numpy() for t in sysm.results())
         out["soft_mac"] = {"ms_per_step": float(np.mean(sm_t)) * 1e3, "particles_per_s": s.n / float(np.mean(sm_t)),
                            "interactions_per_particle": {"p2p": sst["sumP2P"] / s.n, "m2p": sst["sumM2P"] / s.n},
-                           "accuracy_vs_fp64_direct": accuracy(ctx, sysm, s, ord3, ax3, ay3, az3, _pot)}
+                           "accuracy_vs_fp64_direct": accuracy(ctx, sysm, s, ord3, ax3, ay3, az3, _pot, p=p3)}
         prm.soft_mac = 0
     sysm.close()
     del dev
     if not args.no_serving:
+        log("serving stream")
         out["serving"] = serving_bench()
     if not args.no_batching:
+        log("batches of small systems")
         out["batching"] = batching_bench()
     if not args.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(s)
+        log("CPU baseline")
+        out["cpu_baseline"] = cpu_baseline(s, num_shells=1 if periodic else 0,
+                                           ewald_settings=EWALD_SETTINGS if periodic else None)
     print(json.dumps(out))
 
 
