@@ -64,7 +64,8 @@ int ryo_ctx_set_traversal_waves(ryo_ctx* ctx, int waves_per_block);
 /* cstone::computeSfcKeys(execution::Gpu, x, y, z, keys, n, Box<float>) -- cstone/sfc/sfc_gpu.h:37,
  * sfc_gpu.cu:23-44. limits = host float[6] {xmin,xmax,ymin,ymax,zmin,zmax}; kind 0 = Hilbert, 1 = Morton.
  * With skip_removed != 0 a key already equal to 2^63 (removeKey) is left untouched, as in the reference;
- * with 0 the keys array is write-only. Cubic boxes only (RYO_ERR_UNSUPPORTED otherwise). */
+ * with 0 the keys array is write-only. Non-cubic boxes get the mixed-dimension Hilbert keys of
+ * hilbert.hpp:117-244 (box bits reduced per axis as Box<float>::getBoxDimBits, box.hpp:160-198). */
 int ryo_sfc_keys_f32(ryo_ctx* ctx, const float* x, const float* y, const float* z, uint64_t* keys, size_t n,
                      const float* limits, int kind, int skip_removed);
 /* cstone::computeSfcKeys for double coordinates (the Box<double> instantiations, sfc_gpu.cu:46-62): cell indices
@@ -113,6 +114,25 @@ int ryo_gather_f64(ryo_ctx* ctx, const int32_t* perm, size_t n, int num_arrays, 
  * *num_leaves (host) receives L. If L > capacity, returns RYO_ERR_CAPACITY and *num_leaves = L. */
 int ryo_build_leaves_u64(ryo_ctx* ctx, const uint64_t* sorted_keys, size_t n, unsigned bucket_size,
                          uint64_t* leaves, uint32_t* counts, uint32_t* layout, size_t capacity, int* num_leaves);
+
+/* The cornerstone leaf array one count / rebalance step at a time, for callers that iterate
+ * updateOctreeGpu themselves (cstone/tree/update_gpu.cuh:43-68; ryo_build_leaves_u64 gives the converged array in
+ * one call). `tree` holds num_nodes + 1 keys ending with 2^63.
+ * cstone::computeNodeCountsGpu (csarray_gpu.h:41-48, csarray_gpu.cu:86-122): counts[i] = particles of the sorted
+ * keys[0..n) in [tree[i], tree[i+1]), capped at max_count (the reference's count guess only speeds its search). */
+int ryo_node_counts_u64(ryo_ctx* ctx, const uint64_t* tree, int num_nodes, const uint64_t* keys, size_t n,
+                        unsigned max_count, uint32_t* counts);
+/* cstone::computeNodeOpsGpu (csarray_gpu.h:59-65, csarray_gpu.cu:182-205): node_ops [num_nodes + 1] receives the
+ * exclusive scan of the per-node operation (0 merge, 1 keep, 8/64/512/4096 split; calculateNodeOp,
+ * csarray.hpp:267-290); *new_num_nodes (host) = node_ops[num_nodes]. Whether any node changed is kept in the context
+ * for the next ryo_rebalance_tree_u64 (the reference's rebalanceChangeCounter). Synchronizes the stream. */
+int ryo_node_ops_u64(ryo_ctx* ctx, const uint64_t* tree, int num_nodes, const uint32_t* counts, unsigned bucket_size,
+                     int32_t* node_ops, int* new_num_nodes);
+/* cstone::rebalanceTreeGpu (csarray_gpu.h:67-73, csarray_gpu.cu:212-231): new_tree [new_num_nodes + 1] from the
+ * scanned operations (processNode, csarray.hpp:339-366); *converged (host) = no node changed in the last
+ * ryo_node_ops_u64 on this context. Synchronizes the stream. */
+int ryo_rebalance_tree_u64(ryo_ctx* ctx, const uint64_t* tree, int num_nodes, int new_num_nodes,
+                           const int32_t* node_ops, uint64_t* new_tree, int* converged);
 
 /* ---------------------------------------------------------------- a6: linked octree */
 
@@ -163,6 +183,38 @@ int ryo_upsweep_mdq_f64(ryo_ctx* ctx, const double* x, const double* y, const do
                         const int32_t* parents, const int32_t* leaf_to_internal, const int32_t* level_range,
                         const double* limits, float inv_theta, double* centers, double* multipoles,
                         float* multipoles32);
+
+/* The upsweep one reference step at a time (the call sequence of ryoanji/test/demo.cu:192-219), for callers that
+ * drive it themselves; ryo_upsweep_f32 / _f64 do the same in one FP64 call. `types` names the reference
+ * instantiation: 0 (Tc, Tm, Tf) = (float, float, float) with MType<float>; 1 (double, double, double) with
+ * MType<double>; 2 (double, float, double) with MType<float>. mdq != 0: CartesianMDQpole (12 values per node),
+ * else CartesianQuadrupole (8). Centers are Tf[n][4]. Sums are formed in FP64 and stored in the caller's types.
+ * cstone::computeLeafSourceCenterGpu (source_center_gpu.h:66-75, source_center_gpu.cu:103-162): the |m|-weighted
+ * center {com, sum|m|} of each leaf, stored at leaf_to_internal[leaf] (the leaf part of the map, num_leaves
+ * entries, as TreeBuilder::leafToInternal() returns it). */
+int ryo_leaf_centers(ryo_ctx* ctx, int types, const void* x, const void* y, const void* z, const void* m,
+                     const int32_t* leaf_to_internal, int num_leaves, const uint32_t* layout, void* centers);
+/* cstone::upsweepCentersGpu (source_center_gpu.h:77-88, source_center_gpu.cu:172-205): levels num_levels - 1
+ * down to 0, one launch per level from the HOST level ranges level_range_host[num_levels + 1] (as
+ * OctreeData::levelRange); types 0 float centers, 1 or 2 double centers. */
+int ryo_upsweep_centers(ryo_ctx* ctx, int types, int num_levels, const int32_t* level_range_host,
+                        const int32_t* child_offsets, void* centers);
+/* cstone::setMacGpu (source_center_gpu.h:107-116, source_center_gpu.cu:284-310): centers[i][3] = squared vector-MAC
+ * radius (computeVecMacR2, macs.hpp:64-82, evaluated in FP64), 0 for empty cells. limits = host T[6] (float for
+ * types 0, double otherwise). */
+int ryo_set_mac(ryo_ctx* ctx, int types, const uint64_t* prefixes, int num_nodes, void* centers, float inv_theta,
+                const void* limits);
+/* ryoanji::computeLeafMultipoles (nbody/upsweep_gpu.h:25, upsweep_gpu.cu:33-77): P2M of each leaf about its
+ * center (P2M_add / P2M_finalize, cartesian_qpole.hpp:67-117; the MDQpole form :237-295). */
+int ryo_leaf_multipoles(ryo_ctx* ctx, int types, int mdq, const void* x, const void* y, const void* z, const void* m,
+                        const int32_t* leaf_to_internal, int num_leaves, const uint32_t* layout, const void* centers,
+                        void* multipoles);
+/* ryoanji::upsweepMultipoles (nbody/upsweep_gpu.h:40, upsweep_gpu.cu:84-128): M2M of the cells
+ * [first_cell, last_cell) from their 8 children about the stored centers (addQuadrupole, cartesian_qpole.hpp:187-235;
+ * :359-417 for MDQpole); the caller loops over the levels. types: 0 (T, MType) = (float, float), 1 (double,
+ * double), 2 (double, float). */
+int ryo_upsweep_multipoles(ryo_ctx* ctx, int types, int mdq, int first_cell, int last_cell,
+                           const int32_t* child_offsets, const void* centers, void* multipoles);
 
 /* ---------------------------------------------------------------- a13: target groups */
 
@@ -303,6 +355,14 @@ int ryo_direct_sum_f64(ryo_ctx* ctx, const float* x, const float* y, const float
                        const float* m, size_t n, const int64_t* targets, size_t num_targets, double G,
                        const double* box_l, int num_shells, double* out);
 
+/* ryoanji::directSum<T> (interface/multipole_holder.cuh:92, nbody/direct.cuh:88-107) in the reference's form:
+ * targets [first, last) against sources [0, num_bodies) and their (2 num_shells + 1)^3 images of box_l (host
+ * T[3]); P2P<Exact> accumulated in FP64 and ADDED to p (m_i phi_i), ax, ay, az (G = 1, as the reference).
+ * types 0: float arrays, 1: double arrays. */
+int ryo_direct_sum_range(ryo_ctx* ctx, int types, size_t first, size_t last, size_t num_bodies, const void* box_l,
+                         int num_shells, const void* x, const void* y, const void* z, const void* m, const void* h,
+                         void* p, void* ax, void* ay, void* az);
+
 /* The exact periodic potential and acceleration of the softened particles in a cubic box of side box_length,
  * in FP64, for a list of targets: the Ewald sum over all n sources and all their images (the convention of
  * ewald.hpp:201-327: neutralizing background), with pair images closer than h_i + h_j interacting by the
@@ -313,6 +373,13 @@ int ryo_direct_sum_f64(ryo_ctx* ctx, const float* x, const float* y, const float
 int ryo_ewald_pair_sum_f64(ryo_ctx* ctx, const float* x, const float* y, const float* z, const float* h,
                            const float* m, size_t n, const int64_t* targets, size_t num_targets, double G,
                            double box_length, double alpha_scale, double* out);
+
+/* Device memory for host callers that do not link the HIP runtime themselves (cstone::DeviceVector in
+ * include/ryoanji_amd.hpp, cstone/cuda/device_vector.h:33-79): allocation, release (after the context's stream has
+ * drained) and a copy in any direction (hipMemcpyDefault) that has finished on return. */
+int ryo_malloc(ryo_ctx* ctx, void** ptr, size_t bytes);
+int ryo_free(ryo_ctx* ctx, void* ptr);
+int ryo_memcpy(ryo_ctx* ctx, void* dst, const void* src, size_t bytes);
 
 /* ---------------------------------------------------------------- system pipeline */
 

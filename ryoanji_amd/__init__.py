@@ -122,6 +122,18 @@ SIGNATURES = {
                                                                        _P, _P]),
     "ryo_traverse_mdq_f64_f32": (_I, [_P, _I, _P, _P, _I] + [_P] * 10 + [_P, _P, _P, _P, _P, _D, _I, _P, _P, _P, _P,
                                                                        _P, _P]),
+    "ryo_node_counts_u64": (_I, [_P, _P, _I, _P, _SZ, _U, _P]),
+    "ryo_node_ops_u64": (_I, [_P, _P, _I, _P, _U, _P, C.POINTER(_I)]),
+    "ryo_rebalance_tree_u64": (_I, [_P, _P, _I, _I, _P, _P, C.POINTER(_I)]),
+    "ryo_leaf_centers": (_I, [_P, _I, _P, _P, _P, _P, _P, _I, _P, _P]),
+    "ryo_upsweep_centers": (_I, [_P, _I, _I, _P, _P, _P]),
+    "ryo_set_mac": (_I, [_P, _I, _P, _I, _P, _F, _P]),
+    "ryo_leaf_multipoles": (_I, [_P, _I, _I, _P, _P, _P, _P, _P, _I, _P, _P, _P]),
+    "ryo_upsweep_multipoles": (_I, [_P, _I, _I, _I, _I, _P, _P, _P]),
+    "ryo_direct_sum_range": (_I, [_P, _I, _SZ, _SZ, _SZ, _P, _I, _P, _P, _P, _P, _P, _P, _P, _P, _P]),
+    "ryo_malloc": (_I, [_P, C.POINTER(_P), _SZ]),
+    "ryo_free": (_I, [_P, _P]),
+    "ryo_memcpy": (_I, [_P, _P, _P, _SZ]),
     "ryo_read_stats": (_I, [_P, _P]),
     "ryo_traversal_retries": (C.c_uint64, [_P]),
     "ryo_ewald_f32": (_I, [_P, _P, _P, C.c_uint32, C.c_uint32, _P, _P, _P, _P, _P, _F, _P, _P, _P, _P, _P, _P]),

@@ -59,6 +59,19 @@ int traverseEntry(Context* ctx, int numGroups, const uint32_t* gs, const uint32_
                   int numShells, const float* boxL, float* p, float* ax, float* ay, float* az, double* totalPot,
                   int stackCap, size_t hCount, bool mdq = false, bool noAlloc = false);
 int initNoAllocScratch(Context* ctx);
+// reference-signature stages (stages.hip, ewald_direct.hip)
+int nodeCounts(Context*, const uint64_t*, int, const uint64_t*, size_t, unsigned, uint32_t*);
+int nodeOps(Context*, const uint64_t*, int, const uint32_t*, unsigned, int32_t*, int*);
+int rebalanceTree(Context*, const uint64_t*, int, int, const int32_t*, uint64_t*, int*);
+int leafCenters(Context*, int, const void*, const void*, const void*, const void*, const int32_t*, int,
+                const uint32_t*, void*);
+int upsweepCenters(Context*, int, int, const int32_t*, const int32_t*, void*);
+int setMac(Context*, int, const uint64_t*, int, void*, float, const void*);
+int leafMultipoles(Context*, int, int, const void*, const void*, const void*, const void*, const int32_t*, int,
+                   const uint32_t*, const void*, void*);
+int upsweepMultipoles(Context*, int, int, int, int, const int32_t*, const void*, void*);
+int directSumRange(Context*, int, size_t, size_t, size_t, const void*, int, const void*, const void*, const void*,
+                   const void*, const void*, void*, void*, void*, void*);
 
 int setError(Context* ctx, int code, const char* fmt, ...)
 {
@@ -385,6 +398,102 @@ int ryo_traverse_f32_stackcap(ryo_ctx* h, int stack_cap, int num_groups, const u
     CTX_OR_FAIL(h);
     return traverseEntry(ctx, num_groups, gs, ge, init_node, xt, yt, zt, mt, ht, xs, ys, zs, ms, hs, child, i2l,
                          layout, centers, mp, G, num_shells, box_l, p, ax, ay, az, total_pot, stack_cap, 0);
+}
+
+/* ---------------------------------------------------------------- reference-signature stages */
+
+int ryo_node_counts_u64(ryo_ctx* h, const uint64_t* tree, int num_nodes, const uint64_t* keys, size_t n,
+                        unsigned max_count, uint32_t* counts)
+{
+    CTX_OR_FAIL(h);
+    return nodeCounts(ctx, tree, num_nodes, keys, n, max_count, counts);
+}
+
+int ryo_node_ops_u64(ryo_ctx* h, const uint64_t* tree, int num_nodes, const uint32_t* counts, unsigned bucket_size,
+                     int32_t* node_ops, int* new_num_nodes)
+{
+    CTX_OR_FAIL(h);
+    return nodeOps(ctx, tree, num_nodes, counts, bucket_size, node_ops, new_num_nodes);
+}
+
+int ryo_rebalance_tree_u64(ryo_ctx* h, const uint64_t* tree, int num_nodes, int new_num_nodes,
+                           const int32_t* node_ops, uint64_t* new_tree, int* converged)
+{
+    CTX_OR_FAIL(h);
+    return rebalanceTree(ctx, tree, num_nodes, new_num_nodes, node_ops, new_tree, converged);
+}
+
+int ryo_leaf_centers(ryo_ctx* h, int types, const void* x, const void* y, const void* z, const void* m,
+                     const int32_t* leaf_to_internal, int num_leaves, const uint32_t* layout, void* centers)
+{
+    CTX_OR_FAIL(h);
+    return leafCenters(ctx, types, x, y, z, m, leaf_to_internal, num_leaves, layout, centers);
+}
+
+int ryo_upsweep_centers(ryo_ctx* h, int types, int num_levels, const int32_t* level_range_host,
+                        const int32_t* child_offsets, void* centers)
+{
+    CTX_OR_FAIL(h);
+    return upsweepCenters(ctx, types, num_levels, level_range_host, child_offsets, centers);
+}
+
+int ryo_set_mac(ryo_ctx* h, int types, const uint64_t* prefixes, int num_nodes, void* centers, float inv_theta,
+                const void* limits)
+{
+    CTX_OR_FAIL(h);
+    return setMac(ctx, types, prefixes, num_nodes, centers, inv_theta, limits);
+}
+
+int ryo_leaf_multipoles(ryo_ctx* h, int types, int mdq, const void* x, const void* y, const void* z, const void* m,
+                        const int32_t* leaf_to_internal, int num_leaves, const uint32_t* layout, const void* centers,
+                        void* multipoles)
+{
+    CTX_OR_FAIL(h);
+    return leafMultipoles(ctx, types, mdq, x, y, z, m, leaf_to_internal, num_leaves, layout, centers, multipoles);
+}
+
+int ryo_upsweep_multipoles(ryo_ctx* h, int types, int mdq, int first_cell, int last_cell,
+                           const int32_t* child_offsets, const void* centers, void* multipoles)
+{
+    CTX_OR_FAIL(h);
+    return upsweepMultipoles(ctx, types, mdq, first_cell, last_cell, child_offsets, centers, multipoles);
+}
+
+int ryo_direct_sum_range(ryo_ctx* h, int types, size_t first, size_t last, size_t num_bodies, const void* box_l,
+                         int num_shells, const void* x, const void* y, const void* z, const void* m, const void* hh,
+                         void* p, void* ax, void* ay, void* az)
+{
+    CTX_OR_FAIL(h);
+    return directSumRange(ctx, types, first, last, num_bodies, box_l, num_shells, x, y, z, m, hh, p, ax, ay, az);
+}
+
+int ryo_malloc(ryo_ctx* h, void** ptr, size_t bytes)
+{
+    CTX_OR_FAIL(h);
+    if (!ptr) return setError(ctx, RYO_ERR_ARG, "malloc: null pointer");
+    *ptr = nullptr;
+    if (bytes == 0) return RYO_OK;
+    RYO_HIP(ctx, hipMalloc(ptr, bytes));
+    return RYO_OK;
+}
+
+int ryo_free(ryo_ctx* h, void* ptr)
+{
+    CTX_OR_FAIL(h);
+    if (!ptr) return RYO_OK;
+    RYO_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    RYO_HIP(ctx, hipFree(ptr));
+    return RYO_OK;
+}
+
+int ryo_memcpy(ryo_ctx* h, void* dst, const void* src, size_t bytes)
+{
+    CTX_OR_FAIL(h);
+    if (bytes == 0) return RYO_OK;
+    if (!dst || !src) return setError(ctx, RYO_ERR_ARG, "memcpy: null pointer");
+    RYO_HIP(ctx, hipMemcpyAsync(dst, src, bytes, hipMemcpyDefault, ctx->stream));
+    RYO_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return RYO_OK;
 }
 
 int ryo_read_stats(ryo_ctx* h, uint64_t* stats)

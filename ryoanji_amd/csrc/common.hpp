@@ -4,6 +4,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstdint>
 #include <cstdio>
 #include <string>
@@ -190,6 +191,7 @@ enum ScratchSlot : int
     kSlotAccuracy,
     kSlotPack,
     kSlotSoft,
+    kSlotStages,
     kNumSlots
 };
 

@@ -118,6 +118,13 @@ __device__ inline double2 phasePow(int j, double2 p1, double2 p2, double2 p3)
 
 __device__ inline double2 cmul(double2 a, double2 b) { return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x); }
 
+// smallest |v + off| over v in [lo, hi] (0 when the interval shifted by off contains 0)
+__device__ inline double boxAxisDist(double lo, double hi, double off)
+{
+    const double a = lo + off, b = hi + off;
+    return (a <= 0.0 && 0.0 <= b) ? 0.0 : fmin(fabs(a), fabs(b));
+}
+
 // the same evaluation with the derived quadrupole (wave-uniform, so its values stay in scalar registers)
 __device__ inline void ewaldEvalQ(double* pa, const double* r, const double* g, const EwaldQ& q)
 {
@@ -248,14 +255,32 @@ __global__ __launch_bounds__(256, 4) void ewaldKernel(const EwaldTable* __restri
     const double smallR2 = tab->smallR * L * L;
     const int ns = tab->numEwaldShells, nr = tab->numReplicaShells;
 
-    uint32_t i = first + blockIdx.x * blockDim.x + threadIdx.x;
-    double myPot = 0;
-    if (i < last)
+    uint32_t i       = first + blockIdx.x * blockDim.x + threadIdx.x;
+    const bool valid = i < last;
+    double myPot     = 0;
+    // the wave's bounding box of r = x - center over its valid lanes (targets are SFC neighbours, so the box is
+    // small): an image row or slab is visited only where some point of the box can be within lCut of it (or it
+    // is an inner replica); the per-lane test below still decides every term, so the evaluated set is exactly
+    // the reference's
+    double r[3] = {0, 0, 0};
+    if (valid) r[0] = x[i] - tab->center[0], r[1] = y[i] - tab->center[1], r[2] = z[i] - tab->center[2];
+    double blo[3], bhi[3];
+    for (int d = 0; d < 3; ++d)
     {
-        double px = x[i], py = y[i], pz = z[i];
-        double r[3] = {px - tab->center[0], py - tab->center[1], pz - tab->center[2]};
+        blo[d] = valid ? r[d] : HUGE_VAL;
+        bhi[d] = valid ? r[d] : -HUGE_VAL;
+        for (int o = 32; o >= 1; o >>= 1)
+        {
+            blo[d] = fmin(blo[d], __shfl_xor(blo[d], o));
+            bhi[d] = fmax(bhi[d], __shfl_xor(bhi[d], o));
+        }
+    }
+    if (valid)
+    {
         double pa[4] = {k1 * M, 0, 0, 0};
-        const bool cull = tab->cullRows, prodExp = tab->productExp;
+        const bool prodExp = tab->productExp;
+        // a conservative cut for the bounds: never below the exact lCut^2 test of any lane
+        const double cutB = lCut2 * (1.0 + 1e-9), invL = 1.0 / L;
         // the exponential factors of the images (productExp): px_ = exp(-2 a^2 L r_x)^ix along the loop
         double e0 = 0, fx = 0, fy = 0, fz = 0, px_ = 0, py0 = 0, pz0 = 0;
         if (prodExp)
@@ -267,18 +292,32 @@ __global__ __launch_bounds__(256, 4) void ewaldKernel(const EwaldTable* __restri
         }
         for (int ix = -ns; ix <= ns; ix++, px_ *= fx)
         {
+            const double dx     = boxAxisDist(blo[0], bhi[0], ix * L);
+            const bool innerX   = -nr <= ix && ix <= nr;
+            if (!innerX && dx * dx > cutB) continue;
             const double ex = prodExp ? e0 * px_ * tab->expLat[ix < 0 ? -ix : ix] : 0.0;
             double py_      = py0;
             for (int iy = -ns; iy <= ns; iy++, py_ *= fy)
             {
-                const int row = (ix + ns) * (2 * ns + 1) + (iy + ns);
-                const int zlo = cull ? tab->rowLo[row] : -ns, zhi = cull ? tab->rowHi[row] : ns;
+                const double dy     = boxAxisDist(blo[1], bhi[1], iy * L);
+                const bool innerRow = innerX && (-nr <= iy && iy <= nr);
+                const double rem    = cutB - dx * dx - dy * dy;
+                int zlo = ns + 1, zhi = -ns - 1;
+                if (rem >= 0)
+                {
+                    const double rad = sqrt(rem);
+                    zlo = int(ceil((-rad - bhi[2]) * invL - 1e-9));
+                    zhi = int(floor((rad - blo[2]) * invL + 1e-9));
+                }
+                if (innerRow) zlo = min(zlo, -nr), zhi = max(zhi, nr);
+                // wave-uniform bounds (every lane computed them from the same box): scalar loop control
+                zlo = __builtin_amdgcn_readfirstlane(max(zlo, -ns));
+                zhi = __builtin_amdgcn_readfirstlane(min(zhi, ns));
                 if (zlo > zhi) continue;
                 const double exy = prodExp ? ex * py_ * tab->expLat[iy < 0 ? -iy : iy] : 0.0;
                 double pz_       = pz0;
                 for (int iz = -ns; iz < zlo; ++iz)
                     pz_ *= fz;
-                const bool innerRow = (-nr <= ix && ix <= nr) && (-nr <= iy && iy <= nr);
                 for (int iz = zlo; iz <= zhi; iz++, pz_ *= fz)
                 {
                     const bool inner = innerRow && (-nr <= iz && iz <= nr);
@@ -454,14 +493,16 @@ int ewald(Context* ctx, const double* Mroot, const double* center, uint32_t firs
 constexpr int kDirTile = 256;
 
 // partial sums of one source chunk for a block of targets: out[chunk][target][4]
-__global__ __launch_bounds__(kDirTile) void directKernel(const float* __restrict__ x, const float* __restrict__ y,
-                                                        const float* __restrict__ z, const float* __restrict__ h,
-                                                        const float* __restrict__ m, size_t n,
-                                                        const int64_t* __restrict__ targets, size_t nt,
+// targets: the listed particles, or first, first + 1, ... when targets is NULL
+template<class Tin>
+__global__ __launch_bounds__(kDirTile) void directKernel(const Tin* __restrict__ x, const Tin* __restrict__ y,
+                                                        const Tin* __restrict__ z, const Tin* __restrict__ h,
+                                                        const Tin* __restrict__ m, size_t n,
+                                                        const int64_t* __restrict__ targets, size_t first, size_t nt,
                                                         double3 boxL, int numShells, size_t chunkLen,
                                                         double* __restrict__ part)
 {
-    __shared__ float sx[kDirTile], sy[kDirTile], sz[kDirTile], sm[kDirTile], shh[kDirTile];
+    __shared__ Tin sx[kDirTile], sy[kDirTile], sz[kDirTile], sm[kDirTile], shh[kDirTile];
     const size_t ti = size_t(blockIdx.x) * kDirTile + threadIdx.x;
     const int chunk = blockIdx.y;
     const size_t s0 = size_t(chunk) * chunkLen;
@@ -470,7 +511,7 @@ __global__ __launch_bounds__(kDirTile) void directKernel(const float* __restrict
     double px = 0, py = 0, pz = 0, th = 0;
     if (ti < nt)
     {
-        t  = targets ? size_t(targets[ti]) : ti;
+        t  = targets ? size_t(targets[ti]) : first + ti;
         px = x[t], py = y[t], pz = z[t], th = h[t];
     }
     double a0 = 0, a1 = 0, a2 = 0, a3 = 0;
@@ -538,11 +579,73 @@ int directSum(Context* ctx, const float* x, const float* y, const float* z, cons
     auto* part      = static_cast<double*>(ctx->getScratch(kSlotMisc, size_t(chunks) * nt * 4 * sizeof(double)));
     if (!part) return RYO_ERR_HIP;
     double3 bl = boxL ? make_double3(boxL[0], boxL[1], boxL[2]) : make_double3(0, 0, 0);
-    directKernel<<<dim3(tb, chunks), kDirTile, 0, ctx->stream>>>(x, y, z, h, m, n, targets, nt, bl, numShells,
-                                                                  chunkLen, part);
+    directKernel<float><<<dim3(tb, chunks), kDirTile, 0, ctx->stream>>>(x, y, z, h, m, n, targets, 0, nt, bl,
+                                                                         numShells, chunkLen, part);
     directReduceKernel<<<iceil(nt, 256), 256, 0, ctx->stream>>>(part, chunks, nt, targets, m, G, out);
     RYO_LAUNCH_CHECK(ctx);
     return RYO_OK;
+}
+
+// the reference's form: p[i] += m_i phi_i, a[i] += acc_i for targets [first, first + nt), G = 1 (direct.cuh:36-85)
+template<class T>
+__global__ void directAddKernel(const double* __restrict__ part, int chunks, size_t nt, size_t first,
+                                const T* __restrict__ m, T* p, T* ax, T* ay, T* az)
+{
+    size_t ti = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (ti >= nt) return;
+    double a[4] = {0, 0, 0, 0};
+    for (int c = 0; c < chunks; ++c)
+        for (int k = 0; k < 4; ++k)
+            a[k] += part[(size_t(c) * nt + ti) * 4 + k];
+    const size_t t = first + ti;
+    p[t] += T(double(m[t]) * a[0]);
+    ax[t] += T(a[1]);
+    ay[t] += T(a[2]);
+    az[t] += T(a[3]);
+}
+
+// ryoanji::directSum<T> (direct.cuh:88-107): targets [first, last) against sources [0, numBodies) and their
+// (2 numShells + 1)^3 images of boxL, P2P<Exact>, accumulated in FP64 and added to the caller's T outputs
+template<class T>
+static int directSumRangeT(Context* ctx, size_t first, size_t last, size_t n, const T* boxL, int numShells,
+                           const T* x, const T* y, const T* z, const T* m, const T* h, T* p, T* ax, T* ay, T* az)
+{
+    if (last <= first || n == 0) return RYO_OK;
+    const size_t nt = last - first;
+    unsigned tb     = iceil(nt, kDirTile);
+    int chunks      = int(std::max<size_t>(1, std::min<size_t>((size_t(ctx->numCUs) * 8 + tb - 1) / tb,
+                                                                (n + 4095) / 4096)));
+    size_t chunkLen = (n + chunks - 1) / chunks;
+    auto* part      = static_cast<double*>(ctx->getScratch(kSlotMisc, size_t(chunks) * nt * 4 * sizeof(double)));
+    if (!part) return RYO_ERR_HIP;
+    double3 bl = make_double3(double(boxL[0]), double(boxL[1]), double(boxL[2]));
+    directKernel<T><<<dim3(tb, chunks), kDirTile, 0, ctx->stream>>>(x, y, z, h, m, n, nullptr, first, nt, bl,
+                                                                     numShells, chunkLen, part);
+    directAddKernel<T><<<iceil(nt, 256), 256, 0, ctx->stream>>>(part, chunks, nt, first, m, p, ax, ay, az);
+    RYO_LAUNCH_CHECK(ctx);
+    return RYO_OK;
+}
+
+int directSumRange(Context* ctx, int types, size_t first, size_t last, size_t n, const void* boxL, int numShells,
+                   const void* x, const void* y, const void* z, const void* m, const void* h, void* p, void* ax,
+                   void* ay, void* az)
+{
+    if (last > first && (!boxL || !x || !y || !z || !m || !h || !p || !ax || !ay || !az || last > n))
+        return setError(ctx, RYO_ERR_ARG, "direct sum: bad arguments");
+    if (numShells < 0) return setError(ctx, RYO_ERR_ARG, "direct sum: negative shells");
+    if (types == 0)
+        return directSumRangeT<float>(ctx, first, last, n, static_cast<const float*>(boxL), numShells,
+                                      static_cast<const float*>(x), static_cast<const float*>(y),
+                                      static_cast<const float*>(z), static_cast<const float*>(m),
+                                      static_cast<const float*>(h), static_cast<float*>(p), static_cast<float*>(ax),
+                                      static_cast<float*>(ay), static_cast<float*>(az));
+    if (types == 1)
+        return directSumRangeT<double>(ctx, first, last, n, static_cast<const double*>(boxL), numShells,
+                                       static_cast<const double*>(x), static_cast<const double*>(y),
+                                       static_cast<const double*>(z), static_cast<const double*>(m),
+                                       static_cast<const double*>(h), static_cast<double*>(p),
+                                       static_cast<double*>(ax), static_cast<double*>(ay), static_cast<double*>(az));
+    return setError(ctx, RYO_ERR_ARG, "direct sum: types %d (0 float, 1 double)", types);
 }
 
 /* ============================================================================ Ewald pair sum */

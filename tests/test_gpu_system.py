@@ -125,6 +125,21 @@ def test_ewald_matches_oracle(ctx, settings, lo, hi):
     assert abs(gtot - otot) <= 1e-9 * abs(otot)
 
 
+def test_ewald_unsorted_particles(ctx):
+    """particles in random (unsorted) order: each wavefront's targets span the whole box, so its image bounds
+    admit every image that any point of the box can reach; results per particle as for sorted input"""
+    s = W.uniform_cube(20000, seed=16, lo=0.0, hi=1.0, periodic=True)
+    t = oracle_tree(s)
+    c64, q64 = O.upsweep(t["x"], t["y"], t["z"], t["m"], t["leaves"], t["layout"], s.limits.astype(np.float64), 2.0,
+                         precision="f64")
+    u = dict(x=s.x, y=s.y, z=s.z, m=s.m)  # the generator's order, not SFC order
+    otot, oug, oax, oay, oaz = O.ewald(q64[0], c64[0, :3], s.x, s.y, s.z, s.m, 1.0)
+    gtot, gug, gax, gay, gaz = _ewald_gpu(ctx, q64[0], c64[0, :3], u, s.limits, O.EWALD_DEFAULT)
+    e = rel_err(gax, gay, gaz, oax, oay, oaz)
+    assert np.quantile(e, 0.99) < 1e-6
+    assert abs(gtot - otot) <= 1e-9 * abs(otot)
+
+
 def test_periodic_traversal_and_system(ctx):
     """periodic images (1 shell) + Ewald. In a periodic uniform box the net force is a small residual of large
     image contributions, so errors are measured relative to the RMS acceleration of the system."""
