@@ -230,6 +230,9 @@ void ewaldInitTable(EwaldTable& t, const double* Mroot, const double* center, in
         }
 }
 
+// RowTable: the per-wave row bounds fit the LDS table (at most kMaxCullShells image shells); ProdExp: the image
+// Gaussians come from products of per-axis factors (EwaldTable::productExp)
+template<bool RowTable, bool ProdExp>
 __global__ __launch_bounds__(256, 4) void ewaldKernel(const EwaldTable* __restrict__ tab, uint32_t first, uint32_t last,
                                                   const float* __restrict__ x, const float* __restrict__ y,
                                                   const float* __restrict__ z, const float* __restrict__ m, float G,
@@ -240,6 +243,7 @@ __global__ __launch_bounds__(256, 4) void ewaldKernel(const EwaldTable* __restri
     // coefficients live in scalar registers
     __shared__ double sPot[256];
     __shared__ double2 sErfcx[kErfcxIntervals * kErfcxPairs];
+    __shared__ short2 sRows[4][kMaxRows]; // per wave: the iz range {lo, hi} of each image row (ix, iy)
     for (int k = threadIdx.x; k < kErfcxIntervals * kErfcxPairs; k += blockDim.x)
         sErfcx[k] = make_double2(kErfcx[k / kErfcxPairs][2 * (k % kErfcxPairs)],
                                  kErfcx[k / kErfcxPairs][2 * (k % kErfcxPairs) + 1]);
@@ -275,12 +279,35 @@ __global__ __launch_bounds__(256, 4) void ewaldKernel(const EwaldTable* __restri
             bhi[d] = fmax(bhi[d], __shfl_xor(bhi[d], o));
         }
     }
+    // the iz range of every image row (ix, iy) for this wave's box, one row per lane (rows of a wave are
+    // wave-uniform, so the loop below reads them as scalars); a conservative cut never below any lane's exact
+    // lCut^2 test. Waves without valid lanes get empty rows (their box is empty).
+    const int w          = 2 * ns + 1;
+    const int numRows    = w * w;
+    constexpr bool rowTable = RowTable;
+    const double cutB    = lCut2 * (1.0 + 1e-9), invL = 1.0 / L;
+    const int wave       = threadIdx.x >> 6;
+    if (rowTable)
+        for (int row = threadIdx.x & 63; row < numRows; row += 64)
+        {
+            const int ix = row / w - ns, iy = row % w - ns;
+            const double dx = boxAxisDist(blo[0], bhi[0], ix * L), dy = boxAxisDist(blo[1], bhi[1], iy * L);
+            const double rem = cutB - dx * dx - dy * dy;
+            int zlo = ns + 1, zhi = -ns - 1;
+            if (rem >= 0)
+            {
+                const double rad = sqrt(rem);
+                zlo = int(ceil((-rad - bhi[2]) * invL - 1e-9));
+                zhi = int(floor((rad - blo[2]) * invL + 1e-9));
+            }
+            if (-nr <= ix && ix <= nr && -nr <= iy && iy <= nr) zlo = min(zlo, -nr), zhi = max(zhi, nr);
+            sRows[wave][row] = make_short2(short(max(zlo, -ns)), short(min(zhi, ns)));
+        }
+    __syncthreads();
     if (valid)
     {
         double pa[4] = {k1 * M, 0, 0, 0};
-        const bool prodExp = tab->productExp;
-        // a conservative cut for the bounds: never below the exact lCut^2 test of any lane
-        const double cutB = lCut2 * (1.0 + 1e-9), invL = 1.0 / L;
+        constexpr bool prodExp = ProdExp;
         // the exponential factors of the images (productExp): px_ = exp(-2 a^2 L r_x)^ix along the loop
         double e0 = 0, fx = 0, fy = 0, fz = 0, px_ = 0, py0 = 0, pz0 = 0;
         if (prodExp)
@@ -292,27 +319,41 @@ __global__ __launch_bounds__(256, 4) void ewaldKernel(const EwaldTable* __restri
         }
         for (int ix = -ns; ix <= ns; ix++, px_ *= fx)
         {
-            const double dx     = boxAxisDist(blo[0], bhi[0], ix * L);
-            const bool innerX   = -nr <= ix && ix <= nr;
-            if (!innerX && dx * dx > cutB) continue;
+            const bool innerX = -nr <= ix && ix <= nr;
+            double dx         = 0;
+            if (!rowTable)
+            {
+                dx = boxAxisDist(blo[0], bhi[0], ix * L);
+                if (!innerX && dx * dx > cutB) continue;
+            }
             const double ex = prodExp ? e0 * px_ * tab->expLat[ix < 0 ? -ix : ix] : 0.0;
             double py_      = py0;
             for (int iy = -ns; iy <= ns; iy++, py_ *= fy)
             {
-                const double dy     = boxAxisDist(blo[1], bhi[1], iy * L);
                 const bool innerRow = innerX && (-nr <= iy && iy <= nr);
-                const double rem    = cutB - dx * dx - dy * dy;
-                int zlo = ns + 1, zhi = -ns - 1;
-                if (rem >= 0)
+                int zlo, zhi;
+                if (rowTable)
                 {
-                    const double rad = sqrt(rem);
-                    zlo = int(ceil((-rad - bhi[2]) * invL - 1e-9));
-                    zhi = int(floor((rad - blo[2]) * invL + 1e-9));
+                    const short2 b = sRows[wave][(ix + ns) * w + (iy + ns)];
+                    zlo = __builtin_amdgcn_readfirstlane(int(b.x));
+                    zhi = __builtin_amdgcn_readfirstlane(int(b.y));
                 }
-                if (innerRow) zlo = min(zlo, -nr), zhi = max(zhi, nr);
-                // wave-uniform bounds (every lane computed them from the same box): scalar loop control
-                zlo = __builtin_amdgcn_readfirstlane(max(zlo, -ns));
-                zhi = __builtin_amdgcn_readfirstlane(min(zhi, ns));
+                else
+                {
+                    const double dy  = boxAxisDist(blo[1], bhi[1], iy * L);
+                    const double rem = cutB - dx * dx - dy * dy;
+                    zlo = ns + 1, zhi = -ns - 1;
+                    if (rem >= 0)
+                    {
+                        const double rad = sqrt(rem);
+                        zlo = int(ceil((-rad - bhi[2]) * invL - 1e-9));
+                        zhi = int(floor((rad - blo[2]) * invL + 1e-9));
+                    }
+                    if (innerRow) zlo = min(zlo, -nr), zhi = max(zhi, nr);
+                    // wave-uniform bounds (every lane computed them from the same box): scalar loop control
+                    zlo = __builtin_amdgcn_readfirstlane(max(zlo, -ns));
+                    zhi = __builtin_amdgcn_readfirstlane(min(zhi, ns));
+                }
                 if (zlo > zhi) continue;
                 const double exy = prodExp ? ex * py_ * tab->expLat[iy < 0 ? -iy : iy] : 0.0;
                 double pz_       = pz0;
@@ -471,11 +512,19 @@ int ewald(Context* ctx, const double* Mroot, const double* center, uint32_t firs
     auto* blockPot = reinterpret_cast<double*>(buf + ((sizeof(EwaldTable) + 63) & ~size_t(63)));
     double* dsum   = blockPot + blocks;
     hipStream_t s  = ctx->stream;
-    hipError_t e   = hipMemcpyAsync(dt, t, sizeof(EwaldTable), hipMemcpyHostToDevice, s);
+    const int w     = 2 * t->numEwaldShells + 1;
+    const bool rows = w * w <= kMaxRows, prod = t->productExp != 0;
+    hipError_t e    = hipMemcpyAsync(dt, t, sizeof(EwaldTable), hipMemcpyHostToDevice, s);
     if (e == hipSuccess) e = hipStreamSynchronize(s); // the host table is freed below
     delete t;
     if (e != hipSuccess) return setError(ctx, RYO_ERR_HIP, "ewald: %s", hipGetErrorString(e));
-    ewaldKernel<<<blocks, 256, 0, s>>>(dt, first, last, x, y, z, m, G, ugrav, ax, ay, az, blockPot);
+    if (rows && prod) ewaldKernel<true, true><<<blocks, 256, 0, s>>>(dt, first, last, x, y, z, m, G, ugrav, ax, ay, az,
+                                                                      blockPot);
+    else if (rows) ewaldKernel<true, false><<<blocks, 256, 0, s>>>(dt, first, last, x, y, z, m, G, ugrav, ax, ay, az,
+                                                                    blockPot);
+    else if (prod) ewaldKernel<false, true><<<blocks, 256, 0, s>>>(dt, first, last, x, y, z, m, G, ugrav, ax, ay, az,
+                                                                    blockPot);
+    else ewaldKernel<false, false><<<blocks, 256, 0, s>>>(dt, first, last, x, y, z, m, G, ugrav, ax, ay, az, blockPot);
     sumFixedKernel<<<1, 256, 0, s>>>(blockPot, int(blocks), dsum);
     RYO_LAUNCH_CHECK(ctx);
     if (totalPot)

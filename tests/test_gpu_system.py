@@ -125,6 +125,21 @@ def test_ewald_matches_oracle(ctx, settings, lo, hi):
     assert abs(gtot - otot) <= 1e-9 * abs(otot)
 
 
+def test_ewald_many_image_shells(ctx):
+    """a real-space cut of 8.5 box lengths: 9 image shells, more image rows than the per-wave row table holds, so
+    the bounds are computed in the loop and the Gaussians evaluated per image"""
+    s = W.uniform_cube(2000, seed=17, lo=0.0, hi=1.0, periodic=True)
+    t = oracle_tree(s)
+    c64, q64 = O.upsweep(t["x"], t["y"], t["z"], t["m"], t["leaves"], t["layout"], s.limits.astype(np.float64), 2.0,
+                         precision="f64")
+    settings = (1, 8.5, 2.8, 2.0, 3.0e-3)
+    otot, oug, oax, oay, oaz = O.ewald(q64[0], c64[0, :3], t["x"], t["y"], t["z"], t["m"], 1.0, settings=settings)
+    gtot, gug, gax, gay, gaz = _ewald_gpu(ctx, q64[0], c64[0, :3], t, s.limits, settings)
+    e = rel_err(gax, gay, gaz, oax, oay, oaz)
+    assert np.quantile(e, 0.99) < 1e-6
+    assert abs(gtot - otot) <= 1e-9 * abs(otot)
+
+
 def test_ewald_unsorted_particles(ctx):
     """particles in random (unsorted) order: each wavefront's targets span the whole box, so its image bounds
     admit every image that any point of the box can reach; results per particle as for sorted input"""
