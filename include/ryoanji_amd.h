@@ -512,6 +512,7 @@ typedef struct
     double arrival_ms, admit_ms, finish_ms, total_pot;
     int status, slot;
     double err_median, err_p99, err_max; /* ryo_system_accuracy of the final state, when requested (else 0) */
+    float limits[6];                      /* the box of the final gravity solve (refitted at every step)    */
 } ryo_result;
 
 typedef struct ryo_server ryo_server;

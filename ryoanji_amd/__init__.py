@@ -72,7 +72,8 @@ class Result(C.Structure):
 
     _fields_ = [("arrival_ms", C.c_double), ("admit_ms", C.c_double), ("finish_ms", C.c_double),
                 ("total_pot", C.c_double), ("status", C.c_int), ("slot", C.c_int),
-                ("err_median", C.c_double), ("err_p99", C.c_double), ("err_max", C.c_double)]
+                ("err_median", C.c_double), ("err_p99", C.c_double), ("err_max", C.c_double),
+                ("limits", C.c_float * 6)]
 
 
 # exported symbol -> (restype, argtypes)

@@ -132,6 +132,8 @@ int ryo_server_run(ryo_server* s, int num, const ryo_request* req, ryo_result* r
             if (!rc) rc = ryo_system_leapfrog(slot.sys, &p, r.dt, r.steps, &pot);
             double acc[4] = {0, 0, 0, 0};
             if (!rc && r.accuracy_targets > 0) rc = ryo_system_accuracy(slot.sys, r.accuracy_targets, uint64_t(j), acc);
+            float lim[6] = {0, 0, 0, 0, 0, 0};
+            if (!rc) rc = ryo_system_limits(slot.sys, lim);
             if (!rc && out_state && out_state[j])
             {
                 float* a[8];
@@ -151,6 +153,8 @@ int ryo_server_run(ryo_server* s, int num, const ryo_request* req, ryo_result* r
                 res[j].finish_ms = now();
                 res[j].total_pot = pot;
                 res[j].err_median = acc[0], res[j].err_p99 = acc[1], res[j].err_max = acc[2];
+                for (int q = 0; q < 6; ++q)
+                    res[j].limits[q] = lim[q];
                 res[j].status    = rc;
                 res[j].slot      = k;
                 if (rc && !firstError) firstError = rc;

@@ -65,7 +65,8 @@ class Server:
         if rc:
             raise RuntimeError(f"ryo_server_run failed ({rc})")
         out = [dict(arrival=r.arrival_ms, admit=r.admit_ms, finish=r.finish_ms, pot=r.total_pot, slot=r.slot,
-                    status=r.status, err_median=r.err_median, err_p99=r.err_p99, err_max=r.err_max) for r in res]
+                    status=r.status, err_median=r.err_median, err_p99=r.err_p99, err_max=r.err_max,
+                    limits=np.array(list(r.limits), np.float32)) for r in res]
         if keep_state:
             for i, b in enumerate(bufs):
                 n = systems[i].n

@@ -76,6 +76,53 @@ def test_batch_equals_systems_alone(ctx, group_size):
     b.close()
 
 
+@pytest.mark.parametrize("graph", [False, True])
+def test_batch_matches_oracle(ctx, graph):
+    """every system of a batch (eager and captured-graph solves) against the oracle pipeline on that system alone:
+    sorted keys, permutation, leaves, counts, layout and the linked octree bit for bit; accelerations at p99 <= 1e-5
+    of the oracle traversal with 64-target groups at the system's theta and G; the total potential to 1e-5"""
+    from gpu_util import oracle_tree, rel_err
+
+    import oracle as O
+
+    systems = _systems()
+    thetas = np.array([0.4, 0.5, 0.7, 0.6] * 4, np.float32)[: len(systems)]
+    Gs = np.linspace(0.5, 2.0, len(systems)).astype(np.float32)
+    b = R.Batch(ctx, sum(s.n for s in systems), len(systems))
+    b.load(systems)
+    pots = b.solve(R.default_params(group_size=64), theta=thetas, G=Gs, graph=graph)
+    if graph:  # a replay of the captured graph
+        pots = b.solve(R.default_params(group_size=64), theta=thetas, G=Gs, graph=True)
+    off = b.offsets()
+    (x, y, z, m, h, p, ax, ay, az), order = b.results()
+    ax, ay, az, order = host(ax), host(ay), host(az), host(order)
+    tr = {k: host(v) for k, v in b.tree().items()}
+    for s, sysd in enumerate(systems):
+        t = oracle_tree(sysd, inv_theta=1.0 / float(thetas[s]))
+        p0, p1 = int(off["P"][s]), int(off["P"][s + 1])
+        l0, l1 = int(off["LO"][s]), int(off["LO"][s + 1])
+        n0, n1 = int(off["NO"][s]), int(off["NO"][s + 1])
+        assert np.array_equal(tr["keys"][p0:p1].view(np.uint64), t["sorted_keys"]), s
+        assert np.array_equal(order[p0:p1] - p0, t["perm"]), s
+        assert np.array_equal(tr["leaves"][l0:l1].view(np.uint64), t["leaves"]), s
+        assert np.array_equal(tr["layout"][l0:l1].view(np.uint32), t["layout"]), s
+        assert np.array_equal(tr["counts"][l0:l1 - 1].view(np.uint32), t["counts"]), s
+        nn = n1 - n0
+        assert np.array_equal(tr["prefixes"][n0:n1].view(np.uint64), t["tree"]["prefixes"]), s
+        for k in ("childOffsets", "internalToLeaf", "leafToInternal"):
+            assert np.array_equal(tr[k][n0:n1], t["tree"][k][:nn]), (s, k)
+        assert np.array_equal(tr["levelRange"][23 * s:23 * s + 23], t["tree"]["levelRange"]), s
+        if sysd.n < 2:
+            continue
+        otot, _, oax, oay, oaz, _ = O.compute_gravity(t["leaves"], t["layout"], t["centers"], t["mp"], t["x"], t["y"],
+                                                      t["z"], t["h"], t["m"], sysd.limits, G=float(Gs[s]),
+                                                      group_size=64)
+        e = np.sort(rel_err(ax[p0:p1], ay[p0:p1], az[p0:p1], oax, oay, oaz))
+        assert e[int(0.99 * (e.size - 1))] <= 1e-5, (s, sysd.n, e[int(0.99 * (e.size - 1))])
+        assert abs(pots[s] - otot) <= 1e-5 * abs(otot), (s, pots[s], otot)
+    b.close()
+
+
 def test_batch_reuse_and_capacity(ctx):
     """a batch object serves batches of different composition; loading beyond its capacity is an error"""
     b = R.Batch(ctx, 30000, 8)

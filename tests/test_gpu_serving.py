@@ -32,6 +32,62 @@ def test_same_result_alone_and_in_the_mix(gpu):
         assert abs(a["pot"] - b["pot"]) <= 1e-9 * abs(a["pot"])
 
 
+def _oracle_check(system, result, theta, group_list=None):
+    """the served final state (positions in the final solve's SFC order, accelerations of that solve) against
+    the oracle traversal of the same positions in the same box: the oracle's sort leaves the served order
+    unchanged, and the accelerations agree at p99 <= 1e-5 (over all targets, or the 64-target groups listed)"""
+    import oracle as O
+    from gpu_util import oracle_tree, rel_err
+
+    st = result["state"]
+    order = st[6].view(np.int32)
+    x, y, z = st[0], st[1], st[2]
+    s = W.System(x, y, z, system.m[order], system.h[order], result["limits"])
+    t = oracle_tree(s, inv_theta=1.0 / float(theta))
+    assert np.array_equal(t["perm"], np.arange(s.n)), "served positions are not in the sort order of their box"
+    if group_list is None:
+        _, _, oax, oay, oaz, _ = O.compute_gravity(t["leaves"], t["layout"], t["centers"], t["mp"], t["x"], t["y"],
+                                                   t["z"], t["h"], t["m"], s.limits, group_size=64)
+        idx = np.arange(s.n)
+    else:
+        _, _, oax, oay, oaz, _ = O.compute_gravity(t["leaves"], t["layout"], t["centers"], t["mp"], t["x"], t["y"],
+                                                   t["z"], t["h"], t["m"], s.limits, group_size=64,
+                                                   group_list=group_list, want_pot=False)
+        idx = np.concatenate([np.arange(64 * g, min(64 * g + 64, s.n)) for g in group_list])
+    e = np.sort(rel_err(st[3][idx], st[4][idx], st[5][idx], oax[idx], oay[idx], oaz[idx]))
+    assert e[int(0.99 * (e.size - 1))] <= 1e-5, (s.n, e[int(0.99 * (e.size - 1))])
+
+
+def test_served_results_match_oracle(gpu):
+    """every served request's final accelerations against the oracle on its final positions and box"""
+    systems, params = S.make_stream(8, rate_per_s=0, seed=13, lo_exp=12, hi_exp=16, steps=(1, 3))
+    srv = S.Server(slots=4, max_n=1 << 16)
+    out = srv.run(systems, params, keep_state=True)
+    srv.close()
+    assert len({r["slot"] for r in out}) > 1
+    for i, (sysd, r) in enumerate(zip(systems, out)):
+        assert r["status"] == 0
+        print(f"served request {i}: n={sysd.n} theta={params['theta'][i]} steps={params['steps'][i]}", flush=True)
+        _oracle_check(sysd, r, params["theta"][i])
+
+
+def test_served_large_request_matches_oracle(gpu):
+    """a stream with a request of 2^20 particles (the served stream's size range reaches 2^22) beside small ones;
+    the large request is checked against the oracle on 48 seeded target groups, the small ones in full"""
+    systems, params = S.make_stream(4, rate_per_s=0, seed=14, lo_exp=12, hi_exp=14, steps=(1, 2))
+    systems[0] = W.plummer(1 << 20, seed=1401)
+    params["steps"][0] = 2
+    params["theta"][0] = np.float32(0.5)
+    srv = S.Server(slots=2, max_n=1 << 20)
+    out = srv.run(systems, params, keep_state=True)
+    srv.close()
+    assert all(r["status"] == 0 for r in out)
+    groups = np.sort(np.random.default_rng(5).choice((1 << 20) // 64, 48, replace=False)).astype(np.int32)
+    _oracle_check(systems[0], out[0], params["theta"][0], group_list=groups)
+    for i in range(1, len(systems)):
+        _oracle_check(systems[i], out[i], params["theta"][i])
+
+
 def test_leapfrog_matches_one_solve_per_step(ctx):
     """the resident leapfrog equals kick-drift-kick done by hand with fresh solves"""
     s = W.plummer(20000, seed=5)
