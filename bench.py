@@ -370,6 +370,8 @@ def main():
     ap.add_argument("--workload", default=DEFAULT_WORKLOAD, choices=list(WORKLOADS))
     ap.add_argument("--n", type=int, default=None, help="override the particle count")
     ap.add_argument("--theta", type=float, default=0.5)
+    ap.add_argument("--multipole", default="quadrupole", choices=["quadrupole", "mdq"],
+                    help="CartesianQuadrupole or CartesianMDQpole expansions (MDQpole: open workloads only)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-accuracy", action="store_true")
     ap.add_argument("--no-open", action="store_true", help="skip the open-boundary passes beside a periodic run")
@@ -409,8 +411,14 @@ def main():
     sysm = R.System(ctx, s.n)
     dev = [torch.from_numpy(a).cuda() for a in s.arrays()]
 
+    if args.multipole == "mdq" and spec["periodic"]:
+        print("the Ewald correction is for quadrupole expansions (as the reference's): use an open workload with "
+              "--multipole mdq", file=sys.stderr)
+        sys.exit(2)
+
     def params(theta, periodic):
-        prm = R.default_params(theta=theta, group_size=64, bucket_size=64)
+        prm = R.default_params(theta=theta, group_size=64, bucket_size=64,
+                               multipole=1 if args.multipole == "mdq" else 0)
         if periodic:
             prm.num_shells, prm.ewald = 1, 1
             for k, v in enumerate(EWALD_SETTINGS):
@@ -466,7 +474,8 @@ def main():
         "data": "synthetic",
         "config": {
             "workload": f"{args.workload}: N={s.n} {spec['kind']} particles, theta={args.theta}, cartesian "
-                        f"quadrupole, bucket 64, softening 1e-3, 64-target groups, {bc}",
+                        f"{'MDQpole' if args.multipole == 'mdq' else 'quadrupole'}, bucket 64, softening 1e-3, "
+                        f"64-target groups, {bc}",
             "n": s.n, "theta": args.theta, "bucket": 64, "group_size": 64, "periodic": periodic,
             "traversal_shape": {"stack_per_wave": args.trav_stack or "default",
                                 "blocks_per_cu": args.trav_blocks or "default",

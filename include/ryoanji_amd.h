@@ -409,6 +409,10 @@ typedef struct
                             * with this tolerance factor, the particles' h and extent ratio 8, as
                             * MultipoleHolder::computeSpatialGroups (multipole_holder.cu:36-53, factor 2);
                             * needs group_size 64 (one group-count read-back) */
+    int multipole;         /* resident systems: 0 CartesianQuadrupole, 1 CartesianMDQpole (cartesian_qpole.hpp:237-417:
+                            * 12 values per node, ryo_system_tree's multipoles hold 12 per node); MDQpole systems run
+                            * without the Ewald correction (the reference instantiates computeGravityEwaldGpu for
+                            * CartesianQuadrupole only, ewald.cu:107-110) and without soft_mac */
 } ryo_params;
 
 void ryo_default_params(ryo_params* p);

@@ -97,6 +97,8 @@ def lib(which: str = "orc") -> C.CDLL:
             "upsweep_mdq_f64": (C.c_int, [_P, _P, _P, _P, _P, C.c_int, _P, _P, C.c_float, _P, _P]),
             "compute_gravity_mdq_f32": (C.c_double, [_P, C.c_int, _P, _P, _P, _P, _P, _P, _P, _P, _P, C.c_float,
                                                      C.c_int, C.c_int, C.c_int, _P, _P, _P, _P]),
+            "compute_gravity_mdq_g64_f32": (C.c_double, [_P, C.c_int, _P, _P, _P, _P, _P, _P, _P, _P, _P,
+                                                         C.c_float, C.c_int, _P, _P, _P, _P]),
             "compute_gravity_mdq_f64": (C.c_double, [_P, C.c_int, _P, _P, _P, _P, _P, _P, _P, _P, _P, C.c_float,
                                                      C.c_int, C.c_int, C.c_int, _P, _P, _P, _P]),
             "compute_gravity_mdq_df": (C.c_double, [_P, C.c_int, _P, _P, _P, _P, _P, _P, _P, _P, _P, C.c_float,
@@ -305,14 +307,24 @@ def ref_upsweep_mdq(x, y, z, m, leaves, layout, limits, inv_theta, precision="f3
     return centers, mp
 
 
-def ref_compute_gravity_mdq(tree_leaves, layout, centers, mp, x, y, z, h, m, limits, G=1.0, num_shells=0):
-    """the reference computeGravity (16-target groups) with CartesianMDQpole<float> -> (pot, ugrav, ax, ay, az)"""
+def ref_compute_gravity_mdq(tree_leaves, layout, centers, mp, x, y, z, h, m, limits, G=1.0, num_shells=0,
+                            group_size=16):
+    """the reference computeGravity (16-target groups) with CartesianMDQpole<float> -> (pot, ugrav, ax, ay, az);
+    group_size=64 drives the reference's computeGravityGroup over 64-target groups (n a multiple of 64)"""
     leaves = _c(tree_leaves, np.uint64)
     layout = _c(layout, np.uint32)
     L = leaves.size - 1
     arrs = [_c(a, np.float32) for a in (x, y, z, h, m)]
     n = arrs[0].size
     ug, ax, ay, az = (np.zeros(n, np.float32) for _ in range(4))
+    if group_size == 64:
+        assert n % 64 == 0
+        tot = _f("ref", "compute_gravity_mdq_g64_f32")(_ptr(leaves), L, _ptr(layout), _ptr(_c(centers, np.float32)),
+                                                       _ptr(_c(mp, np.float32)), *(_ptr(a) for a in arrs),
+                                                       _ptr(_c(limits, np.float32)), float(G), num_shells, _ptr(ug),
+                                                       _ptr(ax), _ptr(ay), _ptr(az))
+        return tot, ug, ax, ay, az
+    assert group_size == 16
     tot = _f("ref", "compute_gravity_mdq_f32")(_ptr(leaves), L, _ptr(layout), _ptr(_c(centers, np.float32)),
                                                _ptr(_c(mp, np.float32)), *(_ptr(a) for a in arrs),
                                                _ptr(_c(limits, np.float32)), float(G), num_shells, 0, L, _ptr(ug),

@@ -184,6 +184,54 @@ double ref_compute_gravity_mdq_f32(const uint64_t* leaves, int numLeaves, const 
     return tot;
 }
 
+/* The reference's per-group traversal computeGravityGroup (traversal_cpu.hpp:67-157) with CartesianMDQpole<float>,
+ * driven over groups of 64 consecutive targets (the GPU demo's grouping) with the image loop and output
+ * accumulation of computeGravity (traversal_cpu.hpp:172-230); n must be a multiple of 64 (the reference CPU reads
+ * a partial group's unset targets into its bounding box). Returns 0.5 G sum m phi. */
+double ref_compute_gravity_mdq_g64_f32(const uint64_t* leaves, int numLeaves, const unsigned* layout,
+                                       const float* centers, const float* mpoles, const float* x, const float* y,
+                                       const float* z, const float* h, const float* m, const float* limits, float G,
+                                       int numShells, float* ugrav, float* ax, float* ay, float* az)
+{
+    constexpr unsigned gs = 64;
+    Tree t(leaves, numLeaves);
+    auto& o            = t.o;
+    auto box           = box6(limits);
+    const unsigned n   = layout[numLeaves];
+    const auto* mp     = reinterpret_cast<const ryoanji::CartesianMDQpole<float>*>(mpoles);
+    const auto* cs     = reinterpret_cast<const SourceCenterType<float>*>(centers);
+    double tot         = 0;
+#pragma omp parallel for reduction(+ : tot)
+    for (unsigned i = 0; i < n; i += gs)
+    {
+        util::array<Vec4<float>, gs> targets, acc;
+        for (unsigned k = 0; k < gs; ++k)
+            targets[k] = {x[i + k], y[i + k], z[i + k], h[i + k]}, acc[k] = {0, 0, 0, 0};
+        for (int iz = -numShells; iz <= numShells; ++iz)
+            for (int iy = -numShells; iy <= numShells; ++iy)
+                for (int ix = -numShells; ix <= numShells; ++ix)
+                {
+                    Vec4<float> shift{ix * box.lx(), iy * box.ly(), iz * box.lz(), 0};
+                    auto ts = targets;
+                    for (auto& q : ts)
+                        q -= shift;
+                    ryoanji::computeGravityGroup(ts, o.childOffsets.data(), o.parents.data(), o.internalToLeaf.data(),
+                                                 cs, const_cast<ryoanji::CartesianMDQpole<float>*>(mp), layout, x, y, z,
+                                                 h, m, acc.data());
+                }
+        for (unsigned k = 0; k < gs; ++k)
+        {
+            float u = G * m[i + k] * acc[k][0];
+            tot += u;
+            if (ugrav) ugrav[i + k] += u;
+            ax[i + k] += G * acc[k][1];
+            ay[i + k] += G * acc[k][2];
+            az[i + k] += G * acc[k][3];
+        }
+    }
+    return 0.5 * tot;
+}
+
 /* computeGravity (traversal_cpu.hpp:172-230) over the leaf range [firstLeaf, lastLeaf); the reference groups
  * 16 consecutive targets. Returns the reference's ugravTot increment. */
 double ref_compute_gravity_f32(const uint64_t* leaves, int numLeaves, const unsigned* layout, const float* centers,

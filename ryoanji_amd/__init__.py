@@ -47,6 +47,7 @@ class Params(C.Structure):
         ("init_node", C.c_int),
         ("soft_mac", C.c_int),
         ("group_split_tol", C.c_float),
+        ("multipole", C.c_int),
     ]
 
 
@@ -166,6 +167,8 @@ SIGNATURES = {
 }
 # entry points that are not part of the public header (test hooks)
 EXTRA_SIGNATURES = {
+    "ryo_traverse_mdq_f32_stackcap": (_I, [_P, _I, _I, _P, _P, _I] + [_P] * 10 + [_P, _P, _P, _P, _P, _F, _I, _P, _P,
+                                                                                _P, _P, _P, _P]),
     "ryo_traverse_f32_stackcap": (_I, [_P, _I, _I, _P, _P, _I] + [_P] * 10 + [_P, _P, _P, _P, _P, _F, _I, _P, _P,
                                                                             _P, _P, _P, _P]),
 }
@@ -282,6 +285,7 @@ class System:
 
     def build(self, params: Params):
         self._check(lib().ryo_system_build(self.h, C.byref(params)))
+        self._mdq = params.multipole == 1
 
     def gravity(self, params: Params) -> float:
         pot = _D()
@@ -339,14 +343,15 @@ class System:
 
         info = self.info()
         L, nn = info["numLeaves"], info["numNodes"]
+        w = 12 if getattr(self, "_mdq", False) else 8  # values per node of the last build's multipoles
         p = (_P * 14)()
         self._check(lib().ryo_system_tree(self.h, p))
         spec = [("keys", self.n, torch.int64), ("leaves", L + 1, torch.int64), ("counts", L, torch.int32),
                 ("layout", L + 1, torch.int32), ("prefixes", nn, torch.int64), ("childOffsets", nn, torch.int32),
                 ("parents", max(1, (nn - 1) // 8), torch.int32), ("levelRange", 23, torch.int32),
                 ("internalToLeaf", nn, torch.int32), ("leafToInternal", nn, torch.int32),
-                ("centers", 4 * nn, torch.float32), ("multipoles", 8 * nn, torch.float32),
-                ("centers64", 4 * nn, torch.float64), ("multipoles64", 8 * nn, torch.float64)]
+                ("centers", 4 * nn, torch.float32), ("multipoles", w * nn, torch.float32),
+                ("centers64", 4 * nn, torch.float64), ("multipoles64", w * nn, torch.float64)]
         return {name: self._tensor(p[k], cnt, dt) for k, (name, cnt, dt) in enumerate(spec)}
 
     def close(self):

@@ -543,6 +543,8 @@ int ryo_batch_load(ryo_batch* b, int num_systems, const uint32_t* counts, const 
 int ryo_batch_solve(ryo_batch* b, const ryo_params* prm, const float* theta, const float* Gs, double* pot)
 {
     if (!b || !prm || b->S < 1) return RYO_ERR_ARG;
+    if (prm->multipole != 0)
+        return setError(b->ctx, RYO_ERR_UNSUPPORTED, "batch: CartesianQuadrupole multipoles only");
     Shape sh;
     int rc = uploadTables(b, prm, theta, Gs, sh);
     if (!rc) rc = solveEager(b, sh);
@@ -554,6 +556,7 @@ int ryo_batch_solve_graph(ryo_batch* b, const ryo_params* prm, const float* thet
 {
     if (!b || !prm || b->S < 1) return RYO_ERR_ARG;
     Context* ctx = b->ctx;
+    if (prm->multipole != 0) return setError(ctx, RYO_ERR_UNSUPPORTED, "batch: CartesianQuadrupole multipoles only");
     Shape sh;
     int rc = uploadTables(b, prm, theta, Gs, sh);
     if (rc) return rc;

@@ -496,6 +496,20 @@ int ryo_memcpy(ryo_ctx* h, void* dst, const void* src, size_t bytes)
     return RYO_OK;
 }
 
+/* test hook: the MDQpole traversal with a reduced per-wave stack capacity (forces the split/retry path) */
+int ryo_traverse_mdq_f32_stackcap(ryo_ctx* h, int stack_cap, int num_groups, const uint32_t* gs, const uint32_t* ge,
+                                  int init_node, const float* xt, const float* yt, const float* zt, const float* mt,
+                                  const float* ht, const float* xs, const float* ys, const float* zs,
+                                  const float* ms, const float* hs, const int32_t* child, const int32_t* i2l,
+                                  const uint32_t* layout, const float* centers, const float* mp, float G,
+                                  int num_shells, const float* box_l, float* p, float* ax, float* ay, float* az,
+                                  double* total_pot)
+{
+    CTX_OR_FAIL(h);
+    return traverseEntry(ctx, num_groups, gs, ge, init_node, xt, yt, zt, mt, ht, xs, ys, zs, ms, hs, child, i2l,
+                         layout, centers, mp, G, num_shells, box_l, p, ax, ay, az, total_pot, stack_cap, 0, true);
+}
+
 int ryo_read_stats(ryo_ctx* h, uint64_t* stats)
 {
     CTX_OR_FAIL(h);
@@ -569,6 +583,7 @@ struct ryo_system
     float lastG = 1.0f;       // G, replica shells and Ewald use of the last gravity call
     int lastShells = 0, lastEwald = 0;
     bool integrating = false; // velocities are in use (leapfrog): the build permutes them too
+    bool mdq         = false; // the last build formed CartesianMDQpole multipoles (12 per node)
     hipEvent_t ev[9] = {};
     double phaseMs[8] = {};
 };
@@ -603,7 +618,8 @@ int growTree(ryo_system* s, int L)
         if ((rc = devAlloc(s, &s->prefixes, c)) || (rc = devAlloc(s, &s->child, c + 1)) ||
             (rc = devAlloc(s, &s->parents, c / 8 + 2)) || (rc = devAlloc(s, &s->i2l, c)) ||
             (rc = devAlloc(s, &s->l2i, c)) || (rc = devAlloc(s, &s->centers, 4 * c)) ||
-            (rc = devAlloc(s, &s->mp, 8 * c)) || (rc = devAlloc(s, &s->c64, 4 * c)) || (rc = devAlloc(s, &s->q64, 8 * c)))
+            (rc = devAlloc(s, &s->mp, 12 * c)) || (rc = devAlloc(s, &s->c64, 4 * c)) ||
+            (rc = devAlloc(s, &s->q64, 12 * c)))
             return rc;
         s->nodeCap = c;
     }
@@ -627,6 +643,7 @@ void ryo_default_params(ryo_params* p)
     p->init_node       = 0;
     p->soft_mac        = 0;
     p->group_split_tol = 0.0f;
+    p->multipole       = 0;
 }
 
 int ryo_system_create(ryo_ctx* h, size_t capacity, ryo_system** out)
@@ -783,6 +800,11 @@ int ryo_system_build(ryo_system* s, const ryo_params* prm)
                          nullptr)))
         return rc;
     mark(s, 5);
+    if (prm->multipole != 0 && prm->multipole != 1)
+        return setError(ctx, RYO_ERR_ARG, "system: multipole must be 0 (quadrupole) or 1 (MDQpole)");
+    if (prm->multipole == 1 && prm->soft_mac)
+        return setError(ctx, RYO_ERR_UNSUPPORTED, "system: the softening-safe MAC is for quadrupoles");
+    s->mdq           = prm->multipole == 1;
     double softReach = 0;
     if (prm->soft_mac)
     {
@@ -791,8 +813,12 @@ int ryo_system_build(ryo_system* s, const ryo_params* prm)
         if ((rc = softeningRange(ctx, A[4], n, &hmin, &hmax))) return rc;
         softReach = 2.0 * double(hmax);
     }
-    if ((rc = upsweep(ctx, A[0], A[1], A[2], A[3], s->layout, L, s->prefixes, s->child, s->parents, s->l2i,
-                      s->levelRange, s->limits, 1.0f / prm->theta, s->centers, s->mp, s->c64, s->q64, softReach)))
+    if ((rc = s->mdq ? upsweepMDQ(ctx, A[0], A[1], A[2], A[3], s->layout, L, s->prefixes, s->child, s->parents,
+                                  s->l2i, s->levelRange, s->limits, 1.0f / prm->theta, s->centers, s->mp, s->c64,
+                                  s->q64)
+                     : upsweep(ctx, A[0], A[1], A[2], A[3], s->layout, L, s->prefixes, s->child, s->parents, s->l2i,
+                               s->levelRange, s->limits, 1.0f / prm->theta, s->centers, s->mp, s->c64, s->q64,
+                               softReach)))
         return rc;
     mark(s, 6);
     int gsz = prm->group_size > 0 ? prm->group_size : 64;
@@ -849,9 +875,11 @@ int ryo_system_gravity(ryo_system* s, const ryo_params* prm, double* totalPot)
     float boxL[3] = {s->limits[1] - s->limits[0], s->limits[3] - s->limits[2], s->limits[5] - s->limits[4]};
     int shells    = s->periodic ? prm->num_shells : 0;
     double pot    = 0;
+    if (s->mdq && s->periodic && prm->ewald)
+        return setError(ctx, RYO_ERR_UNSUPPORTED, "system: the Ewald correction is for quadrupole systems");
     rc = traverseEntry(ctx, s->G, s->groups, s->groups + 1, prm->init_node, A[0], A[1], A[2], A[3], A[4], A[0], A[1],
                        A[2], A[3], A[4], s->child, s->i2l, s->layout, s->centers, s->mp, prm->G, shells, boxL, s->p,
-                       s->ax, s->ay, s->az, &pot, 0, s->n);
+                       s->ax, s->ay, s->az, &pot, 0, s->n, s->mdq);
     if (rc) return rc;
     s->phaseMs[6] = ctx->lastTraverseMs;
     s->phaseMs[7] = 0;

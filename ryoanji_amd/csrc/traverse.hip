@@ -565,12 +565,6 @@ __global__ __launch_bounds__(64 * Waves, kWavesPerCU / 4) void traverseKernelT(T
 // waves per block of this context's traversal launches, resident blocks per CU, and the launch itself
 int travWaves(const Context* ctx) { return ctx->travWaves == 2 || ctx->travWaves == 8 ? ctx->travWaves : kTravWaves; }
 
-int travBlocks4(const Context* ctx, size_t groups)
-{
-    const int perCU = ctx->travBlocksPerCU > 0 ? ctx->travBlocksPerCU : kWavesPerCU / 4;
-    return std::max(1, std::min(ctx->numCUs * perCU, int(iceil(groups, 4))));
-}
-
 int travBlocks(const Context* ctx, size_t groups)
 {
     const int W     = travWaves(ctx);
@@ -588,9 +582,11 @@ void launchTraversal(const Context* ctx, size_t groups, int stackCap, hipStream_
     if constexpr (!Batched)
         if (a.mdq)
         {
-            // monopole-dipole-quadrupole multipoles: one block shape (4 waves)
-            const size_t lds4 = size_t(4) * stackCap * sizeof(int);
-            traverseKernelT<false, 4, true><<<travBlocks4(ctx, groups), 256, lds4, s>>>(a, tb);
+            // monopole-dipole-quadrupole multipoles: the same block shapes and register budget (80 VGPRs, no
+            // spills at 6 waves per SIMD)
+            if (W == 2) traverseKernelT<false, 2, true><<<blocks, 128, lds, s>>>(a, tb);
+            else if (W == 8) traverseKernelT<false, 8, true><<<blocks, 512, lds, s>>>(a, tb);
+            else traverseKernelT<false, 4, true><<<blocks, 256, lds, s>>>(a, tb);
             return;
         }
     if (W == 2) traverseKernelT<Batched, 2><<<blocks, 128, lds, s>>>(a, tb);

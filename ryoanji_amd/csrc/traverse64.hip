@@ -423,6 +423,7 @@ int ryo_traverse_f64(ryo_ctx* h, int num_groups, const uint32_t* group_start, co
 {
     if (!h) return RYO_ERR_ARG;
     Context* ctx = &h->c;
+    if (init_node != 0 && init_node != 1) return setError(ctx, RYO_ERR_ARG, "traverse: init_node must be 0 or 1");
     if (num_shells < 0 || (num_shells > 0 && !box_l)) return setError(ctx, RYO_ERR_ARG, "traverse: box lengths");
     return traverse64(ctx,
                       makeArgs(num_groups, group_start, group_end, init_node, xt, yt, zt, mt, ht, xs, ys, zs, ms, hs,
@@ -441,6 +442,7 @@ int ryo_traverse_f64_f32(ryo_ctx* h, int num_groups, const uint32_t* group_start
 {
     if (!h) return RYO_ERR_ARG;
     Context* ctx = &h->c;
+    if (init_node != 0 && init_node != 1) return setError(ctx, RYO_ERR_ARG, "traverse: init_node must be 0 or 1");
     if (num_shells < 0 || (num_shells > 0 && !box_l)) return setError(ctx, RYO_ERR_ARG, "traverse: box lengths");
     return traverse64(ctx,
                       makeArgs(num_groups, group_start, group_end, init_node, xt, yt, zt, mt, ht, xs, ys, zs, ms, hs,
@@ -459,6 +461,7 @@ int ryo_traverse_mdq_f64(ryo_ctx* h, int num_groups, const uint32_t* group_start
 {
     if (!h) return RYO_ERR_ARG;
     Context* ctx = &h->c;
+    if (init_node != 0 && init_node != 1) return setError(ctx, RYO_ERR_ARG, "traverse: init_node must be 0 or 1");
     if (num_shells < 0 || (num_shells > 0 && !box_l)) return setError(ctx, RYO_ERR_ARG, "traverse: box lengths");
     return traverse64(ctx,
                       makeArgs(num_groups, group_start, group_end, init_node, xt, yt, zt, mt, ht, xs, ys, zs, ms, hs,
@@ -477,6 +480,7 @@ int ryo_traverse_mdq_f64_f32(ryo_ctx* h, int num_groups, const uint32_t* group_s
 {
     if (!h) return RYO_ERR_ARG;
     Context* ctx = &h->c;
+    if (init_node != 0 && init_node != 1) return setError(ctx, RYO_ERR_ARG, "traverse: init_node must be 0 or 1");
     if (num_shells < 0 || (num_shells > 0 && !box_l)) return setError(ctx, RYO_ERR_ARG, "traverse: box lengths");
     return traverse64(ctx,
                       makeArgs(num_groups, group_start, group_end, init_node, xt, yt, zt, mt, ht, xs, ys, zs, ms, hs,

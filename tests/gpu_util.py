@@ -133,8 +133,10 @@ def gpu_traverse(ctx, x, y, z, m, h, tree, layout, centers, mp, group_size, G=1.
     common = [R.ptr(groups), R.ptr(groups) + 4, init_node, *(R.ptr(a) for a in arrs), *(R.ptr(a) for a in arrs),
               R.ptr(child), R.ptr(i2l), R.ptr(lay), R.ptr(c), R.ptr(q), float(G), num_shells, bl,
               R.ptr(p) if want_pot else None, R.ptr(ax), R.ptr(ay), R.ptr(az), C.byref(pot)]
-    if mdq:
+    if mdq and stack_cap is None:
         ctx.call("ryo_traverse_mdq_f32", ngroups, *common)
+    elif mdq:
+        ctx.call("ryo_traverse_mdq_f32_stackcap", stack_cap, ngroups, *common)
     elif stack_cap is None:
         ctx.call("ryo_traverse_f32", ngroups, *common)
     else:

@@ -55,4 +55,4 @@ def test_params_mirror_the_header():
     p = R.default_params()
     assert (p.bucket_size, p.theta, p.G, p.group_size, p.num_shells, p.ewald) == (64, 0.5, 1.0, 64, 0, 0)
     assert list(p.ewald_settings) == [1, 2.6, 2.8, 2.0, 3.0e-3]
-    assert (p.init_node, p.soft_mac, p.group_split_tol) == (0, 0, 0.0)
+    assert (p.init_node, p.soft_mac, p.group_split_tol, p.multipole) == (0, 0, 0.0, 0)
