@@ -242,6 +242,16 @@ __global__ __launch_bounds__(256) void tileOffsetKernel(uint32_t* __restrict__ t
     }
 }
 
+// the tile a scatter block works on: blocks are dispatched round-robin over the 8 XCDs (block b on XCD b % 8), and
+// each XCD takes a contiguous range of tiles in order, so the digit runs that consecutive tiles write next to each
+// other (tile t's run of digit d ends where tile t+1's begins) meet in the same L2 and leave it as whole lines
+__device__ inline int xcdTile(int b, int numTiles)
+{
+    const int x = b & 7, k = b >> 3;
+    const int per = numTiles >> 3, extra = numTiles & 7;
+    return x * per + min(x, extra) + k;
+}
+
 // stable scatter of one tile: wave-level match ranking (8 ballots per key), block-level offsets, staging in
 // LDS so that the writes of one digit run are contiguous
 __global__ __launch_bounds__(kSortThreads) void scatterKernel(const uint64_t* __restrict__ keysIn,
@@ -252,7 +262,8 @@ __global__ __launch_bounds__(kSortThreads) void scatterKernel(const uint64_t* __
                                                              const uint32_t* __restrict__ tileOffsets, int numTiles)
 {
     n = deviceCount(dN, n);
-    if (size_t(blockIdx.x) * kSortTile >= n) return; // a tile past the count (the whole block leaves)
+    const int tile = xcdTile(blockIdx.x, numTiles);
+    if (size_t(tile) * kSortTile >= n) return; // a tile past the count (the whole block leaves)
     __shared__ uint64_t sk[kSortTile];
     __shared__ int32_t sv[kSortTile];
     __shared__ uint32_t wcnt[4][kDigits];
@@ -260,7 +271,6 @@ __global__ __launch_bounds__(kSortThreads) void scatterKernel(const uint64_t* __
     __shared__ uint32_t gbase[kDigits];
     __shared__ uint32_t sw[4];
 
-    const int tile = blockIdx.x;
     const size_t base = size_t(tile) * kSortTile;
     const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
     for (int i = threadIdx.x; i < 4 * kDigits; i += kSortThreads)
