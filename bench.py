@@ -245,25 +245,39 @@ def accuracy(ctx, sysm, s, order, ax, ay, az, pot, num_targets=16384, seed=1, bo
 
 def serving_bench(num=32, slots=8, seed=5):
     """Served mixed-size stream (Plummer, N log-uniform in [2^12, 2^22], theta in {0.4,0.5,0.7}, 1-20 leapfrog
-    steps): continuous batching on `slots` slots (state resident on the device) against one-at-a-time plain
-    solves of the same stream (every timestep a full pass from host memory, host kick-drift-kick) and against
-    the resident server restricted to one slot. Throughput is measured with all requests queued at t=0; latency
-    percentiles at Poisson arrival rates around the one-slot server's capacity."""
+    steps): continuous batching on `slots` slots plus the batch lane (requests of at most 2^16 particles packed
+    into one batched solve per step; state resident on the device) against one-at-a-time plain solves of the same
+    stream (every timestep a full pass from host memory, host kick-drift-kick), against the resident server
+    restricted to one slot, and against the same slots without the lane. Throughput is measured with all requests
+    queued at t=0; latency percentiles at Poisson arrival rates around the one-slot server's capacity."""
+    import numpy as np
+
     from ryoanji_amd import serving as S
 
     systems, params = S.make_stream(num, rate_per_s=0, seed=seed)
     max_n = int(max(s.n for s in systems))
     out = {"requests": num, "slots": slots, "n_range": [int(min(s.n for s in systems)), max_n]}
-    plain = S.Server(1, max_n)
-    res1 = plain.run(systems, params)
-    plain.close()
+    one = S.Server(1, max_n, batch_max_n=0)
+    res1 = one.run(systems, params)
+    one.close()
+    slots_only = S.Server(slots, max_n, hbm_budget=64 << 30, batch_max_n=0)
+    resN = slots_only.run(systems, params)
+    slots_only.close()
     mixed = S.Server(slots, max_n, hbm_budget=64 << 30)
     resS = mixed.run(systems, params)
-    s1, sS = S.summarize(systems, params, res1), S.summarize(systems, params, resS)
+    s1, sN, sS = (S.summarize(systems, params, r) for r in (res1, resN, resS))
     sP = S.summarize(systems, params, S.plain_solves(systems, params))
+    lane = [r for r, sd in zip(resS, systems) if r["slot"] == slots]
     out["plain_saturated"] = sP
     out["resident_one_slot_saturated"] = s1
+    out["resident_slots_no_lane_saturated"] = sN
     out["batched_saturated"] = sS
+    out["lane_requests"] = len(lane)
+    small = [i for i, r in enumerate(resS) if r["slot"] == slots]
+    if small:
+        out["lane_request_p50_ms"] = float(np.percentile([resS[i]["finish"] - resS[i]["arrival"] for i in small], 50))
+        out["no_lane_same_requests_p50_ms"] = float(np.percentile([resN[i]["finish"] - resN[i]["arrival"]
+                                                                   for i in small], 50))
     out["throughput_ratio"] = sS["particle_solves_per_s"] / sP["particle_solves_per_s"]
     out["throughput_ratio_vs_resident_one_slot"] = sS["particle_solves_per_s"] / s1["particle_solves_per_s"]
     # precision report of the served results: every request's final accelerations against the FP64 direct sum
@@ -283,7 +297,7 @@ def serving_bench(num=32, slots=8, seed=5):
         r = mixed.run(systems, params)
         rates.append(dict(rate_per_s=rate, **{k: v for k, v in S.summarize(systems, params, r).items()
                                               if k in ("p50_ms", "p99_ms", "requests_per_s")}))
-        r1 = S.Server(1, max_n)
+        r1 = S.Server(1, max_n, batch_max_n=0)
         rr = r1.run(systems, params)
         r1.close()
         rates[-1]["one_slot_p50_ms"] = S.summarize(systems, params, rr)["p50_ms"]

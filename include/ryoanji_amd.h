@@ -521,9 +521,17 @@ typedef struct
 
 typedef struct ryo_server ryo_server;
 
-/* `slots` concurrent workers (each: a HIP stream, a context, a resident system of capacity max_n particles);
- * requests are admitted while their summed HBM footprint stays within hbm_budget bytes (0: unlimited). */
+/* `slots` concurrent workers (each: a HIP stream, a context, a resident system of capacity max_n particles) and
+ * a batch lane for small requests (below); requests are admitted while their summed HBM footprint stays within
+ * hbm_budget bytes (0: unlimited). */
 int ryo_server_create(int slots, size_t hbm_budget, size_t max_n, ryo_server** server);
+/* The batch lane: open-boundary requests of at most max_n particles (default 2^16; 0 turns the lane off) with the
+ * default bucket and group size are packed one after the other into one ryo_batch (capacity particles and
+ * max_systems systems in all, defaults 2^22 and 1024) and advanced in lockstep, one batched solve per leapfrog step
+ * of every member; requests join at the next step and leave when their steps are done. A member's trajectory is the
+ * one it has on a slot (same kick-drift arithmetic, box refit and solve). Results of lane members report slot =
+ * `slots`. Set before ryo_server_run. */
+int ryo_server_set_batching(ryo_server* server, size_t max_n, size_t capacity, int max_systems);
 int ryo_server_destroy(ryo_server* server);
 /* estimated HBM bytes of a resident system of n particles (the admission weight) */
 size_t ryo_server_footprint(size_t n);

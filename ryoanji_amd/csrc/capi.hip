@@ -912,21 +912,27 @@ int ryo_system_gravity(ryo_system* s, const ryo_params* prm, double* totalPot)
     return RYO_OK;
 }
 
-int ryo_system_accuracy(ryo_system* s, int num_targets, uint64_t seed, double* out)
+} // extern "C"
+
+namespace ryo
 {
-    if (!s || !out || num_targets < 1) return RYO_ERR_ARG;
-    Context* ctx = s->ctx;
-    if (!s->haveAcc) return setError(ctx, RYO_ERR_ARG, "accuracy: no gravity results to check");
-    const size_t n  = s->n;
+/* Accuracy statistics of solved particles (device arrays x, y, z, m, h and their results p, ax, ay, az) against the
+ * FP64 direct sum (shells > 0: over the same replica images of boxL) or, with ewaldL > 0, the FP64 Ewald pair sum of
+ * a cubic box of that side, on num_targets splitmix64 draws of the seed: out = {median, 99th percentile, maximum
+ * of |a - a_ref| / |a_ref|, relative error of the sampled potential sum}. Synchronizes the stream. */
+int accuracyStats(Context* ctx, const float* x, const float* y, const float* z, const float* m, const float* h,
+                  size_t n, const float* p, const float* ax, const float* ay, const float* az, double G, int shells,
+                  const double* boxL, double ewaldL, int num_targets, uint64_t seed, double* out)
+{
     const size_t nt = std::min(size_t(num_targets), n);
     // targets: splitmix64 draws of the seed, with replacement
     std::vector<int64_t> tg(nt);
     for (size_t k = 0; k < nt; ++k)
     {
-        uint64_t z = seed + (k + 1) * 0x9E3779B97F4A7C15ull;
-        z          = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
-        z          = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
-        tg[k]      = int64_t((z ^ (z >> 31)) % n);
+        uint64_t zz = seed + (k + 1) * 0x9E3779B97F4A7C15ull;
+        zz          = (zz ^ (zz >> 30)) * 0xBF58476D1CE4E5B9ull;
+        zz          = (zz ^ (zz >> 27)) * 0x94D049BB133111EBull;
+        tg[k]       = int64_t((zz ^ (zz >> 31)) % n);
     }
     const size_t refBytes = nt * 4 * sizeof(double), tgBytes = nt * sizeof(int64_t);
     auto* buf = static_cast<char*>(ctx->getScratch(kSlotAccuracy, refBytes + tgBytes + nt * 4 * (sizeof(float) + 4) + 1024));
@@ -939,22 +945,10 @@ int ryo_system_accuracy(ryo_system* s, int num_targets, uint64_t seed, double* o
     hipStream_t st = ctx->stream;
     RYO_HIP(ctx, hipMemcpyAsync(dtg, tg.data(), tgBytes, hipMemcpyHostToDevice, st));
     RYO_HIP(ctx, hipMemcpyAsync(idx, ti.data(), nt * 4, hipMemcpyHostToDevice, st));
-    float** A = s->a[s->cur];
-    double boxL[3] = {double(s->limits[1] - s->limits[0]), double(s->limits[3] - s->limits[2]),
-                      double(s->limits[5] - s->limits[4])};
-    int rc;
-    if (s->lastEwald)
-    {
-        // the full periodic result (images + Ewald correction) against the FP64 Ewald pair sum
-        if (!(boxL[0] == boxL[1] && boxL[1] == boxL[2]))
-            return setError(ctx, RYO_ERR_UNSUPPORTED, "accuracy: the Ewald reference needs a cubic box");
-        rc = ewaldPairSum(ctx, A[0], A[1], A[2], A[4], A[3], n, dtg, nt, double(s->lastG), boxL[0], 0.0, ref);
-    }
-    else
-        rc = directSum(ctx, A[0], A[1], A[2], A[4], A[3], n, dtg, nt, double(s->lastG), s->lastShells ? boxL : nullptr,
-                       s->lastShells, ref);
+    int rc = ewaldL > 0 ? ewaldPairSum(ctx, x, y, z, h, m, n, dtg, nt, G, ewaldL, 0.0, ref)
+                        : directSum(ctx, x, y, z, h, m, n, dtg, nt, G, shells ? boxL : nullptr, shells, ref);
     if (rc) return rc;
-    const float* src[4] = {s->p, s->ax, s->ay, s->az};
+    const float* src[4] = {p, ax, ay, az};
     float* dst[4]       = {got, got + nt, got + 2 * nt, got + 3 * nt};
     if ((rc = gather(ctx, idx, nt, 4, src, dst))) return rc;
     std::vector<double> hr(nt * 4);
@@ -978,6 +972,29 @@ int ryo_system_accuracy(ryo_system* s, int num_targets, uint64_t seed, double* o
     out[2] = e[nt - 1];
     out[3] = std::abs(potGot - potRef) / std::max(std::abs(potRef), 1e-300);
     return RYO_OK;
+}
+} // namespace ryo
+
+extern "C" {
+
+int ryo_system_accuracy(ryo_system* s, int num_targets, uint64_t seed, double* out)
+{
+    if (!s || !out || num_targets < 1) return RYO_ERR_ARG;
+    Context* ctx = s->ctx;
+    if (!s->haveAcc) return setError(ctx, RYO_ERR_ARG, "accuracy: no gravity results to check");
+    float** A = s->a[s->cur];
+    double boxL[3] = {double(s->limits[1] - s->limits[0]), double(s->limits[3] - s->limits[2]),
+                      double(s->limits[5] - s->limits[4])};
+    double ewaldL = 0;
+    if (s->lastEwald)
+    {
+        // the full periodic result (images + Ewald correction) against the FP64 Ewald pair sum
+        if (!(boxL[0] == boxL[1] && boxL[1] == boxL[2]))
+            return setError(ctx, RYO_ERR_UNSUPPORTED, "accuracy: the Ewald reference needs a cubic box");
+        ewaldL = boxL[0];
+    }
+    return accuracyStats(ctx, A[0], A[1], A[2], A[3], A[4], s->n, s->p, s->ax, s->ay, s->az, double(s->lastG),
+                         s->lastShells, boxL, ewaldL, num_targets, seed, out);
 }
 
 int ryo_system_results(ryo_system* s, float** out, int32_t** order)

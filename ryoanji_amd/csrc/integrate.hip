@@ -4,15 +4,15 @@
 namespace ryo
 {
 
-// v += c_kick * a; then, if c_drift != 0, x += c_drift * v; periodic axes wrap into [lo, lo + L)
-__global__ __launch_bounds__(256) void kickDriftKernel(float* __restrict__ x, float* __restrict__ y,
-                                                      float* __restrict__ z, float* __restrict__ vx,
-                                                      float* __restrict__ vy, float* __restrict__ vz,
-                                                      const float* __restrict__ ax, const float* __restrict__ ay,
-                                                      const float* __restrict__ az, size_t n, float cKick,
-                                                      float cDrift, int periodic, float3 lo, float3 L)
+void cubicBox(const float* mn, const float* mx, float* limits);
+
+// particle i: v += c_kick * a; then, if c_drift != 0, x += c_drift * v; periodic axes wrap into [lo, lo + L)
+__device__ inline void kickDriftOne(size_t i, float* __restrict__ x, float* __restrict__ y, float* __restrict__ z,
+                                    float* __restrict__ vx, float* __restrict__ vy, float* __restrict__ vz,
+                                    const float* __restrict__ ax, const float* __restrict__ ay,
+                                    const float* __restrict__ az, float cKick, float cDrift, int periodic, float3 lo,
+                                    float3 L)
 {
-    for (size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += size_t(gridDim.x) * blockDim.x)
     {
         float u = vx[i] + cKick * ax[i], v = vy[i] + cKick * ay[i], w = vz[i] + cKick * az[i];
         vx[i] = u, vy[i] = v, vz[i] = w;
@@ -31,6 +31,67 @@ __global__ __launch_bounds__(256) void kickDriftKernel(float* __restrict__ x, fl
             }
             x[i] = px, y[i] = py, z[i] = pz;
         }
+    }
+}
+
+__global__ __launch_bounds__(256) void kickDriftKernel(float* __restrict__ x, float* __restrict__ y,
+                                                      float* __restrict__ z, float* __restrict__ vx,
+                                                      float* __restrict__ vy, float* __restrict__ vz,
+                                                      const float* __restrict__ ax, const float* __restrict__ ay,
+                                                      const float* __restrict__ az, size_t n, float cKick,
+                                                      float cDrift, int periodic, float3 lo, float3 L)
+{
+    for (size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += size_t(gridDim.x) * blockDim.x)
+        kickDriftOne(i, x, y, z, vx, vy, vz, ax, ay, az, cKick, cDrift, periodic, lo, L);
+}
+
+// the same update for systems packed one after the other (open boundaries): system s owns particles [P[s], P[s+1])
+// and updates with its own coefficients coef[s] = {c_kick, c_drift}
+__global__ __launch_bounds__(256) void packedKickDriftKernel(float* __restrict__ x, float* __restrict__ y,
+                                                            float* __restrict__ z, float* __restrict__ vx,
+                                                            float* __restrict__ vy, float* __restrict__ vz,
+                                                            const float* __restrict__ ax, const float* __restrict__ ay,
+                                                            const float* __restrict__ az, const uint32_t* __restrict__ P,
+                                                            int S, const float2* __restrict__ coef)
+{
+    const size_t n = P[S];
+    for (size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += size_t(gridDim.x) * blockDim.x)
+    {
+        const float2 c = coef[segmentOf(P, S, uint32_t(i))];
+        kickDriftOne(i, x, y, z, vx, vy, vz, ax, ay, az, c.x, c.y, 0, make_float3(0, 0, 0), make_float3(1, 1, 1));
+    }
+}
+
+// per-system min/max of x, y, z of packed systems: block s reduces system s into out[s][6]
+__global__ __launch_bounds__(256) void packedBoxKernel(const float* __restrict__ x, const float* __restrict__ y,
+                                                      const float* __restrict__ z, const uint32_t* __restrict__ P,
+                                                      float* __restrict__ out)
+{
+    const int sys = blockIdx.x;
+    float mn[3] = {INFINITY, INFINITY, INFINITY}, mx[3] = {-INFINITY, -INFINITY, -INFINITY};
+    for (uint32_t i = P[sys] + threadIdx.x; i < P[sys + 1]; i += blockDim.x)
+    {
+        float p[3] = {x[i], y[i], z[i]};
+        for (int d = 0; d < 3; ++d)
+            mn[d] = fminf(mn[d], p[d]), mx[d] = fmaxf(mx[d], p[d]);
+    }
+    __shared__ float s[6][4];
+    for (int d = 0; d < 3; ++d)
+    {
+        mn[d] = waveMin(mn[d]);
+        mx[d] = waveMax(mx[d]);
+    }
+    int w = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0)
+        for (int d = 0; d < 3; ++d)
+            s[d][w] = mn[d], s[3 + d][w] = mx[d];
+    __syncthreads();
+    if (threadIdx.x < 6)
+    {
+        float v = s[threadIdx.x][0];
+        for (int k = 1; k < 4; ++k)
+            v = threadIdx.x < 3 ? fminf(v, s[threadIdx.x][k]) : fmaxf(v, s[threadIdx.x][k]);
+        out[sys * 6 + threadIdx.x] = v;
     }
 }
 
@@ -94,6 +155,13 @@ int fitBox(Context* ctx, const float* x, const float* y, const float* z, size_t 
     for (unsigned b = 0; b < blocks; ++b)
         for (int d = 0; d < 3; ++d)
             mn[d] = std::min(mn[d], h[b * 6 + d]), mx[d] = std::max(mx[d], h[b * 6 + 3 + d]);
+    cubicBox(mn, mx, limits);
+    return RYO_OK;
+}
+
+// the cubic box of an extent, padded by a relative 1e-4 (host float[6]); all three lengths are the same float
+void cubicBox(const float* mn, const float* mx, float* limits)
+{
     float ext = 0, c[3];
     for (int d = 0; d < 3; ++d)
     {
@@ -107,6 +175,25 @@ int fitBox(Context* ctx, const float* x, const float* y, const float* z, size_t 
     float Lx = limits[1] - limits[0];
     for (int d = 1; d < 3; ++d)
         limits[2 * d + 1] = limits[2 * d] + Lx;
+}
+
+// v += coef[s].x * a, then x += coef[s].y * v for the packed systems of the offset table P (device, S + 1 entries)
+int packedKickDrift(Context* ctx, float* x, float* y, float* z, float* vx, float* vy, float* vz, const float* ax,
+                    const float* ay, const float* az, size_t n, const uint32_t* P, int S, const float2* coef)
+{
+    if (n == 0 || S < 1) return RYO_OK;
+    unsigned blocks = std::min(iceil(n, 256), unsigned(ctx->numCUs * 8));
+    packedKickDriftKernel<<<blocks, 256, 0, ctx->stream>>>(x, y, z, vx, vy, vz, ax, ay, az, P, S, coef);
+    RYO_LAUNCH_CHECK(ctx);
+    return RYO_OK;
+}
+
+// per-system extents of packed systems into out (device, 6 S floats)
+int packedBoxes(Context* ctx, const float* x, const float* y, const float* z, const uint32_t* P, int S, float* out)
+{
+    if (S < 1) return RYO_OK;
+    packedBoxKernel<<<S, 256, 0, ctx->stream>>>(x, y, z, P, out);
+    RYO_LAUNCH_CHECK(ctx);
     return RYO_OK;
 }
 

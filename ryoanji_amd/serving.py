@@ -28,11 +28,18 @@ def make_stream(num, rate_per_s, seed=0, lo_exp=12, hi_exp=22, steps=(1, 20), th
 
 
 class Server:
-    def __init__(self, slots, max_n, hbm_budget=0):
+    """slots resident-system workers and a batch lane for requests of at most batch_max_n particles (None: the
+    library's default; 0: no lane, every request on a slot)"""
+
+    def __init__(self, slots, max_n, hbm_budget=0, batch_max_n=None, batch_capacity=1 << 22, batch_systems=1024):
         self.h = _P()
         rc = lib().ryo_server_create(int(slots), int(hbm_budget), int(max_n), C.byref(self.h))
         if rc:
             raise RuntimeError(f"ryo_server_create failed ({rc})")
+        if batch_max_n is not None:
+            rc = lib().ryo_server_set_batching(self.h, int(batch_max_n), int(batch_capacity), int(batch_systems))
+            if rc:
+                raise RuntimeError(f"ryo_server_set_batching failed ({rc})")
         self.slots = slots
 
     def run(self, systems, params, keep_state=False, arrival_scale=1.0, accuracy_targets=0):

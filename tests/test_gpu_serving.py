@@ -14,15 +14,21 @@ pytestmark = pytest.mark.gpu
 
 
 def test_same_result_alone_and_in_the_mix(gpu):
-    systems, params = S.make_stream(10, rate_per_s=0, seed=3, lo_exp=10, hi_exp=15, steps=(1, 4))
-    mixed = S.Server(slots=4, max_n=1 << 15)
+    """the mix: 4 slots and the batch lane (requests of at most 2^14 particles packed together, the others on
+    slots); alone: one slot and no lane, one request at a time"""
+    systems, params = S.make_stream(12, rate_per_s=0, seed=3, lo_exp=10, hi_exp=17, steps=(1, 4))
+    mixed = S.Server(slots=4, max_n=1 << 17, batch_max_n=1 << 14)
     out_mix = mixed.run(systems, params, keep_state=True)
     mixed.close()
-    alone = S.Server(slots=1, max_n=1 << 15)
+    alone = S.Server(slots=1, max_n=1 << 17, batch_max_n=0)
     out_alone = alone.run(systems, params, keep_state=True)
     alone.close()
     assert all(r["status"] == 0 for r in out_mix + out_alone)
-    assert len({r["slot"] for r in out_mix}) > 1  # requests really ran on several slots
+    lane = [r for r, sd in zip(out_mix, systems) if sd.n <= 1 << 14]
+    assert lane and all(r["slot"] == 4 for r in lane)  # the small requests ran in the batch lane
+    assert len({r["slot"] for r in out_mix}) > 2  # and the others on several slots
+    # several lane members were resident at once (packed into one batched solve per step)
+    assert any(a["admit"] < b["finish"] and b["admit"] < a["finish"] for a in lane for b in lane if a is not b)
     for a, b in zip(out_alone, out_mix):
         # identical kernels on identical data: bit-identical states (the stated tolerance is 1e-6 relative)
         sa, sb = a["state"], b["state"]
@@ -61,7 +67,7 @@ def _oracle_check(system, result, theta, group_list=None):
 def test_served_results_match_oracle(gpu):
     """every served request's final accelerations against the oracle on its final positions and box"""
     systems, params = S.make_stream(8, rate_per_s=0, seed=13, lo_exp=12, hi_exp=16, steps=(1, 3))
-    srv = S.Server(slots=4, max_n=1 << 16)
+    srv = S.Server(slots=4, max_n=1 << 16, batch_max_n=1 << 14)  # the lane and the slots both serve
     out = srv.run(systems, params, keep_state=True)
     srv.close()
     assert len({r["slot"] for r in out}) > 1
