@@ -14,10 +14,11 @@
 namespace ryo
 {
 
-constexpr int kMaxH = 343;
 constexpr double kTwoOverSqrtPi = 1.1283791670955126;
 constexpr int kMaxCullShells = 8; // image rows are culled and exponentials factored for up to 8 shells
 constexpr int kMaxRows       = (2 * kMaxCullShells + 1) * (2 * kMaxCullShells + 1);
+constexpr int kMaxHReps      = 3; // ceil(hCut) <= 3, the reference's maxCeilHcut (ewald.hpp:53)
+constexpr int kMaxKPairs     = ((2 * kMaxHReps + 1) * (2 * kMaxHReps + 1) * (2 * kMaxHReps + 1) - 1) / 2;
 
 // the root quadrupole in the form its evaluation uses: M, the traceless components plus trace over 3, and half
 // the trace (ewald.hpp:97-108)
@@ -31,31 +32,31 @@ struct EwaldTable
     double Mroot[8];
     EwaldQ q; // derived from Mroot once, on the host
     double center[3];
-    int numReplicaShells, numEwaldShells, numH, hReps;
+    int numReplicaShells, numEwaldShells, numKPairs, hReps;
     double L, lCut, alphaScale, smallR, hCut2;
     double kPotSign; // -1: the reference's k-space potential (ewald.hpp:320); +1: consistent with the acceleration
-    double h[kMaxH][5]; // {2 pi h / L (3), hfac_cos, hfac_sin}
-    // real-space images: row (ix, iy) evaluates iz in [rowLo, rowHi] (empty when rowLo > rowHi); every image
-    // outside can be neither an inner replica nor within lCut of any particle of the box
-    int cullRows;
-    int8_t rowLo[kMaxRows], rowHi[kMaxRows];
+    // k space, one entry per pair {h, -h} of lattice vectors with 0 < |h|^2 <= hCut^2: the coefficient e(h) of
+    // the reference's table (ewald.hpp:185-191, hfac_cos) is even in h and the sine coefficient drops out of the
+    // pair, so the pair contributes 2 e(h) kPotSign cos(k.r) to the potential and 2 e(h) sin(k.r) k to the
+    // acceleration, k = 2 pi h / L. Entries {2 e kPotSign, 2 e k_x, 2 e k_y, 2 e k_z} in the order the kernel
+    // visits the half space (hx ascending from 0, hy, hz ascending; hx = 0 only for hy > 0, or hy = 0 and hz > 0)
+    double kc[kMaxKPairs][4];
     // exp(-a^2 |r + nL|^2) = exp(-a^2 |r|^2) prod_d exp(-2 a^2 L r_d)^n_d exp(-a^2 L^2 n_d^2): expLat[i] is
-    // exp(-a^2 L^2 i^2); used when the factors stay far from overflow (productExp)
+    // exp(-a^2 L^2 i^2) and expStep[i + ns] = exp(-a^2 L^2 (2 i + 1)), the ratio of consecutive lattice factors;
+    // used when the factors stay far from overflow (productExp)
     int productExp;
     double expLat[kMaxCullShells + 1];
+    double expStep[2 * kMaxCullShells + 1];
     // k space: row (hx, hy) holds the vectors hz in [-kRowHz, kRowHz] (none when -1), the origin excluded
-    int8_t kRowHz[49];
+    int8_t kRowHz[(2 * kMaxHReps + 1) * (2 * kMaxHReps + 1)];
 };
 
-// 1/sqrt(x) for positive normal x: v_rsq_f64 (relative error below 2^-22) and two Newton steps, within about
-// 1 ulp
+// 1/sqrt(x) for positive normal x: v_rsq_f64 (relative error below 2^-22) and one Newton step, within about
+// 1e-13 relative (the Ewald terms are summed in FP64 and stored as float)
 __device__ inline double rsqrt64(double x)
 {
-    double y       = __builtin_amdgcn_rsq(x);
-    const double h = 0.5 * x;
-    double e       = fma(-h * y, y, 0.5);
-    y              = fma(y, e, y);
-    e              = fma(-h * y, y, 0.5);
+    const double y = __builtin_amdgcn_rsq(x);
+    const double e = fma(-0.5 * x * y, y, 0.5);
     return fma(y, e, y);
 }
 
@@ -75,23 +76,24 @@ __host__ __device__ inline void ewaldEval(double* pa, const double* r, const dou
         pa[1 + d] += g[2] * Qr[d] - r[d] * f;
 }
 
-// erfcx(x) = exp(x^2) erfc(x) for x >= 0 (invX = 1/x): on [0, 8] the degree-11 piece of x's interval of width
-// 1/4 (erfcx_table.hpp, within 2.5e-16 of erfcx), beyond 8 the first three terms of the asymptotic series (there
-// erfc(x) < 1.2e-29, so its relative accuracy of 1e-5 is immaterial). Each lane reads its interval's
-// coefficients from the copy of the table in LDS (c2: pairs of coefficients, constant term first); a wave's
-// targets are SFC neighbours that see an image at nearly the same distance, so the reads are mostly broadcasts.
+// erfcx(x) = exp(x^2) erfc(x) for x >= 0, given y = 4 x: on [0, 8] the degree-11 piece of x's interval of width
+// 1/4 (erfcx_table.hpp, within 2.5e-16 of erfcx), in the local variable t = 2 fract(y) - 1 in [-1, 1). Each lane
+// reads its interval's coefficients from the copy of the table in LDS (c2: pairs of coefficients, constant term
+// first); a wave's targets are SFC neighbours that see an image at nearly the same distance, so the reads are
+// mostly broadcasts. Beyond 8 the last piece is evaluated at a point of [7.75, 8), so erfc(x) = exp(-x^2)
+// erfcx(x) comes out below 1.2e-29 and within the factor x / 8 of its value, far below the FP64 rounding of the
+// replica terms -erf(aR) / R (of order 1 / R) that every lane sums with it.
 constexpr int kErfcxPairs = kErfcxTerms / 2;
 static_assert(kErfcxTerms % 2 == 0, "coefficients are read in pairs");
 
-__device__ inline double erfcxLds(double x, double invX, const double2* __restrict__ c2)
+__device__ inline double erfcxLds(double y, const double2* __restrict__ c2)
 {
-    const bool far   = !(x < 8.0);
-    const int j      = far ? kErfcxIntervals - 1 : int(x * 4.0);
+    const int j      = min(int(y), kErfcxIntervals - 1);
     const double2* c = c2 + j * kErfcxPairs;
     double2 p[kErfcxPairs];
     for (int k = 0; k < kErfcxPairs; ++k)
         p[k] = c[k];
-    const double t = (x - (j + 0.5) * 0.25) * 8.0;
+    const double t = fma(2.0, __builtin_amdgcn_fract(y), -1.0);
     double v       = p[kErfcxPairs - 1].y;
     v              = fma(v, t, p[kErfcxPairs - 1].x);
     for (int k = kErfcxPairs - 2; k >= 0; --k)
@@ -99,24 +101,11 @@ __device__ inline double erfcxLds(double x, double invX, const double2* __restri
         v = fma(v, t, p[k].y);
         v = fma(v, t, p[k].x);
     }
-    if (far)
-    {
-        const double u = invX * invX;
-        v = (0.5 * kTwoOverSqrtPi * invX) * (1.0 + u * (-0.5 + 0.75 * u));
-    }
-    return v;
-}
-
-// exp(i j theta) for j in [-3, 3] from the powers p1 = exp(i theta), p2, p3 (j is wave-uniform)
-__device__ inline double2 phasePow(int j, double2 p1, double2 p2, double2 p3)
-{
-    const int a = j < 0 ? -j : j;
-    double2 v   = a == 0 ? make_double2(1.0, 0.0) : (a == 1 ? p1 : (a == 2 ? p2 : p3));
-    if (j < 0) v.y = -v.y;
     return v;
 }
 
 __device__ inline double2 cmul(double2 a, double2 b) { return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x); }
+__device__ inline double2 conjIf(double2 a, bool c) { return c ? make_double2(a.x, -a.y) : a; }
 
 // smallest |v + off| over v in [lo, hi] (0 when the interval shifted by off contains 0)
 __device__ inline double boxAxisDist(double lo, double hi, double off)
@@ -125,59 +114,40 @@ __device__ inline double boxAxisDist(double lo, double hi, double off)
     return (a <= 0.0 && 0.0 <= b) ? 0.0 : fmin(fabs(a), fabs(b));
 }
 
-// the same evaluation with the derived quadrupole (wave-uniform, so its values stay in scalar registers)
-__device__ inline void ewaldEvalQ(double* pa, const double* r, const double* g, const EwaldQ& q)
+// the lattice factors of the image Gaussians; the factorization is used when every factor stays far from overflow
+// over the image shells
+void ewaldInitFactors(EwaldTable& t)
 {
-    double Qr[3] = {r[0] * q.qxx + r[1] * q.qxy + r[2] * q.qxz, r[0] * q.qxy + r[1] * q.qyy + r[2] * q.qyz,
-                    r[0] * q.qxz + r[1] * q.qyz + r[2] * q.qzz};
-    double rQr = 0.5 * (r[0] * Qr[0] + r[1] * Qr[1] + r[2] * Qr[2]);
-    pa[0] += -g[0] * q.M + g[1] * q.Qtr - g[2] * rQr;
-    double f = g[1] * q.M - g[2] * q.Qtr + g[3] * rQr;
-    for (int d = 0; d < 3; ++d)
-        pa[1 + d] += g[2] * Qr[d] - r[d] * f;
-}
-
-// image rows and the exponential factorization for particles in the box `limits` (with a margin of 5 % of L)
-void ewaldInitCulling(EwaldTable& t, const float* limits)
-{
-    const int ns = t.numEwaldShells, nr = t.numReplicaShells, w = 2 * ns + 1;
-    const double L = t.L;
-    t.cullRows = ns <= kMaxCullShells;
+    const int ns    = t.numEwaldShells;
     const double a2 = t.alphaScale * t.alphaScale;
-    t.productExp = t.cullRows && 2.0 * a2 * ns * 1.1 <= 200.0 && a2 * ns * ns <= 600.0;
+    t.productExp    = ns <= kMaxCullShells && 2.0 * a2 * ns * 1.1 <= 200.0 && a2 * ns * ns <= 600.0;
     for (int i = 0; i <= kMaxCullShells; ++i)
         t.expLat[i] = i <= ns ? std::exp(-a2 * i * i) : 0.0;
-    if (!t.cullRows) return;
-    double lo[3], hi[3];
-    for (int d = 0; d < 3; ++d)
-    {
-        lo[d] = double(limits[2 * d]) - t.center[d] - 0.05 * L;
-        hi[d] = double(limits[2 * d + 1]) - t.center[d] + 0.05 * L;
-    }
-    // smallest |r_d + i L| over r_d in [lo, hi]
-    auto dmin = [&](int d, int i) {
-        const double a = lo[d] + i * L, b = hi[d] + i * L;
-        return (a <= 0 && 0 <= b) ? 0.0 : std::min(std::fabs(a), std::fabs(b));
-    };
-    const double cut2 = t.lCut * t.lCut * L * L * (1 + 1e-9);
-    for (int ix = -ns; ix <= ns; ++ix)
-        for (int iy = -ns; iy <= ns; ++iy)
+    for (int i = -kMaxCullShells; i <= kMaxCullShells; ++i)
+        t.expStep[i + kMaxCullShells] = (t.productExp && -ns <= i && i <= ns) ? std::exp(-a2 * (2 * i + 1)) : 0.0;
+}
+
+// the visiting order of the k-space pairs: row (hx, hy) of the half space with its largest |hz| (or -1)
+template<class F>
+void forEachKPair(const EwaldTable& t, F&& f)
+{
+    const int hr = t.hReps;
+    for (int hx = 0; hx <= hr; ++hx)
+        for (int hy = -hr; hy <= hr; ++hy)
         {
-            const int row = (ix + ns) * w + (iy + ns);
-            const double dx = dmin(0, ix), dy = dmin(1, iy);
-            int l = ns + 1, h = -ns - 1;
-            for (int iz = -ns; iz <= ns; ++iz)
+            if (hx == 0 && hy < 0) continue;
+            const int hzm = t.kRowHz[(hx + hr) * (2 * hr + 1) + (hy + hr)];
+            for (int hz = -hr; hz <= hr; ++hz)
             {
-                const bool inner = std::abs(ix) <= nr && std::abs(iy) <= nr && std::abs(iz) <= nr;
-                const double dz  = dmin(2, iz);
-                if (inner || dx * dx + dy * dy + dz * dz <= cut2) l = std::min(l, iz), h = std::max(h, iz);
+                if (hx == 0 && hy == 0 && hz <= 0) continue;
+                if (std::abs(hz) > hzm) continue;
+                f(hx, hy, hz);
             }
-            t.rowLo[row] = int8_t(l), t.rowHi[row] = int8_t(h);
         }
 }
 
 void ewaldInitTable(EwaldTable& t, const double* Mroot, const double* center, int numReplicaShells, double L,
-                    double lCut, double hCut, double alphaScale, double smallR)
+                    double lCut, double hCut, double alphaScale, double smallR, double kPotSign)
 {
     t.q = EwaldQ{Mroot[0],       (Mroot[1] + Mroot[7]) / 3.0, (Mroot[4] + Mroot[7]) / 3.0,
                  (Mroot[6] + Mroot[7]) / 3.0, Mroot[2] / 3.0,  Mroot[3] / 3.0,
@@ -187,39 +157,16 @@ void ewaldInitTable(EwaldTable& t, const double* Mroot, const double* center, in
         t.Mroot[k] = Mroot[k];
     for (int k = 0; k < 3; ++k)
         t.center[k] = center[k];
+    t.kPotSign         = kPotSign;
     t.numReplicaShells = numReplicaShells;
     t.numEwaldShells   = std::max(int(std::ceil(lCut)), numReplicaShells);
     t.L = L, t.lCut = lCut, t.alphaScale = alphaScale, t.smallR = smallR;
-    t.numH = 0;
+    t.numKPairs = 0;
+    t.hReps     = 0;
     if (t.numEwaldShells == 0) return;
-    const int hReps    = int(std::ceil(hCut));
-    t.hReps            = hReps;
-    t.hCut2            = hCut * hCut;
-    const double alpha = alphaScale / L;
-    const double k4    = M_PI * M_PI / (alpha * alpha * L * L);
-    for (int hx = -hReps; hx <= hReps; hx++)
-        for (int hy = -hReps; hy <= hReps; hy++)
-            for (int hz = -hReps; hz <= hReps; hz++)
-            {
-                double hr[3] = {double(hx), double(hy), double(hz)};
-                double h2    = hr[0] * hr[0] + hr[1] * hr[1] + hr[2] * hr[2];
-                if (h2 == 0 || h2 > hCut * hCut || t.numH >= kMaxH) continue;
-                double g0 = std::exp(-k4 * h2) / (M_PI * h2 * L);
-                double g1 = 2 * M_PI / L * g0;
-                double g2 = -2 * M_PI / L * g1;
-                double g3 = 2 * M_PI / L * g2;
-                double g4 = -2 * M_PI / L * g3;
-                double g5 = 2 * M_PI / L * g4;
-                double gc[6] = {g0, 0, g2, 0, g4, 0}, gs[6] = {0, g1, 0, g3, 0, g5};
-                double a[4] = {0, 0, 0, 0}, b[4] = {0, 0, 0, 0};
-                ewaldEval(a, hr, gc, t.Mroot);
-                ewaldEval(b, hr, gs, t.Mroot);
-                double* e = t.h[t.numH++];
-                for (int d = 0; d < 3; ++d)
-                    e[d] = 2 * M_PI / L * hr[d];
-                e[3] = a[0];
-                e[4] = b[0];
-            }
+    const int hReps = int(std::ceil(hCut));
+    t.hReps         = hReps;
+    t.hCut2         = hCut * hCut;
     for (int hx = -hReps; hx <= hReps; hx++)
         for (int hy = -hReps; hy <= hReps; hy++)
         {
@@ -227,6 +174,70 @@ void ewaldInitTable(EwaldTable& t, const double* Mroot, const double* center, in
             for (int hz = 0; hz <= hReps; ++hz)
                 if (double(hx * hx + hy * hy + hz * hz) <= hCut * hCut) top = hz;
             t.kRowHz[(hx + hReps) * (2 * hReps + 1) + (hy + hReps)] = int8_t(top);
+        }
+    const double alpha = alphaScale / L;
+    const double k4    = M_PI * M_PI / (alpha * alpha * L * L);
+    forEachKPair(t, [&](int hx, int hy, int hz) {
+        double hr[3] = {double(hx), double(hy), double(hz)};
+        double h2    = hr[0] * hr[0] + hr[1] * hr[1] + hr[2] * hr[2];
+        double g0    = std::exp(-k4 * h2) / (M_PI * h2 * L);
+        double g1    = 2 * M_PI / L * g0;
+        double g2    = -2 * M_PI / L * g1;
+        double g4    = 2 * M_PI / L * (-2 * M_PI / L * g2);
+        double gc[6] = {g0, 0, g2, 0, g4, 0};
+        double a[4]  = {0, 0, 0, 0};
+        ewaldEval(a, hr, gc, t.Mroot);
+        double* e = t.kc[t.numKPairs++];
+        e[0]      = 2 * a[0] * kPotSign;
+        for (int d = 0; d < 3; ++d)
+            e[1 + d] = 2 * a[0] * (2 * M_PI / L * hr[d]);
+    });
+}
+
+// k space over the pairs {h, -h} of the table (|h_d| <= HR = hReps): exp(i k.r) as a product of the axis phases
+// exp(i 2 pi h_d r_d / L), whose powers 0..HR every lane holds in registers; the loops unroll completely, so
+// each phase is a fixed register and the table's row bounds are wave-uniform branches
+template<int HR>
+__device__ inline void ewaldKSpace(const EwaldTable* __restrict__ tab, const double* r, double* kp)
+{
+    double2 px[HR + 1], py[HR + 1], pz[HR + 1];
+    {
+        const double w = 2.0 * M_PI / tab->L;
+        double s, c;
+        px[0] = py[0] = pz[0] = make_double2(1.0, 0.0);
+        sincos(w * r[0], &s, &c);
+        px[1] = make_double2(c, s);
+        sincos(w * r[1], &s, &c);
+        py[1] = make_double2(c, s);
+        sincos(w * r[2], &s, &c);
+        pz[1] = make_double2(c, s);
+#pragma unroll
+        for (int j = 2; j <= HR; ++j)
+            px[j] = cmul(px[j - 1], px[1]), py[j] = cmul(py[j - 1], py[1]), pz[j] = cmul(pz[j - 1], pz[1]);
+    }
+    int k = 0;
+#pragma unroll
+    for (int hx = 0; hx <= HR; ++hx)
+#pragma unroll
+        for (int hy = -HR; hy <= HR; ++hy)
+        {
+            if (hx == 0 && hy < 0) continue;
+            const int hzm = tab->kRowHz[(hx + HR) * (2 * HR + 1) + (hy + HR)];
+            if (hzm < 0) continue;
+            const double2 qy  = conjIf(py[hy < 0 ? -hy : hy], hy < 0);
+            const double2 exy = hx == 0 ? qy : (hy == 0 ? px[hx] : cmul(px[hx], qy));
+#pragma unroll
+            for (int hz = -HR; hz <= HR; ++hz)
+            {
+                if (hx == 0 && hy == 0 && hz <= 0) continue;
+                if ((hz < 0 ? -hz : hz) > hzm) continue;
+                const double2 ph = hz == 0 ? exy : cmul(exy, conjIf(pz[hz < 0 ? -hz : hz], hz < 0));
+                const double* e  = tab->kc[k++];
+                kp[0]            = fma(e[0], ph.x, kp[0]);
+                kp[1]            = fma(e[1], ph.y, kp[1]);
+                kp[2]            = fma(e[2], ph.y, kp[2]);
+                kp[3]            = fma(e[3], ph.y, kp[3]);
+            }
         }
 }
 
@@ -252,7 +263,8 @@ __global__ __launch_bounds__(256, 4) void ewaldKernel(const EwaldTable* __restri
 
     const double L = tab->L, lCut = tab->lCut;
     const double lCut2  = lCut * lCut * L * L;
-    const double alpha  = tab->alphaScale / L, alpha2 = alpha * alpha, invAlpha = L / tab->alphaScale;
+    const double alpha  = tab->alphaScale / L, alpha2 = alpha * alpha;
+    const double alpha4 = 4.0 * alpha, twoA2 = 2.0 * alpha2;
     const double k1     = M_PI / (alpha2 * L * L * L);
     const double M      = tab->Mroot[0];
     const double ka     = 2.0 * alpha / sqrt(M_PI);
@@ -317,6 +329,8 @@ __global__ __launch_bounds__(256, 4) void ewaldKernel(const EwaldTable* __restri
             fx  = exp(-c * r[0]), fy = exp(-c * r[1]), fz = exp(-c * r[2]);
             px_ = exp(c * ns * r[0]), py0 = exp(c * ns * r[1]), pz0 = exp(c * ns * r[2]);
         }
+        const double stepK = prodExp ? tab->expLat[1] * tab->expLat[1] : 0.0; // exp(-2 a^2 L^2)
+        const double hqzz  = 0.5 * q.qzz;
         for (int ix = -ns; ix <= ns; ix++, px_ *= fx)
         {
             const bool innerX = -nr <= ix && ix <= nr;
@@ -327,6 +341,7 @@ __global__ __launch_bounds__(256, 4) void ewaldKernel(const EwaldTable* __restri
                 if (!innerX && dx * dx > cutB) continue;
             }
             const double ex = prodExp ? e0 * px_ * tab->expLat[ix < 0 ? -ix : ix] : 0.0;
+            const double Rx = fma(double(ix), L, r[0]);
             double py_      = py0;
             for (int iy = -ns; iy <= ns; iy++, py_ *= fy)
             {
@@ -355,96 +370,85 @@ __global__ __launch_bounds__(256, 4) void ewaldKernel(const EwaldTable* __restri
                     zhi = __builtin_amdgcn_readfirstlane(min(zhi, ns));
                 }
                 if (zlo > zhi) continue;
-                const double exy = prodExp ? ex * py_ * tab->expLat[iy < 0 ? -iy : iy] : 0.0;
-                double pz_       = pz0;
-                for (int iz = -ns; iz < zlo; ++iz)
-                    pz_ *= fz;
-                for (int iz = zlo; iz <= zhi; iz++, pz_ *= fz)
+                // per row: R_x, R_y, the xy part of |R|^2, of Q R and of R.Q.R / 2
+                const double Ry   = fma(double(iy), L, r[1]);
+                const double Rxy2 = fma(Ry, Ry, Rx * Rx);
+                const double Qx   = fma(q.qxy, Ry, q.qxx * Rx);
+                const double Qy   = fma(q.qyy, Ry, q.qxy * Rx);
+                const double Qz   = fma(q.qyz, Ry, q.qxz * Rx);
+                const double c0   = 0.5 * fma(Ry, Qy, Rx * Qx);
+                // the Gaussian of the row's first image and its ratio to the next: E(iz + 1) = E(iz) cz,
+                // cz(iz + 1) = cz(iz) exp(-2 a^2 L^2)
+                double gz = 0, cz = 0;
+                if (prodExp)
                 {
-                    const bool inner = innerRow && (-nr <= iz && iz <= nr);
-                    double R[3] = {r[0] + ix * L, r[1] + iy * L, r[2] + iz * L};
-                    double R2   = R[0] * R[0] + R[1] * R[1] + R[2] * R[2];
+                    double pz_ = pz0;
+                    for (int iz = -ns; iz < zlo; ++iz)
+                        pz_ *= fz;
+                    gz = ex * py_ * tab->expLat[iy < 0 ? -iy : iy] * (pz_ * tab->expLat[zlo < 0 ? -zlo : zlo]);
+                    cz = fz * tab->expStep[zlo + kMaxCullShells];
+                }
+                double Rz = fma(double(zlo), L, r[2]);
+                for (int iz = zlo; iz <= zhi; iz++)
+                {
+                    const bool inner  = innerRow && (-nr <= iz && iz <= nr);
+                    const double Rzi  = Rz;
+                    const double R2   = fma(Rzi, Rzi, Rxy2);
+                    const double e2   = gz;
+                    Rz += L;
+                    if (prodExp) gz *= cz, cz *= stepK;
                     if (R2 > lCut2 && !inner) continue;
-                    double g[6];
+                    double g0, g1, g2, g3;
                     if (R2 < smallR2 && ka > 0)
                     {
-                        double c0 = ka, R2a2 = R2 * alpha2;
-                        g[0] = c0 * (R2a2 / 3.0 - 1.0);
-                        c0 *= 2 * alpha2;
-                        g[1] = c0 * (R2a2 / 5.0 - 1.0 / 3.0);
-                        c0 *= 2 * alpha2;
-                        g[2] = c0 * (R2a2 / 7.0 - 1.0 / 5.0);
-                        c0 *= 2 * alpha2;
-                        g[3] = c0 * (R2a2 / 9.0 - 1.0 / 7.0);
-                        c0 *= 2 * alpha2;
-                        g[4] = c0 * (R2a2 / 11.0 - 1.0 / 9.0);
-                        c0 *= 2 * alpha2;
-                        g[5] = c0 * (R2a2 / 13.0 - 1.0 / 11.0);
+                        double c0s = ka, R2a2 = R2 * alpha2;
+                        g0 = c0s * (R2a2 / 3.0 - 1.0);
+                        c0s *= twoA2;
+                        g1 = c0s * (R2a2 / 5.0 - 1.0 / 3.0);
+                        c0s *= twoA2;
+                        g2 = c0s * (R2a2 / 7.0 - 1.0 / 5.0);
+                        c0s *= twoA2;
+                        g3 = c0s * (R2a2 / 9.0 - 1.0 / 7.0);
                     }
                     else
                     {
-                        double invR = rsqrt64(R2), Rm = R2 * invR, invR2 = invR * invR;
-                        double e2 = prodExp ? exy * (pz_ * tab->expLat[iz < 0 ? -iz : iz]) : exp(-R2 * alpha2);
-                        double a  = e2 * ka * invR2;
-                        double an = 1.0;
-                        // erfc(aR) = exp(-a^2 R^2) erfcx(aR), reusing the exponential above; -erf = erfc - 1
-                        double ec = e2 * erfcxLds(alpha * Rm, invR * invAlpha, sErfcx);
-                        double fn = inner ? ec - 1.0 : ec;
-                        g[0] = fn * invR;
-                        g[1] = g[0] * invR2 + a;
-                        an *= 2 * alpha2;
-                        g[2] = 3 * g[1] * invR2 + an * a;
-                        an *= 2 * alpha2;
-                        g[3] = 5 * g[2] * invR2 + an * a;
-                        an *= 2 * alpha2;
-                        g[4] = 7 * g[3] * invR2 + an * a;
-                        an *= 2 * alpha2;
-                        g[5] = 9 * g[4] * invR2 + an * a;
+                        const double invR = rsqrt64(R2), Rm = R2 * invR, invR2 = invR * invR;
+                        const double eg   = prodExp ? e2 : exp(-R2 * alpha2);
+                        const double yv   = alpha4 * Rm; // 4 alpha R
+                        const double ex2  = erfcxLds(yv, sErfcx);
+                        // erfc(aR) = exp(-a^2 R^2) erfcx(aR) reusing the Gaussian above; -erf = erfc - 1 for the
+                        // replicas the traversal already holds
+                        const double off = inner ? 1.0 : 0.0;
+                        const double fn  = eg * ex2 - off;
+                        const double a  = (eg * ka) * invR2;
+                        const double a1 = a * twoA2;
+                        g0 = fn * invR;
+                        g1 = fma(g0, invR2, a);
+                        g2 = fma(g1, 3.0 * invR2, a1);
+                        g3 = fma(g2, 5.0 * invR2, a1 * twoA2);
                     }
-                    ewaldEvalQ(pa, R, g, q);
+                    // ewaldEvalQ at R = (Rx, Ry, Rz): Q R, R.Q.R / 2 from the row's parts
+                    const double Qrx = fma(q.qxz, Rzi, Qx);
+                    const double Qry = fma(q.qyz, Rzi, Qy);
+                    const double Qrz = fma(q.qzz, Rzi, Qz);
+                    const double rQr = fma(Rzi, fma(hqzz, Rzi, Qz), c0);
+                    pa[0]            = fma(-g0, q.M, pa[0]);
+                    pa[0]            = fma(g1, q.Qtr, pa[0]);
+                    pa[0]            = fma(-g2, rQr, pa[0]);
+                    const double f   = fma(g3, rQr, fma(g1, q.M, -g2 * q.Qtr));
+                    pa[1]            = fma(-Rx, f, fma(g2, Qrx, pa[1]));
+                    pa[2]            = fma(-Ry, f, fma(g2, Qry, pa[2]));
+                    pa[3]            = fma(-Rzi, f, fma(g2, Qrz, pa[3]));
                 }
             }
         }
-        // k space: exp(i 2 pi h.r / L) for the table's lattice vectors, visited in the table's order (hx, hy, hz
-        // ascending, 0 < |h|^2 <= hCut^2), as products of the axis phases exp(i 2 pi h_d r_d / L) (|h_d| <= 3)
-        // instead of one sincos per vector
         double kp[4] = {0, 0, 0, 0};
-        double2 qx[3], qy[3], qz[3]; // exp(i j 2 pi r_d / L), j = 1, 2, 3
+        switch (tab->hReps)
         {
-            const double w = 2.0 * M_PI / L;
-            double s, c;
-            sincos(w * r[0], &s, &c);
-            qx[0] = make_double2(c, s);
-            sincos(w * r[1], &s, &c);
-            qy[0] = make_double2(c, s);
-            sincos(w * r[2], &s, &c);
-            qz[0] = make_double2(c, s);
-            qx[1] = cmul(qx[0], qx[0]), qy[1] = cmul(qy[0], qy[0]), qz[1] = cmul(qz[0], qz[0]);
-            qx[2] = cmul(qx[1], qx[0]), qy[2] = cmul(qy[1], qy[0]), qz[2] = cmul(qz[1], qz[0]);
-        }
-        const int hr = tab->hReps;
-        int k = 0;
-        for (int hx = -hr; hx <= hr; ++hx)
-        {
-            const double2 ex = phasePow(hx, qx[0], qx[1], qx[2]);
-            for (int hy = -hr; hy <= hr; ++hy)
-            {
-                const int hzm = tab->kRowHz[(hx + hr) * (2 * hr + 1) + (hy + hr)];
-                if (hzm < 0) continue;
-                const double2 exy = cmul(ex, phasePow(hy, qy[0], qy[1], qy[2]));
-                for (int hz = -hzm; hz <= hzm; ++hz)
-                {
-                    if (hx == 0 && hy == 0 && hz == 0) continue;
-                    const double2 ph = cmul(exy, phasePow(hz, qz[0], qz[1], qz[2]));
-                    const double* e  = tab->h[k++];
-                    double sum = e[3] * ph.x + e[4] * ph.y;
-                    double dif = e[3] * ph.y - e[4] * ph.x;
-                    kp[0] += tab->kPotSign * sum;
-                    kp[1] += dif * e[0];
-                    kp[2] += dif * e[1];
-                    kp[3] += dif * e[2];
-                }
-            }
+            case 1: ewaldKSpace<1>(tab, r, kp); break;
+            case 2: ewaldKSpace<2>(tab, r, kp); break;
+            case 3: ewaldKSpace<3>(tab, r, kp); break;
+            default: break;
         }
         double pot = pa[0] + kp[0];
         double mi  = m[i];
@@ -491,9 +495,8 @@ int ewald(Context* ctx, const double* Mroot, const double* center, uint32_t firs
                         settings[2]);
     EwaldTable* t = new EwaldTable();
     ewaldInitTable(*t, Mroot, center, int(settings[0]), double(lx), settings[1], settings[2], settings[3],
-                   settings[4]);
-    t->kPotSign = consistentPotential ? 1.0 : -1.0;
-    ewaldInitCulling(*t, limits);
+                   settings[4], consistentPotential ? 1.0 : -1.0);
+    ewaldInitFactors(*t);
     if (t->numEwaldShells == 0 || last <= first)
     {
         delete t;
