@@ -294,6 +294,62 @@ def test_full_size_invariants(ctx, name, mk):
     sysm.close()
 
 
+def test_headline_periodic_full_size(ctx):
+    """the headline configuration: 2^26 uniform particles in a unit periodic cube, theta 0.5, 64-target groups,
+    1 replica shell (27 images) + FP64 Ewald. At full size: the sorted keys are the oracle's keys in stable order,
+    the leaves and counts are the oracle's; the traversal over the images on 48 seeded target groups equals the
+    oracle traversal of the same tree; the full result (images + Ewald) of those targets equals the oracle
+    traversal plus the oracle's FP64 Ewald correction from the oracle's FP64 root multipole"""
+    s = W.uniform_cube(1 << 26, seed=42, lo=0.0, hi=1.0, periodic=True)
+    n = s.n
+    print(f"headline periodic: n={n}", flush=True)
+    sysm = R.System(ctx, n)
+    arrs = [dev(a) for a in s.arrays()]
+    sysm.load(*arrs, s.limits, periodic=True)
+    prm = R.default_params(num_shells=1, ewald=0)
+    sysm.build(prm)
+    sysm.gravity(prm)
+    _, tax, tay, taz, order = (host(a) for a in sysm.results())
+    sysm.gravity(R.default_params(num_shells=1, ewald=1))
+    _, ax, ay, az, _ = (host(a) for a in sysm.results())
+    t = sysm.tree()
+    keys = host(t["keys"], np.uint64)
+    leaves, counts, layout = host(t["leaves"], np.uint64), host(t["counts"], np.uint32), host(t["layout"])
+    del t, arrs
+    sysm.close()
+    assert np.bincount(order, minlength=n).max() == 1
+    ok = O.sfc_keys(s.x, s.y, s.z, s.limits)
+    assert np.array_equal(keys, ok[order])
+    assert np.all((keys[1:] > keys[:-1]) | ((keys[1:] == keys[:-1]) & (order[1:] > order[:-1])))
+    del ok
+    lo, co = O.compute_octree(keys, 64)
+    assert np.array_equal(leaves, lo) and np.array_equal(counts, co)
+    del lo, co, keys
+    x, y, z, m, h = (a[order] for a in s.arrays())
+    c, q = O.upsweep(x, y, z, m, leaves, layout, s.limits, 2.0)
+    groups = np.sort(np.random.default_rng(26).choice(n // 64, 48, replace=False)).astype(np.int32)
+    idx = np.concatenate([np.arange(64 * g, 64 * g + 64) for g in groups])
+    _, _, oax, oay, oaz, _ = O.compute_gravity(leaves, layout, c, q, x, y, z, h, m, s.limits, group_size=64,
+                                               num_shells=1, group_list=groups, want_pot=False)
+    o = np.stack([oax[idx], oay[idx], oaz[idx]], 1)
+    rms = np.sqrt(np.mean(np.sum(o ** 2, axis=1)))
+    d = np.linalg.norm(np.stack([tax[idx], tay[idx], taz[idx]], 1) - o, axis=1) / rms
+    assert np.quantile(d, 0.99) <= 2e-5, np.quantile(d, 0.99)
+    # the image sum (of order 1 near the faces: the field inside a uniform cube of side 3L) and the Ewald
+    # correction (which cancels its linear part) leave a net force of order 1e-3 at this N, so each part is
+    # compared on its own scale: the correction as the difference of the solves with and without it (the
+    # traversal is deterministic, so that difference carries only the float rounding of the sum)
+    c64, q64 = O.upsweep(x, y, z, m, leaves, layout, s.limits.astype(np.float64), 2.0, precision="f64")
+    _, _, eax, eay, eaz = O.ewald(q64[0], c64[0, :3], x[idx], y[idx], z[idx], m[idx], 1.0)
+    eo = np.stack([eax, eay, eaz], 1)
+    eg = np.stack([a[idx].astype(np.float64) - b[idx] for a, b in ((ax, tax), (ay, tay), (az, taz))], 1)
+    rms_e = np.sqrt(np.mean(np.sum(eo ** 2, axis=1)))
+    d = np.linalg.norm(eg - eo, axis=1) / rms_e
+    assert np.quantile(d, 0.99) <= 1e-5, np.quantile(d, 0.99)
+    d = np.linalg.norm(np.stack([ax[idx], ay[idx], az[idx]], 1) - (o + eo), axis=1) / rms
+    assert np.quantile(d, 0.99) <= 3e-5, np.quantile(d, 0.99)
+
+
 def test_consistent_ewald_potential(ctx):
     """ewald = 2: accelerations bit-identical to the reference's correction (ewald = 1), potentials equal to the
     oracle's consistent correction and much closer to the exact Ewald sum"""
