@@ -468,6 +468,25 @@ __global__ __launch_bounds__(256, 4) void ewaldKernel(const EwaldTable* __restri
     if (threadIdx.x == 0) blockPot[blockIdx.x] = sPot[0];
 }
 
+// block b sums v[b * chunk, min(n, (b + 1) * chunk)) into part[b]: the first stage of a sum in a fixed order
+__global__ __launch_bounds__(256) void sumChunksKernel(const double* __restrict__ v, int n, int chunk,
+                                                       double* __restrict__ part)
+{
+    __shared__ double ss[256];
+    const int b = blockIdx.x * chunk, e = min(n, b + chunk);
+    double s = 0;
+    for (int i = b + threadIdx.x; i < e; i += 256)
+        s += v[i];
+    ss[threadIdx.x] = s;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1)
+    {
+        if (int(threadIdx.x) < o) ss[threadIdx.x] += ss[threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) part[blockIdx.x] = ss[0];
+}
+
 __global__ __launch_bounds__(256) void sumFixedKernel(const double* __restrict__ v, int n, double* __restrict__ out)
 {
     __shared__ double ss[256];
@@ -504,7 +523,10 @@ int ewald(Context* ctx, const double* Mroot, const double* center, uint32_t firs
         return RYO_OK;
     }
     unsigned blocks = iceil(last - first, 256);
-    size_t need     = sizeof(EwaldTable) + 8 * size_t(blocks) + 64;
+    // the per-block potentials are summed in two fixed-order stages: up to 256 chunks, then the chunk sums
+    const int chunk = std::max(4096, int(iceil(blocks, 256)));
+    const int parts = int(iceil(blocks, unsigned(chunk)));
+    size_t need     = sizeof(EwaldTable) + 8 * (size_t(blocks) + 1 + size_t(parts)) + 64;
     auto* buf       = static_cast<char*>(ctx->getScratch(kSlotEwald, need));
     if (!buf)
     {
@@ -528,7 +550,9 @@ int ewald(Context* ctx, const double* Mroot, const double* center, uint32_t firs
     else if (prod) ewaldKernel<false, true><<<blocks, 256, 0, s>>>(dt, first, last, x, y, z, m, G, ugrav, ax, ay, az,
                                                                     blockPot);
     else ewaldKernel<false, false><<<blocks, 256, 0, s>>>(dt, first, last, x, y, z, m, G, ugrav, ax, ay, az, blockPot);
-    sumFixedKernel<<<1, 256, 0, s>>>(blockPot, int(blocks), dsum);
+    double* part = dsum + 1;
+    sumChunksKernel<<<parts, 256, 0, s>>>(blockPot, int(blocks), chunk, part);
+    sumFixedKernel<<<1, 256, 0, s>>>(part, parts, dsum);
     RYO_LAUNCH_CHECK(ctx);
     if (totalPot)
     {
