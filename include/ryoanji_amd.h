@@ -460,7 +460,7 @@ int ryo_system_accuracy(ryo_system* sys, int num_targets, uint64_t seed, double*
  * linked octree, FP64 upsweep, groups, traversal) runs once for all of them, driven by per-system offset
  * tables. Each system gets exactly the keys, tree, multipoles and forces it gets when solved on its own
  * (GroupView offsets, cstone/traversal/groups.hpp:28-34, are the reference's per-batch table; the reference
- * solves one system per call). Open boundaries. */
+ * solves one system per call). Systems are open unless ryo_batch_set_periodic marks them periodic. */
 typedef struct ryo_batch ryo_batch;
 int ryo_batch_create(ryo_ctx* ctx, size_t max_particles, int max_systems, ryo_batch** batch);
 int ryo_batch_destroy(ryo_batch* batch);
@@ -468,8 +468,14 @@ int ryo_batch_destroy(ryo_batch* batch);
  * host memory, copied); limits[6 S] the box of each system (cubic or not) */
 int ryo_batch_load(ryo_batch* batch, int num_systems, const uint32_t* counts, const float* x, const float* y,
                    const float* z, const float* m, const float* h, const float* limits);
+/* periodic[S] (host) for the loaded systems: nonzero marks a periodic system, solved as a resident periodic
+ * system is (traversal over the (2 prm.num_shells + 1)^3 images of its box, then, with prm.ewald 1 or 2, the FP64
+ * Ewald correction of ryo_ewald_f32 from its root multipole; a cubic box for the Ewald correction); every
+ * system is open after ryo_batch_load. (traversal_gpu.cu:477-518 traverses the images of any call.) */
+int ryo_batch_set_periodic(ryo_batch* batch, const int* periodic);
 /* one full pass for every system; theta[S] and G[S] per system (host; NULL: prm's value), pot[S] (host, may be
- * NULL) receives each system's 0.5 G sum(m phi). prm: bucket_size and group_size (<= 64). */
+ * NULL) receives each system's 0.5 G sum(m phi) (with its Ewald correction's for a periodic system). prm:
+ * bucket_size and group_size (<= 64); num_shells, ewald and ewald_settings for the periodic systems. */
 int ryo_batch_solve(ryo_batch* batch, const ryo_params* prm, const float* theta, const float* G, double* pot);
 /* The same solve as ryo_batch_solve, bit for bit, run as one captured HIP graph (keys -> sort -> leaves ->
  * octrees -> upsweep -> groups -> traversal, no host round trip inside; ryo_batch_solve reads the leaf counts
@@ -477,7 +483,8 @@ int ryo_batch_solve(ryo_batch* batch, const ryo_params* prm, const float* theta,
  * counts, bucket or group size, init node, traversal shape or a buffer changed; positions, boxes, theta and G
  * may change between replays. Its node arrays hold the capturing solve's totals plus a margin; a replay whose
  * trees outgrow them is detected on the device and solved again exactly (counted in the statistics). The
- * batch's stream must not be capturing. Synchronizes. */
+ * batch's stream must not be capturing. Synchronizes. Open systems only (RYO_ERR_UNSUPPORTED with a periodic
+ * one: the Ewald tables are built on the host from the solved root multipoles). */
 int ryo_batch_solve_graph(ryo_batch* batch, const ryo_params* prm, const float* theta, const float* G,
                           double* pot);
 /* out[4] = {graphs captured, replays, replays solved again because the trees outgrew the graph's arrays,

@@ -158,6 +158,7 @@ SIGNATURES = {
     "ryo_batch_create": (_I, [_P, _SZ, _I, C.POINTER(_P)]),
     "ryo_batch_destroy": (_I, [_P]),
     "ryo_batch_load": (_I, [_P, _I, _P, _P, _P, _P, _P, _P, _P]),
+    "ryo_batch_set_periodic": (_I, [_P, _P]),
     "ryo_batch_solve": (_I, [_P, C.POINTER(Params), _P, _P, _P]),
     "ryo_batch_solve_graph": (_I, [_P, C.POINTER(Params), _P, _P, _P]),
     "ryo_ctx_set_traversal_waves": (_I, [_P, _I]),
@@ -382,8 +383,9 @@ class Batch:
     def _check(self, rc):
         return self.ctx._check(rc)
 
-    def load(self, systems):
-        """systems: objects with x, y, z, m, h arrays (numpy float32) and limits[6]"""
+    def load(self, systems, periodic=None):
+        """systems: objects with x, y, z, m, h arrays (numpy float32) and limits[6]; periodic: per-system flags
+        (default: each system's own `periodic` attribute, else open)"""
         import numpy as np
 
         counts = np.array([s.n for s in systems], np.uint32)
@@ -394,6 +396,11 @@ class Batch:
         self._check(lib().ryo_batch_load(self.h, len(systems), counts.ctypes.data, *(a.ctypes.data for a in cat),
                                          lim.ctypes.data))
         self.S = len(systems)
+        if periodic is None:
+            periodic = [bool(getattr(s, "periodic", False)) for s in systems]
+        flags = np.ascontiguousarray(periodic, np.int32)
+        if flags.any():
+            self._check(lib().ryo_batch_set_periodic(self.h, flags.ctypes.data))
 
     def solve(self, params: Params, theta=None, G=None, graph: bool = False):
         """one pass over every system; graph=True replays the pipeline as one captured HIP graph (the same

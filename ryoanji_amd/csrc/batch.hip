@@ -54,8 +54,12 @@ int batchTraverse(Context* ctx, int numGroups, const uint32_t* gStart, const uin
                   const BatchView& bv, const float* G, const float* x, const float* y, const float* z,
                   const float* m, const float* h, size_t n, const int32_t* child, const int32_t* i2l,
                   const uint32_t* layout, const float* centers, const float* mp, float* p, float* ax, float* ay,
-                  float* az, double* sysPot, int initNode, void* work, const uint32_t* abort, int mode);
+                  float* az, double* sysPot, int initNode, void* work, const uint32_t* abort, int mode,
+                  const SysImages* images);
 size_t batchTraverseWorkBytes(size_t numGroups);
+int ewaldBatch(Context* ctx, int K, const uint32_t* first, const uint32_t* last, const float* lim, const double* roots,
+               const float* G, const double* settings, bool consistentPotential, const float* x, const float* y,
+               const float* z, const float* m, float* ugrav, float* ax, float* ay, float* az, double* pot);
 
 int batchUpsweepEntry(Context* ctx, const float* x, const float* y, const float* z, const float* m,
                       const uint32_t* layout, const uint64_t* prefixes, const int32_t* child, const int32_t* parents,
@@ -106,6 +110,8 @@ struct ryo_batch
     size_t N = 0;
     std::vector<uint32_t> hP, hTS, hLO, hNO, hNIO, hPO, hGO;
     std::vector<float> limits;
+    std::vector<int> periodic; // per system (ryo_batch_set_periodic; all open after a load)
+    std::vector<float> hG;     // per system G of the current solve
 
     // particles: loaded (packed, original order) and sorted (x, y, z, m, h, original index)
     DevBuf in[5], srt[6];
@@ -118,6 +124,7 @@ struct ryo_batch
     DevBuf nodeIdx, nodeKeyBuf, nodeValBuf, nodeSeg, nodeSegBuf, nodePos, nodePosBuf, nodeTmp;
     DevBuf gStart, gEnd, gSys;
     DevBuf travWork, flags; // traversal work arrays; flags[0]: the trees outgrew the node arrays
+    DevBuf images, roots;   // per-system periodic images; the root multipoles of the periodic systems
     std::vector<double> hPot;
     uint32_t totalSlots = 0, totalNodes = 0, totalInternal = 0, totalParents = 0, totalGroups = 0;
     bool solved = false;
@@ -156,7 +163,7 @@ void allBufs(ryo_batch* b, std::vector<DevBuf*>& out)
                       &b->prefixes, &b->child,    &b->parents,  &b->i2l,        &b->l2i,        &b->centers,
                       &b->mp,       &b->c64,      &b->q64,      &b->nodeIdx,    &b->nodeKeyBuf, &b->nodeValBuf,
                       &b->nodeSeg,  &b->nodeSegBuf, &b->nodePos, &b->nodePosBuf, &b->nodeTmp,   &b->gStart,
-                      &b->gEnd,     &b->gSys,     &b->travWork, &b->flags};
+                      &b->gEnd,     &b->gSys,     &b->travWork, &b->flags,     &b->images,    &b->roots};
     for (DevBuf* d : rest) out.push_back(d);
 }
 
@@ -175,6 +182,7 @@ struct Shape
     unsigned gs    = 64;
     int initNode   = 1;
     int numTiles   = 0;
+    bool periodic  = false; // some system is periodic: per-system images (and the Ewald correction when asked)
 };
 
 // host tables known before the build (particle offsets P, leaf tiles TS, groups GO) and the per-system key and
@@ -189,8 +197,8 @@ int uploadTables(ryo_batch* b, const ryo_params* prm, const float* theta, const 
     sh.initNode = prm->init_node;
     if (sh.gs > 64) return setError(ctx, RYO_ERR_UNSUPPORTED, "batch: group size %u > 64", sh.gs);
     if (prm->init_node != 0 && prm->init_node != 1) return setError(ctx, RYO_ERR_ARG, "batch: init_node 0 or 1");
-    if (prm->num_shells != 0 || prm->ewald != 0)
-        return setError(ctx, RYO_ERR_UNSUPPORTED, "batch: open boundaries only");
+    if (prm->num_shells < 0 || prm->ewald < 0 || prm->ewald > 2)
+        return setError(ctx, RYO_ERR_ARG, "batch: num_shells >= 0 and ewald 0, 1 or 2");
     if (prm->soft_mac != 0) return setError(ctx, RYO_ERR_UNSUPPORTED, "batch: the reference MAC only");
     if (prm->group_split_tol != 0.0f) return setError(ctx, RYO_ERR_UNSUPPORTED, "batch: fixed target groups only");
     std::vector<uint32_t> tiles(S), ngroups(S);
@@ -207,6 +215,21 @@ int uploadTables(ryo_batch* b, const ryo_params* prm, const float* theta, const 
         kb[s] = makeKeyBox(&b->limits[6 * s]);
         mb[s] = makeMacBox(&b->limits[6 * s], 1.0f / th);
         gv[s] = Gs ? Gs[s] : prm->G;
+    }
+    b->hG = gv;
+    // periodic systems: (2 num_shells + 1)^3 images of their box (as a resident periodic system); open ones none
+    sh.periodic = std::any_of(b->periodic.begin(), b->periodic.end(), [](int v) { return v != 0; });
+    if (sh.periodic)
+    {
+        std::vector<SysImages> im(S);
+        for (int s = 0; s < S; ++s)
+        {
+            const float* l = &b->limits[6 * s];
+            im[s]          = SysImages{b->periodic[s] ? prm->num_shells : 0, {l[1] - l[0], l[3] - l[2], l[5] - l[4]}};
+        }
+        int rc = b->images.ensure(ctx, S * sizeof(SysImages));
+        if (rc) return rc;
+        RYO_HIP(ctx, hipMemcpyAsync(b->images.p, im.data(), S * sizeof(SysImages), hipMemcpyHostToDevice, st));
     }
     b->hTS = prefix(tiles);
     b->hGO = prefix(ngroups);
@@ -284,7 +307,7 @@ int traversal(ryo_batch* b, const Shape& sh, int mode, const uint32_t* abort)
                          b->N, b->child.as<int32_t>(), b->i2l.as<int32_t>(), b->layout.as<uint32_t>(),
                          b->centers.as<float>(), b->mp.as<float>(), b->p.as<float>(), b->ax.as<float>(),
                          b->ay.as<float>(), b->az.as<float>(), b->sysPot.as<double>(), sh.initNode, b->travWork.p,
-                         abort, mode);
+                         abort, mode, sh.periodic ? b->images.as<SysImages>() : nullptr);
 }
 
 // leaves, linked octrees, upsweep, groups, zeroed outputs and the traversal. slots/nodes/internal/parents:
@@ -365,13 +388,74 @@ int solveEager(ryo_batch* b, const Shape& sh)
     return runBack(b, sh, b->totalSlots, b->totalNodes, b->totalInternal, b->totalParents, false);
 }
 
-// per-system potentials to the host
-int finishSolve(ryo_batch* b, const Shape&, double* pot)
+// the root multipole {Mroot[8], center[3], 0} of each listed system (node 0 of its slice)
+__global__ void batchRootsKernel(const double* __restrict__ c64, const double* __restrict__ q64,
+                                 const uint32_t* __restrict__ NO, const int32_t* __restrict__ sys, int K,
+                                 double* __restrict__ out)
+{
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= K) return;
+    const size_t node = NO[sys[k]];
+    for (int j = 0; j < 8; ++j)
+        out[12 * k + j] = q64[8 * node + j];
+    for (int j = 0; j < 3; ++j)
+        out[12 * k + 8 + j] = c64[4 * node + j];
+    out[12 * k + 11] = 0.0;
+}
+
+// the Ewald correction of the periodic systems (after the traversal), as a resident periodic system takes it;
+// ewPot[s] receives each system's correction potential (0 for the others)
+int batchEwald(ryo_batch* b, const ryo_params* prm, std::vector<double>& ewPot)
+{
+    Context* ctx = b->ctx;
+    const int S  = b->S;
+    ewPot.assign(S, 0.0);
+    std::vector<int32_t> sys;
+    for (int s = 0; s < S; ++s)
+        if (b->periodic[s]) sys.push_back(s);
+    const int K = int(sys.size());
+    if (K == 0 || prm->ewald == 0) return RYO_OK;
+    int rc = b->roots.ensure(ctx, size_t(K) * (12 * 8 + 4));
+    if (rc) return rc;
+    auto* dRoots = b->roots.as<double>();
+    auto* dSys   = reinterpret_cast<int32_t*>(dRoots + 12 * size_t(K));
+    hipStream_t st = ctx->stream;
+    RYO_HIP(ctx, hipMemcpyAsync(dSys, sys.data(), K * 4, hipMemcpyHostToDevice, st));
+    batchRootsKernel<<<iceil(K, 64), 64, 0, st>>>(b->c64.as<double>(), b->q64.as<double>(), b->T(3), dSys, K, dRoots);
+    RYO_LAUNCH_CHECK(ctx);
+    std::vector<double> roots(12 * size_t(K));
+    RYO_HIP(ctx, hipMemcpyAsync(roots.data(), dRoots, roots.size() * 8, hipMemcpyDeviceToHost, st));
+    RYO_HIP(ctx, hipStreamSynchronize(st));
+    std::vector<uint32_t> first(K), last(K);
+    std::vector<float> lim(6 * size_t(K)), G(K);
+    for (int k = 0; k < K; ++k)
+    {
+        const int s = sys[k];
+        first[k] = b->hP[s], last[k] = b->hP[s + 1];
+        std::memcpy(&lim[6 * k], &b->limits[6 * s], 6 * sizeof(float));
+        G[k] = b->hG[s];
+    }
+    std::vector<double> pot(K);
+    rc = ewaldBatch(ctx, K, first.data(), last.data(), lim.data(), roots.data(), G.data(), prm->ewald_settings,
+                    prm->ewald == 2, b->srt[0].as<float>(), b->srt[1].as<float>(), b->srt[2].as<float>(),
+                    b->srt[3].as<float>(), b->p.as<float>(), b->ax.as<float>(), b->ay.as<float>(), b->az.as<float>(),
+                    pot.data());
+    if (rc) return rc;
+    for (int k = 0; k < K; ++k)
+        ewPot[sys[k]] = pot[k];
+    return RYO_OK;
+}
+
+// per-system potentials to the host (plus the Ewald potentials of periodic systems, when given)
+int finishSolve(ryo_batch* b, const Shape&, double* pot, const std::vector<double>* ewPot = nullptr)
 {
     Context* ctx = b->ctx;
     b->hPot.resize(b->S);
     RYO_HIP(ctx, hipMemcpyAsync(b->hPot.data(), b->sysPot.p, b->S * 8, hipMemcpyDeviceToHost, ctx->stream));
     RYO_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    if (ewPot)
+        for (int s = 0; s < b->S; ++s)
+            b->hPot[s] += (*ewPot)[s];
     if (pot) std::memcpy(pot, b->hPot.data(), b->S * sizeof(double));
     b->solved = true;
     return RYO_OK;
@@ -533,9 +617,19 @@ int ryo_batch_load(ryo_batch* b, int num_systems, const uint32_t* counts, const 
     b->N = N;
     b->hP = prefix(std::vector<uint32_t>(counts, counts + num_systems));
     b->limits.assign(limits, limits + 6 * num_systems);
+    b->periodic.assign(num_systems, 0);
     const float* src[5] = {x, y, z, m, hh};
     for (int k = 0; k < 5; ++k)
         RYO_HIP(ctx, hipMemcpyAsync(b->in[k].p, src[k], N * 4, hipMemcpyDefault, ctx->stream));
+    b->solved = false;
+    return RYO_OK;
+}
+
+int ryo_batch_set_periodic(ryo_batch* b, const int* periodic)
+{
+    if (!b || !periodic || b->S < 1) return RYO_ERR_ARG;
+    for (int s = 0; s < b->S; ++s)
+        b->periodic[s] = periodic[s] != 0;
     b->solved = false;
     return RYO_OK;
 }
@@ -546,9 +640,11 @@ int ryo_batch_solve(ryo_batch* b, const ryo_params* prm, const float* theta, con
     if (prm->multipole != 0)
         return setError(b->ctx, RYO_ERR_UNSUPPORTED, "batch: CartesianQuadrupole multipoles only");
     Shape sh;
+    std::vector<double> ewPot;
     int rc = uploadTables(b, prm, theta, Gs, sh);
     if (!rc) rc = solveEager(b, sh);
-    if (!rc) rc = finishSolve(b, sh, pot);
+    if (!rc && sh.periodic) rc = batchEwald(b, prm, ewPot);
+    if (!rc) rc = finishSolve(b, sh, pot, sh.periodic ? &ewPot : nullptr);
     return rc;
 }
 
@@ -560,6 +656,9 @@ int ryo_batch_solve_graph(ryo_batch* b, const ryo_params* prm, const float* thet
     Shape sh;
     int rc = uploadTables(b, prm, theta, Gs, sh);
     if (rc) return rc;
+    if (sh.periodic)
+        return setError(ctx, RYO_ERR_UNSUPPORTED,
+                        "batch graph: open boundaries only (periodic systems take ryo_batch_solve)");
     const bool current = b->exec && b->graphKey == graphKey(b, sh) && b->graphBufs == bufferSignature(b);
     if (!current)
     {

@@ -128,6 +128,13 @@ struct BatchView
     const int32_t* LR;
 };
 
+// the periodic images of one system of a batch: (2 shells + 1)^3 replicas of its box lengths L (shells 0: open)
+struct SysImages
+{
+    int shells;
+    float L[3];
+};
+
 // the segment holding element i: the largest s with off[s] <= i (empty segments are skipped)
 __device__ inline int segmentOf(const uint32_t* off, int S, uint32_t i)
 {

@@ -243,12 +243,14 @@ __device__ inline void ewaldKSpace(const EwaldTable* __restrict__ tab, const dou
 
 // RowTable: the per-wave row bounds fit the LDS table (at most kMaxCullShells image shells); ProdExp: the image
 // Gaussians come from products of per-axis factors (EwaldTable::productExp)
+// one block of 256 particles: particle i of the block's thread (valid: i < last); the block's sum of m phi into
+// *blockPot
 template<bool RowTable, bool ProdExp>
-__global__ __launch_bounds__(256, 4) void ewaldKernel(const EwaldTable* __restrict__ tab, uint32_t first, uint32_t last,
-                                                  const float* __restrict__ x, const float* __restrict__ y,
-                                                  const float* __restrict__ z, const float* __restrict__ m, float G,
-                                                  float* ugrav, float* ax, float* ay, float* az,
-                                                  double* __restrict__ blockPot)
+__device__ __forceinline__ void ewaldBlock(const EwaldTable* __restrict__ tab, uint32_t i, uint32_t last,
+                                           const float* __restrict__ x, const float* __restrict__ y,
+                                           const float* __restrict__ z, const float* __restrict__ m, float G,
+                                           float* ugrav, float* ax, float* ay, float* az,
+                                           double* __restrict__ blockPot)
 {
     // the table is read with wave-uniform addresses (scalar loads), so the quadrupole and the k-space
     // coefficients live in scalar registers
@@ -271,7 +273,6 @@ __global__ __launch_bounds__(256, 4) void ewaldKernel(const EwaldTable* __restri
     const double smallR2 = tab->smallR * L * L;
     const int ns = tab->numEwaldShells, nr = tab->numReplicaShells;
 
-    uint32_t i       = first + blockIdx.x * blockDim.x + threadIdx.x;
     const bool valid = i < last;
     double myPot     = 0;
     // the wave's bounding box of r = x - center over its valid lanes (targets are SFC neighbours, so the box is
@@ -465,7 +466,60 @@ __global__ __launch_bounds__(256, 4) void ewaldKernel(const EwaldTable* __restri
         if (int(threadIdx.x) < o) sPot[threadIdx.x] += sPot[threadIdx.x + o];
         __syncthreads();
     }
-    if (threadIdx.x == 0) blockPot[blockIdx.x] = sPot[0];
+    if (threadIdx.x == 0) *blockPot = sPot[0];
+}
+
+template<bool RowTable, bool ProdExp>
+__global__ __launch_bounds__(256, 4) void ewaldKernel(const EwaldTable* __restrict__ tab, uint32_t first, uint32_t last,
+                                                  const float* __restrict__ x, const float* __restrict__ y,
+                                                  const float* __restrict__ z, const float* __restrict__ m, float G,
+                                                  float* ugrav, float* ax, float* ay, float* az,
+                                                  double* __restrict__ blockPot)
+{
+    ewaldBlock<RowTable, ProdExp>(tab, first + blockIdx.x * blockDim.x + threadIdx.x, last, x, y, z, m, G, ugrav, ax,
+                                  ay, az, blockPot + blockIdx.x);
+}
+
+// block b of a batch's Ewald launch: block `block` of the particles [first, last) of one system, with that
+// system's table and G (the blocks of a system start at its first particle, so its waves hold the particles
+// they hold when the system is corrected alone)
+struct EwaldBatchBlock
+{
+    int table;
+    uint32_t first, last, block;
+    float G;
+};
+
+template<bool RowTable, bool ProdExp>
+__global__ __launch_bounds__(256, 4) void ewaldBatchKernel(const EwaldTable* __restrict__ tabs,
+                                                       const EwaldBatchBlock* __restrict__ blocks,
+                                                       const float* __restrict__ x, const float* __restrict__ y,
+                                                       const float* __restrict__ z, const float* __restrict__ m,
+                                                       float* ugrav, float* ax, float* ay, float* az,
+                                                       double* __restrict__ blockPot)
+{
+    const EwaldBatchBlock bd = blocks[blockIdx.x];
+    ewaldBlock<RowTable, ProdExp>(tabs + bd.table, bd.first + bd.block * blockDim.x + threadIdx.x, bd.last, x, y, z, m,
+                                  bd.G, ugrav, ax, ay, az, blockPot + blockIdx.x);
+}
+
+// seg[k] = fixed-order sum of v[off[k], off[k+1]) (one block per segment)
+__global__ __launch_bounds__(256) void segmentSumKernel(const double* __restrict__ v, const uint32_t* __restrict__ off,
+                                                        double* __restrict__ seg)
+{
+    __shared__ double ss[256];
+    const uint32_t b = off[blockIdx.x], e = off[blockIdx.x + 1];
+    double s = 0;
+    for (uint32_t i = b + threadIdx.x; i < e; i += 256)
+        s += v[i];
+    ss[threadIdx.x] = s;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1)
+    {
+        if (int(threadIdx.x) < o) ss[threadIdx.x] += ss[threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) seg[blockIdx.x] = ss[0];
 }
 
 // block b sums v[b * chunk, min(n, (b + 1) * chunk)) into part[b]: the first stage of a sum in a fixed order
@@ -561,6 +615,85 @@ int ewald(Context* ctx, const double* Mroot, const double* center, uint32_t firs
         RYO_HIP(ctx, hipStreamSynchronize(s));
         *totalPot = 0.5 * double(G) * U;
     }
+    return RYO_OK;
+}
+
+/* The Ewald correction of K periodic systems of a batch (packed arrays x, y, z, m and results): system k owns
+ * particles [first[k], last[k]), a cubic box lim[6 k ..], the FP64 root multipole roots[12 k ..] = {Mroot[8],
+ * center[3], -} and G[k]. One launch for all systems (each block within one system, with its table); the block
+ * potentials of a system are summed in the same two fixed-order stages as for a system corrected alone, so
+ * pot[k] (host) = 0.5 G_k sum(m phi) and every result equal, bit for bit, those of ewald() on that system. */
+int ewaldBatch(Context* ctx, int K, const uint32_t* first, const uint32_t* last, const float* lim, const double* roots,
+               const float* G, const double* settings, bool consistentPotential, const float* x, const float* y,
+               const float* z, const float* m, float* ugrav, float* ax, float* ay, float* az, double* pot)
+{
+    if (K <= 0) return RYO_OK;
+    if (std::ceil(settings[2]) > 3.0)
+        return setError(ctx, RYO_ERR_UNSUPPORTED, "Ewald: hCut %g > 3 (the reference's maxCeilHcut, ewald.hpp:53)",
+                        settings[2]);
+    std::vector<EwaldTable> tabs(K);
+    for (int k = 0; k < K; ++k)
+    {
+        const float* l = lim + 6 * k;
+        float lx = l[1] - l[0], ly = l[3] - l[2], lz = l[5] - l[4];
+        if (!(lx == ly && ly == lz))
+            return setError(ctx, RYO_ERR_UNSUPPORTED, "Ewald gravity requires cubic boxes (periodic system %d)", k);
+        ewaldInitTable(tabs[k], roots + 12 * k, roots + 12 * k + 8, int(settings[0]), double(lx), settings[1],
+                       settings[2], settings[3], settings[4], consistentPotential ? 1.0 : -1.0);
+        ewaldInitFactors(tabs[k]);
+        pot[k] = 0;
+    }
+    if (tabs[0].numEwaldShells == 0) return RYO_OK;
+    // blocks of every system, then the chunks of each system's block potentials (ewald()'s first stage) and the
+    // ranges of each system's chunk sums (its second stage)
+    std::vector<EwaldBatchBlock> blk;
+    std::vector<uint32_t> chunkOff{0}, partOff{0};
+    for (int k = 0; k < K; ++k)
+    {
+        const uint32_t nb = last[k] > first[k] ? iceil(last[k] - first[k], 256) : 0;
+        const uint32_t b0 = uint32_t(blk.size());
+        for (uint32_t j = 0; j < nb; ++j)
+            blk.push_back(EwaldBatchBlock{k, first[k], last[k], j, G[k]});
+        const uint32_t chunk = std::max(4096u, iceil(nb, 256));
+        for (uint32_t c = 0; c < nb; c += chunk)
+            chunkOff.push_back(b0 + std::min(nb, c + chunk));
+        partOff.push_back(uint32_t(chunkOff.size() - 1));
+    }
+    const size_t B = blk.size(), C = chunkOff.size() - 1;
+    if (B == 0) return RYO_OK;
+    auto al = [](size_t v) { return (v + 255) & ~size_t(255); };
+    const size_t oTab = 0, oBlk = al(K * sizeof(EwaldTable)), oPot = oBlk + al(B * sizeof(EwaldBatchBlock)),
+                 oPart = oPot + al(B * 8), oSeg = oPart + al(C * 8), oCo = oSeg + al(size_t(K) * 8),
+                 oPo = oCo + al((C + 1) * 4), need = oPo + al((size_t(K) + 1) * 4);
+    auto* buf = static_cast<char*>(ctx->getScratch(kSlotEwald, need));
+    if (!buf) return RYO_ERR_HIP;
+    auto* dTab  = reinterpret_cast<EwaldTable*>(buf + oTab);
+    auto* dBlk  = reinterpret_cast<EwaldBatchBlock*>(buf + oBlk);
+    auto* dPot  = reinterpret_cast<double*>(buf + oPot);
+    auto* dPart = reinterpret_cast<double*>(buf + oPart);
+    auto* dSeg  = reinterpret_cast<double*>(buf + oSeg);
+    auto* dCo   = reinterpret_cast<uint32_t*>(buf + oCo);
+    auto* dPo   = reinterpret_cast<uint32_t*>(buf + oPo);
+    hipStream_t s = ctx->stream;
+    RYO_HIP(ctx, hipMemcpyAsync(dTab, tabs.data(), K * sizeof(EwaldTable), hipMemcpyHostToDevice, s));
+    RYO_HIP(ctx, hipMemcpyAsync(dBlk, blk.data(), B * sizeof(EwaldBatchBlock), hipMemcpyHostToDevice, s));
+    RYO_HIP(ctx, hipMemcpyAsync(dCo, chunkOff.data(), chunkOff.size() * 4, hipMemcpyHostToDevice, s));
+    RYO_HIP(ctx, hipMemcpyAsync(dPo, partOff.data(), partOff.size() * 4, hipMemcpyHostToDevice, s));
+    const int w     = 2 * tabs[0].numEwaldShells + 1;
+    const bool rows = w * w <= kMaxRows, prod = tabs[0].productExp != 0;
+    const unsigned nb = unsigned(B);
+    if (rows && prod) ewaldBatchKernel<true, true><<<nb, 256, 0, s>>>(dTab, dBlk, x, y, z, m, ugrav, ax, ay, az, dPot);
+    else if (rows) ewaldBatchKernel<true, false><<<nb, 256, 0, s>>>(dTab, dBlk, x, y, z, m, ugrav, ax, ay, az, dPot);
+    else if (prod) ewaldBatchKernel<false, true><<<nb, 256, 0, s>>>(dTab, dBlk, x, y, z, m, ugrav, ax, ay, az, dPot);
+    else ewaldBatchKernel<false, false><<<nb, 256, 0, s>>>(dTab, dBlk, x, y, z, m, ugrav, ax, ay, az, dPot);
+    segmentSumKernel<<<unsigned(C), 256, 0, s>>>(dPot, dCo, dPart);
+    segmentSumKernel<<<unsigned(K), 256, 0, s>>>(dPart, dPo, dSeg);
+    RYO_LAUNCH_CHECK(ctx);
+    std::vector<double> U(K);
+    RYO_HIP(ctx, hipMemcpyAsync(U.data(), dSeg, K * sizeof(double), hipMemcpyDeviceToHost, s));
+    RYO_HIP(ctx, hipStreamSynchronize(s)); // also keeps the host tables alive until the copies are done
+    for (int k = 0; k < K; ++k)
+        pot[k] = 0.5 * double(G[k]) * U[k];
     return RYO_OK;
 }
 

@@ -425,8 +425,9 @@ struct TravBatch
 {
     BatchView bv;
     const int32_t* gSys;
-    const float* G;         // per system
-    const uint32_t* abort;  // nonzero: the trees were not built (arrays too small), traverse nothing
+    const float* G;           // per system
+    const uint32_t* abort;    // nonzero: the trees were not built (arrays too small), traverse nothing
+    const SysImages* images;  // per system (null: every system open)
 };
 
 // 64 * Waves threads; the second bound is the minimum number of waves per SIMD: 6 (<= 80 VGPRs; more would spill)
@@ -486,6 +487,8 @@ __global__ __launch_bounds__(64 * Waves, kWavesPerCU / 4) void traverseKernelT(T
 
         TreeRef tr{a.child, a.i2l, a.layout, a.centers, a.mp, a.xs, a.ys, a.zs, a.ms, a.hs};
         float Gs = a.G;
+        int S    = a.numShells;
+        float bx = a.boxL[0], by = a.boxL[1], bz = a.boxL[2];
         if constexpr (Batched)
         {
             const int s    = tb.gSys[g];
@@ -493,6 +496,11 @@ __global__ __launch_bounds__(64 * Waves, kWavesPerCU / 4) void traverseKernelT(T
             tr = TreeRef{a.child + no, a.i2l + no, a.layout + tb.bv.LO[s], a.centers + no, a.mp + 2 * no,
                          a.xs + p, a.ys + p, a.zs + p, a.ms + p, a.hs + p};
             Gs = tb.G[s];
+            if (tb.images)
+            {
+                const SysImages im = tb.images[s];
+                S = im.shells, bx = im.L[0], by = im.L[1], bz = im.L[2];
+            }
         }
         const uint32_t b = a.gStart[g], e = a.gEnd[g];
         const bool valid = b + lane < e;
@@ -507,12 +515,11 @@ __global__ __launch_bounds__(64 * Waves, kWavesPerCU / 4) void traverseKernelT(T
         Acc acc{0.f, 0.f, 0.f, 0.f};
         unsigned nM2P = 0, nP2P = 0, maxStack = 0;
         bool overflow = false, corrupt = false;
-        const int S = a.numShells;
         for (int iz = -S; iz <= S && !overflow; ++iz)
             for (int iy = -S; iy <= S && !overflow; ++iy)
                 for (int ix = -S; ix <= S && !overflow; ++ix)
                 {
-                    float sx = ix * a.boxL[0], sy = iy * a.boxL[1], sz = iz * a.boxL[2];
+                    float sx = ix * bx, sy = iy * by, sz = iz * bz;
                     Target ts{t.x + sx, t.y + sy, t.z + sz, t.h, t.hij2u};
                     WalkOut o = uniformH
                                     ? walk<true, MDQ>(a, tr, stack, ts, tcx + sx, tcy + sy, tcz + sz, tsx, tsy, tsz,
@@ -991,7 +998,8 @@ int batchTraverse(Context* ctx, int numGroups, const uint32_t* gStart, const uin
                   const BatchView& bv, const float* G, const float* x, const float* y, const float* z,
                   const float* m, const float* h, size_t n, const int32_t* child, const int32_t* i2l,
                   const uint32_t* layout, const float* centers, const float* mp, float* p, float* ax, float* ay,
-                  float* az, double* sysPot, int initNode, void* work, const uint32_t* abort, int mode)
+                  float* az, double* sysPot, int initNode, void* work, const uint32_t* abort, int mode,
+                  const SysImages* images)
 {
     TravArgs a{};
     a.numGroups = numGroups;
@@ -1005,7 +1013,7 @@ int batchTraverse(Context* ctx, int numGroups, const uint32_t* gStart, const uin
     a.p = p, a.ax = ax, a.ay = ay, a.az = az;
     a.hCount = n;
     batchShape(ctx, a, work);
-    TravBatch tb{bv, gSys, G, abort};
+    TravBatch tb{bv, gSys, G, abort, images};
     int rc = RYO_OK;
     if (mode != 2) rc = traverseBatchLaunch(ctx, a, tb, sysPot, mode == 0);
     if (!rc && mode != 1) rc = traverseBatchFinish(ctx, a, tb, sysPot, mode == 0);

@@ -123,6 +123,87 @@ def test_batch_matches_oracle(ctx, graph):
     b.close()
 
 
+def _mixed_periodic():
+    """periodic cubes of several sizes and offsets with open systems between them"""
+    return [W.uniform_cube(1000, seed=201, lo=0.0, hi=1.0, periodic=True), W.plummer(3000, seed=202),
+            W.uniform_cube(4097, seed=203, lo=-0.5, hi=0.5, periodic=True),
+            W.uniform_cube(63, seed=204, lo=2.0, hi=4.0, periodic=True), W.uniform_cube(9000, seed=205),
+            W.uniform_cube(20000, seed=206, lo=0.0, hi=1.0, periodic=True)]
+
+
+@pytest.mark.parametrize("ewald", [0, 1, 2])
+def test_periodic_batch_equals_systems_alone(ctx, ewald):
+    """periodic systems in a batch: images of each system's own box (1 shell) and, with ewald 1 or 2, the FP64
+    Ewald correction from its root multipole; every system bit-identical to the same system solved alone as a
+    resident system (periodic or open), the potentials to 1e-12"""
+    systems = _mixed_periodic()
+    thetas = np.array([0.5, 0.6, 0.4, 0.5, 0.7, 0.5], np.float32)
+    Gs = np.array([1.0, 0.5, 2.0, 1.0, 1.5, 1.0], np.float32)
+    b = R.Batch(ctx, sum(s.n for s in systems), len(systems))
+    b.load(systems)
+    prm = R.default_params(num_shells=1, ewald=ewald)
+    pots = b.solve(prm, theta=thetas, G=Gs)
+    off = b.offsets()
+    (x, y, z, m, h, p, ax, ay, az), order = b.results()
+    arrs = [host(t) for t in (p, ax, ay, az)]
+    order = host(order)
+    for s, sysd in enumerate(systems):
+        print(f"system {s}: n={sysd.n} periodic={sysd.periodic} ewald={ewald}", flush=True)
+        sysm = R.System(ctx, sysd.n)
+        dv = [dev(a) for a in sysd.arrays()]
+        sysm.load(*dv, sysd.limits, periodic=sysd.periodic)
+        prm_s = R.default_params(theta=float(thetas[s]), G=float(Gs[s]), num_shells=1, ewald=ewald)
+        sysm.build(prm_s)
+        pot = sysm.gravity(prm_s)
+        res = [host(t) for t in sysm.results()]
+        sysm.close()
+        p0, p1 = int(off["P"][s]), int(off["P"][s + 1])
+        assert np.array_equal(order[p0:p1] - p0, res[4]), s
+        for k in range(4):
+            assert np.array_equal(arrs[k][p0:p1].view(np.int32), res[k].view(np.int32)), (s, k)
+        assert abs(pots[s] - pot) <= 1e-12 * abs(pot), (s, pots[s], pot)
+    with pytest.raises(R.RyoError):
+        b.solve(prm, theta=thetas, G=Gs, graph=True)  # the captured graph takes open systems only
+    b.close()
+
+
+def test_periodic_batch_matches_oracle(ctx):
+    """the periodic systems of a mixed batch against the oracle traversal over the 27 images plus the oracle's
+    FP64 Ewald correction from the oracle's FP64 root multipole (relative to the RMS acceleration: the periodic
+    net force is a small residual of the image sum and the correction); open systems at p99 <= 1e-5"""
+    from gpu_util import oracle_tree, rel_err
+
+    import oracle as O
+
+    systems = _mixed_periodic()
+    b = R.Batch(ctx, sum(s.n for s in systems), len(systems))
+    b.load(systems)
+    pots = b.solve(R.default_params(num_shells=1, ewald=1))
+    off = b.offsets()
+    (x, y, z, m, h, p, ax, ay, az), order = b.results()
+    ax, ay, az, order = host(ax), host(ay), host(az), host(order)
+    for s, sysd in enumerate(systems):
+        t = oracle_tree(sysd)
+        p0, p1 = int(off["P"][s]), int(off["P"][s + 1])
+        assert np.array_equal(order[p0:p1] - p0, t["perm"]), s
+        args = (t["x"], t["y"], t["z"], t["h"], t["m"])
+        otot, _, oax, oay, oaz, _ = O.compute_gravity(t["leaves"], t["layout"], t["centers"], t["mp"], *args,
+                                                      sysd.limits, group_size=64, num_shells=1 if sysd.periodic else 0)
+        if not sysd.periodic:
+            e = np.sort(rel_err(ax[p0:p1], ay[p0:p1], az[p0:p1], oax, oay, oaz))
+            assert e[int(0.99 * (e.size - 1))] <= 1e-5, (s, e[int(0.99 * (e.size - 1))])
+            continue
+        lim = np.asarray(sysd.limits, np.float64)
+        c64, q64 = O.upsweep(t["x"], t["y"], t["z"], t["m"], t["leaves"], t["layout"], lim, 2.0, precision="f64")
+        etot, _, eax, eay, eaz = O.ewald(q64[0], c64[0, :3], t["x"], t["y"], t["z"], t["m"], float(lim[1] - lim[0]))
+        rx, ry, rz = oax + eax, oay + eay, oaz + eaz
+        rms = np.sqrt(np.mean(oax ** 2 + oay ** 2 + oaz ** 2))
+        d = np.sqrt((ax[p0:p1] - rx) ** 2 + (ay[p0:p1] - ry) ** 2 + (az[p0:p1] - rz) ** 2) / rms
+        assert np.quantile(d, 0.99) <= 1e-4, (s, np.quantile(d, 0.99))
+        assert abs(pots[s] - (otot + etot)) <= 1e-5 * (abs(otot) + abs(etot)), (s, pots[s], otot + etot)
+    b.close()
+
+
 def test_batch_reuse_and_capacity(ctx):
     """a batch object serves batches of different composition; loading beyond its capacity is an error"""
     b = R.Batch(ctx, 30000, 8)
