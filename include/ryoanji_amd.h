@@ -516,6 +516,9 @@ typedef struct
     int group_size;       /* 0: 64 */
     double arrival_ms;    /* relative to the start of ryo_server_run; requests sorted by arrival */
     int accuracy_targets; /* > 0: check the final accelerations against the FP64 direct sum on this many targets */
+    int num_shells;       /* periodic requests: replica shells the traversal walks (ryo_params.num_shells) */
+    int ewald;            /* periodic requests: ryo_params.ewald (0 none, 1 the reference's correction, 2 with
+                             the consistent potential; a cubic box) */
 } ryo_request;
 
 typedef struct
@@ -532,12 +535,13 @@ typedef struct ryo_server ryo_server;
  * a batch lane for small requests (below); requests are admitted while their summed HBM footprint stays within
  * hbm_budget bytes (0: unlimited). */
 int ryo_server_create(int slots, size_t hbm_budget, size_t max_n, ryo_server** server);
-/* The batch lane: open-boundary requests of at most max_n particles (default 2^16; 0 turns the lane off) with the
- * default bucket and group size are packed one after the other into one ryo_batch (capacity particles and
- * max_systems systems in all, defaults 2^22 and 1024) and advanced in lockstep, one batched solve per leapfrog step
- * of every member; requests join at the next step and leave when their steps are done. A member's trajectory is the
- * one it has on a slot (same kick-drift arithmetic, box refit and solve). Results of lane members report slot =
- * `slots`. Set before ryo_server_run. */
+/* The batch lane: requests of at most max_n particles (default 2^16; 0 turns the lane off) with the default bucket
+ * and group size, open or periodic (a periodic one with its own image shells and Ewald mode), are packed one after
+ * the other into one ryo_batch (capacity particles and max_systems systems in all, defaults 2^22 and 1024) and
+ * advanced in lockstep, one batched solve per leapfrog step of every member; requests join at the next step and
+ * leave when their steps are done. A member's trajectory is the one it has on a slot (same kick-drift arithmetic,
+ * periodic wrap or box refit, and solve). Results of lane members report slot = `slots`. Set before
+ * ryo_server_run. */
 int ryo_server_set_batching(ryo_server* server, size_t max_n, size_t capacity, int max_systems);
 int ryo_server_destroy(ryo_server* server);
 /* estimated HBM bytes of a resident system of n particles (the admission weight) */

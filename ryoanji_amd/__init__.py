@@ -65,6 +65,8 @@ class Request(C.Structure):
         ("group_size", C.c_int),
         ("arrival_ms", C.c_double),
         ("accuracy_targets", C.c_int),
+        ("num_shells", C.c_int),
+        ("ewald", C.c_int),
     ]
 
 

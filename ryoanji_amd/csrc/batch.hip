@@ -58,7 +58,7 @@ int batchTraverse(Context* ctx, int numGroups, const uint32_t* gStart, const uin
                   const SysImages* images);
 size_t batchTraverseWorkBytes(size_t numGroups);
 int ewaldBatch(Context* ctx, int K, const uint32_t* first, const uint32_t* last, const float* lim, const double* roots,
-               const float* G, const double* settings, bool consistentPotential, const float* x, const float* y,
+               const float* G, const double* settings, const int* mode, const float* x, const float* y,
                const float* z, const float* m, float* ugrav, float* ax, float* ay, float* az, double* pot);
 
 int batchUpsweepEntry(Context* ctx, const float* x, const float* y, const float* z, const float* m,
@@ -111,6 +111,7 @@ struct ryo_batch
     std::vector<uint32_t> hP, hTS, hLO, hNO, hNIO, hPO, hGO;
     std::vector<float> limits;
     std::vector<int> periodic; // per system (ryo_batch_set_periodic; all open after a load)
+    std::vector<int> sysShells, sysEwald; // per-system image shells and Ewald modes (empty: prm's values)
     std::vector<float> hG;     // per system G of the current solve
 
     // particles: loaded (packed, original order) and sorted (x, y, z, m, h, original index)
@@ -225,7 +226,9 @@ int uploadTables(ryo_batch* b, const ryo_params* prm, const float* theta, const 
         for (int s = 0; s < S; ++s)
         {
             const float* l = &b->limits[6 * s];
-            im[s]          = SysImages{b->periodic[s] ? prm->num_shells : 0, {l[1] - l[0], l[3] - l[2], l[5] - l[4]}};
+            const int sh  = b->sysShells.empty() ? prm->num_shells : b->sysShells[s];
+            if (sh < 0) return setError(ctx, RYO_ERR_ARG, "batch: num_shells of system %d", s);
+            im[s] = SysImages{b->periodic[s] ? sh : 0, {l[1] - l[0], l[3] - l[2], l[5] - l[4]}};
         }
         int rc = b->images.ensure(ctx, S * sizeof(SysImages));
         if (rc) return rc;
@@ -411,10 +414,14 @@ int batchEwald(ryo_batch* b, const ryo_params* prm, std::vector<double>& ewPot)
     const int S  = b->S;
     ewPot.assign(S, 0.0);
     std::vector<int32_t> sys;
+    std::vector<int> mode;
     for (int s = 0; s < S; ++s)
-        if (b->periodic[s]) sys.push_back(s);
+    {
+        const int e = b->sysEwald.empty() ? prm->ewald : b->sysEwald[s];
+        if (b->periodic[s] && e != 0) sys.push_back(s), mode.push_back(e);
+    }
     const int K = int(sys.size());
-    if (K == 0 || prm->ewald == 0) return RYO_OK;
+    if (K == 0) return RYO_OK;
     int rc = b->roots.ensure(ctx, size_t(K) * (12 * 8 + 4));
     if (rc) return rc;
     auto* dRoots = b->roots.as<double>();
@@ -437,7 +444,7 @@ int batchEwald(ryo_batch* b, const ryo_params* prm, std::vector<double>& ewPot)
     }
     std::vector<double> pot(K);
     rc = ewaldBatch(ctx, K, first.data(), last.data(), lim.data(), roots.data(), G.data(), prm->ewald_settings,
-                    prm->ewald == 2, b->srt[0].as<float>(), b->srt[1].as<float>(), b->srt[2].as<float>(),
+                    mode.data(), b->srt[0].as<float>(), b->srt[1].as<float>(), b->srt[2].as<float>(),
                     b->srt[3].as<float>(), b->p.as<float>(), b->ax.as<float>(), b->ay.as<float>(), b->az.as<float>(),
                     pot.data());
     if (rc) return rc;
@@ -544,6 +551,25 @@ int capture(ryo_batch* b, const Shape& sh)
 }
 } // namespace
 
+namespace ryo
+{
+/* per-system boundaries for a caller that mixes them in one batch (the serving lane): periodic flags, image
+ * shells and Ewald modes (0 none, 1 the reference's potential, 2 the consistent one) of the loaded systems;
+ * they replace ryo_batch_set_periodic's flags and prm's num_shells and ewald until the next load */
+int batchSetBoundaries(ryo_batch* b, const int* periodic, const int* shells, const int* ewald)
+{
+    if (!b || !periodic || !shells || !ewald || b->S < 1) return RYO_ERR_ARG;
+    for (int s = 0; s < b->S; ++s)
+        if (shells[s] < 0 || ewald[s] < 0 || ewald[s] > 2)
+            return setError(b->ctx, RYO_ERR_ARG, "batch: boundaries of system %d", s);
+    b->periodic.assign(periodic, periodic + b->S);
+    b->sysShells.assign(shells, shells + b->S);
+    b->sysEwald.assign(ewald, ewald + b->S);
+    b->solved = false;
+    return RYO_OK;
+}
+} // namespace ryo
+
 extern "C" {
 
 int ryo_batch_create(ryo_ctx* h, size_t max_particles, int max_systems, ryo_batch** out)
@@ -618,6 +644,7 @@ int ryo_batch_load(ryo_batch* b, int num_systems, const uint32_t* counts, const 
     b->hP = prefix(std::vector<uint32_t>(counts, counts + num_systems));
     b->limits.assign(limits, limits + 6 * num_systems);
     b->periodic.assign(num_systems, 0);
+    b->sysShells.clear(), b->sysEwald.clear();
     const float* src[5] = {x, y, z, m, hh};
     for (int k = 0; k < 5; ++k)
         RYO_HIP(ctx, hipMemcpyAsync(b->in[k].p, src[k], N * 4, hipMemcpyDefault, ctx->stream));
@@ -630,6 +657,7 @@ int ryo_batch_set_periodic(ryo_batch* b, const int* periodic)
     if (!b || !periodic || b->S < 1) return RYO_ERR_ARG;
     for (int s = 0; s < b->S; ++s)
         b->periodic[s] = periodic[s] != 0;
+    b->sysShells.clear(), b->sysEwald.clear();
     b->solved = false;
     return RYO_OK;
 }

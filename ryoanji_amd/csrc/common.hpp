@@ -135,6 +135,13 @@ struct SysImages
     float L[3];
 };
 
+// the box of one packed system that wraps periodically (periodic 0: open, positions not wrapped)
+struct PackedWrap
+{
+    float lo[3], L[3];
+    int periodic, pad;
+};
+
 // the segment holding element i: the largest s with off[s] <= i (empty segments are skipped)
 __device__ inline int segmentOf(const uint32_t* off, int S, uint32_t i)
 {

@@ -620,11 +620,12 @@ int ewald(Context* ctx, const double* Mroot, const double* center, uint32_t firs
 
 /* The Ewald correction of K periodic systems of a batch (packed arrays x, y, z, m and results): system k owns
  * particles [first[k], last[k]), a cubic box lim[6 k ..], the FP64 root multipole roots[12 k ..] = {Mroot[8],
- * center[3], -} and G[k]. One launch for all systems (each block within one system, with its table); the block
+ * center[3], -}, G[k] and the Ewald mode mode[k] (1: the reference's k-space potential, 2: the consistent one).
+ * One launch for all systems (each block within one system, with its table); the block
  * potentials of a system are summed in the same two fixed-order stages as for a system corrected alone, so
  * pot[k] (host) = 0.5 G_k sum(m phi) and every result equal, bit for bit, those of ewald() on that system. */
 int ewaldBatch(Context* ctx, int K, const uint32_t* first, const uint32_t* last, const float* lim, const double* roots,
-               const float* G, const double* settings, bool consistentPotential, const float* x, const float* y,
+               const float* G, const double* settings, const int* mode, const float* x, const float* y,
                const float* z, const float* m, float* ugrav, float* ax, float* ay, float* az, double* pot)
 {
     if (K <= 0) return RYO_OK;
@@ -639,7 +640,7 @@ int ewaldBatch(Context* ctx, int K, const uint32_t* first, const uint32_t* last,
         if (!(lx == ly && ly == lz))
             return setError(ctx, RYO_ERR_UNSUPPORTED, "Ewald gravity requires cubic boxes (periodic system %d)", k);
         ewaldInitTable(tabs[k], roots + 12 * k, roots + 12 * k + 8, int(settings[0]), double(lx), settings[1],
-                       settings[2], settings[3], settings[4], consistentPotential ? 1.0 : -1.0);
+                       settings[2], settings[3], settings[4], mode[k] == 2 ? 1.0 : -1.0);
         ewaldInitFactors(tabs[k]);
         pot[k] = 0;
     }

@@ -45,20 +45,30 @@ __global__ __launch_bounds__(256) void kickDriftKernel(float* __restrict__ x, fl
         kickDriftOne(i, x, y, z, vx, vy, vz, ax, ay, az, cKick, cDrift, periodic, lo, L);
 }
 
-// the same update for systems packed one after the other (open boundaries): system s owns particles [P[s], P[s+1])
-// and updates with its own coefficients coef[s] = {c_kick, c_drift}
+// the same update for systems packed one after the other: system s owns particles [P[s], P[s+1]) and updates with
+// its own coefficients coef[s] = {c_kick, c_drift} and, when wrap is given, its own periodic box wrap[s]
 __global__ __launch_bounds__(256) void packedKickDriftKernel(float* __restrict__ x, float* __restrict__ y,
                                                             float* __restrict__ z, float* __restrict__ vx,
                                                             float* __restrict__ vy, float* __restrict__ vz,
                                                             const float* __restrict__ ax, const float* __restrict__ ay,
                                                             const float* __restrict__ az, const uint32_t* __restrict__ P,
-                                                            int S, const float2* __restrict__ coef)
+                                                            int S, const float2* __restrict__ coef,
+                                                            const PackedWrap* __restrict__ wrap)
 {
     const size_t n = P[S];
     for (size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += size_t(gridDim.x) * blockDim.x)
     {
-        const float2 c = coef[segmentOf(P, S, uint32_t(i))];
-        kickDriftOne(i, x, y, z, vx, vy, vz, ax, ay, az, c.x, c.y, 0, make_float3(0, 0, 0), make_float3(1, 1, 1));
+        const int s    = segmentOf(P, S, uint32_t(i));
+        const float2 c = coef[s];
+        int periodic   = 0;
+        float3 lo = make_float3(0, 0, 0), L = make_float3(1, 1, 1);
+        if (wrap)
+        {
+            const PackedWrap w = wrap[s];
+            periodic = w.periodic;
+            lo = make_float3(w.lo[0], w.lo[1], w.lo[2]), L = make_float3(w.L[0], w.L[1], w.L[2]);
+        }
+        kickDriftOne(i, x, y, z, vx, vy, vz, ax, ay, az, c.x, c.y, periodic, lo, L);
     }
 }
 
@@ -177,13 +187,15 @@ void cubicBox(const float* mn, const float* mx, float* limits)
         limits[2 * d + 1] = limits[2 * d] + Lx;
 }
 
-// v += coef[s].x * a, then x += coef[s].y * v for the packed systems of the offset table P (device, S + 1 entries)
+// v += coef[s].x * a, then x += coef[s].y * v for the packed systems of the offset table P (device, S + 1 entries);
+// wrap (device, S entries, may be null): the periodic systems' boxes, whose positions wrap as a resident system's
 int packedKickDrift(Context* ctx, float* x, float* y, float* z, float* vx, float* vy, float* vz, const float* ax,
-                    const float* ay, const float* az, size_t n, const uint32_t* P, int S, const float2* coef)
+                    const float* ay, const float* az, size_t n, const uint32_t* P, int S, const float2* coef,
+                    const PackedWrap* wrap)
 {
     if (n == 0 || S < 1) return RYO_OK;
     unsigned blocks = std::min(iceil(n, 256), unsigned(ctx->numCUs * 8));
-    packedKickDriftKernel<<<blocks, 256, 0, ctx->stream>>>(x, y, z, vx, vy, vz, ax, ay, az, P, S, coef);
+    packedKickDriftKernel<<<blocks, 256, 0, ctx->stream>>>(x, y, z, vx, vy, vz, ax, ay, az, P, S, coef, wrap);
     RYO_LAUNCH_CHECK(ctx);
     return RYO_OK;
 }

@@ -5,7 +5,7 @@
 // fit in arrival order). Two kinds of place:
 //   * a slot: a HIP stream, a context and a resident system of its own, for requests of any size; admitted
 //     requests on different slots run concurrently and their kernels overlap on the device;
-//   * the batch lane, for small open-boundary requests: one stream, context and ryo_batch whose members are
+//   * the batch lane, for small requests (open or periodic): one stream, context and ryo_batch whose members are
 //     packed one after the other and advanced in lockstep, one batched solve (every stage one launch sequence for
 //     all members, per-system offset tables) per leapfrog step of all of them. Requests join at the next step
 //     boundary and leave when their own steps are done, so the batch composition changes continuously.
@@ -38,7 +38,9 @@ size_t footprint(size_t n)
 namespace ryo
 {
 int packedKickDrift(Context* ctx, float* x, float* y, float* z, float* vx, float* vy, float* vz, const float* ax,
-                    const float* ay, const float* az, size_t n, const uint32_t* P, int S, const float2* coef);
+                    const float* ay, const float* az, size_t n, const uint32_t* P, int S, const float2* coef,
+                    const PackedWrap* wrap);
+int batchSetBoundaries(ryo_batch* b, const int* periodic, const int* shells, const int* ewald);
 int packedBoxes(Context* ctx, const float* x, const float* y, const float* z, const uint32_t* P, int S, float* out);
 void cubicBox(const float* mn, const float* mx, float* limits);
 int sequence(Context* ctx, int32_t* out, size_t n, int32_t start);
@@ -81,6 +83,7 @@ struct Lane
     uint32_t* dP   = nullptr; // member offsets (S + 1)
     float2* dCoef  = nullptr; // per-member kick / drift coefficients
     float* dBox    = nullptr; // per-member extents [S][6]
+    PackedWrap* dWrap = nullptr; // per-member periodic boxes
 
     int create(int device, size_t cn, int cs)
     {
@@ -93,7 +96,8 @@ struct Lane
                 if (hipMalloc(&p, capN * sizeof(float)) != hipSuccess) return RYO_ERR_HIP;
         if (hipMalloc(&dP, (capS + 1) * sizeof(uint32_t)) != hipSuccess ||
             hipMalloc(&dCoef, capS * sizeof(float2)) != hipSuccess ||
-            hipMalloc(&dBox, capS * 6 * sizeof(float)) != hipSuccess)
+            hipMalloc(&dBox, capS * 6 * sizeof(float)) != hipSuccess ||
+            hipMalloc(&dWrap, capS * sizeof(PackedWrap)) != hipSuccess)
             return RYO_ERR_HIP;
         return RYO_OK;
     }
@@ -103,9 +107,9 @@ struct Lane
         for (auto& set : a)
             for (auto& p : set)
                 if (p) hipFree(p), p = nullptr;
-        for (void* p : {(void*)dP, (void*)dCoef, (void*)dBox})
+        for (void* p : {(void*)dP, (void*)dCoef, (void*)dBox, (void*)dWrap})
             if (p) hipFree(p);
-        dP = nullptr, dCoef = nullptr, dBox = nullptr;
+        dP = nullptr, dCoef = nullptr, dBox = nullptr, dWrap = nullptr;
         if (batch) ryo_batch_destroy(batch), batch = nullptr;
         if (ctx) ryo_ctx_destroy(ctx), ctx = nullptr;
         if (stream) hipStreamDestroy(stream), stream = nullptr;
@@ -238,9 +242,13 @@ int ryo_server_run(ryo_server* s, int num, const ryo_request* req, ryo_result* r
     for (int i = 0; i < num; ++i)
         if (req[i].n == 0 || req[i].n > s->maxN || req[i].steps < 0) return RYO_ERR_ARG;
 
-    // small open-boundary requests with the default bucket and group size go to the batch lane
+    // small requests with the default bucket and group size go to the batch lane (periodic ones with valid
+    // settings and, for the Ewald correction, a cubic box; others take a slot, which reports their error)
     auto laneable = [&](const ryo_request& r) {
-        return s->laneMaxN > 0 && r.n <= s->laneMaxN && r.n <= s->laneCapN && !r.periodic &&
+        const bool cubic = r.limits[1] - r.limits[0] == r.limits[3] - r.limits[2] &&
+                           r.limits[3] - r.limits[2] == r.limits[5] - r.limits[4];
+        const bool bounds = !r.periodic || (r.num_shells >= 0 && r.ewald >= 0 && r.ewald <= 2 && (!r.ewald || cubic));
+        return s->laneMaxN > 0 && r.n <= s->laneMaxN && r.n <= s->laneCapN && bounds &&
                (r.bucket_size == 0 || r.bucket_size == 64) && (r.group_size == 0 || r.group_size == 64);
     };
     bool useLane = false;
@@ -301,6 +309,8 @@ int ryo_server_run(ryo_server* s, int num, const ryo_request* req, ryo_result* r
             p.G     = r.G;
             p.bucket_size = r.bucket_size ? r.bucket_size : 64;
             p.group_size  = r.group_size ? r.group_size : 64;
+            p.num_shells  = r.periodic ? r.num_shells : 0;
+            p.ewald       = r.periodic ? r.ewald : 0;
             double pot = 0;
             int rc = ryo_system_load(slot.sys, r.n, r.x, r.y, r.z, r.m, r.h, r.limits, r.periodic);
             if (!rc) rc = ryo_system_leapfrog(slot.sys, &p, r.dt, r.steps, &pot);
@@ -374,6 +384,9 @@ int ryo_server_run(ryo_server* s, int num, const ryo_request* req, ryo_result* r
             std::vector<float> hBox(6 * size_t(S)), limits(6 * size_t(S)), theta(S), G(S);
             std::vector<uint32_t> counts(S);
             std::vector<double> pots(S, 0.0);
+            std::vector<int> per(S), shells(S), ewm(S);
+            std::vector<PackedWrap> wrap(S);
+            bool anyPeriodic = false;
             for (int k = 0; k < S; ++k)
             {
                 const ryo_request& r = req[mem[k].req];
@@ -381,21 +394,30 @@ int ryo_server_run(ryo_server* s, int num, const ryo_request* req, ryo_result* r
                 counts[k] = uint32_t(mem[k].n);
                 coef[k]   = mem[k].solved ? make_float2(0.5f * r.dt, r.dt) : make_float2(0.0f, 0.0f);
                 theta[k] = r.theta, G[k] = r.G;
+                // a periodic member keeps its box and wraps into it, as a resident periodic system does
+                per[k] = r.periodic != 0, shells[k] = per[k] ? r.num_shells : 0, ewm[k] = per[k] ? r.ewald : 0;
+                anyPeriodic |= per[k] != 0;
+                const float* l = r.limits;
+                wrap[k] = PackedWrap{{l[0], l[2], l[4]}, {l[1] - l[0], l[3] - l[2], l[5] - l[4]}, per[k], 0};
             }
-            // kick and drift of the members past their initial solve, then their boxes refit around them
+            // kick and drift of the members past their initial solve, then the open members' boxes refit around
+            // them
             if (!rc && (hipMemcpyAsync(L.dP, hP.data(), hP.size() * 4, hipMemcpyHostToDevice, st) != hipSuccess ||
                         hipMemcpyAsync(L.dCoef, coef.data(), coef.size() * sizeof(float2), hipMemcpyHostToDevice,
-                                       st) != hipSuccess))
+                                       st) != hipSuccess ||
+                        (anyPeriodic && hipMemcpyAsync(L.dWrap, wrap.data(), wrap.size() * sizeof(PackedWrap),
+                                                       hipMemcpyHostToDevice, st) != hipSuccess)))
                 rc = RYO_ERR_HIP;
+            const PackedWrap* dWrap = anyPeriodic ? L.dWrap : nullptr;
             if (!rc) rc = packedKickDrift(c, A[kLX], A[kLY], A[kLZ], A[kLVX], A[kLVY], A[kLVZ], A[kLAX], A[kLAY],
-                                          A[kLAZ], N, L.dP, S, L.dCoef);
+                                          A[kLAZ], N, L.dP, S, L.dCoef, dWrap);
             if (!rc) rc = packedBoxes(c, A[kLX], A[kLY], A[kLZ], L.dP, S, L.dBox);
             if (!rc && (hipMemcpyAsync(hBox.data(), L.dBox, hBox.size() * 4, hipMemcpyDeviceToHost, st) != hipSuccess ||
                         hipStreamSynchronize(st) != hipSuccess))
                 rc = RYO_ERR_HIP;
             for (int k = 0; k < S; ++k)
             {
-                if (mem[k].solved) cubicBox(&hBox[6 * k], &hBox[6 * k + 3], mem[k].lim);
+                if (mem[k].solved && !per[k]) cubicBox(&hBox[6 * k], &hBox[6 * k + 3], mem[k].lim);
                 std::memcpy(&limits[6 * k], mem[k].lim, sizeof(mem[k].lim));
             }
             // one batched solve of every member
@@ -403,6 +425,7 @@ int ryo_server_run(ryo_server* s, int num, const ryo_request* req, ryo_result* r
             ryo_default_params(&prm);
             if (!rc) rc = ryo_batch_load(L.batch, S, counts.data(), A[kLX], A[kLY], A[kLZ], A[kLM], A[kLH],
                                          limits.data());
+            if (!rc && anyPeriodic) rc = batchSetBoundaries(L.batch, per.data(), shells.data(), ewm.data());
             if (!rc) rc = ryo_batch_solve(L.batch, &prm, theta.data(), G.data(), pots.data());
             float* out[9];
             int32_t* order = nullptr;
@@ -429,7 +452,7 @@ int ryo_server_run(ryo_server* s, int num, const ryo_request* req, ryo_result* r
                                       st) != hipSuccess)
                 rc = RYO_ERR_HIP;
             if (!rc) rc = packedKickDrift(c, A[kLX], A[kLY], A[kLZ], A[kLVX], A[kLVY], A[kLVZ], A[kLAX], A[kLAY],
-                                          A[kLAZ], N, L.dP, S, L.dCoef);
+                                          A[kLAZ], N, L.dP, S, L.dCoef, dWrap);
             // members whose steps are done leave with their final state
             std::vector<char> finished(S, 0);
             std::vector<std::array<double, 4>> accs(S, std::array<double, 4>{0, 0, 0, 0});
@@ -444,9 +467,16 @@ int ryo_server_run(ryo_server* s, int num, const ryo_request* req, ryo_result* r
                 if (!finished[k] || rc) continue;
                 const size_t o = hP[k];
                 if (r.accuracy_targets > 0)
+                {
+                    // as ryo_system_accuracy: the direct sum over the traversal's images, or the Ewald pair sum
+                    // after an Ewald-corrected solve
+                    const double boxL[3] = {double(m.lim[1] - m.lim[0]), double(m.lim[3] - m.lim[2]),
+                                            double(m.lim[5] - m.lim[4])};
+                    const double ewaldL  = per[k] && ewm[k] ? boxL[0] : 0.0;
                     rc = accuracyStats(c, A[kLX] + o, A[kLY] + o, A[kLZ] + o, A[kLM] + o, A[kLH] + o, m.n, A[kLP] + o,
-                                       A[kLAX] + o, A[kLAY] + o, A[kLAZ] + o, double(r.G), 0, nullptr, 0.0,
+                                       A[kLAX] + o, A[kLAY] + o, A[kLAZ] + o, double(r.G), shells[k], boxL, ewaldL,
                                        r.accuracy_targets, uint64_t(m.req), accs[k].data());
+                }
                 if (!rc && out_state && out_state[m.req])
                 {
                     const int src[7] = {kLX, kLY, kLZ, kLAX, kLAY, kLAZ, kLO};

@@ -63,6 +63,10 @@ class Server:
             r.group_size = 64
             r.arrival_ms = float(params["arrival_ms"][i]) * arrival_scale
             r.accuracy_targets = int(accuracy_targets)
+            # periodic requests: the reference's periodic set-up, 1 replica shell and the Ewald correction, unless
+            # the stream's parameters say otherwise
+            r.num_shells = int(np.broadcast_to(params.get("num_shells", 1), (num,))[i]) if s.periodic else 0
+            r.ewald = int(np.broadcast_to(params.get("ewald", 1), (num,))[i]) if s.periodic else 0
         res = (Result * num)()
         states = None
         if keep_state:

@@ -94,6 +94,56 @@ def test_served_large_request_matches_oracle(gpu):
         _oracle_check(systems[i], out[i], params["theta"][i])
 
 
+def test_periodic_requests_alone_and_in_the_mix(gpu):
+    """periodic requests (1 replica shell + the Ewald correction, wrapped into their boxes) served in the batch
+    lane beside open ones: the same final states as one at a time on one slot, positions inside their boxes, and
+    the final accelerations equal to the oracle traversal over the images plus the oracle's FP64 Ewald correction"""
+    import oracle as O
+    from gpu_util import oracle_tree
+
+    systems = [W.uniform_cube(3000, seed=71, lo=0.0, hi=1.0, periodic=True), W.plummer(5000, seed=72),
+               W.uniform_cube(9000, seed=73, lo=-1.0, hi=1.0, periodic=True), W.plummer(20000, seed=74),
+               W.uniform_cube(1500, seed=75, lo=0.0, hi=2.0, periodic=True), W.plummer(40000, seed=76)]
+    n = len(systems)
+    params = dict(theta=np.array([0.5, 0.5, 0.6, 0.4, 0.5, 0.7], np.float32), steps=np.array([2, 3, 1, 2, 3, 1]),
+                  arrival_ms=np.zeros(n), dt=2e-3, num_shells=1, ewald=1)
+    mixed = S.Server(slots=2, max_n=1 << 16, batch_max_n=1 << 14)
+    out_mix = mixed.run(systems, params, keep_state=True)
+    mixed.close()
+    alone = S.Server(slots=1, max_n=1 << 16, batch_max_n=0)
+    out_alone = alone.run(systems, params, keep_state=True)
+    alone.close()
+    assert all(r["status"] == 0 for r in out_mix + out_alone)
+    assert all(r["slot"] == 2 for r, sd in zip(out_mix, systems) if sd.n <= 1 << 14)  # lane members
+    for i, (a, b) in enumerate(zip(out_alone, out_mix)):
+        print(f"request {i}: n={systems[i].n} periodic={systems[i].periodic}", flush=True)
+        sa, sb = a["state"], b["state"]
+        assert np.array_equal(sa[6].view(np.int32), sb[6].view(np.int32)), i
+        err = np.abs(sa[:6] - sb[:6]).max() / (np.abs(sa[:6]).max() + 1e-30)
+        assert err <= 1e-6, (i, err)
+        assert abs(a["pot"] - b["pot"]) <= 1e-9 * abs(a["pot"]), i
+    for i, (sysd, r) in enumerate(zip(systems, out_mix)):
+        if not sysd.periodic:
+            continue
+        st = r["state"]
+        lim = np.asarray(sysd.limits, np.float32)
+        assert np.array_equal(r["limits"], lim), i  # a periodic request keeps its box
+        for d in range(3):
+            assert np.all((st[d] >= lim[2 * d]) & (st[d] < lim[2 * d + 1])), (i, d)
+        order = st[6].view(np.int32)
+        s = W.System(st[0], st[1], st[2], sysd.m[order], sysd.h[order], lim, periodic=True)
+        t = oracle_tree(s, inv_theta=1.0 / float(params["theta"][i]))
+        assert np.array_equal(t["perm"], np.arange(s.n))
+        _, _, oax, oay, oaz, _ = O.compute_gravity(t["leaves"], t["layout"], t["centers"], t["mp"], t["x"], t["y"],
+                                                   t["z"], t["h"], t["m"], lim, group_size=64, num_shells=1)
+        c64, q64 = O.upsweep(t["x"], t["y"], t["z"], t["m"], t["leaves"], t["layout"], lim.astype(np.float64),
+                             1.0 / float(params["theta"][i]), precision="f64")
+        _, _, eax, eay, eaz = O.ewald(q64[0], c64[0, :3], t["x"], t["y"], t["z"], t["m"], float(lim[1] - lim[0]))
+        rms = np.sqrt(np.mean(oax ** 2 + oay ** 2 + oaz ** 2))
+        d = np.sqrt((st[3] - oax - eax) ** 2 + (st[4] - oay - eay) ** 2 + (st[5] - oaz - eaz) ** 2) / rms
+        assert np.quantile(d, 0.99) <= 1e-4, (i, np.quantile(d, 0.99))
+
+
 def test_leapfrog_matches_one_solve_per_step(ctx):
     """the resident leapfrog equals kick-drift-kick done by hand with fresh solves"""
     s = W.plummer(20000, seed=5)
