@@ -21,6 +21,7 @@
 #include "sfc.hpp"
 
 #include <algorithm>
+#include <cmath>
 #include <cstring>
 
 namespace ryo
@@ -227,7 +228,13 @@ int uploadTables(ryo_batch* b, const ryo_params* prm, const float* theta, const 
         {
             const float* l = &b->limits[6 * s];
             const int sh  = b->sysShells.empty() ? prm->num_shells : b->sysShells[s];
+            const int ew  = b->sysEwald.empty() ? prm->ewald : b->sysEwald[s];
             if (sh < 0) return setError(ctx, RYO_ERR_ARG, "batch: num_shells of system %d", s);
+            // the Ewald correction's requirements, checked before the solve (ewald.cuh:25-27, ewald.hpp:53)
+            if (b->periodic[s] && ew != 0 && !(l[1] - l[0] == l[3] - l[2] && l[3] - l[2] == l[5] - l[4]))
+                return setError(ctx, RYO_ERR_UNSUPPORTED, "batch: Ewald gravity requires cubic boxes (system %d)", s);
+            if (b->periodic[s] && ew != 0 && std::ceil(prm->ewald_settings[2]) > 3.0)
+                return setError(ctx, RYO_ERR_UNSUPPORTED, "batch: Ewald hCut %g > 3", prm->ewald_settings[2]);
             im[s] = SysImages{b->periodic[s] ? sh : 0, {l[1] - l[0], l[3] - l[2], l[5] - l[4]}};
         }
         int rc = b->images.ensure(ctx, S * sizeof(SysImages));
