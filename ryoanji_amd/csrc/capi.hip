@@ -25,10 +25,14 @@ int buildLeaves(Context*, const uint64_t*, size_t, unsigned, uint64_t*, uint32_t
 int linkOctree(Context*, const uint64_t*, int, uint64_t*, int32_t*, int32_t*, int32_t*, int32_t*, int32_t*, int32_t*);
 int upsweep(Context*, const float*, const float*, const float*, const float*, const uint32_t*, int, const uint64_t*,
             const int32_t*, const int32_t*, const int32_t*, const int32_t*, const float*, float, float*, float*,
-            double*, double*, double);
+            double*, double*, double, const uint32_t* dStat = nullptr);
 int upsweepMDQ(Context*, const float*, const float*, const float*, const float*, const uint32_t*, int,
                const uint64_t*, const int32_t*, const int32_t*, const int32_t*, const int32_t*, const float*, float,
-               float*, float*, double*, double*);
+               float*, float*, double*, double*, const uint32_t* dStat = nullptr);
+int buildLeavesDevice(Context* ctx, const uint64_t* keys, size_t n, unsigned bucket, uint64_t* leaves,
+                      uint32_t* counts, uint32_t* layout, size_t capacity, uint32_t* stat);
+int linkOctreeDevice(Context* ctx, const uint64_t* leaves, size_t capacity, const uint32_t* stat, uint64_t* prefixes,
+                     int32_t* child, int32_t* parents, int32_t* levelRange, int32_t* i2l, int32_t* l2i);
 int upsweepMDQ64(Context*, const double*, const double*, const double*, const double*, const uint32_t*, int,
                  const uint64_t*, const int32_t*, const int32_t*, const int32_t*, const int32_t*, const double*,
                  float, double*, double*, float*);
@@ -57,7 +61,8 @@ int traverseEntry(Context* ctx, int numGroups, const uint32_t* gs, const uint32_
                   const float* ys, const float* zs, const float* ms, const float* hs, const int32_t* child,
                   const int32_t* i2l, const uint32_t* layout, const float* centers, const float* mp, float G,
                   int numShells, const float* boxL, float* p, float* ax, float* ay, float* az, double* totalPot,
-                  int stackCap, size_t hCount, bool mdq = false, bool noAlloc = false);
+                  int stackCap, size_t hCount, bool mdq = false, bool noAlloc = false,
+                  const uint32_t* abort = nullptr);
 int initNoAllocScratch(Context* ctx);
 // reference-signature stages (stages.hip, ewald_direct.hip)
 int nodeCounts(Context*, const uint64_t*, int, const uint64_t*, size_t, unsigned, uint32_t*);
@@ -584,6 +589,14 @@ struct ryo_system
     int lastShells = 0, lastEwald = 0;
     bool integrating = false; // velocities are in use (leapfrog): the build permutes them too
     bool mdq         = false; // the last build formed CartesianMDQpole multipoles (12 per node)
+    // a device-sized build (BuildStat counts on the device, copied to pinned host memory with the next
+    // synchronization): its leaf count, overflow and layout check are read after the gravity pass
+    uint32_t* dStat    = nullptr;
+    uint32_t* hStat    = nullptr;
+    bool statPending   = false;
+    bool timingPending = false; // the build's phase times are read at the next synchronization
+    ryo_params buildPrm{};      // the parameters of the last build (to build again when its leaves outgrew)
+    bool regrowing     = false; // inside the second pass after an outgrown build
     hipEvent_t ev[9] = {};
     double phaseMs[8] = {};
 };
@@ -611,10 +624,12 @@ int growTree(ryo_system* s, int L)
             return rc;
         s->leafCap = c;
     }
-    size_t nodes = size_t(L) + size_t(L - 1) / 7;
+    // the node arrays hold the nodes of a full leaf capacity (what a device-sized build may produce)
+    const size_t lc = std::max(s->leafCap, size_t(L));
+    size_t nodes    = lc + (lc - 1) / 7;
     if (nodes > s->nodeCap)
     {
-        size_t c = nodes + nodes / 4 + 64;
+        size_t c = nodes + 64;
         if ((rc = devAlloc(s, &s->prefixes, c)) || (rc = devAlloc(s, &s->child, c + 1)) ||
             (rc = devAlloc(s, &s->parents, c / 8 + 2)) || (rc = devAlloc(s, &s->i2l, c)) ||
             (rc = devAlloc(s, &s->l2i, c)) || (rc = devAlloc(s, &s->centers, 4 * c)) ||
@@ -670,6 +685,9 @@ int ryo_system_create(ryo_ctx* h, size_t capacity, ryo_system** out)
     if (!rc) rc = devAlloc(s, &s->az, capacity);
     for (auto& e : s->ev)
         if (!rc && hipEventCreate(&e) != hipSuccess) rc = setError(ctx, RYO_ERR_HIP, "system: event creation");
+    if (!rc) rc = devAlloc(s, &s->dStat, kStWords);
+    if (!rc && hipHostMalloc(reinterpret_cast<void**>(&s->hStat), kStWords * sizeof(uint32_t)) != hipSuccess)
+        rc = setError(ctx, RYO_ERR_HIP, "system: pinned host memory");
     if (rc)
     {
         ryo_system_destroy(s);
@@ -688,9 +706,10 @@ int ryo_system_destroy(ryo_system* s)
             if (p) hipFree(p);
     void* ptrs[] = {s->keys,    s->keyBuf, s->perm,    s->permBuf, s->sortTmp, s->p,   s->ax,   s->ay,
                     s->az,      s->leaves, s->counts,  s->layout,  s->prefixes, s->child, s->parents, s->i2l,
-                    s->l2i,     s->centers, s->mp,     s->c64,     s->q64,     s->groups, s->levelRange};
+                    s->l2i,     s->centers, s->mp,     s->c64,     s->q64,     s->groups, s->levelRange, s->dStat};
     for (void* p : ptrs)
         if (p) hipFree(p);
+    if (s->hStat) hipHostFree(s->hStat);
     for (auto& e : s->ev)
         if (e) hipEventDestroy(e);
     delete s;
@@ -718,6 +737,7 @@ int ryo_system_load(ryo_system* s, size_t n, const float* x, const float* y, con
     s->haveAcc  = false;
     s->integrating = false;
     s->freshOrder  = true;
+    s->statPending = false; // the last build's tree no longer describes the particles
     const float* src[5] = {x, y, z, m, h};
     for (int k = 0; k < 5; ++k)
         RYO_HIP(ctx, hipMemcpyAsync(s->a[0][k], src[k], n * sizeof(float), hipMemcpyDefault, ctx->stream));
@@ -744,6 +764,93 @@ int ryo_system_store(ryo_system* s, float* x, float* y, float* z, float* m, floa
 static void mark(ryo_system* s, int i)
 {
     if (s->ctx->timing) hipEventRecord(s->ev[i], s->ctx->stream);
+}
+
+// the build's phase times, once its events have completed (the caller has synchronized the stream)
+static void resolveTiming(ryo_system* s)
+{
+    if (!s->timingPending) return;
+    s->timingPending = false;
+    if (hipEventSynchronize(s->ev[6]) != hipSuccess) return;
+    for (int k = 0; k < 6; ++k)
+    {
+        float ms = 0;
+        hipEventElapsedTime(&ms, s->ev[k], s->ev[k + 1]);
+        s->phaseMs[k] = ms;
+    }
+}
+
+// the counts of a device-sized build: 1 when the leaves outgrew the arrays (build again after growTree), an error
+// when the layout does not end at n, else the leaf and node counts are taken over
+constexpr int kBuildRegrow = 1;
+static int resolveStatus(ryo_system* s)
+{
+    if (!s->statPending) return RYO_OK;
+    Context* ctx = s->ctx;
+    RYO_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    s->statPending = false;
+    const uint32_t* h = s->hStat;
+    if (h[kStOverflow]) return kBuildRegrow;
+    if (h[kStLast] != uint32_t(s->n))
+        return setError(ctx, RYO_ERR_HIP, "leaves: layout[L] = %u != n = %zu", h[kStLast], s->n);
+    s->L      = int(h[kStL]);
+    s->nNodes = int(h[kStNodes]);
+    return RYO_OK;
+}
+
+int ryo_system_build(ryo_system* s, const ryo_params* prm);
+
+// the counts of a device-sized build taken over; leaves that outgrew the arrays are built again with grown arrays
+// (from the last build's parameters), so the tree matches the particles before anyone reads it
+static int settleBuild(ryo_system* s)
+{
+    int rc = resolveStatus(s);
+    if (rc != kBuildRegrow) return rc;
+    s->haveAcc = false;
+    if ((rc = growTree(s, int(s->hStat[kStL]))) || (rc = ryo_system_build(s, &s->buildPrm))) return rc;
+    rc = resolveStatus(s);
+    return rc == kBuildRegrow ? setError(s->ctx, RYO_ERR_CAPACITY, "system: leaves outgrew the arrays after growth")
+                              : rc;
+}
+
+// target groups of a built system (fixed groups of group_size, or the spatial split of 64-target windows); the
+// build's phase times are read at the next synchronization
+static int buildGroups(ryo_system* s, const ryo_params* prm, float** A)
+{
+    Context* ctx   = s->ctx;
+    const size_t n = s->n;
+    int rc         = RYO_OK;
+    const int L    = s->L;
+    int gsz = prm->group_size > 0 ? prm->group_size : 64;
+    if (gsz > 64) return setError(ctx, RYO_ERR_ARG, "system: group size must be <= 64");
+    if (!(prm->group_split_tol >= 0.0f))
+        return setError(ctx, RYO_ERR_ARG, "system: group_split_tol must be >= 0");
+    const bool spatial = prm->group_split_tol > 0.0f;
+    if (spatial && gsz != 64)
+        return setError(ctx, RYO_ERR_UNSUPPORTED, "system: spatial groups split windows of 64 targets");
+    size_t G = (n + gsz - 1) / gsz;
+    for (int attempt = 0;; ++attempt)
+    {
+        if (G + 1 > s->groupCap)
+        {
+            s->groupCap = 0;
+            if ((rc = devAlloc(s, &s->groups, G + 1 + G / 4))) return rc;
+            s->groupCap = G + 1 + G / 4;
+        }
+        if (!spatial)
+        {
+            rc = fixedGroups(ctx, 0, uint32_t(n), unsigned(gsz), s->groups, &s->G);
+            break;
+        }
+        // the split count is known only after the split kernel: grow and split again when it exceeds capacity
+        rc = groupSplits(ctx, 0, uint32_t(n), A[0], A[1], A[2], A[4], s->leaves, L, s->layout, s->limits, 64u,
+                         prm->group_split_tol, 8.0f, s->groups, s->groupCap, &s->G);
+        if (rc != RYO_ERR_CAPACITY || attempt > 0) break;
+        G = size_t(s->G);
+    }
+    if (rc) return rc;
+    s->timingPending = ctx->timing != 0;
+    return RYO_OK;
 }
 
 int ryo_system_build(ryo_system* s, const ryo_params* prm)
@@ -781,9 +888,48 @@ int ryo_system_build(ryo_system* s, const ryo_params* prm)
     s->cur ^= 1;
     A = s->a[s->cur];
     mark(s, 3);
+    if (s->leafCap == 0 && (rc = growTree(s, int(std::max<size_t>(64, n / 8))))) return rc;
+    if (!s->levelRange && (rc = devAlloc(s, &s->levelRange, 24))) return rc;
+    if (prm->multipole != 0 && prm->multipole != 1)
+        return setError(ctx, RYO_ERR_ARG, "system: multipole must be 0 (quadrupole) or 1 (MDQpole)");
+    if (prm->multipole == 1 && prm->soft_mac)
+        return setError(ctx, RYO_ERR_UNSUPPORTED, "system: the softening-safe MAC is for quadrupoles");
+    // sized on the device (no host round trip before the traversal): leaves into the current capacity, the linked
+    // tree and the upsweep sized from the device counts; the counts are checked after the gravity pass, which
+    // builds again with larger arrays in the rare case the leaves outgrew them. The single-leaf system, the
+    // softening-safe MAC (which reads the softening range) and spatial groups (which read the leaf count) take the
+    // host-sized sequence below.
+    const bool deviceSized = n > prm->bucket_size && prm->bucket_size > 0 && prm->bucket_size <= 1024 &&
+                             !prm->soft_mac && !(prm->group_split_tol > 0.0f);
+    s->statPending = false;
+    s->buildPrm    = *prm;
+    if (deviceSized)
+    {
+        if ((rc = buildLeavesDevice(ctx, s->keys, n, prm->bucket_size, s->leaves, s->counts, s->layout, s->leafCap,
+                                    s->dStat)))
+            return rc;
+        mark(s, 4);
+        if ((rc = linkOctreeDevice(ctx, s->leaves, s->leafCap, s->dStat, s->prefixes, s->child, s->parents,
+                                   s->levelRange, s->i2l, s->l2i)))
+            return rc;
+        mark(s, 5);
+        s->mdq      = prm->multipole == 1;
+        const int C = int(s->leafCap);
+        if ((rc = s->mdq ? upsweepMDQ(ctx, A[0], A[1], A[2], A[3], s->layout, C, s->prefixes, s->child, s->parents,
+                                      s->l2i, s->levelRange, s->limits, 1.0f / prm->theta, s->centers, s->mp, s->c64,
+                                      s->q64, s->dStat)
+                         : upsweep(ctx, A[0], A[1], A[2], A[3], s->layout, C, s->prefixes, s->child, s->parents,
+                                   s->l2i, s->levelRange, s->limits, 1.0f / prm->theta, s->centers, s->mp, s->c64,
+                                   s->q64, 0.0, s->dStat)))
+            return rc;
+        RYO_HIP(ctx, hipMemcpyAsync(s->hStat, s->dStat, kStWords * sizeof(uint32_t), hipMemcpyDeviceToHost,
+                                    ctx->stream));
+        s->statPending = true;
+        mark(s, 6);
+        return buildGroups(s, prm, A);
+    }
     // leaves: count pass + host read of L, then emission into buffers of the right size
     int L = 0;
-    if (s->leafCap == 0 && (rc = growTree(s, int(std::max<size_t>(64, n / 8))))) return rc;
     rc = buildLeaves(ctx, s->keys, n, prm->bucket_size, s->leaves, s->counts, s->layout, s->leafCap, &L);
     if (rc == RYO_ERR_CAPACITY)
     {
@@ -795,15 +941,10 @@ int ryo_system_build(ryo_system* s, const ryo_params* prm)
     s->L      = L;
     s->nNodes = L + (L - 1) / 7;
     mark(s, 4);
-    if (!s->levelRange && (rc = devAlloc(s, &s->levelRange, 24))) return rc;
     if ((rc = linkOctree(ctx, s->leaves, L, s->prefixes, s->child, s->parents, s->levelRange, s->i2l, s->l2i,
                          nullptr)))
         return rc;
     mark(s, 5);
-    if (prm->multipole != 0 && prm->multipole != 1)
-        return setError(ctx, RYO_ERR_ARG, "system: multipole must be 0 (quadrupole) or 1 (MDQpole)");
-    if (prm->multipole == 1 && prm->soft_mac)
-        return setError(ctx, RYO_ERR_UNSUPPORTED, "system: the softening-safe MAC is for quadrupoles");
     s->mdq           = prm->multipole == 1;
     double softReach = 0;
     if (prm->soft_mac)
@@ -821,45 +962,7 @@ int ryo_system_build(ryo_system* s, const ryo_params* prm)
                                softReach)))
         return rc;
     mark(s, 6);
-    int gsz = prm->group_size > 0 ? prm->group_size : 64;
-    if (gsz > 64) return setError(ctx, RYO_ERR_ARG, "system: group size must be <= 64");
-    if (!(prm->group_split_tol >= 0.0f))
-        return setError(ctx, RYO_ERR_ARG, "system: group_split_tol must be >= 0");
-    const bool spatial = prm->group_split_tol > 0.0f;
-    if (spatial && gsz != 64)
-        return setError(ctx, RYO_ERR_UNSUPPORTED, "system: spatial groups split windows of 64 targets");
-    size_t G = (n + gsz - 1) / gsz;
-    for (int attempt = 0;; ++attempt)
-    {
-        if (G + 1 > s->groupCap)
-        {
-            s->groupCap = 0;
-            if ((rc = devAlloc(s, &s->groups, G + 1 + G / 4))) return rc;
-            s->groupCap = G + 1 + G / 4;
-        }
-        if (!spatial)
-        {
-            rc = fixedGroups(ctx, 0, uint32_t(n), unsigned(gsz), s->groups, &s->G);
-            break;
-        }
-        // the split count is known only after the split kernel: grow and split again when it exceeds capacity
-        rc = groupSplits(ctx, 0, uint32_t(n), A[0], A[1], A[2], A[4], s->leaves, L, s->layout, s->limits, 64u,
-                         prm->group_split_tol, 8.0f, s->groups, s->groupCap, &s->G);
-        if (rc != RYO_ERR_CAPACITY || attempt > 0) break;
-        G = size_t(s->G);
-    }
-    if (rc) return rc;
-    if (ctx->timing)
-    {
-        RYO_HIP(ctx, hipEventSynchronize(s->ev[6]));
-        for (int k = 0; k < 6; ++k)
-        {
-            float ms = 0;
-            hipEventElapsedTime(&ms, s->ev[k], s->ev[k + 1]);
-            s->phaseMs[k] = ms;
-        }
-    }
-    return RYO_OK;
+    return buildGroups(s, prm, A);
 }
 
 int ryo_system_gravity(ryo_system* s, const ryo_params* prm, double* totalPot)
@@ -879,7 +982,27 @@ int ryo_system_gravity(ryo_system* s, const ryo_params* prm, double* totalPot)
         return setError(ctx, RYO_ERR_UNSUPPORTED, "system: the Ewald correction is for quadrupole systems");
     rc = traverseEntry(ctx, s->G, s->groups, s->groups + 1, prm->init_node, A[0], A[1], A[2], A[3], A[4], A[0], A[1],
                        A[2], A[3], A[4], s->child, s->i2l, s->layout, s->centers, s->mp, prm->G, shells, boxL, s->p,
-                       s->ax, s->ay, s->az, &pot, 0, s->n, s->mdq);
+                       s->ax, s->ay, s->az, &pot, 0, s->n, s->mdq, false,
+                       s->statPending ? s->dStat + kStAbort : nullptr);
+    // a device-sized build is checked here, with the traversal's synchronization: leaves that outgrew the arrays
+    // (whatever the traversal made of the clipped tree) mean one more build with grown arrays and a new pass
+    if (s->statPending)
+    {
+        const int st = resolveStatus(s);
+        if (st == kBuildRegrow)
+        {
+            // the arrays grow past the leaf count just read, so the second build fits
+            if (s->regrowing) return setError(ctx, RYO_ERR_CAPACITY, "system: leaves outgrew the arrays after growth");
+            s->regrowing = true;
+            int brc = growTree(s, int(s->hStat[kStL]));
+            if (!brc) brc = ryo_system_build(s, &s->buildPrm);
+            if (!brc) brc = ryo_system_gravity(s, prm, totalPot);
+            s->regrowing = false;
+            return brc;
+        }
+        if (st) return st;
+    }
+    resolveTiming(s);
     if (rc) return rc;
     s->phaseMs[6] = ctx->lastTraverseMs;
     s->phaseMs[7] = 0;
@@ -1011,6 +1134,7 @@ int ryo_system_results(ryo_system* s, float** out, int32_t** order)
 int ryo_system_info(ryo_system* s, int64_t* info)
 {
     if (!s || !info) return RYO_ERR_ARG;
+    if (int rc = settleBuild(s)) return rc;
     info[0] = int64_t(s->n), info[1] = s->L, info[2] = s->nNodes, info[3] = s->G;
     return RYO_OK;
 }
@@ -1018,6 +1142,7 @@ int ryo_system_info(ryo_system* s, int64_t* info)
 int ryo_system_tree(ryo_system* s, void** p)
 {
     if (!s || !p) return RYO_ERR_ARG;
+    if (int rc = settleBuild(s)) return rc;
     void* v[14] = {s->keys, s->leaves, s->counts, s->layout, s->prefixes, s->child, s->parents, s->levelRange,
                    s->i2l,  s->l2i,    s->centers, s->mp,    s->c64,      s->q64};
     for (int k = 0; k < 14; ++k)
@@ -1070,6 +1195,8 @@ int ryo_system_limits(ryo_system* s, float* limits)
 int ryo_system_phase_ms(ryo_system* s, double* ms)
 {
     if (!s || !ms) return RYO_ERR_ARG;
+    if (s->timingPending) hipStreamSynchronize(s->ctx->stream);
+    resolveTiming(s);
     for (int k = 0; k < 8; ++k)
         ms[k] = s->phaseMs[k];
     return RYO_OK;

@@ -128,6 +128,15 @@ struct BatchView
     const int32_t* LR;
 };
 
+// the counts of a device-sized build (device, kStWords words): {L, nodes, internal nodes, L > capacity, layout[L]
+// (n after an overflow), the leaf count the later stages use (L, or 0 when the build aborts), abort (an overflow or
+// a layout not ending at n)}; every later stage of that build takes its sizes from here, within the capacity its
+// launches and arrays are sized for
+enum BuildStat : int
+{
+    kStL = 0, kStNodes, kStInternal, kStOverflow, kStLast, kStLc, kStAbort, kStWords
+};
+
 // the periodic images of one system of a batch: (2 shells + 1)^3 replicas of its box lengths L (shells 0: open)
 struct SysImages
 {

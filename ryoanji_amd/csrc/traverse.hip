@@ -58,6 +58,7 @@ struct TravArgs
     size_t hCount; // targets and sources are h[0..hCount) (resident systems), 0 when unknown
     int ranges;    // contiguous group ranges, one per XCD (1: a single global counter)
     bool mdq;      // multipoles are CartesianMDQpole (12 floats per node: the quadrupole's 8, then px, py, pz, 0)
+    const uint32_t* abort; // device, may be null: nonzero when the tree was not built (traverse nothing)
 };
 
 struct Acc
@@ -438,6 +439,7 @@ __global__ __launch_bounds__(64 * Waves, kWavesPerCU / 4) void traverseKernelT(T
     const int w    = threadIdx.x >> 6;
     const int lane = threadIdx.x & 63;
     int* stack     = stackMem + w * a.stackCap;
+    if (a.abort && *a.abort) return;
     if constexpr (Batched)
         if (tb.abort && *tb.abort) return;
 
@@ -1025,7 +1027,7 @@ int traverseEntry(Context* ctx, int numGroups, const uint32_t* gs, const uint32_
                   const float* ys, const float* zs, const float* ms, const float* hs, const int32_t* child,
                   const int32_t* i2l, const uint32_t* layout, const float* centers, const float* mp, float G,
                   int numShells, const float* boxL, float* p, float* ax, float* ay, float* az, double* totalPot,
-                  int stackCap, size_t hCount, bool mdq, bool noAlloc)
+                  int stackCap, size_t hCount, bool mdq, bool noAlloc, const uint32_t* abort)
 {
     if (numGroups > 0 && (!gs || !ge || !xt || !yt || !zt || !mt || !ht || !xs || !ys || !zs || !ms || !hs ||
                           !child || !i2l || !layout || !centers || !mp || !ax || !ay || !az))
@@ -1045,6 +1047,7 @@ int traverseEntry(Context* ctx, int numGroups, const uint32_t* gs, const uint32_
     a.stackCap = stackCap;
     a.hCount   = hCount;
     a.mdq      = mdq;
+    a.abort    = abort;
     return noAlloc ? traverseNoAlloc(ctx, a, totalPot) : traverse(ctx, a, totalPot);
 }
 

@@ -22,6 +22,8 @@ namespace ryo
 
 int sortPairs(Context* ctx, uint64_t* keys, int32_t* vals, size_t n, uint64_t* keyBuf, int32_t* valBuf, void* tmp,
               size_t tmpBytes);
+int sortPairsDigits(Context* ctx, uint64_t* keys, int32_t* vals, size_t n, uint64_t* keyBuf, int32_t* valBuf,
+                    void* tmp, size_t tmpBytes, int digits, const uint32_t* dN);
 size_t sortTempBytes(size_t n);
 
 /* ============================================================================ leaves */
@@ -290,8 +292,10 @@ __device__ void leafEmitTile(const uint64_t* __restrict__ k, long n, int B, long
     }
 }
 
-__global__ void leafCountsKernel(const uint32_t* __restrict__ layout, uint32_t* __restrict__ counts, int L)
+__global__ void leafCountsKernel(const uint32_t* __restrict__ layout, uint32_t* __restrict__ counts, int L,
+                                 const uint32_t* __restrict__ stat = nullptr)
 {
+    if (stat) L = int(stat[kStLc]);
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < L; i += gridDim.x * blockDim.x)
         counts[i] = layout[i + 1] - layout[i];
 }
@@ -348,6 +352,57 @@ int buildLeaves(Context* ctx, const uint64_t* keys, size_t n, unsigned bucket, u
     return RYO_OK;
 }
 
+// the build's counts from the scanned leaf total (one thread)
+__global__ void leafStatusKernel(const uint32_t* __restrict__ total, const uint32_t* __restrict__ layout, uint32_t n,
+                                 uint32_t capacity, uint32_t* __restrict__ stat)
+{
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    const uint32_t L = *total;
+    // more leaves than the arrays hold, or a layout that does not end at n: every later stage sees an empty tree
+    // and the traversal an abort flag, so none reads through the clipped or inconsistent leaf array; the caller
+    // builds again with grown arrays, or reports the layout
+    const bool over   = L > capacity;
+    const uint32_t last = over ? n : layout[L];
+    const bool abort  = over || last != n;
+    const uint32_t Lc = abort ? 0u : L;
+    const uint32_t nI = Lc > 0 ? (Lc - 1) / 7 : 0;
+    stat[kStL] = L, stat[kStNodes] = Lc + nI, stat[kStInternal] = nI, stat[kStOverflow] = over;
+    stat[kStLast] = last, stat[kStLc] = Lc, stat[kStAbort] = abort;
+}
+
+/* The leaves of a system of more than `bucket` particles sized on the device: count, scan, emission into arrays of
+ * `capacity` leaves and the counts, with no host round trip; stat (device, kStWords words) receives the leaf count,
+ * node counts, an overflow flag (more than `capacity` leaves: the caller builds again with larger arrays) and the
+ * layout terminator for the caller's check (layout[L] must be n). Every later stage sized from stat stays inside
+ * arrays sized for `capacity` leaves. */
+int buildLeavesDevice(Context* ctx, const uint64_t* keys, size_t n, unsigned bucket, uint64_t* leaves,
+                      uint32_t* counts, uint32_t* layout, size_t capacity, uint32_t* stat)
+{
+    hipStream_t s = ctx->stream;
+    if (bucket == 0 || n <= bucket || bucket > unsigned(kMaxBucket))
+        return setError(ctx, RYO_ERR_ARG, "leaves: device-sized build needs n > bucket and bucket <= %d", kMaxBucket);
+    int numBlocks = int((n + kLeafTile - 1) / kLeafTile);
+    const size_t scanWork = size_t(numBlocks) / kScanChunk + 2;
+    auto* sums = static_cast<uint32_t*>(ctx->getScratch(kSlotLeafBlocks, (numBlocks + 1 + scanWork) * sizeof(uint32_t)));
+    if (!sums) return RYO_ERR_HIP;
+    if (bucket <= unsigned(kSmallBucket))
+        leafCountKernel<kSmallBucket><<<numBlocks, kLeafThreads, 0, s>>>(keys, long(n), int(bucket), sums);
+    else
+        leafCountKernel<kMaxBucket><<<numBlocks, kLeafThreads, 0, s>>>(keys, long(n), int(bucket), sums);
+    if (int rc = scanBlockSums(ctx, sums, numBlocks, sums + numBlocks + 1)) return rc;
+    if (bucket <= unsigned(kSmallBucket))
+        leafEmitKernel<kSmallBucket><<<numBlocks, kLeafThreads, 0, s>>>(keys, long(n), int(bucket), sums, leaves,
+                                                                         layout, long(capacity));
+    else
+        leafEmitKernel<kMaxBucket><<<numBlocks, kLeafThreads, 0, s>>>(keys, long(n), int(bucket), sums, leaves,
+                                                                       layout, long(capacity));
+    leafStatusKernel<<<1, 64, 0, s>>>(sums + numBlocks, layout, uint32_t(n), uint32_t(capacity), stat);
+    leafCountsKernel<<<std::max(1u, std::min(iceil(capacity, 256), unsigned(ctx->numCUs * 8))), 256, 0, s>>>(
+        layout, counts, int(capacity), stat);
+    RYO_LAUNCH_CHECK(ctx);
+    return RYO_OK;
+}
+
 /* ============================================================================ linked octree */
 
 __device__ __forceinline__ int digitWeight(int d)
@@ -359,8 +414,9 @@ __device__ __forceinline__ int digitWeight(int d)
 // prefixes and unsorted-layout indices of leaves (at numInternal + leaf) and internal nodes (at the
 // binary-tree index derived from the leaf boundary between the node's 4th and 5th child)
 __global__ void unsortedLayoutKernel(const uint64_t* __restrict__ leaves, int L, int nI, uint64_t* __restrict__ pre,
-                                     int32_t* __restrict__ idx)
+                                     int32_t* __restrict__ idx, const uint32_t* __restrict__ stat = nullptr)
 {
+    if (stat) L = int(stat[kStLc]), nI = int(stat[kStInternal]);
     for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < L; t += gridDim.x * blockDim.x)
     {
         uint64_t key  = leaves[t], next = leaves[t + 1];
@@ -384,8 +440,10 @@ __global__ void unsortedLayoutKernel(const uint64_t* __restrict__ leaves, int L,
     }
 }
 
-__global__ void invertKernel(int32_t* __restrict__ i2l, int32_t* __restrict__ l2i, int n, int nI)
+__global__ void invertKernel(int32_t* __restrict__ i2l, int32_t* __restrict__ l2i, int n, int nI,
+                             const uint32_t* __restrict__ stat = nullptr)
 {
+    if (stat) n = int(stat[kStNodes]), nI = int(stat[kStInternal]);
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
     {
         int u = i2l[i];
@@ -405,8 +463,10 @@ __device__ inline int lowerBound(const uint64_t* a, int lo, int hi, uint64_t v)
     return lo;
 }
 
-__global__ void levelRangeKernel(const uint64_t* __restrict__ pre, int n, int32_t* __restrict__ levelRange)
+__global__ void levelRangeKernel(const uint64_t* __restrict__ pre, int n, int32_t* __restrict__ levelRange,
+                                 const uint32_t* __restrict__ stat = nullptr)
 {
+    if (stat) n = int(stat[kStNodes]);
     int l = threadIdx.x;
     if (l <= int(kMaxLevel)) levelRange[l] = lowerBound(pre, 0, n, encodePlaceholder(0, 3 * l));
     if (l == int(kMaxLevel) + 1) levelRange[l] = n;
@@ -420,8 +480,9 @@ __global__ void zeroI32Kernel(int32_t* a, int n)
 
 __global__ void linkKernel(const uint64_t* __restrict__ pre, int n, int nI, const int32_t* __restrict__ l2i,
                            const int32_t* __restrict__ levelRange, int32_t* __restrict__ child,
-                           int32_t* __restrict__ parents)
+                           int32_t* __restrict__ parents, const uint32_t* __restrict__ stat = nullptr)
 {
+    if (stat) n = int(stat[kStNodes]), nI = int(stat[kStInternal]);
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < nI; i += gridDim.x * blockDim.x)
     {
         int a = l2i[i];
@@ -472,6 +533,37 @@ int linkOctree(Context* ctx, const uint64_t* leaves, int L, uint64_t* prefixes, 
                                     hipMemcpyDeviceToHost, s));
         RYO_HIP(ctx, hipStreamSynchronize(s));
     }
+    return RYO_OK;
+}
+
+/* linkOctree for a device-sized build: launches and arrays for `capacity` leaves, the actual counts from stat
+ * (buildLeavesDevice); no host round trip. */
+int linkOctreeDevice(Context* ctx, const uint64_t* leaves, size_t capacity, const uint32_t* stat, uint64_t* prefixes,
+                     int32_t* child, int32_t* parents, int32_t* levelRange, int32_t* i2l, int32_t* l2i)
+{
+    if (capacity < 1) return setError(ctx, RYO_ERR_ARG, "octree: need at least one leaf");
+    hipStream_t s = ctx->stream;
+    const int Lc = int(capacity), nI = (Lc - 1) / 7, n = Lc + nI;
+    unsigned g = std::max(1u, std::min(iceil(n, 256), unsigned(ctx->numCUs * 8)));
+    unsortedLayoutKernel<<<g, 256, 0, s>>>(leaves, Lc, nI, prefixes, i2l, stat);
+    size_t tb   = sortTempBytes(n);
+    size_t need = n * (sizeof(uint64_t) + sizeof(int32_t)) + tb + 512;
+    auto* buf   = static_cast<char*>(ctx->getScratch(kSlotLinkPairs, need));
+    if (!buf) return RYO_ERR_HIP;
+    auto* kb  = reinterpret_cast<uint64_t*>(buf);
+    auto* vb  = reinterpret_cast<int32_t*>(buf + n * sizeof(uint64_t));
+    void* tmp = buf + ((n * (sizeof(uint64_t) + sizeof(int32_t)) + 255) & ~size_t(255));
+    int rc    = sortPairsDigits(ctx, prefixes, i2l, n, kb, vb, tmp, tb, 8, stat + kStNodes);
+    if (rc) return rc;
+    invertKernel<<<g, 256, 0, s>>>(i2l, l2i, n, nI, stat);
+    levelRangeKernel<<<1, 64, 0, s>>>(prefixes, n, levelRange, stat);
+    zeroI32Kernel<<<g, 256, 0, s>>>(child, n + 1);
+    int np = std::max(1, (n - 1) / 8);
+    zeroI32Kernel<<<std::max(1u, std::min(iceil(np, 256), unsigned(ctx->numCUs * 8))), 256, 0, s>>>(parents, np);
+    if (nI > 0)
+        linkKernel<<<std::max(1u, std::min(iceil(nI, 256), unsigned(ctx->numCUs * 8))), 256, 0, s>>>(
+            prefixes, n, nI, l2i, levelRange, child, parents, stat);
+    RYO_LAUNCH_CHECK(ctx);
     return RYO_OK;
 }
 

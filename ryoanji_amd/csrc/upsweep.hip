@@ -36,6 +36,9 @@ struct UpsweepArgsT
     double* q64; // [n][8]
     MacBox box;
     double4* cmac = nullptr; // [n] {com, mac^2} in double (SourceCenterType<double>)
+    // a device-sized build's counts (BuildStat): the leaves are the first stat[kStLc] (at most numLeaves) and
+    // leafNode is leafToInternal, offset by stat[kStInternal] on the device
+    const uint32_t* dStat = nullptr;
 };
 // multipole values per node: 8 (CartesianQuadrupole), 12 (CartesianMDQpole: the quadrupole's 8, px, py, pz, 0)
 template<bool MDQ>
@@ -135,11 +138,20 @@ __device__ inline void leafUpsweepOne(const UpsweepArgsT<Tin>& a, int leaf, bool
     if (ok && sub == 0) storeNode<MDQ>(a, a.leafNode[leaf], c, q);
 }
 
+// one 8-lane group per leaf, grid-stride (the 8 lanes of a group step together)
 template<class Tin, bool MDQ = false>
 __global__ __launch_bounds__(256) void leafUpsweepKernel(UpsweepArgsT<Tin> a)
 {
-    const int leaf = int((size_t(blockIdx.x) * blockDim.x + threadIdx.x) >> 3);
-    leafUpsweepOne<MDQ>(a, leaf, leaf < a.numLeaves, threadIdx.x & 7);
+    if (a.dStat)
+    {
+        a.numLeaves = min(int(a.dStat[kStLc]), a.numLeaves);
+        a.leafNode += a.dStat[kStInternal];
+    }
+    const int groups = int((size_t(gridDim.x) * blockDim.x) >> 3);
+    const int group  = int((size_t(blockIdx.x) * blockDim.x + threadIdx.x) >> 3);
+    // the loop bound is uniform over the grid, so the 8-lane reductions always see all 8 lanes
+    for (int base = 0; base < a.numLeaves; base += groups)
+        leafUpsweepOne<MDQ>(a, base + group, base + group < a.numLeaves, threadIdx.x & 7);
 }
 
 // one internal node: lane k of the 8-lane group holds child k; center and M2M by 8-lane reductions. MDQ follows
@@ -313,7 +325,7 @@ static int upsweepT(Context* ctx, UpsweepArgsT<Tin> a, int L, const int32_t* lev
     if (!a.c64) a.c64 = reinterpret_cast<double*>(buf + off), off += size_t(n) * 4 * sizeof(double);
     if (!a.q64) a.q64 = reinterpret_cast<double*>(buf + off), off += size_t(n) * kMpN<MDQ> * sizeof(double);
     a.numLeaves = L;
-    a.leafNode += nI; // leafToInternal + numInternal
+    if (!a.dStat) a.leafNode += nI; // leafToInternal + numInternal (device-sized builds: offset on the device)
     const Bits3 bits = boxBits(limits);
     a.box.bits       = bits;
     for (int d = 0; d < 3; ++d)
@@ -325,7 +337,8 @@ static int upsweepT(Context* ctx, UpsweepArgsT<Tin> a, int L, const int32_t* lev
     a.box.invTheta  = invTheta;
     a.box.softReach = softReach;
     hipStream_t s = ctx->stream;
-    leafUpsweepKernel<Tin, MDQ><<<iceil(size_t(L) * 8, 256), 256, 0, s>>>(a);
+    leafUpsweepKernel<Tin, MDQ><<<std::max(1u, std::min(iceil(size_t(L) * 8, 256), unsigned(ctx->numCUs * 64))), 256,
+                                   0, s>>>(a);
     const unsigned levelBlocks = std::max(1u, std::min(iceil(size_t(nI) * 8, 256), unsigned(ctx->numCUs * 4)));
     if (nI > 0)
         for (int l = int(kMaxLevel) - 1; l >= 0; --l)
@@ -337,9 +350,10 @@ static int upsweepT(Context* ctx, UpsweepArgsT<Tin> a, int L, const int32_t* lev
 int upsweep(Context* ctx, const float* x, const float* y, const float* z, const float* m, const uint32_t* layout,
             int L, const uint64_t* prefixes, const int32_t* child, const int32_t* parents, const int32_t* l2i,
             const int32_t* levelRange, const float* limits, float invTheta, float* centers, float* mpoles, double* c64,
-            double* q64, double softReach)
+            double* q64, double softReach, const uint32_t* dStat)
 {
     UpsweepArgs a{};
+    a.dStat = dStat;
     a.x = x, a.y = y, a.z = z, a.m = m;
     a.layout = layout;
     a.leafNode = l2i, a.parents = parents, a.child = child, a.prefixes = prefixes;
@@ -351,9 +365,10 @@ int upsweep(Context* ctx, const float* x, const float* y, const float* z, const 
 int upsweepMDQ(Context* ctx, const float* x, const float* y, const float* z, const float* m, const uint32_t* layout,
                int L, const uint64_t* prefixes, const int32_t* child, const int32_t* parents, const int32_t* l2i,
                const int32_t* levelRange, const float* limits, float invTheta, float* centers, float* mpoles,
-               double* c64, double* q64)
+               double* c64, double* q64, const uint32_t* dStat)
 {
     UpsweepArgs a{};
+    a.dStat = dStat;
     a.x = x, a.y = y, a.z = z, a.m = m;
     a.layout = layout;
     a.leafNode = l2i, a.parents = parents, a.child = child, a.prefixes = prefixes;

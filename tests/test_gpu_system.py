@@ -66,6 +66,37 @@ def test_system_tree_and_forces(ctx, name, mk):
     assert abs(pot - otot) <= 1e-5 * abs(otot)
 
 
+@pytest.mark.parametrize("via", ["gravity", "info"])
+def test_system_leaves_outgrow_the_arrays(ctx, via):
+    """the tree is sized on the device: with bucket 4 the leaves outgrow the first arrays (sized for n / 8), the
+    pass sees an empty tree and an abort flag, and the system builds again with grown arrays, found either by the
+    gravity call or by info() right after the build; tree and forces as the oracle's"""
+    s = W.uniform_cube(20000, seed=44)
+    sysm = R.System(ctx, s.n)
+    arrs = [dev(a) for a in s.arrays()]
+    sysm.load(*arrs, s.limits)
+    prm = R.default_params(bucket_size=4)
+    sysm.build(prm)
+    if via == "info":
+        info = sysm.info()  # settles the build before anything reads the tree
+    pot = sysm.gravity(prm)
+    info = sysm.info()
+    p, ax, ay, az, order = (host(t) for t in sysm.results())
+    t = oracle_tree(s, bucket=4)
+    assert info["numLeaves"] == t["leaves"].size - 1 > s.n // 8
+    assert np.array_equal(order, t["perm"])
+    tr = sysm.tree()
+    assert np.array_equal(host(tr["leaves"], np.uint64), t["leaves"])
+    assert np.array_equal(host(tr["layout"], np.uint32), t["layout"])
+    for k in ("childOffsets", "internalToLeaf", "leafToInternal", "parents"):
+        assert np.array_equal(host(tr[k]), t["tree"][k]), k
+    _, _, oax, oay, oaz, _ = O.compute_gravity(t["leaves"], t["layout"], t["centers"], t["mp"], t["x"], t["y"], t["z"],
+                                               t["h"], t["m"], s.limits, group_size=64)
+    e = np.sort(rel_err(ax, ay, az, oax, oay, oaz))
+    assert e[int(0.99 * e.size)] <= 1e-5, e[int(0.99 * e.size)]
+    sysm.close()
+
+
 def test_accuracy_vs_fp64_direct(ctx):
     """reference bounds at theta 0.5 (global_forces_gpu.cpp:218-220): 1st pct < 2e-3, max < 3e-2, pot < 1e-2"""
     s = W.uniform_cube(65536, seed=3)
