@@ -597,6 +597,9 @@ struct ryo_system
     bool timingPending = false; // the build's phase times are read at the next synchronization
     ryo_params buildPrm{};      // the parameters of the last build (to build again when its leaves outgrew)
     bool regrowing     = false; // inside the second pass after an outgrown build
+    // leaves a device-sized build launches for: the last leaf count with a margin (the particles of a loaded system
+    // change little between builds), 0 for the whole capacity; a build with more leaves overflows and runs again
+    size_t leafHint    = 0;
     hipEvent_t ev[9] = {};
     double phaseMs[8] = {};
 };
@@ -730,6 +733,8 @@ int ryo_system_load(ryo_system* s, size_t n, const float* x, const float* y, con
     if (!s) return RYO_ERR_ARG;
     Context* ctx = s->ctx;
     if (n > s->cap) return setError(ctx, RYO_ERR_CAPACITY, "system: %zu particles > capacity %zu", n, s->cap);
+    s->statPending = false; // the last build's tree no longer describes the particles
+    if (n != s->n) s->leafHint = 0; // a new system of the same size keeps the launch size (a miss builds again)
     s->n = n;
     std::memcpy(s->limits, limits, sizeof(s->limits));
     s->periodic = periodic;
@@ -737,7 +742,6 @@ int ryo_system_load(ryo_system* s, size_t n, const float* x, const float* y, con
     s->haveAcc  = false;
     s->integrating = false;
     s->freshOrder  = true;
-    s->statPending = false; // the last build's tree no longer describes the particles
     const float* src[5] = {x, y, z, m, h};
     for (int k = 0; k < 5; ++k)
         RYO_HIP(ctx, hipMemcpyAsync(s->a[0][k], src[k], n * sizeof(float), hipMemcpyDefault, ctx->stream));
@@ -790,6 +794,7 @@ static int resolveStatus(ryo_system* s)
     RYO_HIP(ctx, hipStreamSynchronize(ctx->stream));
     s->statPending = false;
     const uint32_t* h = s->hStat;
+    s->leafHint       = size_t(h[kStL]) + h[kStL] / 4 + 64;
     if (h[kStOverflow]) return kBuildRegrow;
     if (h[kStLast] != uint32_t(s->n))
         return setError(ctx, RYO_ERR_HIP, "leaves: layout[L] = %u != n = %zu", h[kStLast], s->n);
@@ -905,16 +910,17 @@ int ryo_system_build(ryo_system* s, const ryo_params* prm)
     s->buildPrm    = *prm;
     if (deviceSized)
     {
-        if ((rc = buildLeavesDevice(ctx, s->keys, n, prm->bucket_size, s->leaves, s->counts, s->layout, s->leafCap,
+        const size_t cap = s->leafHint ? std::min(s->leafHint, s->leafCap) : s->leafCap;
+        if ((rc = buildLeavesDevice(ctx, s->keys, n, prm->bucket_size, s->leaves, s->counts, s->layout, cap,
                                     s->dStat)))
             return rc;
         mark(s, 4);
-        if ((rc = linkOctreeDevice(ctx, s->leaves, s->leafCap, s->dStat, s->prefixes, s->child, s->parents,
-                                   s->levelRange, s->i2l, s->l2i)))
+        if ((rc = linkOctreeDevice(ctx, s->leaves, cap, s->dStat, s->prefixes, s->child, s->parents, s->levelRange,
+                                   s->i2l, s->l2i)))
             return rc;
         mark(s, 5);
         s->mdq      = prm->multipole == 1;
-        const int C = int(s->leafCap);
+        const int C = int(cap);
         if ((rc = s->mdq ? upsweepMDQ(ctx, A[0], A[1], A[2], A[3], s->layout, C, s->prefixes, s->child, s->parents,
                                       s->l2i, s->levelRange, s->limits, 1.0f / prm->theta, s->centers, s->mp, s->c64,
                                       s->q64, s->dStat)

@@ -66,24 +66,32 @@ def test_system_tree_and_forces(ctx, name, mk):
     assert abs(pot - otot) <= 1e-5 * abs(otot)
 
 
-@pytest.mark.parametrize("via", ["gravity", "info"])
+@pytest.mark.parametrize("via", ["gravity", "info", "hint"])
 def test_system_leaves_outgrow_the_arrays(ctx, via):
     """the tree is sized on the device: with bucket 4 the leaves outgrow the first arrays (sized for n / 8), the
     pass sees an empty tree and an abort flag, and the system builds again with grown arrays, found either by the
-    gravity call or by info() right after the build; tree and forces as the oracle's"""
+    gravity call or by info() right after the build. "hint": later builds launch for the last leaf count plus a
+    quarter; bucket 4, then 64, then 16 overflows that launch size inside the arrays. Tree and forces as the
+    oracle's"""
     s = W.uniform_cube(20000, seed=44)
     sysm = R.System(ctx, s.n)
     arrs = [dev(a) for a in s.arrays()]
     sysm.load(*arrs, s.limits)
-    prm = R.default_params(bucket_size=4)
-    sysm.build(prm)
-    if via == "info":
-        info = sysm.info()  # settles the build before anything reads the tree
-    pot = sysm.gravity(prm)
-    info = sysm.info()
+    buckets = [4, 64, 16] if via == "hint" else [4]
+    for b in buckets:
+        prm = R.default_params(bucket_size=b)
+        sysm.build(prm)
+        if via == "info":
+            sysm.info()  # settles the build before anything reads the tree
+        pot = sysm.gravity(prm)
+        info = sysm.info()
+        t = oracle_tree(s, bucket=b)
+        assert info["numLeaves"] == t["leaves"].size - 1
+    if via == "hint":
+        assert 4 * t["leaves"].size > 5 * oracle_tree(s, bucket=64)["leaves"].size
+    else:
+        assert info["numLeaves"] > s.n // 8
     p, ax, ay, az, order = (host(t) for t in sysm.results())
-    t = oracle_tree(s, bucket=4)
-    assert info["numLeaves"] == t["leaves"].size - 1 > s.n // 8
     assert np.array_equal(order, t["perm"])
     tr = sysm.tree()
     assert np.array_equal(host(tr["leaves"], np.uint64), t["leaves"])
