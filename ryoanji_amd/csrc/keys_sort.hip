@@ -340,6 +340,117 @@ __global__ __launch_bounds__(kSortThreads) void scatterKernel(const uint64_t* __
     }
 }
 
+// sorts of at most kBlockSortMax pairs in one launch: a single block of 1024 threads holds the pairs in registers
+// (8 per thread) and LDS for every pass: the wave-level match ranking of scatterKernel gives each wave's digit
+// counts, whose sums are the pass's histogram, and the stable scatter goes to LDS, from where each thread reloads
+// its 8 items. One global read and one write in all, and one launch instead of three per pass plus three, which is
+// most of a small sort's time.
+constexpr int kBlockSortThreads = 1024;
+constexpr int kBlockSortWaves   = kBlockSortThreads / 64;
+constexpr int kBlockSortItems   = 8;
+constexpr size_t kBlockSortMax  = size_t(kBlockSortThreads) * kBlockSortItems;
+
+__global__ __launch_bounds__(kBlockSortThreads) void blockSortKernel(uint64_t* __restrict__ keys,
+                                                                     int32_t* __restrict__ vals, size_t n,
+                                                                     const uint32_t* __restrict__ dN, int digits)
+{
+    __shared__ uint64_t sk[kBlockSortMax];
+    __shared__ int32_t sv[kBlockSortMax];
+    __shared__ uint32_t wcnt[kBlockSortWaves][kDigits]; // per-wave digit counts, then their prefixes over waves
+    __shared__ uint32_t offs[kDigits];                   // digit totals, then their exclusive scan
+    n = deviceCount(dN, n);
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const uint64_t ltMask = (1ull << lane) - 1ull;
+    // item r of thread (w, lane) is element w * 512 + r * 64 + lane: in wave order, then item order, then lanes
+    uint64_t k[kBlockSortItems];
+    int32_t v[kBlockSortItems];
+#pragma unroll
+    for (int r = 0; r < kBlockSortItems; ++r)
+    {
+        const size_t idx = size_t(w) * (kBlockSortItems * 64) + r * 64 + lane;
+        k[r] = idx < n ? keys[idx] : 0;
+        v[r] = idx < n ? vals[idx] : 0;
+    }
+    for (int p = 0; p < digits; ++p)
+    {
+        const int shift = 8 * p;
+        for (int i = threadIdx.x; i < kBlockSortWaves * kDigits; i += kBlockSortThreads)
+            (&wcnt[0][0])[i] = 0;
+        __syncthreads();
+        uint32_t rank[kBlockSortItems];
+#pragma unroll
+        for (int r = 0; r < kBlockSortItems; ++r)
+        {
+            const bool valid = size_t(w) * (kBlockSortItems * 64) + r * 64 + lane < n;
+            const unsigned d = unsigned(k[r] >> shift) & 255u;
+            uint64_t match = __ballot(valid);
+#pragma unroll
+            for (int b = 0; b < 8; ++b)
+            {
+                const bool bit = (d >> b) & 1u;
+                const uint64_t bb = __ballot(bit);
+                match &= bit ? bb : ~bb;
+            }
+            const uint32_t pre = valid ? wcnt[w][d] : 0;
+            rank[r] = pre + __popcll(match & ltMask);
+            if (valid && (match >> lane) == 1ull) wcnt[w][d] = pre + __popcll(match);
+        }
+        __syncthreads();
+        if (threadIdx.x < kDigits)
+        {
+            const int d = threadIdx.x;
+            uint32_t c = 0;
+            for (int ww = 0; ww < kBlockSortWaves; ++ww)
+            {
+                const uint32_t x = wcnt[ww][d];
+                wcnt[ww][d] = c;
+                c += x;
+            }
+            offs[d] = c;
+        }
+        __syncthreads();
+        if (w == 0)
+        {
+            // exclusive scan of the 256 digit totals, 4 per lane
+            const uint32_t c0 = offs[4 * lane], c1 = offs[4 * lane + 1], c2 = offs[4 * lane + 2],
+                           c3 = offs[4 * lane + 3];
+            const uint32_t sum = c0 + c1 + c2 + c3;
+            uint32_t inc = sum;
+            for (int o = 1; o < 64; o <<= 1)
+            {
+                const uint32_t t = __shfl_up(inc, o);
+                if (lane >= o) inc += t;
+            }
+            const uint32_t ex = inc - sum;
+            offs[4 * lane] = ex, offs[4 * lane + 1] = ex + c0, offs[4 * lane + 2] = ex + c0 + c1;
+            offs[4 * lane + 3] = ex + c0 + c1 + c2;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int r = 0; r < kBlockSortItems; ++r)
+            if (size_t(w) * (kBlockSortItems * 64) + r * 64 + lane < n)
+            {
+                const unsigned d  = unsigned(k[r] >> shift) & 255u;
+                const uint32_t at = offs[d] + wcnt[w][d] + rank[r];
+                sk[at] = k[r];
+                sv[at] = v[r];
+            }
+        __syncthreads();
+#pragma unroll
+        for (int r = 0; r < kBlockSortItems; ++r)
+        {
+            const size_t idx = size_t(w) * (kBlockSortItems * 64) + r * 64 + lane;
+            if (idx < n) k[r] = sk[idx], v[r] = sv[idx];
+        }
+    }
+#pragma unroll
+    for (int r = 0; r < kBlockSortItems; ++r)
+    {
+        const size_t idx = size_t(w) * (kBlockSortItems * 64) + r * 64 + lane;
+        if (idx < n) keys[idx] = k[r], vals[idx] = v[r];
+    }
+}
+
 __global__ void copyPairsKernel(const uint64_t* __restrict__ kin, const int32_t* __restrict__ vin, size_t n,
                                 const uint32_t* __restrict__ dN, uint64_t* __restrict__ kout,
                                 int32_t* __restrict__ vout)
@@ -417,13 +528,19 @@ int sortPairsDigits(Context* ctx, uint64_t* keys, int32_t* vals, size_t n, uint6
     if (digits < 1 || digits > kPasses) return setError(ctx, RYO_ERR_ARG, "sort: %d digits", digits);
     if (tmpBytes < sortTempBytes(n))
         return setError(ctx, RYO_ERR_ARG, "sort: temp storage too small (%zu < %zu)", tmpBytes, sortTempBytes(n));
+    hipStream_t s = ctx->stream;
+    if (n <= kBlockSortMax)
+    {
+        blockSortKernel<<<1, kBlockSortThreads, 0, s>>>(keys, vals, n, dN, digits);
+        RYO_LAUNCH_CHECK(ctx);
+        return RYO_OK;
+    }
     const int numTiles = int((n + kSortTile - 1) / kSortTile);
     uint32_t* hist     = reinterpret_cast<uint32_t*>(tmp);
     uint32_t* dbase    = hist + kPasses * kDigits;
     uint32_t* tileHist = dbase + kPasses * kDigits;
     const int numChunks = (numTiles + kScanChunk - 1) / kScanChunk;
     uint32_t* chunkSum  = tileHist + size_t(kDigits) * numTiles;
-    hipStream_t s = ctx->stream;
     zeroWordsKernel<<<8, 256, 0, s>>>(hist, kPasses * kDigits); // a kernel, not a memset node, under capture
     histAllKernel<<<gridFor(n, 256, ctx->numCUs * 4), 256, 0, s>>>(keys, n, dN, hist);
     digitBaseKernel<<<kPasses, 256, 0, s>>>(hist, dbase);

@@ -219,6 +219,39 @@ __global__ __launch_bounds__(256) void levelUpsweepKernel(UpsweepArgsT<Tin> a, c
         levelNodeOne<MDQ>(a, node, threadIdx.x & 7);
 }
 
+// a small tree in one launch: one block of 1024 threads (128 groups of 8 lanes) forms the leaves, then every level
+// from the deepest possible parent level to the root, with a barrier between levels (the waves of one work-group
+// share the CU's L1, so a level's stores are visible to the next level's loads after it). The same per-node
+// arithmetic as leafUpsweepKernel and levelUpsweepKernel, so the results are identical; it saves the 21 level
+// launches that dominate a small tree's upsweep.
+constexpr int kBlockUpsweepThreads = 1024;
+constexpr int kBlockUpsweepLeaves  = 2048;
+
+template<class Tin, bool MDQ = false>
+__global__ __launch_bounds__(kBlockUpsweepThreads) void blockUpsweepKernel(UpsweepArgsT<Tin> a,
+                                                                          const int32_t* __restrict__ levelRange)
+{
+    if (a.dStat)
+    {
+        a.numLeaves = min(int(a.dStat[kStLc]), a.numLeaves);
+        a.leafNode += a.dStat[kStInternal];
+    }
+    constexpr int groups = kBlockUpsweepThreads / 8;
+    __shared__ int32_t lr[kMaxLevel + 2];
+    if (threadIdx.x < kMaxLevel + 2) lr[threadIdx.x] = levelRange[threadIdx.x];
+    const int group = threadIdx.x >> 3, sub = threadIdx.x & 7;
+    for (int base = 0; base < a.numLeaves; base += groups)
+        leafUpsweepOne<MDQ>(a, base + group, base + group < a.numLeaves, sub);
+    __syncthreads();
+    for (int l = int(kMaxLevel) - 1; l >= 0; --l)
+    {
+        const int first = lr[l], last = lr[l + 1];
+        for (int node = first + group; node < last; node += groups)
+            levelNodeOne<MDQ>(a, node, sub);
+        __syncthreads();
+    }
+}
+
 /* ---------------------------------------------------------------- batches of systems */
 
 // the arguments of system s: the batch-wide arrays offset to the system's slices (local indices inside)
@@ -337,6 +370,12 @@ static int upsweepT(Context* ctx, UpsweepArgsT<Tin> a, int L, const int32_t* lev
     a.box.invTheta  = invTheta;
     a.box.softReach = softReach;
     hipStream_t s = ctx->stream;
+    if (L <= kBlockUpsweepLeaves)
+    {
+        blockUpsweepKernel<Tin, MDQ><<<1, kBlockUpsweepThreads, 0, s>>>(a, levelRange);
+        RYO_LAUNCH_CHECK(ctx);
+        return RYO_OK;
+    }
     leafUpsweepKernel<Tin, MDQ><<<std::max(1u, std::min(iceil(size_t(L) * 8, 256), unsigned(ctx->numCUs * 64))), 256,
                                    0, s>>>(a);
     const unsigned levelBlocks = std::max(1u, std::min(iceil(size_t(nI) * 8, 256), unsigned(ctx->numCUs * 4)));

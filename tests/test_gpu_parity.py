@@ -56,7 +56,8 @@ def test_keys_remove_key_untouched(ctx):
     assert np.all(g[::7] == np.uint64(1 << 63))
 
 
-@pytest.mark.parametrize("n", [1, 2, 100, 4095, 4096, 4097, 70001, 1 << 20])
+# up to 32768 pairs the sort is one block running every pass; beyond, one launch per step of each pass
+@pytest.mark.parametrize("n", [1, 2, 100, 4095, 4096, 4097, 8191, 8192, 8193, 32767, 32768, 32769, 70001, 1 << 20])
 def test_sort_stable_bit_exact(ctx, n):
     rng = np.random.default_rng(n)
     # few distinct values in the high bytes and ties: stability matters
@@ -68,9 +69,10 @@ def test_sort_stable_bit_exact(ctx, n):
     assert np.array_equal(gv, ov)
 
 
-def test_sort_full_64bit(ctx):
+@pytest.mark.parametrize("n", [200000, 30000])
+def test_sort_full_64bit(ctx, n):
     rng = np.random.default_rng(9)
-    keys = rng.integers(0, np.iinfo(np.uint64).max, 200000, dtype=np.uint64, endpoint=True)
+    keys = rng.integers(0, np.iinfo(np.uint64).max, n, dtype=np.uint64, endpoint=True)
     keys[::3] = keys[1::3][: keys[::3].size]
     gk, gv = gpu_sort(ctx, keys)
     ok, ov = O.sort_by_key(keys)
