@@ -188,6 +188,12 @@ def lib() -> C.CDLL:
     if not os.path.exists(LIB_PATH):
         raise RuntimeError(f"{LIB_PATH} is missing: build it with `make -C ryoanji_amd/csrc` "
                            "(or __graft_entry__.build())")
+    # torch ships its own HIP runtime under the soname libamdhip64.so.7, the soname the library links against
+    # (/opt/rocm's). Loaded after torch, the library binds to torch's runtime; loaded first, it would bring in
+    # /opt/rocm's copy, torch would then load its own beside it, and that second runtime finds no GPU. Importing
+    # torch here keeps one runtime per process whatever the caller imports first.
+    import torch  # noqa: F401
+
     L = C.CDLL(LIB_PATH)
     for name, (res, args) in {**SIGNATURES, **EXTRA_SIGNATURES}.items():
         f = getattr(L, name)

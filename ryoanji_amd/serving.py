@@ -29,7 +29,10 @@ def make_stream(num, rate_per_s, seed=0, lo_exp=12, hi_exp=22, steps=(1, 20), th
 
 class Server:
     """slots resident-system workers and a batch lane for requests of at most batch_max_n particles (None: the
-    library's default; 0: no lane, every request on a slot)"""
+    library's default; 0: no lane, every request on a slot). Each slot and the lane run on a stream of their own:
+    give the process at least slots + 1 hardware queues (the environment variable GPU_MAX_HW_QUEUES, read when HIP
+    initialises; HIP's default is 4, shared round-robin between streams), or a stream's short kernels wait behind
+    another's long ones in a shared queue."""
 
     def __init__(self, slots, max_n, hbm_budget=0, batch_max_n=None, batch_capacity=1 << 22, batch_systems=1024):
         self.h = _P()
