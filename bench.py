@@ -30,7 +30,8 @@ sys.path.insert(0, ROOT)
 # the serving leg runs 8 slot streams and the batch lane's stream in this process: one hardware queue per stream
 # (HIP's default is 4 per process, shared round-robin) keeps a stream's small kernels from queueing behind another
 # stream's long ones; read once, when HIP initialises, so it is set before anything touches the GPU
-os.environ.setdefault("GPU_MAX_HW_QUEUES", "16")
+if int(os.environ.get("GPU_MAX_HW_QUEUES", "4") or 4) < 16:
+    os.environ["GPU_MAX_HW_QUEUES"] = "16"
 
 PEAK_FP32_TFLOPS = 157.3  # MI355X FP32 vector peak (AMD published)
 PEAK_FP32_UNPACKED_TFLOPS = 78.6  # the same without packed FP32 (256 CUs x 4 SIMDs x 16 lanes x 2 flops x 2.4 GHz)

@@ -475,11 +475,14 @@ int ryo_batch_load(ryo_batch* batch, int num_systems, const uint32_t* counts, co
 int ryo_batch_set_periodic(ryo_batch* batch, const int* periodic);
 /* one full pass for every system; theta[S] and G[S] per system (host; NULL: prm's value), pot[S] (host, may be
  * NULL) receives each system's 0.5 G sum(m phi) (with its Ewald correction's for a periodic system). prm:
- * bucket_size and group_size (<= 64); num_shells, ewald and ewald_settings for the periodic systems. */
+ * bucket_size and group_size (<= 64); num_shells, ewald and ewald_settings for the periodic systems. The first
+ * solve of a batch object reads the leaf counts back to size the node arrays exactly and sets capacities (its
+ * totals plus a margin); later solves launch every stage for the capacities and read the overflow flag, tables,
+ * statistics and potentials back once at the end, solving again exactly (and growing the capacities) when the
+ * trees outgrew them. The results are the same either way, bit for bit. */
 int ryo_batch_solve(ryo_batch* batch, const ryo_params* prm, const float* theta, const float* G, double* pot);
 /* The same solve as ryo_batch_solve, bit for bit, run as one captured HIP graph (keys -> sort -> leaves ->
- * octrees -> upsweep -> groups -> traversal, no host round trip inside; ryo_batch_solve reads the leaf counts
- * back once to size the node arrays). The graph is captured on the first call and again when the system
+ * octrees -> upsweep -> groups -> traversal, no host round trip inside). The graph is captured on the first call and again when the system
  * counts, bucket or group size, init node, traversal shape or a buffer changed; positions, boxes, theta and G
  * may change between replays. Its node arrays hold the capturing solve's totals plus a margin; a replay whose
  * trees outgrow them is detected on the device and solved again exactly (counted in the statistics). The
@@ -488,7 +491,7 @@ int ryo_batch_solve(ryo_batch* batch, const ryo_params* prm, const float* theta,
 int ryo_batch_solve_graph(ryo_batch* batch, const ryo_params* prm, const float* theta, const float* G,
                           double* pot);
 /* out[4] = {graphs captured, replays, replays solved again because the trees outgrew the graph's arrays,
- * leaf-slot capacity of the current graph} */
+ * leaf-slot capacity of the node arrays (the current graph's; ryo_batch_solve's after its first call)} */
 int ryo_batch_graph_stats(ryo_batch* batch, uint64_t* out);
 /* device arrays, packed like the input but each system in its SFC order: out[0..4] = x, y, z, m, h,
  * out[5..8] = p, ax, ay, az; *order = index into the loaded (packed) arrays */

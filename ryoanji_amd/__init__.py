@@ -425,7 +425,8 @@ class Batch:
 
     def graph_stats(self):
         """{captures, replays, fallbacks, capacity}: graphs captured, replays, replays solved again exactly
-        because the trees outgrew the captured arrays, leaf-slot capacity of the current graph"""
+        because the trees outgrew the captured arrays, leaf-slot capacity of the node arrays (the current graph's,
+        which the eager solve after its first call uses too)"""
         import numpy as np
 
         out = np.zeros(4, np.uint64)

@@ -11,8 +11,10 @@
 // potentials. Every index stored in a system's slice is local to that system, so each system gets exactly the
 // keys, tree, multipoles and forces it gets when solved on its own.
 //
-// Two ways to run it. ryo_batch_solve reads the leaf-slot offsets back once to size the node arrays exactly.
-// ryo_batch_solve_graph runs the whole sequence as one captured HIP graph with no host round trip inside: node
+// Two ways to run it. ryo_batch_solve sizes the node arrays exactly on its first call (the leaf-slot offsets are
+// read back once) and after that for capacities, launching every stage without a host round trip and reading
+// the overflow flag, tables, statistics and potentials once at the end (a solve that outgrew them is redone
+// exactly). ryo_batch_solve_graph runs the whole sequence as one captured HIP graph with no host round trip inside: node
 // arrays are sized for a capacity (the totals of the solve that captured the graph plus a margin), the node
 // stages read their counts from the device tables, and a replay whose trees outgrow the capacity is detected
 // on the device (every later stage then sees empty trees) and solved again the first way; the graph is
@@ -139,6 +141,12 @@ struct ryo_batch
     uint64_t graphBufs = 0;
     uint32_t capSlots = 0, capNodes = 0, capParents = 0;
     uint64_t captures = 0, replays = 0, fallbacks = 0;
+    // the eager solve on capacity-sized arrays: pinned memory its one read lands in, and the solves it had to
+    // redo the exact way because the trees outgrew the capacities
+    void* hRead            = nullptr;
+    uint64_t sizedFallbacks = 0;
+    int capBasisS          = 0; // systems and particles of the exact solve the capacities were taken from
+    size_t capBasisN       = 0;
 
     uint32_t* T(int k) const { return tab.as<uint32_t>() + size_t(k) * (capS + 1); }
     BatchView view() const
@@ -383,7 +391,12 @@ void hostNodeTables(ryo_batch* b)
     b->totalParents  = b->hPO[S];
 }
 
-// the exact solve: the leaf-slot offsets are read back once to size the node arrays
+int capacities(ryo_batch* b, uint32_t* capSlots, uint32_t* capNodes, uint32_t* capParents);
+
+// the exact solve: the leaf-slot offsets are read back once to size the node arrays. The arrays are grown to the
+// capacities these totals give (not only to the totals), so the capacity-sized solves and graphs that follow
+// find them large enough: growing an array frees the old one, and hipFree waits for the whole device, the other
+// streams' work included
 int solveEager(ryo_batch* b, const Shape& sh)
 {
     Context* ctx = b->ctx;
@@ -394,7 +407,9 @@ int solveEager(ryo_batch* b, const Shape& sh)
     RYO_HIP(ctx, hipMemcpyAsync(b->hLO.data(), b->T(2), (S + 1) * 4, hipMemcpyDeviceToHost, ctx->stream));
     RYO_HIP(ctx, hipStreamSynchronize(ctx->stream));
     hostNodeTables(b);
-    if ((rc = ensureNodes(b, b->totalSlots, b->totalNodes, b->totalParents))) return rc;
+    uint32_t cs, cn, cp;
+    if ((rc = capacities(b, &cs, &cn, &cp))) return rc;
+    if ((rc = ensureNodes(b, cs, cn, cp))) return rc;
     return runBack(b, sh, b->totalSlots, b->totalNodes, b->totalInternal, b->totalParents, false);
 }
 
@@ -461,12 +476,16 @@ int batchEwald(ryo_batch* b, const ryo_params* prm, std::vector<double>& ewPot)
 }
 
 // per-system potentials to the host (plus the Ewald potentials of periodic systems, when given)
-int finishSolve(ryo_batch* b, const Shape&, double* pot, const std::vector<double>* ewPot = nullptr)
+int finishSolve(ryo_batch* b, const Shape&, double* pot, const std::vector<double>* ewPot = nullptr,
+                bool potRead = false)
 {
     Context* ctx = b->ctx;
-    b->hPot.resize(b->S);
-    RYO_HIP(ctx, hipMemcpyAsync(b->hPot.data(), b->sysPot.p, b->S * 8, hipMemcpyDeviceToHost, ctx->stream));
-    RYO_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    if (!potRead)
+    {
+        b->hPot.resize(b->S);
+        RYO_HIP(ctx, hipMemcpyAsync(b->hPot.data(), b->sysPot.p, b->S * 8, hipMemcpyDeviceToHost, ctx->stream));
+        RYO_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    }
     if (ewPot)
         for (int s = 0; s < b->S; ++s)
             b->hPot[s] += (*ewPot)[s];
@@ -507,18 +526,88 @@ void dropGraph(ryo_batch* b)
     b->graphKey.clear();
 }
 
-// capture the whole solve (after the host tables) as one graph, with node arrays sized for the last exact
-// solve's totals plus a quarter and 8 slots per system
+// node-array capacities from the last exact solve's totals: a quarter more and 8 leaf slots per system (the exact
+// solve grows the arrays to them before it writes them)
+int capacities(ryo_batch* b, uint32_t* capSlots, uint32_t* capNodes, uint32_t* capParents)
+{
+    const uint64_t slots64 = uint64_t(b->totalSlots) + b->totalSlots / 4 + 8ull * uint64_t(b->S) + 64;
+    if (slots64 > 0x7fffffffull) return setError(b->ctx, RYO_ERR_CAPACITY, "batch: too many leaf slots");
+    *capSlots   = uint32_t(slots64);
+    *capNodes   = *capSlots + *capSlots / 7 + uint32_t(b->S);
+    *capParents = *capNodes / 8 + uint32_t(b->S);
+    return RYO_OK;
+}
+
+// the eager solve with no host round trip before its end (the stages a graph captures, launched for the
+// capacities), then one read of the overflow flag, the leaf-slot/node/internal/parent tables, the traversal's
+// statistics and the per-system potentials. The first solve, and one whose trees outgrew the capacities (every
+// stage after the tables then saw empty trees), run the exact way and set the capacities for the next ones; so
+// does a batch with more systems or particles than the one the capacities were taken from (a member joined the
+// serving lane), whose trees would most likely not fit. *potRead: the potentials are in hPot. Groups that
+// overflowed a shallow LIFO are retried by the traversal's own finish.
+int solveSized(ryo_batch* b, const Shape& sh, bool* potRead)
+{
+    Context* ctx   = b->ctx;
+    const int S    = b->S;
+    hipStream_t st = ctx->stream;
+    *potRead       = false;
+    int rc;
+    auto exact = [&]() {
+        int r = solveEager(b, sh);
+        if (!r) r = capacities(b, &b->capSlots, &b->capNodes, &b->capParents);
+        b->capBasisS = S, b->capBasisN = b->N;
+        return r;
+    };
+    if (b->capSlots == 0 || b->capNodes == 0 || S > b->capBasisS || b->N > b->capBasisN) return exact();
+    if ((rc = ensureNodes(b, b->capSlots, b->capNodes, b->capParents))) return rc;
+    if ((rc = runFront(b, sh, b->capSlots, b->capNodes, b->capParents))) return rc;
+    if ((rc = runBack(b, sh, b->capSlots, b->capNodes, b->capSlots / 7 + 1, b->capParents, true))) return rc;
+    const size_t S1 = size_t(S) + 1;
+    // layout of the read: stats[7] (u64), potentials[S] (f64), overflow flag, tables LO, NO, NIO, PO (u32)
+    const size_t bytes = 7 * 8 + size_t(b->capS) * 8 + 4 + 4 * (size_t(b->capS) + 1) * 4;
+    if (!b->hRead && hipHostMalloc(&b->hRead, bytes) != hipSuccess)
+    {
+        b->hRead = nullptr;
+        return setError(ctx, RYO_ERR_HIP, "batch: pinned host memory");
+    }
+    auto* hs   = static_cast<unsigned long long*>(b->hRead);
+    auto* hp   = reinterpret_cast<double*>(hs + 7);
+    auto* hf   = reinterpret_cast<uint32_t*>(hp + b->capS);
+    auto* htab = hf + 1;
+    RYO_HIP(ctx, hipMemcpyAsync(hs, ctx->dStats + 8, 7 * 8, hipMemcpyDeviceToHost, st));
+    RYO_HIP(ctx, hipMemcpyAsync(hp, b->sysPot.p, size_t(S) * 8, hipMemcpyDeviceToHost, st));
+    RYO_HIP(ctx, hipMemcpyAsync(hf, b->flags.p, 4, hipMemcpyDeviceToHost, st));
+    for (int k = 0; k < 4; ++k)
+        RYO_HIP(ctx, hipMemcpyAsync(htab + k * S1, b->T(2 + k), S1 * 4, hipMemcpyDeviceToHost, st));
+    RYO_HIP(ctx, hipStreamSynchronize(st));
+    if (*hf)
+    {
+        ++b->sizedFallbacks;
+        return exact();
+    }
+    b->hLO.assign(htab, htab + S1);
+    b->hNO.assign(htab + S1, htab + 2 * S1);
+    b->hNIO.assign(htab + 2 * S1, htab + 3 * S1);
+    b->hPO.assign(htab + 3 * S1, htab + 4 * S1);
+    b->totalSlots = b->hLO[S], b->totalNodes = b->hNO[S];
+    b->totalInternal = b->hNIO[S], b->totalParents = b->hPO[S];
+    if ((hs[5] & 0xffffffffu) != 0 || hs[6] != 0) return traversal(b, sh, 2, nullptr);
+    for (int k = 0; k < 6; ++k)
+        ctx->hStats[k] = k < 5 ? hs[k] : 0;
+    ctx->retriedGroups = 0;
+    b->hPot.assign(hp, hp + S);
+    *potRead = true;
+    return RYO_OK;
+}
+
+// capture the whole solve (after the host tables) as one graph, with node arrays sized for capacities
 int capture(ryo_batch* b, const Shape& sh)
 {
     Context* ctx = b->ctx;
     dropGraph(b);
-    const uint64_t slots64 = uint64_t(b->totalSlots) + b->totalSlots / 4 + 8ull * uint64_t(b->S) + 64;
-    if (slots64 > 0x7fffffffull) return setError(ctx, RYO_ERR_CAPACITY, "batch graph: too many leaf slots");
-    const uint32_t capSlots = uint32_t(slots64);
-    const uint32_t capNodes = capSlots + capSlots / 7 + uint32_t(b->S);
-    const uint32_t capParents = capNodes / 8 + uint32_t(b->S);
-    int rc = ensureNodes(b, capSlots, capNodes, capParents);
+    uint32_t capSlots, capNodes, capParents;
+    int rc = capacities(b, &capSlots, &capNodes, &capParents);
+    if (!rc) rc = ensureNodes(b, capSlots, capNodes, capParents);
     if (rc) return rc;
     if (!b->capStream)
         RYO_HIP(ctx, hipStreamCreateWithFlags(&b->capStream, hipStreamNonBlocking));
@@ -553,6 +642,7 @@ int capture(ryo_batch* b, const Shape& sh)
     b->graphKey  = graphKey(b, sh);
     b->graphBufs = bufferSignature(b);
     b->capSlots = capSlots, b->capNodes = capNodes, b->capParents = capParents;
+    b->capBasisS = b->S, b->capBasisN = b->N;
     ++b->captures;
     return RYO_OK;
 }
@@ -606,6 +696,14 @@ int ryo_batch_create(ryo_ctx* h, size_t max_particles, int max_systems, ryo_batc
     if (!rc) rc = b->G.ensure(ctx, S1 * 4);
     if (!rc) rc = b->sysPot.ensure(ctx, S1 * 8);
     if (!rc) rc = b->blockSums.ensure(ctx, (n / kBatchLeafTile + S1 + 1) * 4);
+    // node arrays for a typical tree of the full capacity (n / 8 leaf slots and 8 per system), so that solves
+    // rarely grow them (a growth frees the old array, which waits for the whole device)
+    if (!rc)
+    {
+        const uint64_t slots = n / 8 + 8 * S1 + 64;
+        const uint64_t nodes = slots + slots / 7 + S1;
+        if (nodes < 0x7fffffffull) rc = ensureNodes(b, uint32_t(slots), uint32_t(nodes), uint32_t(nodes / 8 + S1));
+    }
     if (rc)
     {
         ryo_batch_destroy(b);
@@ -621,6 +719,7 @@ int ryo_batch_destroy(ryo_batch* b)
     hipStreamSynchronize(b->ctx->stream);
     dropGraph(b);
     if (b->capStream) hipStreamDestroy(b->capStream);
+    if (b->hRead) hipHostFree(b->hRead);
     std::vector<DevBuf*> bufs;
     allBufs(b, bufs);
     for (DevBuf* d : bufs)
@@ -676,10 +775,11 @@ int ryo_batch_solve(ryo_batch* b, const ryo_params* prm, const float* theta, con
         return setError(b->ctx, RYO_ERR_UNSUPPORTED, "batch: CartesianQuadrupole multipoles only");
     Shape sh;
     std::vector<double> ewPot;
+    bool potRead = false;
     int rc = uploadTables(b, prm, theta, Gs, sh);
-    if (!rc) rc = solveEager(b, sh);
+    if (!rc) rc = solveSized(b, sh, &potRead);
     if (!rc && sh.periodic) rc = batchEwald(b, prm, ewPot);
-    if (!rc) rc = finishSolve(b, sh, pot, sh.periodic ? &ewPot : nullptr);
+    if (!rc) rc = finishSolve(b, sh, pot, sh.periodic ? &ewPot : nullptr, potRead);
     return rc;
 }
 

@@ -598,7 +598,7 @@ struct ryo_system
     ryo_params buildPrm{};      // the parameters of the last build (to build again when its leaves outgrew)
     bool regrowing     = false; // inside the second pass after an outgrown build
     // leaves a device-sized build launches for: the last leaf count with a margin (the particles of a loaded system
-    // change little between builds), 0 for the whole capacity; a build with more leaves overflows and runs again
+    // change little between builds), 0 for n / 8; a build with more leaves overflows and runs again
     size_t leafHint    = 0;
     hipEvent_t ev[9] = {};
     double phaseMs[8] = {};
@@ -689,6 +689,9 @@ int ryo_system_create(ryo_ctx* h, size_t capacity, ryo_system** out)
     for (auto& e : s->ev)
         if (!rc && hipEventCreate(&e) != hipSuccess) rc = setError(ctx, RYO_ERR_HIP, "system: event creation");
     if (!rc) rc = devAlloc(s, &s->dStat, kStWords);
+    // tree arrays for n / 8 leaves of the full capacity: a build rarely grows them later (a growth frees the old
+    // arrays, and hipFree waits for the whole device, other streams' work included)
+    if (!rc) rc = growTree(s, int(std::min<size_t>(std::max<size_t>(64, capacity / 8), 0x7fffffff)));
     if (!rc && hipHostMalloc(reinterpret_cast<void**>(&s->hStat), kStWords * sizeof(uint32_t)) != hipSuccess)
         rc = setError(ctx, RYO_ERR_HIP, "system: pinned host memory");
     if (rc)
@@ -910,7 +913,10 @@ int ryo_system_build(ryo_system* s, const ryo_params* prm)
     s->buildPrm    = *prm;
     if (deviceSized)
     {
-        const size_t cap = s->leafHint ? std::min(s->leafHint, s->leafCap) : s->leafCap;
+        // launches for the last leaf count plus a quarter, or n / 8 leaves when none is known yet (the arrays
+        // may be far larger: they are sized for the system's capacity)
+        const size_t want = s->leafHint ? s->leafHint : std::max<size_t>(64, n / 8);
+        const size_t cap  = std::min(want, s->leafCap);
         if ((rc = buildLeavesDevice(ctx, s->keys, n, prm->bucket_size, s->leaves, s->counts, s->layout, cap,
                                     s->dStat)))
             return rc;

@@ -377,6 +377,54 @@ def test_graph_solve_outgrowing_its_arrays(ctx):
     b.close()
 
 
+def _fresh_exact(ctx, systems, prm, thetas=None):
+    """the first solve of a new batch object: node arrays sized exactly from the read-back leaf-slot offsets"""
+    b = R.Batch(ctx, sum(s.n for s in systems), len(systems))
+    b.load(systems)
+    pot = b.solve(prm, theta=thetas)
+    out = _batch_outputs(b)
+    b.close()
+    return pot, out
+
+
+def test_sized_solve_equals_exact(ctx):
+    """after its first solve a batch launches every stage for capacities and reads back once at the end: new
+    positions, fewer systems, then trees that outgrow the capacities (solved again exactly, capacities grown), each
+    bit for bit the exact solve of a fresh batch"""
+    systems = _systems()
+    prm = R.default_params()
+    thetas = np.array([0.4, 0.5, 0.7, 0.6] * 4, np.float32)[: len(systems)]
+    sizes = [4096] * 12
+    flat = [W.uniform_cube(n, seed=70 + i) for i, n in enumerate(sizes)]
+    deep = _clumped(sizes, 90)
+    b = R.Batch(ctx, max(sum(s.n for s in systems), sum(sizes)), max(len(systems), len(sizes)))
+    b.load(systems)
+    b.solve(prm, theta=thetas)  # exact; sets the capacities
+    cap0 = b.graph_stats()["capacity"]
+    assert cap0 > 0
+    moved = [W.System(s.x * np.float32(0.5), s.y, s.z, s.m, s.h, s.limits) for s in systems]
+    for batch, th in ((moved, thetas[::-1].copy()), (systems[:3], thetas[:3].copy())):
+        b.load(batch)
+        pot = b.solve(prm, theta=th)  # for the capacities, one read at the end
+        pe, ref = _fresh_exact(ctx, batch, prm, th)
+        _assert_same(ref, _batch_outputs(b))
+        assert np.array_equal(pot, pe)
+    b.load(flat)
+    b.solve(prm)
+    cap1 = b.graph_stats()["capacity"]
+    b.load(deep)
+    pot = b.solve(prm)  # outgrows the capacities: solved again exactly
+    pe, ref = _fresh_exact(ctx, deep, prm)
+    assert int(ref["off_LO"][-1]) > cap1
+    _assert_same(ref, _batch_outputs(b))
+    assert np.array_equal(pot, pe)
+    assert b.graph_stats()["capacity"] > int(ref["off_LO"][-1])  # grown for the next solves
+    pot = b.solve(prm)  # within the grown capacities
+    _assert_same(ref, _batch_outputs(b))
+    assert np.array_equal(pot, pe)
+    b.close()
+
+
 def test_shallow_lifo_retries_at_full_depth(ctx):
     """with a LIFO too shallow for some groups, those groups are traversed again at full depth and every
     result equals the full-depth solve, in both ways of solving"""
