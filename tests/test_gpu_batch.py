@@ -164,6 +164,11 @@ def test_periodic_batch_equals_systems_alone(ctx, ewald):
         assert abs(pots[s] - pot) <= 1e-12 * abs(pot), (s, pots[s], pot)
     with pytest.raises(R.RyoError):
         b.solve(prm, theta=thetas, G=Gs, graph=True)  # the captured graph takes open systems only
+    # the second solve of the batch runs for capacities with one read at the end: the same arrays and potentials
+    first = _batch_outputs(b)
+    pots2 = b.solve(prm, theta=thetas, G=Gs)
+    _assert_same(first, _batch_outputs(b))
+    assert np.array_equal(pots, pots2)
     b.close()
 
 
