@@ -192,12 +192,15 @@ __global__ __launch_bounds__(256) void tileChunkSumKernel(const uint32_t* __rest
     chunkSum[size_t(c) * kDigits + d] = sum;
 }
 
+// digitBase null: offsets from the start of the digit's run, and totals[d] = keys of digit d (the scatter then adds
+// the runs' starts)
 __global__ __launch_bounds__(256) void tileChunkScanKernel(uint32_t* __restrict__ chunkSum, int numChunks,
-                                                      const uint32_t* __restrict__ digitBase)
+                                                      const uint32_t* __restrict__ digitBase,
+                                                      uint32_t* __restrict__ totals)
 {
     __shared__ uint32_t sw[4];
     const int d = blockIdx.x;
-    uint32_t carry = digitBase[d];
+    uint32_t carry = digitBase ? digitBase[d] : 0u;
     for (int c0 = 0; c0 < numChunks; c0 += 256)
     {
         const int c = c0 + threadIdx.x;
@@ -207,16 +210,18 @@ __global__ __launch_bounds__(256) void tileChunkScanKernel(uint32_t* __restrict_
         if (c < numChunks) chunkSum[size_t(c) * kDigits + d] = carry + ex;
         carry += tot;
     }
+    if (totals && threadIdx.x == 0) totals[d] = carry;
 }
 
 // the same offsets in one launch for few tiles (one block per digit; its reads are strided, which costs little
 // when there are few tiles and saves two launches per pass)
 __global__ __launch_bounds__(256) void tileScanSmallKernel(uint32_t* __restrict__ tileHist, int numTiles,
-                                                          const uint32_t* __restrict__ digitBase)
+                                                          const uint32_t* __restrict__ digitBase,
+                                                          uint32_t* __restrict__ totals)
 {
     __shared__ uint32_t sw[4];
     const int d = blockIdx.x;
-    uint32_t carry = digitBase[d];
+    uint32_t carry = digitBase ? digitBase[d] : 0u;
     for (int t0 = 0; t0 < numTiles; t0 += 256)
     {
         const int t = t0 + threadIdx.x;
@@ -226,6 +231,7 @@ __global__ __launch_bounds__(256) void tileScanSmallKernel(uint32_t* __restrict_
         if (t < numTiles) tileHist[size_t(t) * kDigits + d] = carry + ex;
         carry += tot;
     }
+    if (totals && threadIdx.x == 0) totals[d] = carry;
 }
 
 __global__ __launch_bounds__(256) void tileOffsetKernel(uint32_t* __restrict__ tileHist, int numTiles,
@@ -252,29 +258,30 @@ __device__ inline int xcdTile(int b, int numTiles)
     return x * per + min(x, extra) + k;
 }
 
-// stable scatter of one tile: wave-level match ranking (8 ballots per key), block-level offsets, staging in
-// LDS so that the writes of one digit run are contiguous
-__global__ __launch_bounds__(kSortThreads) void scatterKernel(const uint64_t* __restrict__ keysIn,
-                                                             const int32_t* __restrict__ valsIn,
-                                                             uint64_t* __restrict__ keysOut,
-                                                             int32_t* __restrict__ valsOut, size_t n,
-                                                             const uint32_t* __restrict__ dN, int shift,
-                                                             const uint32_t* __restrict__ tileOffsets, int numTiles)
+// LDS of one tile's scatter
+struct ScatterLds
 {
-    n = deviceCount(dN, n);
-    const int tile = xcdTile(blockIdx.x, numTiles);
-    if (size_t(tile) * kSortTile >= n) return; // a tile past the count (the whole block leaves)
-    __shared__ uint64_t sk[kSortTile];
-    __shared__ int32_t sv[kSortTile];
-    __shared__ uint32_t wcnt[4][kDigits];
-    __shared__ uint32_t tstart[kDigits];
-    __shared__ uint32_t gbase[kDigits];
-    __shared__ uint32_t sw[4];
+    uint64_t sk[kSortTile];
+    int32_t sv[kSortTile];
+    uint32_t wcnt[4][kDigits];
+    uint32_t tstart[kDigits];
+    uint32_t gbase[kDigits];
+    uint32_t sw[4];
+};
 
-    const size_t base = size_t(tile) * kSortTile;
+// stable scatter of the nValid (<= kSortTile) pairs from `base`: wave-level match ranking (8 ballots per key),
+// block-level offsets, staging in LDS so that the writes of one digit run are contiguous. offs[d] is where this
+// tile's first pair of digit d goes; a destination at or past nOut (offsets that do not describe these keys) is
+// not written and raises *err instead. Starts and ends with the block in step, so that a block can run it for
+// one tile after another.
+__device__ __forceinline__ void scatterTile(const uint64_t* __restrict__ keysIn, const int32_t* __restrict__ valsIn,
+                                   uint64_t* __restrict__ keysOut, int32_t* __restrict__ valsOut, size_t base,
+                                   int nValid, int shift, const uint32_t* __restrict__ offs,
+                                   const uint32_t* __restrict__ digitBase, size_t nOut, uint32_t* err, ScatterLds& L)
+{
     const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
     for (int i = threadIdx.x; i < 4 * kDigits; i += kSortThreads)
-        (&wcnt[0][0])[i] = 0;
+        (&L.wcnt[0][0])[i] = 0;
     __syncthreads();
 
     uint64_t k[kSortItems];
@@ -284,10 +291,10 @@ __global__ __launch_bounds__(kSortThreads) void scatterKernel(const uint64_t* __
 #pragma unroll
     for (int r = 0; r < kSortItems; ++r)
     {
-        size_t idx = base + size_t(w) * (kSortItems * 64) + r * 64 + lane;
-        bool valid = idx < n;
-        k[r] = valid ? keysIn[idx] : 0;
-        v[r] = valid ? valsIn[idx] : 0;
+        const int idx    = w * (kSortItems * 64) + r * 64 + lane;
+        const bool valid = idx < nValid;
+        k[r] = valid ? keysIn[base + idx] : 0;
+        v[r] = valid ? valsIn[base + idx] : 0;
         unsigned d = unsigned(k[r] >> shift) & 255u;
         uint64_t match = __ballot(valid);
 #pragma unroll
@@ -297,91 +304,128 @@ __global__ __launch_bounds__(kSortThreads) void scatterKernel(const uint64_t* __
             uint64_t bb = __ballot(bit);
             match &= bit ? bb : ~bb;
         }
-        uint32_t pre = valid ? wcnt[w][d] : 0;
+        uint32_t pre = valid ? L.wcnt[w][d] : 0;
         rank[r] = pre + __popcll(match & ltMask);
         bool leader = valid && (match >> lane) == 1ull;
-        if (leader) wcnt[w][d] = pre + __popcll(match);
+        if (leader) L.wcnt[w][d] = pre + __popcll(match);
     }
     __syncthreads();
     {
         int d = threadIdx.x;
-        uint32_t c0 = wcnt[0][d], c1 = wcnt[1][d], c2 = wcnt[2][d], c3 = wcnt[3][d];
+        uint32_t c0 = L.wcnt[0][d], c1 = L.wcnt[1][d], c2 = L.wcnt[2][d], c3 = L.wcnt[3][d];
         uint32_t tot;
-        uint32_t start = blockExclusiveScan256(c0 + c1 + c2 + c3, sw, &tot);
-        wcnt[0][d] = 0;
-        wcnt[1][d] = c0;
-        wcnt[2][d] = c0 + c1;
-        wcnt[3][d] = c0 + c1 + c2;
-        tstart[d] = start;
-        gbase[d] = tileOffsets[size_t(tile) * kDigits + d];
+        uint32_t start = blockExclusiveScan256(c0 + c1 + c2 + c3, L.sw, &tot);
+        L.wcnt[0][d] = 0;
+        L.wcnt[1][d] = c0;
+        L.wcnt[2][d] = c0 + c1;
+        L.wcnt[3][d] = c0 + c1 + c2;
+        L.tstart[d] = start;
+        L.gbase[d]  = offs[d] + (digitBase ? digitBase[d] : 0u);
     }
     __syncthreads();
 #pragma unroll
     for (int r = 0; r < kSortItems; ++r)
     {
-        size_t idx = base + size_t(w) * (kSortItems * 64) + r * 64 + lane;
-        if (idx < n)
+        const int idx = w * (kSortItems * 64) + r * 64 + lane;
+        if (idx < nValid)
         {
             unsigned d = unsigned(k[r] >> shift) & 255u;
-            uint32_t pos = tstart[d] + wcnt[w][d] + rank[r];
-            sk[pos] = k[r];
-            sv[pos] = v[r];
+            uint32_t pos = L.tstart[d] + L.wcnt[w][d] + rank[r];
+            L.sk[pos] = k[r];
+            L.sv[pos] = v[r];
         }
     }
     __syncthreads();
-    size_t nValid = n - base < size_t(kSortTile) ? n - base : size_t(kSortTile);
-    for (int j = threadIdx.x; j < int(nValid); j += kSortThreads)
+    for (int j = threadIdx.x; j < nValid; j += kSortThreads)
     {
-        uint64_t key = sk[j];
+        uint64_t key = L.sk[j];
         unsigned d = unsigned(key >> shift) & 255u;
-        size_t out = size_t(gbase[d]) + (j - tstart[d]);
-        keysOut[out] = key;
-        valsOut[out] = sv[j];
+        size_t out = size_t(L.gbase[d]) + (j - L.tstart[d]);
+        if (out < nOut)
+        {
+            keysOut[out] = key;
+            valsOut[out] = L.sv[j];
+        }
+        else
+            atomicOr(err, 1u);
     }
+    __syncthreads();
 }
 
-// sorts of at most kBlockSortMax pairs in one launch: a single block of 1024 threads holds the pairs in registers
-// (8 per thread) and LDS for every pass: the wave-level match ranking of scatterKernel gives each wave's digit
-// counts, whose sums are the pass's histogram, and the stable scatter goes to LDS, from where each thread reloads
-// its 8 items. One global read and one write in all, and one launch instead of three per pass plus three, which is
-// most of a small sort's time.
-constexpr int kBlockSortThreads = 1024;
-constexpr int kBlockSortWaves   = kBlockSortThreads / 64;
-constexpr int kBlockSortItems   = 8;
-constexpr size_t kBlockSortMax  = size_t(kBlockSortThreads) * kBlockSortItems;
-
-__global__ __launch_bounds__(kBlockSortThreads) void blockSortKernel(uint64_t* __restrict__ keys,
-                                                                     int32_t* __restrict__ vals, size_t n,
-                                                                     const uint32_t* __restrict__ dN, int digits)
+__global__ __launch_bounds__(kSortThreads) void scatterKernel(const uint64_t* __restrict__ keysIn,
+                                                             const int32_t* __restrict__ valsIn,
+                                                             uint64_t* __restrict__ keysOut,
+                                                             int32_t* __restrict__ valsOut, size_t n,
+                                                             const uint32_t* __restrict__ dN, int shift,
+                                                             const uint32_t* __restrict__ tileOffsets, int numTiles,
+                                                             const uint32_t* __restrict__ digitBase, uint32_t* err)
 {
-    __shared__ uint64_t sk[kBlockSortMax];
-    __shared__ int32_t sv[kBlockSortMax];
-    __shared__ uint32_t wcnt[kBlockSortWaves][kDigits]; // per-wave digit counts, then their prefixes over waves
-    __shared__ uint32_t offs[kDigits];                   // digit totals, then their exclusive scan
     n = deviceCount(dN, n);
+    const int tile = xcdTile(blockIdx.x, numTiles);
+    if (size_t(tile) * kSortTile >= n) return; // a tile past the count (the whole block leaves)
+    __shared__ ScatterLds L;
+    const size_t base = size_t(tile) * kSortTile;
+    const int nValid  = int(n - base < size_t(kSortTile) ? n - base : size_t(kSortTile));
+    scatterTile(keysIn, valsIn, keysOut, valsOut, base, nValid, shift, tileOffsets + size_t(tile) * kDigits, digitBase,
+                n, err, L);
+}
+
+// Sort of at most ITEMS * THREADS pairs inside one block on their lowest `digits` 8-bit digits: the pairs stay in
+// registers (ITEMS per thread) and LDS for every pass; the wave-level match ranking of scatterTile gives each
+// wave's digit counts, whose sums are the pass's histogram, and the stable scatter goes to LDS, from where each
+// thread reloads its items. One global read and one write in all. Item r of thread (w, lane) is element
+// w * 64 ITEMS + 64 r + lane (wave order, then item order, then lanes), so waves past the count skip their work.
+//
+// With more than kFastDigits digits only the top kFastDigits are sorted first. Pairs whose keys differ in those
+// bits are then in their final order; the runs of keys equal in them (rare: the keys of one range already share
+// every digit above) are sorted on the whole key by insertion, stable, by the thread owning the run's first pair.
+// A run longer than kTieRunMax sends the whole range through every digit instead, from the pairs as they were read
+// (nothing is written before the end). Reads kin/vin, writes kout/vout (which may be the same arrays). Starts and
+// ends with the block in step.
+constexpr int kFastDigits = 3;
+constexpr int kTieRunMax  = 32;
+
+template<int THREADS, int ITEMS>
+struct RangeSortLds
+{
+    static constexpr int kWaves = THREADS / 64;
+    uint64_t sk[THREADS * ITEMS];
+    int32_t sv[THREADS * ITEMS];
+    uint32_t wcnt[kWaves][kDigits]; // per-wave digit counts, then their prefixes over waves
+    uint32_t offs[kDigits];         // digit totals, then their exclusive scan
+};
+
+// stable LSD passes on digits [d0, d1) (d1 > d0) of the items in registers (read from global memory); the sorted
+// range is in L.sk/L.sv afterwards
+template<int THREADS, int ITEMS>
+__device__ __forceinline__ void rangePasses(uint64_t (&k)[ITEMS], int32_t (&v)[ITEMS], int n, int d0, int d1,
+                                            RangeSortLds<THREADS, ITEMS>& L)
+{
+    constexpr int kWaves = THREADS / 64;
     const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const uint64_t ltMask = (1ull << lane) - 1ull;
-    // item r of thread (w, lane) is element w * 512 + r * 64 + lane: in wave order, then item order, then lanes
-    uint64_t k[kBlockSortItems];
-    int32_t v[kBlockSortItems];
-#pragma unroll
-    for (int r = 0; r < kBlockSortItems; ++r)
-    {
-        const size_t idx = size_t(w) * (kBlockSortItems * 64) + r * 64 + lane;
-        k[r] = idx < n ? keys[idx] : 0;
-        v[r] = idx < n ? vals[idx] : 0;
-    }
-    for (int p = 0; p < digits; ++p)
+    for (int p = d0; p < d1; ++p)
     {
         const int shift = 8 * p;
-        for (int i = threadIdx.x; i < kBlockSortWaves * kDigits; i += kBlockSortThreads)
-            (&wcnt[0][0])[i] = 0;
-        __syncthreads();
-        uint32_t rank[kBlockSortItems];
-#pragma unroll
-        for (int r = 0; r < kBlockSortItems; ++r)
+        if (p > d0)
         {
-            const bool valid = size_t(w) * (kBlockSortItems * 64) + r * 64 + lane < n;
+#pragma unroll
+            for (int r = 0; r < ITEMS; ++r)
+            {
+                const int idx = w * (ITEMS * 64) + r * 64 + lane;
+                if (idx < n) k[r] = L.sk[idx], v[r] = L.sv[idx];
+            }
+        }
+        for (int i = threadIdx.x; i < kWaves * kDigits; i += THREADS)
+            (&L.wcnt[0][0])[i] = 0;
+        __syncthreads();
+        uint32_t rank[ITEMS];
+#pragma unroll
+        for (int r = 0; r < ITEMS; ++r)
+        {
+            rank[r] = 0;
+            if (w * (ITEMS * 64) + r * 64 >= n) continue; // the whole wave: nothing in this slot
+            const bool valid = w * (ITEMS * 64) + r * 64 + lane < n;
             const unsigned d = unsigned(k[r] >> shift) & 255u;
             uint64_t match = __ballot(valid);
 #pragma unroll
@@ -391,29 +435,28 @@ __global__ __launch_bounds__(kBlockSortThreads) void blockSortKernel(uint64_t* _
                 const uint64_t bb = __ballot(bit);
                 match &= bit ? bb : ~bb;
             }
-            const uint32_t pre = valid ? wcnt[w][d] : 0;
+            const uint32_t pre = valid ? L.wcnt[w][d] : 0;
             rank[r] = pre + __popcll(match & ltMask);
-            if (valid && (match >> lane) == 1ull) wcnt[w][d] = pre + __popcll(match);
+            if (valid && (match >> lane) == 1ull) L.wcnt[w][d] = pre + __popcll(match);
         }
         __syncthreads();
-        if (threadIdx.x < kDigits)
+        for (int d = threadIdx.x; d < kDigits; d += THREADS)
         {
-            const int d = threadIdx.x;
             uint32_t c = 0;
-            for (int ww = 0; ww < kBlockSortWaves; ++ww)
+            for (int ww = 0; ww < kWaves; ++ww)
             {
-                const uint32_t x = wcnt[ww][d];
-                wcnt[ww][d] = c;
+                const uint32_t x = L.wcnt[ww][d];
+                L.wcnt[ww][d] = c;
                 c += x;
             }
-            offs[d] = c;
+            L.offs[d] = c;
         }
         __syncthreads();
         if (w == 0)
         {
             // exclusive scan of the 256 digit totals, 4 per lane
-            const uint32_t c0 = offs[4 * lane], c1 = offs[4 * lane + 1], c2 = offs[4 * lane + 2],
-                           c3 = offs[4 * lane + 3];
+            const uint32_t c0 = L.offs[4 * lane], c1 = L.offs[4 * lane + 1], c2 = L.offs[4 * lane + 2],
+                           c3 = L.offs[4 * lane + 3];
             const uint32_t sum = c0 + c1 + c2 + c3;
             uint32_t inc = sum;
             for (int o = 1; o < 64; o <<= 1)
@@ -422,32 +465,429 @@ __global__ __launch_bounds__(kBlockSortThreads) void blockSortKernel(uint64_t* _
                 if (lane >= o) inc += t;
             }
             const uint32_t ex = inc - sum;
-            offs[4 * lane] = ex, offs[4 * lane + 1] = ex + c0, offs[4 * lane + 2] = ex + c0 + c1;
-            offs[4 * lane + 3] = ex + c0 + c1 + c2;
+            L.offs[4 * lane] = ex, L.offs[4 * lane + 1] = ex + c0, L.offs[4 * lane + 2] = ex + c0 + c1;
+            L.offs[4 * lane + 3] = ex + c0 + c1 + c2;
         }
         __syncthreads();
 #pragma unroll
-        for (int r = 0; r < kBlockSortItems; ++r)
-            if (size_t(w) * (kBlockSortItems * 64) + r * 64 + lane < n)
+        for (int r = 0; r < ITEMS; ++r)
+            if (w * (ITEMS * 64) + r * 64 + lane < n)
             {
                 const unsigned d  = unsigned(k[r] >> shift) & 255u;
-                const uint32_t at = offs[d] + wcnt[w][d] + rank[r];
-                sk[at] = k[r];
-                sv[at] = v[r];
+                const uint32_t at = L.offs[d] + L.wcnt[w][d] + rank[r];
+                L.sk[at] = k[r];
+                L.sv[at] = v[r];
             }
         __syncthreads();
-#pragma unroll
-        for (int r = 0; r < kBlockSortItems; ++r)
+    }
+}
+
+// the runs of pairs in L.sk/L.sv whose keys are equal in bits [sh, 64), sorted on the whole key by insertion
+// (stable) by the thread owning each run's first pair; returns 1 to the threads that found a run longer than
+// kTieRunMax (left as it is). Keys in a run share those bits, so a run's neighbours stay recognisable while other
+// runs are being sorted.
+template<int THREADS>
+__device__ __forceinline__ int sortTieRuns(uint64_t* sk, int32_t* sv, int n, int sh)
+{
+    int longRun = 0;
+#pragma unroll 1
+    for (int idx = threadIdx.x; idx + 1 < n; idx += THREADS)
+    {
+        const uint64_t pfx = sk[idx] >> sh;
+        if ((sk[idx + 1] >> sh) != pfx || (idx > 0 && (sk[idx - 1] >> sh) == pfx)) continue;
+        int end = idx + 2;
+        while (end < n && end - idx <= kTieRunMax && (sk[end] >> sh) == pfx)
+            ++end;
+        if (end - idx > kTieRunMax)
         {
-            const size_t idx = size_t(w) * (kBlockSortItems * 64) + r * 64 + lane;
-            if (idx < n) k[r] = sk[idx], v[r] = sv[idx];
+            longRun = 1;
+            continue;
+        }
+        for (int i = idx + 1; i < end; ++i)
+        {
+            const uint64_t key = sk[i];
+            const int32_t val  = sv[i];
+            int j              = i;
+            while (j > idx && sk[j - 1] > key)
+            {
+                sk[j] = sk[j - 1];
+                sv[j] = sv[j - 1];
+                --j;
+            }
+            sk[j] = key;
+            sv[j] = val;
         }
     }
-#pragma unroll
-    for (int r = 0; r < kBlockSortItems; ++r)
+    return longRun;
+}
+
+template<int THREADS, int ITEMS>
+__device__ __forceinline__ void sortRangeLds(const uint64_t* kin, const int32_t* vin, uint64_t* kout, int32_t* vout,
+                                             int n, int digits, RangeSortLds<THREADS, ITEMS>& L)
+{
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    if (digits == 0)
     {
-        const size_t idx = size_t(w) * (kBlockSortItems * 64) + r * 64 + lane;
-        if (idx < n) keys[idx] = k[r], vals[idx] = v[r];
+        for (int i = threadIdx.x; i < n; i += THREADS)
+            kout[i] = kin[i], vout[i] = vin[i];
+        return;
+    }
+    int d0 = digits > kFastDigits ? digits - kFastDigits : 0;
+    for (;;)
+    {
+        uint64_t k[ITEMS];
+        int32_t v[ITEMS];
+#pragma unroll
+        for (int r = 0; r < ITEMS; ++r)
+        {
+            const int idx = w * (ITEMS * 64) + r * 64 + lane;
+            k[r] = idx < n ? kin[idx] : 0;
+            v[r] = idx < n ? vin[idx] : 0;
+        }
+        rangePasses<THREADS, ITEMS>(k, v, n, d0, digits, L);
+        if (d0 == 0) break;
+        // the top digits only: fix the runs of keys equal in them, or go through every digit
+        if (!__syncthreads_or(sortTieRuns<THREADS>(L.sk, L.sv, n, 8 * d0))) break;
+        d0 = 0;
+    }
+    __syncthreads(); // the sorted range is in LDS; kout (possibly kin) is written after every read of kin
+    for (int i = threadIdx.x; i < n; i += THREADS)
+        kout[i] = L.sk[i], vout[i] = L.sv[i];
+    __syncthreads();
+}
+
+// sorts of at most kBlockSortMax pairs in one launch: one block of 1024 threads runs every pass (one launch instead
+// of three per pass plus three, which is most of a small sort's time)
+constexpr int kBlockSortThreads = 1024;
+constexpr int kBlockSortItems   = 8;
+constexpr size_t kBlockSortMax  = size_t(kBlockSortThreads) * kBlockSortItems;
+
+__global__ __launch_bounds__(kBlockSortThreads) void blockSortKernel(uint64_t* __restrict__ keys,
+                                                                     int32_t* __restrict__ vals, size_t n,
+                                                                     const uint32_t* __restrict__ dN, int digits)
+{
+    __shared__ RangeSortLds<kBlockSortThreads, kBlockSortItems> L;
+    n = deviceCount(dN, n);
+    sortRangeLds<kBlockSortThreads, kBlockSortItems>(keys, vals, keys, vals, int(n), digits, L);
+}
+
+/* ---------------------------------------------------------------------------- hybrid MSD sort of large inputs
+ *
+ * A full 64-bit sort of more than kBlockSortMax pairs, stable (the order of std::stable_sort on the keys):
+ *  round 0: one stable pass on the top digit (bits 56-63) with the tile-histogram kernels above, into the
+ *           second buffer. Every bucket of equal top digits (a "child") is then classified:
+ *  - a child of more than kLocalMax pairs becomes a segment of the next round, which makes a stable pass on the
+ *    next lower digit inside each segment (tiles never straddle segments, offsets scanned per segment), from one
+ *    buffer into the other; rounds 1..7 repeat this on digits 6..0 for whatever is still large;
+ *  - smaller children are finished in LDS by one block (sortRangeLds) on their remaining lower digits, read from
+ *    the buffer the round wrote and written to the first buffer. Runs of consecutive small children are packed
+ *    into ranges of at most kPackMax pairs, which are then sorted on one more digit (the one that separates
+ *    the packed children: they are in order of it already, and a stable sort on it keeps that).
+ * Each pass is stable and the passes go from the most significant digit down, so equal keys keep their input order
+ * throughout. Uniform 2^26 keys take round 0, round 1 and one LDS sort of ~2048-pair buckets: 3 reads and writes
+ * of the pairs plus 2 reads of the keys, against 8 of each for the LSD passes. Rounds with no large segment left
+ * cost three launches that exit at once; every count lives in device memory, so there is no host round trip.
+ */
+constexpr int kLocalMax  = 8192; // children up to this size are finished in LDS
+constexpr int kPackMax   = 4096; // runs of small children are packed into ranges up to this size
+constexpr int kMsdRounds = 8;    // round r splits on digit 7 - r
+// ranges by size class: up to 1024 pairs (128 threads x 8), 4096 (512 x 8), 8192 (1024 x 8)
+constexpr int kNumRangeClasses = 3;
+__device__ inline int rangeClass(uint32_t len) { return len <= 1024 ? 0 : len <= 4096 ? 1 : 2; }
+
+struct MsdSeg
+{
+    uint32_t start, len, tileStart, pad;
+};
+
+// device counters of an MSD sort (words)
+enum MsdCtl : int
+{
+    kMsdBig   = 0,                      // [8]: large segments of round r
+    kMsdTiles = kMsdBig + kMsdRounds,   // [8]: tiles of round r
+    kMsdRange = kMsdTiles + kMsdRounds, // [4]: ranges of class c
+    kMsdErr   = kMsdRange + kNumRangeClasses,
+    kMsdCtlWords = 32
+};
+
+struct MsdLists
+{
+    uint32_t* ctl;
+    MsdSeg* segs[2];     // large segments of odd / even rounds
+    uint32_t* owner[2];  // segment of each tile, odd / even rounds
+    uint2* ranges[kNumRangeClasses]; // {start, len | digits << 16 | source buffer << 24}
+    uint32_t maxSegs, maxTiles, maxRanges;
+};
+
+__global__ void msdInitKernel(uint32_t* ctl, uint32_t* hist)
+{
+    for (int i = threadIdx.x; i < kMsdCtlWords; i += blockDim.x)
+        ctl[i] = 0;
+    for (int i = threadIdx.x; i < kDigits; i += blockDim.x)
+        hist[i] = 0;
+}
+
+// Children of one segment after its pass on the digit of round r (cnt[d], start[d] in LDS, 256 threads): large
+// ones become segments of round r + 1 (with their tiles), runs of small ones ranges for the LDS sorts. Thread 0
+// walks the 256 children in order; one reservation per list, then the block writes the entries.
+__device__ __forceinline__ void emitChildren(const uint32_t* cnt, const uint32_t* start, int r, MsdLists L)
+{
+    const int rem = kMsdRounds - 1 - r; // digits below the one this round split on
+    const uint32_t src = (r & 1) ? 0u : 1u; // round r wrote the second buffer when r is even
+    __shared__ uint2 packs[kDigits];
+    __shared__ MsdSeg bigs[kDigits];
+    __shared__ int nPack, nBig;
+    __shared__ uint32_t bigTiles, segBase, tileBase, rangeBase[kNumRangeClasses], classOf[kDigits],
+        classIdx[kDigits];
+    if (threadIdx.x == 0)
+    {
+        int np = 0, nb = 0;
+        uint32_t tiles = 0, pStart = 0, pLen = 0, pKids = 0;
+        uint32_t nc[kNumRangeClasses] = {};
+        auto put = [&](uint32_t s, uint32_t len, uint32_t digits) {
+            // a range already in the first buffer with nothing left to sort needs no work
+            if (len == 0 || (digits == 0 && src == 0)) return;
+            const int c = rangeClass(len);
+            packs[np]    = make_uint2(s, len | (digits << 16) | (src << 24));
+            classOf[np]  = c;
+            classIdx[np] = nc[c]++;
+            ++np;
+        };
+        auto flush = [&]() {
+            if (pLen) put(pStart, pLen, rem == 0 ? 0u : uint32_t(rem) + (pKids > 1 ? 1u : 0u));
+            pLen = 0, pKids = 0;
+        };
+        for (int d = 0; d < kDigits; ++d)
+        {
+            const uint32_t c = cnt[d];
+            if (c == 0) continue;
+            if (rem > 0 && c > uint32_t(kLocalMax))
+            {
+                flush();
+                bigs[nb++] = MsdSeg{start[d], c, tiles, 0};
+                tiles += (c + kSortTile - 1) / kSortTile;
+            }
+            else if (c > uint32_t(kPackMax))
+            {
+                flush();
+                // only a child with no digits left can be larger than kLocalMax: it is sorted already
+                put(start[d], c > uint32_t(kLocalMax) ? 0u : c, uint32_t(rem));
+            }
+            else
+            {
+                if (pLen + c > uint32_t(kPackMax)) flush();
+                if (pLen == 0) pStart = start[d];
+                pLen += c;
+                ++pKids;
+            }
+        }
+        flush();
+        nPack = np, nBig = nb, bigTiles = tiles;
+        for (int c = 0; c < kNumRangeClasses; ++c)
+            rangeBase[c] = nc[c] ? atomicAdd(&L.ctl[kMsdRange + c], nc[c]) : 0u;
+        if (nb)
+        {
+            segBase  = atomicAdd(&L.ctl[kMsdBig + r + 1], uint32_t(nb));
+            tileBase = atomicAdd(&L.ctl[kMsdTiles + r + 1], tiles);
+        }
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < nPack; i += blockDim.x)
+    {
+        const uint32_t at = rangeBase[classOf[i]] + classIdx[i];
+        if (at < L.maxRanges) L.ranges[classOf[i]][at] = packs[i];
+        else atomicOr(&L.ctl[kMsdErr], 2u);
+    }
+    if (nBig)
+    {
+        const int nxt = (r + 1) & 1;
+        if (segBase + nBig > L.maxSegs || tileBase + bigTiles > L.maxTiles)
+        {
+            if (threadIdx.x == 0) atomicOr(&L.ctl[kMsdErr], 4u);
+        }
+        else
+        {
+            for (int i = threadIdx.x; i < nBig; i += blockDim.x)
+            {
+                MsdSeg g = bigs[i];
+                g.tileStart += tileBase;
+                L.segs[nxt][segBase + i] = g;
+            }
+            for (int i = 0; i < nBig; ++i)
+            {
+                const uint32_t t0 = tileBase + bigs[i].tileStart;
+                const uint32_t nt = (bigs[i].len + kSortTile - 1) / kSortTile;
+                for (uint32_t t = threadIdx.x; t < nt; t += blockDim.x)
+                    L.owner[nxt][t0 + t] = segBase + i;
+            }
+        }
+    }
+    __syncthreads();
+}
+
+// round 0's children: the buckets of the top digit, from its histogram and their exclusive scan
+__global__ __launch_bounds__(256) void msdRootKernel(const uint32_t* __restrict__ hist,
+                                                    const uint32_t* __restrict__ base, MsdLists L)
+{
+    __shared__ uint32_t cnt[kDigits], st[kDigits];
+    cnt[threadIdx.x] = hist[threadIdx.x];
+    st[threadIdx.x]  = base[threadIdx.x];
+    __syncthreads();
+    emitChildren(cnt, st, 0, L);
+}
+
+// the tiles of round r's segments, spread over the grid so that each XCD takes a contiguous range of tiles in
+// order (as xcdTile does for the one-pass kernels); the grid is a multiple of 8 blocks
+#define MSD_FOR_EACH_TILE(t, T)                                                                                       \
+    for (uint32_t t_per_ = (T) >> 3, t_x_ = blockIdx.x & 7u, t_k_ = blockIdx.x >> 3, t_g_ = gridDim.x >> 3,             \
+                  t_end_ = t_x_ * t_per_ + min(t_x_, (T) & 7u) + t_per_ + (t_x_ < ((T) & 7u) ? 1u : 0u),               \
+                  t = t_x_ * t_per_ + min(t_x_, (T) & 7u) + t_k_;                                                       \
+         t < t_end_; t += t_g_)
+
+// the segment of tile t of round r, checked against the lists and the element count (an entry that does not
+// describe tile t raises the error word and is skipped, so no launch reads or writes past the arrays)
+__device__ __forceinline__ bool msdTileSeg(const MsdLists& L, int r, uint32_t t, size_t n, MsdSeg& g, uint32_t& off,
+                                           int& cnt)
+{
+    const uint32_t S = min(L.ctl[kMsdBig + r], L.maxSegs);
+    const uint32_t o = L.owner[r & 1][t];
+    if (o >= S) return false;
+    g = L.segs[r & 1][o];
+    if (size_t(g.start) + g.len > n || t < g.tileStart || size_t(t - g.tileStart) * kSortTile >= g.len) return false;
+    off = (t - g.tileStart) * kSortTile;
+    cnt = int(min(uint32_t(kSortTile), g.len - off));
+    return true;
+}
+
+__global__ __launch_bounds__(256) void msdTileHistKernel(const uint64_t* __restrict__ keys, size_t n, MsdLists L,
+                                                        int r, uint32_t* __restrict__ tileHist)
+{
+    __shared__ uint32_t h[kDigits];
+    const uint32_t T = min(L.ctl[kMsdTiles + r], L.maxTiles);
+    const int shift  = 8 * (kMsdRounds - 1 - r);
+    MSD_FOR_EACH_TILE(t, T)
+    {
+        __syncthreads();
+        h[threadIdx.x] = 0;
+        MsdSeg g{};
+        uint32_t off = 0;
+        int cnt      = 0;
+        if (!msdTileSeg(L, r, t, n, g, off, cnt))
+        {
+            if (threadIdx.x == 0) atomicOr(&L.ctl[kMsdErr], 16u);
+            cnt = 0;
+        }
+        const uint64_t* k = keys + g.start + off;
+        __syncthreads();
+#pragma unroll 4
+        for (int i = threadIdx.x; i < cnt; i += kSortThreads)
+            atomicAdd(&h[(k[i] >> shift) & 255u], 1u);
+        __syncthreads();
+        tileHist[size_t(t) * kDigits + threadIdx.x] = h[threadIdx.x];
+    }
+}
+
+// per segment of round r: the digit totals over its tiles, their exclusive scan from the segment's start, the
+// running offsets of its tiles (in place of their counts), then its children. 1024 threads: thread (q, d) sums and
+// then offsets digit d over the q-th quarter of the segment's tiles.
+constexpr int kMsdScanThreads = 1024;
+__global__ __launch_bounds__(kMsdScanThreads) void msdScanKernel(size_t n, MsdLists L, int r,
+                                                                uint32_t* __restrict__ tileHist)
+{
+    __shared__ uint32_t cnt[kDigits], st[kDigits], part[4][kDigits];
+    const uint32_t S   = min(L.ctl[kMsdBig + r], L.maxSegs);
+    const MsdSeg* segs = L.segs[r & 1];
+    const int d = threadIdx.x & 255, q = threadIdx.x >> 8, lane = threadIdx.x & 63;
+    for (uint32_t s = blockIdx.x; s < S; s += gridDim.x)
+    {
+        const MsdSeg g    = segs[s];
+        const uint32_t nt = (g.len + kSortTile - 1) / kSortTile;
+        if (size_t(g.start) + g.len > n || size_t(g.tileStart) + nt > L.maxTiles)
+        {
+            if (threadIdx.x == 0) atomicOr(&L.ctl[kMsdErr], 32u);
+            continue;
+        }
+        const uint32_t t0 = nt * q / 4, t1 = nt * (q + 1) / 4;
+        uint32_t* th      = tileHist + size_t(g.tileStart) * kDigits + d;
+        uint32_t sum      = 0;
+#pragma unroll 8
+        for (uint32_t t = t0; t < t1; ++t)
+            sum += th[size_t(t) * kDigits];
+        part[q][d] = sum;
+        __syncthreads();
+        if (q == 0) cnt[d] = part[0][d] + part[1][d] + part[2][d] + part[3][d];
+        __syncthreads();
+        if (threadIdx.x < 64)
+        {
+            // exclusive scan of the 256 digit totals from the segment's start, 4 per lane
+            const uint32_t c0 = cnt[4 * lane], c1 = cnt[4 * lane + 1], c2 = cnt[4 * lane + 2], c3 = cnt[4 * lane + 3];
+            const uint32_t tot = c0 + c1 + c2 + c3;
+            uint32_t inc       = tot;
+            for (int o = 1; o < 64; o <<= 1)
+            {
+                const uint32_t x = __shfl_up(inc, o);
+                if (lane >= o) inc += x;
+            }
+            const uint32_t ex = g.start + inc - tot;
+            st[4 * lane] = ex, st[4 * lane + 1] = ex + c0, st[4 * lane + 2] = ex + c0 + c1;
+            st[4 * lane + 3] = ex + c0 + c1 + c2;
+        }
+        __syncthreads();
+        uint32_t carry = st[d];
+        for (int qq = 0; qq < q; ++qq)
+            carry += part[qq][d];
+        for (uint32_t t = t0; t < t1; ++t)
+        {
+            const uint32_t v = th[size_t(t) * kDigits];
+            th[size_t(t) * kDigits] = carry;
+            carry += v;
+        }
+        __syncthreads();
+        emitChildren(cnt, st, r, L);
+    }
+}
+
+__global__ __launch_bounds__(kSortThreads, 5) void msdScatterKernel(const uint64_t* __restrict__ keysIn,
+                                                                const int32_t* __restrict__ valsIn,
+                                                                uint64_t* __restrict__ keysOut,
+                                                                int32_t* __restrict__ valsOut, size_t n, MsdLists L,
+                                                                int r, const uint32_t* __restrict__ tileHist)
+{
+    __shared__ ScatterLds S;
+    const uint32_t T = min(L.ctl[kMsdTiles + r], L.maxTiles);
+    const int shift  = 8 * (kMsdRounds - 1 - r);
+    MSD_FOR_EACH_TILE(t, T)
+    {
+        MsdSeg g{};
+        uint32_t off = 0;
+        int cnt      = 0;
+        if (!msdTileSeg(L, r, t, n, g, off, cnt)) continue; // the same for the whole block
+        scatterTile(keysIn, valsIn, keysOut, valsOut, size_t(g.start) + off, cnt, shift,
+                    tileHist + size_t(t) * kDigits, nullptr, n, &L.ctl[kMsdErr], S);
+    }
+}
+
+// the ranges of one class: each sorted in LDS on its digits from the buffer its round wrote into the first buffer
+template<int THREADS, int ITEMS>
+__global__ __launch_bounds__(THREADS) void msdLocalSortKernel(MsdLists L, int cls, uint64_t* __restrict__ k0,
+                                                              int32_t* __restrict__ v0, const uint64_t* __restrict__ k1,
+                                                              const int32_t* __restrict__ v1, size_t n)
+{
+    __shared__ RangeSortLds<THREADS, ITEMS> S;
+    const uint32_t num = min(L.ctl[kMsdRange + cls], L.maxRanges);
+    const uint2* ranges = L.ranges[cls];
+    for (uint32_t i = blockIdx.x; i < num; i += gridDim.x)
+    {
+        const uint2 rg        = ranges[i];
+        const uint32_t len    = rg.y & 0xffffu, digits = (rg.y >> 16) & 0xffu, src = rg.y >> 24;
+        if (len > uint32_t(THREADS * ITEMS) || size_t(rg.x) + len > n || digits > 8u)
+        {
+            if (threadIdx.x == 0) atomicOr(&L.ctl[kMsdErr], 8u);
+            continue;
+        }
+        const uint64_t* kin = src ? k1 : k0;
+        const int32_t* vin  = src ? v1 : v0;
+        sortRangeLds<THREADS, ITEMS>(kin + rg.x, vin + rg.x, k0 + rg.x, v0 + rg.x, int(len), int(digits), S);
     }
 }
 
@@ -511,12 +951,53 @@ static unsigned gridFor(size_t n, int threads, int maxBlocks)
     return unsigned(b ? b : 1);
 }
 
-size_t sortTempBytes(size_t n)
+static size_t lsdTempBytes(size_t n)
 {
     size_t numTiles  = (n + kSortTile - 1) / kSortTile;
     size_t numChunks = (numTiles + kScanChunk - 1) / kScanChunk;
     return 256 * sizeof(uint32_t) * (2 * kPasses) + size_t(kDigits) * (numTiles + numChunks) * sizeof(uint32_t) + 256;
 }
+
+// temp storage of the MSD sort: counters, top-digit histogram and bases, the tile histograms (round 0's tiles, or
+// the most tiles a later round can have: segments longer than kLocalMax are disjoint, each adds at most one partial
+// tile), round 0's chunk sums, two segment lists, two tile-owner arrays and the four range lists (every range but
+// the last of a run of small children shares more than kPackMax pairs with the next one)
+struct MsdLayout
+{
+    size_t ctl, hist, base, tileHist, chunkSum, segs[2], owner[2], ranges[kNumRangeClasses], bytes;
+    uint32_t maxSegs, maxTiles, maxRanges, numTiles0, numChunks0;
+};
+
+static MsdLayout msdLayout(size_t n)
+{
+    MsdLayout l{};
+    l.numTiles0  = uint32_t((n + kSortTile - 1) / kSortTile);
+    l.numChunks0 = (l.numTiles0 + kScanChunk - 1) / kScanChunk;
+    l.maxSegs    = uint32_t(n / (kLocalMax + 1) + 1);
+    l.maxTiles   = l.numTiles0 + l.maxSegs;
+    l.maxRanges  = uint32_t(n / 256 + 64);
+    size_t at    = 0;
+    auto take    = [&](size_t bytes) {
+        const size_t r = at;
+        at += (bytes + 255) & ~size_t(255);
+        return r;
+    };
+    l.ctl      = take(kMsdCtlWords * 4);
+    l.hist     = take(kDigits * 4);
+    l.base     = take(kDigits * 4);
+    l.tileHist = take(size_t(l.maxTiles) * kDigits * 4);
+    l.chunkSum = take(size_t(l.numChunks0) * kDigits * 4);
+    for (int i = 0; i < 2; ++i)
+        l.segs[i] = take(size_t(l.maxSegs) * sizeof(MsdSeg));
+    for (int i = 0; i < 2; ++i)
+        l.owner[i] = take(size_t(l.maxTiles) * 4);
+    for (int c = 0; c < kNumRangeClasses; ++c)
+        l.ranges[c] = take(size_t(l.maxRanges) * sizeof(uint2));
+    l.bytes = at;
+    return l;
+}
+
+size_t sortTempBytes(size_t n) { return std::max(lsdTempBytes(n), n > kBlockSortMax ? msdLayout(n).bytes : 0); }
 
 // stable LSD sort on the lowest `digits` 8-bit digits of the keys; the result is in keys/vals. With dN, n is the
 // capacity the launches are sized for and *dN (device memory) the number of elements sorted; the sequence then
@@ -541,6 +1022,7 @@ int sortPairsDigits(Context* ctx, uint64_t* keys, int32_t* vals, size_t n, uint6
     uint32_t* tileHist = dbase + kPasses * kDigits;
     const int numChunks = (numTiles + kScanChunk - 1) / kScanChunk;
     uint32_t* chunkSum  = tileHist + size_t(kDigits) * numTiles;
+    uint32_t* err       = chunkSum + size_t(kDigits) * numChunks; // in the last 256 bytes
     zeroWordsKernel<<<8, 256, 0, s>>>(hist, kPasses * kDigits); // a kernel, not a memset node, under capture
     histAllKernel<<<gridFor(n, 256, ctx->numCUs * 4), 256, 0, s>>>(keys, n, dN, hist);
     digitBaseKernel<<<kPasses, 256, 0, s>>>(hist, dbase);
@@ -553,14 +1035,15 @@ int sortPairsDigits(Context* ctx, uint64_t* keys, int32_t* vals, size_t n, uint6
         int shift = 8 * p;
         tileHistKernel<<<numTiles, kSortThreads, 0, s>>>(kin, n, dN, shift, tileHist, numTiles);
         if (numTiles <= kSmallScanTiles)
-            tileScanSmallKernel<<<kDigits, 256, 0, s>>>(tileHist, numTiles, dbase + p * kDigits);
+            tileScanSmallKernel<<<kDigits, 256, 0, s>>>(tileHist, numTiles, dbase + p * kDigits, nullptr);
         else
         {
             tileChunkSumKernel<<<numChunks, 256, 0, s>>>(tileHist, numTiles, chunkSum);
-            tileChunkScanKernel<<<kDigits, 256, 0, s>>>(chunkSum, numChunks, dbase + p * kDigits);
+            tileChunkScanKernel<<<kDigits, 256, 0, s>>>(chunkSum, numChunks, dbase + p * kDigits, nullptr);
             tileOffsetKernel<<<numChunks, 256, 0, s>>>(tileHist, numTiles, chunkSum);
         }
-        scatterKernel<<<numTiles, kSortThreads, 0, s>>>(kin, vin, kout, vout, n, dN, shift, tileHist, numTiles);
+        scatterKernel<<<numTiles, kSortThreads, 0, s>>>(kin, vin, kout, vout, n, dN, shift, tileHist, numTiles,
+                                                        nullptr, err);
         std::swap(kin, kout);
         std::swap(vin, vout);
     }
@@ -570,9 +1053,81 @@ int sortPairsDigits(Context* ctx, uint64_t* keys, int32_t* vals, size_t n, uint6
     return RYO_OK;
 }
 
+// the full sort of more than kBlockSortMax pairs (fewer than 2^31) by MSD rounds and LDS sorts
+static int sortPairsMsd(Context* ctx, uint64_t* keys, int32_t* vals, size_t n, uint64_t* keyBuf, int32_t* valBuf,
+                        void* tmp)
+{
+    const MsdLayout l = msdLayout(n);
+    char* t           = static_cast<char*>(tmp);
+    MsdLists L;
+    L.ctl = reinterpret_cast<uint32_t*>(t + l.ctl);
+    for (int i = 0; i < 2; ++i)
+    {
+        L.segs[i]  = reinterpret_cast<MsdSeg*>(t + l.segs[i]);
+        L.owner[i] = reinterpret_cast<uint32_t*>(t + l.owner[i]);
+    }
+    for (int c = 0; c < kNumRangeClasses; ++c)
+        L.ranges[c] = reinterpret_cast<uint2*>(t + l.ranges[c]);
+    L.maxSegs = l.maxSegs, L.maxTiles = l.maxTiles, L.maxRanges = l.maxRanges;
+    uint32_t* hist     = reinterpret_cast<uint32_t*>(t + l.hist);
+    uint32_t* base     = reinterpret_cast<uint32_t*>(t + l.base);
+    uint32_t* tileHist = reinterpret_cast<uint32_t*>(t + l.tileHist);
+    uint32_t* chunkSum = reinterpret_cast<uint32_t*>(t + l.chunkSum);
+    const int numTiles = int(l.numTiles0), numChunks = int(l.numChunks0);
+    constexpr int kTop = 8 * (kMsdRounds - 1);
+    hipStream_t s = ctx->stream;
+
+    // round 0: the stable pass on the top digit into the second buffer, then its children
+    // (the tile scan gives each tile's offsets within the digit's run and the digit totals, whose exclusive scan
+    // the scatter adds)
+    msdInitKernel<<<1, 256, 0, s>>>(L.ctl, hist);
+    tileHistKernel<<<numTiles, kSortThreads, 0, s>>>(keys, n, nullptr, kTop, tileHist, numTiles);
+    if (numTiles <= kSmallScanTiles)
+        tileScanSmallKernel<<<kDigits, 256, 0, s>>>(tileHist, numTiles, nullptr, hist);
+    else
+    {
+        tileChunkSumKernel<<<numChunks, 256, 0, s>>>(tileHist, numTiles, chunkSum);
+        tileChunkScanKernel<<<kDigits, 256, 0, s>>>(chunkSum, numChunks, nullptr, hist);
+        tileOffsetKernel<<<numChunks, 256, 0, s>>>(tileHist, numTiles, chunkSum);
+    }
+    digitBaseKernel<<<1, 256, 0, s>>>(hist, base);
+    scatterKernel<<<numTiles, kSortThreads, 0, s>>>(keys, vals, keyBuf, valBuf, n, nullptr, kTop, tileHist, numTiles,
+                                                    base, L.ctl + kMsdErr);
+    msdRootKernel<<<1, 256, 0, s>>>(hist, base, L);
+
+    // rounds 1..7: the segments still larger than kLocalMax, on the next lower digit, between the buffers
+    const unsigned tileGrid = std::max(8u, std::min(l.maxTiles, unsigned(ctx->numCUs) * 8u) & ~7u);
+    const unsigned segGrid  = std::min(l.maxSegs, unsigned(ctx->numCUs) * 2u);
+    for (int r = 1; r < kMsdRounds; ++r)
+    {
+        uint64_t* kin  = (r & 1) ? keyBuf : keys;
+        int32_t* vin   = (r & 1) ? valBuf : vals;
+        uint64_t* kout = (r & 1) ? keys : keyBuf;
+        int32_t* vout  = (r & 1) ? vals : valBuf;
+        msdTileHistKernel<<<tileGrid, 256, 0, s>>>(kin, n, L, r, tileHist);
+        msdScanKernel<<<segGrid, kMsdScanThreads, 0, s>>>(n, L, r, tileHist);
+        msdScatterKernel<<<tileGrid, kSortThreads, 0, s>>>(kin, vin, kout, vout, n, L, r, tileHist);
+    }
+
+    // the ranges, by size class, into the first buffer
+    const unsigned cus = unsigned(ctx->numCUs);
+    msdLocalSortKernel<128, 8><<<cus * 10, 128, 0, s>>>(L, 0, keys, vals, keyBuf, valBuf, n);
+    msdLocalSortKernel<512, 8><<<cus * 2, 512, 0, s>>>(L, 1, keys, vals, keyBuf, valBuf, n);
+    msdLocalSortKernel<1024, 8><<<cus, 1024, 0, s>>>(L, 2, keys, vals, keyBuf, valBuf, n);
+    RYO_LAUNCH_CHECK(ctx);
+    return RYO_OK;
+}
+
+// the full 64-bit stable sort; tmp: sortTempBytes(n) bytes
 int sortPairs(Context* ctx, uint64_t* keys, int32_t* vals, size_t n, uint64_t* keyBuf, int32_t* valBuf, void* tmp,
               size_t tmpBytes)
 {
+    if (n > kBlockSortMax && n < (size_t(1) << 31))
+    {
+        if (tmpBytes < sortTempBytes(n))
+            return setError(ctx, RYO_ERR_ARG, "sort: temp storage too small (%zu < %zu)", tmpBytes, sortTempBytes(n));
+        return sortPairsMsd(ctx, keys, vals, n, keyBuf, valBuf, tmp);
+    }
     return sortPairsDigits(ctx, keys, vals, n, keyBuf, valBuf, tmp, tmpBytes, kPasses, nullptr);
 }
 

@@ -56,8 +56,9 @@ def test_keys_remove_key_untouched(ctx):
     assert np.all(g[::7] == np.uint64(1 << 63))
 
 
-# up to 32768 pairs the sort is one block running every pass; beyond, one launch per step of each pass
-@pytest.mark.parametrize("n", [1, 2, 100, 4095, 4096, 4097, 8191, 8192, 8193, 32767, 32768, 32769, 70001, 1 << 20])
+# up to 8192 pairs the sort is one block running every pass; beyond, most-significant-digit rounds and LDS sorts
+@pytest.mark.parametrize("n", [1, 2, 100, 4095, 4096, 4097, 8191, 8192, 8193, 32767, 32768, 32769, 70001, 1 << 20,
+                               3_000_017])
 def test_sort_stable_bit_exact(ctx, n):
     rng = np.random.default_rng(n)
     # few distinct values in the high bytes and ties: stability matters
@@ -74,6 +75,57 @@ def test_sort_full_64bit(ctx, n):
     rng = np.random.default_rng(9)
     keys = rng.integers(0, np.iinfo(np.uint64).max, n, dtype=np.uint64, endpoint=True)
     keys[::3] = keys[1::3][: keys[::3].size]
+    gk, gv = gpu_sort(ctx, keys)
+    ok, ov = O.sort_by_key(keys)
+    assert np.array_equal(gk, ok) and np.array_equal(gv, ov)
+
+
+def _clustered_keys(kind, n, rng):
+    """key sets that drive the large-sort path through its rarer branches"""
+    u64 = np.uint64
+    if kind == "equal":  # one key: every round keeps a single large segment, nothing is left for the LDS sorts
+        return np.full(n, u64(0x1234_5678_9ABC_DEF0), dtype=np.uint64)
+    if kind == "deep":  # top 40 bits shared, a few hundred distinct low values: rounds 1..5 split nothing
+        return u64(0x5A5A_5A5A_5A00_0000) | rng.integers(0, 300, n, dtype=np.uint64)
+    if kind == "mixed_depth":  # a dense cluster beside sparse keys: large segments at several depths at once
+        k = rng.integers(0, np.iinfo(np.uint64).max >> np.uint64(1), n, dtype=np.uint64)
+        m = rng.random(n) < 0.6
+        k[m] = u64(0x0123_4567_0000_0000) | rng.integers(0, 1 << 22, int(m.sum()), dtype=np.uint64)
+        m2 = rng.random(n) < 0.1
+        k[m2] = u64(0x0123_4567_89AB_CD00) | rng.integers(0, 16, int(m2.sum()), dtype=np.uint64)
+        return k
+    if kind == "removed":  # removal sentinels (2^63) among 63-bit keys
+        k = rng.integers(0, np.iinfo(np.uint64).max >> np.uint64(1), n, dtype=np.uint64)
+        k[rng.random(n) < 0.3] = u64(1 << 63)
+        return k
+    if kind == "small_buckets":  # many top-digit buckets of a few hundred keys: packed ranges of several buckets
+        return (rng.integers(0, 1 << 16, n, dtype=np.uint64) << np.uint64(48)) | rng.integers(0, 1 << 12, n,
+                                                                                                dtype=np.uint64)
+    if kind == "sizes":  # second-level buckets of every LDS size class (< 512, < 2048, < 4096, < 8192 keys)
+        top = rng.choice([1, 2, 3, 4], n, p=[0.1, 0.2, 0.3, 0.4]).astype(np.uint64)
+        return (top << np.uint64(56)) | (rng.integers(0, 64, n, dtype=np.uint64) << np.uint64(48)) | \
+            rng.integers(0, 1 << 30, n, dtype=np.uint64)
+    raise ValueError(kind)
+
+
+@pytest.mark.parametrize("kind,n", [("equal", 300000), ("deep", 250000), ("mixed_depth", 1 << 20),
+                                    ("removed", 200001), ("small_buckets", 1 << 21), ("sizes", 400000),
+                                    ("deep", 8193)])
+def test_sort_clustered_bit_exact(ctx, kind, n):
+    """stable order and permutation on key sets that need deep rounds, pack buckets or keep equal-key runs"""
+    rng = np.random.default_rng(n + len(kind))
+    keys = _clustered_keys(kind, n, rng)
+    vals = rng.permutation(n).astype(np.int32)
+    gk, gv = gpu_sort(ctx, keys, vals)
+    ok, ov = O.sort_by_key(keys, vals)
+    assert np.array_equal(gk, ok)
+    assert np.array_equal(gv, ov)
+
+
+def test_sort_sfc_keys_large(ctx):
+    """Hilbert keys of a clustered (Plummer) system of 2^22 particles, bit-exact keys and permutation"""
+    s = W.plummer(1 << 22, seed=5)
+    keys = O.sfc_keys(s.x, s.y, s.z, s.limits)
     gk, gv = gpu_sort(ctx, keys)
     ok, ov = O.sort_by_key(keys)
     assert np.array_equal(gk, ok) and np.array_equal(gv, ov)
