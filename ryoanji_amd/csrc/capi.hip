@@ -16,6 +16,7 @@ template<class T>
 int sfcKeys32(Context*, const T*, const T*, const T*, uint32_t*, size_t, const T*, int, int);
 size_t sortTempBytes(size_t n);
 int sortPairs(Context*, uint64_t*, int32_t*, size_t, uint64_t*, int32_t*, void*, size_t);
+int sortPairsIndex(Context*, uint64_t*, int32_t*, size_t, uint64_t*, int32_t*, void*, size_t);
 int sequence(Context*, int32_t*, size_t, int32_t);
 int gather(Context*, const int32_t*, size_t, int, const float* const*, float* const*);
 int gather64(Context*, const int32_t*, size_t, int, const double* const*, double* const*);
@@ -884,12 +885,12 @@ int ryo_system_build(ryo_system* s, const ryo_params* prm)
     const bool packed = s->freshOrder && num == 6;
     float4* rec = nullptr;
     mark(s, 0);
-    if ((rc = packed ? sfcKeysPacked(ctx, in, s->keys, s->perm, n, s->limits, &rec)
+    if ((rc = packed ? sfcKeysPacked(ctx, in, s->keys, nullptr, n, s->limits, &rec)
                      : sfcKeys(ctx, A[0], A[1], A[2], s->keys, n, s->limits, 0, 0)))
         return rc;
     mark(s, 1);
-    if (!packed && (rc = sequence(ctx, s->perm, n, 0))) return rc;
-    if ((rc = sortPairs(ctx, s->keys, s->perm, n, s->keyBuf, s->permBuf, s->sortTmp, s->sortTmpBytes))) return rc;
+    if ((rc = sortPairsIndex(ctx, s->keys, s->perm, n, s->keyBuf, s->permBuf, s->sortTmp, s->sortTmpBytes)))
+        return rc;
     mark(s, 2);
     if ((rc = packed ? gatherRecords(ctx, s->perm, n, rec, out) : gather(ctx, s->perm, n, num, in, out))) return rc;
     s->freshOrder = false;

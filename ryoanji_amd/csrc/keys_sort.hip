@@ -96,7 +96,7 @@ __global__ __launch_bounds__(256) void sfcKeysKernel(const T* __restrict__ x, co
         keys[i] = KeyT(k);
         if constexpr (Pack)
         {
-            pk.perm[i]       = int32_t(i);
+            if (pk.perm) pk.perm[i] = int32_t(i);
             pk.rec[2 * i]     = make_float4(float(px), float(py), float(pz), pk.m[i]);
             pk.rec[2 * i + 1] = make_float4(pk.h[i], pk.o[i], 0.0f, 0.0f);
         }
@@ -269,7 +269,8 @@ struct ScatterLds
     uint32_t sw[4];
 };
 
-// stable scatter of the nValid (<= kSortTile) pairs from `base`: wave-level match ranking (8 ballots per key),
+// stable scatter of the nValid (<= kSortTile) pairs from `base` (valsIn null: each value is the pair's index, the
+// identity permutation a sort of fresh keys starts from): wave-level match ranking (8 ballots per key),
 // block-level offsets, staging in LDS so that the writes of one digit run are contiguous. offs[d] is where this
 // tile's first pair of digit d goes; a destination at or past nOut (offsets that do not describe these keys) is
 // not written and raises *err instead. Starts and ends with the block in step, so that a block can run it for
@@ -294,7 +295,7 @@ __device__ __forceinline__ void scatterTile(const uint64_t* __restrict__ keysIn,
         const int idx    = w * (kSortItems * 64) + r * 64 + lane;
         const bool valid = idx < nValid;
         k[r] = valid ? keysIn[base + idx] : 0;
-        v[r] = valid ? valsIn[base + idx] : 0;
+        v[r] = valid ? (valsIn ? valsIn[base + idx] : int32_t(base + idx)) : 0; // no values: the index
         unsigned d = unsigned(k[r] >> shift) & 255u;
         uint64_t match = __ballot(valid);
 #pragma unroll
@@ -1053,9 +1054,11 @@ int sortPairsDigits(Context* ctx, uint64_t* keys, int32_t* vals, size_t n, uint6
     return RYO_OK;
 }
 
+int sequence(Context* ctx, int32_t* out, size_t n, int32_t start);
+
 // the full sort of more than kBlockSortMax pairs (fewer than 2^31) by MSD rounds and LDS sorts
 static int sortPairsMsd(Context* ctx, uint64_t* keys, int32_t* vals, size_t n, uint64_t* keyBuf, int32_t* valBuf,
-                        void* tmp)
+                        void* tmp, bool valsAreIndex)
 {
     const MsdLayout l = msdLayout(n);
     char* t           = static_cast<char*>(tmp);
@@ -1091,12 +1094,15 @@ static int sortPairsMsd(Context* ctx, uint64_t* keys, int32_t* vals, size_t n, u
         tileOffsetKernel<<<numChunks, 256, 0, s>>>(tileHist, numTiles, chunkSum);
     }
     digitBaseKernel<<<1, 256, 0, s>>>(hist, base);
-    scatterKernel<<<numTiles, kSortThreads, 0, s>>>(keys, vals, keyBuf, valBuf, n, nullptr, kTop, tileHist, numTiles,
-                                                    base, L.ctl + kMsdErr);
+    scatterKernel<<<numTiles, kSortThreads, 0, s>>>(keys, valsAreIndex ? nullptr : vals, keyBuf, valBuf, n, nullptr,
+                                                    kTop, tileHist, numTiles, base, L.ctl + kMsdErr);
     msdRootKernel<<<1, 256, 0, s>>>(hist, base, L);
 
     // rounds 1..7: the segments still larger than kLocalMax, on the next lower digit, between the buffers
-    const unsigned tileGrid = std::max(8u, std::min(l.maxTiles, unsigned(ctx->numCUs) * 8u) & ~7u);
+    // grids of what is resident at once (5 scatter blocks per CU with 30 KB of LDS each, 8 histogram blocks), so
+    // that no second wave of blocks starts late; multiples of 8 for the XCD mapping
+    const unsigned histGrid    = std::max(8u, std::min(l.maxTiles, unsigned(ctx->numCUs) * 8u) & ~7u);
+    const unsigned scatterGrid = std::max(8u, std::min(l.maxTiles, unsigned(ctx->numCUs) * 5u) & ~7u);
     const unsigned segGrid  = std::min(l.maxSegs, unsigned(ctx->numCUs) * 2u);
     for (int r = 1; r < kMsdRounds; ++r)
     {
@@ -1104,9 +1110,9 @@ static int sortPairsMsd(Context* ctx, uint64_t* keys, int32_t* vals, size_t n, u
         int32_t* vin   = (r & 1) ? valBuf : vals;
         uint64_t* kout = (r & 1) ? keys : keyBuf;
         int32_t* vout  = (r & 1) ? vals : valBuf;
-        msdTileHistKernel<<<tileGrid, 256, 0, s>>>(kin, n, L, r, tileHist);
+        msdTileHistKernel<<<histGrid, 256, 0, s>>>(kin, n, L, r, tileHist);
         msdScanKernel<<<segGrid, kMsdScanThreads, 0, s>>>(n, L, r, tileHist);
-        msdScatterKernel<<<tileGrid, kSortThreads, 0, s>>>(kin, vin, kout, vout, n, L, r, tileHist);
+        msdScatterKernel<<<scatterGrid, kSortThreads, 0, s>>>(kin, vin, kout, vout, n, L, r, tileHist);
     }
 
     // the ranges, by size class, into the first buffer
@@ -1126,8 +1132,23 @@ int sortPairs(Context* ctx, uint64_t* keys, int32_t* vals, size_t n, uint64_t* k
     {
         if (tmpBytes < sortTempBytes(n))
             return setError(ctx, RYO_ERR_ARG, "sort: temp storage too small (%zu < %zu)", tmpBytes, sortTempBytes(n));
-        return sortPairsMsd(ctx, keys, vals, n, keyBuf, valBuf, tmp);
+        return sortPairsMsd(ctx, keys, vals, n, keyBuf, valBuf, tmp, false);
     }
+    return sortPairsDigits(ctx, keys, vals, n, keyBuf, valBuf, tmp, tmpBytes, kPasses, nullptr);
+}
+
+// the same sort of keys paired with their indices 0..n-1 (vals is written, not read: the sort's first pass makes
+// the identity permutation instead of reading one)
+int sortPairsIndex(Context* ctx, uint64_t* keys, int32_t* vals, size_t n, uint64_t* keyBuf, int32_t* valBuf,
+                   void* tmp, size_t tmpBytes)
+{
+    if (n > kBlockSortMax && n < (size_t(1) << 31))
+    {
+        if (tmpBytes < sortTempBytes(n))
+            return setError(ctx, RYO_ERR_ARG, "sort: temp storage too small (%zu < %zu)", tmpBytes, sortTempBytes(n));
+        return sortPairsMsd(ctx, keys, vals, n, keyBuf, valBuf, tmp, true);
+    }
+    if (int rc = sequence(ctx, vals, n, 0)) return rc;
     return sortPairsDigits(ctx, keys, vals, n, keyBuf, valBuf, tmp, tmpBytes, kPasses, nullptr);
 }
 
@@ -1297,7 +1318,7 @@ int sfcKeysPacked(Context* ctx, const float* const* in, uint64_t* keys, int32_t*
     if (int rc = makeKeyBox(ctx, limits, b)) return rc;
     auto* r = static_cast<float4*>(ctx->getScratch(kSlotPack, 32 * n));
     if (!r) return RYO_ERR_HIP;
-    const PackOut pk{in[3], in[4], in[5], perm, r};
+    const PackOut pk{in[3], in[4], in[5], perm, r}; // perm may be null (the sort makes the identity permutation)
     const unsigned grid = gridFor(n, 256, ctx->numCUs * 16);
     if (isCubic(b.bits))
         sfcKeysKernel<float, true, true><<<grid, 256, 0, ctx->stream>>>(in[0], in[1], in[2], keys, n, b, 0, 0, pk);

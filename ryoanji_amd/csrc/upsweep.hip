@@ -95,25 +95,18 @@ __device__ inline void leafUpsweepOne(const UpsweepArgsT<Tin>& a, int leaf, bool
     uint32_t b = 0, e = 0;
     if (ok) b = a.layout[leaf], e = a.layout[leaf + 1];
 
-    double c[4] = {0, 0, 0, 0};
-    for (uint32_t p = b + sub; p < e; p += 8)
-    {
-        double w = fabs(double(a.m[p]));
-        c[0] += w * double(a.x[p]);
-        c[1] += w * double(a.y[p]);
-        c[2] += w * double(a.z[p]);
-        c[3] += w;
-    }
-    for (int k = 0; k < 4; ++k)
-        c[k] = red8(c[k]);
-    double inv = c[3] != 0.0 ? 1.0 / c[3] : 1.0;
-    c[0] *= inv, c[1] *= inv, c[2] *= inv;
-
+    double c[4]  = {0, 0, 0, 0};
     double q[12] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
-    for (uint32_t p = b + sub; p < e; p += 8)
-    {
-        double rx = double(a.x[p]) - c[0], ry = double(a.y[p]) - c[1], rz = double(a.z[p]) - c[2];
-        double mi = double(a.m[p]);
+    auto center = [&](Tin px, Tin py, Tin pz, Tin pm) {
+        double w = fabs(double(pm));
+        c[0] += w * double(px);
+        c[1] += w * double(py);
+        c[2] += w * double(pz);
+        c[3] += w;
+    };
+    auto moments = [&](Tin px, Tin py, Tin pz, Tin pm) {
+        double rx = double(px) - c[0], ry = double(py) - c[1], rz = double(pz) - c[2];
+        double mi = double(pm);
         q[0] += mi;
         q[1] += rx * rx * mi;
         q[2] += rx * ry * mi;
@@ -122,6 +115,45 @@ __device__ inline void leafUpsweepOne(const UpsweepArgsT<Tin>& a, int leaf, bool
         q[5] += ry * rz * mi;
         q[6] += rz * rz * mi;
         if constexpr (MDQ) q[8] += mi * rx, q[9] += mi * ry, q[10] += mi * rz;
+    };
+    auto centerDone = [&]() {
+        for (int k = 0; k < 4; ++k)
+            c[k] = red8(c[k]);
+        double inv = c[3] != 0.0 ? 1.0 / c[3] : 1.0;
+        c[0] *= inv, c[1] *= inv, c[2] *= inv;
+    };
+    // leaves of at most 64 particles (every leaf of a bucket of at most 64 but those at the deepest level): the
+    // lane's 8 particles are loaded at once and kept in registers for both sums, which add the same terms in the
+    // same order as the loops below (the slots past the leaf are skipped)
+    constexpr int kRegSlots = 8;
+    if (e - b <= uint32_t(8 * kRegSlots))
+    {
+        Tin px[kRegSlots], py[kRegSlots], pz[kRegSlots], pm[kRegSlots];
+#pragma unroll
+        for (int i = 0; i < kRegSlots; ++i)
+        {
+            const uint32_t p = b + sub + 8 * i;
+            const bool in    = p < e;
+            px[i] = in ? a.x[p] : Tin(0);
+            py[i] = in ? a.y[p] : Tin(0);
+            pz[i] = in ? a.z[p] : Tin(0);
+            pm[i] = in ? a.m[p] : Tin(0);
+        }
+#pragma unroll
+        for (int i = 0; i < kRegSlots; ++i)
+            if (b + sub + 8 * i < e) center(px[i], py[i], pz[i], pm[i]);
+        centerDone();
+#pragma unroll
+        for (int i = 0; i < kRegSlots; ++i)
+            if (b + sub + 8 * i < e) moments(px[i], py[i], pz[i], pm[i]);
+    }
+    else
+    {
+        for (uint32_t p = b + sub; p < e; p += 8)
+            center(a.x[p], a.y[p], a.z[p], a.m[p]);
+        centerDone();
+        for (uint32_t p = b + sub; p < e; p += 8)
+            moments(a.x[p], a.y[p], a.z[p], a.m[p]);
     }
     for (int k = 0; k < 7; ++k)
         q[k] = red8(q[k]);
