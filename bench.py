@@ -401,6 +401,9 @@ def main():
     ap.add_argument("--trav-stack", type=int, default=0, help="traversal LIFO entries per wave (0: default)")
     ap.add_argument("--trav-blocks", type=int, default=0, help="traversal blocks per CU (0: default)")
     ap.add_argument("--trav-ranges", type=int, default=0, help="traversal group ranges (0: one per XCD)")
+    ap.add_argument("--build-only", action="store_true",
+                    help="time only the key -> sort -> build -> upsweep phase (a profiling run of the build kernels); "
+                         "prints the build phases and build_roofline")
     ap.add_argument("--copy-calibration-mb", type=int, default=0,
                     help="first run a plain device copy of this many MiB (reference kernel for HBM counters)")
     args = ap.parse_args()
@@ -470,6 +473,25 @@ def main():
 
     periodic = spec["periodic"]
     prm = params(args.theta, periodic)
+    if args.build_only:
+        # the build alone, from the unsorted initial state each time (the phase the build roofline prices)
+        phase = []
+        for i in range(args.warmup + args.steps):
+            sysm.load(*dev, s.limits, periodic=periodic)
+            sysm.build(prm)
+            torch.cuda.synchronize()
+            if i >= args.warmup:
+                phase.append(sysm.phase_ms())
+            # settles the device-sized build's counts as the gravity pass does (the next build then launches for
+            # the leaf count it found, as in a full pass)
+            sysm.info()
+        info = sysm.info()
+        ph = {k: float(np.mean([p[k] for p in phase])) for k in ("keys", "sort", "permute", "leaves", "link",
+                                                                 "upsweep")}
+        log(f"build {sum(ph.values()):.2f} ms: " + ", ".join(f"{k} {v:.3f}" for k, v in ph.items()))
+        print(json.dumps({"workload": args.workload, "n": s.n, "steps": args.steps, "build_phases_ms": ph,
+                          "build_roofline": build_roofline(phase, s.n, info["numNodes"]), "tree": info}))
+        return
     log(f"N={s.n}: {args.warmup} warmup + {args.steps} timed passes")
     t_pass, phase, stats, pot = run(prm, periodic, args.steps, args.warmup)
     log(f"pass {t_pass * 1e3:.1f} ms ({s.n / t_pass / 1e6:.1f} M particles/s)")

@@ -391,8 +391,22 @@ struct IBox3
 __host__ __device__ inline IBox3 cellIBox(uint64_t key, unsigned level, const Bits3& bits)
 {
     IBox3 r{{0, 0, 0}, {0, 0, 0}};
-    if (!validMixDKey(key, bits)) return r;
     const unsigned lfr = kMaxLevel - level;
+    if (isCubic(bits))
+    {
+        // the decode runs from the lowest level up and the cell's lower 3 lfr key bits are zero: those levels
+        // leave the coordinates' upper bits as they are (they only reflect and permute bits below `level`, which
+        // the box drops), so the top `level` levels decoded on their own give the box corner directly
+        unsigned c[3];
+        hilbert3DDecode(key >> (3 * lfr), int(level), c[0], c[1], c[2]);
+        for (int d = 0; d < 3; ++d)
+        {
+            r.lo[d]  = c[d] << lfr;
+            r.len[d] = 1u << lfr;
+        }
+        return r;
+    }
+    if (!validMixDKey(key, bits)) return r;
     unsigned c[3];
     hilbertMixDDecode(key, bits, c[0], c[1], c[2]);
     for (int d = 0; d < 3; ++d)
