@@ -66,7 +66,7 @@ constexpr int kSmallBucket = 64;
 /* Per tile [t0, t1): leaf start/level of particles t0-1..t1-1 into s.start/s.level, and the number of
  * leaves each particle of the tile emits (returned for the first kLeafTile/256 items of each thread). */
 template<int MaxB>
-__device__ void leafTile(const uint64_t* __restrict__ k, long n, int B, long t0, LeafTileSmem<MaxB>& s, int em[4])
+__device__ __forceinline__ void leafTile(const uint64_t* __restrict__ k, long n, int B, long t0, LeafTileSmem<MaxB>& s, int em[4])
 {
     const long t1    = min(t0 + long(kLeafTile), n);
     const long kbase = t0 - 1 - B; // s.keys[j - kbase] = k[j]
@@ -231,7 +231,7 @@ int scanBlockSums(Context* ctx, uint32_t* sums, int n, uint32_t* work)
 
 // leaves of the tile starting at particle t0, written from position `base` on
 template<int MaxB>
-__device__ void leafEmitTile(const uint64_t* __restrict__ k, long n, int B, long t0, uint32_t base,
+__device__ __forceinline__ void leafEmitTile(const uint64_t* __restrict__ k, long n, int B, long t0, uint32_t base,
                              uint64_t* __restrict__ leaves, uint32_t* __restrict__ layout, long capacity,
                              LeafTileSmem<MaxB>& s);
 
@@ -246,7 +246,7 @@ __global__ __launch_bounds__(kLeafThreads) void leafEmitKernel(const uint64_t* _
 }
 
 template<int MaxB>
-__device__ void leafEmitTile(const uint64_t* __restrict__ k, long n, int B, long t0, uint32_t base,
+__device__ __forceinline__ void leafEmitTile(const uint64_t* __restrict__ k, long n, int B, long t0, uint32_t base,
                              uint64_t* __restrict__ leaves, uint32_t* __restrict__ layout, long capacity,
                              LeafTileSmem<MaxB>& s)
 {
