@@ -157,12 +157,50 @@ __global__ __launch_bounds__(256) void digitBaseKernel(const uint32_t* __restric
     base[p * kDigits + d] = blockExclusiveScan256(hist[p * kDigits + d], sw, &tot);
 }
 
+// The LSD passes to run: a pass whose digit is the same for every key (one digit bucket holds them all) leaves the
+// order as it is and is skipped. plan[p] = 1 for the passes that run, plan[kPasses + p] = the buffer pass p reads
+// (0: keys/vals, 1: keyBuf/valBuf), plan[2 kPasses] = the buffer holding the result. Node prefixes, whose high
+// digits are zero, skip most passes; full-range keys skip none.
+constexpr int kPlanWords = 2 * kPasses + 1;
+__global__ __launch_bounds__(256) void lsdPlanKernel(const uint32_t* __restrict__ hist, size_t n,
+                                                    const uint32_t* __restrict__ dN, int digits,
+                                                    uint32_t* __restrict__ plan)
+{
+    __shared__ uint32_t constant[kPasses];
+    const size_t count = deviceCount(dN, n);
+    if (threadIdx.x < kPasses) constant[threadIdx.x] = 0;
+    __syncthreads();
+    for (int p = 0; p < kPasses; ++p)
+        if (hist[p * kDigits + threadIdx.x] == count) constant[p] = 1; // also every pass of an empty sort
+    __syncthreads();
+    if (threadIdx.x == 0)
+    {
+        uint32_t buf = 0;
+        for (int p = 0; p < kPasses; ++p)
+        {
+            const uint32_t run = p < digits && !constant[p];
+            plan[p]           = run;
+            plan[kPasses + p] = buf;
+            buf ^= run;
+        }
+        plan[2 * kPasses] = buf;
+    }
+}
+
 // per-tile digit counts of one pass, one contiguous row per tile: tileHist[tile * 256 + d] (a coalesced 1 KB write
 // per tile, read back the same way by the offset scan and the scatter)
+// With a plan (LSD passes), pass `shift / 8` reads keys or keys1 as the plan says and does nothing when skipped.
 __global__ __launch_bounds__(kSortThreads) void tileHistKernel(const uint64_t* __restrict__ keys, size_t n,
                                                               const uint32_t* __restrict__ dN, int shift,
-                                                              uint32_t* __restrict__ tileHist, int numTiles)
+                                                              uint32_t* __restrict__ tileHist, int numTiles,
+                                                              const uint64_t* __restrict__ keys1 = nullptr,
+                                                              const uint32_t* __restrict__ plan = nullptr)
 {
+    if (plan)
+    {
+        if (!plan[shift / 8]) return;
+        if (plan[kPasses + shift / 8]) keys = keys1;
+    }
     __shared__ uint32_t h[kDigits];
     n = deviceCount(dN, n); // tiles past the count record empty histograms
     h[threadIdx.x] = 0;
@@ -182,8 +220,10 @@ __global__ __launch_bounds__(kSortThreads) void tileHistKernel(const uint64_t* _
 // exclusive scan over the chunks from the digit's base, (3) per chunk the running offsets of its tiles, written in
 // place of the counts: tileHist[tile * 256 + d] = base[d] + (keys of digit d in earlier tiles)
 __global__ __launch_bounds__(256) void tileChunkSumKernel(const uint32_t* __restrict__ tileHist, int numTiles,
-                                                     uint32_t* __restrict__ chunkSum)
+                                                     uint32_t* __restrict__ chunkSum,
+                                                     const uint32_t* __restrict__ run = nullptr)
 {
+    if (run && !*run) return; // a skipped pass
     const int c = blockIdx.x, d = threadIdx.x;
     const int t1 = min(numTiles, (c + 1) * kScanChunk);
     uint32_t sum = 0;
@@ -196,8 +236,10 @@ __global__ __launch_bounds__(256) void tileChunkSumKernel(const uint32_t* __rest
 // the runs' starts)
 __global__ __launch_bounds__(256) void tileChunkScanKernel(uint32_t* __restrict__ chunkSum, int numChunks,
                                                       const uint32_t* __restrict__ digitBase,
-                                                      uint32_t* __restrict__ totals)
+                                                      uint32_t* __restrict__ totals,
+                                                     const uint32_t* __restrict__ run = nullptr)
 {
+    if (run && !*run) return; // a skipped pass
     __shared__ uint32_t sw[4];
     const int d = blockIdx.x;
     uint32_t carry = digitBase ? digitBase[d] : 0u;
@@ -217,8 +259,10 @@ __global__ __launch_bounds__(256) void tileChunkScanKernel(uint32_t* __restrict_
 // when there are few tiles and saves two launches per pass)
 __global__ __launch_bounds__(256) void tileScanSmallKernel(uint32_t* __restrict__ tileHist, int numTiles,
                                                           const uint32_t* __restrict__ digitBase,
-                                                          uint32_t* __restrict__ totals)
+                                                          uint32_t* __restrict__ totals,
+                                                     const uint32_t* __restrict__ run = nullptr)
 {
+    if (run && !*run) return; // a skipped pass
     __shared__ uint32_t sw[4];
     const int d = blockIdx.x;
     uint32_t carry = digitBase ? digitBase[d] : 0u;
@@ -235,8 +279,10 @@ __global__ __launch_bounds__(256) void tileScanSmallKernel(uint32_t* __restrict_
 }
 
 __global__ __launch_bounds__(256) void tileOffsetKernel(uint32_t* __restrict__ tileHist, int numTiles,
-                                                       const uint32_t* __restrict__ chunkSum)
+                                                       const uint32_t* __restrict__ chunkSum,
+                                                     const uint32_t* __restrict__ run = nullptr)
 {
+    if (run && !*run) return; // a skipped pass
     const int c = blockIdx.x, d = threadIdx.x;
     const int t1 = min(numTiles, (c + 1) * kScanChunk);
     uint32_t carry = chunkSum[size_t(c) * kDigits + d];
@@ -353,14 +399,26 @@ __device__ __forceinline__ void scatterTile(const uint64_t* __restrict__ keysIn,
     __syncthreads();
 }
 
-__global__ __launch_bounds__(kSortThreads) void scatterKernel(const uint64_t* __restrict__ keysIn,
-                                                             const int32_t* __restrict__ valsIn,
-                                                             uint64_t* __restrict__ keysOut,
-                                                             int32_t* __restrict__ valsOut, size_t n,
+__global__ __launch_bounds__(kSortThreads) void scatterKernel(uint64_t* keysIn, int32_t* valsIn, uint64_t* keysOut,
+                                                             int32_t* valsOut, size_t n,
                                                              const uint32_t* __restrict__ dN, int shift,
                                                              const uint32_t* __restrict__ tileOffsets, int numTiles,
-                                                             const uint32_t* __restrict__ digitBase, uint32_t* err)
+                                                             const uint32_t* __restrict__ digitBase, uint32_t* err,
+                                                             const uint32_t* __restrict__ plan = nullptr)
 {
+    // with a plan (LSD passes) the two buffers are (keysIn, valsIn) and (keysOut, valsOut), and the plan says which
+    // one pass `shift / 8` reads, or that it is skipped
+    if (plan)
+    {
+        if (!plan[shift / 8]) return;
+        if (plan[kPasses + shift / 8])
+        {
+            uint64_t* k = keysIn;
+            int32_t* v  = valsIn;
+            keysIn = keysOut, valsIn = valsOut;
+            keysOut = k, valsOut = v;
+        }
+    }
     n = deviceCount(dN, n);
     const int tile = xcdTile(blockIdx.x, numTiles);
     if (size_t(tile) * kSortTile >= n) return; // a tile past the count (the whole block leaves)
@@ -894,8 +952,9 @@ __global__ __launch_bounds__(THREADS) void msdLocalSortKernel(MsdLists L, int cl
 
 __global__ void copyPairsKernel(const uint64_t* __restrict__ kin, const int32_t* __restrict__ vin, size_t n,
                                 const uint32_t* __restrict__ dN, uint64_t* __restrict__ kout,
-                                int32_t* __restrict__ vout)
+                                int32_t* __restrict__ vout, const uint32_t* __restrict__ plan = nullptr)
 {
+    if (plan && !plan[2 * kPasses]) return; // the LSD passes left the result in place
     n = deviceCount(dN, n);
     for (size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += size_t(gridDim.x) * blockDim.x)
         kout[i] = kin[i], vout[i] = vin[i];
@@ -1023,33 +1082,30 @@ int sortPairsDigits(Context* ctx, uint64_t* keys, int32_t* vals, size_t n, uint6
     uint32_t* tileHist = dbase + kPasses * kDigits;
     const int numChunks = (numTiles + kScanChunk - 1) / kScanChunk;
     uint32_t* chunkSum  = tileHist + size_t(kDigits) * numTiles;
-    uint32_t* err       = chunkSum + size_t(kDigits) * numChunks; // in the last 256 bytes
+    uint32_t* plan      = chunkSum + size_t(kDigits) * numChunks; // in the last 256 bytes, with the error word
+    uint32_t* err       = plan + kPlanWords;
     zeroWordsKernel<<<8, 256, 0, s>>>(hist, kPasses * kDigits); // a kernel, not a memset node, under capture
     histAllKernel<<<gridFor(n, 256, ctx->numCUs * 4), 256, 0, s>>>(keys, n, dN, hist);
     digitBaseKernel<<<kPasses, 256, 0, s>>>(hist, dbase);
-    uint64_t* kin = keys;
-    uint64_t* kout = keyBuf;
-    int32_t* vin = vals;
-    int32_t* vout = valBuf;
+    lsdPlanKernel<<<1, 256, 0, s>>>(hist, n, dN, digits, plan);
+    // every pass is launched; the plan, made on the device, skips the passes whose digit is constant and says which
+    // buffer each of the others reads
     for (int p = 0; p < digits; ++p)
     {
         int shift = 8 * p;
-        tileHistKernel<<<numTiles, kSortThreads, 0, s>>>(kin, n, dN, shift, tileHist, numTiles);
+        tileHistKernel<<<numTiles, kSortThreads, 0, s>>>(keys, n, dN, shift, tileHist, numTiles, keyBuf, plan);
         if (numTiles <= kSmallScanTiles)
-            tileScanSmallKernel<<<kDigits, 256, 0, s>>>(tileHist, numTiles, dbase + p * kDigits, nullptr);
+            tileScanSmallKernel<<<kDigits, 256, 0, s>>>(tileHist, numTiles, dbase + p * kDigits, nullptr, plan + p);
         else
         {
-            tileChunkSumKernel<<<numChunks, 256, 0, s>>>(tileHist, numTiles, chunkSum);
-            tileChunkScanKernel<<<kDigits, 256, 0, s>>>(chunkSum, numChunks, dbase + p * kDigits, nullptr);
-            tileOffsetKernel<<<numChunks, 256, 0, s>>>(tileHist, numTiles, chunkSum);
+            tileChunkSumKernel<<<numChunks, 256, 0, s>>>(tileHist, numTiles, chunkSum, plan + p);
+            tileChunkScanKernel<<<kDigits, 256, 0, s>>>(chunkSum, numChunks, dbase + p * kDigits, nullptr, plan + p);
+            tileOffsetKernel<<<numChunks, 256, 0, s>>>(tileHist, numTiles, chunkSum, plan + p);
         }
-        scatterKernel<<<numTiles, kSortThreads, 0, s>>>(kin, vin, kout, vout, n, dN, shift, tileHist, numTiles,
-                                                        nullptr, err);
-        std::swap(kin, kout);
-        std::swap(vin, vout);
+        scatterKernel<<<numTiles, kSortThreads, 0, s>>>(keys, vals, keyBuf, valBuf, n, dN, shift, tileHist, numTiles,
+                                                        nullptr, err, plan);
     }
-    if (kin != keys)
-        copyPairsKernel<<<gridFor(n, 256, ctx->numCUs * 16), 256, 0, s>>>(kin, vin, n, dN, keys, vals);
+    copyPairsKernel<<<gridFor(n, 256, ctx->numCUs * 16), 256, 0, s>>>(keyBuf, valBuf, n, dN, keys, vals, plan);
     RYO_LAUNCH_CHECK(ctx);
     return RYO_OK;
 }
