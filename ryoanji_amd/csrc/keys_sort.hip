@@ -650,6 +650,7 @@ __global__ __launch_bounds__(kBlockSortThreads) void blockSortKernel(uint64_t* _
 constexpr int kLocalMax  = 8192; // children up to this size are finished in LDS
 constexpr int kPackMax   = 4096; // runs of small children are packed into ranges up to this size
 constexpr int kMsdRounds = 8;    // round r splits on digit 7 - r
+constexpr int kMaxSortSegments = 1 << 16; // segmented sorts of more segments take the two-sort path
 // ranges by size class: up to 1024 pairs (128 threads x 8), 4096 (512 x 8), 8192 (1024 x 8)
 constexpr int kNumRangeClasses = 3;
 __device__ inline int rangeClass(uint32_t len) { return len <= 1024 ? 0 : len <= 4096 ? 1 : 2; }
@@ -802,6 +803,59 @@ __global__ __launch_bounds__(256) void msdRootKernel(const uint32_t* __restrict_
                   t_end_ = t_x_ * t_per_ + min(t_x_, (T) & 7u) + t_per_ + (t_x_ < ((T) & 7u) ? 1u : 0u),               \
                   t = t_x_ * t_per_ + min(t_x_, (T) & 7u) + t_k_;                                                       \
          t < t_end_; t += t_g_)
+
+// Start of a segmented sort: segment s holds pairs [off[s], off[s+1]) (within the count) and is sorted on its own.
+// A segment of more than kLocalMax pairs is one of round 0's segments, a smaller one a range sorted in LDS on all 8
+// digits in place; different segments never meet in a tile or a range, so each one's order is that of a sort of
+// the segment alone. One block; every counter of the sort starts here.
+__global__ __launch_bounds__(256) void msdSegInitKernel(const uint32_t* __restrict__ off, int S, size_t n,
+                                                       const uint32_t* __restrict__ dN, MsdLists L)
+{
+    __shared__ uint32_t nBig, nTiles, nRange[kNumRangeClasses];
+    if (threadIdx.x == 0)
+    {
+        nBig = 0, nTiles = 0;
+        for (int c = 0; c < kNumRangeClasses; ++c)
+            nRange[c] = 0;
+    }
+    for (int i = threadIdx.x; i < kMsdCtlWords; i += blockDim.x)
+        L.ctl[i] = 0;
+    __syncthreads();
+    const size_t count = deviceCount(dN, n);
+    for (int s = threadIdx.x; s < S; s += blockDim.x)
+    {
+        const uint32_t b = uint32_t(min(size_t(off[s + 1]), count)), a = min(off[s], b);
+        const uint32_t len = b - a;
+        if (len > uint32_t(kLocalMax))
+        {
+            const uint32_t nt = (len + kSortTile - 1) / kSortTile;
+            const uint32_t i = atomicAdd(&nBig, 1u), t0 = atomicAdd(&nTiles, nt);
+            if (i < L.maxSegs && t0 + nt <= L.maxTiles)
+            {
+                L.segs[0][i] = MsdSeg{a, len, t0, 0};
+                for (uint32_t t = 0; t < nt; ++t)
+                    L.owner[0][t0 + t] = i;
+            }
+            else
+                atomicOr(&L.ctl[kMsdErr], 64u);
+        }
+        else if (len > 1)
+        {
+            const int c      = rangeClass(len);
+            const uint32_t i = atomicAdd(&nRange[c], 1u);
+            if (i < L.maxRanges) L.ranges[c][i] = make_uint2(a, len | (uint32_t(kMsdRounds) << 16));
+            else atomicOr(&L.ctl[kMsdErr], 64u);
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x == 0)
+    {
+        L.ctl[kMsdBig]   = min(nBig, L.maxSegs);
+        L.ctl[kMsdTiles] = min(nTiles, L.maxTiles);
+        for (int c = 0; c < kNumRangeClasses; ++c)
+            L.ctl[kMsdRange + c] = min(nRange[c], L.maxRanges);
+    }
+}
 
 // the segment of tile t of round r, checked against the lists and the element count (an entry that does not
 // describe tile t raises the error word and is skipped, so no launch reads or writes past the arrays)
@@ -1035,7 +1089,8 @@ static MsdLayout msdLayout(size_t n)
     l.numChunks0 = (l.numTiles0 + kScanChunk - 1) / kScanChunk;
     l.maxSegs    = uint32_t(n / (kLocalMax + 1) + 1);
     l.maxTiles   = l.numTiles0 + l.maxSegs;
-    l.maxRanges  = uint32_t(n / 256 + 64);
+    // ranges made by the rounds, plus one per segment of a segmented sort (at most kMaxSortSegments)
+    l.maxRanges  = uint32_t(n / 256 + 64 + std::min(n / 2, size_t(kMaxSortSegments)));
     size_t at    = 0;
     auto take    = [&](size_t bytes) {
         const size_t r = at;
@@ -1057,7 +1112,7 @@ static MsdLayout msdLayout(size_t n)
     return l;
 }
 
-size_t sortTempBytes(size_t n) { return std::max(lsdTempBytes(n), n > kBlockSortMax ? msdLayout(n).bytes : 0); }
+size_t sortTempBytes(size_t n) { return std::max(lsdTempBytes(n), msdLayout(n).bytes); }
 
 // stable LSD sort on the lowest `digits` 8-bit digits of the keys; the result is in keys/vals. With dN, n is the
 // capacity the launches are sized for and *dN (device memory) the number of elements sorted; the sequence then
@@ -1112,12 +1167,9 @@ int sortPairsDigits(Context* ctx, uint64_t* keys, int32_t* vals, size_t n, uint6
 
 int sequence(Context* ctx, int32_t* out, size_t n, int32_t start);
 
-// the full sort of more than kBlockSortMax pairs (fewer than 2^31) by MSD rounds and LDS sorts
-static int sortPairsMsd(Context* ctx, uint64_t* keys, int32_t* vals, size_t n, uint64_t* keyBuf, int32_t* valBuf,
-                        void* tmp, bool valsAreIndex)
+static MsdLists msdLists(const MsdLayout& l, void* tmp)
 {
-    const MsdLayout l = msdLayout(n);
-    char* t           = static_cast<char*>(tmp);
+    char* t = static_cast<char*>(tmp);
     MsdLists L;
     L.ctl = reinterpret_cast<uint32_t*>(t + l.ctl);
     for (int i = 0; i < 2; ++i)
@@ -1128,6 +1180,44 @@ static int sortPairsMsd(Context* ctx, uint64_t* keys, int32_t* vals, size_t n, u
     for (int c = 0; c < kNumRangeClasses; ++c)
         L.ranges[c] = reinterpret_cast<uint2*>(t + l.ranges[c]);
     L.maxSegs = l.maxSegs, L.maxTiles = l.maxTiles, L.maxRanges = l.maxRanges;
+    return L;
+}
+
+// rounds r0..7 on the segments still larger than kLocalMax (round r reads keyBuf when r is odd), then the LDS sorts
+// of every range into keys/vals
+static void msdRoundsAndRanges(Context* ctx, const MsdLayout& l, const MsdLists& L, int r0, uint64_t* keys,
+                               int32_t* vals, uint64_t* keyBuf, int32_t* valBuf, size_t n, uint32_t* tileHist)
+{
+    hipStream_t s = ctx->stream;
+    // grids of what is resident at once (5 scatter blocks per CU with 30 KB of LDS each, 8 histogram blocks), so
+    // that no second wave of blocks starts late; multiples of 8 for the XCD mapping
+    const unsigned histGrid    = std::max(8u, std::min(l.maxTiles, unsigned(ctx->numCUs) * 8u) & ~7u);
+    const unsigned scatterGrid = std::max(8u, std::min(l.maxTiles, unsigned(ctx->numCUs) * 5u) & ~7u);
+    const unsigned segGrid     = std::max(1u, std::min(l.maxSegs, unsigned(ctx->numCUs) * 2u));
+    for (int r = r0; r < kMsdRounds; ++r)
+    {
+        uint64_t* kin  = (r & 1) ? keyBuf : keys;
+        int32_t* vin   = (r & 1) ? valBuf : vals;
+        uint64_t* kout = (r & 1) ? keys : keyBuf;
+        int32_t* vout  = (r & 1) ? vals : valBuf;
+        msdTileHistKernel<<<histGrid, 256, 0, s>>>(kin, n, L, r, tileHist);
+        msdScanKernel<<<segGrid, kMsdScanThreads, 0, s>>>(n, L, r, tileHist);
+        msdScatterKernel<<<scatterGrid, kSortThreads, 0, s>>>(kin, vin, kout, vout, n, L, r, tileHist);
+    }
+    // the ranges, by size class, into the first buffer
+    const unsigned cus = unsigned(ctx->numCUs);
+    msdLocalSortKernel<128, 8><<<cus * 10, 128, 0, s>>>(L, 0, keys, vals, keyBuf, valBuf, n);
+    msdLocalSortKernel<512, 8><<<cus * 2, 512, 0, s>>>(L, 1, keys, vals, keyBuf, valBuf, n);
+    msdLocalSortKernel<1024, 8><<<cus, 1024, 0, s>>>(L, 2, keys, vals, keyBuf, valBuf, n);
+}
+
+// the full sort of more than kBlockSortMax pairs (fewer than 2^31) by MSD rounds and LDS sorts
+static int sortPairsMsd(Context* ctx, uint64_t* keys, int32_t* vals, size_t n, uint64_t* keyBuf, int32_t* valBuf,
+                        void* tmp, bool valsAreIndex)
+{
+    const MsdLayout l  = msdLayout(n);
+    const MsdLists L   = msdLists(l, tmp);
+    char* t            = static_cast<char*>(tmp);
     uint32_t* hist     = reinterpret_cast<uint32_t*>(t + l.hist);
     uint32_t* base     = reinterpret_cast<uint32_t*>(t + l.base);
     uint32_t* tileHist = reinterpret_cast<uint32_t*>(t + l.tileHist);
@@ -1153,29 +1243,23 @@ static int sortPairsMsd(Context* ctx, uint64_t* keys, int32_t* vals, size_t n, u
     scatterKernel<<<numTiles, kSortThreads, 0, s>>>(keys, valsAreIndex ? nullptr : vals, keyBuf, valBuf, n, nullptr,
                                                     kTop, tileHist, numTiles, base, L.ctl + kMsdErr);
     msdRootKernel<<<1, 256, 0, s>>>(hist, base, L);
-
     // rounds 1..7: the segments still larger than kLocalMax, on the next lower digit, between the buffers
-    // grids of what is resident at once (5 scatter blocks per CU with 30 KB of LDS each, 8 histogram blocks), so
-    // that no second wave of blocks starts late; multiples of 8 for the XCD mapping
-    const unsigned histGrid    = std::max(8u, std::min(l.maxTiles, unsigned(ctx->numCUs) * 8u) & ~7u);
-    const unsigned scatterGrid = std::max(8u, std::min(l.maxTiles, unsigned(ctx->numCUs) * 5u) & ~7u);
-    const unsigned segGrid  = std::min(l.maxSegs, unsigned(ctx->numCUs) * 2u);
-    for (int r = 1; r < kMsdRounds; ++r)
-    {
-        uint64_t* kin  = (r & 1) ? keyBuf : keys;
-        int32_t* vin   = (r & 1) ? valBuf : vals;
-        uint64_t* kout = (r & 1) ? keys : keyBuf;
-        int32_t* vout  = (r & 1) ? vals : valBuf;
-        msdTileHistKernel<<<histGrid, 256, 0, s>>>(kin, n, L, r, tileHist);
-        msdScanKernel<<<segGrid, kMsdScanThreads, 0, s>>>(n, L, r, tileHist);
-        msdScatterKernel<<<scatterGrid, kSortThreads, 0, s>>>(kin, vin, kout, vout, n, L, r, tileHist);
-    }
+    msdRoundsAndRanges(ctx, l, L, 1, keys, vals, keyBuf, valBuf, n, tileHist);
+    RYO_LAUNCH_CHECK(ctx);
+    return RYO_OK;
+}
 
-    // the ranges, by size class, into the first buffer
-    const unsigned cus = unsigned(ctx->numCUs);
-    msdLocalSortKernel<128, 8><<<cus * 10, 128, 0, s>>>(L, 0, keys, vals, keyBuf, valBuf, n);
-    msdLocalSortKernel<512, 8><<<cus * 2, 512, 0, s>>>(L, 1, keys, vals, keyBuf, valBuf, n);
-    msdLocalSortKernel<1024, 8><<<cus, 1024, 0, s>>>(L, 2, keys, vals, keyBuf, valBuf, n);
+// Each segment [off[s], off[s+1]) sorted on its own (S <= kMaxSortSegments, fewer than 2^31 pairs): MSD rounds from
+// round 0 for the segments larger than kLocalMax, an LDS sort for each smaller one. With dN, n is the capacity and
+// *dN the count (offsets past it are clipped).
+static int segmentedSortMsd(Context* ctx, uint64_t* keys, int32_t* vals, size_t n, const uint32_t* off, int S,
+                            uint64_t* keyBuf, int32_t* valBuf, void* tmp, const uint32_t* dN)
+{
+    const MsdLayout l  = msdLayout(n);
+    const MsdLists L   = msdLists(l, tmp);
+    uint32_t* tileHist = reinterpret_cast<uint32_t*>(static_cast<char*>(tmp) + l.tileHist);
+    msdSegInitKernel<<<1, 256, 0, ctx->stream>>>(off, S, n, dN, L);
+    msdRoundsAndRanges(ctx, l, L, 0, keys, vals, keyBuf, valBuf, n, tileHist);
     RYO_LAUNCH_CHECK(ctx);
     return RYO_OK;
 }
@@ -1285,6 +1369,12 @@ int segmentedSortPairs(Context* ctx, uint64_t* keys, int32_t* vals, size_t n, co
                        int32_t* posBuf, void* tmp, size_t tmpBytes, const uint32_t* dN)
 {
     if (n == 0) return RYO_OK;
+    if (S >= 1 && S <= kMaxSortSegments && n < (size_t(1) << 31))
+    {
+        if (tmpBytes < sortTempBytes(n))
+            return setError(ctx, RYO_ERR_ARG, "sort: temp storage too small (%zu < %zu)", tmpBytes, sortTempBytes(n));
+        return segmentedSortMsd(ctx, keys, vals, n, off, S, keyBuf, valBuf, tmp, dN);
+    }
     int rc = sortPairsDigits(ctx, keys, vals, n, keyBuf, valBuf, tmp, tmpBytes, kPasses, dN);
     if (rc) return rc;
     if (S <= 1) return RYO_OK;
