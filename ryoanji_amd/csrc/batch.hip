@@ -302,10 +302,17 @@ int ensureNodes(ryo_batch* b, uint32_t slots1, uint32_t nodes1, uint32_t parents
     if (!rc) rc = b->leaves.ensure(ctx, slots * 8);
     if (!rc) rc = b->layout.ensure(ctx, slots * 4);
     if (!rc) rc = b->counts.ensure(ctx, slots * 4);
-    for (DevBuf* d : {&b->prefixes, &b->nodeKeyBuf, &b->nodeSeg, &b->nodeSegBuf})
+    for (DevBuf* d : {&b->prefixes, &b->nodeKeyBuf})
         if (!rc) rc = d->ensure(ctx, nn * 8);
-    for (DevBuf* d : {&b->child, &b->i2l, &b->l2i, &b->nodeIdx, &b->nodeValBuf, &b->nodePos, &b->nodePosBuf})
+    for (DevBuf* d : {&b->child, &b->i2l, &b->l2i, &b->nodeIdx, &b->nodeValBuf})
         if (!rc) rc = d->ensure(ctx, nn * 4);
+    if (b->capS > kMaxSortSegments)
+    {
+        for (DevBuf* d : {&b->nodeSeg, &b->nodeSegBuf})
+            if (!rc) rc = d->ensure(ctx, nn * 8);
+        for (DevBuf* d : {&b->nodePos, &b->nodePosBuf})
+            if (!rc) rc = d->ensure(ctx, nn * 4);
+    }
     if (!rc) rc = b->parents.ensure(ctx, np * 4);
     if (!rc) rc = b->centers.ensure(ctx, nn * 16);
     if (!rc) rc = b->mp.ensure(ctx, nn * 32);
@@ -683,10 +690,19 @@ int ryo_batch_create(ryo_ctx* h, size_t max_particles, int max_systems, ryo_batc
         if (!rc) rc = d.ensure(ctx, n * 4);
     for (auto& d : b->srt)
         if (!rc) rc = d.ensure(ctx, n * 4);
-    for (DevBuf* d : {&b->keys, &b->keyBuf, &b->segKeys, &b->segBuf})
+    for (DevBuf* d : {&b->keys, &b->keyBuf})
         if (!rc) rc = d->ensure(ctx, n * 8);
-    for (DevBuf* d : {&b->vals, &b->valBuf, &b->pos, &b->posBuf, &b->p, &b->ax, &b->ay, &b->az})
+    for (DevBuf* d : {&b->vals, &b->valBuf, &b->p, &b->ax, &b->ay, &b->az})
         if (!rc) rc = d->ensure(ctx, n * 4);
+    // the segment keys and positions of the two-sort segmented sort, only for batches of more systems than the
+    // segmented sort takes one at a time
+    if (max_systems > kMaxSortSegments)
+    {
+        for (DevBuf* d : {&b->segKeys, &b->segBuf})
+            if (!rc) rc = d->ensure(ctx, n * 8);
+        for (DevBuf* d : {&b->pos, &b->posBuf})
+            if (!rc) rc = d->ensure(ctx, n * 4);
+    }
     if (!rc) rc = b->sortTmp.ensure(ctx, sortTempBytes(n));
     if (!rc) rc = b->tab.ensure(ctx, 7 * S1 * 4);
     if (!rc) rc = b->LP.ensure(ctx, (kMaxLevel + 1) * S1 * 4);

@@ -18,6 +18,9 @@ namespace ryo
 constexpr unsigned kMaxLevel = 21;           // 64-bit keys: 21 octree levels, 63 key bits
 constexpr uint64_t kKeyEnd   = 1ull << 63;   // end of the key range; also the "remove" key
 constexpr int kWave          = 64;
+// segmented sorts of at most this many segments sort each segment on its own (MSD rounds and LDS range sorts);
+// more segments take a full sort and a second stable pass on the segment of each value
+constexpr int kMaxSortSegments = 1 << 16;
 
 /* ------------------------------------------------------------------ key arithmetic (host+device) */
 

@@ -650,7 +650,6 @@ __global__ __launch_bounds__(kBlockSortThreads) void blockSortKernel(uint64_t* _
 constexpr int kLocalMax  = 8192; // children up to this size are finished in LDS
 constexpr int kPackMax   = 4096; // runs of small children are packed into ranges up to this size
 constexpr int kMsdRounds = 8;    // round r splits on digit 7 - r
-constexpr int kMaxSortSegments = 1 << 16; // segmented sorts of more segments take the two-sort path
 // ranges by size class: up to 1024 pairs (128 threads x 8), 4096 (512 x 8), 8192 (1024 x 8)
 constexpr int kNumRangeClasses = 3;
 __device__ inline int rangeClass(uint32_t len) { return len <= 1024 ? 0 : len <= 4096 ? 1 : 2; }
