@@ -86,8 +86,9 @@ int ryo_sfc_keys_u32_f64(ryo_ctx* ctx, const double* x, const double* y, const d
 /* cstone::sortByKeyTempStorage<uint64_t, int>(n) -- cstone/primitives/primitives_gpu.h:120-124 */
 size_t ryo_sort_temp_bytes(size_t n);
 /* cstone::sortByKey(exec, keys, keys + n, values, keyBuf, valueBuf, tmp, tmpBytes) --
- * primitives_gpu.h:126-130, primitives_gpu.cu:310-358: stable LSD sort of (u64 key, i32 value) pairs over
- * all 64 key bits; the result is in keys/values, keyBuf/valueBuf are scratch of n elements each. */
+ * primitives_gpu.h:126-130, primitives_gpu.cu:310-358: stable sort of (u64 key, i32 value) pairs over all 64
+ * key bits (equal keys keep their order); the result is in keys/values, keyBuf/valueBuf are scratch of n elements
+ * each, tmp holds ryo_sort_temp_bytes(n) bytes. */
 int ryo_sort_pairs_u64_i32(ryo_ctx* ctx, uint64_t* keys, int32_t* values, size_t n, uint64_t* keyBuf,
                            int32_t* valueBuf, void* tmp, size_t tmpBytes);
 /* cstone::sequence (primitives_gpu.cu:265-273) */
