@@ -142,19 +142,6 @@ __device__ __forceinline__ void leafTile(const uint64_t* __restrict__ k, long n,
     }
 }
 
-template<int MaxB>
-__global__ __launch_bounds__(kLeafThreads) void leafCountKernel(const uint64_t* __restrict__ k, long n, int B,
-                                                               uint32_t* __restrict__ blockSums)
-{
-    __shared__ LeafTileSmem<MaxB> s;
-    int em[4];
-    long t0 = long(blockIdx.x) * kLeafTile;
-    leafTile(k, n, B, t0, s, em);
-    uint32_t v = em[0] + em[1] + em[2] + em[3], tot;
-    blockExclusiveScan256(v, s.sw, &tot);
-    if (threadIdx.x == 0) blockSums[blockIdx.x] = tot;
-}
-
 // exclusive scan of the block sums in place (one block); total into blockSums[numBlocks]
 __global__ __launch_bounds__(256) void scanBlocksKernel(uint32_t* __restrict__ sums, int numBlocks)
 {
@@ -231,25 +218,18 @@ int scanBlockSums(Context* ctx, uint32_t* sums, int n, uint32_t* work)
 
 // leaves of the tile starting at particle t0, written from position `base` on
 template<int MaxB>
-__device__ __forceinline__ void leafEmitTile(const uint64_t* __restrict__ k, long n, int B, long t0, uint32_t base,
-                             uint64_t* __restrict__ leaves, uint32_t* __restrict__ layout, long capacity,
-                             LeafTileSmem<MaxB>& s);
+__device__ __forceinline__ uint32_t leafEmitTile(const uint64_t* __restrict__ k, long n, int B, long t0,
+                                                 uint32_t base, uint64_t* __restrict__ leaves,
+                                                 uint32_t* __restrict__ layout, long capacity, LeafTileSmem<MaxB>& s);
 
-template<int MaxB>
-__global__ __launch_bounds__(kLeafThreads) void leafEmitKernel(const uint64_t* __restrict__ k, long n, int B,
-                                                              const uint32_t* __restrict__ blockOffsets,
-                                                              uint64_t* __restrict__ leaves,
-                                                              uint32_t* __restrict__ layout, long capacity)
-{
-    __shared__ LeafTileSmem<MaxB> s;
-    leafEmitTile(k, n, B, long(blockIdx.x) * kLeafTile, blockOffsets[blockIdx.x], leaves, layout, capacity, s);
-}
 
+// returns the number of leaves the tile emits (the terminators of the last tile not counted)
 template<int MaxB>
-__device__ __forceinline__ void leafEmitTile(const uint64_t* __restrict__ k, long n, int B, long t0, uint32_t base,
-                             uint64_t* __restrict__ leaves, uint32_t* __restrict__ layout, long capacity,
-                             LeafTileSmem<MaxB>& s)
+__device__ __forceinline__ uint32_t leafEmitTile(const uint64_t* __restrict__ k, long n, int B, long t0,
+                                                 uint32_t base, uint64_t* __restrict__ leaves,
+                                                 uint32_t* __restrict__ layout, long capacity, LeafTileSmem<MaxB>& s)
 {
+    const uint32_t base0 = base;
     int em[4];
     leafTile(k, n, B, t0, s, em);
     // thread-contiguous order: item r of thread t is particle t0 + r*256 + t -> scan in that order
@@ -290,6 +270,51 @@ __device__ __forceinline__ void leafEmitTile(const uint64_t* __restrict__ k, lon
             if (pos <= capacity) leaves[pos] = kKeyEnd, layout[pos] = uint32_t(n);
         }
     }
+    return base - base0;
+}
+
+/* Leaves through a staging area: the count pass emits each tile's leaves at their positions within the tile into a
+ * per-tile area of `stage` entries (as many as fit) and records the tile's count; once the counts are scanned, the
+ * place pass copies every tile that fit to its offset and emits the others (and the last tile, which also writes
+ * the terminators) again from the keys. The emission is computed once for almost every tile instead of twice. */
+__host__ __device__ inline int leafStageSize(int B) { return 2 * kLeafTile / B + 32 < 256 ? 2 * kLeafTile / B + 32 : 256; }
+
+template<int MaxB>
+__global__ __launch_bounds__(kLeafThreads) void leafCountStageKernel(const uint64_t* __restrict__ k, long n, int B,
+                                                                    uint32_t* __restrict__ blockSums,
+                                                                    uint64_t* __restrict__ stageLeaves,
+                                                                    uint32_t* __restrict__ stageLayout, int stage)
+{
+    __shared__ LeafTileSmem<MaxB> s;
+    // capacity stage - 1: the last tile's terminator (written at the position past its leaves) stays in the area
+    const uint32_t tot = leafEmitTile(k, n, B, long(blockIdx.x) * kLeafTile, 0u,
+                                      stageLeaves + size_t(blockIdx.x) * stage,
+                                      stageLayout + size_t(blockIdx.x) * stage, long(stage - 1), s);
+    if (threadIdx.x == 0) blockSums[blockIdx.x] = tot;
+}
+
+template<int MaxB>
+__global__ __launch_bounds__(kLeafThreads) void leafPlaceKernel(const uint64_t* __restrict__ k, long n, int B,
+                                                               const uint32_t* __restrict__ blockOffsets,
+                                                               const uint64_t* __restrict__ stageLeaves,
+                                                               const uint32_t* __restrict__ stageLayout, int stage,
+                                                               uint64_t* __restrict__ leaves,
+                                                               uint32_t* __restrict__ layout, long capacity)
+{
+    __shared__ LeafTileSmem<MaxB> s;
+    const uint32_t t = blockIdx.x, base = blockOffsets[t], tot = blockOffsets[t + 1] - base;
+    if (t + 1 == gridDim.x || tot >= uint32_t(stage - 1))
+    {
+        leafEmitTile(k, n, B, long(t) * kLeafTile, base, leaves, layout, capacity, s);
+        return;
+    }
+    const uint64_t* sl = stageLeaves + size_t(t) * stage;
+    const uint32_t* sy = stageLayout + size_t(t) * stage;
+    for (uint32_t j = threadIdx.x; j < tot; j += kLeafThreads)
+    {
+        const long pos = long(base) + long(j);
+        if (pos < capacity) leaves[pos] = sl[j], layout[pos] = sy[j];
+    }
 }
 
 __global__ void leafCountsKernel(const uint32_t* __restrict__ layout, uint32_t* __restrict__ counts, int L,
@@ -324,12 +349,20 @@ int buildLeaves(Context* ctx, const uint64_t* keys, size_t n, unsigned bucket, u
         return setError(ctx, RYO_ERR_UNSUPPORTED, "leaves: bucket size %u > %d", bucket, kMaxBucket);
     int numBlocks = int((n + kLeafTile - 1) / kLeafTile);
     const size_t scanWork = size_t(numBlocks) / kScanChunk + 2;
-    auto* sums = static_cast<uint32_t*>(ctx->getScratch(kSlotLeafBlocks, (numBlocks + 1 + scanWork) * sizeof(uint32_t)));
-    if (!sums) return RYO_ERR_HIP;
+    const int stage = leafStageSize(int(bucket));
+    const size_t sumBytes   = ((numBlocks + 1 + scanWork) * sizeof(uint32_t) + 255) & ~size_t(255);
+    const size_t stageCount = size_t(numBlocks) * stage;
+    auto* buf = static_cast<char*>(ctx->getScratch(kSlotLeafBlocks, sumBytes + stageCount * 12));
+    if (!buf) return RYO_ERR_HIP;
+    auto* sums        = reinterpret_cast<uint32_t*>(buf);
+    auto* stageLeaves = reinterpret_cast<uint64_t*>(buf + sumBytes);
+    auto* stageLayout = reinterpret_cast<uint32_t*>(buf + sumBytes + stageCount * 8);
     if (bucket <= unsigned(kSmallBucket))
-        leafCountKernel<kSmallBucket><<<numBlocks, kLeafThreads, 0, s>>>(keys, long(n), int(bucket), sums);
+        leafCountStageKernel<kSmallBucket><<<numBlocks, kLeafThreads, 0, s>>>(keys, long(n), int(bucket), sums,
+                                                                               stageLeaves, stageLayout, stage);
     else
-        leafCountKernel<kMaxBucket><<<numBlocks, kLeafThreads, 0, s>>>(keys, long(n), int(bucket), sums);
+        leafCountStageKernel<kMaxBucket><<<numBlocks, kLeafThreads, 0, s>>>(keys, long(n), int(bucket), sums,
+                                                                             stageLeaves, stageLayout, stage);
     if (int rc = scanBlockSums(ctx, sums, numBlocks, sums + numBlocks + 1)) return rc;
     uint32_t L = 0;
     RYO_HIP(ctx, hipMemcpyAsync(&L, sums + numBlocks, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
@@ -337,11 +370,11 @@ int buildLeaves(Context* ctx, const uint64_t* keys, size_t n, unsigned bucket, u
     *numLeaves = int(L);
     if (size_t(L) > capacity) return setError(ctx, RYO_ERR_CAPACITY, "leaves: %u leaves > capacity %zu", L, capacity);
     if (bucket <= unsigned(kSmallBucket))
-        leafEmitKernel<kSmallBucket><<<numBlocks, kLeafThreads, 0, s>>>(keys, long(n), int(bucket), sums, leaves,
-                                                                         layout, long(L));
+        leafPlaceKernel<kSmallBucket><<<numBlocks, kLeafThreads, 0, s>>>(keys, long(n), int(bucket), sums, stageLeaves,
+                                                                          stageLayout, stage, leaves, layout, long(L));
     else
-        leafEmitKernel<kMaxBucket><<<numBlocks, kLeafThreads, 0, s>>>(keys, long(n), int(bucket), sums, leaves,
-                                                                       layout, long(L));
+        leafPlaceKernel<kMaxBucket><<<numBlocks, kLeafThreads, 0, s>>>(keys, long(n), int(bucket), sums, stageLeaves,
+                                                                        stageLayout, stage, leaves, layout, long(L));
     leafCountsKernel<<<iceil(L, 256), 256, 0, s>>>(layout, counts, int(L));
     RYO_LAUNCH_CHECK(ctx);
     // the layout drives every later particle access: check its terminator before anything reads it
@@ -383,19 +416,29 @@ int buildLeavesDevice(Context* ctx, const uint64_t* keys, size_t n, unsigned buc
         return setError(ctx, RYO_ERR_ARG, "leaves: device-sized build needs n > bucket and bucket <= %d", kMaxBucket);
     int numBlocks = int((n + kLeafTile - 1) / kLeafTile);
     const size_t scanWork = size_t(numBlocks) / kScanChunk + 2;
-    auto* sums = static_cast<uint32_t*>(ctx->getScratch(kSlotLeafBlocks, (numBlocks + 1 + scanWork) * sizeof(uint32_t)));
-    if (!sums) return RYO_ERR_HIP;
+    const int stage = leafStageSize(int(bucket));
+    const size_t sumBytes   = ((numBlocks + 1 + scanWork) * sizeof(uint32_t) + 255) & ~size_t(255);
+    const size_t stageCount = size_t(numBlocks) * stage;
+    auto* buf = static_cast<char*>(ctx->getScratch(kSlotLeafBlocks, sumBytes + stageCount * 12));
+    if (!buf) return RYO_ERR_HIP;
+    auto* sums        = reinterpret_cast<uint32_t*>(buf);
+    auto* stageLeaves = reinterpret_cast<uint64_t*>(buf + sumBytes);
+    auto* stageLayout = reinterpret_cast<uint32_t*>(buf + sumBytes + stageCount * 8);
     if (bucket <= unsigned(kSmallBucket))
-        leafCountKernel<kSmallBucket><<<numBlocks, kLeafThreads, 0, s>>>(keys, long(n), int(bucket), sums);
+        leafCountStageKernel<kSmallBucket><<<numBlocks, kLeafThreads, 0, s>>>(keys, long(n), int(bucket), sums,
+                                                                               stageLeaves, stageLayout, stage);
     else
-        leafCountKernel<kMaxBucket><<<numBlocks, kLeafThreads, 0, s>>>(keys, long(n), int(bucket), sums);
+        leafCountStageKernel<kMaxBucket><<<numBlocks, kLeafThreads, 0, s>>>(keys, long(n), int(bucket), sums,
+                                                                             stageLeaves, stageLayout, stage);
     if (int rc = scanBlockSums(ctx, sums, numBlocks, sums + numBlocks + 1)) return rc;
     if (bucket <= unsigned(kSmallBucket))
-        leafEmitKernel<kSmallBucket><<<numBlocks, kLeafThreads, 0, s>>>(keys, long(n), int(bucket), sums, leaves,
-                                                                         layout, long(capacity));
+        leafPlaceKernel<kSmallBucket><<<numBlocks, kLeafThreads, 0, s>>>(keys, long(n), int(bucket), sums, stageLeaves,
+                                                                          stageLayout, stage, leaves, layout,
+                                                                          long(capacity));
     else
-        leafEmitKernel<kMaxBucket><<<numBlocks, kLeafThreads, 0, s>>>(keys, long(n), int(bucket), sums, leaves,
-                                                                       layout, long(capacity));
+        leafPlaceKernel<kMaxBucket><<<numBlocks, kLeafThreads, 0, s>>>(keys, long(n), int(bucket), sums, stageLeaves,
+                                                                        stageLayout, stage, leaves, layout,
+                                                                        long(capacity));
     leafStatusKernel<<<1, 64, 0, s>>>(sums + numBlocks, layout, uint32_t(n), uint32_t(capacity), stat);
     leafCountsKernel<<<std::max(1u, std::min(iceil(capacity, 256), unsigned(ctx->numCUs * 8))), 256, 0, s>>>(
         layout, counts, int(capacity), stat);
