@@ -650,6 +650,7 @@ __global__ __launch_bounds__(kBlockSortThreads) void blockSortKernel(uint64_t* _
 constexpr int kLocalMax  = 8192; // children up to this size are finished in LDS
 constexpr int kPackMax   = 4096; // runs of small children are packed into ranges up to this size
 constexpr int kMsdRounds = 8;    // round r splits on digit 7 - r
+constexpr unsigned kMsdScatterPerCU = 5; // MSD round scatter blocks per CU (the scatter's LDS fits 5)
 // ranges by size class: up to 1024 pairs (128 threads x 8), 4096 (512 x 8), 8192 (1024 x 8)
 constexpr int kNumRangeClasses = 3;
 __device__ inline int rangeClass(uint32_t len) { return len <= 1024 ? 0 : len <= 4096 ? 1 : 2; }
@@ -1191,7 +1192,7 @@ static void msdRoundsAndRanges(Context* ctx, const MsdLayout& l, const MsdLists&
     // grids of what is resident at once (5 scatter blocks per CU with 30 KB of LDS each, 8 histogram blocks), so
     // that no second wave of blocks starts late; multiples of 8 for the XCD mapping
     const unsigned histGrid    = std::max(8u, std::min(l.maxTiles, unsigned(ctx->numCUs) * 8u) & ~7u);
-    const unsigned scatterGrid = std::max(8u, std::min(l.maxTiles, unsigned(ctx->numCUs) * 5u) & ~7u);
+    const unsigned scatterGrid = std::max(8u, std::min(l.maxTiles, unsigned(ctx->numCUs) * kMsdScatterPerCU) & ~7u);
     const unsigned segGrid     = std::max(1u, std::min(l.maxSegs, unsigned(ctx->numCUs) * 2u));
     for (int r = r0; r < kMsdRounds; ++r)
     {
