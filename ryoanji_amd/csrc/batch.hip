@@ -36,7 +36,7 @@ int batchKeys(Context* ctx, const float* x, const float* y, const float* z, size
               const KeyBox* boxes, uint64_t* keys, int32_t* vals);
 int segmentedSortPairs(Context* ctx, uint64_t* keys, int32_t* vals, size_t n, const uint32_t* off, int S,
                        uint64_t* keyBuf, int32_t* valBuf, uint64_t* segKeys, uint64_t* segBuf, int32_t* pos,
-                       int32_t* posBuf, void* tmp, size_t tmpBytes, const uint32_t* dN);
+                       int32_t* posBuf, void* tmp, size_t tmpBytes, const uint32_t* dN, size_t maxSegment);
 int copyWords(Context* ctx, void* dst, const void* src, size_t words);
 int zeroWords(Context* ctx, void* dst, size_t words);
 int batchNodeTables(Context* ctx, const BatchView& bv, uint32_t capSlots, uint32_t capNodes, uint32_t capParents,
@@ -268,6 +268,25 @@ int uploadTables(ryo_batch* b, const ryo_params* prm, const float* theta, const 
     return RYO_OK;
 }
 
+// the largest system of the batch (particles)
+size_t maxSystem(const ryo_batch* b)
+{
+    uint32_t m = 0;
+    for (size_t s = 0; s + 1 < b->hP.size(); ++s)
+        m = std::max(m, b->hP[s + 1] - b->hP[s]);
+    return m;
+}
+
+// a bound on the octree nodes of a system of n particles with bucket B: a node is internal only if it holds more
+// than B particles, so a level has at most min(8^l, n / (B + 1)) internal nodes, and nodes = 8 internal + 1
+size_t maxNodes(size_t n, unsigned B)
+{
+    size_t internal = 0, perLevel = n / (size_t(B) + 1), cells = 1;
+    for (unsigned l = 0; l < kMaxLevel; ++l, cells = std::min(cells * 8, size_t(1) << 40))
+        internal += std::min(cells, perLevel);
+    return 8 * internal + 1;
+}
+
 // keys, segmented sort, gather, leaf counts and offsets, node tables (device); no host synchronization
 int runFront(ryo_batch* b, const Shape& sh, uint32_t capSlots, uint32_t capNodes, uint32_t capParents)
 {
@@ -285,7 +304,7 @@ int runFront(ryo_batch* b, const Shape& sh, uint32_t capSlots, uint32_t capNodes
     if ((rc = batchKeys(ctx, in[0], in[1], in[2], N, bv.P, S, b->keyBoxes.as<KeyBox>(), keys, vals))) return rc;
     if ((rc = segmentedSortPairs(ctx, keys, vals, N, bv.P, S, b->keyBuf.as<uint64_t>(), b->valBuf.as<int32_t>(),
                                  b->segKeys.as<uint64_t>(), b->segBuf.as<uint64_t>(), b->pos.as<int32_t>(),
-                                 b->posBuf.as<int32_t>(), b->sortTmp.p, b->sortTmp.size, nullptr)))
+                                 b->posBuf.as<int32_t>(), b->sortTmp.p, b->sortTmp.size, nullptr, maxSystem(b))))
         return rc;
     if ((rc = gather(ctx, vals, N, 5, in, out))) return rc;
     if ((rc = copyWords(ctx, b->srt[5].p, vals, N))) return rc;
@@ -355,7 +374,7 @@ int runBack(ryo_batch* b, const Shape& sh, uint32_t slots, uint32_t nodes, uint3
                                  b->nodeValBuf.as<int32_t>(), b->nodeSeg.as<uint64_t>(),
                                  b->nodeSegBuf.as<uint64_t>(), b->nodePos.as<int32_t>(),
                                  b->nodePosBuf.as<int32_t>(), b->nodeTmp.p, b->nodeTmp.size,
-                                 captured ? bv.NO + S : nullptr)))
+                                 captured ? bv.NO + S : nullptr, maxNodes(maxSystem(b), sh.B))))
         return rc;
     if ((rc = batchLinkFinish(ctx, pre, idx, bv, nodes, internal, parents, b->i2l.as<int32_t>(),
                               b->l2i.as<int32_t>(), b->LR.as<int32_t>(), b->LP.as<uint32_t>(), b->child.as<int32_t>(),

@@ -1253,13 +1253,15 @@ static int sortPairsMsd(Context* ctx, uint64_t* keys, int32_t* vals, size_t n, u
 // round 0 for the segments larger than kLocalMax, an LDS sort for each smaller one. With dN, n is the capacity and
 // *dN the count (offsets past it are clipped).
 static int segmentedSortMsd(Context* ctx, uint64_t* keys, int32_t* vals, size_t n, const uint32_t* off, int S,
-                            uint64_t* keyBuf, int32_t* valBuf, void* tmp, const uint32_t* dN)
+                            uint64_t* keyBuf, int32_t* valBuf, void* tmp, const uint32_t* dN, size_t maxSegment)
 {
     const MsdLayout l  = msdLayout(n);
     const MsdLists L   = msdLists(l, tmp);
     uint32_t* tileHist = reinterpret_cast<uint32_t*>(static_cast<char*>(tmp) + l.tileHist);
     msdSegInitKernel<<<1, 256, 0, ctx->stream>>>(off, S, n, dN, L);
-    msdRoundsAndRanges(ctx, l, L, 0, keys, vals, keyBuf, valBuf, n, tileHist);
+    // no segment larger than an LDS sort: the rounds would find nothing to split, so they are not launched
+    msdRoundsAndRanges(ctx, l, L, maxSegment <= size_t(kLocalMax) ? kMsdRounds : 0, keys, vals, keyBuf, valBuf, n,
+                       tileHist);
     RYO_LAUNCH_CHECK(ctx);
     return RYO_OK;
 }
@@ -1363,17 +1365,18 @@ int batchKeys(Context* ctx, const float* x, const float* y, const float* z, size
 /* stable sort of (keys, vals) grouped by segment: a full sort on the keys, then a stable pass on the segment of
  * each value (segment = the s with off[s] <= val < off[s+1]); keys ascend inside each segment and equal keys
  * keep their order. keyBuf/valBuf/segKeys/segBuf/pos/posBuf: n entries each; tmp: sortTempBytes(n). With dN, n is
- * the capacity and *dN (device) the element count, as in sortPairsDigits. */
+ * the capacity and *dN (device) the element count, as in sortPairsDigits. maxSegment: a bound the caller knows on
+ * the longest segment (SIZE_MAX if none). */
 int segmentedSortPairs(Context* ctx, uint64_t* keys, int32_t* vals, size_t n, const uint32_t* off, int S,
                        uint64_t* keyBuf, int32_t* valBuf, uint64_t* segKeys, uint64_t* segBuf, int32_t* pos,
-                       int32_t* posBuf, void* tmp, size_t tmpBytes, const uint32_t* dN)
+                       int32_t* posBuf, void* tmp, size_t tmpBytes, const uint32_t* dN, size_t maxSegment)
 {
     if (n == 0) return RYO_OK;
     if (S >= 1 && S <= kMaxSortSegments && n < (size_t(1) << 31))
     {
         if (tmpBytes < sortTempBytes(n))
             return setError(ctx, RYO_ERR_ARG, "sort: temp storage too small (%zu < %zu)", tmpBytes, sortTempBytes(n));
-        return segmentedSortMsd(ctx, keys, vals, n, off, S, keyBuf, valBuf, tmp, dN);
+        return segmentedSortMsd(ctx, keys, vals, n, off, S, keyBuf, valBuf, tmp, dN, maxSegment);
     }
     int rc = sortPairsDigits(ctx, keys, vals, n, keyBuf, valBuf, tmp, tmpBytes, kPasses, dN);
     if (rc) return rc;
