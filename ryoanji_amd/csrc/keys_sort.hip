@@ -441,8 +441,9 @@ __global__ __launch_bounds__(kSortThreads) void scatterKernel(uint64_t* keysIn, 
 // A run longer than kTieRunMax sends the whole range through every digit instead, from the pairs as they were read
 // (nothing is written before the end). Reads kin/vin, writes kout/vout (which may be the same arrays). Starts and
 // ends with the block in step.
-constexpr int kFastDigits = 3;
-constexpr int kTieRunMax  = 32;
+constexpr int kFastDigits  = 3;
+constexpr int kTieRunMax   = 32;
+constexpr int kLongRunsMax = 8; // more runs longer than kTieRunMax: the whole range goes through every digit
 
 template<int THREADS, int ITEMS>
 struct RangeSortLds
@@ -452,14 +453,19 @@ struct RangeSortLds
     int32_t sv[THREADS * ITEMS];
     uint32_t wcnt[kWaves][kDigits]; // per-wave digit counts, then their prefixes over waves
     uint32_t offs[kDigits];         // digit totals, then their exclusive scan
+    uint32_t numLong;               // runs longer than kTieRunMax after the top digits, and where they start
+    uint32_t longStart[kLongRunsMax];
+    int runEnd;
 };
 
 // stable LSD passes on digits [d0, d1) (d1 > d0) of the items in registers (read from global memory); the sorted
 // range is in L.sk/L.sv afterwards
 template<int THREADS, int ITEMS>
 __device__ __forceinline__ void rangePasses(uint64_t (&k)[ITEMS], int32_t (&v)[ITEMS], int n, int d0, int d1,
-                                            RangeSortLds<THREADS, ITEMS>& L)
+                                            RangeSortLds<THREADS, ITEMS>& L, int base = 0)
 {
+    uint64_t* sk = L.sk + base; // the n pairs sorted here sit at [base, base + n) of the LDS arrays
+    int32_t* sv  = L.sv + base;
     constexpr int kWaves = THREADS / 64;
     const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const uint64_t ltMask = (1ull << lane) - 1ull;
@@ -472,7 +478,7 @@ __device__ __forceinline__ void rangePasses(uint64_t (&k)[ITEMS], int32_t (&v)[I
             for (int r = 0; r < ITEMS; ++r)
             {
                 const int idx = w * (ITEMS * 64) + r * 64 + lane;
-                if (idx < n) k[r] = L.sk[idx], v[r] = L.sv[idx];
+                if (idx < n) k[r] = sk[idx], v[r] = sv[idx];
             }
         }
         for (int i = threadIdx.x; i < kWaves * kDigits; i += THREADS)
@@ -534,21 +540,21 @@ __device__ __forceinline__ void rangePasses(uint64_t (&k)[ITEMS], int32_t (&v)[I
             {
                 const unsigned d  = unsigned(k[r] >> shift) & 255u;
                 const uint32_t at = L.offs[d] + L.wcnt[w][d] + rank[r];
-                L.sk[at] = k[r];
-                L.sv[at] = v[r];
+                sk[at] = k[r];
+                sv[at] = v[r];
             }
         __syncthreads();
     }
 }
 
 // the runs of pairs in L.sk/L.sv whose keys are equal in bits [sh, 64), sorted on the whole key by insertion
-// (stable) by the thread owning each run's first pair; returns 1 to the threads that found a run longer than
-// kTieRunMax (left as it is). Keys in a run share those bits, so a run's neighbours stay recognisable while other
-// runs are being sorted.
+// (stable) by the thread owning each run's first pair; a run longer than kTieRunMax is left as it is and its start
+// recorded in longStart (the first kLongRunsMax of them; *numLong counts them all). Keys in a run share those bits,
+// so a run's neighbours stay recognisable while other runs are being sorted.
 template<int THREADS>
-__device__ __forceinline__ int sortTieRuns(uint64_t* sk, int32_t* sv, int n, int sh)
+__device__ __forceinline__ void sortTieRuns(uint64_t* sk, int32_t* sv, int n, int sh, uint32_t* numLong,
+                                            uint32_t* longStart)
 {
-    int longRun = 0;
 #pragma unroll 1
     for (int idx = threadIdx.x; idx + 1 < n; idx += THREADS)
     {
@@ -559,7 +565,8 @@ __device__ __forceinline__ int sortTieRuns(uint64_t* sk, int32_t* sv, int n, int
             ++end;
         if (end - idx > kTieRunMax)
         {
-            longRun = 1;
+            const uint32_t i = atomicAdd(numLong, 1u);
+            if (i < uint32_t(kLongRunsMax)) longStart[i] = uint32_t(idx);
             continue;
         }
         for (int i = idx + 1; i < end; ++i)
@@ -577,7 +584,26 @@ __device__ __forceinline__ int sortTieRuns(uint64_t* sk, int32_t* sv, int n, int
             sv[j] = val;
         }
     }
-    return longRun;
+}
+
+// the end of the run of keys equal to sk[s] in bits [sh, 64) that starts at s (the whole block, which returns it
+// to every thread): chunks of THREADS pairs, each thread testing one
+template<int THREADS, int ITEMS>
+__device__ __forceinline__ int longRunEnd(int s, int n, int sh, RangeSortLds<THREADS, ITEMS>& L)
+{
+    const uint64_t pfx = L.sk[s] >> sh;
+    for (int c = s + 1; c < n; c += THREADS)
+    {
+        if (threadIdx.x == 0) L.runEnd = n;
+        __syncthreads();
+        const int i = c + int(threadIdx.x);
+        if (i < n && (L.sk[i] >> sh) != pfx) atomicMin(&L.runEnd, i);
+        __syncthreads();
+        const int e = L.runEnd;
+        __syncthreads();
+        if (e < n) return e;
+    }
+    return n;
 }
 
 template<int THREADS, int ITEMS>
@@ -603,11 +629,35 @@ __device__ __forceinline__ void sortRangeLds(const uint64_t* kin, const int32_t*
             k[r] = idx < n ? kin[idx] : 0;
             v[r] = idx < n ? vin[idx] : 0;
         }
+        if (threadIdx.x == 0) L.numLong = 0;
         rangePasses<THREADS, ITEMS>(k, v, n, d0, digits, L);
         if (d0 == 0) break;
-        // the top digits only: fix the runs of keys equal in them, or go through every digit
-        if (!__syncthreads_or(sortTieRuns<THREADS>(L.sk, L.sv, n, 8 * d0))) break;
-        d0 = 0;
+        // the top digits only: the runs of keys equal in them are sorted on the lower digits, short ones by
+        // insertion, the few long ones each by the block in LDS; many long runs: every digit, from the start
+        sortTieRuns<THREADS>(L.sk, L.sv, n, 8 * d0, &L.numLong, L.longStart);
+        __syncthreads();
+        const uint32_t numLong = L.numLong;
+        if (numLong > uint32_t(kLongRunsMax))
+        {
+            __syncthreads();
+            d0 = 0;
+            continue;
+        }
+        for (uint32_t r = 0; r < numLong; ++r)
+        {
+            const int s0  = int(L.longStart[r]);
+            const int len = longRunEnd<THREADS, ITEMS>(s0, n, 8 * d0, L) - s0;
+#pragma unroll
+            for (int q = 0; q < ITEMS; ++q)
+            {
+                const int idx = w * (ITEMS * 64) + q * 64 + lane;
+                k[q] = idx < len ? L.sk[s0 + idx] : 0;
+                v[q] = idx < len ? L.sv[s0 + idx] : 0;
+            }
+            rangePasses<THREADS, ITEMS>(k, v, len, 0, d0, L, s0);
+            __syncthreads();
+        }
+        break;
     }
     __syncthreads(); // the sorted range is in LDS; kout (possibly kin) is written after every read of kin
     for (int i = threadIdx.x; i < n; i += THREADS)

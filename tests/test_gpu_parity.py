@@ -105,12 +105,24 @@ def _clustered_keys(kind, n, rng):
         top = rng.choice([1, 2, 3, 4], n, p=[0.1, 0.2, 0.3, 0.4]).astype(np.uint64)
         return (top << np.uint64(56)) | (rng.integers(0, 64, n, dtype=np.uint64) << np.uint64(48)) | \
             rng.integers(0, 1 << 30, n, dtype=np.uint64)
+    if kind.startswith("clusters"):  # clusters of keys equal in their top 40 bits beside random keys: runs longer
+        # than the insertion limit after a range's top digits (`clusters_<count>_<size>`)
+        _, count, size = kind.split("_")
+        count, size = int(count), int(size)
+        k = rng.integers(0, np.iinfo(np.uint64).max >> np.uint64(1), n, dtype=np.uint64)
+        tops = rng.integers(0, 1 << 39, count, dtype=np.uint64) << np.uint64(24)
+        at = rng.permutation(n)[: count * size].reshape(count, size)
+        for c in range(count):
+            k[at[c]] = tops[c] | rng.integers(0, 1 << 24, size, dtype=np.uint64)
+        return k
     raise ValueError(kind)
 
 
 @pytest.mark.parametrize("kind,n", [("equal", 300000), ("deep", 250000), ("mixed_depth", 1 << 20),
                                     ("removed", 200001), ("small_buckets", 1 << 21), ("sizes", 400000),
-                                    ("deep", 8193)])
+                                    ("deep", 8193), ("clusters_3_1000", 6000), ("clusters_8_500", 8000),
+                                    ("clusters_9_400", 8000), ("clusters_20_5000", 200000),
+                                    ("clusters_4_3000", 1 << 20)])
 def test_sort_clustered_bit_exact(ctx, kind, n):
     """stable order and permutation on key sets that need deep rounds, pack buckets or keep equal-key runs"""
     rng = np.random.default_rng(n + len(kind))
