@@ -880,14 +880,12 @@ __global__ __launch_bounds__(256) void msdSegInitKernel(const uint32_t* __restri
         {
             const uint32_t nt = (len + kSortTile - 1) / kSortTile;
             const uint32_t i = atomicAdd(&nBig, 1u), t0 = atomicAdd(&nTiles, nt);
-            if (i < L.maxSegs && t0 + nt <= L.maxTiles)
-            {
-                L.segs[0][i] = MsdSeg{a, len, t0, 0};
-                for (uint32_t t = 0; t < nt; ++t)
-                    L.owner[0][t0 + t] = i;
-            }
+            if (i < L.maxSegs && t0 + nt <= L.maxTiles) L.segs[0][i] = MsdSeg{a, len, t0, 0};
             else
+            {
+                if (i < L.maxSegs) L.segs[0][i] = MsdSeg{a, 0, 0, 0}; // no tiles of its own (and an error)
                 atomicOr(&L.ctl[kMsdErr], 64u);
+            }
         }
         else if (len > 1)
         {
@@ -898,9 +896,18 @@ __global__ __launch_bounds__(256) void msdSegInitKernel(const uint32_t* __restri
         }
     }
     __syncthreads();
+    // the owners of the large segments' tiles, written by the whole block (a segment may have thousands of tiles)
+    const uint32_t big = min(nBig, L.maxSegs);
+    for (uint32_t i = 0; i < big; ++i)
+    {
+        const MsdSeg g    = L.segs[0][i];
+        const uint32_t nt = (g.len + kSortTile - 1) / kSortTile;
+        for (uint32_t t = threadIdx.x; t < nt; t += blockDim.x)
+            if (g.tileStart + t < L.maxTiles) L.owner[0][g.tileStart + t] = i;
+    }
     if (threadIdx.x == 0)
     {
-        L.ctl[kMsdBig]   = min(nBig, L.maxSegs);
+        L.ctl[kMsdBig]   = big;
         L.ctl[kMsdTiles] = min(nTiles, L.maxTiles);
         for (int c = 0; c < kNumRangeClasses; ++c)
             L.ctl[kMsdRange + c] = min(nRange[c], L.maxRanges);
