@@ -91,6 +91,10 @@ size_t ryo_sort_temp_bytes(size_t n);
  * each, tmp holds ryo_sort_temp_bytes(n) bytes. */
 int ryo_sort_pairs_u64_i32(ryo_ctx* ctx, uint64_t* keys, int32_t* values, size_t n, uint64_t* keyBuf,
                            int32_t* valueBuf, void* tmp, size_t tmpBytes);
+/* No counterpart in the reference: after ryo_sort_pairs_u64_i32 on the same n and tmp, waits for the context's
+ * stream and reports RYO_ERR_HIP if the sort's kernels found offsets or work lists that do not describe the keys
+ * (they skip such accesses instead of reading or writing outside the arrays, so the order would be wrong). */
+int ryo_sort_check(ryo_ctx* ctx, const void* tmp, size_t n);
 /* cstone::sequence (primitives_gpu.cu:265-273) */
 int ryo_sequence_i32(ryo_ctx* ctx, int32_t* out, size_t n, int32_t start);
 

@@ -101,6 +101,7 @@ SIGNATURES = {
     "ryo_sfc_keys_f64": (_I, [_P, _P, _P, _P, _P, _SZ, _P, _I, _I]),
     "ryo_sort_temp_bytes": (_SZ, [_SZ]),
     "ryo_sort_pairs_u64_i32": (_I, [_P, _P, _P, _SZ, _P, _P, _P, _SZ]),
+    "ryo_sort_check": (_I, [_P, _P, _SZ]),
     "ryo_sequence_i32": (_I, [_P, _P, _SZ, C.c_int32]),
     "ryo_gather_f32": (_I, [_P, _P, _SZ, _I, _P, _P]),
     "ryo_gather_f64": (_I, [_P, _P, _SZ, _I, _P, _P]),

@@ -17,6 +17,7 @@ int sfcKeys32(Context*, const T*, const T*, const T*, uint32_t*, size_t, const T
 size_t sortTempBytes(size_t n);
 int sortPairs(Context*, uint64_t*, int32_t*, size_t, uint64_t*, int32_t*, void*, size_t);
 int sortPairsIndex(Context*, uint64_t*, int32_t*, size_t, uint64_t*, int32_t*, void*, size_t);
+int sortCheck(Context*, const void*, size_t);
 int sequence(Context*, int32_t*, size_t, int32_t);
 int gather(Context*, const int32_t*, size_t, int, const float* const*, float* const*);
 int gather64(Context*, const int32_t*, size_t, int, const double* const*, double* const*);
@@ -253,6 +254,14 @@ int ryo_sort_pairs_u64_i32(ryo_ctx* h, uint64_t* keys, int32_t* values, size_t n
     if (n && (!keys || !values || !keyBuf || !valueBuf || !tmp)) return setError(ctx, RYO_ERR_ARG, "sort: null pointer");
     if (n >= (size_t(1) << 31)) return setError(ctx, RYO_ERR_ARG, "sort: n must be < 2^31");
     return sortPairs(ctx, keys, values, n, keyBuf, valueBuf, tmp, tmpBytes);
+}
+
+int ryo_sort_check(ryo_ctx* h, const void* tmp, size_t n)
+{
+    CTX_OR_FAIL(h);
+    if (n && !tmp) return setError(ctx, RYO_ERR_ARG, "sort check: null pointer");
+    RYO_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return sortCheck(ctx, tmp, n);
 }
 
 int ryo_sequence_i32(ryo_ctx* h, int32_t* out, size_t n, int32_t start)

@@ -1323,6 +1323,18 @@ static int segmentedSortMsd(Context* ctx, uint64_t* keys, int32_t* vals, size_t 
     return RYO_OK;
 }
 
+// the guard word of the last sortPairs / sortPairsIndex of n pairs in tmp (the caller has synchronized the stream):
+// nonzero when a kernel found offsets or list entries that do not describe the keys (it then skipped the access)
+int sortCheck(Context* ctx, const void* tmp, size_t n)
+{
+    if (n <= kBlockSortMax || n >= (size_t(1) << 31)) return RYO_OK; // the one-block sort has no lists to check
+    uint32_t err = 0;
+    const char* w = static_cast<const char*>(tmp) + msdLayout(n).ctl + sizeof(uint32_t) * kMsdErr;
+    RYO_HIP(ctx, hipMemcpy(&err, w, sizeof(err), hipMemcpyDeviceToHost));
+    if (err) return setError(ctx, RYO_ERR_HIP, "sort: inconsistent offsets or lists (guard bits 0x%x)", err);
+    return RYO_OK;
+}
+
 // the full 64-bit stable sort; tmp: sortTempBytes(n) bytes
 int sortPairs(Context* ctx, uint64_t* keys, int32_t* vals, size_t n, uint64_t* keyBuf, int32_t* valBuf, void* tmp,
               size_t tmpBytes)

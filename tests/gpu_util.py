@@ -54,6 +54,7 @@ def gpu_sort(ctx, keys, values=None):
     tb = R.lib().ryo_sort_temp_bytes(n)
     tmp = zeros(tb, torch.uint8)
     ctx.call("ryo_sort_pairs_u64_i32", R.ptr(k), R.ptr(v), n, R.ptr(kb), R.ptr(vb), R.ptr(tmp), tb)
+    ctx.call("ryo_sort_check", R.ptr(tmp), n)  # no kernel met offsets or lists inconsistent with the keys
     return host(k, np.uint64), host(v)
 
 
