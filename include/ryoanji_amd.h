@@ -467,6 +467,7 @@ int ryo_system_accuracy(ryo_system* sys, int num_targets, uint64_t seed, double*
  * (GroupView offsets, cstone/traversal/groups.hpp:28-34, are the reference's per-batch table; the reference
  * solves one system per call). Systems are open unless ryo_batch_set_periodic marks them periodic. */
 typedef struct ryo_batch ryo_batch;
+/* max_systems at most 65536 (RYO_ERR_UNSUPPORTED beyond: each system is a segment of the batch's sorts) */
 int ryo_batch_create(ryo_ctx* ctx, size_t max_particles, int max_systems, ryo_batch** batch);
 int ryo_batch_destroy(ryo_batch* batch);
 /* counts[S] particles per system (each >= 1); x, y, z, m, h: the systems packed one after the other (device or

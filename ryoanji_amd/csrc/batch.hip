@@ -35,8 +35,8 @@ MacBox makeMacBox(const float* limits, float invTheta);
 int batchKeys(Context* ctx, const float* x, const float* y, const float* z, size_t n, const uint32_t* P, int S,
               const KeyBox* boxes, uint64_t* keys, int32_t* vals);
 int segmentedSortPairs(Context* ctx, uint64_t* keys, int32_t* vals, size_t n, const uint32_t* off, int S,
-                       uint64_t* keyBuf, int32_t* valBuf, uint64_t* segKeys, uint64_t* segBuf, int32_t* pos,
-                       int32_t* posBuf, void* tmp, size_t tmpBytes, const uint32_t* dN, size_t maxSegment);
+                       uint64_t* keyBuf, int32_t* valBuf, void* tmp, size_t tmpBytes, const uint32_t* dN,
+                       size_t maxSegment);
 int copyWords(Context* ctx, void* dst, const void* src, size_t words);
 int zeroWords(Context* ctx, void* dst, size_t words);
 int batchNodeTables(Context* ctx, const BatchView& bv, uint32_t capSlots, uint32_t capNodes, uint32_t capParents,
@@ -119,13 +119,13 @@ struct ryo_batch
 
     // particles: loaded (packed, original order) and sorted (x, y, z, m, h, original index)
     DevBuf in[5], srt[6];
-    DevBuf keys, keyBuf, segKeys, segBuf, vals, valBuf, pos, posBuf, sortTmp;
+    DevBuf keys, keyBuf, vals, valBuf, sortTmp;
     DevBuf p, ax, ay, az;
     // tables (u32 offsets: P, TS, LO, NO, NIO, PO, GO; LP; LR) and per-system boxes
     DevBuf tab, LP, LR, keyBoxes, macBoxes, G, sysPot, blockSums;
     // trees
     DevBuf leaves, layout, counts, prefixes, child, parents, i2l, l2i, centers, mp, c64, q64;
-    DevBuf nodeIdx, nodeKeyBuf, nodeValBuf, nodeSeg, nodeSegBuf, nodePos, nodePosBuf, nodeTmp;
+    DevBuf nodeIdx, nodeKeyBuf, nodeValBuf, nodeTmp;
     DevBuf gStart, gEnd, gSys;
     DevBuf travWork, flags; // traversal work arrays; flags[0]: the trees outgrew the node arrays
     DevBuf images, roots;   // per-system periodic images; the root multipoles of the periodic systems
@@ -166,14 +166,13 @@ void allBufs(ryo_batch* b, std::vector<DevBuf*>& out)
 {
     for (auto& d : b->in) out.push_back(&d);
     for (auto& d : b->srt) out.push_back(&d);
-    DevBuf* rest[] = {&b->keys,     &b->keyBuf,   &b->segKeys,  &b->segBuf,     &b->vals,       &b->valBuf,
-                      &b->pos,      &b->posBuf,   &b->sortTmp,  &b->p,          &b->ax,         &b->ay,
-                      &b->az,       &b->tab,      &b->LP,       &b->LR,         &b->keyBoxes,   &b->macBoxes,
-                      &b->G,        &b->sysPot,   &b->blockSums, &b->leaves,    &b->layout,     &b->counts,
-                      &b->prefixes, &b->child,    &b->parents,  &b->i2l,        &b->l2i,        &b->centers,
-                      &b->mp,       &b->c64,      &b->q64,      &b->nodeIdx,    &b->nodeKeyBuf, &b->nodeValBuf,
-                      &b->nodeSeg,  &b->nodeSegBuf, &b->nodePos, &b->nodePosBuf, &b->nodeTmp,   &b->gStart,
-                      &b->gEnd,     &b->gSys,     &b->travWork, &b->flags,     &b->images,    &b->roots};
+    DevBuf* rest[] = {&b->keys,     &b->keyBuf,   &b->vals,     &b->valBuf,     &b->sortTmp,    &b->p,
+                      &b->ax,       &b->ay,       &b->az,       &b->tab,        &b->LP,         &b->LR,
+                      &b->keyBoxes, &b->macBoxes, &b->G,        &b->sysPot,     &b->blockSums,  &b->leaves,
+                      &b->layout,   &b->counts,   &b->prefixes, &b->child,      &b->parents,    &b->i2l,
+                      &b->l2i,      &b->centers,  &b->mp,       &b->c64,        &b->q64,        &b->nodeIdx,
+                      &b->nodeKeyBuf, &b->nodeValBuf, &b->nodeTmp, &b->gStart,  &b->gEnd,       &b->gSys,
+                      &b->travWork, &b->flags,    &b->images,   &b->roots};
     for (DevBuf* d : rest) out.push_back(d);
 }
 
@@ -303,8 +302,7 @@ int runFront(ryo_batch* b, const Shape& sh, uint32_t capSlots, uint32_t capNodes
     int rc;
     if ((rc = batchKeys(ctx, in[0], in[1], in[2], N, bv.P, S, b->keyBoxes.as<KeyBox>(), keys, vals))) return rc;
     if ((rc = segmentedSortPairs(ctx, keys, vals, N, bv.P, S, b->keyBuf.as<uint64_t>(), b->valBuf.as<int32_t>(),
-                                 b->segKeys.as<uint64_t>(), b->segBuf.as<uint64_t>(), b->pos.as<int32_t>(),
-                                 b->posBuf.as<int32_t>(), b->sortTmp.p, b->sortTmp.size, nullptr, maxSystem(b))))
+                                 b->sortTmp.p, b->sortTmp.size, nullptr, maxSystem(b))))
         return rc;
     if ((rc = gather(ctx, vals, N, 5, in, out))) return rc;
     if ((rc = copyWords(ctx, b->srt[5].p, vals, N))) return rc;
@@ -325,13 +323,6 @@ int ensureNodes(ryo_batch* b, uint32_t slots1, uint32_t nodes1, uint32_t parents
         if (!rc) rc = d->ensure(ctx, nn * 8);
     for (DevBuf* d : {&b->child, &b->i2l, &b->l2i, &b->nodeIdx, &b->nodeValBuf})
         if (!rc) rc = d->ensure(ctx, nn * 4);
-    if (b->capS > kMaxSortSegments)
-    {
-        for (DevBuf* d : {&b->nodeSeg, &b->nodeSegBuf})
-            if (!rc) rc = d->ensure(ctx, nn * 8);
-        for (DevBuf* d : {&b->nodePos, &b->nodePosBuf})
-            if (!rc) rc = d->ensure(ctx, nn * 4);
-    }
     if (!rc) rc = b->parents.ensure(ctx, np * 4);
     if (!rc) rc = b->centers.ensure(ctx, nn * 16);
     if (!rc) rc = b->mp.ensure(ctx, nn * 32);
@@ -371,9 +362,7 @@ int runBack(ryo_batch* b, const Shape& sh, uint32_t slots, uint32_t nodes, uint3
     auto* idx = b->nodeIdx.as<int32_t>();
     if ((rc = batchUnsortedLayout(ctx, b->leaves.as<uint64_t>(), bv, slots, pre, idx))) return rc;
     if ((rc = segmentedSortPairs(ctx, pre, idx, nodes, bv.NO, S, b->nodeKeyBuf.as<uint64_t>(),
-                                 b->nodeValBuf.as<int32_t>(), b->nodeSeg.as<uint64_t>(),
-                                 b->nodeSegBuf.as<uint64_t>(), b->nodePos.as<int32_t>(),
-                                 b->nodePosBuf.as<int32_t>(), b->nodeTmp.p, b->nodeTmp.size,
+                                 b->nodeValBuf.as<int32_t>(), b->nodeTmp.p, b->nodeTmp.size,
                                  captured ? bv.NO + S : nullptr, maxNodes(maxSystem(b), sh.B))))
         return rc;
     if ((rc = batchLinkFinish(ctx, pre, idx, bv, nodes, internal, parents, b->i2l.as<int32_t>(),
@@ -699,6 +688,9 @@ int ryo_batch_create(ryo_ctx* h, size_t max_particles, int max_systems, ryo_batc
 {
     if (!h || !out || max_particles == 0 || max_systems < 1) return RYO_ERR_ARG;
     Context* ctx = &h->c;
+    if (max_systems > kMaxSortSegments)
+        return setError(ctx, RYO_ERR_UNSUPPORTED, "batch: at most %d systems (%d asked)", kMaxSortSegments,
+                        max_systems);
     auto* b      = new ryo_batch();
     b->ctx  = ctx;
     b->capN = max_particles;
@@ -713,15 +705,6 @@ int ryo_batch_create(ryo_ctx* h, size_t max_particles, int max_systems, ryo_batc
         if (!rc) rc = d->ensure(ctx, n * 8);
     for (DevBuf* d : {&b->vals, &b->valBuf, &b->p, &b->ax, &b->ay, &b->az})
         if (!rc) rc = d->ensure(ctx, n * 4);
-    // the segment keys and positions of the two-sort segmented sort, only for batches of more systems than the
-    // segmented sort takes one at a time
-    if (max_systems > kMaxSortSegments)
-    {
-        for (DevBuf* d : {&b->segKeys, &b->segBuf})
-            if (!rc) rc = d->ensure(ctx, n * 8);
-        for (DevBuf* d : {&b->pos, &b->posBuf})
-            if (!rc) rc = d->ensure(ctx, n * 4);
-    }
     if (!rc) rc = b->sortTmp.ensure(ctx, sortTempBytes(n));
     if (!rc) rc = b->tab.ensure(ctx, 7 * S1 * 4);
     if (!rc) rc = b->LP.ensure(ctx, (kMaxLevel + 1) * S1 * 4);

@@ -1383,30 +1383,7 @@ __global__ __launch_bounds__(256) void batchKeysKernel(const float* __restrict__
     }
 }
 
-// the segment of each (sorted) value as the key of a second, stable pass; pos = current position
-__global__ void segmentKeysKernel(const int32_t* __restrict__ vals, size_t n, const uint32_t* __restrict__ dN,
-                                  const uint32_t* __restrict__ off, int S, uint64_t* __restrict__ segKeys,
-                                  int32_t* __restrict__ pos)
-{
-    n = deviceCount(dN, n);
-    for (size_t j = size_t(blockIdx.x) * blockDim.x + threadIdx.x; j < n; j += size_t(gridDim.x) * blockDim.x)
-    {
-        segKeys[j] = uint64_t(segmentOf(off, S, uint32_t(vals[j])));
-        pos[j]     = int32_t(j);
-    }
-}
 
-__global__ void applyPosKernel(const int32_t* __restrict__ pos, size_t n, const uint32_t* __restrict__ dN,
-                               const uint64_t* __restrict__ kin, const int32_t* __restrict__ vin,
-                               uint64_t* __restrict__ kout, int32_t* __restrict__ vout)
-{
-    n = deviceCount(dN, n);
-    for (size_t j = size_t(blockIdx.x) * blockDim.x + threadIdx.x; j < n; j += size_t(gridDim.x) * blockDim.x)
-    {
-        kout[j] = kin[pos[j]];
-        vout[j] = vin[pos[j]];
-    }
-}
 
 KeyBox makeKeyBox(const float* limits)
 {
@@ -1431,35 +1408,21 @@ int batchKeys(Context* ctx, const float* x, const float* y, const float* z, size
     return RYO_OK;
 }
 
-/* stable sort of (keys, vals) grouped by segment: a full sort on the keys, then a stable pass on the segment of
- * each value (segment = the s with off[s] <= val < off[s+1]); keys ascend inside each segment and equal keys
- * keep their order. keyBuf/valBuf/segKeys/segBuf/pos/posBuf: n entries each; tmp: sortTempBytes(n). With dN, n is
- * the capacity and *dN (device) the element count, as in sortPairsDigits. maxSegment: a bound the caller knows on
- * the longest segment (SIZE_MAX if none). */
+/* stable sort of (keys, vals) grouped by segment: each segment [off[s], off[s+1]) of the pairs (1 <= S <=
+ * kMaxSortSegments, fewer than 2^31 pairs) sorted on its own, keys ascending and equal keys in their order.
+ * keyBuf/valBuf: n entries each; tmp: sortTempBytes(n). With dN, n is the capacity and *dN (device) the element
+ * count (offsets past it are clipped). maxSegment: a bound the caller knows on the longest segment (SIZE_MAX if
+ * none). */
 int segmentedSortPairs(Context* ctx, uint64_t* keys, int32_t* vals, size_t n, const uint32_t* off, int S,
-                       uint64_t* keyBuf, int32_t* valBuf, uint64_t* segKeys, uint64_t* segBuf, int32_t* pos,
-                       int32_t* posBuf, void* tmp, size_t tmpBytes, const uint32_t* dN, size_t maxSegment)
+                       uint64_t* keyBuf, int32_t* valBuf, void* tmp, size_t tmpBytes, const uint32_t* dN,
+                       size_t maxSegment)
 {
     if (n == 0) return RYO_OK;
-    if (S >= 1 && S <= kMaxSortSegments && n < (size_t(1) << 31))
-    {
-        if (tmpBytes < sortTempBytes(n))
-            return setError(ctx, RYO_ERR_ARG, "sort: temp storage too small (%zu < %zu)", tmpBytes, sortTempBytes(n));
-        return segmentedSortMsd(ctx, keys, vals, n, off, S, keyBuf, valBuf, tmp, dN, maxSegment);
-    }
-    int rc = sortPairsDigits(ctx, keys, vals, n, keyBuf, valBuf, tmp, tmpBytes, kPasses, dN);
-    if (rc) return rc;
-    if (S <= 1) return RYO_OK;
-    hipStream_t s = ctx->stream;
-    const unsigned grid = gridFor(n, 256, ctx->numCUs * 16);
-    segmentKeysKernel<<<grid, 256, 0, s>>>(vals, n, dN, off, S, segKeys, pos);
-    const int digits = S <= 256 ? 1 : (S <= 65536 ? 2 : 3);
-    rc = sortPairsDigits(ctx, segKeys, pos, n, segBuf, posBuf, tmp, tmpBytes, digits, dN);
-    if (rc) return rc;
-    applyPosKernel<<<grid, 256, 0, s>>>(pos, n, dN, keys, vals, keyBuf, valBuf);
-    copyPairsKernel<<<grid, 256, 0, s>>>(keyBuf, valBuf, n, dN, keys, vals);
-    RYO_LAUNCH_CHECK(ctx);
-    return RYO_OK;
+    if (S < 1 || S > kMaxSortSegments || n >= (size_t(1) << 31))
+        return setError(ctx, RYO_ERR_ARG, "segmented sort: %d segments, %zu pairs", S, n);
+    if (tmpBytes < sortTempBytes(n))
+        return setError(ctx, RYO_ERR_ARG, "sort: temp storage too small (%zu < %zu)", tmpBytes, sortTempBytes(n));
+    return segmentedSortMsd(ctx, keys, vals, n, off, S, keyBuf, valBuf, tmp, dN, maxSegment);
 }
 
 template<class T>
