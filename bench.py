@@ -293,22 +293,28 @@ def serving_bench(num=32, slots=8, seed=5):
     out["served_accuracy"] = {"targets_per_request": 256,
                               "max_err_p99_by_theta": {f"{k:.1f}": v for k, v in sorted(worst.items())},
                               "max_err_median": max(r["err_median"] for r in racc)}
+    # the same server admitting least work first (a request that has waited 2 s goes first)
+    least = S.Server(slots, max_n, hbm_budget=64 << 30, order="least_work", max_wait_ms=2000.0)
+    out["batched_saturated_least_work"] = S.summarize(systems, params, least.run(systems, params))
     # latency under Poisson arrivals at fractions of the one-slot server's capacity
     rates = []
     cap_rps = s1["requests_per_s"]
+    keys = ("p50_ms", "p99_ms", "requests_per_s")
     for frac in (0.5, 2.0):
         rate = frac * cap_rps
         _, p = S.make_stream(num, rate_per_s=rate, seed=seed)
         params["arrival_ms"] = p["arrival_ms"]
         r = mixed.run(systems, params)
-        rates.append(dict(rate_per_s=rate, **{k: v for k, v in S.summarize(systems, params, r).items()
-                                              if k in ("p50_ms", "p99_ms", "requests_per_s")}))
+        rates.append(dict(rate_per_s=rate, **{k: v for k, v in S.summarize(systems, params, r).items() if k in keys}))
+        sl = S.summarize(systems, params, least.run(systems, params))
+        rates[-1]["least_work_p50_ms"], rates[-1]["least_work_p99_ms"] = sl["p50_ms"], sl["p99_ms"]
         r1 = S.Server(1, max_n, batch_max_n=0)
         rr = r1.run(systems, params)
         r1.close()
         rates[-1]["one_slot_p50_ms"] = S.summarize(systems, params, rr)["p50_ms"]
         rates[-1]["one_slot_p99_ms"] = S.summarize(systems, params, rr)["p99_ms"]
     mixed.close()
+    least.close()
     out["poisson"] = rates
     return out
 

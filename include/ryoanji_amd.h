@@ -552,6 +552,12 @@ int ryo_server_create(int slots, size_t hbm_budget, size_t max_n, ryo_server** s
  * periodic wrap or box refit, and solve). Results of lane members report slot = `slots`. Set before
  * ryo_server_run. */
 int ryo_server_set_batching(ryo_server* server, size_t max_n, size_t capacity, int max_systems);
+/* Admission order of waiting requests (each still first fit: a request that does not fit the free places or the
+ * HBM budget lets later ones in). order 0 (default): arrival. order 1: least work first, work = n x (steps + 1)
+ * solves; with max_wait_ms > 0 requests that have waited that long go ahead of the others, in arrival order, so
+ * none waits forever behind smaller ones. The order changes when requests run, not what they compute. Set before
+ * ryo_server_run. */
+int ryo_server_set_order(ryo_server* server, int order, double max_wait_ms);
 int ryo_server_destroy(ryo_server* server);
 /* estimated HBM bytes of a resident system of n particles (the admission weight) */
 size_t ryo_server_footprint(size_t n);

@@ -86,6 +86,7 @@ SIGNATURES = {
     "ryo_server_create": (_I, [_I, _SZ, _SZ, C.POINTER(_P)]),
     "ryo_server_destroy": (_I, [_P]),
     "ryo_server_set_batching": (_I, [_P, _SZ, _SZ, _I]),
+    "ryo_server_set_order": (_I, [_P, _I, C.c_double]),
     "ryo_server_footprint": (_SZ, [_SZ]),
     "ryo_server_run": (_I, [_P, _I, C.POINTER(Request), C.POINTER(Result), _P]),
     "ryo_ctx_create": (_I, [C.POINTER(_P), _P]),

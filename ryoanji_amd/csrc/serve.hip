@@ -2,7 +2,7 @@
 //
 // Requests (particle systems of any size, each running some leapfrog steps) arrive over time. A dispatcher
 // admits an arrived request as soon as it has a place AND its HBM footprint fits in the remaining budget (first
-// fit in arrival order). Two kinds of place:
+// fit in arrival order, or least work first with a cap on the wait, ryo_server_set_order). Two kinds of place:
 //   * a slot: a HIP stream, a context and a resident system of its own, for requests of any size; admitted
 //     requests on different slots run concurrently and their kernels overlap on the device;
 //   * the batch lane, for small requests (open or periodic): one stream, context and ryo_batch whose members are
@@ -21,7 +21,9 @@
 #include <deque>
 #include <mutex>
 #include <thread>
+#include <algorithm>
 #include <array>
+#include <tuple>
 
 namespace
 {
@@ -175,6 +177,10 @@ struct ryo_server
     int laneCapS    = 1024;
     Lane lane;
     bool laneReady = false;
+    // admission order: 0 arrival; 1 least work (particles x solves) first, requests that have waited maxWaitMs
+    // or longer (when it is positive) ahead of the others in arrival order
+    int order        = 0;
+    double maxWaitMs = 0.0;
 };
 
 extern "C" {
@@ -213,6 +219,14 @@ int ryo_server_set_batching(ryo_server* s, size_t max_n, size_t capacity, int ma
     s->laneMaxN = max_n;
     s->laneCapN = capacity;
     s->laneCapS = max_systems;
+    return RYO_OK;
+}
+
+int ryo_server_set_order(ryo_server* s, int order, double max_wait_ms)
+{
+    if (!s || order < 0 || order > 1 || !(max_wait_ms >= 0.0)) return RYO_ERR_ARG;
+    s->order     = order;
+    s->maxWaitMs = max_wait_ms;
     return RYO_OK;
 }
 
@@ -528,7 +542,18 @@ int ryo_server_run(ryo_server* s, int num, const ryo_request* req, ryo_result* r
                 res[next].arrival_ms = req[next].arrival_ms;
                 waiting.push_back(next++);
             }
-            // first fit in arrival order: the lane for a small request while it has room, else an idle slot;
+            if (s->order == 1 && waiting.size() > 1)
+            {
+                // least work first (the initial solve and one per step, times the particles); with a positive
+                // maxWaitMs, requests that have waited that long go ahead of the others, in arrival order
+                const double tNow = now();
+                auto key = [&](int j) {
+                    const bool aged = s->maxWaitMs > 0.0 && tNow - req[j].arrival_ms >= s->maxWaitMs;
+                    return std::make_tuple(aged ? 0 : 1, aged ? 0.0 : double(req[j].n) * double(req[j].steps + 1), j);
+                };
+                std::sort(waiting.begin(), waiting.end(), [&](int a, int b) { return key(a) < key(b); });
+            }
+            // first fit in the admission order: the lane for a small request while it has room, else an idle slot;
             // both within the HBM budget
             for (auto it = waiting.begin(); it != waiting.end();)
             {

@@ -32,9 +32,12 @@ class Server:
     library's default; 0: no lane, every request on a slot). Each slot and the lane run on a stream of their own:
     give the process at least slots + 1 hardware queues (the environment variable GPU_MAX_HW_QUEUES, read when HIP
     initialises; HIP's default is 4, shared round-robin between streams), or a stream's short kernels wait behind
-    another's long ones in a shared queue."""
+    another's long ones in a shared queue. order "arrival" admits waiting requests first fit in arrival order;
+    "least_work" admits the one with the fewest particle-solves first, except that requests that have waited
+    max_wait_ms (when positive) go first in arrival order."""
 
-    def __init__(self, slots, max_n, hbm_budget=0, batch_max_n=None, batch_capacity=1 << 22, batch_systems=1024):
+    def __init__(self, slots, max_n, hbm_budget=0, batch_max_n=None, batch_capacity=1 << 22, batch_systems=1024,
+                 order="arrival", max_wait_ms=0.0):
         self.h = _P()
         rc = lib().ryo_server_create(int(slots), int(hbm_budget), int(max_n), C.byref(self.h))
         if rc:
@@ -43,6 +46,9 @@ class Server:
             rc = lib().ryo_server_set_batching(self.h, int(batch_max_n), int(batch_capacity), int(batch_systems))
             if rc:
                 raise RuntimeError(f"ryo_server_set_batching failed ({rc})")
+        rc = lib().ryo_server_set_order(self.h, {"arrival": 0, "least_work": 1}[order], float(max_wait_ms))
+        if rc:
+            raise RuntimeError(f"ryo_server_set_order failed ({rc})")
         self.slots = slots
 
     def run(self, systems, params, keep_state=False, arrival_scale=1.0, accuracy_targets=0):

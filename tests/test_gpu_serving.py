@@ -221,3 +221,25 @@ def test_admission_budget_limits_concurrency(gpu):
         live += d
         peak = max(peak, live)
     assert peak <= 2
+
+
+def test_least_work_order(gpu):
+    """one slot, no lane, everything queued at t=0: least work first admits the requests in ascending
+    n x (steps + 1) and computes what arrival order computes; a wait cap every request has already reached gives
+    arrival order back"""
+    systems, params = S.make_stream(8, rate_per_s=0, seed=6, lo_exp=10, hi_exp=14, steps=(1, 3))
+    outs = {}
+    for name, order, cap in (("arrival", "arrival", 0.0), ("least", "least_work", 0.0),
+                             ("capped", "least_work", 1e-6)):
+        srv = S.Server(slots=1, max_n=1 << 14, batch_max_n=0, order=order, max_wait_ms=cap)
+        outs[name] = srv.run(systems, params, keep_state=True)
+        srv.close()
+        assert all(r["status"] == 0 for r in outs[name])
+    work = [s.n * (int(k) + 1) for s, k in zip(systems, params["steps"])]
+    by_admit = lambda out: [i for i, _ in sorted(enumerate(out), key=lambda e: e[1]["admit"])]
+    assert by_admit(outs["least"]) == sorted(range(len(systems)), key=lambda i: (work[i], i))
+    assert by_admit(outs["arrival"]) == list(range(len(systems)))
+    assert by_admit(outs["capped"]) == list(range(len(systems)))
+    assert by_admit(outs["least"]) != by_admit(outs["arrival"])  # the seed gives a stream out of work order
+    for a, b in zip(outs["arrival"], outs["least"]):
+        assert np.array_equal(a["state"], b["state"]) and a["pot"] == b["pot"]

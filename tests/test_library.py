@@ -56,3 +56,8 @@ def test_params_mirror_the_header():
     assert (p.bucket_size, p.theta, p.G, p.group_size, p.num_shells, p.ewald) == (64, 0.5, 1.0, 64, 0, 0)
     assert list(p.ewald_settings) == [1, 2.6, 2.8, 2.0, 3.0e-3]
     assert (p.init_node, p.soft_mac, p.group_split_tol, p.multipole) == (0, 0, 0.0, 0)
+
+
+def test_server_order_arguments_are_checked_on_the_host():
+    lib = R.lib()
+    assert lib.ryo_server_set_order(None, 1, 0.0) != 0
