@@ -304,6 +304,28 @@ __device__ inline int xcdTile(int b, int numTiles)
     return x * per + min(x, extra) + k;
 }
 
+// The lanes of the wave that hold this lane's 8-bit digit d, among the lanes of the wave-uniform mask `valid`: one
+// ballot per bit, after which each lane keeps the lanes whose bit agrees with its own (the bit spread over a whole
+// word, then xnor with the ballot, per 32-lane half). `below`: how many of them sit in lower lanes (this lane's rank
+// among them); `count`: how many there are. The highest of them is the lane with below + 1 == count.
+struct WaveMatch
+{
+    uint32_t below, count;
+};
+__device__ __forceinline__ WaveMatch waveMatch(unsigned d, uint64_t valid)
+{
+    uint32_t lo = uint32_t(valid), hi = uint32_t(valid >> 32);
+#pragma unroll
+    for (int b = 0; b < 8; ++b)
+    {
+        const uint32_t fill = uint32_t(int32_t(d << (31 - b)) >> 31); // all ones where bit b of d is set
+        const uint64_t bb   = __builtin_amdgcn_ballot_w64(fill != 0u);
+        lo &= ~(uint32_t(bb) ^ fill);
+        hi &= ~(uint32_t(bb >> 32) ^ fill);
+    }
+    return {__builtin_amdgcn_mbcnt_hi(hi, __builtin_amdgcn_mbcnt_lo(lo, 0u)), uint32_t(__popc(lo) + __popc(hi))};
+}
+
 // LDS of one tile's scatter
 struct ScatterLds
 {
@@ -334,7 +356,6 @@ __device__ __forceinline__ void scatterTile(const uint64_t* __restrict__ keysIn,
     uint64_t k[kSortItems];
     int32_t v[kSortItems];
     uint32_t rank[kSortItems];
-    const uint64_t ltMask = (1ull << lane) - 1ull;
 #pragma unroll
     for (int r = 0; r < kSortItems; ++r)
     {
@@ -343,18 +364,10 @@ __device__ __forceinline__ void scatterTile(const uint64_t* __restrict__ keysIn,
         k[r] = valid ? keysIn[base + idx] : 0;
         v[r] = valid ? (valsIn ? valsIn[base + idx] : int32_t(base + idx)) : 0; // no values: the index
         unsigned d = unsigned(k[r] >> shift) & 255u;
-        uint64_t match = __ballot(valid);
-#pragma unroll
-        for (int b = 0; b < 8; ++b)
-        {
-            bool bit = (d >> b) & 1u;
-            uint64_t bb = __ballot(bit);
-            match &= bit ? bb : ~bb;
-        }
+        const WaveMatch m = waveMatch(d, __ballot(valid));
         uint32_t pre = valid ? L.wcnt[w][d] : 0;
-        rank[r] = pre + __popcll(match & ltMask);
-        bool leader = valid && (match >> lane) == 1ull;
-        if (leader) L.wcnt[w][d] = pre + __popcll(match);
+        rank[r] = pre + m.below;
+        if (valid && m.below + 1 == m.count) L.wcnt[w][d] = pre + m.count;
     }
     __syncthreads();
     {
@@ -468,7 +481,6 @@ __device__ __forceinline__ void rangePasses(uint64_t (&k)[ITEMS], int32_t (&v)[I
     int32_t* sv  = L.sv + base;
     constexpr int kWaves = THREADS / 64;
     const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const uint64_t ltMask = (1ull << lane) - 1ull;
     for (int p = d0; p < d1; ++p)
     {
         const int shift = 8 * p;
@@ -485,24 +497,19 @@ __device__ __forceinline__ void rangePasses(uint64_t (&k)[ITEMS], int32_t (&v)[I
             (&L.wcnt[0][0])[i] = 0;
         __syncthreads();
         uint32_t rank[ITEMS];
+        unsigned dig[ITEMS];
 #pragma unroll
         for (int r = 0; r < ITEMS; ++r)
         {
             rank[r] = 0;
+            dig[r]  = unsigned(k[r] >> shift) & 255u;
             if (w * (ITEMS * 64) + r * 64 >= n) continue; // the whole wave: nothing in this slot
-            const bool valid = w * (ITEMS * 64) + r * 64 + lane < n;
-            const unsigned d = unsigned(k[r] >> shift) & 255u;
-            uint64_t match = __ballot(valid);
-#pragma unroll
-            for (int b = 0; b < 8; ++b)
-            {
-                const bool bit = (d >> b) & 1u;
-                const uint64_t bb = __ballot(bit);
-                match &= bit ? bb : ~bb;
-            }
+            const bool valid  = w * (ITEMS * 64) + r * 64 + lane < n;
+            const unsigned d  = dig[r];
+            const WaveMatch m = waveMatch(d, __ballot(valid));
             const uint32_t pre = valid ? L.wcnt[w][d] : 0;
-            rank[r] = pre + __popcll(match & ltMask);
-            if (valid && (match >> lane) == 1ull) L.wcnt[w][d] = pre + __popcll(match);
+            rank[r] = pre + m.below;
+            if (valid && m.below + 1 == m.count) L.wcnt[w][d] = pre + m.count;
         }
         __syncthreads();
         for (int d = threadIdx.x; d < kDigits; d += THREADS)
@@ -538,7 +545,7 @@ __device__ __forceinline__ void rangePasses(uint64_t (&k)[ITEMS], int32_t (&v)[I
         for (int r = 0; r < ITEMS; ++r)
             if (w * (ITEMS * 64) + r * 64 + lane < n)
             {
-                const unsigned d  = unsigned(k[r] >> shift) & 255u;
+                const unsigned d  = dig[r];
                 const uint32_t at = L.offs[d] + L.wcnt[w][d] + rank[r];
                 sk[at] = k[r];
                 sv[at] = v[r];
