@@ -305,24 +305,27 @@ __device__ inline int xcdTile(int b, int numTiles)
 }
 
 // The lanes of the wave that hold this lane's 8-bit digit d, among the lanes of the wave-uniform mask `valid`: one
-// ballot per bit, after which each lane keeps the lanes whose bit agrees with its own (the bit spread over a whole
-// word, then xnor with the ballot, per 32-lane half). `below`: how many of them sit in lower lanes (this lane's rank
-// among them); `count`: how many there are. The highest of them is the lane with below + 1 == count.
+// ballot per bit, after which each lane rules out the lanes whose bit differs from its own (the ballot xor the bit
+// spread over a whole word, per 32-lane half). 4 VALU per bit on gfx950: the sign-extended bit field, the compare
+// that makes the ballot from it, and one three-input bit operation per half (acc | (ballot ^ bit)). `below`: how
+// many of the lanes sit below this one (its rank among them); `count`: how many there are. The highest of them is
+// the lane with below + 1 == count.
 struct WaveMatch
 {
     uint32_t below, count;
 };
 __device__ __forceinline__ WaveMatch waveMatch(unsigned d, uint64_t valid)
 {
-    uint32_t lo = uint32_t(valid), hi = uint32_t(valid >> 32);
+    uint32_t lo = ~uint32_t(valid), hi = ~uint32_t(valid >> 32); // lanes ruled out
 #pragma unroll
     for (int b = 0; b < 8; ++b)
     {
-        const uint32_t fill = uint32_t(int32_t(d << (31 - b)) >> 31); // all ones where bit b of d is set
-        const uint64_t bb   = __builtin_amdgcn_ballot_w64(fill != 0u);
-        lo &= ~(uint32_t(bb) ^ fill);
-        hi &= ~(uint32_t(bb >> 32) ^ fill);
+        const uint32_t bit = uint32_t(__builtin_amdgcn_sbfe(int(d), b, 1)); // all ones where bit b of d is set
+        const uint64_t bb  = __builtin_amdgcn_uicmp(bit, 0u, 33);            // ballot of bit != 0 (33: ne)
+        lo = __builtin_amdgcn_bitop3_b32(lo, uint32_t(bb), bit, 0xF6);       // lo | (ballot ^ bit)
+        hi = __builtin_amdgcn_bitop3_b32(hi, uint32_t(bb >> 32), bit, 0xF6);
     }
+    lo = ~lo, hi = ~hi;
     return {__builtin_amdgcn_mbcnt_hi(hi, __builtin_amdgcn_mbcnt_lo(lo, 0u)), uint32_t(__popc(lo) + __popc(hi))};
 }
 
