@@ -747,8 +747,8 @@ __global__ void msdInitKernel(uint32_t* ctl, uint32_t* hist)
         hist[i] = 0;
 }
 
-// Children of one segment after its pass on the digit of round r (cnt[d], start[d] in LDS, 256 threads): large
-// ones become segments of round r + 1 (with their tiles), runs of small ones ranges for the LDS sorts. Thread 0
+// Children of one segment after its pass on the digit of round r (cnt[d], start[d] in LDS, at least 64 threads):
+// large ones become segments of round r + 1 (with their tiles), runs of small ones ranges for the LDS sorts. Wave 0
 // walks the 256 children in order; one reservation per list, then the block writes the entries.
 __device__ __forceinline__ void emitChildren(const uint32_t* cnt, const uint32_t* start, int r, MsdLists L)
 {
@@ -759,8 +759,16 @@ __device__ __forceinline__ void emitChildren(const uint32_t* cnt, const uint32_t
     __shared__ int nPack, nBig;
     __shared__ uint32_t bigTiles, segBase, tileBase, rangeBase[kNumRangeClasses], classOf[kDigits],
         classIdx[kDigits];
-    if (threadIdx.x == 0)
+    if (threadIdx.x < 64)
     {
+        // wave 0 walks the children in order: lane l holds the counts and starts of children l, l + 64, l + 128 and
+        // l + 192, which the walk reads with readlane, so its state and branches are wave-uniform (scalar) and
+        // only lane 0 writes the lists (one LDS round trip per child in a one-thread walk cost ~40 us per call)
+        const int lane = int(threadIdx.x);
+        uint32_t cv[4], sv[4];
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+            cv[j] = cnt[64 * j + lane], sv[j] = start[64 * j + lane];
         int np = 0, nb = 0;
         uint32_t tiles = 0, pStart = 0, pLen = 0, pKids = 0;
         uint32_t nc[kNumRangeClasses] = {};
@@ -768,47 +776,62 @@ __device__ __forceinline__ void emitChildren(const uint32_t* cnt, const uint32_t
             // a range already in the first buffer with nothing left to sort needs no work
             if (len == 0 || (digits == 0 && src == 0)) return;
             const int c = rangeClass(len);
-            packs[np]    = make_uint2(s, len | (digits << 16) | (src << 24));
-            classOf[np]  = c;
-            classIdx[np] = nc[c]++;
+            if (lane == 0)
+            {
+                packs[np]    = make_uint2(s, len | (digits << 16) | (src << 24));
+                classOf[np]  = c;
+                classIdx[np] = nc[c];
+            }
+            ++nc[c];
             ++np;
         };
         auto flush = [&]() {
             if (pLen) put(pStart, pLen, rem == 0 ? 0u : uint32_t(rem) + (pKids > 1 ? 1u : 0u));
             pLen = 0, pKids = 0;
         };
-        for (int d = 0; d < kDigits; ++d)
-        {
-            const uint32_t c = cnt[d];
-            if (c == 0) continue;
-            if (rem > 0 && c > uint32_t(kLocalMax))
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+            for (int l = 0; l < 64; ++l)
             {
-                flush();
-                bigs[nb++] = MsdSeg{start[d], c, tiles, 0};
-                tiles += (c + kSortTile - 1) / kSortTile;
+                const uint32_t c = uint32_t(__builtin_amdgcn_readlane(int(cv[j]), l));
+                if (c == 0) continue;
+                const uint32_t st = uint32_t(__builtin_amdgcn_readlane(int(sv[j]), l));
+                if (rem > 0 && c > uint32_t(kLocalMax))
+                {
+                    flush();
+                    if (lane == 0) bigs[nb] = MsdSeg{st, c, tiles, 0};
+                    ++nb;
+                    tiles += (c + kSortTile - 1) / kSortTile;
+                }
+                else if (c > uint32_t(kPackMax))
+                {
+                    flush();
+                    // only a child with no digits left can be larger than kLocalMax: it is sorted already
+                    put(st, c > uint32_t(kLocalMax) ? 0u : c, uint32_t(rem));
+                }
+                else
+                {
+                    if (pLen + c > uint32_t(kPackMax)) flush();
+                    if (pLen == 0) pStart = st;
+                    pLen += c;
+                    ++pKids;
+                }
             }
-            else if (c > uint32_t(kPackMax))
-            {
-                flush();
-                // only a child with no digits left can be larger than kLocalMax: it is sorted already
-                put(start[d], c > uint32_t(kLocalMax) ? 0u : c, uint32_t(rem));
-            }
-            else
-            {
-                if (pLen + c > uint32_t(kPackMax)) flush();
-                if (pLen == 0) pStart = start[d];
-                pLen += c;
-                ++pKids;
-            }
-        }
         flush();
-        nPack = np, nBig = nb, bigTiles = tiles;
-        for (int c = 0; c < kNumRangeClasses; ++c)
-            rangeBase[c] = nc[c] ? atomicAdd(&L.ctl[kMsdRange + c], nc[c]) : 0u;
-        if (nb)
+        if (lane == 0)
         {
-            segBase  = atomicAdd(&L.ctl[kMsdBig + r + 1], uint32_t(nb));
-            tileBase = atomicAdd(&L.ctl[kMsdTiles + r + 1], tiles);
+            // the reservations in flight together (an add of zero only reads the counter)
+            nPack = np, nBig = nb, bigTiles = tiles;
+            uint32_t base[kNumRangeClasses];
+#pragma unroll
+            for (int c = 0; c < kNumRangeClasses; ++c)
+                base[c] = atomicAdd(&L.ctl[kMsdRange + c], nc[c]);
+            const uint32_t sb = atomicAdd(&L.ctl[kMsdBig + r + 1], uint32_t(nb));
+            const uint32_t tb = atomicAdd(&L.ctl[kMsdTiles + r + 1], tiles);
+#pragma unroll
+            for (int c = 0; c < kNumRangeClasses; ++c)
+                rangeBase[c] = base[c];
+            segBase = sb, tileBase = tb;
         }
     }
     __syncthreads();
@@ -833,11 +856,13 @@ __device__ __forceinline__ void emitChildren(const uint32_t* cnt, const uint32_t
                 g.tileStart += tileBase;
                 L.segs[nxt][segBase + i] = g;
             }
-            for (int i = 0; i < nBig; ++i)
+            // one wave per segment at a time
+            const int waves = int(blockDim.x >> 6), w = int(threadIdx.x >> 6), lane = int(threadIdx.x & 63);
+            for (int i = w; i < nBig; i += waves)
             {
                 const uint32_t t0 = tileBase + bigs[i].tileStart;
                 const uint32_t nt = (bigs[i].len + kSortTile - 1) / kSortTile;
-                for (uint32_t t = threadIdx.x; t < nt; t += blockDim.x)
+                for (uint32_t t = uint32_t(lane); t < nt; t += 64u)
                     L.owner[nxt][t0 + t] = segBase + i;
             }
         }
