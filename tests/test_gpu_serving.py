@@ -243,3 +243,23 @@ def test_least_work_order(gpu):
     assert by_admit(outs["least"]) != by_admit(outs["arrival"])  # the seed gives a stream out of work order
     for a, b in zip(outs["arrival"], outs["least"]):
         assert np.array_equal(a["state"], b["state"]) and a["pot"] == b["pot"]
+
+
+def test_least_work_order_with_slots_and_lane(gpu):
+    """several slots and the batch lane: the admission order changes which requests share the lane and the device,
+    not their results (the final order bit for bit, states and potentials to the tolerances of the alone-vs-mix
+    test, lane members included)"""
+    systems, params = S.make_stream(12, rate_per_s=0, seed=8, lo_exp=10, hi_exp=16, steps=(1, 4))
+    outs = []
+    for order in ("arrival", "least_work"):
+        srv = S.Server(slots=4, max_n=1 << 16, batch_max_n=1 << 13, order=order, max_wait_ms=5000.0)
+        outs.append(srv.run(systems, params, keep_state=True))
+        srv.close()
+    a, b = outs
+    assert all(r["status"] == 0 for r in a + b)
+    assert any(r["slot"] == 4 for r in b) and any(r["slot"] < 4 for r in b)  # lane and slots both used
+    for ra, rb in zip(a, b):
+        sa, sb = ra["state"], rb["state"]
+        assert np.array_equal(sa[6].view(np.int32), sb[6].view(np.int32))
+        assert np.abs(sa[:6] - sb[:6]).max() <= 1e-6 * (np.abs(sa[:6]).max() + 1e-30)
+        assert abs(ra["pot"] - rb["pot"]) <= 1e-9 * abs(ra["pot"])
